@@ -1,0 +1,223 @@
+/*
+ * mgsb.h -- C ABI of libmgsb, the MI355X-native multigrid projective-dynamics
+ * tetrahedral solver.  Drop-in for the solver path of
+ * buuugyellow/MultiGridSoftBody: .msh load, fixed-vertex set, step() and
+ * position readback in the Render DLL's layout.
+ *
+ * Conventions:
+ *   - plain pointers and sizes only; caller keeps ownership of what it passes,
+ *     arrays are copied;
+ *   - every function returns int: 0 on success, a negative MGSB_ERR_* code on
+ *     failure, with a message from mgsb_last_error() (per thread);
+ *   - no function exits the process or throws across the ABI;
+ *   - one solver handle is not thread-safe; distinct handles may be stepped
+ *     from different threads (each owns its own HIP stream).
+ *
+ * Each declaration names the reference interface it replaces (file:line in
+ * the reference repository).
+ */
+#ifndef MGSB_H
+#define MGSB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGSB_OK 0
+#define MGSB_ERR_ARG (-1)
+#define MGSB_ERR_IO (-2)
+#define MGSB_ERR_MESH (-3)
+#define MGSB_ERR_HIP (-4)
+#define MGSB_ERR_STATE (-5)
+#define MGSB_ERR_NOMEM (-6)
+
+/* Step schedules. */
+#define MGSB_SCHED_SINGLE 0   /* PDSolver::Step, PDSolver.cpp:243-293                */
+#define MGSB_SCHED_TWOLEVEL 1 /* PDSolver_MG::Step, PDSolver_MG.cpp:183-253          */
+#define MGSB_SCHED_VCYCLE 2   /* residual-correction V-cycle (no reference schedule) */
+#define MGSB_SCHED_LEARNED 3  /* V-cycle with a learned coarse warm start            */
+
+/* Local-step variants. */
+#define MGSB_IF_PLAIN 0 /* calculateIF, PDSolver.cu:128-183                         */
+#define MGSB_IF_AC 1    /* getTetFRV2 + calTetIFV1, PDSolverAc.cu:447-662 (|f|<1e-5 dropped) */
+
+#define MGSB_MAX_SPHERES 8
+
+/* Tet mesh (SoftObject's arrays, SoftObject.h:22-27).  Arrays returned by the
+ * loaders/generators are owned by the library: release with mgsb_mesh_free. */
+typedef struct mgsb_mesh {
+    int32_t nverts;
+    int32_t ntets;
+    int32_t ntris;
+    float* pos;        /* 3*nverts, xyz                                      */
+    int32_t* tets;     /* 4*ntets, 0-based vertex ids, positive orientation  */
+    uint32_t* tris;    /* 3*ntris surface triangles (TetFaceExtraction)      */
+    uint32_t* tri_opp; /* ntris, tet vertex opposite each surface triangle   */
+} mgsb_mesh;
+
+/* Solver constants.  mgsb_default_params gives the reference's values
+ * (PDSolver::Init, PDSolver.cpp:166-176). */
+typedef struct mgsb_params {
+    float dt;          /* 1/60                                                   */
+    float damping;     /* 0.95                                                   */
+    float stiffness;   /* ARAP weight per unit rest volume (8000)                */
+    float density;     /* lumped mass = density * volume / 4 (1)                 */
+    float rho;         /* Chebyshev spectral radius (0.9992); <=0: estimate      */
+    float relax;       /* under-relaxation of the Jacobi step (0.6)              */
+    float gravity[3];  /* (0,-9.8,0)                                             */
+    float kcol;        /* sphere collision stiffness (1000)                      */
+    int32_t iters;        /* PD iterations per step, fine/single level (20)      */
+    int32_t iters_coarse; /* two-level schedule: coarse iterations (20)          */
+    int32_t schedule;     /* MGSB_SCHED_*                                        */
+    int32_t if_mode;      /* MGSB_IF_*                                           */
+    int32_t tri_dcd_coarse; /* two-level: triangle DCD on the coarse level (1)   */
+    /* V-cycle schedule */
+    int32_t pre_sweeps;   /* nonlinear Jacobi sweeps before the coarse correction */
+    int32_t post_sweeps;  /* ... after                                           */
+    int32_t coarse_sweeps;/* linear Chebyshev-Jacobi sweeps per coarse level     */
+    int32_t max_cycles;   /* V-cycles per step at most                           */
+    float tol;            /* relative residual target per step (1e-3)           */
+    int32_t check_every;  /* cycles between residual checks (0: only at end)     */
+    int32_t use_graph;    /* capture fixed-count steps in a hipGraph             */
+    int32_t block_tet;    /* workgroup size of per-tet kernels (0: tuned)        */
+    int32_t block_vert;   /* workgroup size of per-vertex kernels (0: tuned)     */
+    int32_t device;       /* HIP device ordinal                                  */
+    int32_t reserved[8];
+} mgsb_params;
+
+typedef struct mgsb_step_stats {
+    double res0;          /* ||g(s)||, FP64, at the start of the last step        */
+    double res;           /* ||g(x)|| at its end                                  */
+    double rel;           /* res / res0                                           */
+    int32_t cycles;       /* V-cycles run in the last step                        */
+    int32_t fallback;     /* learned schedule: 1 if the warm start was rejected   */
+    int32_t steps;        /* steps taken by this call                             */
+    int32_t reserved;
+} mgsb_step_stats;
+
+/* Energy diagnostics, runCalEnergy (PDSolver.cu:816-913): Ek, Ep, Ec, Nc, dX. */
+typedef struct mgsb_energy_t {
+    double Ek, Ep, Ec, Nc, dX;
+} mgsb_energy_t;
+
+/* Replaces SoftObject::ReadFromFile's $Nodes/$Elements reader
+ * (SoftObject.cpp:85-130) plus SoftObject::TetFaceExtraction
+ * (SoftObject.cpp:154-251): gmsh 2.2 ASCII; tets (type 4) kept, other element
+ * types skipped by their node counts; surface triangles extracted from tets. */
+int mgsb_load_msh(const char* path, mgsb_mesh* out);
+
+/* SoftObject::TetFaceExtraction (SoftObject.cpp:154-251): (re)computes
+ * tris/tri_opp from tets; same windings and the same (descending key) order. */
+int mgsb_extract_surface(mgsb_mesh* m);
+
+/* Structured box [lo,hi] of nx*ny*nz hex cells, 6 positively oriented tets per
+ * hex (Kuhn split), surface extracted.  No reference counterpart (the
+ * reference meshes come from gmsh); used by the box workloads. */
+int mgsb_make_box(int32_t nx, int32_t ny, int32_t nz, const float lo[3], const float hi[3], mgsb_mesh* out);
+
+/* Voxelised superquadric blob |x/a|^p+|y/b|^p+|z/c|^p <= (1+amp*sum_8 sin)^p,
+ * h voxel size, 6 tets per voxel; voxels sit on the lattice h*Z^3, so blobs
+ * at h, 3h, 9h nest.  No reference counterpart (organ-like workload). */
+int mgsb_make_blob(float a, float b, float c, float p, float amp, uint32_t seed, float h, mgsb_mesh* out);
+
+void mgsb_mesh_free(mgsb_mesh* m);
+
+/* InitVolumeConstraint (PDSolver.cpp:12-132), host only: per tet Dm^-1 (9,
+ * row-major), AC (12), rest volume; per vertex lumped mass (density applied)
+ * and Jacobi diagonal sum stiffness*vol*AC^2.  Any output may be NULL.
+ * Fails with MGSB_ERR_MESH on a degenerate rest tet (|det Dm| < 1e-6), where
+ * the reference calls exit(0) (utils.cu:244-248). */
+int mgsb_precompute(const mgsb_mesh* m, float stiffness, float density, float* B, float* AC, float* vol,
+                    float* mass, float* diag);
+
+/* Barycentric binding of n points into the tets of mesh m, host only.
+ * mode 0: PDSolver_MG::interpolate (PDSolver_MG.cpp:5-58), first tet in index
+ *         order with all weights in [-1e-5, 1+1e-5];
+ * mode 1: coarse-in-fine binding of PDSolver_MG::Init (PDSolver_MG.cpp:133-168),
+ *         first tet passing pointInTet, face-area weights.
+ * ids/w receive 4 per point; returns the number of unbound points (>= 0) or
+ * a negative error. */
+int mgsb_bind_points(const mgsb_mesh* m, const float* pts, int32_t n, int32_t mode, int32_t* ids, float* w);
+
+/* PDSolver::Init constants (PDSolver.cpp:166-176) plus V-cycle defaults. */
+void mgsb_default_params(mgsb_params* p);
+
+typedef struct mgsb_solver mgsb_solver;
+
+/* PDSolver::Init (PDSolver.cpp:164-200, PDSolver.h:102) for nlevels == 1 and
+ * PDSolver_MG::Init (PDSolver_MG.cpp:60-181) for nlevels >= 2.  levels[0] is
+ * the finest mesh; levels[i+1] is coarser than levels[i].  The solver
+ * precomputes (InitVolumeConstraint, PDSolver.cpp:12-132), builds the
+ * barycentric level maps, and uploads everything to the device. */
+int mgsb_create(const mgsb_mesh* levels, int32_t nlevels, const mgsb_params* p, mgsb_solver** out);
+void mgsb_destroy(mgsb_solver* s);
+
+/* Parameters that may change between steps (iteration counts, schedule,
+ * stiffness excluded: stiffness and density are baked at create). */
+int mgsb_set_params(mgsb_solver* s, const mgsb_params* p);
+
+/* PDSolver::SetFixedVert (PDSolver.cpp:134-162): fixflag per vertex of a
+ * level, 1 = free, 0 = pinned. */
+int mgsb_set_fixed(mgsb_solver* s, int32_t level, const float* fixflag);
+/* SphereFixer rule (PDSolver.cpp:145-161): pin vertices strictly inside any of
+ * n spheres (cx,cy,cz,r) on every level. */
+int mgsb_fix_spheres(mgsb_solver* s, const float* spheres, int32_t n);
+/* Pin vertices with |pos[axis]-value| < tol on every level (the y=10 rule,
+ * PDSolver.cpp:137-143). */
+int mgsb_fix_plane(mgsb_solver* s, int32_t axis, float value, float tol);
+
+/* SphereCollider (Collider.h:5-20): centre, radius, velocity (the centre
+ * advances by vel*dt at the start of each step, as UpdateCollider moves it
+ * before Step, Simulator.cpp:146-149), active flag. */
+int mgsb_set_sphere(mgsb_solver* s, int32_t id, const float center[3], float radius, const float vel[3], int32_t active);
+/* Ground plane n.x >= d as a penalty constraint of stiffness k (no reference
+ * counterpart). active=0 disables it. */
+int mgsb_set_plane(mgsb_solver* s, const float n[3], float d, float k, int32_t active);
+
+/* Per-vertex velocity of the finest level (3*V). */
+int mgsb_set_velocity(mgsb_solver* s, const float* v);
+/* Positions of a level (3*V). */
+int mgsb_set_positions(mgsb_solver* s, int32_t level, const float* x);
+
+/* PDSolver::Step / PDSolver_MG::Step (PDSolver.cpp:243-293,
+ * PDSolver_MG.cpp:183-253) run nsteps times. */
+int mgsb_step(mgsb_solver* s, int32_t nsteps, mgsb_step_stats* stats);
+/* Blocks until the solver's stream is idle. */
+int mgsb_sync(mgsb_solver* s);
+
+/* runCpyTetVertForRender (PDSolver.cu:261-268): positions of a level (3*V). */
+int mgsb_read_positions(mgsb_solver* s, int32_t level, float* xyz);
+int mgsb_read_velocity(mgsb_solver* s, int32_t level, float* v);
+/* runUpdateOutsideTetVertNormal (PDSolver.cu:394-460): vertex normals of the
+ * finest level (3*V), unit averages of adjacent surface triangle normals. */
+int mgsb_read_normals(mgsb_solver* s, float* n);
+/* The render repack of main.cpp:251-259: 9 floats per vertex
+ * (pos, normal, 0,0,0) for Application::UpdateMesh (application.hpp:56). */
+int mgsb_read_render9(mgsb_solver* s, float* v9);
+/* Collision flags / depths of the finest level from the last iteration
+ * (tetVertIsCollided_d / tetVertCollisionDepth_d, PDSolver.cu:264-265). */
+int mgsb_read_collision(mgsb_solver* s, int32_t* flags, float* depth);
+/* runCalEnergy (PDSolver.cu:816-913) on the finest level, FP64 sums on device. */
+int mgsb_energy(mgsb_solver* s, mgsb_energy_t* e);
+/* ||g(x)|| of the finest level at its current state (FP64). */
+int mgsb_residual(mgsb_solver* s, double* res);
+
+/* Sizes of a level. */
+int mgsb_level_info(mgsb_solver* s, int32_t level, int32_t* nverts, int32_t* ntets, int32_t* ntris);
+
+/* Kernel timing: when on, HIP events bracket every launch of the dominant
+ * per-tet kernel on the solver's stream; mgsb_kernel_time returns the summed
+ * milliseconds and the launch count since timing was switched on. */
+int mgsb_set_timing(mgsb_solver* s, int32_t on);
+int mgsb_kernel_time(mgsb_solver* s, int32_t kernel, double* ms, int64_t* launches);
+
+const char* mgsb_last_error(void);
+const char* mgsb_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MGSB_H */

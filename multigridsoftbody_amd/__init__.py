@@ -1,0 +1,445 @@
+"""multigridsoftbody_amd -- MI355X-native multigrid projective-dynamics solver.
+
+Python host layer over the C ABI of ``libmgsb.so`` (declared in
+``include/mgsb.h``).  It mirrors the reference solver interface
+(``PDSolver`` / ``PDSolver_MG`` of MultiGridSoftBody/src/PDSolver.h:70-110 and
+PDSolver_MG.h:4-25): build from tet-mesh arrays, set the fixed-vertex set,
+``Step()``, read positions back.  All arithmetic runs in the HIP kernels of
+the library; this module only marshals arrays.
+
+The library must have been built;
+importing raises if it is missing -- there is no CPU fallback.
+Build with ``python multigridsoftbody_amd/build.py``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmgsb.so")
+
+SCHED_SINGLE, SCHED_TWOLEVEL, SCHED_VCYCLE, SCHED_LEARNED = 0, 1, 2, 3
+IF_PLAIN, IF_AC = 0, 1
+MAX_SPHERES = 8
+
+
+class MgsbError(RuntimeError):
+    """A libmgsb call returned a negative status."""
+
+
+class mgsb_mesh(C.Structure):
+    _fields_ = [
+        ("nverts", C.c_int32),
+        ("ntets", C.c_int32),
+        ("ntris", C.c_int32),
+        ("pos", C.POINTER(C.c_float)),
+        ("tets", C.POINTER(C.c_int32)),
+        ("tris", C.POINTER(C.c_uint32)),
+        ("tri_opp", C.POINTER(C.c_uint32)),
+    ]
+
+
+class mgsb_params(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float),
+        ("damping", C.c_float),
+        ("stiffness", C.c_float),
+        ("density", C.c_float),
+        ("rho", C.c_float),
+        ("relax", C.c_float),
+        ("gravity", C.c_float * 3),
+        ("kcol", C.c_float),
+        ("iters", C.c_int32),
+        ("iters_coarse", C.c_int32),
+        ("schedule", C.c_int32),
+        ("if_mode", C.c_int32),
+        ("tri_dcd_coarse", C.c_int32),
+        ("pre_sweeps", C.c_int32),
+        ("post_sweeps", C.c_int32),
+        ("coarse_sweeps", C.c_int32),
+        ("max_cycles", C.c_int32),
+        ("tol", C.c_float),
+        ("check_every", C.c_int32),
+        ("use_graph", C.c_int32),
+        ("block_tet", C.c_int32),
+        ("block_vert", C.c_int32),
+        ("device", C.c_int32),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+class mgsb_step_stats(C.Structure):
+    _fields_ = [
+        ("res0", C.c_double),
+        ("res", C.c_double),
+        ("rel", C.c_double),
+        ("cycles", C.c_int32),
+        ("fallback", C.c_int32),
+        ("steps", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class mgsb_energy_t(C.Structure):
+    _fields_ = [("Ek", C.c_double), ("Ep", C.c_double), ("Ec", C.c_double), ("Nc", C.c_double), ("dX", C.c_double)]
+
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+# name -> (restype, argtypes); every symbol declared in include/mgsb.h
+SIGNATURES = {
+    "mgsb_load_msh": (C.c_int, [C.c_char_p, C.POINTER(mgsb_mesh)]),
+    "mgsb_extract_surface": (C.c_int, [C.POINTER(mgsb_mesh)]),
+    "mgsb_make_box": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _fp, _fp, C.POINTER(mgsb_mesh)]),
+    "mgsb_make_blob": (C.c_int, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_float,
+                                 C.POINTER(mgsb_mesh)]),
+    "mgsb_mesh_free": (None, [C.POINTER(mgsb_mesh)]),
+    "mgsb_precompute": (C.c_int, [C.POINTER(mgsb_mesh), C.c_float, C.c_float, _fp, _fp, _fp, _fp, _fp]),
+    "mgsb_bind_points": (C.c_int, [C.POINTER(mgsb_mesh), _fp, C.c_int32, C.c_int32, _ip, _fp]),
+    "mgsb_default_params": (None, [C.POINTER(mgsb_params)]),
+    "mgsb_create": (C.c_int, [C.POINTER(mgsb_mesh), C.c_int32, C.POINTER(mgsb_params), C.POINTER(_vp)]),
+    "mgsb_destroy": (None, [_vp]),
+    "mgsb_set_params": (C.c_int, [_vp, C.POINTER(mgsb_params)]),
+    "mgsb_set_fixed": (C.c_int, [_vp, C.c_int32, _fp]),
+    "mgsb_fix_spheres": (C.c_int, [_vp, _fp, C.c_int32]),
+    "mgsb_fix_plane": (C.c_int, [_vp, C.c_int32, C.c_float, C.c_float]),
+    "mgsb_set_sphere": (C.c_int, [_vp, C.c_int32, _fp, C.c_float, _fp, C.c_int32]),
+    "mgsb_set_plane": (C.c_int, [_vp, _fp, C.c_float, C.c_float, C.c_int32]),
+    "mgsb_set_velocity": (C.c_int, [_vp, _fp]),
+    "mgsb_set_positions": (C.c_int, [_vp, C.c_int32, _fp]),
+    "mgsb_step": (C.c_int, [_vp, C.c_int32, C.POINTER(mgsb_step_stats)]),
+    "mgsb_sync": (C.c_int, [_vp]),
+    "mgsb_read_positions": (C.c_int, [_vp, C.c_int32, _fp]),
+    "mgsb_read_velocity": (C.c_int, [_vp, C.c_int32, _fp]),
+    "mgsb_read_normals": (C.c_int, [_vp, _fp]),
+    "mgsb_read_render9": (C.c_int, [_vp, _fp]),
+    "mgsb_read_collision": (C.c_int, [_vp, _ip, _fp]),
+    "mgsb_energy": (C.c_int, [_vp, C.POINTER(mgsb_energy_t)]),
+    "mgsb_residual": (C.c_int, [_vp, _dp]),
+    "mgsb_level_info": (C.c_int, [_vp, C.c_int32, _ip, _ip, _ip]),
+    "mgsb_set_timing": (C.c_int, [_vp, C.c_int32]),
+    "mgsb_kernel_time": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
+    "mgsb_last_error": (C.c_char_p, []),
+    "mgsb_version": (C.c_char_p, []),
+}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python multigridsoftbody_amd/build.py` (hipcc, gfx950)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise MgsbError(f"{what} failed ({rc}): {lib.mgsb_last_error().decode(errors='replace')}")
+    return rc
+
+
+def _f32(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} floats, got {a.size}")
+    return a
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_fp)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+class Mesh:
+    """A tet mesh: positions (V,3) float32, tets (T,4) int32, surface tris
+    (S,3) uint32 and the opposite tet vertex of each (S,) -- the arrays of
+    SoftObject (SoftObject.h:22-27)."""
+
+    def __init__(self, pos, tets, tris=None, tri_opp=None):
+        self.pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        self.tets = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+        if tris is None:
+            self.tris = None
+            m = self._as_c()
+            _check(lib.mgsb_extract_surface(C.byref(m)), "mgsb_extract_surface")
+            S = m.ntris
+            tris = np.ctypeslib.as_array(m.tris, shape=(max(S, 1) * 3,))[: S * 3].copy()
+            tri_opp = np.ctypeslib.as_array(m.tri_opp, shape=(max(S, 1),))[:S].copy()
+            m.pos = None  # numpy-owned: only the surface arrays belong to the library
+            m.tets = None
+            lib.mgsb_mesh_free(C.byref(m))
+        self.tris = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
+        self.tri_opp = None if tri_opp is None else np.ascontiguousarray(tri_opp, dtype=np.uint32)
+
+    @property
+    def nverts(self):
+        return self.pos.shape[0]
+
+    @property
+    def ntets(self):
+        return self.tets.shape[0]
+
+    @property
+    def ntris(self):
+        return self.tris.shape[0]
+
+    def _as_c(self):
+        m = mgsb_mesh()
+        m.nverts, m.ntets = self.nverts, self.ntets
+        m.pos = _fptr(self.pos)
+        m.tets = _iptr(self.tets)
+        if getattr(self, "tris", None) is not None:
+            m.ntris = self.tris.shape[0]
+            m.tris = self.tris.ctypes.data_as(C.POINTER(C.c_uint32))
+        else:
+            m.ntris = 0
+        return m
+
+    @staticmethod
+    def _take(m):
+        """Copy arrays out of a library-owned mgsb_mesh and free it."""
+        try:
+            V, T, S = m.nverts, m.ntets, m.ntris
+            pos = np.ctypeslib.as_array(m.pos, shape=(V * 3,)).copy().reshape(V, 3)
+            tets = np.ctypeslib.as_array(m.tets, shape=(T * 4,)).copy().reshape(T, 4)
+            tris = np.ctypeslib.as_array(m.tris, shape=(max(S, 1) * 3,))[: S * 3].copy().reshape(S, 3)
+            opp = np.ctypeslib.as_array(m.tri_opp, shape=(max(S, 1),))[:S].copy()
+        finally:
+            if m.pos or m.tets:
+                lib.mgsb_mesh_free(C.byref(m))
+        return pos, tets, tris, opp
+
+    @classmethod
+    def load_msh(cls, path):
+        """SoftObject::ReadFromFile + TetFaceExtraction (SoftObject.cpp:85-251)."""
+        m = mgsb_mesh()
+        _check(lib.mgsb_load_msh(os.fsencode(path), C.byref(m)), f"mgsb_load_msh({path})")
+        return cls(*cls._take(m))
+
+    @classmethod
+    def box(cls, nx, ny, nz, lo=(-1.0, -10.0, -1.0), hi=(1.0, 10.0, 1.0)):
+        """Structured box of nx*ny*nz hexes, 6 tets each."""
+        m = mgsb_mesh()
+        lo_ = _f32(lo, 3)
+        hi_ = _f32(hi, 3)
+        _check(lib.mgsb_make_box(nx, ny, nz, _fptr(lo_), _fptr(hi_), C.byref(m)), "mgsb_make_box")
+        return cls(*cls._take(m))
+
+    @classmethod
+    def blob(cls, a=6.0, b=3.0, c=4.0, p=2.5, amp=0.15, seed=0, h=0.25):
+        """Voxelised superquadric blob, 6 tets per voxel."""
+        m = mgsb_mesh()
+        _check(lib.mgsb_make_blob(a, b, c, p, amp, seed, h, C.byref(m)), "mgsb_make_blob")
+        return cls(*cls._take(m))
+
+    def precompute(self, stiffness, density=1.0):
+        """InitVolumeConstraint (PDSolver.cpp:12-132) on the host."""
+        T, V = self.ntets, self.nverts
+        B = np.zeros(T * 9, np.float32)
+        AC = np.zeros(T * 12, np.float32)
+        vol = np.zeros(T, np.float32)
+        mass = np.zeros(V, np.float32)
+        diag = np.zeros(V, np.float32)
+        m = self._as_c()
+        _check(lib.mgsb_precompute(C.byref(m), stiffness, density, _fptr(B), _fptr(AC), _fptr(vol), _fptr(mass),
+                                   _fptr(diag)), "mgsb_precompute")
+        return {"B": B.reshape(T, 9), "AC": AC.reshape(T, 12), "vol": vol, "mass": mass, "diag": diag}
+
+    def bind(self, points, mode=0):
+        """Barycentric binding of points into this mesh's tets
+        (PDSolver_MG.cpp:5-58 for mode 0, :133-168 for mode 1)."""
+        pts = _f32(points).reshape(-1, 3)
+        n = pts.shape[0]
+        ids = np.zeros(n * 4, np.int32)
+        w = np.zeros(n * 4, np.float32)
+        m = self._as_c()
+        unf = _check(lib.mgsb_bind_points(C.byref(m), _fptr(pts), n, mode, _iptr(ids), _fptr(w)), "mgsb_bind_points")
+        return ids.reshape(n, 4), w.reshape(n, 4), unf
+
+
+def default_params(**kw):
+    """PDSolver::Init constants (PDSolver.cpp:166-176), overridden by kw."""
+    p = mgsb_params()
+    lib.mgsb_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "gravity":
+            p.gravity[:] = [float(x) for x in v]
+        elif not hasattr(p, k):
+            raise KeyError(k)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Solver:
+    """One solver handle over 1..n nested levels (levels[0] finest).
+
+    nlevels == 1 mirrors PDSolver (PDSolver.h:70-110); nlevels == 2 with the
+    two-level schedule mirrors PDSolver_MG (PDSolver_MG.h:4-25)."""
+
+    def __init__(self, levels, params=None, **kw):
+        if isinstance(levels, Mesh):
+            levels = [levels]
+        self.levels = list(levels)
+        self.params = params if params is not None else default_params(**kw)
+        arr = (mgsb_mesh * len(self.levels))(*[m._as_c() for m in self.levels])
+        h = _vp()
+        _check(lib.mgsb_create(arr, len(self.levels), C.byref(self.params), C.byref(h)), "mgsb_create")
+        self._h = h
+        self.stats = mgsb_step_stats()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.mgsb_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- configuration ----------------------------------------------------
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            if k == "gravity":
+                self.params.gravity[:] = [float(x) for x in v]
+            else:
+                setattr(self.params, k, v)
+        _check(lib.mgsb_set_params(self._h, C.byref(self.params)), "mgsb_set_params")
+
+    def set_fixed(self, fixflag, level=0):
+        """fixflag per vertex: 1 free, 0 pinned (PDSolver.cpp:134-162)."""
+        f = _f32(fixflag, self.levels[level].nverts)
+        _check(lib.mgsb_set_fixed(self._h, level, _fptr(f)), "mgsb_set_fixed")
+
+    def fix_spheres(self, spheres):
+        s = _f32(spheres).reshape(-1, 4)
+        _check(lib.mgsb_fix_spheres(self._h, _fptr(s), s.shape[0]), "mgsb_fix_spheres")
+
+    def fix_plane(self, axis, value, tol=1e-5):
+        _check(lib.mgsb_fix_plane(self._h, axis, value, tol), "mgsb_fix_plane")
+
+    def set_sphere(self, sid, center, radius, vel=(0.0, 0.0, 0.0), active=True):
+        c = _f32(center, 3)
+        v = _f32(vel, 3)
+        _check(lib.mgsb_set_sphere(self._h, sid, _fptr(c), radius, _fptr(v), int(active)), "mgsb_set_sphere")
+
+    def set_plane(self, normal, d, k, active=True):
+        n = _f32(normal, 3)
+        _check(lib.mgsb_set_plane(self._h, _fptr(n), d, k, int(active)), "mgsb_set_plane")
+
+    def set_velocity(self, v):
+        a = _f32(v, self.levels[0].nverts * 3)
+        _check(lib.mgsb_set_velocity(self._h, _fptr(a)), "mgsb_set_velocity")
+
+    def set_positions(self, x, level=0):
+        a = _f32(x, self.levels[level].nverts * 3)
+        _check(lib.mgsb_set_positions(self._h, level, _fptr(a)), "mgsb_set_positions")
+
+    # --- stepping -----------------------------------------------------------
+    def step(self, nsteps=1):
+        _check(lib.mgsb_step(self._h, nsteps, C.byref(self.stats)), "mgsb_step")
+        return self.stats
+
+    def sync(self):
+        _check(lib.mgsb_sync(self._h), "mgsb_sync")
+
+    # --- readback -----------------------------------------------------------
+    def positions(self, level=0):
+        out = np.zeros(self.levels[level].nverts * 3, np.float32)
+        _check(lib.mgsb_read_positions(self._h, level, _fptr(out)), "mgsb_read_positions")
+        return out.reshape(-1, 3)
+
+    def velocity(self, level=0):
+        out = np.zeros(self.levels[level].nverts * 3, np.float32)
+        _check(lib.mgsb_read_velocity(self._h, level, _fptr(out)), "mgsb_read_velocity")
+        return out.reshape(-1, 3)
+
+    def normals(self):
+        out = np.zeros(self.levels[0].nverts * 3, np.float32)
+        _check(lib.mgsb_read_normals(self._h, _fptr(out)), "mgsb_read_normals")
+        return out.reshape(-1, 3)
+
+    def render9(self):
+        out = np.zeros(self.levels[0].nverts * 9, np.float32)
+        _check(lib.mgsb_read_render9(self._h, _fptr(out)), "mgsb_read_render9")
+        return out.reshape(-1, 9)
+
+    def collision(self):
+        V = self.levels[0].nverts
+        flags = np.zeros(V, np.int32)
+        depth = np.zeros(V, np.float32)
+        _check(lib.mgsb_read_collision(self._h, _iptr(flags), _fptr(depth)), "mgsb_read_collision")
+        return flags, depth
+
+    def energy(self):
+        e = mgsb_energy_t()
+        _check(lib.mgsb_energy(self._h, C.byref(e)), "mgsb_energy")
+        return {"Ek": e.Ek, "Ep": e.Ep, "Ec": e.Ec, "Nc": e.Nc, "dX": e.dX}
+
+    def residual(self):
+        r = C.c_double()
+        _check(lib.mgsb_residual(self._h, C.byref(r)), "mgsb_residual")
+        return r.value
+
+    def set_timing(self, on):
+        _check(lib.mgsb_set_timing(self._h, int(on)), "mgsb_set_timing")
+
+    def kernel_time(self, kernel=0):
+        ms = C.c_double()
+        n = C.c_int64()
+        _check(lib.mgsb_kernel_time(self._h, kernel, C.byref(ms), C.byref(n)), "mgsb_kernel_time")
+        return ms.value, n.value
+
+
+class PDSolver(Solver):
+    """Single-level solver with the reference's method names
+    (PDSolver.h:70-110)."""
+
+    def __init__(self, mesh, params=None, **kw):
+        kw.setdefault("schedule", SCHED_SINGLE)
+        super().__init__([mesh], params, **kw)
+
+    def SetFixedVert(self, fixflag):  # noqa: N802  (reference name)
+        self.set_fixed(fixflag)
+
+    def Step(self):  # noqa: N802
+        self.step(1)
+
+
+class PDSolver_MG(Solver):  # noqa: N801  (reference name)
+    """Two-level solver, coarse + fine (PDSolver_MG.h:4-25); levels are
+    passed fine first."""
+
+    def __init__(self, coarse, fine, params=None, **kw):
+        kw.setdefault("schedule", SCHED_TWOLEVEL)
+        kw.setdefault("if_mode", IF_PLAIN)
+        super().__init__([fine, coarse], params, **kw)
+
+    def Step(self):  # noqa: N802
+        self.step(1)
+
+
+__all__ = ["lib", "Mesh", "Solver", "PDSolver", "PDSolver_MG", "default_params", "MgsbError", "SCHED_SINGLE",
+           "SCHED_TWOLEVEL", "SCHED_VCYCLE", "SCHED_LEARNED", "IF_PLAIN", "IF_AC"]

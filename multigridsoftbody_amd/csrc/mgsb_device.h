@@ -1,0 +1,163 @@
+// Device-side math of libmgsb (gfx950).  Every function follows the
+// reference's FP32 operation order; the library is compiled with
+// -ffp-contract=off so no multiply-add is fused behind the source's back.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mgsb {
+
+#define MGSB_MAX_SPH 8
+
+// Colliders live in device memory so a captured step graph sees moves made
+// between replays (sphere centres advance on device each step).
+struct DevColliders {
+    int nsph;
+    int plane_on;
+    float kcol;
+    float kplane;
+    float4 sph[MGSB_MAX_SPH];  // cx cy cz r
+    float4 vel[MGSB_MAX_SPH];  // vx vy vz active
+    float4 plane;              // nx ny nz d
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8).  This
+// remaps them so each XCD walks one contiguous chunk of the index space and
+// neighbouring tets/vertices share that XCD's L2.
+__device__ __forceinline__ int xcd_block(int bid, int nb) {
+    const int x = bid & 7, local = bid >> 3;
+    const int per = nb >> 3, rem = nb & 7;
+    return x * per + (x < rem ? x : rem) + local;
+}
+
+__device__ __forceinline__ float3 f3(float4 a) { return make_float3(a.x, a.y, a.z); }
+__device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 add3(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 mul3(float3 a, float s) { return make_float3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float3 div3(float3 a, float s) { return make_float3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float len3(float3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+// The reference compares floats against the double literal 1e-6; for a float
+// x that is exactly x <= 1e-6f (no float lies in (1e-6f, 1e-6]).
+__device__ __forceinline__ bool lt_1e6(float x) { return x <= 1e-6f; }
+
+// Franca closed-form polar decomposition R = F U^-1 from the invariants of
+// C = F^T F (GetRotation_D, utils.cu:123-226; getTetFRV2, PDSolverAc.cu:494-595).
+// Fallbacks return R = I where the reference does.  Writes R - F into RF.
+__device__ __forceinline__ void polar_rf(const float F[9], float RF[9]) {
+    float C[9], C2[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) C[i * 3 + j] = F[0 * 3 + i] * F[0 * 3 + j] + F[1 * 3 + i] * F[1 * 3 + j] + F[2 * 3 + i] * F[2 * 3 + j];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            C2[i * 3 + j] = C[i * 3 + 0] * C[j * 3 + 0] + C[i * 3 + 1] * C[j * 3 + 1] + C[i * 3 + 2] * C[j * 3 + 2];
+    const float det = F[0] * F[4] * F[8] + F[1] * F[5] * F[6] + F[3] * F[7] * F[2] - F[2] * F[4] * F[6] -
+                      F[1] * F[3] * F[8] - F[0] * F[5] * F[7];
+    const float I_c = C[0] + C[4] + C[8];
+    const float I_c2 = I_c * I_c;
+    const float II_c = 0.5f * (I_c2 - C2[0] - C2[4] - C2[8]);
+    const float III_c = det * det;
+    const float k = I_c2 - 3 * II_c;
+    float iu[9];
+    bool ok = true;
+    if (k < 1e-6f) {
+        if (lt_1e6(I_c)) {
+            ok = false;
+        } else {
+            const float inv_lambda = 1 / sqrtf(I_c / 3);
+#pragma unroll
+            for (int i = 0; i < 9; i++) iu[i] = 0.f;
+            iu[0] = iu[4] = iu[8] = inv_lambda;
+        }
+    } else {
+        const float l = I_c * (I_c * I_c - 4.5f * II_c) + 13.5f * III_c;
+        const float k_root = sqrtf(k);
+        float value = l / (k * k_root);
+        value = value < -1.0f ? -1.0f : value;
+        value = value > 1.0f ? 1.0f : value;
+        const float phi = acosf(value);
+        const float lambda2 = (I_c + 2 * k_root * cosf(phi / 3)) / 3;
+        const float lambda = sqrtf(lambda2);
+        const float III_u = det;
+        float I_u = 0.f, II_u = 0.f;
+        if (lt_1e6(fabsf(lambda))) ok = false;
+        if (ok && lt_1e6(-lambda2 + I_c + 2 * III_u / lambda)) ok = false;
+        if (ok) {
+            I_u = lambda + sqrtf(-lambda2 + I_c + 2 * III_u / lambda);
+            II_u = (I_u * I_u - I_c) * 0.5f;
+            if (lt_1e6(fabsf(I_u * II_u - III_u))) ok = false;
+        }
+        if (ok && lt_1e6(fabsf(III_u))) ok = false;
+        if (ok) {
+            float inv_rate = 1 / (I_u * II_u - III_u);
+            const float fdiag = I_u * III_u * inv_rate;
+            const float fc = (I_u * I_u - II_u) * inv_rate;
+            float U[9];
+#pragma unroll
+            for (int i = 0; i < 9; i++) U[i] = fc * C[i] - inv_rate * C2[i];
+            U[0] += fdiag;
+            U[4] += fdiag;
+            U[8] += fdiag;
+            inv_rate = 1 / III_u;
+            const float gdiag = II_u * inv_rate;
+            const float gu = -I_u * inv_rate;
+#pragma unroll
+            for (int i = 0; i < 9; i++) iu[i] = gu * U[i] + inv_rate * C[i];
+            iu[0] += gdiag;
+            iu[4] += gdiag;
+            iu[8] += gdiag;
+        }
+    }
+    if (ok) {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                RF[i * 3 + j] = (F[i * 3 + 0] * iu[0 * 3 + j] + F[i * 3 + 1] * iu[1 * 3 + j] + F[i * 3 + 2] * iu[2 * 3 + j]) -
+                                F[i * 3 + j];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; i++) RF[i] = ((i % 4) == 0 ? 1.f : 0.f) - F[i];
+    }
+}
+
+// barycentricCoordinate on four points (utils.cu:335-364), GetVolumn inline.
+__device__ __forceinline__ void barycentric4(float3 P, float3 a, float3 b, float3 c, float3 d, float w[4]) {
+    const float3 AB = sub3(b, a), AC = sub3(c, a), AD = sub3(d, a);
+    const float M0 = AB.x, M1 = AC.x, M2 = AD.x, M3 = AB.y, M4 = AC.y, M5 = AD.y, M6 = AB.z, M7 = AC.z, M8 = AD.z;
+    const float R0 = M4 * M8 - M7 * M5;
+    const float R1 = M7 * M2 - M1 * M8;
+    const float R2 = M1 * M5 - M4 * M2;
+    const float Vol = (M0 * R0 + M3 * R1 + M6 * R2) / 6.0f;
+    const float3 center = mul3(add3(add3(add3(a, b), c), d), 0.25f);
+    const float3 tv[4] = {a, b, c, d};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const float3 q = tv[i], f0 = tv[(i + 1) & 3], f1 = tv[(i + 2) & 3], f2 = tv[(i + 3) & 3];
+        float3 n = cross3(sub3(f1, f0), sub3(f2, f0));
+        if (dot3(n, sub3(q, f0)) > 0) n = make_float3(-n.x, -n.y, -n.z);
+        const float l = len3(n);
+        const float area = l * 0.5f;
+        n = div3(n, l);
+        w[i] = 0.25f - dot3(sub3(P, center), n) * area / (3 * Vol);
+    }
+}
+
+// pointInTriangle (utils.cu:366-383).
+__device__ __forceinline__ bool point_in_triangle(float3 P, float3 A, float3 B, float3 C) {
+    const float3 v0 = sub3(C, A), v1 = sub3(B, A), v2 = sub3(P, A);
+    const float d00 = dot3(v0, v0), d01 = dot3(v0, v1), d02 = dot3(v0, v2), d11 = dot3(v1, v1), d12 = dot3(v1, v2);
+    const float inv = 1.0f / (d00 * d11 - d01 * d01);
+    const float u = (d11 * d02 - d01 * d12) * inv;
+    const float w = (d00 * d12 - d01 * d02) * inv;
+    return (u >= 0) && (w >= 0) && (u + w <= 1.0f);
+}
+
+}  // namespace mgsb

@@ -1,0 +1,63 @@
+// Internal host-side structures of libmgsb shared by the mesh/precompute code
+// (mgsb_host.cpp) and the device solver (mgsb_solver.hip).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mgsb.h"
+
+namespace mgsb {
+
+void set_error(const std::string& msg);
+
+// One tet mesh level after InitVolumeConstraint (PDSolver.cpp:12-132), plus
+// the deterministic gather structures.
+struct HostLevel {
+    int V = 0, T = 0, S = 0;
+    std::vector<float> X;         // rest positions, 3V
+    std::vector<int32_t> tets;    // 4T
+    std::vector<uint32_t> tris;   // 3S
+    std::vector<float> B;         // Dm^-1, 9T row-major per tet
+    std::vector<float> AC;        // 3x4 gradient matrix, 12T
+    std::vector<float> vol;       // T
+    std::vector<float> mass;      // V (density applied)
+    std::vector<float> diag;      // V, Jacobi diagonal sum k vol AC^2
+    std::vector<float> center;    // 3T
+    std::vector<float> faceN;     // 12T, outward unit normal of the face opposite corner f
+    std::vector<float> faceA;     // 4T
+    std::vector<float> fix;       // V, 1 free / 0 pinned
+    // vertex -> incident (tet, corner), tets ascending.  slot[t*4+c] is the
+    // position of (t,c) in that order.
+    std::vector<int32_t> rp;      // V+1
+    std::vector<int32_t> slot;    // 4T
+    // vertex -> incident (surface tri, corner), tris ascending.
+    std::vector<int32_t> trp;     // V+1
+    std::vector<int32_t> tslot;   // 3S
+    int bad_tet = -1;
+};
+
+// Builds a level: copies the mesh, precomputes, builds gather structures.
+// stiffness = ARAP weight per unit volume, density scales the lumped mass.
+int build_level(const mgsb_mesh& m, float stiffness, float density, HostLevel& L);
+
+// Barycentric binding of `n` points to tets of `mesh` following
+// PDSolver_MG::interpolate (PDSolver_MG.cpp:5-58): first tet in index order
+// whose weights (barycentricCoordinate, utils.cu:335-364) all lie in
+// [-1e-5, 1+1e-5].  Spatial-hash candidates replace the brute-force scan;
+// the lowest accepted index is the same.  Returns the number of points with no
+// tet (those get ids 0 and weights 0, as the reference leaves them).
+int bind_points_interp(const HostLevel& mesh, const float* pts, int n, std::vector<int32_t>& ids,
+                       std::vector<float>& w);
+
+// Coarse-in-fine binding of PDSolver_MG::Init (PDSolver_MG.cpp:133-168):
+// first tet in index order passing pointInTet (utils.cu:298-311), weights from
+// the precomputed-face barycentric form (utils.cu:313-333).
+int bind_points_average(const HostLevel& mesh, const float* pts, int n, std::vector<int32_t>& ids,
+                        std::vector<float>& w);
+
+// barycentricCoordinate on four points (utils.cu:335-364).
+void barycentric4(const float* P, const float* A, const float* B, const float* C, const float* D, float* w);
+
+}  // namespace mgsb

@@ -1,0 +1,1377 @@
+// libmgsb device solver: hand-written gfx950 kernels for the projective-
+// dynamics (PD) ARAP tetrahedral solver and the host orchestration of its
+// step schedules.  See include/mgsb.h for the C ABI.
+//
+// Data layout in HBM (per level):
+//   per vertex : float4 x, x_prev, s, x_last, v   (xyz + pad; 16 B aligned
+//                gathers), float4 vc = (m/h^2, diag, fixflag, m)
+//   per tet    : int4 tet, int4 slot, 9 SoA streams of Dm^-1 (B[k*T+t]),
+//                float vol
+//   per (tet, corner) in vertex-sorted order : float4 cf  (corner forces)
+//   vertex -> (tet,corner) CSR row pointer rp[V+1]
+// The per-tet kernel (one tet per lane) writes its 4 corner forces to their
+// CSR slots; the per-vertex kernel sums its contiguous CSR range in tet order
+// (deterministic, no atomics) and fuses collision response and the Jacobi /
+// under-relaxation / Chebyshev update.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "mgsb_device.h"
+#include "mgsb_internal.h"
+
+using namespace mgsb;
+
+#define HIPCHK(expr)                                                                                      \
+    do {                                                                                                  \
+        hipError_t _e = (expr);                                                                           \
+        if (_e != hipSuccess) {                                                                           \
+            set_error(std::string("HIP error ") + hipGetErrorString(_e) + " at " #expr);                  \
+            return MGSB_ERR_HIP;                                                                          \
+        }                                                                                                 \
+    } while (0)
+
+namespace {
+
+constexpr int kTetModePolar = 0;  // calculateIF
+constexpr int kTetModeAc = 1;     // calTetIFV1: drop |f| < 1e-5
+constexpr int kTetModeLinear = 2; // -F(e) AC w, no clamp: linear operator on corrections
+
+constexpr int kVertPoint = 1;     // sphere point DCD + plane
+constexpr int kVertTri = 2;       // gather triangle DCD contributions
+constexpr int kVertWriteCol = 4;  // write collision flags / depth / energy
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// calculateST (PDSolver.cu:72-116).  Block 0 / lane 0 first advances sphere
+// centres by vel*dt (UpdateCollider before Step, Simulator.cpp:146-149).
+__global__ void __launch_bounds__(256) k_st(int V, float4* __restrict__ x, float4* __restrict__ v,
+                                            float4* __restrict__ xlast, float4* __restrict__ s,
+                                            float4* __restrict__ xprev, const float4* __restrict__ vc, float dt,
+                                            float damp, float gx, float gy, float gz, DevColliders* col, int advance) {
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (advance && blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int q = 0; q < col->nsph; q++) {
+            float4 c = col->sph[q];
+            const float4 w = col->vel[q];
+            c.x = c.x + w.x * dt;
+            c.y = c.y + w.y * dt;
+            c.z = c.z + w.z * dt;
+            col->sph[q] = c;
+        }
+    }
+    if (i >= V) return;
+    float4 p = x[i], u = v[i];
+    const float fix = vc[i].z;
+    xlast[i] = p;
+    const float df = damp * fix;
+    u.x = u.x * df;
+    u.y = u.y * df;
+    u.z = u.z * df;
+    u.x = u.x + gx * dt * fix;
+    u.y = u.y + gy * dt * fix;
+    u.z = u.z + gz * dt * fix;
+    p.x = p.x + u.x * dt * fix;
+    p.y = p.y + u.y * dt * fix;
+    p.z = p.z + u.z * dt * fix;
+    v[i] = u;
+    x[i] = p;
+    s[i] = p;
+    xprev[i] = p;
+}
+
+// calculateV (PDSolver.cu:245-252).
+__global__ void __launch_bounds__(256) k_vel(int V, const float4* __restrict__ x, const float4* __restrict__ xlast,
+                                             float4* __restrict__ v, float dt) {
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const float4 p = x[i], q = xlast[i];
+    v[i] = make_float4((p.x - q.x) / dt, (p.y - q.y) / dt, (p.z - q.z) / dt, 0.f);
+}
+
+// DCDByTriangle_sphere_V2 (PDSolver.cu:612-680) for one triangle and one
+// sphere; returns the response centre - hit in mv, true on contact.
+__device__ __forceinline__ bool tri_sphere_hit(float3 A, float3 B, float3 C, float3 n, float3 center, float radius,
+                                               float3& mv) {
+    const float distance = fabsf(dot3(n, sub3(center, A)));
+    if (distance > radius) return false;
+    const float3 projected = sub3(center, mul3(n, dot3(n, sub3(center, A))));
+    float3 hit;
+    if (point_in_triangle(projected, A, B, C)) {
+        const float sign = dot3(n, sub3(center, A)) < 0 ? -1.f : 1.f;
+        hit = add3(center, mul3(mul3(n, radius - distance), sign));
+        mv = sub3(center, hit);
+        return true;
+    }
+    const float3 tv[3] = {A, B, C};
+    for (int i = 0; i < 3; i++) {
+        const float3 Bg = tv[i], En = tv[(i + 1) % 3];
+        float3 ed = sub3(En, Bg);
+        const float l = len3(ed);
+        ed = div3(ed, len3(ed));
+        const float3 bc = sub3(center, Bg);
+        const float t = dot3(bc, ed);
+        if (t < 0 || t > l) continue;
+        const float3 closest = add3(Bg, mul3(ed, t));
+        float3 radial = sub3(center, closest);
+        const float dist = len3(radial);
+        if (dist >= radius) continue;
+        const float3 Oth = tv[(i + 2) % 3];
+        if (dot3(cross3(ed, sub3(Oth, Bg)), cross3(ed, sub3(projected, Bg))) >= 0) continue;
+        radial = div3(radial, len3(radial));
+        hit = add3(closest, mul3(radial, radius));
+        mv = sub3(center, hit);
+        return true;
+    }
+    for (int i = 0; i < 3; i++) {
+        float3 VO = sub3(center, tv[i]);
+        if (len3(VO) > radius) continue;
+        VO = div3(VO, len3(VO));
+        hit = add3(tv[i], mul3(VO, radius));
+        mv = sub3(center, hit);
+        return true;
+    }
+    return false;
+}
+
+// Per-tet local step, one tet per lane: F = Ds Dm^-1, R - F by the Franca
+// polar decomposition, P = (R - F) AC with AC rebuilt from Dm^-1, NaN -> 0,
+// clamp to [-10, 10], times vol * k (calculateIF, PDSolver.cu:128-183;
+// calTetIFV1's |f| < 1e-5 drop in Ac mode, PDSolverAc.cu:630-662).  Corner j's
+// force goes to cf[slot_j].  Lanes T..T+S-1 (when ntri > 0) run the triangle-
+// sphere DCD (PDSolver.cu:612-680) and write (k*(centre-hit), k) per corner.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x, const int4* __restrict__ tet,
+                                             const float* __restrict__ B, const float* __restrict__ vol,
+                                             const int4* __restrict__ slot, float4* __restrict__ cf, float k,
+                                             int ntri, const int4* __restrict__ tri, const int4* __restrict__ tslot,
+                                             float4* __restrict__ tcf, const DevColliders* __restrict__ col) {
+    const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (g >= T) {
+        const int t = g - T;
+        if (t >= ntri) return;
+        const int4 id = tri[t];
+        const float3 A = f3(x[id.x]), Bq = f3(x[id.y]), Cq = f3(x[id.z]);
+        float3 n = cross3(sub3(Bq, A), sub3(Cq, A));
+        n = div3(n, len3(n));
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        const float kc = col->kcol;
+        for (int q = 0; q < col->nsph; q++) {
+            if (col->vel[q].w == 0.f) continue;
+            const float4 sp = col->sph[q];
+            float3 mv;
+            if (tri_sphere_hit(A, Bq, Cq, n, make_float3(sp.x, sp.y, sp.z), sp.w, mv)) {
+                acc.x = acc.x + kc * mv.x;
+                acc.y = acc.y + kc * mv.y;
+                acc.z = acc.z + kc * mv.z;
+                acc.w = acc.w + kc;
+            }
+        }
+        const int4 sl = tslot[t];
+        tcf[sl.x] = acc;
+        tcf[sl.y] = acc;
+        tcf[sl.z] = acc;
+        return;
+    }
+    const int4 id = tet[g];
+    const float4 p0 = x[id.x], p1 = x[id.y], p2 = x[id.z], p3 = x[id.w];
+    float b[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) b[q] = B[(size_t)q * T + g];
+    float D[9];
+    D[0] = p1.x - p0.x; D[1] = p2.x - p0.x; D[2] = p3.x - p0.x;
+    D[3] = p1.y - p0.y; D[4] = p2.y - p0.y; D[5] = p3.y - p0.y;
+    D[6] = p1.z - p0.z; D[7] = p2.z - p0.z; D[8] = p3.z - p0.z;
+    float F[9];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) F[r * 3 + c] = D[r * 3 + 0] * b[c] + D[r * 3 + 1] * b[3 + c] + D[r * 3 + 2] * b[6 + c];
+    float RF[9];
+    if (MODE == kTetModeLinear) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) RF[q] = -F[q];
+    } else {
+        polar_rf(F, RF);
+    }
+    // AC[r][0] = -B[0][r] - B[1][r] - B[2][r], AC[r][j] = B[j-1][r]
+    float AC[12];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        AC[r * 4 + 0] = -b[r] - b[3 + r] - b[6 + r];
+        AC[r * 4 + 1] = b[r];
+        AC[r * 4 + 2] = b[3 + r];
+        AC[r * 4 + 3] = b[6 + r];
+    }
+    const float vt = vol[g];
+    const int4 sl = slot[g];
+    const int sls[4] = {sl.x, sl.y, sl.z, sl.w};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        float f[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            float acc = RF[i * 3 + 0] * AC[0 * 4 + j] + RF[i * 3 + 1] * AC[1 * 4 + j] + RF[i * 3 + 2] * AC[2 * 4 + j];
+            if (MODE != kTetModeLinear) {
+                if (isnan(acc)) acc = 0.f;
+                acc = acc > 10.f ? 10.f : acc;
+                acc = acc < -10.f ? -10.f : acc;
+            }
+            float r = acc * vt * k;
+            // |r| < 1e-5 against the double literal is exactly |r| <= 1e-5f
+            if (MODE == kTetModeAc && fabsf(r) <= 1e-5f) r = 0.f;
+            f[i] = r;
+        }
+        cf[sls[j]] = make_float4(f[0], f[1], f[2], 0.f);
+    }
+}
+
+// Per-vertex update: gather the CSR range of corner forces (tet order), add
+// collision response, then calculatePOS (PDSolver.cu:192-235): Jacobi step,
+// under-relaxation, Chebyshev extrapolation with omega.  Sphere point DCD is
+// DCDByPoint_sphere (PDSolver.cu:277-331) including its 1.5x radius and t/2.
+__global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, float4* __restrict__ xprev,
+                                              const float4* __restrict__ s, const float4* __restrict__ vc,
+                                              const int* __restrict__ rp, const float4* __restrict__ cf,
+                                              const int* __restrict__ trp, const float4* __restrict__ tcf,
+                                              const DevColliders* __restrict__ col, float omega, float relax,
+                                              int flags, int* __restrict__ cflag, float* __restrict__ cdepth,
+                                              float* __restrict__ cenergy) {
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    const int e0 = rp[i], e1 = rp[i + 1];
+    for (int e = e0; e < e1; e++) {
+        const float4 q = cf[e];
+        fx = fx + q.x;
+        fy = fy + q.y;
+        fz = fz + q.z;
+    }
+    const float4 p = x[i];
+    float cdx = 0.f, cdy = 0.f, cdz = 0.f, cfx = 0.f, cfy = 0.f, cfz = 0.f;
+    int flag = 0;
+    float depth = 0.f, energy = 0.f;
+    if (flags & kVertTri) {
+        const int t0 = trp[i], t1 = trp[i + 1];
+        for (int e = t0; e < t1; e++) {
+            const float4 q = tcf[e];
+            if (q.w != 0.f) {
+                flag += 1;
+                cdx = cdx + q.w;
+                cdy = cdy + q.w;
+                cdz = cdz + q.w;
+                cfx = cfx + q.x;
+                cfy = cfy + q.y;
+                cfz = cfz + q.z;
+            }
+        }
+    }
+    if (flags & kVertPoint) {
+        const float kc = col->kcol;
+        const float3 P = f3(p);
+        for (int q = 0; q < col->nsph; q++) {
+            if (col->vel[q].w == 0.f) continue;
+            const float4 sp = col->sph[q];
+            const float3 center = make_float3(sp.x, sp.y, sp.z);
+            const float3 cp = sub3(P, center);
+            const float3 dir = div3(cp, len3(cp));
+            const float distance = len3(cp);
+            const float oldR = sp.w;
+            const float r = sp.w * 1.5f;
+            if (distance < r) {
+                const float bq = 2 * dot3(cp, dir);
+                const float cq = dot3(cp, cp) - r * r;
+                float t = (-bq + sqrtf(bq * bq - 4.f * cq)) / 2.f;
+                if (t > 0.5f * oldR) {
+                    flag = 1;
+                    depth = t - 0.5f * oldR;
+                }
+                t = t / 2.0f;
+                const float3 colP = add3(P, mul3(dir, t));
+                const float3 ccp = sub3(colP, center);
+                const float3 n = div3(ccp, len3(ccp));
+                const float nd = dot3(n, dir);
+                cdx = cdx + kc * n.x * n.x;
+                cdy = cdy + kc * n.y * n.y;
+                cdz = cdz + kc * n.z * n.z;
+                cfx = cfx + kc * t * nd * n.x;
+                cfy = cfy + kc * t * nd * n.y;
+                cfz = cfz + kc * t * nd * n.z;
+                const float ee = dot3(sub3(P, colP), n);
+                energy = energy + kc * (ee * ee);
+            }
+        }
+        if (col->plane_on) {
+            const float4 pl = col->plane;
+            const float3 n = make_float3(pl.x, pl.y, pl.z);
+            const float sd = dot3(n, P) - pl.w;
+            if (sd < 0.f) {
+                const float dp = -sd;
+                const float kp = col->kplane;
+                flag = 1;
+                if (dp > depth) depth = dp;
+                cdx = cdx + kp * n.x * n.x;
+                cdy = cdy + kp * n.y * n.y;
+                cdz = cdz + kp * n.z * n.z;
+                cfx = cfx + kp * dp * n.x;
+                cfy = cfy + kp * dp * n.y;
+                cfz = cfz + kp * dp * n.z;
+                energy = energy + kp * (dp * dp);
+            }
+        }
+    }
+    if (flags & kVertWriteCol) {
+        cflag[i] = flag;
+        cdepth[i] = depth;
+        cenergy[i] = energy;
+    }
+    const float4 c = vc[i];  // (m/h^2, diag, fix, m)
+    const float dc = c.x, dg = c.y, fix = c.z;
+    const float4 st = s[i], pp = xprev[i];
+    float nx = (dc * (st.x - p.x) + (fx + cfx)) / (dg + cdx + dc) * fix + p.x;
+    float ny = (dc * (st.y - p.y) + (fy + cfy)) / (dg + cdy + dc) * fix + p.y;
+    float nz = (dc * (st.z - p.z) + (fz + cfz)) / (dg + cdz + dc) * fix + p.z;
+    nx = (nx - p.x) * relax + p.x;
+    ny = (ny - p.y) * relax + p.y;
+    nz = (nz - p.z) * relax + p.z;
+    nx = omega * (nx - pp.x) + pp.x;
+    ny = omega * (ny - pp.y) + pp.y;
+    nz = omega * (nz - pp.z) + pp.z;
+    xprev[i] = p;
+    x[i] = make_float4(nx, ny, nz, 0.f);
+}
+
+// updatePointInTet (PDSolver.cu:724-744): dst.x = dst.x_prev = sum w src.x.
+__global__ void __launch_bounds__(256) k_interp(int N, float4* __restrict__ dx, float4* __restrict__ dxprev,
+                                                const float4* __restrict__ src, const int4* __restrict__ ids,
+                                                const float4* __restrict__ w) {
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int4 id = ids[i];
+    const float4 ww = w[i];
+    const float4 a = src[id.x], b = src[id.y], c = src[id.z], d = src[id.w];
+    const float4 r = make_float4(ww.x * a.x + ww.y * b.x + ww.z * c.x + ww.w * d.x,
+                                 ww.x * a.y + ww.y * b.y + ww.z * c.y + ww.w * d.y,
+                                 ww.x * a.z + ww.y * b.z + ww.z * c.z + ww.w * d.z, 0.f);
+    dx[i] = r;
+    dxprev[i] = r;
+}
+
+// updateMapping (PDSolver.cu:764-785).
+__global__ void __launch_bounds__(256) k_update_mapping(int N, const float4* __restrict__ xf,
+                                                        const float4* __restrict__ xc, const int4* __restrict__ ids,
+                                                        float4* __restrict__ w) {
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int4 id = ids[i];
+    float ww[4];
+    barycentric4(f3(xf[i]), f3(xc[id.x]), f3(xc[id.y]), f3(xc[id.z]), f3(xc[id.w]), ww);
+    w[i] = make_float4(ww[0], ww[1], ww[2], ww[3]);
+}
+
+// UpdateTriNormal (PDSolver.cu:394-425): unit normal per surface triangle.
+__global__ void __launch_bounds__(256) k_tri_normal(int S, const float4* __restrict__ x, const int4* __restrict__ tri,
+                                                    float4* __restrict__ tn) {
+    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= S) return;
+    const int4 id = tri[t];
+    const float3 A = f3(x[id.x]), B = f3(x[id.y]), C = f3(x[id.z]);
+    float3 n = cross3(sub3(B, A), sub3(C, A));
+    n = div3(n, len3(n));
+    tn[t] = make_float4(n.x, n.y, n.z, 0.f);
+}
+
+// AvgOutsideTetVertNormal (PDSolver.cu:436-447) over a deterministic
+// vertex -> triangle gather instead of atomics.
+__global__ void __launch_bounds__(256) k_vert_normal(int V, const int* __restrict__ trp,
+                                                     const int* __restrict__ trid, const float4* __restrict__ tn,
+                                                     float4* __restrict__ vn) {
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    float nx = 0.f, ny = 0.f, nz = 0.f;
+    for (int e = trp[i]; e < trp[i + 1]; e++) {
+        const float4 q = tn[trid[e]];
+        nx = nx + q.x;
+        ny = ny + q.y;
+        nz = nz + q.z;
+    }
+    const float l = sqrtf(nx * nx + ny * ny + nz * nz);
+    if (l > 1.1920928955078125e-7f) {
+        nx = nx / l;
+        ny = ny / l;
+        nz = nz / l;
+    }
+    vn[i] = make_float4(nx, ny, nz, 0.f);
+}
+
+// Block reduction of NV doubles (wave64 shuffles, then LDS across waves).
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* __restrict__ out) {
+    __shared__ double sh[NV][8];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        double a = v[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+        if (lane == 0) sh[q][wid] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const int nw = (blockDim.x + 63) >> 6;
+        double a = 0.0;
+        for (int w = 0; w < nw; w++) a += sh[threadIdx.x][w];
+        out[(size_t)blockIdx.x * NV + threadIdx.x] = a;
+    }
+}
+
+// Final pass of a two-stage reduction: one block sums nb partials of NV values
+// in block order (deterministic).
+template <int NV>
+__global__ void __launch_bounds__(256) k_reduce_final(int nb, const double* __restrict__ part,
+                                                      double* __restrict__ out) {
+    double v[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) v[q] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < NV; q++) v[q] += part[(size_t)b * NV + q];
+    __shared__ double sh[NV][256];
+#pragma unroll
+    for (int q = 0; q < NV; q++) sh[q][threadIdx.x] = v[q];
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double a = 0.0;
+        for (int t = 0; t < (int)blockDim.x; t++) a += sh[threadIdx.x][t];
+        out[threadIdx.x] = a;
+    }
+}
+
+// runCalEnergy (PDSolver.cu:816-913), per-vertex part: dX^2, m |x-s|^2,
+// collision energy, collided count (FP64 partial sums).
+__global__ void __launch_bounds__(256) k_energy_vert(int V, const float4* __restrict__ x,
+                                                     const float4* __restrict__ xprev, const float4* __restrict__ s,
+                                                     const float4* __restrict__ vc, const int* __restrict__ cflag,
+                                                     const float* __restrict__ cenergy, double* __restrict__ part) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < V) {
+        const float4 p = x[i], q = xprev[i], st = s[i];
+        const double ax = (double)p.x - q.x, ay = (double)p.y - q.y, az = (double)p.z - q.z;
+        const double bx = (double)p.x - st.x, by = (double)p.y - st.y, bz = (double)p.z - st.z;
+        v[0] = ax * ax + ay * ay + az * az;
+        v[1] = (double)vc[i].w * (bx * bx + by * by + bz * bz);
+        v[2] = (double)cenergy[i];
+        v[3] = cflag[i] > 0 ? 1.0 : 0.0;
+    }
+    block_reduce_store<4>(v, part);
+}
+
+// Per-tet part: k vol / 2 |F - R|^2.
+__global__ void __launch_bounds__(256) k_energy_tet(int T, const float4* __restrict__ x,
+                                                    const int4* __restrict__ tet, const float* __restrict__ B,
+                                                    const float* __restrict__ vol, float k, double* __restrict__ part) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {0.0};
+    if (g < T) {
+        const int4 id = tet[g];
+        const float4 p0 = x[id.x], p1 = x[id.y], p2 = x[id.z], p3 = x[id.w];
+        float b[9];
+        for (int q = 0; q < 9; q++) b[q] = B[(size_t)q * T + g];
+        float D[9] = {p1.x - p0.x, p2.x - p0.x, p3.x - p0.x, p1.y - p0.y, p2.y - p0.y,
+                      p3.y - p0.y, p1.z - p0.z, p2.z - p0.z, p3.z - p0.z};
+        float F[9], RF[9];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) F[r * 3 + c] = D[r * 3 + 0] * b[c] + D[r * 3 + 1] * b[3 + c] + D[r * 3 + 2] * b[6 + c];
+        polar_rf(F, RF);
+        double e = 0.0;
+        for (int q = 0; q < 9; q++) e += (double)RF[q] * (double)RF[q];
+        v[0] = e * (double)vol[g] * (double)k / 2.0;
+    }
+    block_reduce_store<1>(v, part);
+}
+
+// Nonlinear residual g = m/h^2 (s - x) + f_el + f_col over free DOFs; writes
+// g (float4) when r != nullptr and the FP64 partial of |g|^2.
+__global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__ x, const float4* __restrict__ s,
+                                               const float4* __restrict__ vc, const int* __restrict__ rp,
+                                               const float4* __restrict__ cf, const DevColliders* __restrict__ col,
+                                               int flags, float4* __restrict__ r, double* __restrict__ part) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {0.0};
+    if (i < V) {
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        for (int e = rp[i]; e < rp[i + 1]; e++) {
+            const float4 q = cf[e];
+            fx = fx + q.x;
+            fy = fy + q.y;
+            fz = fz + q.z;
+        }
+        const float4 p = x[i];
+        float cfx = 0.f, cfy = 0.f, cfz = 0.f;
+        if (flags & kVertPoint) {
+            const float kc = col->kcol;
+            const float3 P = f3(p);
+            for (int q = 0; q < col->nsph; q++) {
+                if (col->vel[q].w == 0.f) continue;
+                const float4 sp = col->sph[q];
+                const float3 center = make_float3(sp.x, sp.y, sp.z);
+                const float3 cp = sub3(P, center);
+                const float3 dir = div3(cp, len3(cp));
+                const float distance = len3(cp);
+                const float rr = sp.w * 1.5f;
+                if (distance < rr) {
+                    const float bq = 2 * dot3(cp, dir);
+                    const float cq = dot3(cp, cp) - rr * rr;
+                    float t = (-bq + sqrtf(bq * bq - 4.f * cq)) / 2.f;
+                    t = t / 2.0f;
+                    const float3 colP = add3(P, mul3(dir, t));
+                    const float3 ccp = sub3(colP, center);
+                    const float3 n = div3(ccp, len3(ccp));
+                    const float nd = dot3(n, dir);
+                    cfx = cfx + kc * t * nd * n.x;
+                    cfy = cfy + kc * t * nd * n.y;
+                    cfz = cfz + kc * t * nd * n.z;
+                }
+            }
+            if (col->plane_on) {
+                const float4 pl = col->plane;
+                const float sd = pl.x * p.x + pl.y * p.y + pl.z * p.z - pl.w;
+                if (sd < 0.f) {
+                    cfx = cfx + col->kplane * (-sd) * pl.x;
+                    cfy = cfy + col->kplane * (-sd) * pl.y;
+                    cfz = cfz + col->kplane * (-sd) * pl.z;
+                }
+            }
+        }
+        const float4 c = vc[i], st = s[i];
+        const float fix = c.z;
+        const float gx = (c.x * (st.x - p.x) + (fx + cfx)) * fix;
+        const float gy = (c.x * (st.y - p.y) + (fy + cfy)) * fix;
+        const float gz = (c.x * (st.z - p.z) + (fz + cfz)) * fix;
+        if (r) r[i] = make_float4(gx, gy, gz, 0.f);
+        v[0] = (double)gx * gx + (double)gy * gy + (double)gz * gz;
+    }
+    block_reduce_store<1>(v, part);
+}
+
+inline int nblk(int n, int bs) { return (n + bs - 1) / bs; }
+
+// ---------------------------------------------------------------------------
+// Host structures
+// ---------------------------------------------------------------------------
+template <class T>
+int dalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        return MGSB_ERR_NOMEM;
+    }
+    return MGSB_OK;
+}
+
+struct DevLevel {
+    HostLevel h;
+    int V = 0, T = 0, S = 0;
+    float4 *x = nullptr, *xprev = nullptr, *s = nullptr, *xlast = nullptr, *v = nullptr, *vc = nullptr;
+    int4 *tet = nullptr, *slot = nullptr;
+    float *B = nullptr, *vol = nullptr;
+    int* rp = nullptr;
+    float4* cf = nullptr;
+    int4 *tri = nullptr, *tslot = nullptr;
+    int *trp = nullptr, *trid = nullptr;
+    float4 *tcf = nullptr, *tn = nullptr, *vn = nullptr;
+    int* cflag = nullptr;
+    float *cdepth = nullptr, *cenergy = nullptr;
+    std::vector<void*> owned;
+};
+
+struct LevelMap {  // between level l (fine) and l+1 (coarse)
+    int4* iids = nullptr;   // fine vertex -> coarse tet corners
+    float4* iw = nullptr;
+    int4* aids = nullptr;   // coarse vertex -> fine tet corners
+    float4* aw = nullptr;
+    std::vector<int32_t> h_iids, h_aids;
+    std::vector<float> h_iw, h_aw;
+};
+
+}  // namespace
+
+struct mgsb_solver {
+    mgsb_params p{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<DevLevel> L;
+    std::vector<LevelMap> M;
+    DevColliders hcol{};
+    DevColliders* dcol = nullptr;
+    double* dpart = nullptr;  // reduction partials
+    size_t npart = 0;
+    double* dred = nullptr;   // reduction results (16)
+    bool graph_valid = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+    size_t ev_used = 0;
+    double time_ms = 0.0;
+    int64_t time_launches = 0;
+    mgsb_step_stats last{};
+    std::vector<void*> owned;
+
+    ~mgsb_solver() {
+        if (stream) hipStreamSynchronize(stream);
+        if (gexec) hipGraphExecDestroy(gexec);
+        if (graph) hipGraphDestroy(graph);
+        for (auto& e : ev_pool) {
+            hipEventDestroy(e.first);
+            hipEventDestroy(e.second);
+        }
+        for (auto& lv : L)
+            for (void* q : lv.owned) hipFree(q);
+        for (void* q : owned) hipFree(q);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <class T>
+int up(mgsb_solver* s, std::vector<void*>& own, T** d, const T* h, size_t n) {
+    int rc = dalloc(d, n);
+    if (rc) return rc;
+    own.push_back((void*)*d);
+    if (h && n) HIPCHK(hipMemcpyAsync(*d, h, n * sizeof(T), hipMemcpyHostToDevice, s->stream));
+    return MGSB_OK;
+}
+
+int upload_vc(mgsb_solver* s, DevLevel& d) {
+    std::vector<float4> vc(d.V);
+    const float dt = s->p.dt;
+    for (int i = 0; i < d.V; i++) vc[i] = make_float4(d.h.mass[i] / (dt * dt), d.h.diag[i], d.h.fix[i], d.h.mass[i]);
+    HIPCHK(hipMemcpyAsync(d.vc, vc.data(), sizeof(float4) * d.V, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return MGSB_OK;
+}
+
+int upload_level(mgsb_solver* s, DevLevel& d) {
+    const HostLevel& h = d.h;
+    d.V = h.V;
+    d.T = h.T;
+    d.S = h.S;
+    int rc;
+    std::vector<float4> x4(h.V);
+    for (int i = 0; i < h.V; i++) x4[i] = make_float4(h.X[i * 3], h.X[i * 3 + 1], h.X[i * 3 + 2], 0.f);
+    std::vector<float4> zero(h.V, make_float4(0.f, 0.f, 0.f, 0.f));
+    if ((rc = up(s, d.owned, &d.x, x4.data(), h.V))) return rc;
+    if ((rc = up(s, d.owned, &d.xprev, x4.data(), h.V))) return rc;
+    if ((rc = up(s, d.owned, &d.s, x4.data(), h.V))) return rc;
+    if ((rc = up(s, d.owned, &d.xlast, x4.data(), h.V))) return rc;
+    if ((rc = up(s, d.owned, &d.v, zero.data(), h.V))) return rc;
+    if ((rc = up<float4>(s, d.owned, &d.vc, nullptr, h.V))) return rc;
+    std::vector<int4> t4(h.T), s4(h.T);
+    for (int t = 0; t < h.T; t++) {
+        t4[t] = make_int4(h.tets[t * 4], h.tets[t * 4 + 1], h.tets[t * 4 + 2], h.tets[t * 4 + 3]);
+        s4[t] = make_int4(h.slot[t * 4], h.slot[t * 4 + 1], h.slot[t * 4 + 2], h.slot[t * 4 + 3]);
+    }
+    if ((rc = up(s, d.owned, &d.tet, t4.data(), h.T))) return rc;
+    if ((rc = up(s, d.owned, &d.slot, s4.data(), h.T))) return rc;
+    std::vector<float> bs((size_t)h.T * 9);
+    for (int t = 0; t < h.T; t++)
+        for (int q = 0; q < 9; q++) bs[(size_t)q * h.T + t] = h.B[(size_t)t * 9 + q];
+    if ((rc = up(s, d.owned, &d.B, bs.data(), bs.size()))) return rc;
+    if ((rc = up(s, d.owned, &d.vol, h.vol.data(), h.T))) return rc;
+    if ((rc = up(s, d.owned, &d.rp, h.rp.data(), h.rp.size()))) return rc;
+    if ((rc = up<float4>(s, d.owned, &d.cf, nullptr, (size_t)h.T * 4))) return rc;
+    std::vector<int4> tr4(std::max(1, h.S)), ts4(std::max(1, h.S));
+    for (int t = 0; t < h.S; t++) {
+        tr4[t] = make_int4((int)h.tris[t * 3], (int)h.tris[t * 3 + 1], (int)h.tris[t * 3 + 2], 0);
+        ts4[t] = make_int4(h.tslot[t * 3], h.tslot[t * 3 + 1], h.tslot[t * 3 + 2], 0);
+    }
+    // vertex -> triangle ids in CSR order (for vertex normals)
+    std::vector<int32_t> trid(std::max<size_t>(1, (size_t)h.S * 3));
+    for (int t = 0; t < h.S; t++)
+        for (int c = 0; c < 3; c++) trid[h.tslot[t * 3 + c]] = t;
+    if ((rc = up(s, d.owned, &d.tri, tr4.data(), tr4.size()))) return rc;
+    if ((rc = up(s, d.owned, &d.tslot, ts4.data(), ts4.size()))) return rc;
+    if ((rc = up(s, d.owned, &d.trp, h.trp.data(), h.trp.size()))) return rc;
+    if ((rc = up(s, d.owned, &d.trid, trid.data(), trid.size()))) return rc;
+    std::vector<float4> tz(std::max<size_t>(1, (size_t)h.S * 3), make_float4(0.f, 0.f, 0.f, 0.f));
+    if ((rc = up(s, d.owned, &d.tcf, tz.data(), tz.size()))) return rc;
+    if ((rc = up<float4>(s, d.owned, &d.tn, nullptr, std::max(1, h.S)))) return rc;
+    if ((rc = up<float4>(s, d.owned, &d.vn, nullptr, h.V))) return rc;
+    std::vector<int> iz(h.V, 0);
+    std::vector<float> fz(h.V, 0.f);
+    if ((rc = up(s, d.owned, &d.cflag, iz.data(), h.V))) return rc;
+    if ((rc = up(s, d.owned, &d.cdepth, fz.data(), h.V))) return rc;
+    if ((rc = up(s, d.owned, &d.cenergy, fz.data(), h.V))) return rc;
+    return upload_vc(s, d);
+}
+
+int push_colliders(mgsb_solver* s) {
+    HIPCHK(hipMemcpyAsync(s->dcol, &s->hcol, sizeof(DevColliders), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return MGSB_OK;
+}
+
+int pull_colliders(mgsb_solver* s) {
+    HIPCHK(hipMemcpyAsync(&s->hcol, s->dcol, sizeof(DevColliders), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return MGSB_OK;
+}
+
+int block_tet(const mgsb_solver* s) { return s->p.block_tet > 0 ? s->p.block_tet : 256; }
+int block_vert(const mgsb_solver* s) { return s->p.block_vert > 0 ? s->p.block_vert : 256; }
+
+// --- launch helpers (all on s->stream) --------------------------------------
+int timed_begin(mgsb_solver* s, int kernel) {
+    if (!s->timing || kernel != 0) return MGSB_OK;
+    if (s->ev_used == s->ev_pool.size()) {
+        hipEvent_t a, b;
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        s->ev_pool.push_back({a, b});
+    }
+    HIPCHK(hipEventRecord(s->ev_pool[s->ev_used].first, s->stream));
+    return MGSB_OK;
+}
+int timed_end(mgsb_solver* s, int kernel) {
+    if (!s->timing || kernel != 0) return MGSB_OK;
+    HIPCHK(hipEventRecord(s->ev_pool[s->ev_used].second, s->stream));
+    s->ev_used++;
+    return MGSB_OK;
+}
+
+int launch_tet(mgsb_solver* s, DevLevel& d, int mode, bool with_tri, const float4* xsrc = nullptr) {
+    const int bs = block_tet(s);
+    const int ntri = with_tri ? d.S : 0;
+    const int n = d.T + ntri;
+    const float4* xx = xsrc ? xsrc : d.x;
+    int rc = timed_begin(s, 0);
+    if (rc) return rc;
+    if (mode == kTetModePolar)
+        k_tet<kTetModePolar><<<nblk(n, bs), bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf,
+                                                                   s->p.stiffness, ntri, d.tri, d.tslot, d.tcf, s->dcol);
+    else if (mode == kTetModeAc)
+        k_tet<kTetModeAc><<<nblk(n, bs), bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf,
+                                                                s->p.stiffness, ntri, d.tri, d.tslot, d.tcf, s->dcol);
+    else
+        k_tet<kTetModeLinear><<<nblk(n, bs), bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf,
+                                                                    s->p.stiffness, ntri, d.tri, d.tslot, d.tcf, s->dcol);
+    HIPCHK(hipGetLastError());
+    return timed_end(s, 0);
+}
+
+int launch_vert(mgsb_solver* s, DevLevel& d, float omega, int flags) {
+    const int bs = block_vert(s);
+    k_vert<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.x, d.xprev, d.s, d.vc, d.rp, d.cf, d.trp, d.tcf, s->dcol, omega,
+                                                 s->p.relax, flags, d.cflag, d.cdepth, d.cenergy);
+    HIPCHK(hipGetLastError());
+    return MGSB_OK;
+}
+
+int launch_st(mgsb_solver* s, DevLevel& d, int advance) {
+    const int bs = block_vert(s);
+    k_st<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.x, d.v, d.xlast, d.s, d.xprev, d.vc, s->p.dt, s->p.damping,
+                                               s->p.gravity[0], s->p.gravity[1], s->p.gravity[2], s->dcol, advance);
+    HIPCHK(hipGetLastError());
+    return MGSB_OK;
+}
+
+int launch_vel(mgsb_solver* s, DevLevel& d) {
+    const int bs = block_vert(s);
+    k_vel<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.x, d.xlast, d.v, s->p.dt);
+    HIPCHK(hipGetLastError());
+    return MGSB_OK;
+}
+
+// One reference PD iteration on a level (runClearTemp, DCD, IF, omega, POS).
+int pd_iteration(mgsb_solver* s, DevLevel& d, int mode, float omega, bool tri_dcd, bool last) {
+    int rc = launch_tet(s, d, mode, tri_dcd);
+    if (rc) return rc;
+    int flags = tri_dcd ? kVertTri : kVertPoint;
+    if (last) flags |= kVertWriteCol;
+    return launch_vert(s, d, omega, flags);
+}
+
+// PDSolver::Step (PDSolver.cpp:243-293).
+int enqueue_step_single(mgsb_solver* s) {
+    DevLevel& d = s->L[0];
+    int rc = launch_st(s, d, 1);
+    if (rc) return rc;
+    float omega = 1.0f;
+    const float rho = s->p.rho;
+    for (int i = 0; i < s->p.iters; i++) {
+        omega = 4 / (4 - rho * rho * omega);
+        if ((rc = pd_iteration(s, d, s->p.if_mode, omega, false, i == s->p.iters - 1))) return rc;
+    }
+    return launch_vel(s, d);
+}
+
+// PDSolver_MG::Step (PDSolver_MG.cpp:183-253) between levels 1 (coarse) and 0.
+int enqueue_step_twolevel(mgsb_solver* s) {
+    DevLevel& f = s->L[0];
+    DevLevel& c = s->L[1];
+    LevelMap& m = s->M[0];
+    int rc;
+    if ((rc = launch_st(s, c, 1))) return rc;
+    if ((rc = launch_st(s, f, 0))) return rc;
+    const float rho = s->p.rho;
+    float omega = 1.0f;
+    for (int i = 0; i < s->p.iters_coarse; i++) {
+        omega = 4 / (4 - rho * rho * omega);
+        if ((rc = pd_iteration(s, c, s->p.if_mode, omega, s->p.tri_dcd_coarse != 0, i == s->p.iters_coarse - 1)))
+            return rc;
+    }
+    const int bs = block_vert(s);
+    k_interp<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, c.x, m.iids, m.iw);
+    HIPCHK(hipGetLastError());
+    omega = 1.0f;
+    for (int i = 0; i < s->p.iters; i++) {
+        omega = 4 / (4 - rho * rho * omega);
+        if ((rc = pd_iteration(s, f, s->p.if_mode, omega, false, i == s->p.iters - 1))) return rc;
+    }
+    k_interp<<<nblk(c.V, bs), bs, 0, s->stream>>>(c.V, c.x, c.xprev, f.x, m.aids, m.aw);
+    HIPCHK(hipGetLastError());
+    k_update_mapping<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, c.x, m.iids, m.iw);
+    HIPCHK(hipGetLastError());
+    if ((rc = launch_vel(s, c))) return rc;
+    return launch_vel(s, f);
+}
+
+int enqueue_step(mgsb_solver* s) {
+    switch (s->p.schedule) {
+        case MGSB_SCHED_SINGLE: return enqueue_step_single(s);
+        case MGSB_SCHED_TWOLEVEL: return enqueue_step_twolevel(s);
+        default: set_error("schedule not available"); return MGSB_ERR_ARG;
+    }
+}
+
+int ensure_partials(mgsb_solver* s, size_t n) {
+    if (s->npart >= n) return MGSB_OK;
+    if (s->dpart) HIPCHK(hipFree(s->dpart));
+    s->dpart = nullptr;
+    int rc = dalloc(&s->dpart, n);
+    if (rc) return rc;
+    s->npart = n;
+    return MGSB_OK;
+}
+
+int reduce_finish(mgsb_solver* s, int nb, int nv, double* host_out) {
+    if (nv == 1)
+        k_reduce_final<1><<<1, 256, 0, s->stream>>>(nb, s->dpart, s->dred);
+    else
+        k_reduce_final<4><<<1, 256, 0, s->stream>>>(nb, s->dpart, s->dred);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host_out, s->dred, sizeof(double) * nv, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return MGSB_OK;
+}
+
+// ||g(x)|| of level 0 (FP64).
+int residual_norm(mgsb_solver* s, double* out) {
+    DevLevel& d = s->L[0];
+    int rc = launch_tet(s, d, kTetModePolar, false);
+    if (rc) return rc;
+    const int nb = nblk(d.V, 256);
+    if ((rc = ensure_partials(s, (size_t)nb * 4))) return rc;
+    k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, nullptr, s->dpart);
+    HIPCHK(hipGetLastError());
+    double sq = 0;
+    if ((rc = reduce_finish(s, nb, 1, &sq))) return rc;
+    *out = std::sqrt(sq);
+    return MGSB_OK;
+}
+
+int check_level(mgsb_solver* s, int level) {
+    if (!s) {
+        set_error("null solver");
+        return MGSB_ERR_ARG;
+    }
+    if (level < 0 || level >= (int)s->L.size()) {
+        set_error("level out of range");
+        return MGSB_ERR_ARG;
+    }
+    return MGSB_OK;
+}
+
+int read_f4(mgsb_solver* s, const float4* d, int n, float* out) {
+    std::vector<float4> h(n);
+    HIPCHK(hipMemcpyAsync(h.data(), d, sizeof(float4) * n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (int i = 0; i < n; i++) {
+        out[i * 3] = h[i].x;
+        out[i * 3 + 1] = h[i].y;
+        out[i * 3 + 2] = h[i].z;
+    }
+    return MGSB_OK;
+}
+
+int write_f4(mgsb_solver* s, float4* d, int n, const float* in) {
+    std::vector<float4> h(n);
+    for (int i = 0; i < n; i++) h[i] = make_float4(in[i * 3], in[i * 3 + 1], in[i * 3 + 2], 0.f);
+    HIPCHK(hipMemcpyAsync(d, h.data(), sizeof(float4) * n, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return MGSB_OK;
+}
+
+int validate_params(const mgsb_params* p, int nlevels) {
+    if (!(p->dt > 0) || p->iters < 0 || p->iters_coarse < 0) {
+        set_error("invalid params: dt must be > 0, iteration counts >= 0");
+        return MGSB_ERR_ARG;
+    }
+    if (p->schedule == MGSB_SCHED_TWOLEVEL && nlevels < 2) {
+        set_error("two-level schedule needs 2 levels");
+        return MGSB_ERR_ARG;
+    }
+    if (p->schedule < 0 || p->schedule > MGSB_SCHED_LEARNED) {
+        set_error("unknown schedule");
+        return MGSB_ERR_ARG;
+    }
+    if (p->if_mode != MGSB_IF_PLAIN && p->if_mode != MGSB_IF_AC) {
+        set_error("unknown if_mode");
+        return MGSB_ERR_ARG;
+    }
+    const int bt = p->block_tet, bv = p->block_vert;
+    auto okb = [](int b) { return b == 0 || (b >= 64 && b <= 256 && b % 64 == 0); };
+    if (!okb(bt) || !okb(bv)) {
+        set_error("block sizes must be 0 or a multiple of 64 in [64, 256]");
+        return MGSB_ERR_ARG;
+    }
+    return MGSB_OK;
+}
+
+int invalidate_graph(mgsb_solver* s) {
+    if (s->gexec) HIPCHK(hipGraphExecDestroy(s->gexec));
+    if (s->graph) HIPCHK(hipGraphDestroy(s->graph));
+    s->gexec = nullptr;
+    s->graph = nullptr;
+    s->graph_valid = false;
+    return MGSB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mgsb_create(const mgsb_mesh* levels, int32_t nlevels, const mgsb_params* p, mgsb_solver** out) {
+    try {
+        if (!levels || nlevels < 1 || !p || !out) {
+            set_error("mgsb_create: bad arguments");
+            return MGSB_ERR_ARG;
+        }
+        *out = nullptr;
+        int rc = validate_params(p, nlevels);
+        if (rc) return rc;
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        if (p->device < 0 || p->device >= ndev) {
+            set_error("mgsb_create: no HIP device " + std::to_string(p->device));
+            return MGSB_ERR_HIP;
+        }
+        HIPCHK(hipSetDevice(p->device));
+        auto s = std::make_unique<mgsb_solver>();
+        s->p = *p;
+        s->device = p->device;
+        HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        s->L.resize(nlevels);
+        for (int l = 0; l < nlevels; l++) {
+            if ((rc = build_level(levels[l], p->stiffness, p->density, s->L[l].h))) return rc;
+        }
+        s->M.resize(nlevels - 1);
+        for (int l = 0; l + 1 < nlevels; l++) {
+            HostLevel& fine = s->L[l].h;
+            HostLevel& coarse = s->L[l + 1].h;
+            LevelMap& m = s->M[l];
+            int uf = bind_points_interp(coarse, fine.X.data(), fine.V, m.h_iids, m.h_iw);
+            if (uf) {
+                set_error("mgsb_create: " + std::to_string(uf) + " vertices of level " + std::to_string(l) +
+                          " lie in no tet of level " + std::to_string(l + 1));
+                return MGSB_ERR_MESH;
+            }
+            int uc = bind_points_average(fine, coarse.X.data(), coarse.V, m.h_aids, m.h_aw);
+            if (uc) {
+                set_error("mgsb_create: " + std::to_string(uc) + " vertices of level " + std::to_string(l + 1) +
+                          " lie in no tet of level " + std::to_string(l));
+                return MGSB_ERR_MESH;
+            }
+        }
+        for (auto& d : s->L)
+            if ((rc = upload_level(s.get(), d))) return rc;
+        for (auto& m : s->M) {
+            std::vector<int4> i4(m.h_iids.size() / 4), a4(m.h_aids.size() / 4);
+            std::vector<float4> w4(i4.size()), b4(a4.size());
+            for (size_t i = 0; i < i4.size(); i++) {
+                i4[i] = make_int4(m.h_iids[i * 4], m.h_iids[i * 4 + 1], m.h_iids[i * 4 + 2], m.h_iids[i * 4 + 3]);
+                w4[i] = make_float4(m.h_iw[i * 4], m.h_iw[i * 4 + 1], m.h_iw[i * 4 + 2], m.h_iw[i * 4 + 3]);
+            }
+            for (size_t i = 0; i < a4.size(); i++) {
+                a4[i] = make_int4(m.h_aids[i * 4], m.h_aids[i * 4 + 1], m.h_aids[i * 4 + 2], m.h_aids[i * 4 + 3]);
+                b4[i] = make_float4(m.h_aw[i * 4], m.h_aw[i * 4 + 1], m.h_aw[i * 4 + 2], m.h_aw[i * 4 + 3]);
+            }
+            if ((rc = up(s.get(), s->owned, &m.iids, i4.data(), i4.size()))) return rc;
+            if ((rc = up(s.get(), s->owned, &m.iw, w4.data(), w4.size()))) return rc;
+            if ((rc = up(s.get(), s->owned, &m.aids, a4.data(), a4.size()))) return rc;
+            if ((rc = up(s.get(), s->owned, &m.aw, b4.data(), b4.size()))) return rc;
+        }
+        if ((rc = dalloc(&s->dcol, 1))) return rc;
+        s->owned.push_back(s->dcol);
+        std::memset(&s->hcol, 0, sizeof(DevColliders));
+        s->hcol.kcol = p->kcol;
+        if ((rc = push_colliders(s.get()))) return rc;
+        if ((rc = dalloc(&s->dred, 16))) return rc;
+        s->owned.push_back(s->dred);
+        HIPCHK(hipStreamSynchronize(s->stream));
+        *out = s.release();
+        return MGSB_OK;
+    } catch (const std::exception& e) {
+        set_error(std::string("mgsb_create: ") + e.what());
+        return MGSB_ERR_NOMEM;
+    }
+}
+
+void mgsb_destroy(mgsb_solver* s) {
+    if (!s) return;
+    hipSetDevice(s->device);
+    if (s->dpart) hipFree(s->dpart);
+    delete s;
+}
+
+int mgsb_set_params(mgsb_solver* s, const mgsb_params* p) {
+    if (!s || !p) {
+        set_error("mgsb_set_params: null argument");
+        return MGSB_ERR_ARG;
+    }
+    int rc = validate_params(p, (int)s->L.size());
+    if (rc) return rc;
+    if (p->stiffness != s->p.stiffness || p->density != s->p.density || p->device != s->p.device) {
+        set_error("mgsb_set_params: stiffness, density and device are fixed at create");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    const bool dt_changed = p->dt != s->p.dt;
+    s->p = *p;
+    s->hcol.kcol = p->kcol;
+    if ((rc = pull_colliders(s))) return rc;
+    s->hcol.kcol = p->kcol;
+    if ((rc = push_colliders(s))) return rc;
+    if (dt_changed)
+        for (auto& d : s->L)
+            if ((rc = upload_vc(s, d))) return rc;
+    return invalidate_graph(s);
+}
+
+int mgsb_set_fixed(mgsb_solver* s, int32_t level, const float* fixflag) {
+    int rc = check_level(s, level);
+    if (rc) return rc;
+    if (!fixflag) {
+        set_error("mgsb_set_fixed: null array");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    DevLevel& d = s->L[level];
+    for (int i = 0; i < d.V; i++) d.h.fix[i] = fixflag[i];
+    return upload_vc(s, d);
+}
+
+int mgsb_fix_spheres(mgsb_solver* s, const float* spheres, int32_t n) {
+    if (!s || (n > 0 && !spheres) || n < 0) {
+        set_error("mgsb_fix_spheres: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    for (auto& d : s->L) {
+        for (int i = 0; i < d.V; i++) {
+            const float* X = &d.h.X[(size_t)i * 3];
+            for (int q = 0; q < n; q++) {
+                const float* c = spheres + q * 4;
+                const float dx = X[0] - c[0], dy = X[1] - c[1], dz = X[2] - c[2];
+                if (std::sqrt(dx * dx + dy * dy + dz * dz) < c[3]) {
+                    d.h.fix[i] = 0.f;
+                    break;
+                }
+            }
+        }
+        int rc = upload_vc(s, d);
+        if (rc) return rc;
+    }
+    return MGSB_OK;
+}
+
+int mgsb_fix_plane(mgsb_solver* s, int32_t axis, float value, float tol) {
+    if (!s || axis < 0 || axis > 2) {
+        set_error("mgsb_fix_plane: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    for (auto& d : s->L) {
+        for (int i = 0; i < d.V; i++)
+            if (std::fabs(d.h.X[(size_t)i * 3 + axis] - value) < tol) d.h.fix[i] = 0.f;
+        int rc = upload_vc(s, d);
+        if (rc) return rc;
+    }
+    return MGSB_OK;
+}
+
+int mgsb_set_sphere(mgsb_solver* s, int32_t id, const float center[3], float radius, const float vel[3],
+                    int32_t active) {
+    if (!s || id < 0 || id >= MGSB_MAX_SPHERES || !center) {
+        set_error("mgsb_set_sphere: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    int rc = pull_colliders(s);
+    if (rc) return rc;
+    s->hcol.sph[id] = make_float4(center[0], center[1], center[2], radius);
+    s->hcol.vel[id] = make_float4(vel ? vel[0] : 0.f, vel ? vel[1] : 0.f, vel ? vel[2] : 0.f, active ? 1.f : 0.f);
+    if (id + 1 > s->hcol.nsph) s->hcol.nsph = id + 1;
+    return push_colliders(s);
+}
+
+int mgsb_set_plane(mgsb_solver* s, const float n[3], float d, float k, int32_t active) {
+    if (!s || !n) {
+        set_error("mgsb_set_plane: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    int rc = pull_colliders(s);
+    if (rc) return rc;
+    s->hcol.plane = make_float4(n[0], n[1], n[2], d);
+    s->hcol.kplane = k;
+    s->hcol.plane_on = active ? 1 : 0;
+    return push_colliders(s);
+}
+
+int mgsb_set_velocity(mgsb_solver* s, const float* v) {
+    if (!s || !v) {
+        set_error("mgsb_set_velocity: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    return write_f4(s, s->L[0].v, s->L[0].V, v);
+}
+
+int mgsb_set_positions(mgsb_solver* s, int32_t level, const float* x) {
+    int rc = check_level(s, level);
+    if (rc) return rc;
+    if (!x) {
+        set_error("mgsb_set_positions: null array");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    return write_f4(s, s->L[level].x, s->L[level].V, x);
+}
+
+int mgsb_step(mgsb_solver* s, int32_t nsteps, mgsb_step_stats* stats) {
+    if (!s || nsteps < 0) {
+        set_error("mgsb_step: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    int rc;
+    const bool graphable = s->p.use_graph && !s->timing &&
+                           (s->p.schedule == MGSB_SCHED_SINGLE || s->p.schedule == MGSB_SCHED_TWOLEVEL);
+    for (int st = 0; st < nsteps; st++) {
+        if (graphable) {
+            if (!s->graph_valid) {
+                HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+                rc = enqueue_step(s);
+                hipGraph_t g = nullptr;
+                hipError_t ce = hipStreamEndCapture(s->stream, &g);
+                if (rc) {
+                    if (g) hipGraphDestroy(g);
+                    return rc;
+                }
+                HIPCHK(ce);
+                s->graph = g;
+                HIPCHK(hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
+                s->graph_valid = true;
+            }
+            HIPCHK(hipGraphLaunch(s->gexec, s->stream));
+        } else {
+            if ((rc = enqueue_step(s))) return rc;
+        }
+    }
+    s->last.steps = nsteps;
+    if (stats) *stats = s->last;
+    HIPCHK(hipGetLastError());
+    return MGSB_OK;
+}
+
+int mgsb_sync(mgsb_solver* s) {
+    if (!s) {
+        set_error("mgsb_sync: null solver");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return MGSB_OK;
+}
+
+int mgsb_read_positions(mgsb_solver* s, int32_t level, float* xyz) {
+    int rc = check_level(s, level);
+    if (rc) return rc;
+    if (!xyz) {
+        set_error("mgsb_read_positions: null array");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    return read_f4(s, s->L[level].x, s->L[level].V, xyz);
+}
+
+int mgsb_read_velocity(mgsb_solver* s, int32_t level, float* v) {
+    int rc = check_level(s, level);
+    if (rc) return rc;
+    if (!v) {
+        set_error("mgsb_read_velocity: null array");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    return read_f4(s, s->L[level].v, s->L[level].V, v);
+}
+
+static int compute_normals(mgsb_solver* s) {
+    DevLevel& d = s->L[0];
+    const int bs = 256;
+    if (d.S > 0) {
+        k_tri_normal<<<nblk(d.S, bs), bs, 0, s->stream>>>(d.S, d.x, d.tri, d.tn);
+        HIPCHK(hipGetLastError());
+    }
+    k_vert_normal<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.trp, d.trid, d.tn, d.vn);
+    HIPCHK(hipGetLastError());
+    return MGSB_OK;
+}
+
+int mgsb_read_normals(mgsb_solver* s, float* n) {
+    if (!s || !n) {
+        set_error("mgsb_read_normals: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    int rc = compute_normals(s);
+    if (rc) return rc;
+    return read_f4(s, s->L[0].vn, s->L[0].V, n);
+}
+
+int mgsb_read_render9(mgsb_solver* s, float* v9) {
+    if (!s || !v9) {
+        set_error("mgsb_read_render9: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    const int V = s->L[0].V;
+    std::vector<float> x((size_t)V * 3), n((size_t)V * 3);
+    int rc = mgsb_read_positions(s, 0, x.data());
+    if (rc) return rc;
+    if ((rc = mgsb_read_normals(s, n.data()))) return rc;
+    for (int i = 0; i < V; i++) {
+        for (int c = 0; c < 3; c++) {
+            v9[(size_t)i * 9 + c] = x[(size_t)i * 3 + c];
+            v9[(size_t)i * 9 + 3 + c] = n[(size_t)i * 3 + c];
+            v9[(size_t)i * 9 + 6 + c] = 0.f;
+        }
+    }
+    return MGSB_OK;
+}
+
+int mgsb_read_collision(mgsb_solver* s, int32_t* flags, float* depth) {
+    if (!s) {
+        set_error("mgsb_read_collision: null solver");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    DevLevel& d = s->L[0];
+    if (flags) HIPCHK(hipMemcpyAsync(flags, d.cflag, sizeof(int) * d.V, hipMemcpyDeviceToHost, s->stream));
+    if (depth) HIPCHK(hipMemcpyAsync(depth, d.cdepth, sizeof(float) * d.V, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return MGSB_OK;
+}
+
+int mgsb_energy(mgsb_solver* s, mgsb_energy_t* e) {
+    if (!s || !e) {
+        set_error("mgsb_energy: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    DevLevel& d = s->L[0];
+    const int nbv = nblk(d.V, 256), nbt = nblk(d.T, 256);
+    int rc = ensure_partials(s, (size_t)std::max(nbv, nbt) * 4);
+    if (rc) return rc;
+    k_energy_vert<<<nbv, 256, 0, s->stream>>>(d.V, d.x, d.xprev, d.s, d.vc, d.cflag, d.cenergy, s->dpart);
+    HIPCHK(hipGetLastError());
+    double v4[4];
+    if ((rc = reduce_finish(s, nbv, 4, v4))) return rc;
+    k_energy_tet<<<nbt, 256, 0, s->stream>>>(d.T, d.x, d.tet, d.B, d.vol, s->p.stiffness, s->dpart);
+    HIPCHK(hipGetLastError());
+    double ep = 0;
+    if ((rc = reduce_finish(s, nbt, 1, &ep))) return rc;
+    const double dt = s->p.dt;
+    e->dX = std::sqrt(v4[0]);
+    e->Ek = v4[1] / (2 * dt * dt);
+    e->Ep = ep;
+    e->Ec = v4[2];
+    e->Nc = v4[3];
+    return MGSB_OK;
+}
+
+int mgsb_residual(mgsb_solver* s, double* res) {
+    if (!s || !res) {
+        set_error("mgsb_residual: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    return residual_norm(s, res);
+}
+
+int mgsb_level_info(mgsb_solver* s, int32_t level, int32_t* nverts, int32_t* ntets, int32_t* ntris) {
+    int rc = check_level(s, level);
+    if (rc) return rc;
+    if (nverts) *nverts = s->L[level].V;
+    if (ntets) *ntets = s->L[level].T;
+    if (ntris) *ntris = s->L[level].S;
+    return MGSB_OK;
+}
+
+int mgsb_set_timing(mgsb_solver* s, int32_t on) {
+    if (!s) {
+        set_error("mgsb_set_timing: null solver");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->timing = on != 0;
+    s->ev_used = 0;
+    s->time_ms = 0.0;
+    s->time_launches = 0;
+    return MGSB_OK;
+}
+
+int mgsb_kernel_time(mgsb_solver* s, int32_t kernel, double* ms, int64_t* launches) {
+    if (!s || kernel != 0) {
+        set_error("mgsb_kernel_time: only kernel 0 (per-tet local step) is timed");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < s->ev_used; i++) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, s->ev_pool[i].first, s->ev_pool[i].second));
+        s->time_ms += t;
+        s->time_launches++;
+    }
+    s->ev_used = 0;
+    if (ms) *ms = s->time_ms;
+    if (launches) *launches = s->time_launches;
+    return MGSB_OK;
+}
+
+}  // extern "C"

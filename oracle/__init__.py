@@ -1,0 +1,237 @@
+"""CPU oracle of the reference PD solver -- TEST INFRASTRUCTURE ONLY.
+
+ctypes loader for ``oracle/liboracle.so`` (built from ``mgsb_oracle.c`` by
+``oracle/Makefile``).  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+CPU-baseline leg of ``bench.py`` use it, as the checker.  The product never
+imports it.
+
+Two precisions share one source: ``Level(..., fp64=False)`` follows the
+reference's FP32 operation order; ``fp64=True`` is the FP64 yardstick.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+MAX_SPHERES = 8
+
+
+class orc_params(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float),
+        ("damping", C.c_float),
+        ("stiffness", C.c_float),
+        ("density", C.c_float),
+        ("rho", C.c_float),
+        ("relax", C.c_double),
+        ("gravity", C.c_float * 3),
+        ("kcol", C.c_float),
+        ("iters", C.c_int),
+        ("iters_coarse", C.c_int),
+        ("if_mode", C.c_int),
+        ("tri_dcd_coarse", C.c_int),
+        ("nspheres", C.c_int),
+        ("sphere", (C.c_float * 8) * MAX_SPHERES),
+        ("plane", C.c_float * 5),
+    ]
+
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+_SIGS = {
+    "level_create": (_vp, [C.c_int, C.c_int, C.c_int, _fp, _ip, C.POINTER(C.c_uint32), C.c_float, C.c_float]),
+    "level_free": (None, [_vp]),
+    "level_bad_tet": (C.c_int, [_vp]),
+    "set_threads": (None, [C.c_int]),
+    "level_set_fix": (None, [_vp, _fp]),
+    "level_set_velocity": (None, [_vp, _fp]),
+    "level_set_positions": (None, [_vp, _fp]),
+    "level_get": (C.c_int, [_vp, C.c_int, _dp]),
+    "step_single": (None, [_vp, C.POINTER(orc_params), C.c_int]),
+    "mg_create": (_vp, [_vp, _vp]),
+    "mg_free": (None, [_vp]),
+    "mg_unfound": (C.c_int, [_vp, C.c_int]),
+    "mg_get_maps": (None, [_vp, _ip, _fp, _ip, _fp]),
+    "step_mg": (None, [_vp, C.POINTER(orc_params), C.c_int]),
+    "energy": (None, [_vp, C.POINTER(orc_params), _dp]),
+    "vertex_normals": (None, [_vp]),
+    "residual_norm": (C.c_double, [_vp, C.POINTER(orc_params)]),
+    "polar_f": (C.c_float, [_fp, _fp, C.c_int]),
+    "inverse3_f": (C.c_float, [_fp, _fp]),
+    "barycentric4": (None, [_fp, _fp, _fp]),
+    "tet_corner_forces": (None, [_vp, C.c_int, _dp]),
+    "collide": (None, [_vp, C.POINTER(orc_params), C.c_int]),
+}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built; run `make -C oracle`")
+    lib = C.CDLL(LIB_PATH)
+    for pfx in ("orc32_", "orc64_"):
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, pfx + name)
+            fn.restype = res
+            fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+# level_get selectors
+X, V, S, XPREV, XLAST, MASS, DIAG, VOL, B, AC, CDIAG, CFORCE, CFLAG, CDEPTH, CENERGY, VNORMAL, TRIN, FIX, FORCE, \
+    CENTER, FACEN, FACEA = range(22)
+
+
+def params(dt=1 / 60, damping=0.95, stiffness=8000.0, density=1.0, rho=0.9992, relax=0.6, gravity=(0, -9.8, 0),
+           kcol=1000.0, iters=20, iters_coarse=20, if_mode=1, tri_dcd_coarse=1, spheres=(), plane=None):
+    """orc_params with the reference defaults (PDSolver.cpp:166-176).
+    spheres: iterable of (cx, cy, cz, r, active, vx, vy, vz)."""
+    p = orc_params()
+    p.dt, p.damping, p.stiffness, p.density, p.rho, p.relax = dt, damping, stiffness, density, rho, relax
+    p.gravity[:] = [float(g) for g in gravity]
+    p.kcol, p.iters, p.iters_coarse, p.if_mode, p.tri_dcd_coarse = kcol, iters, iters_coarse, if_mode, tri_dcd_coarse
+    spheres = list(spheres)
+    assert len(spheres) <= MAX_SPHERES
+    p.nspheres = len(spheres)
+    for i, s in enumerate(spheres):
+        vals = list(s) + [0.0] * (8 - len(s))
+        p.sphere[i][:] = [float(v) for v in vals]
+    if plane is not None:
+        p.plane[:] = [float(v) for v in plane]
+    return p
+
+
+def set_threads(n):
+    lib.orc32_set_threads(n)
+    lib.orc64_set_threads(n)
+
+
+class Level:
+    """One oracle mesh level (the state of a reference PDSolver)."""
+
+    def __init__(self, pos, tets, tris=None, stiffness=8000.0, density=1.0, fp64=False):
+        self.pfx = "orc64_" if fp64 else "orc32_"
+        self.pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        self.tets = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+        self.tris = np.ascontiguousarray(tris if tris is not None else np.zeros((0, 3)), np.uint32).reshape(-1, 3)
+        self.nv, self.nt, self.ns = self.pos.shape[0], self.tets.shape[0], self.tris.shape[0]
+        self._h = self._f("level_create")(self.nv, self.nt, self.ns, self.pos.ctypes.data_as(_fp),
+                                          self.tets.ctypes.data_as(_ip),
+                                          self.tris.ctypes.data_as(C.POINTER(C.c_uint32)), stiffness, density)
+
+    def _f(self, name):
+        return getattr(lib, self.pfx + name)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._f("level_free")(h)
+            self._h = None
+
+    @property
+    def bad_tet(self):
+        return self._f("level_bad_tet")(self._h)
+
+    def set_fix(self, fix):
+        a = np.ascontiguousarray(fix, np.float32)
+        self._f("level_set_fix")(self._h, a.ctypes.data_as(_fp))
+
+    def set_velocity(self, v):
+        a = np.ascontiguousarray(v, np.float32).reshape(-1)
+        self._f("level_set_velocity")(self._h, a.ctypes.data_as(_fp))
+
+    def set_positions(self, x):
+        a = np.ascontiguousarray(x, np.float32).reshape(-1)
+        self._f("level_set_positions")(self._h, a.ctypes.data_as(_fp))
+
+    def get(self, which):
+        sizes = {X: 3 * self.nv, V: 3 * self.nv, S: 3 * self.nv, XPREV: 3 * self.nv, XLAST: 3 * self.nv,
+                 MASS: self.nv, DIAG: self.nv, VOL: self.nt, B: 9 * self.nt, AC: 12 * self.nt, CDIAG: 3 * self.nv,
+                 CFORCE: 3 * self.nv, CFLAG: self.nv, CDEPTH: self.nv, CENERGY: self.nv, VNORMAL: 3 * self.nv,
+                 TRIN: 3 * self.ns, FIX: self.nv, FORCE: 3 * self.nv, CENTER: 3 * self.nt, FACEN: 12 * self.nt,
+                 FACEA: 4 * self.nt}
+        out = np.zeros(max(1, sizes[which]), np.float64)
+        n = self._f("level_get")(self._h, which, out.ctypes.data_as(_dp))
+        return out[:n]
+
+    def positions(self):
+        return self.get(X).reshape(-1, 3)
+
+    def step(self, p, nsteps=1):
+        self._f("step_single")(self._h, C.byref(p), nsteps)
+
+    def energy(self, p):
+        out = np.zeros(5, np.float64)
+        self._f("energy")(self._h, C.byref(p), out.ctypes.data_as(_dp))
+        return dict(zip(("Ek", "Ep", "Ec", "Nc", "dX"), out.tolist()))
+
+    def vertex_normals(self):
+        self._f("vertex_normals")(self._h)
+        return self.get(VNORMAL).reshape(-1, 3)
+
+    def residual(self, p):
+        return self._f("residual_norm")(self._h, C.byref(p))
+
+    def corner_forces(self, mode=0):
+        out = np.zeros(self.nt * 12, np.float64)
+        self._f("tet_corner_forces")(self._h, mode, out.ctypes.data_as(_dp))
+        return out.reshape(self.nt, 4, 3)
+
+    def collide(self, p, tri=False):
+        self._f("collide")(self._h, C.byref(p), int(tri))
+
+
+class TwoLevel:
+    """Oracle of PDSolver_MG (coarse + fine)."""
+
+    def __init__(self, coarse, fine):
+        assert coarse.pfx == fine.pfx
+        self.coarse, self.fine, self.pfx = coarse, fine, coarse.pfx
+        self._h = getattr(lib, self.pfx + "mg_create")(coarse._h, fine._h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            getattr(lib, self.pfx + "mg_free")(h)
+            self._h = None
+
+    def unfound(self):
+        f = getattr(lib, self.pfx + "mg_unfound")
+        return f(self._h, 0), f(self._h, 1)
+
+    def maps(self):
+        nf, nc = self.fine.nv, self.coarse.nv
+        iids = np.zeros(nf * 4, np.int32)
+        iw = np.zeros(nf * 4, np.float32)
+        aids = np.zeros(nc * 4, np.int32)
+        aw = np.zeros(nc * 4, np.float32)
+        getattr(lib, self.pfx + "mg_get_maps")(self._h, iids.ctypes.data_as(_ip), iw.ctypes.data_as(_fp),
+                                               aids.ctypes.data_as(_ip), aw.ctypes.data_as(_fp))
+        return iids.reshape(nf, 4), iw.reshape(nf, 4), aids.reshape(nc, 4), aw.reshape(nc, 4)
+
+    def step(self, p, nsteps=1):
+        getattr(lib, self.pfx + "step_mg")(self._h, C.byref(p), nsteps)
+
+
+def polar(F, variant=0, fp64=False):
+    F = np.ascontiguousarray(F, np.float32).reshape(9)
+    R = np.zeros(9, np.float32)
+    det = getattr(lib, ("orc64_" if fp64 else "orc32_") + "polar_f")(F.ctypes.data_as(_fp), R.ctypes.data_as(_fp),
+                                                                    variant)
+    return R.reshape(3, 3), det
+
+
+def barycentric4(P, tet4x3, fp64=False):
+    P = np.ascontiguousarray(P, np.float32).reshape(3)
+    T = np.ascontiguousarray(tet4x3, np.float32).reshape(12)
+    w = np.zeros(4, np.float32)
+    getattr(lib, ("orc64_" if fp64 else "orc32_") + "barycentric4")(P.ctypes.data_as(_fp), T.ctypes.data_as(_fp),
+                                                                   w.ctypes.data_as(_fp))
+    return w

@@ -1,0 +1,1052 @@
+/*
+ * mgsb_oracle.c -- CPU restatement of the reference projective-dynamics (PD)
+ * tetrahedral solver of buuugyellow/MultiGridSoftBody.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle: tests, the smoke check
+ * and the CPU-baseline leg of the benchmark call it as the checker.  The
+ * product (libmgsb, HIP) never links, loads or calls it.
+ *
+ * The same source is compiled twice:
+ *   - default      : `real` = float, prefix orc32_.  Follows the reference's
+ *                    FP32 operation order, including the places where the
+ *                    reference promotes to double through a double literal
+ *                    (GetRotation_D's `l`, calculatePOS's 0.6 relaxation).
+ *   - ORC_DOUBLE   : `real` = double, prefix orc64_.  The FP64 yardstick.
+ *
+ * Summation order of scattered forces: the reference scatters with atomicAdd
+ * (order not defined).  Here every per-vertex sum runs over incident tets in
+ * increasing tet index, i.e. the order of a sequential scatter loop over tets.
+ * The OpenMP path computes per-tet contributions in parallel, then gathers
+ * per vertex in that same order, so 1 thread and N threads agree bitwise.
+ *
+ * Citations are to files of the reference repository.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef ORC_DOUBLE
+typedef float real;
+#define PFX(n) orc32_##n
+#define R_SQRT(x) sqrtf(x)
+#define R_ACOS(x) acosf(x)
+#define R_COS(x) cosf(x)
+#define R_FABS(x) fabsf(x)
+#else
+typedef double real;
+#define PFX(n) orc64_##n
+#define R_SQRT(x) sqrt(x)
+#define R_ACOS(x) acos(x)
+#define R_COS(x) cos(x)
+#define R_FABS(x) fabs(x)
+#endif
+
+#define ORC_MAX_SPHERES 8
+
+/* ------------------------------------------------------------------------- */
+/* Parameters (always float at the interface, converted to `real` inside).    */
+/* ------------------------------------------------------------------------- */
+typedef struct orc_params {
+    float dt, damping, stiffness, density, rho;
+    double relax;         /* 0.6 in the reference, a double literal          */
+    float gravity[3];
+    float kcol;           /* collision stiffness (PDSolver.cpp:171)            */
+    int iters;            /* PD iterations per step (fine / single level)      */
+    int iters_coarse;     /* coarse iterations in the two-level schedule       */
+    int if_mode;          /* 0: calculateIF (PDSolver.cu:128); 1: Ac path      */
+                          /*    getTetFRV2 + calTetIFV1 (PDSolverAc.cu:447,630)*/
+    int tri_dcd_coarse;   /* two-level schedule: triangle DCD on coarse level  */
+    int nspheres;
+    float sphere[ORC_MAX_SPHERES][8]; /* cx cy cz r active vx vy vz           */
+    float plane[5];       /* nx ny nz d k (k>0 = active); not in the reference */
+} orc_params;
+
+/* ------------------------------------------------------------------------- */
+/* Small vector helpers, same association as utils.cu:277-296.                */
+/* ------------------------------------------------------------------------- */
+typedef struct { real x, y, z; } v3;
+static inline v3 v3m(real x, real y, real z) { v3 r = {x, y, z}; return r; }
+static inline v3 vsub(v3 a, v3 b) { return v3m(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vadd(v3 a, v3 b) { return v3m(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vmul(v3 a, real s) { return v3m(a.x * s, a.y * s, a.z * s); }
+static inline v3 vdiv(v3 a, real s) { return v3m(a.x / s, a.y / s, a.z / s); }
+static inline real vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 vcross(v3 a, v3 b) {
+    return v3m(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline real vlen(v3 a) { return R_SQRT(a.x * a.x + a.y * a.y + a.z * a.z); }
+static inline v3 vnormalize(v3 a) { return vdiv(a, vlen(a)); }
+static inline v3 vload(const real* p) { return v3m(p[0], p[1], p[2]); }
+
+/* ------------------------------------------------------------------------- */
+/* Matrix_Inverse_3 (utils.cu:233-252): adjugate / det, returns det.          */
+/* The reference exits on |det| < 1e-6; here the caller gets the det back.    */
+/* ------------------------------------------------------------------------- */
+static real mat_inverse3(const real* A, real* R) {
+    R[0] = A[4] * A[8] - A[7] * A[5];
+    R[1] = A[7] * A[2] - A[1] * A[8];
+    R[2] = A[1] * A[5] - A[4] * A[2];
+    R[3] = A[5] * A[6] - A[3] * A[8];
+    R[4] = A[0] * A[8] - A[2] * A[6];
+    R[5] = A[2] * A[3] - A[0] * A[5];
+    R[6] = A[3] * A[7] - A[4] * A[6];
+    R[7] = A[1] * A[6] - A[0] * A[7];
+    R[8] = A[0] * A[4] - A[1] * A[3];
+    real det = A[0] * R[0] + A[3] * R[1] + A[6] * R[2];
+    if (R_FABS(det) < 1e-6) return det;
+    real inv_det = 1 / det;
+    for (int i = 0; i < 9; i++) R[i] *= inv_det;
+    return det;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Franca closed-form polar decomposition R = F U^{-1}.                       */
+/* variant 0: GetRotation_D (utils.cu:123-226), `l` evaluated in double as    */
+/*            the reference's double literals make it.                        */
+/* variant 1: getTetFRV2 (PDSolverAc.cu:494-595), all-float constants.        */
+/* Fallbacks return R = I exactly where the reference does.  Returns det F.   */
+/* ------------------------------------------------------------------------- */
+static real polar(const real F[9], real R[9], int variant) {
+    for (int i = 0; i < 9; i++) R[i] = 0;
+    R[0] = R[4] = R[8] = 1;
+    real C[9], C2[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            real acc = 0;
+            for (int k = 0; k < 3; k++) acc += F[k * 3 + i] * F[k * 3 + j];
+            C[i * 3 + j] = acc;
+        }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            real acc = 0;
+            for (int k = 0; k < 3; k++) acc += C[i * 3 + k] * C[j * 3 + k];
+            C2[i * 3 + j] = acc;
+        }
+    real det = F[0] * F[4] * F[8] + F[1] * F[5] * F[6] + F[3] * F[7] * F[2] - F[2] * F[4] * F[6] -
+               F[1] * F[3] * F[8] - F[0] * F[5] * F[7];
+    real I_c = C[0] + C[4] + C[8];
+    real I_c2 = I_c * I_c;
+    real II_c = (real)0.5 * (I_c2 - C2[0] - C2[4] - C2[8]);
+    real III_c = det * det;
+    real k = I_c2 - 3 * II_c;
+    real inv_U[9];
+    if (k < (real)1e-6f) {
+        if (I_c < 1e-6) return det;
+        real inv_lambda = 1 / R_SQRT(I_c / 3);
+        for (int i = 0; i < 9; i++) inv_U[i] = 0;
+        inv_U[0] = inv_U[4] = inv_U[8] = inv_lambda;
+    } else {
+        real l;
+        if (variant == 0)
+            l = (real)((double)I_c * ((double)(I_c * I_c) - 4.5 * (double)II_c) + 13.5 * (double)III_c);
+        else
+            l = I_c * (I_c * I_c - (real)4.5f * II_c) + (real)13.5f * III_c;
+        real k_root = R_SQRT(k);
+        real value = l / (k * k_root);
+        if (value < -1.0) value = -1.0;
+        if (value > 1.0) value = 1.0;
+        real phi = R_ACOS(value);
+        real lambda2 = (I_c + 2 * k_root * R_COS(phi / 3)) / 3;
+        real lambda = R_SQRT(lambda2);
+        real III_u;
+        if (variant == 0) {
+            III_u = R_SQRT(III_c);
+            if (det < 0) III_u = -III_u;
+        } else {
+            III_u = det;
+        }
+        if (R_FABS(lambda) < 1e-6) return det;
+        if (-lambda2 + I_c + 2 * III_u / lambda < 1e-6) return det;
+        real I_u = lambda + R_SQRT(-lambda2 + I_c + 2 * III_u / lambda);
+        real II_u = (I_u * I_u - I_c) * (real)0.5;
+        if (R_FABS(I_u * II_u - III_u) < 1e-6) return det;
+        real inv_rate = 1 / (I_u * II_u - III_u);
+        real factor = I_u * III_u * inv_rate;
+        real U[9];
+        for (int i = 0; i < 9; i++) U[i] = 0;
+        U[0] = U[4] = U[8] = factor;
+        factor = (I_u * I_u - II_u) * inv_rate;
+        for (int i = 0; i < 9; i++) U[i] += factor * C[i] - inv_rate * C2[i];
+        if (R_FABS(III_u) < 1e-6) return det;
+        inv_rate = 1 / III_u;
+        factor = II_u * inv_rate;
+        for (int i = 0; i < 9; i++) inv_U[i] = 0;
+        inv_U[0] = inv_U[4] = inv_U[8] = factor;
+        factor = -I_u * inv_rate;
+        for (int i = 0; i < 9; i++) inv_U[i] += factor * U[i] + inv_rate * C[i];
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            real acc = 0;
+            for (int k2 = 0; k2 < 3; k2++) acc += F[i * 3 + k2] * inv_U[k2 * 3 + j];
+            R[i * 3 + j] = acc;
+        }
+    return det;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Level: one tet mesh with the reference's per-level state (PDSolver.h:8-68). */
+/* ------------------------------------------------------------------------- */
+typedef struct orc_level {
+    int V, T, S;
+    int* tets;
+    unsigned* tris;
+    real *x0, *x, *v, *fext, *xlast, *s, *xprev;
+    real *B, *AC, *vol, *mass, *diag, *fix, *center, *faceN, *faceA;
+    real *force, *cdiag, *cforce, *cenergy, *cdepth;
+    int* cflag;
+    real* cf;        /* per tet 12: corner j component i at [t*12 + j*3 + i] */
+    int *rp, *ri;    /* vertex -> incident (t*4+c), t ascending               */
+    int *trp, *tri;  /* vertex -> incident (tri*3+c), tri ascending           */
+    real *tc;        /* per tri corner: 3 force + 1 diag                      */
+    real *trin;      /* tri normals                                           */
+    real *vnormal;   /* vertex normals                                        */
+    float stiffness, density;
+    int bad_tet;     /* index of first degenerate tet or -1                   */
+} orc_level;
+
+static void* xcalloc(size_t n, size_t s) { return calloc(n ? n : 1, s); }
+
+static void build_csr(int V, int n_items, int corners, const int* items, int** rp_out, int** ri_out) {
+    int* rp = (int*)xcalloc((size_t)V + 1, sizeof(int));
+    for (int e = 0; e < n_items * corners; e++) rp[items[e] + 1]++;
+    for (int i = 0; i < V; i++) rp[i + 1] += rp[i];
+    int* fill = (int*)xcalloc((size_t)V, sizeof(int));
+    int* ri = (int*)xcalloc((size_t)n_items * corners, sizeof(int));
+    for (int e = 0; e < n_items * corners; e++) {  /* ascending item order */
+        int v = items[e];
+        ri[rp[v] + fill[v]++] = e;
+    }
+    free(fill);
+    *rp_out = rp;
+    *ri_out = ri;
+}
+
+/* InitVolumeConstraint (PDSolver.cpp:12-132). */
+static void precompute(orc_level* L) {
+    int T = L->T;
+    real k = (real)L->stiffness;
+    L->bad_tet = -1;
+    for (int i = 0; i < T; i++) {
+        int v0 = L->tets[i * 4 + 0], v1 = L->tets[i * 4 + 1], v2 = L->tets[i * 4 + 2], v3i = L->tets[i * 4 + 3];
+        const real* X = L->x0;
+        real D[9];
+        D[0] = X[v1 * 3 + 0] - X[v0 * 3 + 0];
+        D[1] = X[v2 * 3 + 0] - X[v0 * 3 + 0];
+        D[2] = X[v3i * 3 + 0] - X[v0 * 3 + 0];
+        D[3] = X[v1 * 3 + 1] - X[v0 * 3 + 1];
+        D[4] = X[v2 * 3 + 1] - X[v0 * 3 + 1];
+        D[5] = X[v3i * 3 + 1] - X[v0 * 3 + 1];
+        D[6] = X[v1 * 3 + 2] - X[v0 * 3 + 2];
+        D[7] = X[v2 * 3 + 2] - X[v0 * 3 + 2];
+        D[8] = X[v3i * 3 + 2] - X[v0 * 3 + 2];
+        real det = mat_inverse3(D, &L->B[i * 9]);
+        if (R_FABS(det) < 1e-6 && L->bad_tet < 0) L->bad_tet = i;
+        L->vol[i] = R_FABS(det) / 6;
+        real vq = L->vol[i] / 4;
+        L->mass[v0] += vq;
+        L->mass[v1] += vq;
+        L->mass[v2] += vq;
+        L->mass[v3i] += vq;
+        const real* b = &L->B[i * 9];
+        real* ac = &L->AC[i * 12];
+        ac[0] = -b[0] - b[3] - b[6]; ac[1] = b[0]; ac[2] = b[3]; ac[3] = b[6];
+        ac[4] = -b[1] - b[4] - b[7]; ac[5] = b[1]; ac[6] = b[4]; ac[7] = b[7];
+        ac[8] = -b[2] - b[5] - b[8]; ac[9] = b[2]; ac[10] = b[5]; ac[11] = b[8];
+        int vs[4] = {v0, v1, v2, v3i};
+        for (int j = 0; j < 4; j++)
+            for (int r = 0; r < 3; r++) L->diag[vs[j]] += ac[r * 4 + j] * ac[r * 4 + j] * L->vol[i] * k;
+        for (int c = 0; c < 3; c++) {
+            real acc = 0;
+            for (int j = 0; j < 4; j++) acc += X[vs[j] * 3 + c];
+            L->center[i * 3 + c] = acc / 4;
+        }
+        int faces[4][3] = {{v1, v2, v3i}, {v0, v2, v3i}, {v0, v1, v3i}, {v0, v1, v2}};
+        for (int f = 0; f < 4; f++) {
+            v3 A = vload(X + faces[f][0] * 3), Bq = vload(X + faces[f][1] * 3), Cq = vload(X + faces[f][2] * 3);
+            v3 Dq = vload(X + vs[f] * 3);
+            v3 n = vcross(vsub(Bq, A), vsub(Cq, A));
+            if (vdot(n, vsub(Dq, A)) > 0) n = v3m(-n.x, -n.y, -n.z);
+            real len = vlen(n);
+            L->faceA[i * 4 + f] = (real)0.5 * len;
+            n = vdiv(n, len);
+            L->faceN[i * 12 + f * 3 + 0] = n.x;
+            L->faceN[i * 12 + f * 3 + 1] = n.y;
+            L->faceN[i * 12 + f * 3 + 2] = n.z;
+        }
+    }
+    real rho = (real)L->density;
+    if (rho != 1)
+        for (int i = 0; i < L->V; i++) L->mass[i] *= rho;
+}
+
+orc_level* PFX(level_create)(int V, int T, int S, const float* pos, const int* tets, const unsigned* tris,
+                            float stiffness, float density) {
+    orc_level* L = (orc_level*)calloc(1, sizeof(orc_level));
+    L->V = V; L->T = T; L->S = S;
+    L->stiffness = stiffness; L->density = density;
+    L->tets = (int*)xcalloc((size_t)T * 4, sizeof(int));
+    memcpy(L->tets, tets, sizeof(int) * 4 * (size_t)T);
+    L->tris = (unsigned*)xcalloc((size_t)S * 3, sizeof(unsigned));
+    if (S) memcpy(L->tris, tris, sizeof(unsigned) * 3 * (size_t)S);
+    size_t v3n = (size_t)V * 3;
+    real** vec3s[] = {&L->x0, &L->x, &L->v, &L->fext, &L->xlast, &L->s, &L->xprev, &L->force, &L->cdiag,
+                      &L->cforce, &L->vnormal};
+    for (size_t q = 0; q < sizeof(vec3s) / sizeof(vec3s[0]); q++) *vec3s[q] = (real*)xcalloc(v3n, sizeof(real));
+    L->mass = (real*)xcalloc(V, sizeof(real));
+    L->diag = (real*)xcalloc(V, sizeof(real));
+    L->fix = (real*)xcalloc(V, sizeof(real));
+    L->cenergy = (real*)xcalloc(V, sizeof(real));
+    L->cdepth = (real*)xcalloc(V, sizeof(real));
+    L->cflag = (int*)xcalloc(V, sizeof(int));
+    L->B = (real*)xcalloc((size_t)T * 9, sizeof(real));
+    L->AC = (real*)xcalloc((size_t)T * 12, sizeof(real));
+    L->vol = (real*)xcalloc(T, sizeof(real));
+    L->center = (real*)xcalloc((size_t)T * 3, sizeof(real));
+    L->faceN = (real*)xcalloc((size_t)T * 12, sizeof(real));
+    L->faceA = (real*)xcalloc((size_t)T * 4, sizeof(real));
+    L->cf = (real*)xcalloc((size_t)T * 12, sizeof(real));
+    L->tc = (real*)xcalloc((size_t)S * 12, sizeof(real));
+    L->trin = (real*)xcalloc((size_t)S * 3, sizeof(real));
+    for (size_t i = 0; i < v3n; i++) L->x0[i] = L->x[i] = (real)pos[i];
+    for (int i = 0; i < V; i++) L->fix[i] = 1;
+    precompute(L);
+    build_csr(V, T, 4, L->tets, &L->rp, &L->ri);
+    build_csr(V, S, 3, (const int*)L->tris, &L->trp, &L->tri);
+    return L;
+}
+
+void PFX(level_free)(orc_level* L) {
+    if (!L) return;
+    void* ps[] = {L->tets, L->tris, L->x0, L->x, L->v, L->fext, L->xlast, L->s, L->xprev, L->B, L->AC, L->vol,
+                  L->mass, L->diag, L->fix, L->center, L->faceN, L->faceA, L->force, L->cdiag, L->cforce,
+                  L->cenergy, L->cdepth, L->cflag, L->cf, L->rp, L->ri, L->trp, L->tri, L->tc, L->trin,
+                  L->vnormal};
+    for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
+    free(L);
+}
+
+int PFX(level_bad_tet)(const orc_level* L) { return L->bad_tet; }
+
+void PFX(set_threads)(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* fixflag: 1 = free, 0 = pinned (PDSolver.cpp:134-162). */
+void PFX(level_set_fix)(orc_level* L, const float* fix) {
+    for (int i = 0; i < L->V; i++) L->fix[i] = (real)fix[i];
+}
+void PFX(level_set_velocity)(orc_level* L, const float* v) {
+    for (int i = 0; i < L->V * 3; i++) L->v[i] = (real)v[i];
+}
+void PFX(level_set_positions)(orc_level* L, const float* x) {
+    for (int i = 0; i < L->V * 3; i++) L->x[i] = (real)x[i];
+}
+
+/* which: 0 x, 1 v, 2 s, 3 xprev, 4 xlast, 5 mass, 6 diag, 7 vol, 8 B(9T), 9 AC(12T),
+ *        10 cdiag, 11 cforce, 12 cflag(as real), 13 cdepth, 14 cenergy, 15 vnormal,
+ *        16 trin, 17 fix, 18 force, 19 center(3T), 20 faceN(12T), 21 faceA(4T) */
+static const real* level_array(const orc_level* L, int which, size_t* n) {
+    switch (which) {
+        case 0: *n = (size_t)L->V * 3; return L->x;
+        case 1: *n = (size_t)L->V * 3; return L->v;
+        case 2: *n = (size_t)L->V * 3; return L->s;
+        case 3: *n = (size_t)L->V * 3; return L->xprev;
+        case 4: *n = (size_t)L->V * 3; return L->xlast;
+        case 5: *n = (size_t)L->V; return L->mass;
+        case 6: *n = (size_t)L->V; return L->diag;
+        case 7: *n = (size_t)L->T; return L->vol;
+        case 8: *n = (size_t)L->T * 9; return L->B;
+        case 9: *n = (size_t)L->T * 12; return L->AC;
+        case 10: *n = (size_t)L->V * 3; return L->cdiag;
+        case 11: *n = (size_t)L->V * 3; return L->cforce;
+        case 13: *n = (size_t)L->V; return L->cdepth;
+        case 14: *n = (size_t)L->V; return L->cenergy;
+        case 15: *n = (size_t)L->V * 3; return L->vnormal;
+        case 16: *n = (size_t)L->S * 3; return L->trin;
+        case 17: *n = (size_t)L->V; return L->fix;
+        case 18: *n = (size_t)L->V * 3; return L->force;
+        case 19: *n = (size_t)L->T * 3; return L->center;
+        case 20: *n = (size_t)L->T * 12; return L->faceN;
+        case 21: *n = (size_t)L->T * 4; return L->faceA;
+        default: *n = 0; return NULL;
+    }
+}
+
+int PFX(level_get)(const orc_level* L, int which, double* out) {
+    if (which == 12) {
+        for (int i = 0; i < L->V; i++) out[i] = L->cflag[i];
+        return L->V;
+    }
+    size_t n;
+    const real* a = level_array(L, which, &n);
+    if (!a) return -1;
+    for (size_t i = 0; i < n; i++) out[i] = (double)a[i];
+    return (int)n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* calculateST (PDSolver.cu:72-116).                                           */
+/* ------------------------------------------------------------------------- */
+static void calc_st(orc_level* L, const orc_params* p) {
+    real dt = p->dt, damp = p->damping;
+    real g[3] = {p->gravity[0], p->gravity[1], p->gravity[2]};
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < L->V; i++) {
+        real fix = L->fix[i];
+        for (int c = 0; c < 3; c++) {
+            int q = i * 3 + c;
+            L->xlast[q] = L->x[q];
+            L->v[q] *= damp * fix;
+            L->v[q] += g[c] * dt * fix;
+            L->v[q] += L->fext[q] * dt * fix;
+            L->x[q] += L->v[q] * dt * fix;
+            L->s[q] = L->x[q];
+            L->xprev[q] = L->x[q];
+            L->fext[q] = 0;
+        }
+    }
+}
+
+/* calculateV (PDSolver.cu:245-252). */
+static void calc_v(orc_level* L, const orc_params* p) {
+    real dt = p->dt;
+#pragma omp parallel for schedule(static)
+    for (int q = 0; q < L->V * 3; q++) L->v[q] = (L->x[q] - L->xlast[q]) / dt;
+}
+
+/* Per-tet elastic contribution: calculateIF (PDSolver.cu:128-183, mode 0) or
+ * getTetFRV2 + calTetIFV1 (PDSolverAc.cu:447-662, mode 1: float constants and
+ * contributions below 1e-5 in magnitude dropped). */
+static void tet_forces(orc_level* L, int mode) {
+    real k = (real)L->stiffness;
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < L->T; t++) {
+        const int* id = &L->tets[t * 4];
+        const real* X = L->x;
+        real D[9];
+        D[0] = X[id[1] * 3 + 0] - X[id[0] * 3 + 0];
+        D[1] = X[id[2] * 3 + 0] - X[id[0] * 3 + 0];
+        D[2] = X[id[3] * 3 + 0] - X[id[0] * 3 + 0];
+        D[3] = X[id[1] * 3 + 1] - X[id[0] * 3 + 1];
+        D[4] = X[id[2] * 3 + 1] - X[id[0] * 3 + 1];
+        D[5] = X[id[3] * 3 + 1] - X[id[0] * 3 + 1];
+        D[6] = X[id[1] * 3 + 2] - X[id[0] * 3 + 2];
+        D[7] = X[id[2] * 3 + 2] - X[id[0] * 3 + 2];
+        D[8] = X[id[3] * 3 + 2] - X[id[0] * 3 + 2];
+        const real* B = &L->B[t * 9];
+        real F[9];
+        F[0] = D[0] * B[0] + D[1] * B[3] + D[2] * B[6];
+        F[1] = D[0] * B[1] + D[1] * B[4] + D[2] * B[7];
+        F[2] = D[0] * B[2] + D[1] * B[5] + D[2] * B[8];
+        F[3] = D[3] * B[0] + D[4] * B[3] + D[5] * B[6];
+        F[4] = D[3] * B[1] + D[4] * B[4] + D[5] * B[7];
+        F[5] = D[3] * B[2] + D[4] * B[5] + D[5] * B[8];
+        F[6] = D[6] * B[0] + D[7] * B[3] + D[8] * B[6];
+        F[7] = D[6] * B[1] + D[7] * B[4] + D[8] * B[7];
+        F[8] = D[6] * B[2] + D[7] * B[5] + D[8] * B[8];
+        real R[9];
+        polar(F, R, mode == 1 ? 1 : 0);
+        for (int i = 0; i < 9; i++) R[i] = R[i] - F[i];
+        const real* AC = &L->AC[t * 12];
+        real vol = L->vol[t];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 4; j++) {
+                real acc = 0;
+                for (int m = 0; m < 3; m++) acc += R[i * 3 + m] * AC[m * 4 + j];
+                if (isnan(acc)) acc = 0;
+                acc = acc > 10 ? 10 : acc;
+                acc = acc < -10 ? -10 : acc;
+                real r = acc * vol * k;
+                if (mode == 1 && R_FABS(r) < 1e-5) r = 0;
+                L->cf[t * 12 + j * 3 + i] = r;
+            }
+    }
+}
+
+/* Gather in increasing tet order == sequential scatter order. */
+static void gather_forces(orc_level* L) {
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < L->V; v++) {
+        real f[3] = {0, 0, 0};
+        for (int e = L->rp[v]; e < L->rp[v + 1]; e++) {
+            int te = L->ri[e];
+            int t = te >> 2, c = te & 3;
+            for (int i = 0; i < 3; i++) f[i] += L->cf[t * 12 + c * 3 + i];
+        }
+        for (int i = 0; i < 3; i++) L->force[v * 3 + i] = f[i];
+    }
+}
+
+/* runClearCollision (PDSolver.cu:691-698). */
+static void clear_collision(orc_level* L) {
+    memset(L->cdiag, 0, sizeof(real) * 3 * (size_t)L->V);
+    memset(L->cforce, 0, sizeof(real) * 3 * (size_t)L->V);
+    memset(L->cenergy, 0, sizeof(real) * (size_t)L->V);
+    memset(L->cdepth, 0, sizeof(real) * (size_t)L->V);
+    memset(L->cflag, 0, sizeof(int) * (size_t)L->V);
+}
+
+/* DCDByPoint_sphere (PDSolver.cu:277-331), one active sphere, all vertices. */
+static void dcd_point_sphere(orc_level* L, v3 center, real radius, real kc) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < L->V; i++) {
+        v3 p = vload(L->x + i * 3);
+        v3 cp = vsub(p, center);
+        v3 dir = vdiv(cp, vlen(cp));
+        real distance = vlen(cp);
+        real oldR = radius;
+        real r = radius * (real)1.5f;
+        if (distance < r) {
+            real a = 1;
+            real b = 2 * vdot(cp, dir);
+            real c = vdot(cp, cp) - r * r;
+            real t = (-b + R_SQRT(b * b - 4 * a * c)) / (2 * a);
+            if (t > (real)0.5f * oldR) {
+                L->cflag[i] = 1;
+                L->cdepth[i] = t - (real)0.5f * oldR;
+            }
+            t = t / (real)2.0f;
+            v3 colP = vadd(p, vmul(dir, t));
+            v3 ccp = vsub(colP, center);
+            v3 n = vdiv(ccp, vlen(ccp));
+            real nd = vdot(n, dir);
+            L->cdiag[i * 3 + 0] += kc * n.x * n.x;
+            L->cdiag[i * 3 + 1] += kc * n.y * n.y;
+            L->cdiag[i * 3 + 2] += kc * n.z * n.z;
+            L->cforce[i * 3 + 0] += kc * t * nd * n.x;
+            L->cforce[i * 3 + 1] += kc * t * nd * n.y;
+            L->cforce[i * 3 + 2] += kc * t * nd * n.z;
+            real e = vdot(vsub(p, colP), n);
+            L->cenergy[i] += kc * (e * e);
+        }
+    }
+}
+
+/* Ground plane n.x = d as a penalty constraint k((x - q).n)^2 with q the
+ * projection on the plane.  Not in the reference; same response form as the
+ * sphere's (PDSolver.cu:314-321). */
+static void dcd_point_plane(orc_level* L, v3 n, real d, real kp) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < L->V; i++) {
+        v3 p = vload(L->x + i * 3);
+        real sd = vdot(n, p) - d;
+        if (sd < 0) {
+            real depth = -sd;
+            L->cflag[i] = 1;
+            if (depth > L->cdepth[i]) L->cdepth[i] = depth;
+            L->cdiag[i * 3 + 0] += kp * n.x * n.x;
+            L->cdiag[i * 3 + 1] += kp * n.y * n.y;
+            L->cdiag[i * 3 + 2] += kp * n.z * n.z;
+            L->cforce[i * 3 + 0] += kp * depth * n.x;
+            L->cforce[i * 3 + 1] += kp * depth * n.y;
+            L->cforce[i * 3 + 2] += kp * depth * n.z;
+            L->cenergy[i] += kp * (depth * depth);
+        }
+    }
+}
+
+/* pointInTriangle (utils.cu:366-383). */
+static int point_in_triangle(v3 P, v3 A, v3 B, v3 C) {
+    v3 v0 = vsub(C, A), v1 = vsub(B, A), v2 = vsub(P, A);
+    real d00 = vdot(v0, v0), d01 = vdot(v0, v1), d02 = vdot(v0, v2), d11 = vdot(v1, v1), d12 = vdot(v1, v2);
+    real inv = (real)1.0f / (d00 * d11 - d01 * d01);
+    real u = (d11 * d02 - d01 * d12) * inv;
+    real w = (d00 * d12 - d01 * d02) * inv;
+    return (u >= 0) && (w >= 0) && (u + w <= (real)1.0f);
+}
+
+/* UpdateTriNormal (PDSolver.cu:394-425) without the vertex accumulation. */
+static void tri_normals(orc_level* L) {
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < L->S; t++) {
+        v3 A = vload(L->x + L->tris[t * 3 + 0] * 3), B = vload(L->x + L->tris[t * 3 + 1] * 3),
+           C = vload(L->x + L->tris[t * 3 + 2] * 3);
+        v3 n = vnormalize(vcross(vsub(B, A), vsub(C, A)));
+        L->trin[t * 3 + 0] = n.x;
+        L->trin[t * 3 + 1] = n.y;
+        L->trin[t * 3 + 2] = n.z;
+    }
+}
+
+/* DCDByTriangle_sphere_V2 (PDSolver.cu:612-680) with the response of
+ * DCDByTriangle_sphere_response (:462-501): each colliding triangle adds k to
+ * the collision diagonal and k*(center - hit) to the collision force of its
+ * three vertices.  Contributions are stored per (tri, corner) and gathered. */
+static void dcd_tri_sphere(orc_level* L, v3 center, real radius, real kc) {
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < L->S; t++) {
+        v3 A = vload(L->x + L->tris[t * 3 + 0] * 3), B = vload(L->x + L->tris[t * 3 + 1] * 3),
+           C = vload(L->x + L->tris[t * 3 + 2] * 3);
+        v3 n = vload(L->trin + t * 3);
+        real distance = R_FABS(vdot(n, vsub(center, A)));
+        if (distance > radius) continue;
+        v3 projected = vsub(center, vmul(n, vdot(n, vsub(center, A))));
+        int hitf = 0;
+        v3 hit = v3m(0, 0, 0);
+        if (point_in_triangle(projected, A, B, C)) {
+            int sign = 1;
+            if (vdot(n, vsub(center, A)) < 0) sign = -1;
+            hit = vadd(center, vmul(vmul(n, radius - distance), (real)sign));
+            hitf = 1;
+        }
+        v3 tv[3] = {A, B, C};
+        for (int i = 0; i < 3 && !hitf; i++) {
+            v3 Bg = tv[i], En = tv[(i + 1) % 3];
+            v3 ed = vsub(En, Bg);
+            real len = vlen(ed);
+            ed = vnormalize(ed);
+            v3 bc = vsub(center, Bg);
+            real tt = vdot(bc, ed);
+            if (tt < 0 || tt > len) continue;
+            v3 closest = vadd(Bg, vmul(ed, tt));
+            v3 radial = vsub(center, closest);
+            real dist = vlen(radial);
+            if (dist >= radius) continue;
+            v3 Oth = tv[(i + 2) % 3];
+            v3 BO = vsub(Oth, Bg), BP = vsub(projected, Bg);
+            if (vdot(vcross(ed, BO), vcross(ed, BP)) >= 0) continue;
+            radial = vnormalize(radial);
+            hit = vadd(closest, vmul(radial, radius));
+            hitf = 1;
+        }
+        for (int i = 0; i < 3 && !hitf; i++) {
+            v3 Vq = tv[i];
+            v3 VO = vsub(center, Vq);
+            if (vlen(VO) > radius) continue;
+            VO = vnormalize(VO);
+            hit = vadd(Vq, vmul(VO, radius));
+            hitf = 1;
+        }
+        if (!hitf) continue;
+        v3 mv = vsub(center, hit);
+        for (int c = 0; c < 3; c++) {
+            L->tc[t * 12 + c * 4 + 0] += kc * mv.x;
+            L->tc[t * 12 + c * 4 + 1] += kc * mv.y;
+            L->tc[t * 12 + c * 4 + 2] += kc * mv.z;
+            L->tc[t * 12 + c * 4 + 3] += kc;
+        }
+    }
+}
+
+static void gather_tri_collision(orc_level* L) {
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < L->V; v++) {
+        for (int e = L->trp[v]; e < L->trp[v + 1]; e++) {
+            int te = L->tri[e];
+            int t = te / 3, c = te % 3;
+            const real* q = &L->tc[t * 12 + c * 4];
+            if (q[3] != 0) {
+                L->cflag[v] += 1;
+                for (int i = 0; i < 3; i++) {
+                    L->cdiag[v * 3 + i] += q[3];
+                    L->cforce[v * 3 + i] += q[i];
+                }
+            }
+        }
+    }
+}
+
+/* DCDByPoint (PDSolver.cpp:228-233) + plane. */
+static void dcd_point(orc_level* L, const orc_params* p) {
+    clear_collision(L);
+    for (int s = 0; s < p->nspheres; s++) {
+        const float* sp = p->sphere[s];
+        if (sp[4] != 0) dcd_point_sphere(L, v3m(sp[0], sp[1], sp[2]), sp[3], p->kcol);
+    }
+    if (p->plane[4] > 0) dcd_point_plane(L, v3m(p->plane[0], p->plane[1], p->plane[2]), p->plane[3], p->plane[4]);
+}
+
+/* DCDByTriangle (PDSolver.cpp:235-241). */
+static void dcd_triangle(orc_level* L, const orc_params* p) {
+    clear_collision(L);
+    tri_normals(L);
+    memset(L->tc, 0, sizeof(real) * 12 * (size_t)L->S);
+    int any = 0;
+    for (int s = 0; s < p->nspheres; s++) {
+        const float* sp = p->sphere[s];
+        if (sp[4] != 0) {
+            dcd_tri_sphere(L, v3m(sp[0], sp[1], sp[2]), sp[3], p->kcol);
+            any = 1;
+        }
+    }
+    if (any) gather_tri_collision(L);
+}
+
+/* calculatePOS (PDSolver.cu:192-235).  The 0.6 under-relaxation is a double
+ * literal in the reference, so the FP32 oracle evaluates that line in double. */
+static void calc_pos(orc_level* L, const orc_params* p, real omega) {
+    real dt = p->dt;
+    double relax = p->relax;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < L->V; i++) {
+        real fix = L->fix[i];
+        real dc = L->mass[i] / (dt * dt);
+        for (int c = 0; c < 3; c++) {
+            int q = i * 3 + c;
+            real pos = L->x[q];
+            real elem = L->force[q] + L->cforce[q];
+            real next = (dc * (L->s[q] - pos) + elem) / (L->diag[i] + L->cdiag[q] + dc) * fix + pos;
+            next = (real)((double)(next - pos) * relax + (double)pos);
+            next = omega * (next - L->xprev[q]) + L->xprev[q];
+            L->xprev[q] = pos;
+            L->x[q] = next;
+        }
+    }
+}
+
+static void move_spheres(orc_params* p) {
+    for (int s = 0; s < p->nspheres; s++) {
+        float* sp = p->sphere[s];
+        sp[0] += sp[5] * p->dt;
+        sp[1] += sp[6] * p->dt;
+        sp[2] += sp[7] * p->dt;
+    }
+}
+
+/* One PD iteration on a level: runClearTemp; DCD; IF; omega; POS. */
+static void pd_iteration(orc_level* L, const orc_params* p, real omega, int tri_dcd) {
+    if (tri_dcd)
+        dcd_triangle(L, p);
+    else
+        dcd_point(L, p);
+    tet_forces(L, p->if_mode);
+    gather_forces(L);
+    calc_pos(L, p, omega);
+}
+
+/* PDSolver::Step (PDSolver.cpp:243-293); sphere colliders move first, as
+ * Simulator::UpdateCollider does before Step (Simulator.cpp:146-149).
+ * `p` is updated in place (sphere centres advance). */
+void PFX(step_single)(orc_level* L, orc_params* p, int nsteps) {
+    for (int st = 0; st < nsteps; st++) {
+        move_spheres(p);
+        calc_st(L, p);
+        real omega = 1;
+        real rho = p->rho;
+        for (int i = 0; i < p->iters; i++) {
+            omega = 4 / (4 - rho * rho * omega);
+            pd_iteration(L, p, omega, 0);
+        }
+        calc_v(L, p);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Two-level scheme (PDSolver_MG.cpp).                                        */
+/* ------------------------------------------------------------------------- */
+/* GetVolumn (utils.cu:254-275). */
+static real get_volume(v3 A, v3 B, v3 C, v3 D) {
+    v3 AB = vsub(B, A), AC = vsub(C, A), AD = vsub(D, A);
+    real M[9] = {AB.x, AC.x, AD.x, AB.y, AC.y, AD.y, AB.z, AC.z, AD.z};
+    real R0 = M[4] * M[8] - M[7] * M[5];
+    real R1 = M[7] * M[2] - M[1] * M[8];
+    real R2 = M[1] * M[5] - M[4] * M[2];
+    real det = M[0] * R0 + M[3] * R1 + M[6] * R2;
+    return det / (real)6.0f;
+}
+
+/* barycentricCoordinate, overload on four points (utils.cu:335-364). */
+static void barycentric4(v3 P, v3 A, v3 B, v3 C, v3 D, real* w) {
+    real V = get_volume(A, B, C, D);
+    v3 center = vmul(vadd(vadd(vadd(A, B), C), D), (real)0.25f);
+    v3 tv[4] = {A, B, C, D};
+    static const int pf[4][4] = {{0, 1, 2, 3}, {1, 2, 3, 0}, {2, 3, 0, 1}, {3, 0, 1, 2}};
+    for (int i = 0; i < 4; i++) {
+        v3 p = tv[pf[i][0]], f0 = tv[pf[i][1]], f1 = tv[pf[i][2]], f2 = tv[pf[i][3]];
+        v3 n = vcross(vsub(f1, f0), vsub(f2, f0));
+        if (vdot(n, vsub(p, f0)) > 0) n = v3m(-n.x, -n.y, -n.z);
+        real len = vlen(n);
+        real area = len * (real)0.5f;
+        n = vdiv(n, len);
+        w[i] = (real)0.25f - vdot(vsub(P, center), n) * area / (3 * V);
+    }
+}
+
+void PFX(barycentric4)(const float* P, const float* tet12, float* w) {
+    real ww[4];
+    barycentric4(v3m(P[0], P[1], P[2]), v3m(tet12[0], tet12[1], tet12[2]), v3m(tet12[3], tet12[4], tet12[5]),
+                 v3m(tet12[6], tet12[7], tet12[8]), v3m(tet12[9], tet12[10], tet12[11]), ww);
+    for (int i = 0; i < 4; i++) w[i] = (float)ww[i];
+}
+
+typedef struct orc_mg {
+    orc_level *coarse, *fine;
+    int* iids;   /* fine vertex -> 4 coarse vertex ids   */
+    real* iw;    /* weights                              */
+    int* aids;   /* coarse vertex -> 4 fine vertex ids   */
+    real* aw;
+    int n_unfound_fine, n_unfound_coarse;
+} orc_mg;
+
+/* PDSolver_MG::interpolate (PDSolver_MG.cpp:5-58): each fine vertex binds to
+ * the first coarse tet (in index order) whose four weights all lie in
+ * [-1e-5, 1+1e-5].  Brute force, O(V_fine * T_coarse). */
+static void mg_interpolate_map(orc_mg* M) {
+    orc_level *c = M->coarse, *f = M->fine;
+    M->n_unfound_fine = 0;
+    for (int vi = 0; vi < f->V; vi++) {
+        v3 P = vload(f->x + vi * 3);
+        int found = 0;
+        for (int t = 0; t < c->T && !found; t++) {
+            const int* id = &c->tets[t * 4];
+            real w[4];
+            barycentric4(P, vload(c->x + id[0] * 3), vload(c->x + id[1] * 3), vload(c->x + id[2] * 3),
+                         vload(c->x + id[3] * 3), w);
+            int in = 1;
+            for (int i = 0; i < 4; i++)
+                if (w[i] < -1e-5 || w[i] > 1 + 1e-5) in = 0;
+            if (in) {
+                for (int i = 0; i < 4; i++) {
+                    M->iids[vi * 4 + i] = id[i];
+                    M->iw[vi * 4 + i] = w[i];
+                }
+                found = 1;
+            }
+        }
+        if (!found) M->n_unfound_fine++;
+    }
+}
+
+/* pointInTet (utils.cu:298-311). */
+static int point_in_tet(const real* tv12, const real* faceN12, v3 P) {
+    for (int k = 0; k < 4; k++) {
+        int ia = (k + 1) % 4;
+        v3 A = vload(tv12 + ia * 3);
+        v3 n = vload(faceN12 + k * 3);
+        if (vdot(vsub(P, A), n) > 1e-6) return 0;
+    }
+    return 1;
+}
+
+/* Coarse-in-fine binding (PDSolver_MG.cpp:133-168): first fine tet (index
+ * order) passing pointInTet, weights from the precomputed-face barycentric
+ * overload (utils.cu:313-333). */
+static void mg_average_map(orc_mg* M) {
+    orc_level *c = M->coarse, *f = M->fine;
+    M->n_unfound_coarse = 0;
+    for (int vc = 0; vc < c->V; vc++) {
+        v3 P = vload(c->x + vc * 3);
+        int found = 0;
+        for (int t = 0; t < f->T && !found; t++) {
+            const int* id = &f->tets[t * 4];
+            real tv[12];
+            for (int i = 0; i < 4; i++)
+                for (int q = 0; q < 3; q++) tv[i * 3 + q] = f->x[id[i] * 3 + q];
+            if (!point_in_tet(tv, f->faceN + t * 12, P)) continue;
+            v3 rB = vload(f->center + t * 3);
+            real V = f->vol[t];
+            for (int i = 0; i < 4; i++) {
+                v3 n = vload(f->faceN + t * 12 + i * 3);
+                M->aids[vc * 4 + i] = id[i];
+                M->aw[vc * 4 + i] = (real)0.25f - vdot(vsub(P, rB), n) * f->faceA[t * 4 + i] / (3 * V);
+            }
+            found = 1;
+        }
+        if (!found) M->n_unfound_coarse++;
+    }
+}
+
+orc_mg* PFX(mg_create)(orc_level* coarse, orc_level* fine) {
+    orc_mg* M = (orc_mg*)calloc(1, sizeof(orc_mg));
+    M->coarse = coarse;
+    M->fine = fine;
+    M->iids = (int*)xcalloc((size_t)fine->V * 4, sizeof(int));
+    M->iw = (real*)xcalloc((size_t)fine->V * 4, sizeof(real));
+    M->aids = (int*)xcalloc((size_t)coarse->V * 4, sizeof(int));
+    M->aw = (real*)xcalloc((size_t)coarse->V * 4, sizeof(real));
+    mg_interpolate_map(M);
+    mg_average_map(M);
+    return M;
+}
+
+void PFX(mg_free)(orc_mg* M) {
+    if (!M) return;
+    free(M->iids); free(M->iw); free(M->aids); free(M->aw);
+    free(M);
+}
+
+int PFX(mg_unfound)(const orc_mg* M, int which) { return which ? M->n_unfound_coarse : M->n_unfound_fine; }
+
+void PFX(mg_get_maps)(const orc_mg* M, int* iids, float* iw, int* aids, float* aw) {
+    for (int i = 0; i < M->fine->V * 4; i++) { iids[i] = M->iids[i]; iw[i] = (float)M->iw[i]; }
+    for (int i = 0; i < M->coarse->V * 4; i++) { aids[i] = M->aids[i]; aw[i] = (float)M->aw[i]; }
+}
+
+/* updatePointInTet (PDSolver.cu:724-744): dst.x = dst.xprev = sum w src.x. */
+static void update_point_in_tet(orc_level* dst, const orc_level* src, const int* ids, const real* w) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < dst->V; i++) {
+        const int* id = ids + i * 4;
+        const real* ww = w + i * 4;
+        for (int c = 0; c < 3; c++) {
+            real r = ww[0] * src->x[id[0] * 3 + c] + ww[1] * src->x[id[1] * 3 + c] + ww[2] * src->x[id[2] * 3 + c] +
+                     ww[3] * src->x[id[3] * 3 + c];
+            dst->x[i * 3 + c] = r;
+            dst->xprev[i * 3 + c] = r;
+        }
+    }
+}
+
+/* updateMapping (PDSolver.cu:764-785). */
+static void update_mapping(orc_mg* M) {
+    orc_level *c = M->coarse, *f = M->fine;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < f->V; i++) {
+        const int* id = M->iids + i * 4;
+        real w[4];
+        barycentric4(vload(f->x + i * 3), vload(c->x + id[0] * 3), vload(c->x + id[1] * 3), vload(c->x + id[2] * 3),
+                     vload(c->x + id[3] * 3), w);
+        for (int q = 0; q < 4; q++) M->iw[i * 4 + q] = w[q];
+    }
+}
+
+/* PDSolver_MG::Step (PDSolver_MG.cpp:183-253). The same parameter block serves
+ * both levels, as both PDSolver::Init calls set the same constants. */
+void PFX(step_mg)(orc_mg* M, orc_params* p, int nsteps) {
+    orc_level *c = M->coarse, *f = M->fine;
+    real rho = p->rho;
+    for (int st = 0; st < nsteps; st++) {
+        move_spheres(p);
+        calc_st(c, p);
+        calc_st(f, p);
+        real omega = 1;
+        int mode = p->if_mode;
+        orc_params pc = *p;
+        pc.if_mode = mode;
+        for (int i = 0; i < p->iters_coarse; i++) {
+            omega = 4 / (4 - rho * rho * omega);
+            pd_iteration(c, &pc, omega, p->tri_dcd_coarse);
+        }
+        update_point_in_tet(f, c, M->iids, M->iw);
+        omega = 1;
+        for (int i = 0; i < p->iters; i++) {
+            omega = 4 / (4 - rho * rho * omega);
+            pd_iteration(f, p, omega, 0);
+        }
+        update_point_in_tet(c, f, M->aids, M->aw);
+        update_mapping(M);
+        calc_v(c, p);
+        calc_v(f, p);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Diagnostics.                                                               */
+/* ------------------------------------------------------------------------- */
+/* runCalEnergy (PDSolver.cu:816-913): out = {Ek, Ep, Ec, Nc, dX}, in double. */
+void PFX(energy)(const orc_level* L, const orc_params* p, double* out) {
+    double Ek = 0, Ep = 0, Ec = 0, Nc = 0, dX = 0;
+    double dt = p->dt;
+    for (int i = 0; i < L->V; i++) {
+        double dx2 = 0, ds2 = 0;
+        for (int c = 0; c < 3; c++) {
+            double a = (double)L->x[i * 3 + c] - (double)L->xprev[i * 3 + c];
+            double b = (double)L->x[i * 3 + c] - (double)L->s[i * 3 + c];
+            dx2 += a * a;
+            ds2 += b * b;
+        }
+        dX += dx2;
+        Ek += (double)L->mass[i] * ds2;
+    }
+    dX = sqrt(dX);
+    Ek = Ek / (2 * dt * dt);
+    for (int t = 0; t < L->T; t++) {
+        const int* id = &L->tets[t * 4];
+        const real* X = L->x;
+        real D[9];
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) D[r * 3 + q] = X[id[q + 1] * 3 + r] - X[id[0] * 3 + r];
+        const real* B = &L->B[t * 9];
+        real F[9], R[9];
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) F[r * 3 + q] = D[r * 3 + 0] * B[q] + D[r * 3 + 1] * B[3 + q] + D[r * 3 + 2] * B[6 + q];
+        polar(F, R, 0);
+        double e = 0;
+        for (int i = 0; i < 9; i++) e += ((double)F[i] - R[i]) * ((double)F[i] - R[i]);
+        Ep += e * (double)L->vol[t] * (double)L->stiffness / 2;
+    }
+    for (int i = 0; i < L->V; i++) {
+        Ec += L->cenergy[i];
+        Nc += L->cflag[i] > 0 ? 1 : 0;
+    }
+    out[0] = Ek; out[1] = Ep; out[2] = Ec; out[3] = Nc; out[4] = dX;
+}
+
+/* Vertex normals, runUpdateOutsideTetVertNormal (PDSolver.cu:394-460):
+ * unit triangle normals summed per vertex then normalised if |n| > FLT_EPSILON. */
+void PFX(vertex_normals)(orc_level* L) {
+    tri_normals(L);
+    for (int v = 0; v < L->V; v++) {
+        real n[3] = {0, 0, 0};
+        for (int e = L->trp[v]; e < L->trp[v + 1]; e++) {
+            int t = L->tri[e] / 3;
+            for (int i = 0; i < 3; i++) n[i] += L->trin[t * 3 + i];
+        }
+        real len = R_SQRT(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (len > (real)1.1920928955078125e-7) {
+            for (int i = 0; i < 3; i++) n[i] /= len;
+        }
+        for (int i = 0; i < 3; i++) L->vnormal[v * 3 + i] = n[i];
+    }
+}
+
+/* Nonlinear PD residual g(x) = M/h^2 (s - x) + f_el(x) + f_col(x) over free
+ * DOFs, FP64 norm.  The residual the V-cycle drives down; the reference has no
+ * residual (it logs energies), so this is the build's own definition. */
+double PFX(residual_norm)(orc_level* L, const orc_params* p) {
+    dcd_point(L, p);
+    tet_forces(L, 0);
+    gather_forces(L);
+    double acc = 0;
+    real dt = p->dt;
+    for (int i = 0; i < L->V; i++) {
+        if (L->fix[i] == 0) continue;
+        real dc = L->mass[i] / (dt * dt);
+        for (int c = 0; c < 3; c++) {
+            int q = i * 3 + c;
+            double g = (double)dc * ((double)L->s[q] - (double)L->x[q]) + (double)L->force[q] + (double)L->cforce[q];
+            acc += g * g;
+        }
+    }
+    return sqrt(acc);
+}
+
+/* Unit-level entry points for tests. */
+float PFX(polar_f)(const float* F, float* R, int variant) {
+    real FF[9], RR[9];
+    for (int i = 0; i < 9; i++) FF[i] = F[i];
+    real det = polar(FF, RR, variant);
+    for (int i = 0; i < 9; i++) R[i] = (float)RR[i];
+    return (float)det;
+}
+
+float PFX(inverse3_f)(const float* A, float* R) {
+    real AA[9], RR[9];
+    for (int i = 0; i < 9; i++) AA[i] = A[i];
+    real det = mat_inverse3(AA, RR);
+    for (int i = 0; i < 9; i++) R[i] = (float)RR[i];
+    return (float)det;
+}
+
+/* Per-tet corner forces at the current positions (12 per tet, corner-major). */
+void PFX(tet_corner_forces)(orc_level* L, int mode, double* out) {
+    tet_forces(L, mode);
+    for (size_t i = 0; i < (size_t)L->T * 12; i++) out[i] = (double)L->cf[i];
+}
+
+/* One collision pass on the current positions (point DCD or triangle DCD). */
+void PFX(collide)(orc_level* L, const orc_params* p, int tri) {
+    if (tri)
+        dcd_triangle(L, p);
+    else
+        dcd_point(L, p);
+}
