@@ -1,16 +1,22 @@
 """Benchmark of the multigrid PD solver on one MI355X.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload medium]
-                [--schedule vcycle|learned|single|twolevel] [--no-cpu]
+                [--schedule vcycle|single|twolevel] [--fixed-cycles N] [--no-cpu]
 
-One step = one simulation step of the whole body (inertial prediction, PD
-iterations / V-cycles to the residual target, velocity update), all inputs
-resident in HBM.  Prints ONE JSON line.  The CPU baseline is the oracle
-(a C restatement of the reference solver, OpenMP) timed on a bounded sample
-of the same workload on this host.
+One step = one simulation step of the whole body: inertial prediction, then
+V-cycles until the relative residual ||g(x)|| / ||g(s)|| <= 1e-3 (FP64 norms,
+checked on device after every cycle), then the velocity update.  Inputs are
+resident in HBM; the timed region is K back-to-back steps (one hipGraph per
+step) bracketed by stream synchronisations.  Every timed step's final residual
+is folded into device-side run statistics and reported (max_rel, failed).
+
+Prints ONE JSON line.  The CPU baseline is the oracle (C restatement, FP32,
+OpenMP) running the same V-cycle schedule on the same workload on this host,
+timed over a bounded number of steps.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -18,10 +24,10 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-import numpy as np  # noqa: E402
-
 METRIC = "sim steps/sec at rel. residual ≤1e-3, learned vs plain MG; achieved HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+VCYCLE = dict(pre_sweeps=2, post_sweeps=2, coarse_sweeps=2, max_cycles=12, tol=1e-3, relax_coarse=0.6,
+              rho_smooth=0.0, coarse_direct=1)
 
 
 def log(*a):
@@ -32,112 +38,136 @@ def build_workload(name):
     from multigridsoftbody_amd import workloads as W
 
     cfg = {"small": W.SMALL, "medium": W.MEDIUM, "large": W.LARGE}[name]
-    levels = W.beam_hierarchy(cfg["ns"])
-    return cfg, levels
+    return cfg, W.beam_hierarchy(cfg["ns"])
 
 
-def make_solver(levels, schedule, cfg, use_graph=1):
+def make_solver(levels, schedule, cfg, use_graph=1, fixed_cycles=0):
     import multigridsoftbody_amd as M
     from multigridsoftbody_amd import workloads as W
 
     phys = W.physics(gravity=(0.0, -9.8, 0.0) if cfg["gravity"] else (0.0, 0.0, 0.0))
-    sched = {"single": M.SCHED_SINGLE, "twolevel": M.SCHED_TWOLEVEL, "vcycle": M.SCHED_VCYCLE,
-             "learned": M.SCHED_LEARNED}[schedule]
-    lv = levels if sched != M.SCHED_SINGLE else levels[:1]
-    if sched == M.SCHED_TWOLEVEL:
-        lv = [levels[0], levels[-1]]
-    s = M.Solver(lv, schedule=sched, if_mode=M.IF_AC if sched == M.SCHED_SINGLE else M.IF_PLAIN,
-                 iters=cfg["iters"], iters_coarse=cfg["iters"], use_graph=use_graph, **phys)
+    if schedule == "vcycle":
+        kw = dict(VCYCLE)
+        if fixed_cycles:
+            kw.update(max_cycles=fixed_cycles, check_every=0)
+        else:
+            kw.update(check_every=1)
+        s = M.Solver(levels, schedule=M.SCHED_VCYCLE, if_mode=M.IF_STABLE, use_graph=use_graph, **kw, **phys)
+    elif schedule == "twolevel":
+        s = M.Solver([levels[0], levels[-1]], schedule=M.SCHED_TWOLEVEL, if_mode=M.IF_PLAIN, iters=cfg["iters"],
+                     iters_coarse=cfg["iters"], use_graph=use_graph, **phys)
+    else:
+        s = M.Solver(levels[:1], schedule=M.SCHED_SINGLE, if_mode=M.IF_AC, iters=cfg["iters"], use_graph=use_graph,
+                     **phys)
     s.fix_plane(1, 10.0, 1e-5)
     fix = W.top_fixflag(levels[0])
     s.set_velocity(W.seeded_velocity(levels[0], fix))
+    if cfg["name"].startswith("large"):
+        c = W.large_colliders()
+        s.set_sphere(0, c["sphere"]["center"], c["sphere"]["radius"], c["sphere"]["vel"])
+        s.set_params(kcol=c["k"])
+        s.set_plane(c["plane"]["normal"], c["plane"]["d"], c["k"])
     return s
 
 
-def cpu_baseline(levels, cfg, budget_s=15.0):
-    """Oracle (FP32 reference order, OpenMP) on the finest level, single-level
-    reference schedule with the same iteration count; times as many steps as
-    fit in ~budget_s."""
+def cpu_baseline(levels, cfg, schedule, budget_s=15.0):
+    """The oracle on the same workload and schedule, FP32, OpenMP threads."""
     import oracle as O
     from multigridsoftbody_amd import workloads as W
 
     threads = min(16, os.cpu_count() or 1)
     O.set_threads(threads)
-    mesh = levels[0]
     phys = W.physics(gravity=(0.0, -9.8, 0.0) if cfg["gravity"] else (0.0, 0.0, 0.0))
-    L = O.Level(mesh.pos, mesh.tets, mesh.tris, stiffness=phys["stiffness"], density=phys["density"])
-    fix = W.top_fixflag(mesh)
-    L.set_fix(fix)
-    L.set_velocity(W.seeded_velocity(mesh, fix))
-    p = O.params(dt=phys["dt"], stiffness=phys["stiffness"], density=phys["density"], gravity=phys["gravity"],
-                 iters=cfg["iters"], if_mode=1)
-    L.step(p, 1)  # warm-up
+    lv = levels if schedule == "vcycle" else levels[:1]
+    Ls = [O.Level(m.pos, m.tets, m.tris, stiffness=phys["stiffness"], density=phys["density"]) for m in lv]
+    fix = W.top_fixflag(levels[0])
+    Ls[0].set_fix(fix)
+    Ls[0].set_velocity(W.seeded_velocity(levels[0], fix))
+    if schedule == "vcycle":
+        t0 = time.perf_counter()
+        ML = O.MultiLevel(Ls, relax_coarse=VCYCLE["relax_coarse"], dt=phys["dt"])
+        log(f"[bench] oracle multilevel set-up {time.perf_counter() - t0:.1f}s (not timed)")
+        p = O.params(dt=phys["dt"], stiffness=phys["stiffness"], density=phys["density"], gravity=phys["gravity"],
+                     if_mode=3)
+        vp = O.vparams(pre=VCYCLE["pre_sweeps"], post=VCYCLE["post_sweeps"], coarse_sweeps=VCYCLE["coarse_sweeps"],
+                       max_cycles=VCYCLE["max_cycles"], tol=VCYCLE["tol"], check=1)
+        step = lambda: ML.step(p, vp, 1)  # noqa: E731
+        what = "V-cycle schedule (same parameters, adaptive cycles to rel. residual 1e-3)"
+    else:
+        p = O.params(dt=phys["dt"], stiffness=phys["stiffness"], density=phys["density"], gravity=phys["gravity"],
+                     iters=cfg["iters"], if_mode=1)
+        step = lambda: Ls[0].step(p, 1)  # noqa: E731
+        what = f"single-level reference schedule ({cfg['iters']} PD iterations/step)"
+    step()
     n, t0 = 0, time.perf_counter()
     while time.perf_counter() - t0 < budget_s:
-        L.step(p, 1)
+        step()
         n += 1
     dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} steps of the single-level reference schedule ({cfg['iters']} PD iterations/step, "
-                      f"Ac local step) on the {mesh.ntets}-tet finest level, FP32 oracle, {threads} OpenMP threads"}
+    return {"value": round(n / dt, 4), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} steps (after 1 warm-up) of the {what} on the {levels[0].ntets}-tet workload, "
+                      f"FP32 oracle, {threads} OpenMP threads"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="medium", choices=["small", "medium", "large"])
-    ap.add_argument("--schedule", default="single", choices=["single", "twolevel", "vcycle", "learned"])
+    ap.add_argument("--schedule", default="vcycle", choices=["single", "twolevel", "vcycle"])
+    ap.add_argument("--fixed-cycles", type=int, default=0,
+                    help="V-cycles per step without the convergence check (profiling)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(per-body sharding across GPUs is described in DESIGN.md, not built).", file=sys.stderr)
+              "(sharding across GPUs is described in DESIGN.md, not built).", file=sys.stderr)
         sys.exit(2)
 
     log(f"[bench] workload={args.workload} schedule={args.schedule} steps={args.steps} warmup={args.warmup}")
     cfg, levels = build_workload(args.workload)
-    s = make_solver(levels, args.schedule, cfg, use_graph=0 if args.no_graph else 1)
-    log("[bench] solver created")
+    t0 = time.perf_counter()
+    s = make_solver(levels, args.schedule, cfg, use_graph=0 if args.no_graph else 1, fixed_cycles=args.fixed_cycles)
+    log(f"[bench] solver created in {time.perf_counter() - t0:.1f}s; tets per level {[m.ntets for m in levels]}")
     s.step(args.warmup)
     s.sync()
+    if args.schedule == "vcycle":
+        s.run_stats(reset=True)
     log("[bench] warmup done")
     t0 = time.perf_counter()
     s.step(args.steps)
     s.sync()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
+    elapsed = time.perf_counter() - t0
     steps_per_s = args.steps / elapsed
-    stats = s.stats
     log(f"[bench] timed {args.steps} steps in {elapsed:.3f}s -> {steps_per_s:.1f} steps/s")
+    run = s.run_stats(reset=True) if args.schedule == "vcycle" else None
+    if run:
+        log(f"[bench] run stats {run}")
 
-    # relative residual of the state reached (FP64 norms on device)
-    res_end = s.residual()
-
-    # dominant kernel: per-tet local step on the finest level, timed with HIP
-    # events on the solver's stream over a separate pass of the same steps
+    # dominant kernel: finest-level local step, HIP events around each launch
+    # on the solver's stream, over a pass with a fixed cycle count (no skipped
+    # launches) equal to the mean cycles of the timed run
     T, V = levels[0].ntets, levels[0].nverts
+    mean_cycles = (run["total_cycles"] / max(1, run["steps"])) if run else None
+    if args.schedule == "vcycle" and not args.fixed_cycles:
+        s.set_params(max_cycles=max(1, math.ceil(mean_cycles)), check_every=0)
     s.set_timing(True)
     nt = min(args.steps, 50)
     s.step(nt)
     ms, launches = s.kernel_time(0)
     s.set_timing(False)
-    # launches on coarse levels are cheaper; attribute by tet count
-    lv_t = [lv.ntets for lv in s.levels]
     avg_ms = ms / max(1, launches)
     alg_bytes = 56 * T + 12 * V
-    # per-launch average over all levels weighs small coarse launches in; the
-    # fine-level share dominates the time and is what the roofline is stated for
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    log(f"[bench] per-tet kernel: {launches} launches, avg {avg_ms*1e3:.2f} us, levels tets={lv_t}")
+    log(f"[bench] finest-level local step: {launches} launches, avg {avg_ms * 1e3:.2f} us")
 
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
-        cpu = cpu_baseline(levels, cfg)
-        log(f"[bench] cpu baseline {cpu['value']:.3f} steps/s")
+        cpu = cpu_baseline(levels, cfg, args.schedule)
+        log(f"[bench] cpu baseline {cpu['value']} steps/s")
 
     out = {
         "metric": METRIC,
@@ -154,11 +184,13 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": cfg["name"],
-            "levels_tets": lv_t,
+            "levels_tets": [m.ntets for m in levels] if args.schedule == "vcycle" else [levels[0].ntets],
             "schedule": args.schedule,
-            "pd_iterations_per_step": cfg["iters"],
-            "rel_residual_end": None,
-            "abs_residual_end": res_end,
+            "solver": "plain MG (V-cycle, no learned warm start)" if args.schedule == "vcycle" else args.schedule,
+            "tol": VCYCLE["tol"] if args.schedule == "vcycle" else None,
+            "mean_cycles_per_step": round(mean_cycles, 3) if mean_cycles else None,
+            "max_rel_residual": run["max_rel"] if run else None,
+            "steps_over_tol": run["failed_steps"] if run else None,
             "graph": not args.no_graph,
         },
         "roofline": {
@@ -168,14 +200,12 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": None,
-            "kernel": "k_tet (per-tet local step)",
+            "kernel": "k_tet (finest-level local step)",
             "avg_us": round(avg_ms * 1e3, 3),
             "alg_bytes_per_launch": alg_bytes,
         },
         "cpu_baseline": cpu,
     }
-    if stats.res0 > 0:
-        out["config"]["rel_residual_end"] = stats.rel
     print(json.dumps(out), flush=True)
 
 

@@ -42,6 +42,10 @@ extern "C" {
 /* Local-step variants. */
 #define MGSB_IF_PLAIN 0 /* calculateIF, PDSolver.cu:128-183                         */
 #define MGSB_IF_AC 1    /* getTetFRV2 + calTetIFV1, PDSolverAc.cu:447-662 (|f|<1e-5 dropped) */
+#define MGSB_IF_STABLE 2 /* calculateIF with the Franca polar evaluated from deviatoric     \
+                            invariants (k = 1.5 tr C'^2, l = 13.5 det C'): same maths, keeps \
+                            small anisotropic strains the reference's FP32 k < 1e-6 branch   \
+                            drops; the V-cycle's default                                     */
 
 #define MGSB_MAX_SPHERES 8
 
@@ -84,7 +88,11 @@ typedef struct mgsb_params {
     int32_t block_tet;    /* workgroup size of per-tet kernels (0: tuned)        */
     int32_t block_vert;   /* workgroup size of per-vertex kernels (0: tuned)     */
     int32_t device;       /* HIP device ordinal                                  */
-    int32_t reserved[8];
+    float rho_smooth;     /* V-cycle: Chebyshev rho of the fine smoother (0: damped Jacobi) */
+    float relax_coarse;   /* V-cycle: damped-Jacobi weight on coarse levels (0.6) */
+    float rho_coarse;     /* V-cycle: Chebyshev rho on coarse levels; <= 0: power iteration */
+    int32_t coarse_direct;/* V-cycle: FP64 Cholesky solve on the coarsest level (1) */
+    int32_t reserved[4];
 } mgsb_params;
 
 typedef struct mgsb_step_stats {
@@ -96,6 +104,14 @@ typedef struct mgsb_step_stats {
     int32_t steps;        /* steps taken by this call                             */
     int32_t reserved;
 } mgsb_step_stats;
+
+/* Aggregates over the V-cycle steps since the last reset. */
+typedef struct mgsb_run_stats_t {
+    double max_rel;         /* largest end-of-step ||g|| / ||g(s)||            */
+    int64_t total_cycles;   /* V-cycles summed over the steps                  */
+    int32_t steps;
+    int32_t failed_steps;   /* steps whose final relative residual > tol       */
+} mgsb_run_stats_t;
 
 /* Energy diagnostics, runCalEnergy (PDSolver.cu:816-913): Ek, Ep, Ec, Nc, dX. */
 typedef struct mgsb_energy_t {
@@ -204,12 +220,24 @@ int mgsb_energy(mgsb_solver* s, mgsb_energy_t* e);
 /* ||g(x)|| of the finest level at its current state (FP64). */
 int mgsb_residual(mgsb_solver* s, double* res);
 
+/* Run aggregates of the V-cycle schedules (no reference counterpart: the
+ * reference has no residual); reset != 0 clears them after reading. */
+int mgsb_run_stats(mgsb_solver* s, mgsb_run_stats_t* out, int32_t reset);
+
+/* Chebyshev rho used on a coarse level of the V-cycle (power iteration on
+ * I - relax_coarse D^-1 A at create, or rho_coarse), and the number of free
+ * vertices of the coarsest level's dense FP64 Cholesky factor (-1: none). */
+int mgsb_level_rho(mgsb_solver* s, int32_t level, double* rho);
+int mgsb_coarse_nfree(mgsb_solver* s, int32_t* nfree);
+
 /* Sizes of a level. */
 int mgsb_level_info(mgsb_solver* s, int32_t level, int32_t* nverts, int32_t* ntets, int32_t* ntris);
 
 /* Kernel timing: when on, HIP events bracket every launch of the dominant
- * per-tet kernel on the solver's stream; mgsb_kernel_time returns the summed
- * milliseconds and the launch count since timing was switched on. */
+ * kernel -- the per-tet local step on the finest level (kernel 0) -- on the
+ * solver's stream (steps then run eagerly, not as a graph);
+ * mgsb_kernel_time returns the summed milliseconds and the launch count since
+ * timing was switched on. */
 int mgsb_set_timing(mgsb_solver* s, int32_t on);
 int mgsb_kernel_time(mgsb_solver* s, int32_t kernel, double* ms, int64_t* launches);
 

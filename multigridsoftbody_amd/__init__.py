@@ -20,7 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmgsb.so")
 
 SCHED_SINGLE, SCHED_TWOLEVEL, SCHED_VCYCLE, SCHED_LEARNED = 0, 1, 2, 3
-IF_PLAIN, IF_AC = 0, 1
+IF_PLAIN, IF_AC, IF_STABLE = 0, 1, 2
 MAX_SPHERES = 8
 
 
@@ -65,7 +65,11 @@ class mgsb_params(C.Structure):
         ("block_tet", C.c_int32),
         ("block_vert", C.c_int32),
         ("device", C.c_int32),
-        ("reserved", C.c_int32 * 8),
+        ("rho_smooth", C.c_float),
+        ("relax_coarse", C.c_float),
+        ("rho_coarse", C.c_float),
+        ("coarse_direct", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -79,6 +83,11 @@ class mgsb_step_stats(C.Structure):
         ("steps", C.c_int32),
         ("reserved", C.c_int32),
     ]
+
+
+class mgsb_run_stats_t(C.Structure):
+    _fields_ = [("max_rel", C.c_double), ("total_cycles", C.c_int64), ("steps", C.c_int32),
+                ("failed_steps", C.c_int32)]
 
 
 class mgsb_energy_t(C.Structure):
@@ -121,6 +130,9 @@ SIGNATURES = {
     "mgsb_energy": (C.c_int, [_vp, C.POINTER(mgsb_energy_t)]),
     "mgsb_residual": (C.c_int, [_vp, _dp]),
     "mgsb_level_info": (C.c_int, [_vp, C.c_int32, _ip, _ip, _ip]),
+    "mgsb_level_rho": (C.c_int, [_vp, C.c_int32, _dp]),
+    "mgsb_run_stats": (C.c_int, [_vp, C.POINTER(mgsb_run_stats_t), C.c_int32]),
+    "mgsb_coarse_nfree": (C.c_int, [_vp, _ip]),
     "mgsb_set_timing": (C.c_int, [_vp, C.c_int32]),
     "mgsb_kernel_time": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
     "mgsb_last_error": (C.c_char_p, []),
@@ -403,6 +415,22 @@ class Solver:
         _check(lib.mgsb_residual(self._h, C.byref(r)), "mgsb_residual")
         return r.value
 
+    def run_stats(self, reset=False):
+        st = mgsb_run_stats_t()
+        _check(lib.mgsb_run_stats(self._h, C.byref(st), int(reset)), "mgsb_run_stats")
+        return {"max_rel": st.max_rel, "total_cycles": st.total_cycles, "steps": st.steps,
+                "failed_steps": st.failed_steps}
+
+    def level_rho(self, level):
+        r = C.c_double()
+        _check(lib.mgsb_level_rho(self._h, level, C.byref(r)), "mgsb_level_rho")
+        return r.value
+
+    def coarse_nfree(self):
+        n = C.c_int32()
+        _check(lib.mgsb_coarse_nfree(self._h, C.byref(n)), "mgsb_coarse_nfree")
+        return n.value
+
     def set_timing(self, on):
         _check(lib.mgsb_set_timing(self._h, int(on)), "mgsb_set_timing")
 
@@ -442,4 +470,4 @@ class PDSolver_MG(Solver):  # noqa: N801  (reference name)
 
 
 __all__ = ["lib", "Mesh", "Solver", "PDSolver", "PDSolver_MG", "default_params", "MgsbError", "SCHED_SINGLE",
-           "SCHED_TWOLEVEL", "SCHED_VCYCLE", "SCHED_LEARNED", "IF_PLAIN", "IF_AC"]
+           "SCHED_TWOLEVEL", "SCHED_VCYCLE", "SCHED_LEARNED", "IF_PLAIN", "IF_AC", "IF_STABLE"]

@@ -128,6 +128,93 @@ __device__ __forceinline__ void polar_rf(const float F[9], float RF[9]) {
     }
 }
 
+// The same Franca formulas evaluated from deviatoric invariants: with
+// C' = C - I_c/3 I, k = I_c^2 - 3 II_c = 1.5 tr(C'^2) and l = 13.5 det C'.
+// Algebraically identical to polar_rf, but k no longer comes from the
+// difference of two O(10) numbers, so small anisotropic strains survive in
+// FP32 (the reference form drops them below k < 1e-6).  Isotropic branch only
+// for k < 1e-30.  Writes R - F into RF.
+__device__ __forceinline__ void polar_rf_stable(const float F[9], float RF[9]) {
+    float C[9], C2[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) C[i * 3 + j] = F[0 * 3 + i] * F[0 * 3 + j] + F[1 * 3 + i] * F[1 * 3 + j] + F[2 * 3 + i] * F[2 * 3 + j];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            C2[i * 3 + j] = C[i * 3 + 0] * C[j * 3 + 0] + C[i * 3 + 1] * C[j * 3 + 1] + C[i * 3 + 2] * C[j * 3 + 2];
+    const float det = F[0] * F[4] * F[8] + F[1] * F[5] * F[6] + F[3] * F[7] * F[2] - F[2] * F[4] * F[6] -
+                      F[1] * F[3] * F[8] - F[0] * F[5] * F[7];
+    const float I_c = C[0] + C[4] + C[8];
+    const float m = I_c / 3;
+    const float D0 = C[0] - m, D4 = C[4] - m, D8 = C[8] - m;
+    const float k = 1.5f * (D0 * D0 + D4 * D4 + D8 * D8 + 2 * (C[1] * C[1] + C[2] * C[2] + C[5] * C[5]));
+    float iu[9];
+    bool ok = true;
+    if (k < 1e-30f) {
+        if (lt_1e6(I_c)) {
+            ok = false;
+        } else {
+            const float inv_lambda = 1 / sqrtf(m);
+#pragma unroll
+            for (int i = 0; i < 9; i++) iu[i] = 0.f;
+            iu[0] = iu[4] = iu[8] = inv_lambda;
+        }
+    } else {
+        const float l = 13.5f * (D0 * (D4 * D8 - C[5] * C[5]) - C[1] * (C[1] * D8 - C[5] * C[2]) +
+                                 C[2] * (C[1] * C[5] - D4 * C[2]));
+        const float k_root = sqrtf(k);
+        float value = l / (k * k_root);
+        value = value < -1.0f ? -1.0f : value;
+        value = value > 1.0f ? 1.0f : value;
+        const float phi = acosf(value);
+        const float lambda2 = m + 2.0f / 3 * k_root * cosf(phi / 3);
+        const float lambda = sqrtf(lambda2);
+        const float III_u = det;
+        float I_u = 0.f, II_u = 0.f;
+        if (lt_1e6(fabsf(lambda))) ok = false;
+        if (ok && lt_1e6(-lambda2 + I_c + 2 * III_u / lambda)) ok = false;
+        if (ok) {
+            I_u = lambda + sqrtf(-lambda2 + I_c + 2 * III_u / lambda);
+            II_u = (I_u * I_u - I_c) * 0.5f;
+            if (lt_1e6(fabsf(I_u * II_u - III_u))) ok = false;
+        }
+        if (ok && lt_1e6(fabsf(III_u))) ok = false;
+        if (ok) {
+            float inv_rate = 1 / (I_u * II_u - III_u);
+            const float fdiag = I_u * III_u * inv_rate;
+            const float fc = (I_u * I_u - II_u) * inv_rate;
+            float U[9];
+#pragma unroll
+            for (int i = 0; i < 9; i++) U[i] = fc * C[i] - inv_rate * C2[i];
+            U[0] += fdiag;
+            U[4] += fdiag;
+            U[8] += fdiag;
+            inv_rate = 1 / III_u;
+            const float gdiag = II_u * inv_rate;
+            const float gu = -I_u * inv_rate;
+#pragma unroll
+            for (int i = 0; i < 9; i++) iu[i] = gu * U[i] + inv_rate * C[i];
+            iu[0] += gdiag;
+            iu[4] += gdiag;
+            iu[8] += gdiag;
+        }
+    }
+    if (ok) {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                RF[i * 3 + j] = (F[i * 3 + 0] * iu[0 * 3 + j] + F[i * 3 + 1] * iu[1 * 3 + j] + F[i * 3 + 2] * iu[2 * 3 + j]) -
+                                F[i * 3 + j];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; i++) RF[i] = ((i % 4) == 0 ? 1.f : 0.f) - F[i];
+    }
+}
+
 // barycentricCoordinate on four points (utils.cu:335-364), GetVolumn inline.
 __device__ __forceinline__ void barycentric4(float3 P, float3 a, float3 b, float3 c, float3 d, float w[4]) {
     const float3 AB = sub3(b, a), AC = sub3(c, a), AD = sub3(d, a);

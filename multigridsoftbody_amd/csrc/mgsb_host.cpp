@@ -802,6 +802,10 @@ void mgsb_default_params(mgsb_params* p) {
     p->block_tet = 0;
     p->block_vert = 0;
     p->device = 0;
+    p->rho_smooth = 0.0f;
+    p->relax_coarse = 0.6f;
+    p->rho_coarse = 0.0f;
+    p->coarse_direct = 1;
 }
 
 const char* mgsb_version(void) { return "mgsb 0.1 (gfx950)"; }

@@ -60,4 +60,13 @@ int bind_points_average(const HostLevel& mesh, const float* pts, int n, std::vec
 // barycentricCoordinate on four points (utils.cu:335-364).
 void barycentric4(const float* P, const float* A, const float* B, const float* C, const float* D, float* w);
 
+// V-cycle set-up (mgsb_mg.cpp).
+void build_restriction(int Vf, int Vc, const std::vector<int32_t>& iids, const std::vector<float>& iw,
+                       std::vector<int32_t>& rtp, std::vector<int32_t>& rti, std::vector<float>& rtw);
+void derive_vfix(const std::vector<float>& vfix_fine, const std::vector<int32_t>& aids, const std::vector<float>& aw,
+                 int Vc, std::vector<float>& vfix_coarse);
+double power_rho(const HostLevel& L, const std::vector<float>& vfix, double dt, double relax, double k);
+int coarse_cholesky(const HostLevel& L, const std::vector<float>& vfix, double dt, double k,
+                    std::vector<int32_t>& dense, std::vector<double>& W, int& n);
+
 }  // namespace mgsb

@@ -1,0 +1,140 @@
+// Host-side set-up of the residual-correction V-cycle: restriction as the
+// transposed prolongation (CSR, fine vertices ascending, so device sums are
+// deterministic), Dirichlet masks of the coarse corrections, the Chebyshev
+// spectral radius per coarse level, and the FP64 Cholesky factor of the
+// coarsest level.  All of it runs once per create (and on dt changes).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "mgsb_internal.h"
+
+namespace mgsb {
+
+void build_restriction(int Vf, int Vc, const std::vector<int32_t>& iids, const std::vector<float>& iw,
+                       std::vector<int32_t>& rtp, std::vector<int32_t>& rti, std::vector<float>& rtw) {
+    rtp.assign((size_t)Vc + 1, 0);
+    for (int i = 0; i < Vf; i++)
+        for (int q = 0; q < 4; q++) rtp[iids[(size_t)i * 4 + q] + 1]++;
+    for (int j = 0; j < Vc; j++) rtp[j + 1] += rtp[j];
+    rti.assign((size_t)rtp[Vc], 0);
+    rtw.assign((size_t)rtp[Vc], 0.f);
+    std::vector<int32_t> fill((size_t)Vc, 0);
+    for (int i = 0; i < Vf; i++)  // ascending fine vertex
+        for (int q = 0; q < 4; q++) {
+            int j = iids[(size_t)i * 4 + q];
+            int e = rtp[j] + fill[j]++;
+            rti[e] = i;
+            rtw[e] = iw[(size_t)i * 4 + q];
+        }
+}
+
+// A coarse vertex is pinned iff the largest-weight corner of its binding into
+// the finer level (coarse-in-fine map) is pinned; on nested grids that is the
+// coincident fine vertex.
+void derive_vfix(const std::vector<float>& vfix_fine, const std::vector<int32_t>& aids, const std::vector<float>& aw,
+                 int Vc, std::vector<float>& vfix_coarse) {
+    vfix_coarse.assign(Vc, 1.f);
+    for (int j = 0; j < Vc; j++) {
+        int best = 0;
+        for (int q = 1; q < 4; q++)
+            if (aw[(size_t)j * 4 + q] > aw[(size_t)j * 4 + best]) best = q;
+        vfix_coarse[j] = vfix_fine[aids[(size_t)j * 4 + best]] != 0.f ? 1.f : 0.f;
+    }
+}
+
+// (L w)_a = sum_t k vol sum_k (Ds(w) B)_k AC[k][a] for a scalar field w.
+static void apply_stiffness(const HostLevel& L, double k, const std::vector<double>& w, std::vector<double>& out) {
+    std::fill(out.begin(), out.end(), 0.0);
+    for (int t = 0; t < L.T; t++) {
+        const int32_t* id = &L.tets[(size_t)t * 4];
+        const float* B = &L.B[(size_t)t * 9];
+        const float* AC = &L.AC[(size_t)t * 12];
+        double d[3] = {w[id[1]] - w[id[0]], w[id[2]] - w[id[0]], w[id[3]] - w[id[0]]};
+        double f[3];
+        for (int c = 0; c < 3; c++) f[c] = d[0] * B[c] + d[1] * B[3 + c] + d[2] * B[6 + c];
+        double kv = k * L.vol[t];
+        for (int a = 0; a < 4; a++) out[id[a]] += kv * (f[0] * AC[a] + f[1] * AC[4 + a] + f[2] * AC[8 + a]);
+    }
+}
+
+// Spectral radius of J = I - relax D^-1 A on the free vertices (A = m/h^2 +
+// L, D its diagonal), 100 power-iteration steps from a fixed start vector.
+double power_rho(const HostLevel& L, const std::vector<float>& vfix, double dt, double relax, double k) {
+    const int V = L.V;
+    std::vector<double> w(V), q(V);
+    for (int i = 0; i < V; i++) w[i] = vfix[i] * (1.0 + 0.5 * std::sin(1.7 * i));
+    double lam = 0;
+    for (int it = 0; it < 100; it++) {
+        double nw = 0;
+        for (int i = 0; i < V; i++) nw += w[i] * w[i];
+        nw = std::sqrt(nw);
+        if (nw == 0) break;
+        for (int i = 0; i < V; i++) w[i] /= nw;
+        apply_stiffness(L, k, w, q);
+        double ny = 0;
+        for (int i = 0; i < V; i++) {
+            double dc = (double)L.mass[i] / (dt * dt);
+            double Aw = dc * w[i] + q[i];
+            double y = vfix[i] * (w[i] - relax * Aw / (dc + (double)L.diag[i]));
+            w[i] = y;
+            ny += y * y;
+        }
+        lam = std::sqrt(ny);
+    }
+    return lam;
+}
+
+// Dense A = m/h^2 + L on the free vertices, FP64 Cholesky A = C C^T, and
+// W = C^-1 (lower triangular), so that A^-1 r = W^T (W r).
+int coarse_cholesky(const HostLevel& L, const std::vector<float>& vfix, double dt, double k,
+                    std::vector<int32_t>& dense, std::vector<double>& W, int& n) {
+    const int V = L.V;
+    dense.assign(V, -1);
+    n = 0;
+    for (int i = 0; i < V; i++)
+        if (vfix[i] != 0.f) dense[i] = n++;
+    if (n == 0) {
+        W.clear();
+        return 0;
+    }
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (int i = 0; i < V; i++)
+        if (dense[i] >= 0) A[(size_t)dense[i] * n + dense[i]] += (double)L.mass[i] / (dt * dt);
+    for (int t = 0; t < L.T; t++) {
+        const float* AC = &L.AC[(size_t)t * 12];
+        double w = (double)L.vol[t] * k;
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) {
+                int ia = dense[L.tets[(size_t)t * 4 + a]], ib = dense[L.tets[(size_t)t * 4 + b]];
+                if (ia < 0 || ib < 0) continue;
+                double s = 0;
+                for (int r = 0; r < 3; r++) s += (double)AC[r * 4 + a] * (double)AC[r * 4 + b];
+                A[(size_t)ia * n + ib] += w * s;
+            }
+    }
+    for (int j = 0; j < n; j++) {
+        double d = A[(size_t)j * n + j];
+        for (int p = 0; p < j; p++) d -= A[(size_t)j * n + p] * A[(size_t)j * n + p];
+        if (!(d > 0)) return -1;
+        d = std::sqrt(d);
+        A[(size_t)j * n + j] = d;
+        for (int i = j + 1; i < n; i++) {
+            double s = A[(size_t)i * n + j];
+            for (int p = 0; p < j; p++) s -= A[(size_t)i * n + p] * A[(size_t)j * n + p];
+            A[(size_t)i * n + j] = s / d;
+        }
+    }
+    // W = C^-1 by forward substitution on the identity, column by column
+    W.assign((size_t)n * n, 0.0);
+    for (int c = 0; c < n; c++) {
+        for (int i = c; i < n; i++) {
+            double s = (i == c) ? 1.0 : 0.0;
+            for (int p = c; p < i; p++) s -= A[(size_t)i * n + p] * W[(size_t)p * n + c];
+            W[(size_t)i * n + c] = s / A[(size_t)i * n + i];
+        }
+    }
+    return 0;
+}
+
+}  // namespace mgsb

@@ -42,6 +42,7 @@ namespace {
 constexpr int kTetModePolar = 0;  // calculateIF
 constexpr int kTetModeAc = 1;     // calTetIFV1: drop |f| < 1e-5
 constexpr int kTetModeLinear = 2; // -F(e) AC w, no clamp: linear operator on corrections
+constexpr int kTetModeStable = 3; // calculateIF with the deviatoric-invariant polar
 
 constexpr int kVertPoint = 1;     // sphere point DCD + plane
 constexpr int kVertTri = 2;       // gather triangle DCD contributions
@@ -153,7 +154,9 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
                                              const float* __restrict__ B, const float* __restrict__ vol,
                                              const int4* __restrict__ slot, float4* __restrict__ cf, float k,
                                              int ntri, const int4* __restrict__ tri, const int4* __restrict__ tslot,
-                                             float4* __restrict__ tcf, const DevColliders* __restrict__ col) {
+                                             float4* __restrict__ tcf, const DevColliders* __restrict__ col,
+                                             const int* __restrict__ skip) {
+    if (skip && *skip) return;
     const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (g >= T) {
         const int t = g - T;
@@ -199,6 +202,8 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
     if (MODE == kTetModeLinear) {
 #pragma unroll
         for (int q = 0; q < 9; q++) RF[q] = -F[q];
+    } else if (MODE == kTetModeStable) {
+        polar_rf_stable(F, RF);
     } else {
         polar_rf(F, RF);
     }
@@ -244,7 +249,8 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
                                               const int* __restrict__ trp, const float4* __restrict__ tcf,
                                               const DevColliders* __restrict__ col, float omega, float relax,
                                               int flags, int* __restrict__ cflag, float* __restrict__ cdepth,
-                                              float* __restrict__ cenergy) {
+                                              float* __restrict__ cenergy, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= V) return;
     float fx = 0.f, fy = 0.f, fz = 0.f;
@@ -504,7 +510,9 @@ __global__ void __launch_bounds__(256) k_energy_tet(int T, const float4* __restr
 __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__ x, const float4* __restrict__ s,
                                                const float4* __restrict__ vc, const int* __restrict__ rp,
                                                const float4* __restrict__ cf, const DevColliders* __restrict__ col,
-                                               int flags, float4* __restrict__ r, double* __restrict__ part) {
+                                               int flags, float4* __restrict__ r, double* __restrict__ part,
+                                               float4* __restrict__ xprev, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
     if (i < V) {
@@ -558,9 +566,234 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
         const float gy = (c.x * (st.y - p.y) + (fy + cfy)) * fix;
         const float gz = (c.x * (st.z - p.z) + (fz + cfz)) * fix;
         if (r) r[i] = make_float4(gx, gy, gz, 0.f);
+        if (xprev) xprev[i] = p;
         v[0] = (double)gx * gx + (double)gy * gy + (double)gz * gz;
     }
-    block_reduce_store<1>(v, part);
+    if (part) block_reduce_store<1>(v, part);
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle kernels.  Every kernel of a cycle returns at once when the step's
+// convergence flag is set, so a captured step with max_cycles cycles costs
+// only empty launches after convergence.
+// ---------------------------------------------------------------------------
+struct VState {
+    double res0;    // ||g(s)||
+    double res;     // ||g(x)|| after the last cycle
+    int converged;  // res <= tol * res0 (checked after each cycle)
+    int cycles;
+    int check;
+    float tol;
+    // run aggregates since the last reset (updated at the end of each step)
+    double max_rel;
+    long long total_cycles;
+    int nsteps;
+    int nfail;      // steps that ended with res > tol * res0
+};
+
+// End of a step: fold the step's outcome into the run aggregates.
+__global__ void k_step_done(VState* st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double rel = st->res0 > 0.0 ? st->res / st->res0 : 0.0;
+    if (rel > st->max_rel) st->max_rel = rel;
+    st->total_cycles += st->cycles;
+    st->nsteps += 1;
+    if (rel > (double)st->tol) st->nfail += 1;
+}
+
+// mode 0: step start (res0 = res); mode 1: end of a cycle.  One block sums
+// the per-block partials in block order (deterministic).
+__global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict__ part, VState* st, int mode) {
+    if (mode == 1 && st->converged) return;
+    __shared__ double sh[256];
+    double a = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) a += part[b];
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < (int)blockDim.x; q++) t += sh[q];
+        const double r = sqrt(t);
+        if (mode == 0) {
+            st->res0 = r;
+            st->res = r;
+            st->cycles = 0;
+            st->converged = (st->check && r == 0.0) ? 1 : 0;
+        } else {
+            st->res = r;
+            st->cycles += 1;
+            if (st->check && r <= (double)st->tol * st->res0) st->converged = 1;
+        }
+    }
+}
+
+// r_c[j] = vfix_j * sum_{fine i ascending} w_ij r_f[i]  (P^T, no atomics).
+__global__ void __launch_bounds__(256) k_restrict(int Vc, const int* __restrict__ rtp, const int* __restrict__ rti,
+                                                  const float* __restrict__ rtw, const float4* __restrict__ rf,
+                                                  const float4* __restrict__ vcl, float4* __restrict__ rc,
+                                                  const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int j = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (j >= Vc) return;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    for (int e = rtp[j]; e < rtp[j + 1]; e++) {
+        const float w = rtw[e];
+        const float4 q = rf[rti[e]];
+        ax = ax + w * q.x;
+        ay = ay + w * q.y;
+        az = az + w * q.z;
+    }
+    const float fx = vcl[j].z;
+    rc[j] = make_float4(ax * fx, ay * fx, az * fx, 0.f);
+}
+
+// dst[i] += mask_i * (P src)[i]; dst2 (optional) receives the new value.
+__global__ void __launch_bounds__(256) k_prolong_add(int N, float4* __restrict__ dst, float4* __restrict__ dst2,
+                                                     const float4* __restrict__ src, const int4* __restrict__ ids,
+                                                     const float4* __restrict__ w, const float4* __restrict__ maskv,
+                                                     const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int4 id = ids[i];
+    const float4 ww = w[i];
+    const float4 a = src[id.x], b = src[id.y], c = src[id.z], d = src[id.w];
+    const float vx = ww.x * a.x + ww.y * b.x + ww.z * c.x + ww.w * d.x;
+    const float vy = ww.x * a.y + ww.y * b.y + ww.z * c.y + ww.w * d.y;
+    const float vz = ww.x * a.z + ww.y * b.z + ww.z * c.z + ww.w * d.z;
+    const float m = maskv[i].z;
+    float4 o = dst[i];
+    o.x = o.x + m * vx;
+    o.y = o.y + m * vy;
+    o.z = o.z + m * vz;
+    dst[i] = o;
+    if (dst2) dst2[i] = o;
+}
+
+// Damped Jacobi + Chebyshev sweep on A e = r with A e = m/h^2 e - f_lin(e)
+// (f_lin gathered from the linear tet pass).  first_zero: e = e_prev = 0.
+__global__ void __launch_bounds__(256) k_lin_smooth(int V, float4* __restrict__ e, float4* __restrict__ ep,
+                                                    const float4* __restrict__ r, const float4* __restrict__ vcl,
+                                                    const int* __restrict__ rp, const float4* __restrict__ cf,
+                                                    float omega, float relax, int first_zero,
+                                                    const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const float4 c = vcl[i];
+    const float d = c.x + c.y;
+    const float fx = c.z;
+    const float4 rr = r[i];
+    if (first_zero) {
+        float yx = 0.f + fx * (relax * ((rr.x - 0.f) / d));
+        float yy = 0.f + fx * (relax * ((rr.y - 0.f) / d));
+        float yz = 0.f + fx * (relax * ((rr.z - 0.f) / d));
+        yx = omega * (yx - 0.f) + 0.f;
+        yy = omega * (yy - 0.f) + 0.f;
+        yz = omega * (yz - 0.f) + 0.f;
+        ep[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        e[i] = make_float4(yx, yy, yz, 0.f);
+        return;
+    }
+    float fxs = 0.f, fys = 0.f, fzs = 0.f;
+    for (int q = rp[i]; q < rp[i + 1]; q++) {
+        const float4 t = cf[q];
+        fxs = fxs + t.x;
+        fys = fys + t.y;
+        fzs = fzs + t.z;
+    }
+    const float4 ec = e[i], pc = ep[i];
+    const float qx = c.x * ec.x - fxs, qy = c.x * ec.y - fys, qz = c.x * ec.z - fzs;
+    float yx = ec.x + fx * (relax * ((rr.x - qx) / d));
+    float yy = ec.y + fx * (relax * ((rr.y - qy) / d));
+    float yz = ec.z + fx * (relax * ((rr.z - qz) / d));
+    yx = omega * (yx - pc.x) + pc.x;
+    yy = omega * (yy - pc.y) + pc.y;
+    yz = omega * (yz - pc.z) + pc.z;
+    ep[i] = ec;
+    e[i] = make_float4(yx, yy, yz, 0.f);
+}
+
+// rr = vfix (r - A e).
+__global__ void __launch_bounds__(256) k_lin_resid(int V, const float4* __restrict__ e, const float4* __restrict__ r,
+                                                   const float4* __restrict__ vcl, const int* __restrict__ rp,
+                                                   const float4* __restrict__ cf, float4* __restrict__ rr,
+                                                   const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    float fxs = 0.f, fys = 0.f, fzs = 0.f;
+    for (int q = rp[i]; q < rp[i + 1]; q++) {
+        const float4 t = cf[q];
+        fxs = fxs + t.x;
+        fys = fys + t.y;
+        fzs = fzs + t.z;
+    }
+    const float4 c = vcl[i], ec = e[i], rv = r[i];
+    const float fx = c.z;
+    rr[i] = make_float4(fx * (rv.x - (c.x * ec.x - fxs)), fx * (rv.y - (c.x * ec.y - fys)),
+                        fx * (rv.z - (c.x * ec.z - fzs)), 0.f);
+}
+
+__global__ void __launch_bounds__(256) k_zero4(int n, float4* __restrict__ a, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+constexpr int kMaxDense = 1024;
+
+// Coarsest level: e = A^-1 r in FP64 with the Cholesky factor C (A = C C^T)
+// applied through W = C^-1: y = W r, e = W^T y.  One workgroup; W is n x n
+// row-major lower triangular; dense[i] maps vertices to rows (-1: pinned).
+__global__ void __launch_bounds__(256) k_coarse_solve(int Vc, int n, const int* __restrict__ dense,
+                                                      const double* __restrict__ W, const float4* __restrict__ r,
+                                                      float4* __restrict__ e, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ double rb[3][kMaxDense];
+    __shared__ double yb[3][kMaxDense];
+    for (int i = threadIdx.x; i < Vc; i += blockDim.x) {
+        const int d = dense[i];
+        if (d >= 0) {
+            const float4 q = r[i];
+            rb[0][d] = q.x;
+            rb[1][d] = q.y;
+            rb[2][d] = q.z;
+        }
+    }
+    __syncthreads();
+    for (int row = threadIdx.x; row < n; row += blockDim.x) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        const double* wr = W + (size_t)row * n;
+        for (int k = 0; k <= row; k++) {
+            const double w = wr[k];
+            a0 += w * rb[0][k];
+            a1 += w * rb[1][k];
+            a2 += w * rb[2][k];
+        }
+        yb[0][row] = a0;
+        yb[1][row] = a1;
+        yb[2][row] = a2;
+    }
+    __syncthreads();
+    for (int row = threadIdx.x; row < n; row += blockDim.x) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int k = row; k < n; k++) {
+            const double w = W[(size_t)k * n + row];
+            a0 += w * yb[0][k];
+            a1 += w * yb[1][k];
+            a2 += w * yb[2][k];
+        }
+        rb[0][row] = a0;
+        rb[1][row] = a1;
+        rb[2][row] = a2;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < Vc; i += blockDim.x) {
+        const int d = dense[i];
+        e[i] = d >= 0 ? make_float4((float)rb[0][d], (float)rb[1][d], (float)rb[2][d], 0.f)
+                      : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
 }
 
 inline int nblk(int n, int bs) { return (n + bs - 1) / bs; }
@@ -593,6 +826,17 @@ struct DevLevel {
     float4 *tcf = nullptr, *tn = nullptr, *vn = nullptr;
     int* cflag = nullptr;
     float *cdepth = nullptr, *cenergy = nullptr;
+    // V-cycle state: residual / correction of this level, Dirichlet mask of
+    // the correction in vcl = (m/h^2, diag, vfix, m), restriction into this
+    // level from the next finer one (P^T as CSR), coarsest dense factor.
+    float4 *r = nullptr, *e = nullptr, *ep = nullptr, *rr = nullptr, *vcl = nullptr;
+    int *rtp = nullptr, *rti = nullptr;
+    float* rtw = nullptr;
+    int* dense = nullptr;
+    double* W = nullptr;
+    int nfree = -1;
+    std::vector<float> vfix;
+    double rho = 0.0;
     std::vector<void*> owned;
 };
 
@@ -627,6 +871,8 @@ struct mgsb_solver {
     double time_ms = 0.0;
     int64_t time_launches = 0;
     mgsb_step_stats last{};
+    VState* vstate = nullptr;
+    double* vpart = nullptr;  // partials of the level-0 residual norm
     std::vector<void*> owned;
 
     ~mgsb_solver() {
@@ -730,6 +976,16 @@ int pull_colliders(mgsb_solver* s) {
     return MGSB_OK;
 }
 
+int push_vstate(mgsb_solver* s) {
+    if (!s->vstate) return MGSB_OK;
+    VState v{};
+    v.check = s->p.check_every != 0 ? 1 : 0;
+    v.tol = s->p.tol;
+    HIPCHK(hipMemcpyAsync(s->vstate, &v, sizeof(VState), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return MGSB_OK;
+}
+
 int block_tet(const mgsb_solver* s) { return s->p.block_tet > 0 ? s->p.block_tet : 256; }
 int block_vert(const mgsb_solver* s) { return s->p.block_vert > 0 ? s->p.block_vert : 256; }
 
@@ -752,30 +1008,47 @@ int timed_end(mgsb_solver* s, int kernel) {
     return MGSB_OK;
 }
 
-int launch_tet(mgsb_solver* s, DevLevel& d, int mode, bool with_tri, const float4* xsrc = nullptr) {
+int launch_tet(mgsb_solver* s, DevLevel& d, int mode, bool with_tri, const float4* xsrc = nullptr,
+               const int* skip = nullptr) {
     const int bs = block_tet(s);
     const int ntri = with_tri ? d.S : 0;
     const int n = d.T + ntri;
     const float4* xx = xsrc ? xsrc : d.x;
-    int rc = timed_begin(s, 0);
+    // timing covers the finest level's local step only (kernel 0)
+    const int kid = (&d == &s->L[0] && mode != kTetModeLinear) ? 0 : 1;
+    int rc = timed_begin(s, kid);
     if (rc) return rc;
-    if (mode == kTetModePolar)
-        k_tet<kTetModePolar><<<nblk(n, bs), bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf,
-                                                                   s->p.stiffness, ntri, d.tri, d.tslot, d.tcf, s->dcol);
-    else if (mode == kTetModeAc)
-        k_tet<kTetModeAc><<<nblk(n, bs), bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf,
-                                                                s->p.stiffness, ntri, d.tri, d.tslot, d.tcf, s->dcol);
-    else
-        k_tet<kTetModeLinear><<<nblk(n, bs), bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf,
-                                                                    s->p.stiffness, ntri, d.tri, d.tslot, d.tcf, s->dcol);
+    const int nb = nblk(n, bs);
+    switch (mode) {
+        case kTetModePolar:
+            k_tet<kTetModePolar><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness,
+                                                           ntri, d.tri, d.tslot, d.tcf, s->dcol, skip);
+            break;
+        case kTetModeAc:
+            k_tet<kTetModeAc><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness, ntri,
+                                                        d.tri, d.tslot, d.tcf, s->dcol, skip);
+            break;
+        case kTetModeStable:
+            k_tet<kTetModeStable><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness,
+                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip);
+            break;
+        default:
+            k_tet<kTetModeLinear><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness,
+                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip);
+    }
     HIPCHK(hipGetLastError());
-    return timed_end(s, 0);
+    return timed_end(s, kid);
 }
 
-int launch_vert(mgsb_solver* s, DevLevel& d, float omega, int flags) {
+// MGSB_IF_* -> per-tet kernel mode
+int tet_mode_of(int if_mode) {
+    return if_mode == MGSB_IF_AC ? kTetModeAc : (if_mode == MGSB_IF_STABLE ? kTetModeStable : kTetModePolar);
+}
+
+int launch_vert(mgsb_solver* s, DevLevel& d, float omega, int flags, const int* skip = nullptr) {
     const int bs = block_vert(s);
     k_vert<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.x, d.xprev, d.s, d.vc, d.rp, d.cf, d.trp, d.tcf, s->dcol, omega,
-                                                 s->p.relax, flags, d.cflag, d.cdepth, d.cenergy);
+                                                 s->p.relax, flags, d.cflag, d.cdepth, d.cenergy, skip);
     HIPCHK(hipGetLastError());
     return MGSB_OK;
 }
@@ -796,8 +1069,8 @@ int launch_vel(mgsb_solver* s, DevLevel& d) {
 }
 
 // One reference PD iteration on a level (runClearTemp, DCD, IF, omega, POS).
-int pd_iteration(mgsb_solver* s, DevLevel& d, int mode, float omega, bool tri_dcd, bool last) {
-    int rc = launch_tet(s, d, mode, tri_dcd);
+int pd_iteration(mgsb_solver* s, DevLevel& d, int if_mode, float omega, bool tri_dcd, bool last) {
+    int rc = launch_tet(s, d, tet_mode_of(if_mode), tri_dcd);
     if (rc) return rc;
     int flags = tri_dcd ? kVertTri : kVertPoint;
     if (last) flags |= kVertWriteCol;
@@ -849,12 +1122,187 @@ int enqueue_step_twolevel(mgsb_solver* s) {
     return launch_vel(s, f);
 }
 
+// --- V-cycle -------------------------------------------------------------------
+float cheb_next(float omega, float rho) { return 4 / (4 - rho * rho * omega); }
+
+int lin_smooth(mgsb_solver* s, int l, int sweeps, bool zero, const int* skip) {
+    DevLevel& d = s->L[l];
+    const int bs = block_vert(s);
+    const float rho = (float)d.rho;
+    float omega = 1.0f;
+    for (int k = 0; k < sweeps; k++) {
+        omega = cheb_next(omega, rho);
+        const bool fz = zero && k == 0;
+        if (!fz) {
+            int rc = launch_tet(s, d, kTetModeLinear, false, d.e, skip);
+            if (rc) return rc;
+        }
+        k_lin_smooth<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.e, d.ep, d.r, d.vcl, d.rp, d.cf, omega,
+                                                          s->p.relax_coarse, fz ? 1 : 0, skip);
+        HIPCHK(hipGetLastError());
+    }
+    return MGSB_OK;
+}
+
+// Linear V-cycle on A_l e_l = r_l (l >= 1).
+int vcycle_lin(mgsb_solver* s, int l, const int* skip) {
+    DevLevel& d = s->L[l];
+    const int nl = (int)s->L.size();
+    const int bs = block_vert(s);
+    int rc;
+    if (l == nl - 1) {
+        if (s->p.coarse_direct && d.nfree >= 0) {
+            k_coarse_solve<<<1, 256, 0, s->stream>>>(d.V, d.nfree, d.dense, d.W, d.r, d.e, skip);
+            HIPCHK(hipGetLastError());
+            return MGSB_OK;
+        }
+        return lin_smooth(s, l, std::max(1, s->p.coarse_sweeps), true, skip);
+    }
+    if ((rc = lin_smooth(s, l, s->p.coarse_sweeps, true, skip))) return rc;
+    if (s->p.coarse_sweeps == 0) {
+        k_zero4<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.e, skip);
+        HIPCHK(hipGetLastError());
+    }
+    if ((rc = launch_tet(s, d, kTetModeLinear, false, d.e, skip))) return rc;
+    k_lin_resid<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.e, d.r, d.vcl, d.rp, d.cf, d.rr, skip);
+    HIPCHK(hipGetLastError());
+    DevLevel& c = s->L[l + 1];
+    k_restrict<<<nblk(c.V, bs), bs, 0, s->stream>>>(c.V, c.rtp, c.rti, c.rtw, d.rr, c.vcl, c.r, skip);
+    HIPCHK(hipGetLastError());
+    if ((rc = vcycle_lin(s, l + 1, skip))) return rc;
+    LevelMap& m = s->M[l];
+    k_prolong_add<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.e, d.ep, c.e, m.iids, m.iw, d.vcl, skip);
+    HIPCHK(hipGetLastError());
+    return lin_smooth(s, l, s->p.coarse_sweeps, false, skip);
+}
+
+int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, bool norm, bool restart, const int* skip) {
+    DevLevel& d = s->L[0];
+    int rc;
+    if (need_tet && (rc = launch_tet(s, d, tet_mode_of(s->p.if_mode), false, nullptr, skip))) return rc;
+    const int nb = nblk(d.V, 256);
+    k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, write_r ? d.r : nullptr,
+                                       norm ? s->vpart : nullptr, restart ? d.xprev : nullptr, skip);
+    HIPCHK(hipGetLastError());
+    return MGSB_OK;
+}
+
+// Residual-correction V-cycle step (see include/mgsb.h, MGSB_SCHED_VCYCLE).
+int enqueue_step_vcycle(mgsb_solver* s) {
+    DevLevel& f = s->L[0];
+    const int* skip = &s->vstate->converged;
+    const int nbv = nblk(f.V, 256);
+    const int bs = block_vert(s);
+    int rc;
+    if ((rc = launch_st(s, f, 1))) return rc;
+    if ((rc = fine_residual(s, true, false, true, true, nullptr))) return rc;
+    k_check<<<1, 256, 0, s->stream>>>(nbv, s->vpart, s->vstate, 0);
+    HIPCHK(hipGetLastError());
+    const float rs = s->p.rho_smooth;
+    for (int c = 0; c < s->p.max_cycles; c++) {
+        float omega = 1.0f;
+        for (int k = 0; k < s->p.pre_sweeps; k++) {
+            omega = cheb_next(omega, rs);
+            if (k > 0 && (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip))) return rc;
+            if ((rc = launch_vert(s, f, omega, kVertPoint, skip))) return rc;
+        }
+        if ((rc = fine_residual(s, s->p.pre_sweeps > 0, true, false, false, skip))) return rc;
+        DevLevel& c1 = s->L[1];
+        k_restrict<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, c1.rtp, c1.rti, c1.rtw, f.r, c1.vcl, c1.r, skip);
+        HIPCHK(hipGetLastError());
+        if ((rc = vcycle_lin(s, 1, skip))) return rc;
+        k_prolong_add<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, c1.e, s->M[0].iids, s->M[0].iw, f.vc,
+                                                           skip);
+        HIPCHK(hipGetLastError());
+        omega = 1.0f;
+        for (int k = 0; k < s->p.post_sweeps; k++) {
+            omega = cheb_next(omega, rs);
+            if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip))) return rc;
+            if ((rc = launch_vert(s, f, omega, kVertPoint | (k == s->p.post_sweeps - 1 ? kVertWriteCol : 0), skip)))
+                return rc;
+        }
+        if ((rc = fine_residual(s, true, false, true, true, skip))) return rc;
+        k_check<<<1, 256, 0, s->stream>>>(nbv, s->vpart, s->vstate, 1);
+        HIPCHK(hipGetLastError());
+    }
+    k_step_done<<<1, 64, 0, s->stream>>>(s->vstate);
+    HIPCHK(hipGetLastError());
+    return launch_vel(s, f);
+}
+
 int enqueue_step(mgsb_solver* s) {
     switch (s->p.schedule) {
         case MGSB_SCHED_SINGLE: return enqueue_step_single(s);
         case MGSB_SCHED_TWOLEVEL: return enqueue_step_twolevel(s);
+        case MGSB_SCHED_VCYCLE: return enqueue_step_vcycle(s);
         default: set_error("schedule not available"); return MGSB_ERR_ARG;
     }
+}
+
+// Coarse Dirichlet masks, vcl, rho and the coarsest factor from the current
+// level-0 fixflags (V-cycle set-up; rerun when fixflags or dt change).
+int setup_vcycle(mgsb_solver* s) {
+    const int nl = (int)s->L.size();
+    if (nl < 2) return MGSB_OK;
+    s->L[0].vfix = s->L[0].h.fix;
+    const float dt = s->p.dt;
+    for (int l = 1; l < nl; l++) {
+        DevLevel& c = s->L[l];
+        derive_vfix(s->L[l - 1].vfix, s->M[l - 1].h_aids, s->M[l - 1].h_aw, c.V, c.vfix);
+        std::vector<float4> vcl(c.V);
+        for (int i = 0; i < c.V; i++)
+            vcl[i] = make_float4(c.h.mass[i] / (dt * dt), c.h.diag[i], c.vfix[i], c.h.mass[i]);
+        HIPCHK(hipMemcpyAsync(c.vcl, vcl.data(), sizeof(float4) * c.V, hipMemcpyHostToDevice, s->stream));
+        c.rho = s->p.rho_coarse > 0 ? s->p.rho_coarse
+                                     : power_rho(c.h, c.vfix, dt, s->p.relax_coarse, s->p.stiffness);
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    DevLevel& cc = s->L[nl - 1];
+    std::vector<int32_t> dense;
+    std::vector<double> W;
+    int n = 0;
+    cc.nfree = -1;
+    if (cc.W) {
+        HIPCHK(hipFree(cc.W));
+        cc.W = nullptr;
+    }
+    int nfree_est = 0;
+    for (float v : cc.vfix) nfree_est += v != 0.f;
+    if (nfree_est <= kMaxDense && coarse_cholesky(cc.h, cc.vfix, dt, s->p.stiffness, dense, W, n) == 0 && n > 0) {
+        int rc = dalloc(&cc.W, W.size());
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(cc.W, W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(cc.dense, dense.data(), sizeof(int) * dense.size(), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        cc.nfree = n;
+    }
+    return MGSB_OK;
+}
+
+int alloc_vcycle(mgsb_solver* s) {
+    const int nl = (int)s->L.size();
+    int rc;
+    for (int l = 0; l < nl; l++) {
+        DevLevel& d = s->L[l];
+        if ((rc = up<float4>(s, d.owned, &d.r, nullptr, d.V))) return rc;
+        if (l == 0) continue;
+        if ((rc = up<float4>(s, d.owned, &d.e, nullptr, d.V))) return rc;
+        if ((rc = up<float4>(s, d.owned, &d.ep, nullptr, d.V))) return rc;
+        if ((rc = up<float4>(s, d.owned, &d.rr, nullptr, d.V))) return rc;
+        if ((rc = up<float4>(s, d.owned, &d.vcl, nullptr, d.V))) return rc;
+        if ((rc = up<int>(s, d.owned, &d.dense, nullptr, d.V))) return rc;
+        std::vector<int32_t> rtp, rti;
+        std::vector<float> rtw;
+        build_restriction(s->L[l - 1].V, d.V, s->M[l - 1].h_iids, s->M[l - 1].h_iw, rtp, rti, rtw);
+        if ((rc = up(s, d.owned, &d.rtp, rtp.data(), rtp.size()))) return rc;
+        if ((rc = up(s, d.owned, &d.rti, rti.data(), std::max<size_t>(1, rti.size())))) return rc;
+        if ((rc = up(s, d.owned, &d.rtw, rtw.data(), std::max<size_t>(1, rtw.size())))) return rc;
+    }
+    if ((rc = dalloc(&s->vstate, 1))) return rc;
+    s->owned.push_back(s->vstate);
+    if ((rc = dalloc(&s->vpart, (size_t)nblk(s->L[0].V, 256)))) return rc;
+    s->owned.push_back(s->vpart);
+    return push_vstate(s);
 }
 
 int ensure_partials(mgsb_solver* s, size_t n) {
@@ -881,11 +1329,12 @@ int reduce_finish(mgsb_solver* s, int nb, int nv, double* host_out) {
 // ||g(x)|| of level 0 (FP64).
 int residual_norm(mgsb_solver* s, double* out) {
     DevLevel& d = s->L[0];
-    int rc = launch_tet(s, d, kTetModePolar, false);
+    int rc = launch_tet(s, d, tet_mode_of(s->p.if_mode), false);
     if (rc) return rc;
     const int nb = nblk(d.V, 256);
     if ((rc = ensure_partials(s, (size_t)nb * 4))) return rc;
-    k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, nullptr, s->dpart);
+    k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, nullptr, s->dpart,
+                                       nullptr, nullptr);
     HIPCHK(hipGetLastError());
     double sq = 0;
     if ((rc = reduce_finish(s, nb, 1, &sq))) return rc;
@@ -930,15 +1379,23 @@ int validate_params(const mgsb_params* p, int nlevels) {
         set_error("invalid params: dt must be > 0, iteration counts >= 0");
         return MGSB_ERR_ARG;
     }
-    if (p->schedule == MGSB_SCHED_TWOLEVEL && nlevels < 2) {
-        set_error("two-level schedule needs 2 levels");
+    if (p->schedule == MGSB_SCHED_TWOLEVEL && nlevels != 2) {
+        set_error("two-level schedule needs exactly 2 levels");
+        return MGSB_ERR_ARG;
+    }
+    if ((p->schedule == MGSB_SCHED_VCYCLE || p->schedule == MGSB_SCHED_LEARNED) && nlevels < 2) {
+        set_error("V-cycle schedules need at least 2 levels");
+        return MGSB_ERR_ARG;
+    }
+    if (p->pre_sweeps < 0 || p->post_sweeps < 0 || p->coarse_sweeps < 0 || p->max_cycles < 0) {
+        set_error("sweep and cycle counts must be >= 0");
         return MGSB_ERR_ARG;
     }
     if (p->schedule < 0 || p->schedule > MGSB_SCHED_LEARNED) {
         set_error("unknown schedule");
         return MGSB_ERR_ARG;
     }
-    if (p->if_mode != MGSB_IF_PLAIN && p->if_mode != MGSB_IF_AC) {
+    if (p->if_mode != MGSB_IF_PLAIN && p->if_mode != MGSB_IF_AC && p->if_mode != MGSB_IF_STABLE) {
         set_error("unknown if_mode");
         return MGSB_ERR_ARG;
     }
@@ -1031,6 +1488,10 @@ int mgsb_create(const mgsb_mesh* levels, int32_t nlevels, const mgsb_params* p, 
         if ((rc = push_colliders(s.get()))) return rc;
         if ((rc = dalloc(&s->dred, 16))) return rc;
         s->owned.push_back(s->dred);
+        if (nlevels >= 2) {
+            if ((rc = alloc_vcycle(s.get()))) return rc;
+            if ((rc = setup_vcycle(s.get()))) return rc;
+        }
         HIPCHK(hipStreamSynchronize(s->stream));
         *out = s.release();
         return MGSB_OK;
@@ -1068,6 +1529,10 @@ int mgsb_set_params(mgsb_solver* s, const mgsb_params* p) {
     if (dt_changed)
         for (auto& d : s->L)
             if ((rc = upload_vc(s, d))) return rc;
+    if (s->L.size() >= 2) {
+        if ((rc = setup_vcycle(s))) return rc;
+        if ((rc = push_vstate(s))) return rc;
+    }
     return invalidate_graph(s);
 }
 
@@ -1081,7 +1546,9 @@ int mgsb_set_fixed(mgsb_solver* s, int32_t level, const float* fixflag) {
     HIPCHK(hipSetDevice(s->device));
     DevLevel& d = s->L[level];
     for (int i = 0; i < d.V; i++) d.h.fix[i] = fixflag[i];
-    return upload_vc(s, d);
+    int rc2 = upload_vc(s, d);
+    if (rc2) return rc2;
+    return s->L.size() >= 2 ? setup_vcycle(s) : MGSB_OK;
 }
 
 int mgsb_fix_spheres(mgsb_solver* s, const float* spheres, int32_t n) {
@@ -1105,7 +1572,7 @@ int mgsb_fix_spheres(mgsb_solver* s, const float* spheres, int32_t n) {
         int rc = upload_vc(s, d);
         if (rc) return rc;
     }
-    return MGSB_OK;
+    return s->L.size() >= 2 ? setup_vcycle(s) : MGSB_OK;
 }
 
 int mgsb_fix_plane(mgsb_solver* s, int32_t axis, float value, float tol) {
@@ -1120,7 +1587,7 @@ int mgsb_fix_plane(mgsb_solver* s, int32_t axis, float value, float tol) {
         int rc = upload_vc(s, d);
         if (rc) return rc;
     }
-    return MGSB_OK;
+    return s->L.size() >= 2 ? setup_vcycle(s) : MGSB_OK;
 }
 
 int mgsb_set_sphere(mgsb_solver* s, int32_t id, const float center[3], float radius, const float vel[3],
@@ -1179,8 +1646,7 @@ int mgsb_step(mgsb_solver* s, int32_t nsteps, mgsb_step_stats* stats) {
     }
     HIPCHK(hipSetDevice(s->device));
     int rc;
-    const bool graphable = s->p.use_graph && !s->timing &&
-                           (s->p.schedule == MGSB_SCHED_SINGLE || s->p.schedule == MGSB_SCHED_TWOLEVEL);
+    const bool graphable = s->p.use_graph && !s->timing;
     for (int st = 0; st < nsteps; st++) {
         if (graphable) {
             if (!s->graph_valid) {
@@ -1203,6 +1669,15 @@ int mgsb_step(mgsb_solver* s, int32_t nsteps, mgsb_step_stats* stats) {
         }
     }
     s->last.steps = nsteps;
+    if (s->vstate && (s->p.schedule == MGSB_SCHED_VCYCLE || s->p.schedule == MGSB_SCHED_LEARNED) && stats) {
+        VState v;
+        HIPCHK(hipMemcpyAsync(&v, s->vstate, sizeof(VState), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        s->last.res0 = v.res0;
+        s->last.res = v.res;
+        s->last.rel = v.res0 > 0 ? v.res / v.res0 : 0.0;
+        s->last.cycles = v.cycles;
+    }
     if (stats) *stats = s->last;
     HIPCHK(hipGetLastError());
     return MGSB_OK;
@@ -1338,6 +1813,52 @@ int mgsb_level_info(mgsb_solver* s, int32_t level, int32_t* nverts, int32_t* nte
     if (nverts) *nverts = s->L[level].V;
     if (ntets) *ntets = s->L[level].T;
     if (ntris) *ntris = s->L[level].S;
+    return MGSB_OK;
+}
+
+int mgsb_run_stats(mgsb_solver* s, mgsb_run_stats_t* out, int32_t reset) {
+    if (!s || !out) {
+        set_error("mgsb_run_stats: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    std::memset(out, 0, sizeof(*out));
+    if (!s->vstate) return MGSB_OK;
+    HIPCHK(hipSetDevice(s->device));
+    VState v;
+    HIPCHK(hipMemcpyAsync(&v, s->vstate, sizeof(VState), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    out->max_rel = v.max_rel;
+    out->total_cycles = v.total_cycles;
+    out->steps = v.nsteps;
+    out->failed_steps = v.nfail;
+    if (reset) {
+        v.max_rel = 0.0;
+        v.total_cycles = 0;
+        v.nsteps = 0;
+        v.nfail = 0;
+        HIPCHK(hipMemcpyAsync(s->vstate, &v, sizeof(VState), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    return MGSB_OK;
+}
+
+int mgsb_level_rho(mgsb_solver* s, int32_t level, double* rho) {
+    int rc = check_level(s, level);
+    if (rc) return rc;
+    if (!rho) {
+        set_error("mgsb_level_rho: null output");
+        return MGSB_ERR_ARG;
+    }
+    *rho = s->L[level].rho;
+    return MGSB_OK;
+}
+
+int mgsb_coarse_nfree(mgsb_solver* s, int32_t* nfree) {
+    if (!s || !nfree) {
+        set_error("mgsb_coarse_nfree: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    *nfree = s->L.size() >= 2 ? s->L.back().nfree : -1;
     return MGSB_OK;
 }
 

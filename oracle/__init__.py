@@ -39,6 +39,21 @@ class orc_params(C.Structure):
     ]
 
 
+class orc_vparams(C.Structure):
+    _fields_ = [
+        ("pre", C.c_int),
+        ("post", C.c_int),
+        ("coarse_sweeps", C.c_int),
+        ("max_cycles", C.c_int),
+        ("coarse_direct", C.c_int),
+        ("check", C.c_int),
+        ("tol", C.c_float),
+        ("rho_smooth", C.c_float),
+        ("relax_coarse", C.c_float),
+        ("rho_coarse", C.c_float),
+    ]
+
+
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
@@ -67,6 +82,14 @@ _SIGS = {
     "barycentric4": (None, [_fp, _fp, _fp]),
     "tet_corner_forces": (None, [_vp, C.c_int, _dp]),
     "collide": (None, [_vp, C.POINTER(orc_params), C.c_int]),
+    "ml_create": (_vp, [C.POINTER(_vp), C.c_int, C.c_float, C.c_float, C.c_float]),
+    "ml_free": (None, [_vp]),
+    "ml_rho": (C.c_double, [_vp, C.c_int]),
+    "ml_nfree": (C.c_int, [_vp]),
+    "ml_get_vfix": (None, [_vp, C.c_int, _fp]),
+    "ml_stats": (None, [_vp, _dp]),
+    "step_vcycle": (None, [_vp, C.POINTER(orc_params), C.POINTER(orc_vparams), C.c_int]),
+    "ml_coarse_solve": (None, [_vp, _fp, _fp]),
 }
 
 
@@ -235,3 +258,59 @@ def barycentric4(P, tet4x3, fp64=False):
     getattr(lib, ("orc64_" if fp64 else "orc32_") + "barycentric4")(P.ctypes.data_as(_fp), T.ctypes.data_as(_fp),
                                                                    w.ctypes.data_as(_fp))
     return w
+
+
+def vparams(pre=2, post=2, coarse_sweeps=2, max_cycles=8, coarse_direct=1, check=1, tol=1e-3, rho_smooth=0.0,
+            relax_coarse=0.6, rho_coarse=0.0):
+    vp = orc_vparams()
+    vp.pre, vp.post, vp.coarse_sweeps, vp.max_cycles = pre, post, coarse_sweeps, max_cycles
+    vp.coarse_direct, vp.check, vp.tol, vp.rho_smooth = coarse_direct, check, tol, rho_smooth
+    vp.relax_coarse, vp.rho_coarse = relax_coarse, rho_coarse
+    return vp
+
+
+class MultiLevel:
+    """Oracle of the residual-correction V-cycle over nested levels
+    (levels[0] finest; its fixflag must be set before construction)."""
+
+    def __init__(self, levels, relax_coarse=0.6, dt=1 / 60, rho_coarse=0.0):
+        self.levels = list(levels)
+        self.pfx = levels[0].pfx
+        arr = (_vp * len(levels))(*[lv._h for lv in levels])
+        self._h = getattr(lib, self.pfx + "ml_create")(arr, len(levels), relax_coarse, dt, rho_coarse)
+        if not self._h:
+            raise ValueError("ml_create failed")
+
+    def _f(self, name):
+        return getattr(lib, self.pfx + name)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._f("ml_free")(h)
+            self._h = None
+
+    def rho(self, level):
+        return self._f("ml_rho")(self._h, level)
+
+    def nfree(self):
+        return self._f("ml_nfree")(self._h)
+
+    def vfix(self, level):
+        out = np.zeros(self.levels[level].nv, np.float32)
+        self._f("ml_get_vfix")(self._h, level, out.ctypes.data_as(_fp))
+        return out
+
+    def step(self, p, vp, nsteps=1):
+        self._f("step_vcycle")(self._h, C.byref(p), C.byref(vp), nsteps)
+
+    def coarse_solve(self, r):
+        r = np.ascontiguousarray(r, np.float32).reshape(-1)
+        e = np.zeros_like(r)
+        self._f("ml_coarse_solve")(self._h, r.ctypes.data_as(_fp), e.ctypes.data_as(_fp))
+        return e.reshape(-1, 3)
+
+    def stats(self):
+        out = np.zeros(3, np.float64)
+        self._f("ml_stats")(self._h, out.ctypes.data_as(_dp))
+        return {"res0": out[0], "res": out[1], "cycles": int(out[2]), "rel": out[1] / out[0] if out[0] else 0.0}

@@ -59,6 +59,7 @@ typedef struct orc_params {
     int iters_coarse;     /* coarse iterations in the two-level schedule       */
     int if_mode;          /* 0: calculateIF (PDSolver.cu:128); 1: Ac path      */
                           /*    getTetFRV2 + calTetIFV1 (PDSolverAc.cu:447,630)*/
+                          /* 3: calculateIF with the deviatoric-invariant polar */
     int tri_dcd_coarse;   /* two-level schedule: triangle DCD on coarse level  */
     int nspheres;
     float sphere[ORC_MAX_SPHERES][8]; /* cx cy cz r active vx vy vz           */
@@ -134,7 +135,48 @@ static real polar(const real F[9], real R[9], int variant) {
     real III_c = det * det;
     real k = I_c2 - 3 * II_c;
     real inv_U[9];
-    if (k < (real)1e-6f) {
+    if (variant == 2) {
+        /* Same Franca formulas from deviatoric invariants: with C' = C - I_c/3 I,
+         * k = I_c^2 - 3 II_c = 1.5 tr(C'^2) and l = 13.5 det C', which keeps
+         * the anisotropic part of small strains that the O(1) difference
+         * loses in FP32; the isotropic branch is kept for k == 0 only. */
+        real m = I_c / 3;
+        real D0 = C[0] - m, D4 = C[4] - m, D8 = C[8] - m;
+        k = (real)1.5 * (D0 * D0 + D4 * D4 + D8 * D8 + 2 * (C[1] * C[1] + C[2] * C[2] + C[5] * C[5]));
+        if (k < (real)1e-30f) {
+            if (I_c < 1e-6) return det;
+            real inv_lambda = 1 / R_SQRT(m);
+            for (int i = 0; i < 9; i++) inv_U[i] = 0;
+            inv_U[0] = inv_U[4] = inv_U[8] = inv_lambda;
+        } else {
+            real l = (real)13.5 * (D0 * (D4 * D8 - C[5] * C[5]) - C[1] * (C[1] * D8 - C[5] * C[2]) +
+                                   C[2] * (C[1] * C[5] - D4 * C[2]));
+            real k_root = R_SQRT(k);
+            real value = l / (k * k_root);
+            if (value < -1.0) value = -1.0;
+            if (value > 1.0) value = 1.0;
+            real phi = R_ACOS(value);
+            real lambda2 = m + (real)2 / 3 * k_root * R_COS(phi / 3);
+            real lambda = R_SQRT(lambda2);
+            real III_u = det;
+            if (R_FABS(lambda) < 1e-6) return det;
+            if (-lambda2 + I_c + 2 * III_u / lambda < 1e-6) return det;
+            real I_u = lambda + R_SQRT(-lambda2 + I_c + 2 * III_u / lambda);
+            real II_u = (I_u * I_u - I_c) * (real)0.5;
+            if (R_FABS(I_u * II_u - III_u) < 1e-6) return det;
+            if (R_FABS(III_u) < 1e-6) return det;
+            real inv_rate = 1 / (I_u * II_u - III_u);
+            real fdiag = I_u * III_u * inv_rate;
+            real fc = (I_u * I_u - II_u) * inv_rate;
+            real U[9];
+            for (int i = 0; i < 9; i++) U[i] = fc * C[i] - inv_rate * C2[i];
+            U[0] += fdiag; U[4] += fdiag; U[8] += fdiag;
+            inv_rate = 1 / III_u;
+            real gdiag = II_u * inv_rate, gu = -I_u * inv_rate;
+            for (int i = 0; i < 9; i++) inv_U[i] = gu * U[i] + inv_rate * C[i];
+            inv_U[0] += gdiag; inv_U[4] += gdiag; inv_U[8] += gdiag;
+        }
+    } else if (k < (real)1e-6f) {
         if (I_c < 1e-6) return det;
         real inv_lambda = 1 / R_SQRT(I_c / 3);
         for (int i = 0; i < 9; i++) inv_U[i] = 0;
@@ -426,12 +468,12 @@ static void calc_v(orc_level* L, const orc_params* p) {
 /* Per-tet elastic contribution: calculateIF (PDSolver.cu:128-183, mode 0) or
  * getTetFRV2 + calTetIFV1 (PDSolverAc.cu:447-662, mode 1: float constants and
  * contributions below 1e-5 in magnitude dropped). */
-static void tet_forces(orc_level* L, int mode) {
+static void tet_forces_on(orc_level* L, const real* Xsrc, int mode) {
     real k = (real)L->stiffness;
 #pragma omp parallel for schedule(static)
     for (int t = 0; t < L->T; t++) {
         const int* id = &L->tets[t * 4];
-        const real* X = L->x;
+        const real* X = Xsrc;
         real D[9];
         D[0] = X[id[1] * 3 + 0] - X[id[0] * 3 + 0];
         D[1] = X[id[2] * 3 + 0] - X[id[0] * 3 + 0];
@@ -454,23 +496,31 @@ static void tet_forces(orc_level* L, int mode) {
         F[7] = D[6] * B[1] + D[7] * B[4] + D[8] * B[7];
         F[8] = D[6] * B[2] + D[7] * B[5] + D[8] * B[8];
         real R[9];
-        polar(F, R, mode == 1 ? 1 : 0);
-        for (int i = 0; i < 9; i++) R[i] = R[i] - F[i];
+        if (mode == 2) {  /* linear operator on a correction: R := 0, no clamp */
+            for (int i = 0; i < 9; i++) R[i] = -F[i];
+        } else {
+            polar(F, R, mode == 1 ? 1 : (mode == 3 ? 2 : 0));
+            for (int i = 0; i < 9; i++) R[i] = R[i] - F[i];
+        }
         const real* AC = &L->AC[t * 12];
         real vol = L->vol[t];
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 4; j++) {
                 real acc = 0;
                 for (int m = 0; m < 3; m++) acc += R[i * 3 + m] * AC[m * 4 + j];
-                if (isnan(acc)) acc = 0;
-                acc = acc > 10 ? 10 : acc;
-                acc = acc < -10 ? -10 : acc;
+                if (mode != 2) {
+                    if (isnan(acc)) acc = 0;
+                    acc = acc > 10 ? 10 : acc;
+                    acc = acc < -10 ? -10 : acc;
+                }
                 real r = acc * vol * k;
                 if (mode == 1 && R_FABS(r) < 1e-5) r = 0;
                 L->cf[t * 12 + j * 3 + i] = r;
             }
     }
 }
+
+static void tet_forces(orc_level* L, int mode) { tet_forces_on(L, L->x, mode); }
 
 /* Gather in increasing tet order == sequential scatter order. */
 static void gather_forces(orc_level* L) {
@@ -1049,4 +1099,393 @@ void PFX(collide)(orc_level* L, const orc_params* p, int tri) {
         dcd_triangle(L, p);
     else
         dcd_point(L, p);
+}
+
+/* ========================================================================= */
+/* Residual-correction V-cycle (this build's multigrid schedule; the         */
+/* reference's own two-level scheme is step_mg above).                      */
+/*                                                                           */
+/* Per step: inertial prediction; then up to max_cycles cycles of            */
+/*   pre  : `pre` nonlinear Jacobi/Chebyshev sweeps on level 0 (the          */
+/*          reference sweep: local step + calculatePOS),                     */
+/*   g    : residual g = m/h^2 (s - x) + f_el + f_col on free DOFs,          */
+/*   coarse correction: r_1 = P^T g; linear V-cycle on A_l e = r_l for       */
+/*          l >= 1 (A = m/h^2 + L, L the ARAP stiffness at R = 0), damped   */
+/*          Jacobi with Chebyshev acceleration (rho per level by power       */
+/*          iteration), FP64 Cholesky on the coarsest level;                 */
+/*   x += P e_1 (free DOFs), post-smoothing, residual norm (FP64).           */
+/* The cycle loop stops once ||g|| <= tol * ||g(s)|| when `check` is set.    */
+/* ========================================================================= */
+#define ORC_MAXL 8
+
+typedef struct orc_vparams {
+    int pre, post, coarse_sweeps, max_cycles, coarse_direct, check;
+    float tol, rho_smooth, relax_coarse, rho_coarse;
+} orc_vparams;
+
+typedef struct orc_ml {
+    int nl;
+    orc_level* L[ORC_MAXL];
+    int* pid[ORC_MAXL];  /* level l vertex -> 4 vertex ids of level l+1 */
+    real* pw[ORC_MAXL];
+    real* vfix[ORC_MAXL];
+    real *e[ORC_MAXL], *ep[ORC_MAXL], *r[ORC_MAXL], *q[ORC_MAXL];
+    double rho[ORC_MAXL];
+    int nfree;
+    int* dense;          /* coarsest vertex -> dense index or -1 */
+    double* chol;        /* lower Cholesky factor, nfree^2       */
+    double res0, res;
+    int cycles;
+    float relax_coarse, dt;
+} orc_ml;
+
+/* q = A e = m/h^2 e - f_lin(e) on level l (f_lin from the linear tet pass). */
+static void ml_apply(orc_ml* M, int l, const real* e, real* q) {
+    orc_level* L = M->L[l];
+    tet_forces_on(L, e, 2);
+    gather_forces(L);
+    real dt = M->dt;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < L->V; i++) {
+        real dc = L->mass[i] / (dt * dt);
+        for (int c = 0; c < 3; c++) q[i * 3 + c] = dc * e[i * 3 + c] - L->force[i * 3 + c];
+    }
+}
+
+/* Spectral radius of J = I - relax D^-1 A on free DOFs of level l, power
+ * iteration (100 steps, fixed start vector), FP64. */
+static double ml_power_rho(orc_ml* M, int l) {
+    orc_level* L = M->L[l];
+    int V = L->V;
+    double dt = M->dt, relax = M->relax_coarse;
+    real* w = (real*)xcalloc((size_t)V * 3, sizeof(real));
+    real* q = (real*)xcalloc((size_t)V * 3, sizeof(real));
+    double* wd = (double*)xcalloc((size_t)V, sizeof(double));
+    for (int i = 0; i < V; i++) wd[i] = M->vfix[l][i] * (1.0 + 0.5 * sin(1.7 * i));
+    double lam = 0;
+    for (int it = 0; it < 100; it++) {
+        double nw = 0;
+        for (int i = 0; i < V; i++) nw += wd[i] * wd[i];
+        nw = sqrt(nw);
+        if (nw == 0) break;
+        for (int i = 0; i < V; i++) {
+            wd[i] /= nw;
+            for (int c = 0; c < 3; c++) w[i * 3 + c] = (real)wd[i];
+        }
+        ml_apply(M, l, w, q);
+        double ny = 0;
+        for (int i = 0; i < V; i++) {
+            double d = (double)L->mass[i] / (dt * dt) + (double)L->diag[i];
+            double y = M->vfix[l][i] * (wd[i] - relax * (double)q[i * 3] / d);
+            wd[i] = y;
+            ny += y * y;
+        }
+        lam = sqrt(ny);
+    }
+    free(w); free(q); free(wd);
+    return lam;
+}
+
+/* Dense A on the free vertices of the coarsest level, FP64 Cholesky. */
+static int ml_factor_coarsest(orc_ml* M) {
+    int l = M->nl - 1;
+    orc_level* L = M->L[l];
+    int V = L->V, n = 0;
+    M->dense = (int*)xcalloc((size_t)V, sizeof(int));
+    for (int i = 0; i < V; i++) M->dense[i] = M->vfix[l][i] != 0 ? n++ : -1;
+    M->nfree = n;
+    double* A = (double*)xcalloc((size_t)n * n, sizeof(double));
+    double dt = M->dt, k = L->stiffness;
+    for (int i = 0; i < V; i++)
+        if (M->dense[i] >= 0) A[(size_t)M->dense[i] * n + M->dense[i]] += (double)L->mass[i] / (dt * dt);
+    for (int t = 0; t < L->T; t++) {
+        const real* AC = &L->AC[t * 12];
+        double w = (double)L->vol[t] * k;
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) {
+                int ia = M->dense[L->tets[t * 4 + a]], ib = M->dense[L->tets[t * 4 + b]];
+                if (ia < 0 || ib < 0) continue;
+                double s = 0;
+                for (int r = 0; r < 3; r++) s += (double)AC[r * 4 + a] * (double)AC[r * 4 + b];
+                A[(size_t)ia * n + ib] += w * s;
+            }
+    }
+    for (int j = 0; j < n; j++) {
+        double d = A[(size_t)j * n + j];
+        for (int p = 0; p < j; p++) d -= A[(size_t)j * n + p] * A[(size_t)j * n + p];
+        if (d <= 0) { free(A); return -1; }
+        d = sqrt(d);
+        A[(size_t)j * n + j] = d;
+        for (int i = j + 1; i < n; i++) {
+            double s = A[(size_t)i * n + j];
+            for (int p = 0; p < j; p++) s -= A[(size_t)i * n + p] * A[(size_t)j * n + p];
+            A[(size_t)i * n + j] = s / d;
+        }
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = i + 1; j < n; j++) A[(size_t)i * n + j] = 0;
+    M->chol = A;
+    return 0;
+}
+
+static void ml_solve_coarsest(orc_ml* M, const real* r, real* e) {
+    int l = M->nl - 1, n = M->nfree, V = M->L[l]->V;
+    double* y = (double*)xcalloc((size_t)n * 3, sizeof(double));
+    for (int i = 0; i < V; i++)
+        if (M->dense[i] >= 0)
+            for (int c = 0; c < 3; c++) y[(size_t)M->dense[i] * 3 + c] = r[i * 3 + c];
+    const double* Lc = M->chol;
+    for (int c = 0; c < 3; c++) {
+        for (int i = 0; i < n; i++) {
+            double s = y[(size_t)i * 3 + c];
+            for (int p = 0; p < i; p++) s -= Lc[(size_t)i * n + p] * y[(size_t)p * 3 + c];
+            y[(size_t)i * 3 + c] = s / Lc[(size_t)i * n + i];
+        }
+        for (int i = n - 1; i >= 0; i--) {
+            double s = y[(size_t)i * 3 + c];
+            for (int p = i + 1; p < n; p++) s -= Lc[(size_t)p * n + i] * y[(size_t)p * 3 + c];
+            y[(size_t)i * 3 + c] = s / Lc[(size_t)i * n + i];
+        }
+    }
+    for (int i = 0; i < V; i++)
+        for (int c = 0; c < 3; c++) e[i * 3 + c] = M->dense[i] >= 0 ? (real)y[(size_t)M->dense[i] * 3 + c] : 0;
+    free(y);
+}
+
+/* r_{l+1} = P_l^T r_l, accumulated over fine vertices in ascending order. */
+static void ml_restrict(orc_ml* M, int l, const real* rf, real* rc) {
+    orc_level* C = M->L[l + 1];
+    memset(rc, 0, sizeof(real) * 3 * (size_t)C->V);
+    for (int i = 0; i < M->L[l]->V; i++)
+        for (int q = 0; q < 4; q++) {
+            int j = M->pid[l][i * 4 + q];
+            real w = M->pw[l][i * 4 + q];
+            for (int c = 0; c < 3; c++) rc[j * 3 + c] += w * rf[i * 3 + c];
+        }
+    for (int j = 0; j < C->V; j++)
+        for (int c = 0; c < 3; c++) rc[j * 3 + c] *= M->vfix[l + 1][j];
+}
+
+/* dst (level l) += mask * P_l src (level l+1). */
+static void ml_prolong_add(orc_ml* M, int l, const real* src, real* dst, const real* mask) {
+    orc_level* F = M->L[l];
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < F->V; i++) {
+        const int* id = M->pid[l] + i * 4;
+        const real* w = M->pw[l] + i * 4;
+        for (int c = 0; c < 3; c++) {
+            real v = w[0] * src[id[0] * 3 + c] + w[1] * src[id[1] * 3 + c] + w[2] * src[id[2] * 3 + c] +
+                     w[3] * src[id[3] * 3 + c];
+            dst[i * 3 + c] = dst[i * 3 + c] + mask[i] * v;
+        }
+    }
+}
+
+/* Damped Jacobi + Chebyshev sweeps on A_l e = r_l; `zero` = e starts at 0. */
+static void ml_smooth(orc_ml* M, int l, int sweeps, int zero) {
+    orc_level* L = M->L[l];
+    real *e = M->e[l], *ep = M->ep[l], *r = M->r[l], *q = M->q[l];
+    real relax = M->relax_coarse, dt = M->dt;
+    real rho = (real)M->rho[l];
+    real omega = 1;
+    for (int k = 0; k < sweeps; k++) {
+        omega = 4 / (4 - rho * rho * omega);
+        int first_zero = zero && k == 0;
+        if (!first_zero) ml_apply(M, l, e, q);
+        for (int i = 0; i < L->V; i++) {
+            real d = L->mass[i] / (dt * dt) + L->diag[i];
+            real fx = M->vfix[l][i];
+            for (int c = 0; c < 3; c++) {
+                int j = i * 3 + c;
+                real ec = first_zero ? 0 : e[j];
+                real qc = first_zero ? 0 : q[j];
+                real epc = first_zero ? 0 : ep[j];
+                real y = ec + fx * (relax * ((r[j] - qc) / d));
+                y = omega * (y - epc) + epc;
+                ep[j] = ec;
+                e[j] = y;
+            }
+        }
+    }
+}
+
+static void ml_vcycle_lin(orc_ml* M, int l, const orc_vparams* vp) {
+    orc_level* L = M->L[l];
+    size_t n3 = (size_t)L->V * 3;
+    if (l == M->nl - 1) {
+        if (vp->coarse_direct && M->chol) {
+            ml_solve_coarsest(M, M->r[l], M->e[l]);
+        } else {
+            ml_smooth(M, l, vp->coarse_sweeps, 1);
+        }
+        return;
+    }
+    ml_smooth(M, l, vp->coarse_sweeps, 1);
+    ml_apply(M, l, M->e[l], M->q[l]);
+    real* rr = (real*)xcalloc(n3, sizeof(real));
+    for (int i = 0; i < L->V; i++)
+        for (int c = 0; c < 3; c++) rr[i * 3 + c] = M->vfix[l][i] * (M->r[l][i * 3 + c] - M->q[l][i * 3 + c]);
+    ml_restrict(M, l, rr, M->r[l + 1]);
+    free(rr);
+    ml_vcycle_lin(M, l + 1, vp);
+    ml_prolong_add(M, l, M->e[l + 1], M->e[l], M->vfix[l]);
+    memcpy(M->ep[l], M->e[l], sizeof(real) * n3);
+    /* post-smoothing from the corrected e (not zero) */
+    {
+        orc_level* Lq = L;
+        real *e = M->e[l], *ep = M->ep[l], *r = M->r[l], *q = M->q[l];
+        real relax = M->relax_coarse, dt = M->dt, rho = (real)M->rho[l], omega = 1;
+        for (int k = 0; k < vp->coarse_sweeps; k++) {
+            omega = 4 / (4 - rho * rho * omega);
+            ml_apply(M, l, e, q);
+            for (int i = 0; i < Lq->V; i++) {
+                real d = Lq->mass[i] / (dt * dt) + Lq->diag[i];
+                real fx = M->vfix[l][i];
+                for (int c = 0; c < 3; c++) {
+                    int j = i * 3 + c;
+                    real y = e[j] + fx * (relax * ((r[j] - q[j]) / d));
+                    y = omega * (y - ep[j]) + ep[j];
+                    ep[j] = e[j];
+                    e[j] = y;
+                }
+            }
+        }
+    }
+}
+
+/* g(x) on level 0 into r[0] (free DOFs), returns ||g|| (FP64); also restarts
+ * the fine Chebyshev history (x_prev = x). */
+static double ml_fine_residual(orc_ml* M, const orc_params* p) {
+    orc_level* L = M->L[0];
+    dcd_point(L, p);
+    tet_forces(L, p->if_mode);
+    gather_forces(L);
+    double acc = 0;
+    real dt = p->dt;
+    for (int i = 0; i < L->V; i++) {
+        real dc = L->mass[i] / (dt * dt);
+        real fx = L->fix[i];
+        for (int c = 0; c < 3; c++) {
+            int q = i * 3 + c;
+            real g = (dc * (L->s[q] - L->x[q]) + (L->force[q] + L->cforce[q])) * fx;
+            M->r[0][q] = g;
+            acc += (double)g * (double)g;
+            L->xprev[q] = L->x[q];
+        }
+    }
+    return sqrt(acc);
+}
+
+orc_ml* PFX(ml_create)(orc_level** levels, int nl, float relax_coarse, float dt, float rho_coarse) {
+    if (nl < 2 || nl > ORC_MAXL) return NULL;
+    orc_ml* M = (orc_ml*)calloc(1, sizeof(orc_ml));
+    M->nl = nl;
+    M->relax_coarse = relax_coarse;
+    M->dt = dt;
+    for (int l = 0; l < nl; l++) {
+        M->L[l] = levels[l];
+        size_t n3 = (size_t)levels[l]->V * 3;
+        M->e[l] = (real*)xcalloc(n3, sizeof(real));
+        M->ep[l] = (real*)xcalloc(n3, sizeof(real));
+        M->r[l] = (real*)xcalloc(n3, sizeof(real));
+        M->q[l] = (real*)xcalloc(n3, sizeof(real));
+        M->vfix[l] = (real*)xcalloc(levels[l]->V, sizeof(real));
+    }
+    for (int i = 0; i < levels[0]->V; i++) M->vfix[0][i] = levels[0]->fix[i];
+    for (int l = 0; l + 1 < nl; l++) {
+        orc_mg tmp;
+        memset(&tmp, 0, sizeof(tmp));
+        tmp.coarse = levels[l + 1];
+        tmp.fine = levels[l];
+        tmp.iids = (int*)xcalloc((size_t)levels[l]->V * 4, sizeof(int));
+        tmp.iw = (real*)xcalloc((size_t)levels[l]->V * 4, sizeof(real));
+        tmp.aids = (int*)xcalloc((size_t)levels[l + 1]->V * 4, sizeof(int));
+        tmp.aw = (real*)xcalloc((size_t)levels[l + 1]->V * 4, sizeof(real));
+        mg_interpolate_map(&tmp);
+        mg_average_map(&tmp);
+        M->pid[l] = tmp.iids;
+        M->pw[l] = tmp.iw;
+        /* coarse vertex pinned iff its largest-weight fine corner is pinned */
+        for (int j = 0; j < levels[l + 1]->V; j++) {
+            int best = 0;
+            for (int q = 1; q < 4; q++)
+                if (tmp.aw[j * 4 + q] > tmp.aw[j * 4 + best]) best = q;
+            M->vfix[l + 1][j] = M->vfix[l][tmp.aids[j * 4 + best]] != 0 ? 1 : 0;
+        }
+        free(tmp.aids);
+        free(tmp.aw);
+    }
+    for (int l = 1; l < nl; l++) M->rho[l] = rho_coarse > 0 ? rho_coarse : ml_power_rho(M, l);
+    if (ml_factor_coarsest(M)) {
+        free(M->chol);
+        M->chol = NULL;
+    }
+    return M;
+}
+
+void PFX(ml_free)(orc_ml* M) {
+    if (!M) return;
+    for (int l = 0; l < M->nl; l++) {
+        free(M->e[l]); free(M->ep[l]); free(M->r[l]); free(M->q[l]); free(M->vfix[l]);
+        free(M->pid[l]); free(M->pw[l]);
+    }
+    free(M->dense);
+    free(M->chol);
+    free(M);
+}
+
+double PFX(ml_rho)(const orc_ml* M, int l) { return M->rho[l]; }
+int PFX(ml_nfree)(const orc_ml* M) { return M->chol ? M->nfree : -1; }
+void PFX(ml_get_vfix)(const orc_ml* M, int l, float* out) {
+    for (int i = 0; i < M->L[l]->V; i++) out[i] = (float)M->vfix[l][i];
+}
+void PFX(ml_stats)(const orc_ml* M, double* out) {
+    out[0] = M->res0;
+    out[1] = M->res;
+    out[2] = M->cycles;
+}
+
+void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nsteps) {
+    orc_level* L0 = M->L[0];
+    real rs = vp->rho_smooth;
+    for (int st = 0; st < nsteps; st++) {
+        move_spheres(p);
+        calc_st(L0, p);
+        M->res0 = ml_fine_residual(M, p);
+        M->res = M->res0;
+        M->cycles = 0;
+        for (int c = 0; c < vp->max_cycles; c++) {
+            if (vp->check && M->res <= (double)vp->tol * M->res0) break;
+            real omega = 1;
+            for (int k = 0; k < vp->pre; k++) {
+                omega = 4 / (4 - rs * rs * omega);
+                pd_iteration(L0, p, omega, 0);
+            }
+            ml_fine_residual(M, p);
+            ml_restrict(M, 0, M->r[0], M->r[1]);
+            ml_vcycle_lin(M, 1, vp);
+            ml_prolong_add(M, 0, M->e[1], L0->x, L0->fix);
+            memcpy(L0->xprev, L0->x, sizeof(real) * 3 * (size_t)L0->V);
+            omega = 1;
+            for (int k = 0; k < vp->post; k++) {
+                omega = 4 / (4 - rs * rs * omega);
+                pd_iteration(L0, p, omega, 0);
+            }
+            M->res = ml_fine_residual(M, p);
+            M->cycles++;
+        }
+        calc_v(L0, p);
+    }
+}
+
+/* Coarsest-level solve on a given right-hand side (test entry point). */
+void PFX(ml_coarse_solve)(orc_ml* M, const float* r, float* e) {
+    int l = M->nl - 1, V = M->L[l]->V;
+    real* rr = (real*)xcalloc((size_t)V * 3, sizeof(real));
+    real* ee = (real*)xcalloc((size_t)V * 3, sizeof(real));
+    for (int i = 0; i < V * 3; i++) rr[i] = r[i];
+    ml_solve_coarsest(M, rr, ee);
+    for (int i = 0; i < V * 3; i++) e[i] = (float)ee[i];
+    free(rr);
+    free(ee);
 }

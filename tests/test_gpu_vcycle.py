@@ -1,0 +1,90 @@
+"""Device V-cycle (MGSB_SCHED_VCYCLE) against the oracle's restatement of the
+same schedule: trajectories with a fixed cycle count, adaptive cycle counts
+and the per-step residual target, per-level rho, graph replay."""
+import numpy as np
+import pytest
+
+import oracle as O
+import multigridsoftbody_amd as M
+from multigridsoftbody_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-4
+
+
+def rel_err(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def make_pair(ns, steps_kw, fp64=False, gravity=True, **vkw):
+    lv = W.beam_hierarchy(ns)
+    phys = W.physics(gravity=(0, -9.8, 0) if gravity else (0, 0, 0))
+    fix = W.top_fixflag(lv[0])
+    v0 = W.seeded_velocity(lv[0], fix)
+    s = M.Solver(lv, schedule=M.SCHED_VCYCLE, if_mode=M.IF_STABLE, **vkw, **phys)
+    s.fix_plane(1, 10.0, 1e-5)
+    s.set_velocity(v0)
+    Ls = [O.Level(m.pos, m.tets, m.tris, stiffness=phys["stiffness"], density=phys["density"], fp64=fp64)
+          for m in lv]
+    Ls[0].set_fix(fix)
+    Ls[0].set_velocity(v0)
+    ML = O.MultiLevel(Ls, relax_coarse=vkw.get("relax_coarse", 0.6), dt=phys["dt"])
+    p = O.params(dt=phys["dt"], stiffness=phys["stiffness"], density=phys["density"], gravity=phys["gravity"],
+                 if_mode=3)
+    return lv, s, ML, Ls, p
+
+
+@pytest.mark.parametrize("ns", [(4, 1), (8, 4, 1)])
+def test_vcycle_fixed_cycles_matches_oracle(ns):
+    kw = dict(pre_sweeps=2, post_sweeps=2, coarse_sweeps=2, max_cycles=3, check_every=0)
+    lv, s, ML, Ls, p = make_pair(ns, None, **kw)
+    for l in range(1, len(ns)):
+        assert abs(s.level_rho(l) - ML.rho(l)) < 1e-4 * ML.rho(l)
+    assert s.coarse_nfree() == ML.nfree()
+    vp = O.vparams(pre=2, post=2, coarse_sweeps=2, max_cycles=3, check=0)
+    s.step(60)
+    ML.step(p, vp, 60)
+    g, o = s.positions(), Ls[0].positions()
+    err = rel_err(g, o)
+    st = s.stats
+    print(f"[vcycle fixed {ns}] rel(gpu,o32)={err:.3e} gpu rel res={st.rel:.3e} oracle rel res={ML.stats()['rel']:.3e}")
+    assert st.cycles == 3
+    assert err < REL_TOL
+    assert abs(st.rel - ML.stats()["rel"]) < 0.05 * ML.stats()["rel"] + 1e-7
+
+
+def test_vcycle_adaptive_meets_target_each_step():
+    kw = dict(pre_sweeps=2, post_sweeps=2, coarse_sweeps=2, max_cycles=12, check_every=1, tol=1e-3)
+    lv, s, ML, Ls, p = make_pair((4, 1), None, **kw)
+    vp = O.vparams(pre=2, post=2, coarse_sweeps=2, max_cycles=12, check=1, tol=1e-3)
+    gc, oc = [], []
+    for _ in range(80):
+        st = s.step(1)
+        ML.step(p, vp, 1)
+        gc.append(st.cycles)
+        oc.append(ML.stats()["cycles"])
+        assert st.rel <= 1e-3, st.rel
+    agree = np.mean(np.array(gc) == np.array(oc))
+    print(f"[vcycle adaptive] cycles gpu mean {np.mean(gc):.2f} oracle mean {np.mean(oc):.2f} agree {agree:.2f}")
+    assert agree >= 0.9
+    assert rel_err(s.positions(), Ls[0].positions()) < REL_TOL
+
+
+def test_vcycle_graph_replay_matches_eager():
+    lv = W.beam_hierarchy((4, 1))
+    fix = W.top_fixflag(lv[0])
+    v0 = W.seeded_velocity(lv[0], fix)
+    res = []
+    for g in (0, 1):
+        s = M.Solver(lv, schedule=M.SCHED_VCYCLE, if_mode=M.IF_STABLE, max_cycles=8, check_every=1, tol=1e-3,
+                     use_graph=g, **W.physics())
+        s.fix_plane(1, 10.0, 1e-5)
+        s.set_velocity(v0)
+        for _ in range(4):
+            st = s.step(5)
+            s.sync()
+            res.append((s.positions(), st.cycles, st.rel))
+    n = len(res) // 2
+    for a, b in zip(res[:n], res[n:]):
+        assert np.array_equal(a[0], b[0]) and a[1] == b[1] and a[2] == b[2]
