@@ -26,8 +26,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "sim steps/sec at rel. residual ≤1e-3, learned vs plain MG; achieved HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-VCYCLE = dict(pre_sweeps=2, post_sweeps=2, coarse_sweeps=2, max_cycles=12, tol=1e-3, relax_coarse=0.6,
-              rho_smooth=0.0, coarse_direct=1)
+VCYCLE = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, tol=1e-3, relax_coarse=0.6,
+              rho_smooth=0.9, coarse_direct=1)
 
 
 def log(*a):
@@ -90,7 +90,7 @@ def cpu_baseline(levels, cfg, schedule, budget_s=15.0):
         p = O.params(dt=phys["dt"], stiffness=phys["stiffness"], density=phys["density"], gravity=phys["gravity"],
                      if_mode=3)
         vp = O.vparams(pre=VCYCLE["pre_sweeps"], post=VCYCLE["post_sweeps"], coarse_sweeps=VCYCLE["coarse_sweeps"],
-                       max_cycles=VCYCLE["max_cycles"], tol=VCYCLE["tol"], check=1)
+                       max_cycles=VCYCLE["max_cycles"], tol=VCYCLE["tol"], check=1, rho_smooth=VCYCLE["rho_smooth"])
         step = lambda: ML.step(p, vp, 1)  # noqa: E731
         what = "V-cycle schedule (same parameters, adaptive cycles to rel. residual 1e-3)"
     else:

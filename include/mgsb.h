@@ -111,6 +111,8 @@ typedef struct mgsb_run_stats_t {
     int64_t total_cycles;   /* V-cycles summed over the steps                  */
     int32_t steps;
     int32_t failed_steps;   /* steps whose final relative residual > tol       */
+    int32_t fallbacks;      /* learned schedule: warm starts rejected          */
+    int32_t reserved;
 } mgsb_run_stats_t;
 
 /* Energy diagnostics, runCalEnergy (PDSolver.cu:816-913): Ek, Ep, Ec, Nc, dX. */
@@ -223,6 +225,33 @@ int mgsb_residual(mgsb_solver* s, double* res);
 /* Run aggregates of the V-cycle schedules (no reference counterpart: the
  * reference has no residual); reset != 0 clears them after reading. */
 int mgsb_run_stats(mgsb_solver* s, mgsb_run_stats_t* out, int32_t reset);
+
+/* Learned coarse correction (MGSB_SCHED_LEARNED; no reference counterpart).
+ * A 4-layer MLP (12 -> 128 -> 128 -> 128 -> 3, ReLU, BF16 MFMA with FP32
+ * accumulation, FP32 master weights) maps per-vertex features of level 1
+ * (restricted residual of the inertial prediction, injected velocity, rest
+ * position, previous step's coarse correction) to that step's coarse
+ * correction; the prolongated prediction warm-starts the V-cycles, and is
+ * rejected (x reset to the inertial prediction) when it raises the residual.
+ *
+ * record: V-cycle / learned steps store (features, achieved correction) for
+ * up to max_steps steps (0 stops); train: Adam on the recorded samples with
+ * standardised inputs/outputs, loss_hist receives the mean loss per epoch. */
+int mgsb_learn_record(mgsb_solver* s, int32_t max_steps);
+int mgsb_learn_samples(mgsb_solver* s, int64_t* nsamples);
+int mgsb_learn_clear(mgsb_solver* s);
+int mgsb_learn_train(mgsb_solver* s, int32_t epochs, int32_t batch, float lr, uint32_t seed, float* loss_hist);
+int mgsb_learn_nparams(mgsb_solver* s, int32_t* n);
+/* Flat parameters W1[12][128] b1 W2[128][128] b2 W3[128][128] b3 W4[128][3]
+ * b4; set != 0 writes them (and resets Adam state). */
+int mgsb_learn_weights(mgsb_solver* s, float* w, int32_t n, int32_t set);
+/* Standardisation: in_mu[12] in_sd[12] out_mu[3] out_sd[3]. */
+int mgsb_learn_norm(mgsb_solver* s, float* norm, int32_t set);
+/* Test entry points: MLP outputs on raw inputs x (n x 12) through the fused
+ * inference kernel (path 0) or the training GEMMs (path 1); gradient of the
+ * mean squared error against targets t (n x 3). */
+int mgsb_mlp_forward(mgsb_solver* s, const float* x, int32_t n, float* y, int32_t path);
+int mgsb_mlp_grad(mgsb_solver* s, const float* x, const float* t, int32_t n, float* grad, double* loss);
 
 /* Chebyshev rho used on a coarse level of the V-cycle (power iteration on
  * I - relax_coarse D^-1 A at create, or rho_coarse), and the number of free

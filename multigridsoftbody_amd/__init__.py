@@ -87,7 +87,7 @@ class mgsb_step_stats(C.Structure):
 
 class mgsb_run_stats_t(C.Structure):
     _fields_ = [("max_rel", C.c_double), ("total_cycles", C.c_int64), ("steps", C.c_int32),
-                ("failed_steps", C.c_int32)]
+                ("failed_steps", C.c_int32), ("fallbacks", C.c_int32), ("reserved", C.c_int32)]
 
 
 class mgsb_energy_t(C.Structure):
@@ -131,6 +131,15 @@ SIGNATURES = {
     "mgsb_residual": (C.c_int, [_vp, _dp]),
     "mgsb_level_info": (C.c_int, [_vp, C.c_int32, _ip, _ip, _ip]),
     "mgsb_level_rho": (C.c_int, [_vp, C.c_int32, _dp]),
+    "mgsb_learn_record": (C.c_int, [_vp, C.c_int32]),
+    "mgsb_learn_samples": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
+    "mgsb_learn_clear": (C.c_int, [_vp]),
+    "mgsb_learn_train": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_float, C.c_uint32, _fp]),
+    "mgsb_learn_nparams": (C.c_int, [_vp, _ip]),
+    "mgsb_learn_weights": (C.c_int, [_vp, _fp, C.c_int32, C.c_int32]),
+    "mgsb_learn_norm": (C.c_int, [_vp, _fp, C.c_int32]),
+    "mgsb_mlp_forward": (C.c_int, [_vp, _fp, C.c_int32, _fp, C.c_int32]),
+    "mgsb_mlp_grad": (C.c_int, [_vp, _fp, _fp, C.c_int32, _fp, _dp]),
     "mgsb_run_stats": (C.c_int, [_vp, C.POINTER(mgsb_run_stats_t), C.c_int32]),
     "mgsb_coarse_nfree": (C.c_int, [_vp, _ip]),
     "mgsb_set_timing": (C.c_int, [_vp, C.c_int32]),
@@ -419,7 +428,60 @@ class Solver:
         st = mgsb_run_stats_t()
         _check(lib.mgsb_run_stats(self._h, C.byref(st), int(reset)), "mgsb_run_stats")
         return {"max_rel": st.max_rel, "total_cycles": st.total_cycles, "steps": st.steps,
-                "failed_steps": st.failed_steps}
+                "failed_steps": st.failed_steps, "fallbacks": st.fallbacks}
+
+    # --- learned coarse correction -------------------------------------------
+    def learn_record(self, max_steps):
+        _check(lib.mgsb_learn_record(self._h, max_steps), "mgsb_learn_record")
+
+    def learn_samples(self):
+        n = C.c_int64()
+        _check(lib.mgsb_learn_samples(self._h, C.byref(n)), "mgsb_learn_samples")
+        return n.value
+
+    def learn_clear(self):
+        _check(lib.mgsb_learn_clear(self._h), "mgsb_learn_clear")
+
+    def learn_train(self, epochs=5, batch=4096, lr=1e-3, seed=0):
+        hist = np.zeros(epochs, np.float32)
+        _check(lib.mgsb_learn_train(self._h, epochs, batch, lr, seed, _fptr(hist)), "mgsb_learn_train")
+        return hist
+
+    def learn_nparams(self):
+        n = C.c_int32()
+        _check(lib.mgsb_learn_nparams(self._h, C.byref(n)), "mgsb_learn_nparams")
+        return n.value
+
+    def learn_weights(self, w=None):
+        n = self.learn_nparams()
+        if w is None:
+            out = np.zeros(n, np.float32)
+            _check(lib.mgsb_learn_weights(self._h, _fptr(out), n, 0), "mgsb_learn_weights")
+            return out
+        a = _f32(w, n)
+        _check(lib.mgsb_learn_weights(self._h, _fptr(a), n, 1), "mgsb_learn_weights")
+
+    def learn_norm(self, norm=None):
+        if norm is None:
+            out = np.zeros(30, np.float32)
+            _check(lib.mgsb_learn_norm(self._h, _fptr(out), 0), "mgsb_learn_norm")
+            return out
+        a = _f32(norm, 30)
+        _check(lib.mgsb_learn_norm(self._h, _fptr(a), 1), "mgsb_learn_norm")
+
+    def mlp_forward(self, x, path=0):
+        x = _f32(x).reshape(-1, 12)
+        y = np.zeros((x.shape[0], 3), np.float32)
+        _check(lib.mgsb_mlp_forward(self._h, _fptr(x), x.shape[0], _fptr(y), path), "mgsb_mlp_forward")
+        return y
+
+    def mlp_grad(self, x, t):
+        x = _f32(x).reshape(-1, 12)
+        t = _f32(t).reshape(-1, 3)
+        g = np.zeros(self.learn_nparams(), np.float32)
+        loss = C.c_double()
+        _check(lib.mgsb_mlp_grad(self._h, _fptr(x), _fptr(t), x.shape[0], _fptr(g), C.byref(loss)), "mgsb_mlp_grad")
+        return g, loss.value
 
     def level_rho(self, level):
         r = C.c_double()

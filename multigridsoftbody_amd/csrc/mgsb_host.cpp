@@ -792,9 +792,9 @@ void mgsb_default_params(mgsb_params* p) {
     p->schedule = MGSB_SCHED_SINGLE;
     p->if_mode = MGSB_IF_AC;
     p->tri_dcd_coarse = 1;
-    p->pre_sweeps = 2;
-    p->post_sweeps = 2;
-    p->coarse_sweeps = 4;
+    p->pre_sweeps = 5;
+    p->post_sweeps = 5;
+    p->coarse_sweeps = 2;
     p->max_cycles = 8;
     p->tol = 1e-3f;
     p->check_every = 0;
@@ -802,7 +802,7 @@ void mgsb_default_params(mgsb_params* p) {
     p->block_tet = 0;
     p->block_vert = 0;
     p->device = 0;
-    p->rho_smooth = 0.0f;
+    p->rho_smooth = 0.9f;
     p->relax_coarse = 0.6f;
     p->rho_coarse = 0.0f;
     p->coarse_direct = 1;

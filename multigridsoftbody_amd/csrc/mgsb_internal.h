@@ -68,5 +68,7 @@ void derive_vfix(const std::vector<float>& vfix_fine, const std::vector<int32_t>
 double power_rho(const HostLevel& L, const std::vector<float>& vfix, double dt, double relax, double k);
 int coarse_cholesky(const HostLevel& L, const std::vector<float>& vfix, double dt, double k,
                     std::vector<int32_t>& dense, std::vector<double>& W, int& n);
+void build_level_matrix(const HostLevel& L, double k, double dt, std::vector<int32_t>& ap,
+                        std::vector<int32_t>& aj, std::vector<float>& av);
 
 }  // namespace mgsb

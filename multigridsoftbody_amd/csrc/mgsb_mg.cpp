@@ -138,3 +138,48 @@ int coarse_cholesky(const HostLevel& L, const std::vector<float>& vfix, double d
 }
 
 }  // namespace mgsb
+
+namespace mgsb {
+
+// Scalar CSR matrix of A = m/h^2 I + L on all vertices of a level (the same
+// operator acts on x, y and z), L_ab = sum_t k vol_t sum_r AC_t[r][a] AC_t[r][b]
+// accumulated in FP64, columns ascending per row.
+void build_level_matrix(const HostLevel& L, double k, double dt, std::vector<int32_t>& ap,
+                        std::vector<int32_t>& aj, std::vector<float>& av) {
+    const int V = L.V;
+    std::vector<std::vector<std::pair<int, double>>> rows(V);
+    for (int t = 0; t < L.T; t++) {
+        const float* AC = &L.AC[(size_t)t * 12];
+        const double w = (double)L.vol[t] * k;
+        for (int a = 0; a < 4; a++) {
+            const int va = L.tets[(size_t)t * 4 + a];
+            for (int b = 0; b < 4; b++) {
+                const int vb = L.tets[(size_t)t * 4 + b];
+                double s = 0;
+                for (int r = 0; r < 3; r++) s += (double)AC[r * 4 + a] * (double)AC[r * 4 + b];
+                rows[va].push_back({vb, w * s});
+            }
+        }
+    }
+    ap.assign((size_t)V + 1, 0);
+    aj.clear();
+    av.clear();
+    for (int i = 0; i < V; i++) {
+        auto& r = rows[i];
+        r.push_back({i, (double)L.mass[i] / (dt * dt)});
+        std::sort(r.begin(), r.end(), [](const std::pair<int, double>& x, const std::pair<int, double>& y) {
+            return x.first < y.first;
+        });
+        size_t q = 0;
+        while (q < r.size()) {
+            int j = r[q].first;
+            double s = 0;
+            while (q < r.size() && r[q].first == j) s += r[q++].second;
+            aj.push_back(j);
+            av.push_back((float)s);
+        }
+        ap[i + 1] = (int32_t)aj.size();
+    }
+}
+
+}  // namespace mgsb

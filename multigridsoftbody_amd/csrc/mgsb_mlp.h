@@ -1,0 +1,31 @@
+// Interface of the learned-correction MLP (mgsb_mlp.hip) used by the solver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#define MLP_HIDDEN 128
+
+namespace mgsb {
+
+struct Mlp;
+
+int mlp_nparams(int in, int out);
+int mlp_create(Mlp** out, int in, int outd, uint32_t seed, hipStream_t st);
+void mlp_destroy(Mlp* p);
+int mlp_params_count(const Mlp* p);
+int mlp_in(const Mlp* p);
+int mlp_out(const Mlp* p);
+int mlp_set_params(Mlp* p, const float* w, int n, hipStream_t st);
+int mlp_get_params(Mlp* p, float* w, int n, hipStream_t st);
+// Fused BF16-MFMA inference: X (n x in, device) -> Y (row stride ldy, device).
+int mlp_infer(Mlp* p, const float* X, int n, float* Y, int ldy, hipStream_t st, const int* skip);
+// Test paths: gradients of the mean squared error, and the GEMM-path forward.
+int mlp_grad(Mlp* p, const float* X_host, const float* T_host, int n, float* grad_host, double* loss, hipStream_t st);
+int mlp_forward_train_path(Mlp* p, const float* X_host, int n, float* Y_host, hipStream_t st);
+// Adam training on device-resident samples X (n x in), T (n x out).
+int mlp_train(Mlp* p, const float* X, const float* T, long n, int epochs, int batch, float lr, uint32_t seed,
+              float* loss_hist, hipStream_t st);
+
+}  // namespace mgsb

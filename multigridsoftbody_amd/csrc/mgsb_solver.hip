@@ -25,6 +25,7 @@
 
 #include "mgsb_device.h"
 #include "mgsb_internal.h"
+#include "mgsb_mlp.h"
 
 using namespace mgsb;
 
@@ -51,6 +52,37 @@ constexpr int kVertWriteCol = 4;  // write collision flags / depth / energy
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
+
+// Sum of the xyz of cf[e0..e1) in index order.  Eight loads are issued
+// before the adds that consume them, so a vertex waits on memory about once
+// per eight entries instead of once per entry; the summation order (and the
+// result, bit for bit) is that of the plain loop.
+__device__ __forceinline__ void gather_sum(const float4* __restrict__ cf, int e0, int e1, float& fx, float& fy,
+                                           float& fz) {
+    int e = e0;
+    for (; e + 8 <= e1; e += 8) {
+        float4 q[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) q[k] = cf[e + k];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            fx = fx + q[k].x;
+            fy = fy + q[k].y;
+            fz = fz + q[k].z;
+        }
+    }
+    const int rem = e1 - e;
+    float4 q[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) q[k] = k < rem ? cf[e + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (k < rem) {
+            fx = fx + q[k].x;
+            fy = fy + q[k].y;
+            fz = fz + q[k].z;
+        }
+}
 
 // calculateST (PDSolver.cu:72-116).  Block 0 / lane 0 first advances sphere
 // centres by vel*dt (UpdateCollider before Step, Simulator.cpp:146-149).
@@ -254,13 +286,7 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= V) return;
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    const int e0 = rp[i], e1 = rp[i + 1];
-    for (int e = e0; e < e1; e++) {
-        const float4 q = cf[e];
-        fx = fx + q.x;
-        fy = fy + q.y;
-        fz = fz + q.z;
-    }
+    gather_sum(cf, rp[i], rp[i + 1], fx, fy, fz);
     const float4 p = x[i];
     float cdx = 0.f, cdy = 0.f, cdz = 0.f, cfx = 0.f, cfy = 0.f, cfz = 0.f;
     int flag = 0;
@@ -517,12 +543,7 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
     double v[1] = {0.0};
     if (i < V) {
         float fx = 0.f, fy = 0.f, fz = 0.f;
-        for (int e = rp[i]; e < rp[i + 1]; e++) {
-            const float4 q = cf[e];
-            fx = fx + q.x;
-            fy = fy + q.y;
-            fz = fz + q.z;
-        }
+        gather_sum(cf, rp[i], rp[i + 1], fx, fy, fz);
         const float4 p = x[i];
         float cfx = 0.f, cfy = 0.f, cfz = 0.f;
         if (flags & kVertPoint) {
@@ -589,6 +610,9 @@ struct VState {
     long long total_cycles;
     int nsteps;
     int nfail;      // steps that ended with res > tol * res0
+    int fallback;   // learned schedule: the warm start was rejected this step
+    int nfallback;
+    double res_ws;  // residual right after the warm start
 };
 
 // End of a step: fold the step's outcome into the run aggregates.
@@ -599,6 +623,87 @@ __global__ void k_step_done(VState* st) {
     st->total_cycles += st->cycles;
     st->nsteps += 1;
     if (rel > (double)st->tol) st->nfail += 1;
+    st->nfallback += st->fallback;
+}
+
+// Learned schedule: put x back to the inertial prediction s when the judge
+// rejected the warm start.
+__global__ void __launch_bounds__(256) k_revert(int V, float4* __restrict__ x, float4* __restrict__ xprev,
+                                                const float4* __restrict__ s, const VState* st) {
+    if (!st->fallback) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const float4 q = s[i];
+    x[i] = q;
+    xprev[i] = q;
+}
+
+// Features of the learned coarse correction at each vertex of level 1:
+// restricted residual g(s) (3), fine velocity injected through the
+// coarse-in-fine binding (3), rest position (3), the previous step's coarse
+// correction (3); normalised with (mu, sd).  Raw values optionally recorded.
+constexpr int kFeat = 12;
+__global__ void __launch_bounds__(256) k_features(int Vc, const float4* __restrict__ r1, const float4* __restrict__ vf,
+                                                  const int4* __restrict__ aids, const float4* __restrict__ aw,
+                                                  const float4* __restrict__ X0, const float4* __restrict__ eprev,
+                                                  const float* __restrict__ norm, float* __restrict__ feat,
+                                                  float* __restrict__ rec) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= Vc) return;
+    const int4 id = aids[j];
+    const float4 w = aw[j];
+    const float4 a = vf[id.x], b = vf[id.y], c = vf[id.z], d = vf[id.w];
+    const float4 r = r1[j], X = X0[j], ep = eprev[j];
+    const float raw[kFeat] = {r.x, r.y, r.z,
+                              w.x * a.x + w.y * b.x + w.z * c.x + w.w * d.x,
+                              w.x * a.y + w.y * b.y + w.z * c.y + w.w * d.y,
+                              w.x * a.z + w.y * b.z + w.z * c.z + w.w * d.z,
+                              X.x, X.y, X.z, ep.x, ep.y, ep.z};
+#pragma unroll
+    for (int q = 0; q < kFeat; q++) {
+        if (rec) rec[(long)j * kFeat + q] = raw[q];
+        feat[(long)j * kFeat + q] = (raw[q] - norm[q]) / norm[kFeat + q];
+    }
+}
+
+// epred = vfix * (pred * out_sd + out_mu)
+__global__ void __launch_bounds__(256) k_apply_pred(int Vc, const float* __restrict__ pred,
+                                                    const float* __restrict__ norm, const float4* __restrict__ vcl,
+                                                    float4* __restrict__ epred) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= Vc) return;
+    const float* mu = norm + 2 * kFeat;
+    const float* sd = norm + 2 * kFeat + 3;
+    const float m = vcl[j].z;
+    epred[j] = make_float4(m * (pred[j * 3 + 0] * sd[0] + mu[0]), m * (pred[j * 3 + 1] * sd[1] + mu[1]),
+                           m * (pred[j * 3 + 2] * sd[2] + mu[2]), 0.f);
+}
+
+// Coarse correction achieved this step: (x - s) injected at each level-1
+// vertex through its coarse-in-fine binding; kept for the next step's
+// features and optionally recorded as the training target.
+__global__ void __launch_bounds__(256) k_target(int Vc, const float4* __restrict__ x, const float4* __restrict__ s,
+                                                const int4* __restrict__ aids, const float4* __restrict__ aw,
+                                                float4* __restrict__ eprev, float* __restrict__ rec) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= Vc) return;
+    const int4 id = aids[j];
+    const float4 w = aw[j];
+    const int ids[4] = {id.x, id.y, id.z, id.w};
+    const float ws[4] = {w.x, w.y, w.z, w.w};
+    float dx = 0.f, dy = 0.f, dz = 0.f;
+    for (int q = 0; q < 4; q++) {
+        const float4 a = x[ids[q]], b = s[ids[q]];
+        dx = dx + ws[q] * (a.x - b.x);
+        dy = dy + ws[q] * (a.y - b.y);
+        dz = dz + ws[q] * (a.z - b.z);
+    }
+    eprev[j] = make_float4(dx, dy, dz, 0.f);
+    if (rec) {
+        rec[(long)j * 3 + 0] = dx;
+        rec[(long)j * 3 + 1] = dy;
+        rec[(long)j * 3 + 2] = dz;
+    }
 }
 
 // mode 0: step start (res0 = res); mode 1: end of a cycle.  One block sums
@@ -618,7 +723,16 @@ __global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict_
             st->res0 = r;
             st->res = r;
             st->cycles = 0;
+            st->fallback = 0;
             st->converged = (st->check && r == 0.0) ? 1 : 0;
+        } else if (mode == 2) {  // judge of the learned warm start
+            st->res_ws = r;
+            if (r > st->res0) {
+                st->fallback = 1;  // worse than the inertial prediction: revert
+            } else {
+                st->res = r;
+                if (st->check && r <= (double)st->tol * st->res0) st->converged = 1;
+            }
         } else {
             st->res = r;
             st->cycles += 1;
@@ -627,24 +741,35 @@ __global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict_
     }
 }
 
-// r_c[j] = vfix_j * sum_{fine i ascending} w_ij r_f[i]  (P^T, no atomics).
+// r_c[j] = vfix_j * sum_i w_ij r_f[i]  (P^T, no atomics).  One wave per
+// coarse vertex: lane l sums entries l, l+64, ... of the row, then a fixed
+// shuffle tree combines the lanes (deterministic order).
 __global__ void __launch_bounds__(256) k_restrict(int Vc, const int* __restrict__ rtp, const int* __restrict__ rti,
                                                   const float* __restrict__ rtw, const float4* __restrict__ rf,
                                                   const float4* __restrict__ vcl, float4* __restrict__ rc,
                                                   const int* __restrict__ skip) {
     if (skip && *skip) return;
-    const int j = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (j >= Vc) return;
+    const int j = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (j >= Vc) return;  // whole wave
     float ax = 0.f, ay = 0.f, az = 0.f;
-    for (int e = rtp[j]; e < rtp[j + 1]; e++) {
+    for (int e = rtp[j] + lane; e < rtp[j + 1]; e += 64) {
         const float w = rtw[e];
         const float4 q = rf[rti[e]];
         ax = ax + w * q.x;
         ay = ay + w * q.y;
         az = az + w * q.z;
     }
-    const float fx = vcl[j].z;
-    rc[j] = make_float4(ax * fx, ay * fx, az * fx, 0.f);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        ax += __shfl_down(ax, off, 64);
+        ay += __shfl_down(ay, off, 64);
+        az += __shfl_down(az, off, 64);
+    }
+    if (lane == 0) {
+        const float fx = vcl[j].z;
+        rc[j] = make_float4(ax * fx, ay * fx, az * fx, 0.f);
+    }
 }
 
 // dst[i] += mask_i * (P src)[i]; dst2 (optional) receives the new value.
@@ -670,69 +795,89 @@ __global__ void __launch_bounds__(256) k_prolong_add(int N, float4* __restrict__
     if (dst2) dst2[i] = o;
 }
 
-// Damped Jacobi + Chebyshev sweep on A e = r with A e = m/h^2 e - f_lin(e)
-// (f_lin gathered from the linear tet pass).  first_zero: e = e_prev = 0.
-__global__ void __launch_bounds__(256) k_lin_smooth(int V, float4* __restrict__ e, float4* __restrict__ ep,
-                                                    const float4* __restrict__ r, const float4* __restrict__ vcl,
-                                                    const int* __restrict__ rp, const float4* __restrict__ cf,
-                                                    float omega, float relax, int first_zero,
-                                                    const int* __restrict__ skip) {
-    if (skip && *skip) return;
-    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= V) return;
+// Row i of the scalar level matrix times a float4 field (columns ascending;
+// loads issued four at a time, sums in row order).
+__device__ __forceinline__ float3 spmv_row(int i, const int* __restrict__ ap, const int* __restrict__ aj,
+                                           const float* __restrict__ av, const float4* __restrict__ x) {
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    const int q0 = ap[i], q1 = ap[i + 1];
+    for (int q = q0; q < q1; q += 16) {
+        const int n = q1 - q;
+        int j[16];
+        float a[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            j[k] = k < n ? aj[q + k] : 0;
+            a[k] = k < n ? av[q + k] : 0.f;
+        }
+        float4 xv[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) xv[k] = k < n ? x[j[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            if (k < n) {
+                sx = sx + a[k] * xv[k].x;
+                sy = sy + a[k] * xv[k].y;
+                sz = sz + a[k] * xv[k].z;
+            }
+    }
+    return make_float3(sx, sy, sz);
+}
+
+// One damped-Jacobi + Chebyshev sweep on A e = r for vertex i.  e is read at
+// neighbours, so the new value goes into ep[i] (which only vertex i reads);
+// the caller then swaps the roles of e and ep.  vcl = (A_ii, 0, vfix, m).
+__device__ __forceinline__ void jacobi_cheb_row(int i, const int* __restrict__ ap, const int* __restrict__ aj,
+                                                const float* __restrict__ av, float4* __restrict__ e,
+                                                float4* __restrict__ ep, const float4* __restrict__ r,
+                                                const float4* __restrict__ vcl, float omega, float relax,
+                                                int first_zero) {
     const float4 c = vcl[i];
-    const float d = c.x + c.y;
+    const float d = c.x;
     const float fx = c.z;
     const float4 rr = r[i];
     if (first_zero) {
-        float yx = 0.f + fx * (relax * ((rr.x - 0.f) / d));
-        float yy = 0.f + fx * (relax * ((rr.y - 0.f) / d));
-        float yz = 0.f + fx * (relax * ((rr.z - 0.f) / d));
-        yx = omega * (yx - 0.f) + 0.f;
-        yy = omega * (yy - 0.f) + 0.f;
-        yz = omega * (yz - 0.f) + 0.f;
-        ep[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        e[i] = make_float4(yx, yy, yz, 0.f);
+        float yx = fx * (relax * (rr.x / d));
+        float yy = fx * (relax * (rr.y / d));
+        float yz = fx * (relax * (rr.z / d));
+        ep[i] = make_float4(omega * yx, omega * yy, omega * yz, 0.f);
+        e[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
-    float fxs = 0.f, fys = 0.f, fzs = 0.f;
-    for (int q = rp[i]; q < rp[i + 1]; q++) {
-        const float4 t = cf[q];
-        fxs = fxs + t.x;
-        fys = fys + t.y;
-        fzs = fzs + t.z;
-    }
+    const float3 Ae = spmv_row(i, ap, aj, av, e);
     const float4 ec = e[i], pc = ep[i];
-    const float qx = c.x * ec.x - fxs, qy = c.x * ec.y - fys, qz = c.x * ec.z - fzs;
-    float yx = ec.x + fx * (relax * ((rr.x - qx) / d));
-    float yy = ec.y + fx * (relax * ((rr.y - qy) / d));
-    float yz = ec.z + fx * (relax * ((rr.z - qz) / d));
+    float yx = ec.x + fx * (relax * ((rr.x - Ae.x) / d));
+    float yy = ec.y + fx * (relax * ((rr.y - Ae.y) / d));
+    float yz = ec.z + fx * (relax * ((rr.z - Ae.z) / d));
     yx = omega * (yx - pc.x) + pc.x;
     yy = omega * (yy - pc.y) + pc.y;
     yz = omega * (yz - pc.z) + pc.z;
-    ep[i] = ec;
-    e[i] = make_float4(yx, yy, yz, 0.f);
+    ep[i] = make_float4(yx, yy, yz, 0.f);
 }
 
-// rr = vfix (r - A e).
-__global__ void __launch_bounds__(256) k_lin_resid(int V, const float4* __restrict__ e, const float4* __restrict__ r,
-                                                   const float4* __restrict__ vcl, const int* __restrict__ rp,
-                                                   const float4* __restrict__ cf, float4* __restrict__ rr,
-                                                   const int* __restrict__ skip) {
+__global__ void __launch_bounds__(256) k_spmv_smooth(int V, const int* __restrict__ ap, const int* __restrict__ aj,
+                                                     const float* __restrict__ av, float4* __restrict__ e,
+                                                     float4* __restrict__ ep, const float4* __restrict__ r,
+                                                     const float4* __restrict__ vcl, float omega, float relax,
+                                                     int first_zero, const int* __restrict__ skip) {
     if (skip && *skip) return;
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= V) return;
-    float fxs = 0.f, fys = 0.f, fzs = 0.f;
-    for (int q = rp[i]; q < rp[i + 1]; q++) {
-        const float4 t = cf[q];
-        fxs = fxs + t.x;
-        fys = fys + t.y;
-        fzs = fzs + t.z;
-    }
-    const float4 c = vcl[i], ec = e[i], rv = r[i];
-    const float fx = c.z;
-    rr[i] = make_float4(fx * (rv.x - (c.x * ec.x - fxs)), fx * (rv.y - (c.x * ec.y - fys)),
-                        fx * (rv.z - (c.x * ec.z - fzs)), 0.f);
+    jacobi_cheb_row(i, ap, aj, av, e, ep, r, vcl, omega, relax, first_zero);
+}
+
+// rr = vfix (r - A e).
+__global__ void __launch_bounds__(256) k_spmv_resid(int V, const int* __restrict__ ap, const int* __restrict__ aj,
+                                                    const float* __restrict__ av, const float4* __restrict__ e,
+                                                    const float4* __restrict__ r, const float4* __restrict__ vcl,
+                                                    float4* __restrict__ rr, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const float3 Ae = spmv_row(i, ap, aj, av, e);
+    const float4 rv = r[i];
+    const float fx = vcl[i].z;
+    rr[i] = make_float4(fx * (rv.x - Ae.x), fx * (rv.y - Ae.y), fx * (rv.z - Ae.z), 0.f);
 }
 
 __global__ void __launch_bounds__(256) k_zero4(int n, float4* __restrict__ a, const int* __restrict__ skip) {
@@ -796,6 +941,172 @@ __global__ void __launch_bounds__(256) k_coarse_solve(int Vc, int n, const int* 
     }
 }
 
+// Small coarse levels of the linear V-cycle in ONE workgroup: level 0 of the
+// descriptor array already holds its right-hand side r; the workgroup runs
+// pre-smoothing, residual, restriction, ... down to the coarsest level, the
+// coarsest solve, then prolongation and post-smoothing back up, with
+// workgroup barriers between phases instead of kernel launches.  On return
+// the first level's correction is in e0.
+struct TailLevel {
+    int V, nfree;
+    const int *ap, *aj;
+    const float* av;
+    const float4* vcl;
+    float4 *e0, *e1, *r, *rr;
+    const int *rtp, *rti;   // restriction into this level from the previous one
+    const float* rtw;
+    const int4* iids;       // prolongation from the next level into this one
+    const float4* iw;
+    const int* dense;
+    const double* W;
+    float rho;
+    int pad;
+};
+
+__global__ void __launch_bounds__(1024) k_coarse_tail(const TailLevel* __restrict__ tl, int n, int sweeps,
+                                                      float relax, int direct, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ double rb[3][kMaxDense];
+    __shared__ double yb[3][kMaxDense];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    int flip[8];  // 0: e in e0, 1: e in e1 (per level)
+    for (int q = 0; q < n; q++) flip[q] = 0;
+    auto smooth = [&](int q, int ns, bool zero) {
+        const TailLevel& L = tl[q];
+        float omega = 1.0f;
+        for (int k = 0; k < ns; k++) {
+            omega = 4 / (4 - L.rho * L.rho * omega);
+            float4* e = flip[q] ? L.e1 : L.e0;
+            float4* ep = flip[q] ? L.e0 : L.e1;
+            for (int i = tid; i < L.V; i += nt)
+                jacobi_cheb_row(i, L.ap, L.aj, L.av, e, ep, L.r, L.vcl, omega, relax, zero && k == 0);
+            __syncthreads();
+            flip[q] ^= 1;
+        }
+    };
+    // down
+    for (int q = 0; q + 1 < n; q++) {
+        const TailLevel& L = tl[q];
+        smooth(q, sweeps, true);
+        const float4* e = flip[q] ? L.e1 : L.e0;
+        for (int i = tid; i < L.V; i += nt) {
+            const float3 Ae = spmv_row(i, L.ap, L.aj, L.av, e);
+            const float4 rv = L.r[i];
+            const float fx = L.vcl[i].z;
+            L.rr[i] = make_float4(fx * (rv.x - Ae.x), fx * (rv.y - Ae.y), fx * (rv.z - Ae.z), 0.f);
+        }
+        __syncthreads();
+        const TailLevel& C = tl[q + 1];
+        {  // one wave per coarse vertex, lanes stride the row, shuffle tree
+            const int wave = tid >> 6, lane = tid & 63, nw = nt >> 6;
+            for (int j = wave; j < C.V; j += nw) {
+                float ax = 0.f, ay = 0.f, az = 0.f;
+                for (int t = C.rtp[j] + lane; t < C.rtp[j + 1]; t += 64) {
+                    const float w = C.rtw[t];
+                    const float4 v = L.rr[C.rti[t]];
+                    ax = ax + w * v.x;
+                    ay = ay + w * v.y;
+                    az = az + w * v.z;
+                }
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    ax += __shfl_down(ax, off, 64);
+                    ay += __shfl_down(ay, off, 64);
+                    az += __shfl_down(az, off, 64);
+                }
+                if (lane == 0) {
+                    const float fx = C.vcl[j].z;
+                    C.r[j] = make_float4(ax * fx, ay * fx, az * fx, 0.f);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // coarsest
+    {
+        const TailLevel& L = tl[n - 1];
+        if (direct && L.nfree >= 0) {
+            for (int i = tid; i < L.V; i += nt) {
+                const int d = L.dense[i];
+                if (d >= 0) {
+                    const float4 v = L.r[i];
+                    rb[0][d] = v.x;
+                    rb[1][d] = v.y;
+                    rb[2][d] = v.z;
+                }
+            }
+            __syncthreads();
+            const int m = L.nfree;
+            for (int row = tid; row < m; row += nt) {
+                double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+                const double* wr = L.W + (size_t)row * m;
+                for (int k = 0; k <= row; k++) {
+                    const double w = wr[k];
+                    a0 += w * rb[0][k];
+                    a1 += w * rb[1][k];
+                    a2 += w * rb[2][k];
+                }
+                yb[0][row] = a0;
+                yb[1][row] = a1;
+                yb[2][row] = a2;
+            }
+            __syncthreads();
+            for (int row = tid; row < m; row += nt) {
+                double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+                for (int k = row; k < m; k++) {
+                    const double w = L.W[(size_t)k * m + row];
+                    a0 += w * yb[0][k];
+                    a1 += w * yb[1][k];
+                    a2 += w * yb[2][k];
+                }
+                rb[0][row] = a0;
+                rb[1][row] = a1;
+                rb[2][row] = a2;
+            }
+            __syncthreads();
+            for (int i = tid; i < L.V; i += nt) {
+                const int d = L.dense[i];
+                L.e0[i] = d >= 0 ? make_float4((float)rb[0][d], (float)rb[1][d], (float)rb[2][d], 0.f)
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            flip[n - 1] = 0;
+            __syncthreads();
+        } else {
+            smooth(n - 1, sweeps > 0 ? sweeps : 1, true);
+        }
+    }
+    // up
+    for (int q = n - 2; q >= 0; q--) {
+        const TailLevel& L = tl[q];
+        const TailLevel& C = tl[q + 1];
+        const float4* ec = flip[q + 1] ? C.e1 : C.e0;
+        float4* e = flip[q] ? L.e1 : L.e0;
+        float4* ep = flip[q] ? L.e0 : L.e1;
+        for (int i = tid; i < L.V; i += nt) {
+            const int4 id = L.iids[i];
+            const float4 ww = L.iw[i];
+            const float4 a = ec[id.x], b = ec[id.y], c = ec[id.z], d = ec[id.w];
+            const float vx = ww.x * a.x + ww.y * b.x + ww.z * c.x + ww.w * d.x;
+            const float vy = ww.x * a.y + ww.y * b.y + ww.z * c.y + ww.w * d.y;
+            const float vz = ww.x * a.z + ww.y * b.z + ww.z * c.z + ww.w * d.z;
+            const float m = L.vcl[i].z;
+            float4 o = e[i];
+            o.x = o.x + m * vx;
+            o.y = o.y + m * vy;
+            o.z = o.z + m * vz;
+            e[i] = o;
+            ep[i] = o;
+        }
+        __syncthreads();
+        smooth(q, sweeps, false);
+    }
+    // leave the first level's correction in e0
+    if (flip[0]) {
+        const TailLevel& L = tl[0];
+        for (int i = tid; i < L.V; i += nt) L.e0[i] = L.e1[i];
+    }
+}
+
 inline int nblk(int n, int bs) { return (n + bs - 1) / bs; }
 
 // ---------------------------------------------------------------------------
@@ -832,6 +1143,10 @@ struct DevLevel {
     float4 *r = nullptr, *e = nullptr, *ep = nullptr, *rr = nullptr, *vcl = nullptr;
     int *rtp = nullptr, *rti = nullptr;
     float* rtw = nullptr;
+    int *ap = nullptr, *aj = nullptr;  // scalar level matrix A = m/h^2 + L (coarse levels)
+    float* av = nullptr;
+    std::vector<float> adiag;          // its diagonal
+    int cur = 0;                       // which of e / ep holds the correction
     int* dense = nullptr;
     double* W = nullptr;
     int nfree = -1;
@@ -873,6 +1188,15 @@ struct mgsb_solver {
     mgsb_step_stats last{};
     VState* vstate = nullptr;
     double* vpart = nullptr;  // partials of the level-0 residual norm
+    TailLevel* tail = nullptr; // descriptors of the one-workgroup coarse tail
+    int tail_from = -1;        // first level handled by the tail kernel (-1: none)
+    // learned coarse correction (level 1)
+    Mlp* mlp = nullptr;
+    float *feat = nullptr, *pred = nullptr, *norm = nullptr;
+    float4 *epred = nullptr, *eprev = nullptr, *X1 = nullptr;
+    float *recX = nullptr, *recT = nullptr;  // recorded samples (raw)
+    int rec_cap = 0, rec_steps = 0;           // in steps of V1 samples
+    bool recording = false;
     std::vector<void*> owned;
 
     ~mgsb_solver() {
@@ -886,6 +1210,7 @@ struct mgsb_solver {
         for (auto& lv : L)
             for (void* q : lv.owned) hipFree(q);
         for (void* q : owned) hipFree(q);
+        if (mlp) mlp_destroy(mlp);
         if (stream) hipStreamDestroy(stream);
     }
 };
@@ -1125,6 +1450,11 @@ int enqueue_step_twolevel(mgsb_solver* s) {
 // --- V-cycle -------------------------------------------------------------------
 float cheb_next(float omega, float rho) { return 4 / (4 - rho * rho * omega); }
 
+constexpr int kTailMaxV = 4096;  // levels this small run inside the one-workgroup tail
+
+float4* cur_e(DevLevel& d) { return d.cur ? d.ep : d.e; }
+float4* cur_ep(DevLevel& d) { return d.cur ? d.e : d.ep; }
+
 int lin_smooth(mgsb_solver* s, int l, int sweeps, bool zero, const int* skip) {
     DevLevel& d = s->L[l];
     const int bs = block_vert(s);
@@ -1132,24 +1462,28 @@ int lin_smooth(mgsb_solver* s, int l, int sweeps, bool zero, const int* skip) {
     float omega = 1.0f;
     for (int k = 0; k < sweeps; k++) {
         omega = cheb_next(omega, rho);
-        const bool fz = zero && k == 0;
-        if (!fz) {
-            int rc = launch_tet(s, d, kTetModeLinear, false, d.e, skip);
-            if (rc) return rc;
-        }
-        k_lin_smooth<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.e, d.ep, d.r, d.vcl, d.rp, d.cf, omega,
-                                                          s->p.relax_coarse, fz ? 1 : 0, skip);
+        k_spmv_smooth<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.ap, d.aj, d.av, cur_e(d), cur_ep(d), d.r, d.vcl,
+                                                           omega, s->p.relax_coarse, (zero && k == 0) ? 1 : 0, skip);
         HIPCHK(hipGetLastError());
+        d.cur ^= 1;
     }
     return MGSB_OK;
 }
 
-// Linear V-cycle on A_l e_l = r_l (l >= 1).
+// Linear V-cycle on A_l e_l = r_l (l >= 1); the result is in cur_e(L[l]).
 int vcycle_lin(mgsb_solver* s, int l, const int* skip) {
     DevLevel& d = s->L[l];
     const int nl = (int)s->L.size();
     const int bs = block_vert(s);
     int rc;
+    if (l == s->tail_from) {
+        k_coarse_tail<<<1, 1024, 0, s->stream>>>(s->tail, nl - l, s->p.coarse_sweeps, s->p.relax_coarse,
+                                                 s->p.coarse_direct, skip);
+        HIPCHK(hipGetLastError());
+        d.cur = 0;
+        return MGSB_OK;
+    }
+    d.cur = 0;
     if (l == nl - 1) {
         if (s->p.coarse_direct && d.nfree >= 0) {
             k_coarse_solve<<<1, 256, 0, s->stream>>>(d.V, d.nfree, d.dense, d.W, d.r, d.e, skip);
@@ -1159,19 +1493,14 @@ int vcycle_lin(mgsb_solver* s, int l, const int* skip) {
         return lin_smooth(s, l, std::max(1, s->p.coarse_sweeps), true, skip);
     }
     if ((rc = lin_smooth(s, l, s->p.coarse_sweeps, true, skip))) return rc;
-    if (s->p.coarse_sweeps == 0) {
-        k_zero4<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.e, skip);
-        HIPCHK(hipGetLastError());
-    }
-    if ((rc = launch_tet(s, d, kTetModeLinear, false, d.e, skip))) return rc;
-    k_lin_resid<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.e, d.r, d.vcl, d.rp, d.cf, d.rr, skip);
+    k_spmv_resid<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.ap, d.aj, d.av, cur_e(d), d.r, d.vcl, d.rr, skip);
     HIPCHK(hipGetLastError());
     DevLevel& c = s->L[l + 1];
-    k_restrict<<<nblk(c.V, bs), bs, 0, s->stream>>>(c.V, c.rtp, c.rti, c.rtw, d.rr, c.vcl, c.r, skip);
+    k_restrict<<<nblk(c.V * 64, 256), 256, 0, s->stream>>>(c.V, c.rtp, c.rti, c.rtw, d.rr, c.vcl, c.r, skip);
     HIPCHK(hipGetLastError());
     if ((rc = vcycle_lin(s, l + 1, skip))) return rc;
     LevelMap& m = s->M[l];
-    k_prolong_add<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.e, d.ep, c.e, m.iids, m.iw, d.vcl, skip);
+    k_prolong_add<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, cur_e(d), cur_ep(d), cur_e(c), m.iids, m.iw, d.vcl, skip);
     HIPCHK(hipGetLastError());
     return lin_smooth(s, l, s->p.coarse_sweeps, false, skip);
 }
@@ -1187,17 +1516,42 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, bool norm, bool r
     return MGSB_OK;
 }
 
-// Residual-correction V-cycle step (see include/mgsb.h, MGSB_SCHED_VCYCLE).
-int enqueue_step_vcycle(mgsb_solver* s) {
+// Residual-correction V-cycle step (see include/mgsb.h, MGSB_SCHED_VCYCLE);
+// with learned = true the MLP's coarse correction warm-starts the cycles.
+int enqueue_step_vcycle(mgsb_solver* s, bool learned) {
     DevLevel& f = s->L[0];
+    DevLevel& c1 = s->L[1];
     const int* skip = &s->vstate->converged;
     const int nbv = nblk(f.V, 256);
     const int bs = block_vert(s);
+    const bool features = learned || s->recording;
+    float* recX = s->recording ? s->recX + (size_t)s->rec_steps * c1.V * kFeat : nullptr;
+    float* recT = s->recording ? s->recT + (size_t)s->rec_steps * c1.V * 3 : nullptr;
     int rc;
     if ((rc = launch_st(s, f, 1))) return rc;
-    if ((rc = fine_residual(s, true, false, true, true, nullptr))) return rc;
+    if ((rc = fine_residual(s, true, features, true, true, nullptr))) return rc;
     k_check<<<1, 256, 0, s->stream>>>(nbv, s->vpart, s->vstate, 0);
     HIPCHK(hipGetLastError());
+    if (features) {
+        k_restrict<<<nblk(c1.V * 64, 256), 256, 0, s->stream>>>(c1.V, c1.rtp, c1.rti, c1.rtw, f.r, c1.vcl, c1.r,
+                                                                nullptr);
+        k_features<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, c1.r, f.v, s->M[0].aids, s->M[0].aw, s->X1, s->eprev,
+                                                         s->norm, s->feat, recX);
+        HIPCHK(hipGetLastError());
+    }
+    if (learned) {
+        if ((rc = mlp_infer(s->mlp, s->feat, c1.V, s->pred, 3, s->stream, skip))) return rc;
+        k_apply_pred<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, s->pred, s->norm, c1.vcl, s->epred);
+        k_prolong_add<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, s->epred, s->M[0].iids, s->M[0].iw,
+                                                           f.vc, skip);
+        HIPCHK(hipGetLastError());
+        if ((rc = fine_residual(s, true, false, true, true, skip))) return rc;
+        k_check<<<1, 256, 0, s->stream>>>(nbv, s->vpart, s->vstate, 2);
+        k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.s, s->vstate);
+        HIPCHK(hipGetLastError());
+        // cf may hold forces at the rejected warm start: recompute at x
+        if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip))) return rc;
+    }
     const float rs = s->p.rho_smooth;
     for (int c = 0; c < s->p.max_cycles; c++) {
         float omega = 1.0f;
@@ -1208,11 +1562,12 @@ int enqueue_step_vcycle(mgsb_solver* s) {
         }
         if ((rc = fine_residual(s, s->p.pre_sweeps > 0, true, false, false, skip))) return rc;
         DevLevel& c1 = s->L[1];
-        k_restrict<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, c1.rtp, c1.rti, c1.rtw, f.r, c1.vcl, c1.r, skip);
+        k_restrict<<<nblk(c1.V * 64, 256), 256, 0, s->stream>>>(c1.V, c1.rtp, c1.rti, c1.rtw, f.r, c1.vcl, c1.r,
+                                                                skip);
         HIPCHK(hipGetLastError());
         if ((rc = vcycle_lin(s, 1, skip))) return rc;
-        k_prolong_add<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, c1.e, s->M[0].iids, s->M[0].iw, f.vc,
-                                                           skip);
+        k_prolong_add<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, cur_e(c1), s->M[0].iids, s->M[0].iw,
+                                                           f.vc, skip);
         HIPCHK(hipGetLastError());
         omega = 1.0f;
         for (int k = 0; k < s->p.post_sweeps; k++) {
@@ -1227,6 +1582,11 @@ int enqueue_step_vcycle(mgsb_solver* s) {
     }
     k_step_done<<<1, 64, 0, s->stream>>>(s->vstate);
     HIPCHK(hipGetLastError());
+    if (features || s->eprev) {
+        k_target<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, f.x, f.s, s->M[0].aids, s->M[0].aw, s->eprev, recT);
+        HIPCHK(hipGetLastError());
+    }
+    if (s->recording) s->rec_steps++;
     return launch_vel(s, f);
 }
 
@@ -1234,7 +1594,8 @@ int enqueue_step(mgsb_solver* s) {
     switch (s->p.schedule) {
         case MGSB_SCHED_SINGLE: return enqueue_step_single(s);
         case MGSB_SCHED_TWOLEVEL: return enqueue_step_twolevel(s);
-        case MGSB_SCHED_VCYCLE: return enqueue_step_vcycle(s);
+        case MGSB_SCHED_VCYCLE: return enqueue_step_vcycle(s, false);
+        case MGSB_SCHED_LEARNED: return enqueue_step_vcycle(s, true);
         default: set_error("schedule not available"); return MGSB_ERR_ARG;
     }
 }
@@ -1249,9 +1610,23 @@ int setup_vcycle(mgsb_solver* s) {
     for (int l = 1; l < nl; l++) {
         DevLevel& c = s->L[l];
         derive_vfix(s->L[l - 1].vfix, s->M[l - 1].h_aids, s->M[l - 1].h_aw, c.V, c.vfix);
-        std::vector<float4> vcl(c.V);
+        std::vector<int32_t> ap, aj;
+        std::vector<float> av;
+        build_level_matrix(c.h, s->p.stiffness, dt, ap, aj, av);
+        c.adiag.assign(c.V, 0.f);
         for (int i = 0; i < c.V; i++)
-            vcl[i] = make_float4(c.h.mass[i] / (dt * dt), c.h.diag[i], c.vfix[i], c.h.mass[i]);
+            for (int q = ap[i]; q < ap[i + 1]; q++)
+                if (aj[q] == i) c.adiag[i] = av[q];
+        if (!c.ap) {
+            int rc = up(s, c.owned, &c.ap, ap.data(), ap.size());
+            if (rc) return rc;
+            if ((rc = up(s, c.owned, &c.aj, aj.data(), aj.size()))) return rc;
+            if ((rc = up(s, c.owned, &c.av, av.data(), av.size()))) return rc;
+        } else {
+            HIPCHK(hipMemcpyAsync(c.av, av.data(), sizeof(float) * av.size(), hipMemcpyHostToDevice, s->stream));
+        }
+        std::vector<float4> vcl(c.V);
+        for (int i = 0; i < c.V; i++) vcl[i] = make_float4(c.adiag[i], 0.f, c.vfix[i], c.h.mass[i]);
         HIPCHK(hipMemcpyAsync(c.vcl, vcl.data(), sizeof(float4) * c.V, hipMemcpyHostToDevice, s->stream));
         c.rho = s->p.rho_coarse > 0 ? s->p.rho_coarse
                                      : power_rho(c.h, c.vfix, dt, s->p.relax_coarse, s->p.stiffness);
@@ -1276,6 +1651,42 @@ int setup_vcycle(mgsb_solver* s) {
         HIPCHK(hipStreamSynchronize(s->stream));
         cc.nfree = n;
     }
+    // one-workgroup tail: the longest suffix of levels (>= 1) with V <= kTailMaxV
+    s->tail_from = -1;
+    for (int l = nl - 1; l >= 1 && s->L[l].V <= kTailMaxV; l--) s->tail_from = l;
+    if (s->tail_from >= 1) {
+        std::vector<TailLevel> tl;
+        for (int l = s->tail_from; l < nl; l++) {
+            DevLevel& d = s->L[l];
+            TailLevel t{};
+            t.V = d.V;
+            t.nfree = d.nfree;
+            t.ap = d.ap;
+            t.aj = d.aj;
+            t.av = d.av;
+            t.vcl = d.vcl;
+            t.e0 = d.e;
+            t.e1 = d.ep;
+            t.r = d.r;
+            t.rr = d.rr;
+            t.rtp = d.rtp;
+            t.rti = d.rti;
+            t.rtw = d.rtw;
+            t.iids = l + 1 < nl ? s->M[l].iids : nullptr;
+            t.iw = l + 1 < nl ? s->M[l].iw : nullptr;
+            t.dense = d.dense;
+            t.W = d.W;
+            t.rho = (float)d.rho;
+            tl.push_back(t);
+        }
+        if (!s->tail) {
+            int rc = dalloc(&s->tail, 8);
+            if (rc) return rc;
+            s->owned.push_back(s->tail);
+        }
+        HIPCHK(hipMemcpyAsync(s->tail, tl.data(), sizeof(TailLevel) * tl.size(), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
     return MGSB_OK;
 }
 
@@ -1297,6 +1708,21 @@ int alloc_vcycle(mgsb_solver* s) {
         if ((rc = up(s, d.owned, &d.rtp, rtp.data(), rtp.size()))) return rc;
         if ((rc = up(s, d.owned, &d.rti, rti.data(), std::max<size_t>(1, rti.size())))) return rc;
         if ((rc = up(s, d.owned, &d.rtw, rtw.data(), std::max<size_t>(1, rtw.size())))) return rc;
+    }
+    {
+        DevLevel& c1 = s->L[1];
+        if ((rc = up<float>(s, s->owned, &s->feat, nullptr, (size_t)c1.V * kFeat))) return rc;
+        if ((rc = up<float>(s, s->owned, &s->pred, nullptr, (size_t)c1.V * 3))) return rc;
+        std::vector<float> norm(2 * kFeat + 6, 0.f);
+        for (int q = 0; q < kFeat; q++) norm[kFeat + q] = 1.f;
+        norm[2 * kFeat + 3] = norm[2 * kFeat + 4] = norm[2 * kFeat + 5] = 1.f;
+        if ((rc = up(s, s->owned, &s->norm, norm.data(), norm.size()))) return rc;
+        std::vector<float4> z(c1.V, make_float4(0.f, 0.f, 0.f, 0.f)), x1(c1.V);
+        for (int i = 0; i < c1.V; i++) x1[i] = make_float4(c1.h.X[i * 3], c1.h.X[i * 3 + 1], c1.h.X[i * 3 + 2], 0.f);
+        if ((rc = up(s, s->owned, &s->epred, z.data(), c1.V))) return rc;
+        if ((rc = up(s, s->owned, &s->eprev, z.data(), c1.V))) return rc;
+        if ((rc = up(s, s->owned, &s->X1, x1.data(), c1.V))) return rc;
+        if ((rc = mlp_create(&s->mlp, kFeat, 3, 1234u, s->stream))) return rc;
     }
     if ((rc = dalloc(&s->vstate, 1))) return rc;
     s->owned.push_back(s->vstate);
@@ -1387,8 +1813,12 @@ int validate_params(const mgsb_params* p, int nlevels) {
         set_error("V-cycle schedules need at least 2 levels");
         return MGSB_ERR_ARG;
     }
-    if (p->pre_sweeps < 0 || p->post_sweeps < 0 || p->coarse_sweeps < 0 || p->max_cycles < 0) {
-        set_error("sweep and cycle counts must be >= 0");
+    if (p->pre_sweeps < 0 || p->post_sweeps < 0 || p->coarse_sweeps < 1 || p->max_cycles < 0) {
+        set_error("sweep counts must be >= 0 (coarse_sweeps >= 1), max_cycles >= 0");
+        return MGSB_ERR_ARG;
+    }
+    if (nlevels > 8) {
+        set_error("at most 8 levels");
         return MGSB_ERR_ARG;
     }
     if (p->schedule < 0 || p->schedule > MGSB_SCHED_LEARNED) {
@@ -1646,8 +2076,9 @@ int mgsb_step(mgsb_solver* s, int32_t nsteps, mgsb_step_stats* stats) {
     }
     HIPCHK(hipSetDevice(s->device));
     int rc;
-    const bool graphable = s->p.use_graph && !s->timing;
+    const bool graphable = s->p.use_graph && !s->timing && !s->recording;
     for (int st = 0; st < nsteps; st++) {
+        if (s->recording && s->rec_steps >= s->rec_cap) s->recording = false;
         if (graphable) {
             if (!s->graph_valid) {
                 HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
@@ -1831,15 +2262,206 @@ int mgsb_run_stats(mgsb_solver* s, mgsb_run_stats_t* out, int32_t reset) {
     out->total_cycles = v.total_cycles;
     out->steps = v.nsteps;
     out->failed_steps = v.nfail;
+    out->fallbacks = v.nfallback;
     if (reset) {
         v.max_rel = 0.0;
         v.total_cycles = 0;
         v.nsteps = 0;
         v.nfail = 0;
+        v.nfallback = 0;
         HIPCHK(hipMemcpyAsync(s->vstate, &v, sizeof(VState), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
     }
     return MGSB_OK;
+}
+
+static int need_mlp(mgsb_solver* s, const char* what) {
+    if (!s || !s->mlp) {
+        set_error(std::string(what) + ": the solver has no learned correction (needs >= 2 levels)");
+        return MGSB_ERR_STATE;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    return MGSB_OK;
+}
+
+int mgsb_learn_record(mgsb_solver* s, int32_t max_steps) {
+    int rc = need_mlp(s, "mgsb_learn_record");
+    if (rc) return rc;
+    if (max_steps < 0) {
+        set_error("mgsb_learn_record: max_steps < 0");
+        return MGSB_ERR_ARG;
+    }
+    if (max_steps == 0) {
+        s->recording = false;
+        return MGSB_OK;
+    }
+    const int V1 = s->L[1].V;
+    if (max_steps > s->rec_cap) {
+        float *nx = nullptr, *nt = nullptr;
+        if ((rc = dalloc(&nx, (size_t)max_steps * V1 * kFeat))) return rc;
+        if ((rc = dalloc(&nt, (size_t)max_steps * V1 * 3))) {
+            (void)hipFree(nx);
+            return rc;
+        }
+        if (s->rec_steps > 0) {
+            HIPCHK(hipMemcpyAsync(nx, s->recX, sizeof(float) * (size_t)s->rec_steps * V1 * kFeat,
+                                  hipMemcpyDeviceToDevice, s->stream));
+            HIPCHK(hipMemcpyAsync(nt, s->recT, sizeof(float) * (size_t)s->rec_steps * V1 * 3, hipMemcpyDeviceToDevice,
+                                  s->stream));
+            HIPCHK(hipStreamSynchronize(s->stream));
+        }
+        if (s->recX) HIPCHK(hipFree(s->recX));
+        if (s->recT) HIPCHK(hipFree(s->recT));
+        s->recX = nx;
+        s->recT = nt;
+        s->rec_cap = max_steps;
+    }
+    s->recording = true;
+    return invalidate_graph(s);
+}
+
+int mgsb_learn_samples(mgsb_solver* s, int64_t* n) {
+    int rc = need_mlp(s, "mgsb_learn_samples");
+    if (rc) return rc;
+    if (!n) {
+        set_error("mgsb_learn_samples: null output");
+        return MGSB_ERR_ARG;
+    }
+    *n = (int64_t)s->rec_steps * s->L[1].V;
+    return MGSB_OK;
+}
+
+int mgsb_learn_clear(mgsb_solver* s) {
+    int rc = need_mlp(s, "mgsb_learn_clear");
+    if (rc) return rc;
+    s->rec_steps = 0;
+    return MGSB_OK;
+}
+
+int mgsb_learn_train(mgsb_solver* s, int32_t epochs, int32_t batch, float lr, uint32_t seed, float* loss_hist) {
+    int rc = need_mlp(s, "mgsb_learn_train");
+    if (rc) return rc;
+    const long n = (long)s->rec_steps * s->L[1].V;
+    if (n < batch || epochs < 1 || !(lr > 0)) {
+        set_error("mgsb_learn_train: need recorded samples >= batch, epochs >= 1, lr > 0");
+        return MGSB_ERR_ARG;
+    }
+    std::vector<float> X((size_t)n * kFeat), T((size_t)n * 3);
+    HIPCHK(hipMemcpyAsync(X.data(), s->recX, sizeof(float) * X.size(), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(T.data(), s->recT, sizeof(float) * T.size(), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    // per-feature / per-output mean and standard deviation (FP64), then
+    // standardise; constant features get sd 1
+    std::vector<float> norm(2 * kFeat + 6);
+    auto stats = [&](const std::vector<float>& A, int w, int c, float& mu, float& sd) {
+        double m = 0, q = 0;
+        for (long i = 0; i < n; i++) m += A[(size_t)i * w + c];
+        m /= n;
+        for (long i = 0; i < n; i++) {
+            double d = A[(size_t)i * w + c] - m;
+            q += d * d;
+        }
+        q = std::sqrt(q / n);
+        mu = (float)m;
+        sd = q > 1e-12 ? (float)q : 1.f;
+    };
+    for (int c = 0; c < kFeat; c++) stats(X, kFeat, c, norm[c], norm[kFeat + c]);
+    for (int c = 0; c < 3; c++) stats(T, 3, c, norm[2 * kFeat + c], norm[2 * kFeat + 3 + c]);
+    for (long i = 0; i < n; i++) {
+        for (int c = 0; c < kFeat; c++) X[(size_t)i * kFeat + c] = (X[(size_t)i * kFeat + c] - norm[c]) / norm[kFeat + c];
+        for (int c = 0; c < 3; c++)
+            T[(size_t)i * 3 + c] = (T[(size_t)i * 3 + c] - norm[2 * kFeat + c]) / norm[2 * kFeat + 3 + c];
+    }
+    float *dX = nullptr, *dT = nullptr;
+    if ((rc = dalloc(&dX, X.size()))) return rc;
+    if ((rc = dalloc(&dT, T.size()))) {
+        (void)hipFree(dX);
+        return rc;
+    }
+    HIPCHK(hipMemcpyAsync(dX, X.data(), sizeof(float) * X.size(), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(dT, T.data(), sizeof(float) * T.size(), hipMemcpyHostToDevice, s->stream));
+    rc = mlp_train(s->mlp, dX, dT, n, epochs, batch, lr, seed, loss_hist, s->stream);
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipFree(dX);
+    (void)hipFree(dT);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(s->norm, norm.data(), sizeof(float) * norm.size(), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return invalidate_graph(s);
+}
+
+int mgsb_learn_nparams(mgsb_solver* s, int32_t* n) {
+    int rc = need_mlp(s, "mgsb_learn_nparams");
+    if (rc) return rc;
+    if (!n) {
+        set_error("mgsb_learn_nparams: null output");
+        return MGSB_ERR_ARG;
+    }
+    *n = mlp_params_count(s->mlp);
+    return MGSB_OK;
+}
+
+int mgsb_learn_weights(mgsb_solver* s, float* w, int32_t n, int32_t set) {
+    int rc = need_mlp(s, "mgsb_learn_weights");
+    if (rc) return rc;
+    if (!w) {
+        set_error("mgsb_learn_weights: null array");
+        return MGSB_ERR_ARG;
+    }
+    rc = set ? mlp_set_params(s->mlp, w, n, s->stream) : mlp_get_params(s->mlp, w, n, s->stream);
+    if (rc) return rc;
+    return set ? invalidate_graph(s) : MGSB_OK;
+}
+
+int mgsb_learn_norm(mgsb_solver* s, float* norm, int32_t set) {
+    int rc = need_mlp(s, "mgsb_learn_norm");
+    if (rc) return rc;
+    if (!norm) {
+        set_error("mgsb_learn_norm: null array");
+        return MGSB_ERR_ARG;
+    }
+    const size_t nb = sizeof(float) * (2 * kFeat + 6);
+    if (set)
+        HIPCHK(hipMemcpyAsync(s->norm, norm, nb, hipMemcpyHostToDevice, s->stream));
+    else
+        HIPCHK(hipMemcpyAsync(norm, s->norm, nb, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return MGSB_OK;
+}
+
+int mgsb_mlp_forward(mgsb_solver* s, const float* x, int32_t n, float* y, int32_t path) {
+    int rc = need_mlp(s, "mgsb_mlp_forward");
+    if (rc) return rc;
+    if (!x || !y || n < 1) {
+        set_error("mgsb_mlp_forward: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    if (path == 1) return mlp_forward_train_path(s->mlp, x, n, y, s->stream);
+    float *dx = nullptr, *dy = nullptr;
+    if ((rc = dalloc(&dx, (size_t)n * kFeat))) return rc;
+    if ((rc = dalloc(&dy, (size_t)n * 3))) {
+        (void)hipFree(dx);
+        return rc;
+    }
+    HIPCHK(hipMemcpyAsync(dx, x, sizeof(float) * (size_t)n * kFeat, hipMemcpyHostToDevice, s->stream));
+    rc = mlp_infer(s->mlp, dx, n, dy, 3, s->stream, nullptr);
+    if (!rc) {
+        HIPCHK(hipMemcpyAsync(y, dy, sizeof(float) * (size_t)n * 3, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    return rc;
+}
+
+int mgsb_mlp_grad(mgsb_solver* s, const float* x, const float* t, int32_t n, float* grad, double* loss) {
+    int rc = need_mlp(s, "mgsb_mlp_grad");
+    if (rc) return rc;
+    if (!x || !t || !grad || n < 1) {
+        set_error("mgsb_mlp_grad: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    return mlp_grad(s->mlp, x, t, n, grad, loss, s->stream);
 }
 
 int mgsb_level_rho(mgsb_solver* s, int32_t level, double* rho) {

@@ -90,6 +90,8 @@ _SIGS = {
     "ml_stats": (None, [_vp, _dp]),
     "step_vcycle": (None, [_vp, C.POINTER(orc_params), C.POINTER(orc_vparams), C.c_int]),
     "ml_coarse_solve": (None, [_vp, _fp, _fp]),
+    "mlp_forward": (None, [_fp, C.c_int, C.c_int, _fp, C.c_int, _dp, C.c_int]),
+    "mlp_grad": (C.c_double, [_fp, C.c_int, C.c_int, _fp, _fp, C.c_int, _dp, C.c_int]),
 }
 
 
@@ -314,3 +316,30 @@ class MultiLevel:
         out = np.zeros(3, np.float64)
         self._f("ml_stats")(self._h, out.ctypes.data_as(_dp))
         return {"res0": out[0], "res": out[1], "cycles": int(out[2]), "rel": out[1] / out[0] if out[0] else 0.0}
+
+
+MLP_H = 128
+
+
+def mlp_nparams(inp=12, out=3):
+    H = MLP_H
+    return inp * H + H + 2 * (H * H + H) + H * out + out
+
+
+def mlp_forward(w, x, inp=12, out=3, bf16=True):
+    w = np.ascontiguousarray(w, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, inp)
+    y = np.zeros((x.shape[0], out), np.float64)
+    lib.orc64_mlp_forward(w.ctypes.data_as(_fp), inp, out, x.ctypes.data_as(_fp), x.shape[0], y.ctypes.data_as(_dp),
+                          int(bf16))
+    return y
+
+
+def mlp_grad(w, x, t, inp=12, out=3, bf16=True):
+    w = np.ascontiguousarray(w, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, inp)
+    t = np.ascontiguousarray(t, np.float32).reshape(-1, out)
+    g = np.zeros(w.size, np.float64)
+    loss = lib.orc64_mlp_grad(w.ctypes.data_as(_fp), inp, out, x.ctypes.data_as(_fp), t.ctypes.data_as(_fp),
+                              x.shape[0], g.ctypes.data_as(_dp), int(bf16))
+    return g, loss

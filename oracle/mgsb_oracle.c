@@ -1489,3 +1489,133 @@ void PFX(ml_coarse_solve)(orc_ml* M, const float* r, float* e) {
     free(rr);
     free(ee);
 }
+
+/* ========================================================================= */
+/* MLP of the learned coarse correction: in -> H -> H -> H -> out, ReLU.     */
+/* FP64 accumulation; with bf16 != 0 every GEMM operand is rounded to BF16    */
+/* (nearest even, via float) where the device kernels round it.  Parameter    */
+/* layout: W1[in][H] b1[H] W2[H][H] b2 W3[H][H] b3 W4[H][out] b4[out].         */
+/* ========================================================================= */
+#define ORC_MLP_H 128
+
+static double orc_bf(double v, int on) {
+    if (!on) return v;
+    float f = (float)v;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+typedef struct {
+    int in, out, off[8];
+} orc_mlp_shape;
+
+static orc_mlp_shape orc_mlp_layout(int in, int out) {
+    orc_mlp_shape s;
+    s.in = in;
+    s.out = out;
+    int sizes[8] = {in * ORC_MLP_H, ORC_MLP_H, ORC_MLP_H * ORC_MLP_H, ORC_MLP_H, ORC_MLP_H * ORC_MLP_H, ORC_MLP_H,
+                    ORC_MLP_H * out, out};
+    int o = 0;
+    for (int i = 0; i < 8; i++) {
+        s.off[i] = o;
+        o += sizes[i];
+    }
+    return s;
+}
+
+/* Z[n][N] = bf(A)[n][K] bf(W)[K][N] + b */
+static void orc_dense(const double* A, int n, int K, const float* W, const float* b, int N, double* Z, int bf) {
+    for (int r = 0; r < n; r++)
+        for (int j = 0; j < N; j++) {
+            double acc = 0;
+            for (int k = 0; k < K; k++) acc += orc_bf(A[(size_t)r * K + k], bf) * orc_bf(W[(size_t)k * N + j], bf);
+            Z[(size_t)r * N + j] = acc + b[j];
+        }
+}
+
+void PFX(mlp_forward)(const float* w, int in, int out, const float* x, int n, double* y, int bf) {
+    orc_mlp_shape s = orc_mlp_layout(in, out);
+    int H = ORC_MLP_H;
+    double* X = (double*)xcalloc((size_t)n * in, sizeof(double));
+    double* A = (double*)xcalloc((size_t)n * H, sizeof(double));
+    double* B = (double*)xcalloc((size_t)n * H, sizeof(double));
+    for (size_t i = 0; i < (size_t)n * in; i++) X[i] = x[i];
+    orc_dense(X, n, in, w + s.off[0], w + s.off[1], H, A, bf);
+    for (size_t i = 0; i < (size_t)n * H; i++) A[i] = A[i] > 0 ? A[i] : 0;
+    orc_dense(A, n, H, w + s.off[2], w + s.off[3], H, B, bf);
+    for (size_t i = 0; i < (size_t)n * H; i++) B[i] = B[i] > 0 ? B[i] : 0;
+    orc_dense(B, n, H, w + s.off[4], w + s.off[5], H, A, bf);
+    for (size_t i = 0; i < (size_t)n * H; i++) A[i] = A[i] > 0 ? A[i] : 0;
+    orc_dense(A, n, H, w + s.off[6], w + s.off[7], out, y, bf);
+    free(X); free(A); free(B);
+}
+
+/* Mean squared error over n*out and its gradient w.r.t. the parameters. */
+double PFX(mlp_grad)(const float* w, int in, int out, const float* x, const float* t, int n, double* grad, int bf) {
+    orc_mlp_shape s = orc_mlp_layout(in, out);
+    int H = ORC_MLP_H;
+    size_t nh = (size_t)n * H;
+    double* X = (double*)xcalloc((size_t)n * in, sizeof(double));
+    double *Z1 = (double*)xcalloc(nh, 8), *H1 = (double*)xcalloc(nh, 8), *Z2 = (double*)xcalloc(nh, 8),
+           *H2 = (double*)xcalloc(nh, 8), *Z3 = (double*)xcalloc(nh, 8), *H3 = (double*)xcalloc(nh, 8);
+    double* Y = (double*)xcalloc((size_t)n * out, 8);
+    double* dY = (double*)xcalloc((size_t)n * out, 8);
+    double *dA = (double*)xcalloc(nh, 8), *dB = (double*)xcalloc(nh, 8);
+    for (size_t i = 0; i < (size_t)n * in; i++) X[i] = x[i];
+    orc_dense(X, n, in, w + s.off[0], w + s.off[1], H, Z1, bf);
+    for (size_t i = 0; i < nh; i++) H1[i] = Z1[i] > 0 ? Z1[i] : 0;
+    orc_dense(H1, n, H, w + s.off[2], w + s.off[3], H, Z2, bf);
+    for (size_t i = 0; i < nh; i++) H2[i] = Z2[i] > 0 ? Z2[i] : 0;
+    orc_dense(H2, n, H, w + s.off[4], w + s.off[5], H, Z3, bf);
+    for (size_t i = 0; i < nh; i++) H3[i] = Z3[i] > 0 ? Z3[i] : 0;
+    orc_dense(H3, n, H, w + s.off[6], w + s.off[7], out, Y, bf);
+    double loss = 0;
+    size_t ny = (size_t)n * out;
+    for (size_t i = 0; i < ny; i++) {
+        double d = Y[i] - t[i];
+        loss += d * d;
+        dY[i] = 2 * d / (double)ny;
+    }
+    loss /= (double)ny;
+    /* one layer back: dW = bf(Hin)^T bf(dZ), db = sum dZ, dIn = bf(dZ) bf(W)^T */
+#define ORC_BACK(Hin, K, dZ, N, Wo, bo, dIn)                                                            \
+    do {                                                                                               \
+        for (int k = 0; k < (K); k++)                                                                  \
+            for (int j = 0; j < (N); j++) {                                                            \
+                double a = 0;                                                                          \
+                for (int r = 0; r < n; r++)                                                            \
+                    a += orc_bf((Hin)[(size_t)r * (K) + k], bf) * orc_bf((dZ)[(size_t)r * (N) + j], bf); \
+                grad[(Wo) + (size_t)k * (N) + j] = a;                                                  \
+            }                                                                                          \
+        for (int j = 0; j < (N); j++) {                                                                \
+            double a = 0;                                                                              \
+            for (int r = 0; r < n; r++) a += (dZ)[(size_t)r * (N) + j];                                \
+            grad[(bo) + j] = a;                                                                        \
+        }                                                                                              \
+        if (dIn)                                                                                       \
+            for (int r = 0; r < n; r++)                                                                \
+                for (int k = 0; k < (K); k++) {                                                        \
+                    double a = 0;                                                                      \
+                    for (int j = 0; j < (N); j++)                                                      \
+                        a += orc_bf((dZ)[(size_t)r * (N) + j], bf) * orc_bf(w[(Wo) + (size_t)k * (N) + j], bf); \
+                    ((double*)(dIn))[(size_t)r * (K) + k] = a;                                         \
+                }                                                                                      \
+    } while (0)
+    ORC_BACK(H3, H, dY, out, s.off[6], s.off[7], dA);
+    for (size_t i = 0; i < nh; i++)
+        if (!(Z3[i] > 0)) dA[i] = 0;
+    ORC_BACK(H2, H, dA, H, s.off[4], s.off[5], dB);
+    for (size_t i = 0; i < nh; i++)
+        if (!(Z2[i] > 0)) dB[i] = 0;
+    ORC_BACK(H1, H, dB, H, s.off[2], s.off[3], dA);
+    for (size_t i = 0; i < nh; i++)
+        if (!(Z1[i] > 0)) dA[i] = 0;
+    ORC_BACK(X, in, dA, H, s.off[0], s.off[1], (double*)0);
+#undef ORC_BACK
+    free(X); free(Z1); free(H1); free(Z2); free(H2); free(Z3); free(H3); free(Y); free(dY); free(dA); free(dB);
+    return loss;
+}

@@ -82,7 +82,12 @@ def test_single_level_reference_constants_msh_with_sphere(golden):
             oflags = L.get(O.CFLAG)
             odepth = L.get(O.CDEPTH)
     report("single Y_4_40_4 + sphere", gpu, *outs)
-    assert flags.sum() > 0, "sphere never touched the beam"
+    # the collision changed the trajectory (compare with no sphere)
+    L = O.Level(mesh.pos, mesh.tets, mesh.tris)
+    L.set_fix(fix)
+    L.step(O.params(), 60)
+    assert np.abs(outs[0] - L.positions()).max() > 0.05, "sphere never touched the beam"
+    # flags / depths of the last iteration (t > r/2 criterion, PDSolver.cu:301-304)
     assert np.array_equal(flags, oflags.astype(np.int32))
     assert np.abs(depth - odepth).max() < 1e-4
     assert rel_err(gpu, outs[0]) < REL_TOL

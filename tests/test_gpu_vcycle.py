@@ -35,14 +35,14 @@ def make_pair(ns, steps_kw, fp64=False, gravity=True, **vkw):
     return lv, s, ML, Ls, p
 
 
-@pytest.mark.parametrize("ns", [(4, 1), (8, 4, 1)])
-def test_vcycle_fixed_cycles_matches_oracle(ns):
-    kw = dict(pre_sweeps=2, post_sweeps=2, coarse_sweeps=2, max_cycles=3, check_every=0)
+@pytest.mark.parametrize("ns,rs", [((4, 1), 0.0), ((8, 4, 1), 0.9)])
+def test_vcycle_fixed_cycles_matches_oracle(ns, rs):
+    kw = dict(pre_sweeps=2, post_sweeps=2, coarse_sweeps=2, max_cycles=3, check_every=0, rho_smooth=rs)
     lv, s, ML, Ls, p = make_pair(ns, None, **kw)
     for l in range(1, len(ns)):
         assert abs(s.level_rho(l) - ML.rho(l)) < 1e-4 * ML.rho(l)
     assert s.coarse_nfree() == ML.nfree()
-    vp = O.vparams(pre=2, post=2, coarse_sweeps=2, max_cycles=3, check=0)
+    vp = O.vparams(pre=2, post=2, coarse_sweeps=2, max_cycles=3, check=0, rho_smooth=rs)
     s.step(60)
     ML.step(p, vp, 60)
     g, o = s.positions(), Ls[0].positions()
@@ -55,9 +55,9 @@ def test_vcycle_fixed_cycles_matches_oracle(ns):
 
 
 def test_vcycle_adaptive_meets_target_each_step():
-    kw = dict(pre_sweeps=2, post_sweeps=2, coarse_sweeps=2, max_cycles=12, check_every=1, tol=1e-3)
+    kw = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, check_every=1, tol=1e-3, rho_smooth=0.9)
     lv, s, ML, Ls, p = make_pair((4, 1), None, **kw)
-    vp = O.vparams(pre=2, post=2, coarse_sweeps=2, max_cycles=12, check=1, tol=1e-3)
+    vp = O.vparams(pre=5, post=5, coarse_sweeps=2, max_cycles=12, check=1, tol=1e-3, rho_smooth=0.9)
     gc, oc = [], []
     for _ in range(80):
         st = s.step(1)
