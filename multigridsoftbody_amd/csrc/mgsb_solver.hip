@@ -89,7 +89,8 @@ __device__ __forceinline__ void gather_sum(const float4* __restrict__ cf, int e0
 __global__ void __launch_bounds__(256) k_st(int V, float4* __restrict__ x, float4* __restrict__ v,
                                             float4* __restrict__ xlast, float4* __restrict__ s,
                                             float4* __restrict__ xprev, const float4* __restrict__ vc, float dt,
-                                            float damp, float gx, float gy, float gz, DevColliders* col, int advance) {
+                                            float damp, float gx, float gy, float gz, DevColliders* col, int advance,
+                                            int split) {
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (advance && blockIdx.x == 0 && threadIdx.x == 0) {
         for (int q = 0; q < col->nsph; q++) {
@@ -116,9 +117,28 @@ __global__ void __launch_bounds__(256) k_st(int V, float4* __restrict__ x, float
     p.y = p.y + u.y * dt * fix;
     p.z = p.z + u.z * dt * fix;
     v[i] = u;
-    x[i] = p;
     s[i] = p;
-    xprev[i] = p;
+    if (split) {  // x and x_prev hold the in-step correction u = x - s
+        x[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        xprev[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+        x[i] = p;
+        xprev[i] = p;
+    }
+}
+
+// End of a split-position step: x = s + u (and x_prev = s + u_prev), then
+// calculateV.
+__global__ void __launch_bounds__(256) k_finish_split(int V, float4* __restrict__ x, float4* __restrict__ xprev,
+                                                      const float4* __restrict__ s, const float4* __restrict__ xlast,
+                                                      float4* __restrict__ v, float dt) {
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const float4 u = x[i], up = xprev[i], st = s[i], q = xlast[i];
+    const float4 p = make_float4(st.x + u.x, st.y + u.y, st.z + u.z, 0.f);
+    x[i] = p;
+    xprev[i] = make_float4(st.x + up.x, st.y + up.y, st.z + up.z, 0.f);
+    v[i] = make_float4((p.x - q.x) / dt, (p.y - q.y) / dt, (p.z - q.z) / dt, 0.f);
 }
 
 // calculateV (PDSolver.cu:245-252).
@@ -187,7 +207,7 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
                                              const int4* __restrict__ slot, float4* __restrict__ cf, float k,
                                              int ntri, const int4* __restrict__ tri, const int4* __restrict__ tslot,
                                              float4* __restrict__ tcf, const DevColliders* __restrict__ col,
-                                             const int* __restrict__ skip) {
+                                             const int* __restrict__ skip, float* __restrict__ Fs, int fs_mode) {
     if (skip && *skip) return;
     const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (g >= T) {
@@ -230,6 +250,15 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
     for (int r = 0; r < 3; r++)
 #pragma unroll
         for (int c = 0; c < 3; c++) F[r * 3 + c] = D[r * 3 + 0] * b[c] + D[r * 3 + 1] * b[3 + c] + D[r * 3 + 2] * b[6 + c];
+    // Split positions x = s + u (V-cycle steps): fs_mode 1 -- x holds s, F(s)
+    // is stored; fs_mode 2 -- x holds u and F = F(s) + Ds(u) Dm^-1.
+    if (fs_mode == 1) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) Fs[(size_t)q * T + g] = F[q];
+    } else if (fs_mode == 2) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) F[q] = Fs[(size_t)q * T + g] + F[q];
+    }
     float RF[9];
     if (MODE == kTetModeLinear) {
 #pragma unroll
@@ -281,13 +310,17 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
                                               const int* __restrict__ trp, const float4* __restrict__ tcf,
                                               const DevColliders* __restrict__ col, float omega, float relax,
                                               int flags, int* __restrict__ cflag, float* __restrict__ cdepth,
-                                              float* __restrict__ cenergy, const int* __restrict__ skip) {
+                                              float* __restrict__ cenergy, const int* __restrict__ skip, int split) {
     if (skip && *skip) return;
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= V) return;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     gather_sum(cf, rp[i], rp[i + 1], fx, fy, fz);
-    const float4 p = x[i];
+    const float4 p = x[i];  // x, or the correction u when split
+    const float4 st = s[i];
+    // position seen by the colliders, and s - x of the inertial term
+    const float4 pa = split ? make_float4(st.x + p.x, st.y + p.y, st.z + p.z, 0.f) : p;
+    const float sx = split ? -p.x : st.x - p.x, sy = split ? -p.y : st.y - p.y, sz = split ? -p.z : st.z - p.z;
     float cdx = 0.f, cdy = 0.f, cdz = 0.f, cfx = 0.f, cfy = 0.f, cfz = 0.f;
     int flag = 0;
     float depth = 0.f, energy = 0.f;
@@ -308,7 +341,7 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
     }
     if (flags & kVertPoint) {
         const float kc = col->kcol;
-        const float3 P = f3(p);
+        const float3 P = f3(pa);
         for (int q = 0; q < col->nsph; q++) {
             if (col->vel[q].w == 0.f) continue;
             const float4 sp = col->sph[q];
@@ -367,10 +400,10 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
     }
     const float4 c = vc[i];  // (m/h^2, diag, fix, m)
     const float dc = c.x, dg = c.y, fix = c.z;
-    const float4 st = s[i], pp = xprev[i];
-    float nx = (dc * (st.x - p.x) + (fx + cfx)) / (dg + cdx + dc) * fix + p.x;
-    float ny = (dc * (st.y - p.y) + (fy + cfy)) / (dg + cdy + dc) * fix + p.y;
-    float nz = (dc * (st.z - p.z) + (fz + cfz)) / (dg + cdz + dc) * fix + p.z;
+    const float4 pp = xprev[i];
+    float nx = (dc * sx + (fx + cfx)) / (dg + cdx + dc) * fix + p.x;
+    float ny = (dc * sy + (fy + cfy)) / (dg + cdy + dc) * fix + p.y;
+    float nz = (dc * sz + (fz + cfz)) / (dg + cdz + dc) * fix + p.z;
     nx = (nx - p.x) * relax + p.x;
     ny = (ny - p.y) * relax + p.y;
     nz = (nz - p.z) * relax + p.z;
@@ -537,7 +570,7 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
                                                const float4* __restrict__ vc, const int* __restrict__ rp,
                                                const float4* __restrict__ cf, const DevColliders* __restrict__ col,
                                                int flags, float4* __restrict__ r, double* __restrict__ part,
-                                               float4* __restrict__ xprev, const int* __restrict__ skip) {
+                                               float4* __restrict__ xprev, const int* __restrict__ skip, int split) {
     if (skip && *skip) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
@@ -545,10 +578,13 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
         float fx = 0.f, fy = 0.f, fz = 0.f;
         gather_sum(cf, rp[i], rp[i + 1], fx, fy, fz);
         const float4 p = x[i];
+        const float4 st = s[i];
+        const float4 pa = split ? make_float4(st.x + p.x, st.y + p.y, st.z + p.z, 0.f) : p;
+        const float sx = split ? -p.x : st.x - p.x, sy = split ? -p.y : st.y - p.y, sz = split ? -p.z : st.z - p.z;
         float cfx = 0.f, cfy = 0.f, cfz = 0.f;
         if (flags & kVertPoint) {
             const float kc = col->kcol;
-            const float3 P = f3(p);
+            const float3 P = f3(pa);
             for (int q = 0; q < col->nsph; q++) {
                 if (col->vel[q].w == 0.f) continue;
                 const float4 sp = col->sph[q];
@@ -573,7 +609,7 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
             }
             if (col->plane_on) {
                 const float4 pl = col->plane;
-                const float sd = pl.x * p.x + pl.y * p.y + pl.z * p.z - pl.w;
+                const float sd = pl.x * pa.x + pl.y * pa.y + pl.z * pa.z - pl.w;
                 if (sd < 0.f) {
                     cfx = cfx + col->kplane * (-sd) * pl.x;
                     cfy = cfy + col->kplane * (-sd) * pl.y;
@@ -581,11 +617,11 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
                 }
             }
         }
-        const float4 c = vc[i], st = s[i];
+        const float4 c = vc[i];
         const float fix = c.z;
-        const float gx = (c.x * (st.x - p.x) + (fx + cfx)) * fix;
-        const float gy = (c.x * (st.y - p.y) + (fy + cfy)) * fix;
-        const float gz = (c.x * (st.z - p.z) + (fz + cfz)) * fix;
+        const float gx = (c.x * sx + (fx + cfx)) * fix;
+        const float gy = (c.x * sy + (fy + cfy)) * fix;
+        const float gz = (c.x * sz + (fz + cfz)) * fix;
         if (r) r[i] = make_float4(gx, gy, gz, 0.f);
         if (xprev) xprev[i] = p;
         v[0] = (double)gx * gx + (double)gy * gy + (double)gz * gz;
@@ -633,7 +669,7 @@ __global__ void __launch_bounds__(256) k_revert(int V, float4* __restrict__ x, f
     if (!st->fallback) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= V) return;
-    const float4 q = s[i];
+    const float4 q = make_float4(0.f, 0.f, 0.f, 0.f);  // split positions: u = 0 is x = s
     x[i] = q;
     xprev[i] = q;
 }
@@ -692,8 +728,8 @@ __global__ void __launch_bounds__(256) k_target(int Vc, const float4* __restrict
     const int ids[4] = {id.x, id.y, id.z, id.w};
     const float ws[4] = {w.x, w.y, w.z, w.w};
     float dx = 0.f, dy = 0.f, dz = 0.f;
-    for (int q = 0; q < 4; q++) {
-        const float4 a = x[ids[q]], b = s[ids[q]];
+    for (int q = 0; q < 4; q++) {  // s == nullptr: x already holds u = x - s
+        const float4 a = x[ids[q]], b = s ? s[ids[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
         dx = dx + ws[q] * (a.x - b.x);
         dy = dy + ws[q] * (a.y - b.y);
         dz = dz + ws[q] * (a.z - b.z);
@@ -1146,6 +1182,7 @@ struct DevLevel {
     int *ap = nullptr, *aj = nullptr;  // scalar level matrix A = m/h^2 + L (coarse levels)
     float* av = nullptr;
     std::vector<float> adiag;          // its diagonal
+    float* Fs = nullptr;               // F(s) per tet (9 SoA streams), split-position steps
     int cur = 0;                       // which of e / ep holds the correction
     int* dense = nullptr;
     double* W = nullptr;
@@ -1334,7 +1371,7 @@ int timed_end(mgsb_solver* s, int kernel) {
 }
 
 int launch_tet(mgsb_solver* s, DevLevel& d, int mode, bool with_tri, const float4* xsrc = nullptr,
-               const int* skip = nullptr) {
+               const int* skip = nullptr, int fs_mode = 0) {
     const int bs = block_tet(s);
     const int ntri = with_tri ? d.S : 0;
     const int n = d.T + ntri;
@@ -1347,19 +1384,19 @@ int launch_tet(mgsb_solver* s, DevLevel& d, int mode, bool with_tri, const float
     switch (mode) {
         case kTetModePolar:
             k_tet<kTetModePolar><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness,
-                                                           ntri, d.tri, d.tslot, d.tcf, s->dcol, skip);
+                                                           ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, d.Fs, fs_mode);
             break;
         case kTetModeAc:
             k_tet<kTetModeAc><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness, ntri,
-                                                        d.tri, d.tslot, d.tcf, s->dcol, skip);
+                                                        d.tri, d.tslot, d.tcf, s->dcol, skip, d.Fs, fs_mode);
             break;
         case kTetModeStable:
             k_tet<kTetModeStable><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness,
-                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip);
+                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, d.Fs, fs_mode);
             break;
         default:
             k_tet<kTetModeLinear><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness,
-                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip);
+                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, d.Fs, fs_mode);
     }
     HIPCHK(hipGetLastError());
     return timed_end(s, kid);
@@ -1370,18 +1407,19 @@ int tet_mode_of(int if_mode) {
     return if_mode == MGSB_IF_AC ? kTetModeAc : (if_mode == MGSB_IF_STABLE ? kTetModeStable : kTetModePolar);
 }
 
-int launch_vert(mgsb_solver* s, DevLevel& d, float omega, int flags, const int* skip = nullptr) {
+int launch_vert(mgsb_solver* s, DevLevel& d, float omega, int flags, const int* skip = nullptr, int split = 0) {
     const int bs = block_vert(s);
     k_vert<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.x, d.xprev, d.s, d.vc, d.rp, d.cf, d.trp, d.tcf, s->dcol, omega,
-                                                 s->p.relax, flags, d.cflag, d.cdepth, d.cenergy, skip);
+                                                 s->p.relax, flags, d.cflag, d.cdepth, d.cenergy, skip, split);
     HIPCHK(hipGetLastError());
     return MGSB_OK;
 }
 
-int launch_st(mgsb_solver* s, DevLevel& d, int advance) {
+int launch_st(mgsb_solver* s, DevLevel& d, int advance, int split = 0) {
     const int bs = block_vert(s);
     k_st<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.x, d.v, d.xlast, d.s, d.xprev, d.vc, s->p.dt, s->p.damping,
-                                               s->p.gravity[0], s->p.gravity[1], s->p.gravity[2], s->dcol, advance);
+                                               s->p.gravity[0], s->p.gravity[1], s->p.gravity[2], s->dcol, advance,
+                                               split);
     HIPCHK(hipGetLastError());
     return MGSB_OK;
 }
@@ -1505,13 +1543,18 @@ int vcycle_lin(mgsb_solver* s, int l, const int* skip) {
     return lin_smooth(s, l, s->p.coarse_sweeps, false, skip);
 }
 
-int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, bool norm, bool restart, const int* skip) {
+// Fine-level residual on split positions (x holds u = x - s).  first: the
+// step's first evaluation, which also stores F(s).
+int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, bool norm, bool restart, const int* skip,
+                  bool first = false) {
     DevLevel& d = s->L[0];
     int rc;
-    if (need_tet && (rc = launch_tet(s, d, tet_mode_of(s->p.if_mode), false, nullptr, skip))) return rc;
+    if (need_tet && (rc = launch_tet(s, d, tet_mode_of(s->p.if_mode), false, first ? d.s : d.x, skip,
+                                     first ? 1 : 2)))
+        return rc;
     const int nb = nblk(d.V, 256);
     k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, write_r ? d.r : nullptr,
-                                       norm ? s->vpart : nullptr, restart ? d.xprev : nullptr, skip);
+                                       norm ? s->vpart : nullptr, restart ? d.xprev : nullptr, skip, 1);
     HIPCHK(hipGetLastError());
     return MGSB_OK;
 }
@@ -1528,8 +1571,8 @@ int enqueue_step_vcycle(mgsb_solver* s, bool learned) {
     float* recX = s->recording ? s->recX + (size_t)s->rec_steps * c1.V * kFeat : nullptr;
     float* recT = s->recording ? s->recT + (size_t)s->rec_steps * c1.V * 3 : nullptr;
     int rc;
-    if ((rc = launch_st(s, f, 1))) return rc;
-    if ((rc = fine_residual(s, true, features, true, true, nullptr))) return rc;
+    if ((rc = launch_st(s, f, 1, 1))) return rc;
+    if ((rc = fine_residual(s, true, features, true, true, nullptr, true))) return rc;
     k_check<<<1, 256, 0, s->stream>>>(nbv, s->vpart, s->vstate, 0);
     HIPCHK(hipGetLastError());
     if (features) {
@@ -1550,15 +1593,15 @@ int enqueue_step_vcycle(mgsb_solver* s, bool learned) {
         k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.s, s->vstate);
         HIPCHK(hipGetLastError());
         // cf may hold forces at the rejected warm start: recompute at x
-        if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip))) return rc;
+        if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, 2))) return rc;
     }
     const float rs = s->p.rho_smooth;
     for (int c = 0; c < s->p.max_cycles; c++) {
         float omega = 1.0f;
         for (int k = 0; k < s->p.pre_sweeps; k++) {
             omega = cheb_next(omega, rs);
-            if (k > 0 && (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip))) return rc;
-            if ((rc = launch_vert(s, f, omega, kVertPoint, skip))) return rc;
+            if (k > 0 && (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, 2))) return rc;
+            if ((rc = launch_vert(s, f, omega, kVertPoint, skip, 1))) return rc;
         }
         if ((rc = fine_residual(s, s->p.pre_sweeps > 0, true, false, false, skip))) return rc;
         DevLevel& c1 = s->L[1];
@@ -1572,8 +1615,9 @@ int enqueue_step_vcycle(mgsb_solver* s, bool learned) {
         omega = 1.0f;
         for (int k = 0; k < s->p.post_sweeps; k++) {
             omega = cheb_next(omega, rs);
-            if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip))) return rc;
-            if ((rc = launch_vert(s, f, omega, kVertPoint | (k == s->p.post_sweeps - 1 ? kVertWriteCol : 0), skip)))
+            if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, 2))) return rc;
+            if ((rc = launch_vert(s, f, omega, kVertPoint | (k == s->p.post_sweeps - 1 ? kVertWriteCol : 0), skip,
+                                  1)))
                 return rc;
         }
         if ((rc = fine_residual(s, true, false, true, true, skip))) return rc;
@@ -1583,11 +1627,13 @@ int enqueue_step_vcycle(mgsb_solver* s, bool learned) {
     k_step_done<<<1, 64, 0, s->stream>>>(s->vstate);
     HIPCHK(hipGetLastError());
     if (features || s->eprev) {
-        k_target<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, f.x, f.s, s->M[0].aids, s->M[0].aw, s->eprev, recT);
+        k_target<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, f.x, nullptr, s->M[0].aids, s->M[0].aw, s->eprev, recT);
         HIPCHK(hipGetLastError());
     }
     if (s->recording) s->rec_steps++;
-    return launch_vel(s, f);
+    k_finish_split<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, f.s, f.xlast, f.v, s->p.dt);
+    HIPCHK(hipGetLastError());
+    return MGSB_OK;
 }
 
 int enqueue_step(mgsb_solver* s) {
@@ -1696,7 +1742,10 @@ int alloc_vcycle(mgsb_solver* s) {
     for (int l = 0; l < nl; l++) {
         DevLevel& d = s->L[l];
         if ((rc = up<float4>(s, d.owned, &d.r, nullptr, d.V))) return rc;
-        if (l == 0) continue;
+        if (l == 0) {
+            if ((rc = up<float>(s, d.owned, &d.Fs, nullptr, (size_t)d.T * 9))) return rc;
+            continue;
+        }
         if ((rc = up<float4>(s, d.owned, &d.e, nullptr, d.V))) return rc;
         if ((rc = up<float4>(s, d.owned, &d.ep, nullptr, d.V))) return rc;
         if ((rc = up<float4>(s, d.owned, &d.rr, nullptr, d.V))) return rc;
@@ -1760,7 +1809,7 @@ int residual_norm(mgsb_solver* s, double* out) {
     const int nb = nblk(d.V, 256);
     if ((rc = ensure_partials(s, (size_t)nb * 4))) return rc;
     k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, nullptr, s->dpart,
-                                       nullptr, nullptr);
+                                       nullptr, nullptr, 0);
     HIPCHK(hipGetLastError());
     double sq = 0;
     if ((rc = reduce_finish(s, nb, 1, &sq))) return rc;

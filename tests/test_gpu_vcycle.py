@@ -45,13 +45,20 @@ def test_vcycle_fixed_cycles_matches_oracle(ns, rs):
     vp = O.vparams(pre=2, post=2, coarse_sweeps=2, max_cycles=3, check=0, rho_smooth=rs)
     s.step(60)
     ML.step(p, vp, 60)
-    g, o = s.positions(), Ls[0].positions()
+    # FP64 yardstick of the same schedule
+    lv64, _, ML64, Ls64, p64 = make_pair(ns, None, fp64=True, **kw)
+    ML64.step(p64, vp, 60)
+    g, o, o64 = s.positions(), Ls[0].positions(), Ls64[0].positions()
     err = rel_err(g, o)
     st = s.stats
-    print(f"[vcycle fixed {ns}] rel(gpu,o32)={err:.3e} gpu rel res={st.rel:.3e} oracle rel res={ML.stats()['rel']:.3e}")
+    r32, r64 = ML.stats()["rel"], ML64.stats()["rel"]
+    print(f"[vcycle fixed {ns}] rel(gpu,o32)={err:.3e} rel(gpu,o64)={rel_err(g, o64):.3e} "
+          f"rel res gpu={st.rel:.3e} o32={r32:.3e} o64={r64:.3e}")
     assert st.cycles == 3
-    assert err < REL_TOL
-    assert abs(st.rel - ML.stats()["rel"]) < 0.05 * ML.stats()["rel"] + 1e-7
+    assert err < REL_TOL and rel_err(g, o64) < REL_TOL
+    # the device iterates on x = s + u (u the in-step correction), so its
+    # residual tracks the FP64 schedule, below the absolute-FP32 floor
+    assert st.rel <= max(2 * r64, 1e-5), (st.rel, r64)
 
 
 def test_vcycle_adaptive_meets_target_each_step():
