@@ -1,18 +1,26 @@
 """Benchmark of the multigrid PD solver on one MI355X.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload medium]
-                [--schedule vcycle|single|twolevel] [--fixed-cycles N] [--no-cpu]
+                [--schedule ab|vcycle|temporal|learned|single|twolevel]
+                [--fixed-cycles N] [--no-cpu]
 
-One step = one simulation step of the whole body: inertial prediction, then
-V-cycles until the relative residual ||g(x)|| / ||g(s)|| <= 1e-3 (FP64 norms,
-checked on device after every cycle), then the velocity update.  Inputs are
-resident in HBM; the timed region is K back-to-back steps (one hipGraph per
-step) bracketed by stream synchronisations.  Every timed step's final residual
-is folded into device-side run statistics and reported (max_rel, failed).
+One step = one simulation step of the whole body: inertial prediction, a warm
+start, V-cycles until the relative residual ||g(x)|| / ||g(s)|| <= 1e-3 (FP64
+norms, checked on device after every half-cycle), then the velocity update.
+Inputs are resident in HBM; the timed region is K back-to-back steps bracketed
+by stream synchronisations.  Every timed step's final residual is folded into
+device-side run statistics and reported (max_rel, steps over tolerance).
+
+Schedules: vcycle = plain MG, each step starting from the inertial prediction
+s; temporal = the same V-cycle started from the temporal extrapolation of the
+last two steps' corrections; learned = temporal start plus the MLP's coarse
+correction (trained first, untimed, on 2,000 recorded steps of a trajectory
+with a different initial velocity).  The default "ab" times all three on the
+same workload and reports the learned schedule as the value.
 
 Prints ONE JSON line.  The CPU baseline is the oracle (C restatement, FP32,
-OpenMP) running the same V-cycle schedule on the same workload on this host,
-timed over a bounded number of steps.
+OpenMP) running the plain V-cycle on the same workload on this host, timed
+over a bounded number of steps.
 """
 import argparse
 import json
@@ -28,6 +36,7 @@ METRIC = "sim steps/sec at rel. residual ≤1e-3, learned vs plain MG; achieved 
 HBM_PEAK_GBS = 8000.0
 VCYCLE = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, tol=1e-3, relax_coarse=0.6,
               rho_smooth=0.9, coarse_direct=1)
+TRAIN = dict(record_steps=2000, epochs=10, batch=4096, lr=2e-3, seed=0, velocity_seed=1)
 
 
 def log(*a):
@@ -41,18 +50,20 @@ def build_workload(name):
     return cfg, W.beam_hierarchy(cfg["ns"])
 
 
-def make_solver(levels, schedule, cfg, use_graph=1, fixed_cycles=0):
+def make_solver(levels, schedule, cfg, use_graph=1, fixed_cycles=0, check_every=2, velocity_seed=0):
     import multigridsoftbody_amd as M
     from multigridsoftbody_amd import workloads as W
 
     phys = W.physics(gravity=(0.0, -9.8, 0.0) if cfg["gravity"] else (0.0, 0.0, 0.0))
-    if schedule == "vcycle":
+    if schedule in ("vcycle", "temporal", "learned"):
         kw = dict(VCYCLE)
         if fixed_cycles:
             kw.update(max_cycles=fixed_cycles, check_every=0)
         else:
-            kw.update(check_every=1)
-        s = M.Solver(levels, schedule=M.SCHED_VCYCLE, if_mode=M.IF_STABLE, use_graph=use_graph, **kw, **phys)
+            kw.update(check_every=check_every if schedule != "vcycle" else 1)
+        sched = M.SCHED_LEARNED if schedule == "learned" else M.SCHED_VCYCLE
+        s = M.Solver(levels, schedule=sched, if_mode=M.IF_STABLE, use_graph=use_graph,
+                     warm_start=0 if schedule == "vcycle" else 1, **kw, **phys)
     elif schedule == "twolevel":
         s = M.Solver([levels[0], levels[-1]], schedule=M.SCHED_TWOLEVEL, if_mode=M.IF_PLAIN, iters=cfg["iters"],
                      iters_coarse=cfg["iters"], use_graph=use_graph, **phys)
@@ -61,13 +72,42 @@ def make_solver(levels, schedule, cfg, use_graph=1, fixed_cycles=0):
                      **phys)
     s.fix_plane(1, 10.0, 1e-5)
     fix = W.top_fixflag(levels[0])
-    s.set_velocity(W.seeded_velocity(levels[0], fix))
+    s.set_velocity(W.seeded_velocity(levels[0], fix, seed=velocity_seed))
     if cfg["name"].startswith("large"):
         c = W.large_colliders()
         s.set_sphere(0, c["sphere"]["center"], c["sphere"]["radius"], c["sphere"]["vel"])
         s.set_params(kcol=c["k"])
         s.set_plane(c["plane"]["normal"], c["plane"]["d"], c["k"])
     return s
+
+
+def train_network(levels, cfg, check_every):
+    """Record TRAIN['record_steps'] steps of the temporal-start V-cycle on a
+    trajectory with another initial velocity, train the MLP on the device."""
+    s = make_solver(levels, "temporal", cfg, check_every=check_every, velocity_seed=TRAIN["velocity_seed"])
+    s.learn_record(TRAIN["record_steps"])
+    t0 = time.perf_counter()
+    s.step(TRAIN["record_steps"])
+    s.sync()
+    t1 = time.perf_counter()
+    hist = s.learn_train(epochs=TRAIN["epochs"], batch=TRAIN["batch"], lr=TRAIN["lr"], seed=TRAIN["seed"])
+    t2 = time.perf_counter()
+    log(f"[bench] recorded {TRAIN['record_steps']} steps in {t1 - t0:.1f}s, trained {TRAIN['epochs']} epochs in "
+        f"{t2 - t1:.1f}s, loss {hist[0]:.4f} -> {hist[-1]:.4f} (not timed)")
+    return s.learn_weights(), s.learn_norm(), [float(hist[0]), float(hist[-1])]
+
+
+def timed_run(s, steps, warmup, vcycle):
+    s.step(warmup)
+    s.sync()
+    if vcycle:
+        s.run_stats(reset=True)
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    elapsed = time.perf_counter() - t0
+    run = s.run_stats(reset=True) if vcycle else None
+    return elapsed, run
 
 
 def cpu_baseline(levels, cfg, schedule, budget_s=15.0):
@@ -115,9 +155,11 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="medium", choices=["small", "medium", "large"])
-    ap.add_argument("--schedule", default="vcycle", choices=["single", "twolevel", "vcycle"])
+    ap.add_argument("--schedule", default="ab", choices=["ab", "vcycle", "temporal", "learned", "single", "twolevel"])
     ap.add_argument("--fixed-cycles", type=int, default=0,
                     help="V-cycles per step without the convergence check (profiling)")
+    ap.add_argument("--check-every", type=int, default=2, choices=[1, 2],
+                    help="warm-started schedules: residual checks after each cycle (1) or half-cycle (2)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     args = ap.parse_args()
@@ -128,34 +170,46 @@ def main():
 
     log(f"[bench] workload={args.workload} schedule={args.schedule} steps={args.steps} warmup={args.warmup}")
     cfg, levels = build_workload(args.workload)
-    t0 = time.perf_counter()
-    s = make_solver(levels, args.schedule, cfg, use_graph=0 if args.no_graph else 1, fixed_cycles=args.fixed_cycles)
-    log(f"[bench] solver created in {time.perf_counter() - t0:.1f}s; tets per level {[m.ntets for m in levels]}")
-    s.step(args.warmup)
-    s.sync()
-    if args.schedule == "vcycle":
-        s.run_stats(reset=True)
-    log("[bench] warmup done")
-    t0 = time.perf_counter()
-    s.step(args.steps)
-    s.sync()
-    elapsed = time.perf_counter() - t0
-    steps_per_s = args.steps / elapsed
-    log(f"[bench] timed {args.steps} steps in {elapsed:.3f}s -> {steps_per_s:.1f} steps/s")
-    run = s.run_stats(reset=True) if args.schedule == "vcycle" else None
-    if run:
-        log(f"[bench] run stats {run}")
+    names = ["vcycle", "temporal", "learned"] if args.schedule == "ab" else [args.schedule]
+    head = names[-1]
+    weights = None
+    losses = None
+    if "learned" in names:
+        weights = train_network(levels, cfg, args.check_every)
+        losses = weights[2]
+    results = {}
+    solvers = {}
+    for name in names:
+        t0 = time.perf_counter()
+        s = make_solver(levels, name, cfg, use_graph=0 if args.no_graph else 1, fixed_cycles=args.fixed_cycles,
+                        check_every=args.check_every)
+        if name == "learned":
+            s.learn_weights(weights[0])
+            s.learn_norm(weights[1])
+        log(f"[bench] {name}: solver created in {time.perf_counter() - t0:.1f}s; tets per level "
+            f"{[m.ntets for m in levels]}")
+        vc = name in ("vcycle", "temporal", "learned")
+        elapsed, run = timed_run(s, args.steps, args.warmup, vc)
+        sps = args.steps / elapsed
+        log(f"[bench] {name}: timed {args.steps} steps in {elapsed:.3f}s -> {sps:.1f} steps/s; run {run}")
+        results[name] = dict(steps_per_s=round(sps, 3), ms_per_step=round(1e3 * elapsed / args.steps, 4))
+        if run:
+            results[name].update(mean_cycles=round(run["total_cycles"] / max(1, run["steps"]), 3),
+                                 mean_half_cycles=round(run["total_halves"] / max(1, run["steps"]), 3),
+                                 max_rel_residual=run["max_rel"], steps_over_tol=run["failed_steps"],
+                                 warm_start_rejections=run["fallbacks"])
+        solvers[name] = s
+    s = solvers[head]
+    steps_per_s = results[head]["steps_per_s"]
 
     # dominant kernel: finest-level local step, HIP events around each launch
-    # on the solver's stream, over a pass with a fixed cycle count (no skipped
-    # launches) equal to the mean cycles of the timed run
+    # on the solver's stream, over an eager pass with one cycle per step (no
+    # skipped launches)
     T, V = levels[0].ntets, levels[0].nverts
-    mean_cycles = (run["total_cycles"] / max(1, run["steps"])) if run else None
-    if args.schedule == "vcycle" and not args.fixed_cycles:
-        s.set_params(max_cycles=max(1, math.ceil(mean_cycles)), check_every=0)
+    if head in ("vcycle", "temporal", "learned") and not args.fixed_cycles:
+        s.set_params(max_cycles=1, check_every=0)
     s.set_timing(True)
-    nt = min(args.steps, 50)
-    s.step(nt)
+    s.step(min(args.steps, 50))
     ms, launches = s.kernel_time(0)
     s.set_timing(False)
     avg_ms = ms / max(1, launches)
@@ -166,33 +220,38 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
-        cpu = cpu_baseline(levels, cfg, args.schedule)
+        cpu = cpu_baseline(levels, cfg, "vcycle" if head in ("temporal", "learned") else head)
         log(f"[bench] cpu baseline {cpu['value']} steps/s")
 
+    config = {
+        "workload": cfg["name"],
+        "levels_tets": [m.ntets for m in levels] if head in ("vcycle", "temporal", "learned") else
+        [levels[0].ntets],
+        "schedule": head,
+        "tol": VCYCLE["tol"] if head in ("vcycle", "temporal", "learned") else None,
+        "graph": not args.no_graph,
+        "schedules": results,
+    }
+    if "vcycle" in results and "learned" in results:
+        config["learned_vs_plain_mg"] = round(results["learned"]["steps_per_s"] / results["vcycle"]["steps_per_s"], 3)
+        config["learned_vs_temporal_only"] = round(results["learned"]["steps_per_s"] /
+                                                   results["temporal"]["steps_per_s"], 3)
+    if losses:
+        config["training"] = dict(TRAIN, loss_first=round(losses[0], 4), loss_last=round(losses[1], 4))
     out = {
         "metric": METRIC,
-        "value": round(steps_per_s, 3),
+        "value": steps_per_s,
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "ms_per_step": results[head]["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": cfg["name"],
-            "levels_tets": [m.ntets for m in levels] if args.schedule == "vcycle" else [levels[0].ntets],
-            "schedule": args.schedule,
-            "solver": "plain MG (V-cycle, no learned warm start)" if args.schedule == "vcycle" else args.schedule,
-            "tol": VCYCLE["tol"] if args.schedule == "vcycle" else None,
-            "mean_cycles_per_step": round(mean_cycles, 3) if mean_cycles else None,
-            "max_rel_residual": run["max_rel"] if run else None,
-            "steps_over_tol": run["failed_steps"] if run else None,
-            "graph": not args.no_graph,
-        },
+        "config": config,
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 2),

@@ -83,7 +83,8 @@ typedef struct mgsb_params {
     int32_t coarse_sweeps;/* linear Chebyshev-Jacobi sweeps per coarse level     */
     int32_t max_cycles;   /* V-cycles per step at most                           */
     float tol;            /* relative residual target per step (1e-3)           */
-    int32_t check_every;  /* cycles between residual checks (0: only at end)     */
+    int32_t check_every;  /* V-cycle convergence checks: 0 none (max_cycles run), */
+                          /* 1 after every cycle, 2 also after pre-smoothing     */
     int32_t use_graph;    /* capture fixed-count steps in a hipGraph             */
     int32_t block_tet;    /* workgroup size of per-tet kernels (0: tuned)        */
     int32_t block_vert;   /* workgroup size of per-vertex kernels (0: tuned)     */
@@ -92,7 +93,14 @@ typedef struct mgsb_params {
     float relax_coarse;   /* V-cycle: damped-Jacobi weight on coarse levels (0.6) */
     float rho_coarse;     /* V-cycle: Chebyshev rho on coarse levels; <= 0: power iteration */
     int32_t coarse_direct;/* V-cycle: FP64 Cholesky solve on the coarsest level (1) */
-    int32_t reserved[4];
+    int32_t warm_start;   /* V-cycle: 0 start from s (the reference's x = s), 1 from the
+                           * temporal extrapolation s + 2 u_{n-1} - u_{n-2} of the
+                           * previous steps' corrections u = x - s (kept only if it
+                           * lowers the residual).  The learned schedule always uses 1. */
+    int32_t learn_gate;   /* learned schedule: after 8 consecutive steps whose learned
+                           * correction was rejected, skip the MLP stage for 8, 16, ...
+                           * 256 steps before trying it again (1, default) or never (0) */
+    int32_t reserved[2];
 } mgsb_params;
 
 typedef struct mgsb_step_stats {
@@ -100,9 +108,11 @@ typedef struct mgsb_step_stats {
     double res;           /* ||g(x)|| at its end                                  */
     double rel;           /* res / res0                                           */
     int32_t cycles;       /* V-cycles run in the last step                        */
-    int32_t fallback;     /* learned schedule: 1 if the warm start was rejected   */
+    int32_t fallback;     /* warm-start stages rejected in the last step: bit 0 the */
+                          /* temporal extrapolation, bit 1 the learned correction  */
     int32_t steps;        /* steps taken by this call                             */
-    int32_t reserved;
+    int32_t halves;       /* half-cycles run in the last step (pre-smoothing half, */
+                          /* coarse-correction + post-smoothing half)              */
 } mgsb_step_stats;
 
 /* Aggregates over the V-cycle steps since the last reset. */
@@ -111,8 +121,8 @@ typedef struct mgsb_run_stats_t {
     int64_t total_cycles;   /* V-cycles summed over the steps                  */
     int32_t steps;
     int32_t failed_steps;   /* steps whose final relative residual > tol       */
-    int32_t fallbacks;      /* learned schedule: warm starts rejected          */
-    int32_t reserved;
+    int32_t fallbacks;      /* steps with a rejected warm-start stage          */
+    int32_t total_halves;   /* half-cycles summed over the steps               */
 } mgsb_run_stats_t;
 
 /* Energy diagnostics, runCalEnergy (PDSolver.cu:816-913): Ek, Ep, Ec, Nc, dX. */
@@ -227,16 +237,19 @@ int mgsb_residual(mgsb_solver* s, double* res);
 int mgsb_run_stats(mgsb_solver* s, mgsb_run_stats_t* out, int32_t reset);
 
 /* Learned coarse correction (MGSB_SCHED_LEARNED; no reference counterpart).
- * A 4-layer MLP (12 -> 128 -> 128 -> 128 -> 3, ReLU, BF16 MFMA with FP32
- * accumulation, FP32 master weights) maps per-vertex features of level 1
- * (restricted residual of the inertial prediction, injected velocity, rest
- * position, previous step's coarse correction) to that step's coarse
- * correction; the prolongated prediction warm-starts the V-cycles, and is
- * rejected (x reset to the inertial prediction) when it raises the residual.
+ * The step starts from the temporal extrapolation x_t = s + 2 u_{n-1} -
+ * u_{n-2} (warm_start = 1).  A 4-layer MLP (12 -> 128 -> 128 -> 128 -> 3,
+ * ReLU, BF16 MFMA with FP32 accumulation, FP32 master weights) maps features
+ * of each level-1 vertex (restricted residual at x_t, injected velocity, rest
+ * position, previous step's coarse delta) to the coarse correction that x_t
+ * still misses; x = x_t + P pred warm-starts the V-cycles.  Each stage is kept
+ * only if it lowers the residual (else x returns to the better of s and x_t),
+ * and the V-cycles then run to the residual target as usual.
  *
- * record: V-cycle / learned steps store (features, achieved correction) for
- * up to max_steps steps (0 stops); train: Adam on the recorded samples with
- * standardised inputs/outputs, loss_hist receives the mean loss per epoch. */
+ * record: V-cycle / learned steps store (features, achieved coarse delta
+ * inject(u - u_t)) for up to max_steps steps (0 stops); train: Adam on the
+ * recorded samples with standardised inputs/outputs, loss_hist receives the
+ * mean loss per epoch. */
 int mgsb_learn_record(mgsb_solver* s, int32_t max_steps);
 int mgsb_learn_samples(mgsb_solver* s, int64_t* nsamples);
 int mgsb_learn_clear(mgsb_solver* s);
@@ -247,6 +260,9 @@ int mgsb_learn_nparams(mgsb_solver* s, int32_t* n);
 int mgsb_learn_weights(mgsb_solver* s, float* w, int32_t n, int32_t set);
 /* Standardisation: in_mu[12] in_sd[12] out_mu[3] out_sd[3]. */
 int mgsb_learn_norm(mgsb_solver* s, float* norm, int32_t set);
+/* Test entry point: the coarse correction (V1 x 3, after the level-1
+ * Dirichlet mask) that the last learned step prolongated into x. */
+int mgsb_learn_last(mgsb_solver* s, float* e1);
 /* Test entry points: MLP outputs on raw inputs x (n x 12) through the fused
  * inference kernel (path 0) or the training GEMMs (path 1); gradient of the
  * mean squared error against targets t (n x 3). */

@@ -69,7 +69,9 @@ class mgsb_params(C.Structure):
         ("relax_coarse", C.c_float),
         ("rho_coarse", C.c_float),
         ("coarse_direct", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("warm_start", C.c_int32),
+        ("learn_gate", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -81,13 +83,13 @@ class mgsb_step_stats(C.Structure):
         ("cycles", C.c_int32),
         ("fallback", C.c_int32),
         ("steps", C.c_int32),
-        ("reserved", C.c_int32),
+        ("halves", C.c_int32),
     ]
 
 
 class mgsb_run_stats_t(C.Structure):
     _fields_ = [("max_rel", C.c_double), ("total_cycles", C.c_int64), ("steps", C.c_int32),
-                ("failed_steps", C.c_int32), ("fallbacks", C.c_int32), ("reserved", C.c_int32)]
+                ("failed_steps", C.c_int32), ("fallbacks", C.c_int32), ("total_halves", C.c_int32)]
 
 
 class mgsb_energy_t(C.Structure):
@@ -138,6 +140,7 @@ SIGNATURES = {
     "mgsb_learn_nparams": (C.c_int, [_vp, _ip]),
     "mgsb_learn_weights": (C.c_int, [_vp, _fp, C.c_int32, C.c_int32]),
     "mgsb_learn_norm": (C.c_int, [_vp, _fp, C.c_int32]),
+    "mgsb_learn_last": (C.c_int, [_vp, _fp]),
     "mgsb_mlp_forward": (C.c_int, [_vp, _fp, C.c_int32, _fp, C.c_int32]),
     "mgsb_mlp_grad": (C.c_int, [_vp, _fp, _fp, C.c_int32, _fp, _dp]),
     "mgsb_run_stats": (C.c_int, [_vp, C.POINTER(mgsb_run_stats_t), C.c_int32]),
@@ -428,7 +431,7 @@ class Solver:
         st = mgsb_run_stats_t()
         _check(lib.mgsb_run_stats(self._h, C.byref(st), int(reset)), "mgsb_run_stats")
         return {"max_rel": st.max_rel, "total_cycles": st.total_cycles, "steps": st.steps,
-                "failed_steps": st.failed_steps, "fallbacks": st.fallbacks}
+                "failed_steps": st.failed_steps, "fallbacks": st.fallbacks, "total_halves": st.total_halves}
 
     # --- learned coarse correction -------------------------------------------
     def learn_record(self, max_steps):
@@ -469,6 +472,11 @@ class Solver:
         a = _f32(norm, 30)
         _check(lib.mgsb_learn_norm(self._h, _fptr(a), 1), "mgsb_learn_norm")
 
+    def learn_last(self):
+        """Coarse correction (V1, 3) the last learned step prolongated into x."""
+        out = np.zeros(self.levels[1].nverts * 3, np.float32)
+        _check(lib.mgsb_learn_last(self._h, _fptr(out)), "mgsb_learn_last")
+        return out.reshape(-1, 3)
     def mlp_forward(self, x, path=0):
         x = _f32(x).reshape(-1, 12)
         y = np.zeros((x.shape[0], 3), np.float32)

@@ -806,6 +806,8 @@ void mgsb_default_params(mgsb_params* p) {
     p->relax_coarse = 0.6f;
     p->rho_coarse = 0.0f;
     p->coarse_direct = 1;
+    p->warm_start = 0;
+    p->learn_gate = 1;
 }
 
 const char* mgsb_version(void) { return "mgsb 0.1 (gfx950)"; }

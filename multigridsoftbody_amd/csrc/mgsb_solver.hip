@@ -127,14 +127,23 @@ __global__ void __launch_bounds__(256) k_st(int V, float4* __restrict__ x, float
     }
 }
 
+struct VState;
+__device__ void vstate_step_done(VState* st);
+
 // End of a split-position step: x = s + u (and x_prev = s + u_prev), then
-// calculateV.
+// calculateV; block 0 folds the step's outcome into the run aggregates.
 __global__ void __launch_bounds__(256) k_finish_split(int V, float4* __restrict__ x, float4* __restrict__ xprev,
                                                       const float4* __restrict__ s, const float4* __restrict__ xlast,
-                                                      float4* __restrict__ v, float dt) {
+                                                      float4* __restrict__ v, float dt, VState* vs,
+                                                      float4* __restrict__ U1, float4* __restrict__ U2) {
+    if (vs && blockIdx.x == 0 && threadIdx.x == 0) vstate_step_done(vs);
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= V) return;
     const float4 u = x[i], up = xprev[i], st = s[i], q = xlast[i];
+    if (U1) {  // corrections of the last two steps (temporal warm start)
+        U2[i] = U1[i];
+        U1[i] = make_float4(u.x, u.y, u.z, 0.f);
+    }
     const float4 p = make_float4(st.x + u.x, st.y + u.y, st.z + u.z, 0.f);
     x[i] = p;
     xprev[i] = make_float4(st.x + up.x, st.y + up.y, st.z + up.z, 0.f);
@@ -564,13 +573,137 @@ __global__ void __launch_bounds__(256) k_energy_tet(int T, const float4* __restr
     block_reduce_store<1>(v, part);
 }
 
+struct VState {
+    double res0;    // ||g(s)||
+    double res;     // ||g(x)|| at the last check
+    int converged;  // res <= tol * res0
+    int cycles;     // full cycles completed this step
+    int check;      // 0: no checks, 1: after each cycle, 2: also after pre-smoothing
+    float tol;
+    // run aggregates since the last reset (updated at the end of each step)
+    double max_rel;
+    long long total_cycles;
+    int nsteps;
+    int nfail;      // steps that ended with res > tol * res0
+    int fallback;   // learned schedule: the warm start was rejected this step
+    int nfallback;
+    double res_ws;  // residual right after the learned correction
+    int halves;     // half-cycles run this step
+    int total_halves;
+    double res_t;   // residual at the temporal extrapolation
+    int accepted;   // warm-start stage x holds: 0 s, 1 temporal, 2 temporal + learned
+    int revert;     // pending revert of x: 0 none, 1 to s, 2 to the temporal extrapolation
+    int cf_ok;      // corner forces are those of the current x (or the step has converged)
+};
+
+// Check modes of the fused residual reduction.
+constexpr int kChkNone = -1;
+constexpr int kChkStart = 0;  // g(s): res0
+constexpr int kChkCycle = 1;  // end of a cycle
+constexpr int kChkWarm = 2;   // after the learned warm start (judge, may reject it)
+constexpr int kChkMid = 3;    // after pre-smoothing
+constexpr int kChkTemporal = 4;  // after the temporal extrapolation
+
+// Convergence bookkeeping from the squared residual norm t.  hm (host-mapped,
+// optional) mirrors (converged, cycles, halves) for the host's cycle loop.
+__device__ void vstate_apply(VState* st, double t, int mode, int* hm) {
+    const double r = sqrt(t);
+    if (mode == kChkStart) {
+        st->res0 = r;
+        st->res = r;
+        st->cycles = 0;
+        st->halves = 0;
+        st->fallback = 0;
+        st->res_t = 0.0;
+        st->res_ws = 0.0;
+        st->accepted = 0;
+        st->revert = 0;
+        st->cf_ok = 1;
+        st->converged = (st->check && r == 0.0) ? 1 : 0;
+    } else if (mode == kChkTemporal || mode == kChkWarm) {
+        // a warm-start stage is kept only if it lowers the best residual so far
+        if (mode == kChkTemporal)
+            st->res_t = r;
+        else
+            st->res_ws = r;
+        if (r > st->res) {
+            st->revert = st->accepted == 1 ? 2 : 1;
+            st->fallback |= mode == kChkTemporal ? 1 : 2;
+            st->cf_ok = 0;
+        } else {
+            st->res = r;
+            st->accepted = mode == kChkTemporal ? 1 : 2;
+            st->revert = 0;
+            st->cf_ok = 1;
+            if (st->check && r <= (double)st->tol * st->res0) st->converged = 1;
+        }
+    } else {
+        st->res = r;
+        if (mode == kChkCycle) {
+            st->cycles += 1;
+            st->halves += st->check == 2 ? 1 : 2;
+        } else {
+            st->halves += 1;
+        }
+        if (st->check && r <= (double)st->tol * st->res0) st->converged = 1;
+    }
+    if (hm) {
+        hm[0] = st->converged;
+        hm[1] = st->cycles;
+        hm[2] = st->halves;
+        hm[3] = st->fallback;
+        hm[4] = st->accepted;
+    }
+}
+
+// End of a step: fold the step's outcome into the run aggregates.
+__device__ void vstate_step_done(VState* st) {
+    const double rel = st->res0 > 0.0 ? st->res / st->res0 : 0.0;
+    if (rel > st->max_rel) st->max_rel = rel;
+    st->total_cycles += st->cycles;
+    st->total_halves += st->halves;
+    st->nsteps += 1;
+    if (rel > (double)st->tol) st->nfail += 1;
+    st->nfallback += st->fallback != 0;
+}
+
+// Second stage of a block-partial reduction, fused into the kernel that
+// produced the partials: each block has stored part[blockIdx.x]; the block
+// that arrives last (device-scope counter) sums all partials in block order
+// -- the same order on every run -- and applies the check.  It resets the
+// counter for the next launch.
+__device__ void last_block_check(const double* part, unsigned* count, VState* st, int mode, int* hm) {
+    __shared__ int s_last;
+    __shared__ double sh[256];
+    if (threadIdx.x == 0) {
+        __threadfence();
+        s_last = atomicAdd(count, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    double a = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+        a += __hip_atomic_load(part + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < (int)blockDim.x; q++) t += sh[q];
+        *count = 0u;
+        vstate_apply(st, t, mode, hm);
+    }
+}
+
 // Nonlinear residual g = m/h^2 (s - x) + f_el + f_col over free DOFs; writes
-// g (float4) when r != nullptr and the FP64 partial of |g|^2.
+// g (float4) when r != nullptr and the FP64 partial of |g|^2.  With
+// mode != kChkNone the last block completes the norm and applies the check.
 __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__ x, const float4* __restrict__ s,
                                                const float4* __restrict__ vc, const int* __restrict__ rp,
                                                const float4* __restrict__ cf, const DevColliders* __restrict__ col,
                                                int flags, float4* __restrict__ r, double* __restrict__ part,
-                                               float4* __restrict__ xprev, const int* __restrict__ skip, int split) {
+                                               float4* __restrict__ xprev, const int* __restrict__ skip, int split,
+                                               VState* st, unsigned* count, int mode, int* hm) {
     if (skip && *skip) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
@@ -626,7 +759,10 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
         if (xprev) xprev[i] = p;
         v[0] = (double)gx * gx + (double)gy * gy + (double)gz * gz;
     }
-    if (part) block_reduce_store<1>(v, part);
+    if (part) {
+        block_reduce_store<1>(v, part);
+        if (mode != kChkNone) last_block_check(part, count, st, mode, hm);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -634,42 +770,36 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
 // convergence flag is set, so a captured step with max_cycles cycles costs
 // only empty launches after convergence.
 // ---------------------------------------------------------------------------
-struct VState {
-    double res0;    // ||g(s)||
-    double res;     // ||g(x)|| after the last cycle
-    int converged;  // res <= tol * res0 (checked after each cycle)
-    int cycles;
-    int check;
-    float tol;
-    // run aggregates since the last reset (updated at the end of each step)
-    double max_rel;
-    long long total_cycles;
-    int nsteps;
-    int nfail;      // steps that ended with res > tol * res0
-    int fallback;   // learned schedule: the warm start was rejected this step
-    int nfallback;
-    double res_ws;  // residual right after the warm start
-};
-
-// End of a step: fold the step's outcome into the run aggregates.
-__global__ void k_step_done(VState* st) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const double rel = st->res0 > 0.0 ? st->res / st->res0 : 0.0;
-    if (rel > st->max_rel) st->max_rel = rel;
-    st->total_cycles += st->cycles;
-    st->nsteps += 1;
-    if (rel > (double)st->tol) st->nfail += 1;
-    st->nfallback += st->fallback;
+// Warm start: x = u_t = fix (2 u_{n-1} - u_{n-2}), the temporal extrapolation
+// of the last two steps' corrections (split positions: x holds u = x - s).
+__global__ void __launch_bounds__(256) k_warm_temporal(int V, float4* __restrict__ x, float4* __restrict__ xprev,
+                                                       const float4* __restrict__ U1, const float4* __restrict__ U2,
+                                                       const float4* __restrict__ vc, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const float4 a = U1[i], b = U2[i];
+    const float f = vc[i].z;
+    const float4 q = make_float4(f * (2.f * a.x - b.x), f * (2.f * a.y - b.y), f * (2.f * a.z - b.z), 0.f);
+    x[i] = q;
+    xprev[i] = q;
 }
 
-// Learned schedule: put x back to the inertial prediction s when the judge
-// rejected the warm start.
+// Judge of a rejected warm-start stage: x returns to s (u = 0) or to the
+// temporal extrapolation.
 __global__ void __launch_bounds__(256) k_revert(int V, float4* __restrict__ x, float4* __restrict__ xprev,
-                                                const float4* __restrict__ s, const VState* st) {
-    if (!st->fallback) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                const float4* __restrict__ U1, const float4* __restrict__ U2,
+                                                const float4* __restrict__ vc, const VState* st) {
+    const int rv = st->revert;
+    if (!rv) return;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= V) return;
-    const float4 q = make_float4(0.f, 0.f, 0.f, 0.f);  // split positions: u = 0 is x = s
+    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (rv == 2) {
+        const float4 a = U1[i], b = U2[i];
+        const float f = vc[i].z;
+        q = make_float4(f * (2.f * a.x - b.x), f * (2.f * a.y - b.y), f * (2.f * a.z - b.z), 0.f);
+    }
     x[i] = q;
     xprev[i] = q;
 }
@@ -715,10 +845,12 @@ __global__ void __launch_bounds__(256) k_apply_pred(int Vc, const float* __restr
                            m * (pred[j * 3 + 2] * sd[2] + mu[2]), 0.f);
 }
 
-// Coarse correction achieved this step: (x - s) injected at each level-1
-// vertex through its coarse-in-fine binding; kept for the next step's
-// features and optionally recorded as the training target.
-__global__ void __launch_bounds__(256) k_target(int Vc, const float4* __restrict__ x, const float4* __restrict__ s,
+// Coarse delta achieved this step: (u - u_t) injected at each level-1 vertex
+// through its coarse-in-fine binding, u = x - s the step's correction (x
+// holds u) and u_t its temporal extrapolation (U1 == nullptr: u_t = 0).  Kept
+// for the next step's features and optionally recorded as training target.
+__global__ void __launch_bounds__(256) k_target(int Vc, const float4* __restrict__ x, const float4* __restrict__ U1,
+                                                const float4* __restrict__ U2, const float4* __restrict__ vc,
                                                 const int4* __restrict__ aids, const float4* __restrict__ aw,
                                                 float4* __restrict__ eprev, float* __restrict__ rec) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -728,52 +860,24 @@ __global__ void __launch_bounds__(256) k_target(int Vc, const float4* __restrict
     const int ids[4] = {id.x, id.y, id.z, id.w};
     const float ws[4] = {w.x, w.y, w.z, w.w};
     float dx = 0.f, dy = 0.f, dz = 0.f;
-    for (int q = 0; q < 4; q++) {  // s == nullptr: x already holds u = x - s
-        const float4 a = x[ids[q]], b = s ? s[ids[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
-        dx = dx + ws[q] * (a.x - b.x);
-        dy = dy + ws[q] * (a.y - b.y);
-        dz = dz + ws[q] * (a.z - b.z);
+    for (int q = 0; q < 4; q++) {
+        float4 a = x[ids[q]];
+        if (U1) {
+            const float4 u1 = U1[ids[q]], u2 = U2[ids[q]];
+            const float f = vc[ids[q]].z;
+            a.x = a.x - f * (2.f * u1.x - u2.x);
+            a.y = a.y - f * (2.f * u1.y - u2.y);
+            a.z = a.z - f * (2.f * u1.z - u2.z);
+        }
+        dx = dx + ws[q] * a.x;
+        dy = dy + ws[q] * a.y;
+        dz = dz + ws[q] * a.z;
     }
     eprev[j] = make_float4(dx, dy, dz, 0.f);
     if (rec) {
         rec[(long)j * 3 + 0] = dx;
         rec[(long)j * 3 + 1] = dy;
         rec[(long)j * 3 + 2] = dz;
-    }
-}
-
-// mode 0: step start (res0 = res); mode 1: end of a cycle.  One block sums
-// the per-block partials in block order (deterministic).
-__global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict__ part, VState* st, int mode) {
-    if (mode == 1 && st->converged) return;
-    __shared__ double sh[256];
-    double a = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) a += part[b];
-    sh[threadIdx.x] = a;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int q = 0; q < (int)blockDim.x; q++) t += sh[q];
-        const double r = sqrt(t);
-        if (mode == 0) {
-            st->res0 = r;
-            st->res = r;
-            st->cycles = 0;
-            st->fallback = 0;
-            st->converged = (st->check && r == 0.0) ? 1 : 0;
-        } else if (mode == 2) {  // judge of the learned warm start
-            st->res_ws = r;
-            if (r > st->res0) {
-                st->fallback = 1;  // worse than the inertial prediction: revert
-            } else {
-                st->res = r;
-                if (st->check && r <= (double)st->tol * st->res0) st->converged = 1;
-            }
-        } else {
-            st->res = r;
-            st->cycles += 1;
-            if (st->check && r <= (double)st->tol * st->res0) st->converged = 1;
-        }
     }
 }
 
@@ -924,12 +1028,65 @@ __global__ void __launch_bounds__(256) k_zero4(int n, float4* __restrict__ a, co
 
 constexpr int kMaxDense = 1024;
 
-// Coarsest level: e = A^-1 r in FP64 with the Cholesky factor C (A = C C^T)
-// applied through W = C^-1: y = W r, e = W^T y.  One workgroup; W is n x n
-// row-major lower triangular; dense[i] maps vertices to rows (-1: pinned).
+__device__ __forceinline__ void wave_sum3(double& a0, double& a1, double& a2) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_down(a0, off, 64);
+        a1 += __shfl_down(a1, off, 64);
+        a2 += __shfl_down(a2, off, 64);
+    }
+}
+
+// Coarsest solve e = W^T (W r) with W = C^-1, A = C C^T (FP64 Cholesky), on
+// LDS vectors: rb[c][0..m) holds r on entry and e on exit, yb is scratch.
+// One wave per row with the lanes across the row (W and Wt = W^T both
+// row-major), so all loads of a row are in flight at once; each row is summed
+// by a fixed shuffle tree.
+__device__ __forceinline__ void dense_solve_block(int m, const double* __restrict__ W, const double* __restrict__ Wt,
+                                                  double (*rb)[kMaxDense], double (*yb)[kMaxDense]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int row = wave; row < m; row += nw) {
+        const double* wr = W + (size_t)row * m;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int k = lane; k <= row; k += 64) {
+            const double w = wr[k];
+            a0 += w * rb[0][k];
+            a1 += w * rb[1][k];
+            a2 += w * rb[2][k];
+        }
+        wave_sum3(a0, a1, a2);
+        if (lane == 0) {
+            yb[0][row] = a0;
+            yb[1][row] = a1;
+            yb[2][row] = a2;
+        }
+    }
+    __syncthreads();
+    for (int row = wave; row < m; row += nw) {
+        const double* wr = Wt + (size_t)row * m;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int k = row + lane; k < m; k += 64) {
+            const double w = wr[k];
+            a0 += w * yb[0][k];
+            a1 += w * yb[1][k];
+            a2 += w * yb[2][k];
+        }
+        wave_sum3(a0, a1, a2);
+        if (lane == 0) {
+            rb[0][row] = a0;
+            rb[1][row] = a1;
+            rb[2][row] = a2;
+        }
+    }
+    __syncthreads();
+}
+
+// Coarsest level on its own (when it is not part of the one-workgroup tail):
+// dense[i] maps vertices to rows of the n x n factor (-1: pinned).
 __global__ void __launch_bounds__(256) k_coarse_solve(int Vc, int n, const int* __restrict__ dense,
-                                                      const double* __restrict__ W, const float4* __restrict__ r,
-                                                      float4* __restrict__ e, const int* __restrict__ skip) {
+                                                      const double* __restrict__ W, const double* __restrict__ Wt,
+                                                      const float4* __restrict__ r, float4* __restrict__ e,
+                                                      const int* __restrict__ skip) {
     if (skip && *skip) return;
     __shared__ double rb[3][kMaxDense];
     __shared__ double yb[3][kMaxDense];
@@ -943,33 +1100,7 @@ __global__ void __launch_bounds__(256) k_coarse_solve(int Vc, int n, const int* 
         }
     }
     __syncthreads();
-    for (int row = threadIdx.x; row < n; row += blockDim.x) {
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        const double* wr = W + (size_t)row * n;
-        for (int k = 0; k <= row; k++) {
-            const double w = wr[k];
-            a0 += w * rb[0][k];
-            a1 += w * rb[1][k];
-            a2 += w * rb[2][k];
-        }
-        yb[0][row] = a0;
-        yb[1][row] = a1;
-        yb[2][row] = a2;
-    }
-    __syncthreads();
-    for (int row = threadIdx.x; row < n; row += blockDim.x) {
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        for (int k = row; k < n; k++) {
-            const double w = W[(size_t)k * n + row];
-            a0 += w * yb[0][k];
-            a1 += w * yb[1][k];
-            a2 += w * yb[2][k];
-        }
-        rb[0][row] = a0;
-        rb[1][row] = a1;
-        rb[2][row] = a2;
-    }
-    __syncthreads();
+    dense_solve_block(n, W, Wt, rb, yb);
     for (int i = threadIdx.x; i < Vc; i += blockDim.x) {
         const int d = dense[i];
         e[i] = d >= 0 ? make_float4((float)rb[0][d], (float)rb[1][d], (float)rb[2][d], 0.f)
@@ -994,7 +1125,7 @@ struct TailLevel {
     const int4* iids;       // prolongation from the next level into this one
     const float4* iw;
     const int* dense;
-    const double* W;
+    const double *W, *Wt;
     float rho;
     int pad;
 };
@@ -1005,26 +1136,26 @@ __global__ void __launch_bounds__(1024) k_coarse_tail(const TailLevel* __restric
     __shared__ double rb[3][kMaxDense];
     __shared__ double yb[3][kMaxDense];
     const int tid = threadIdx.x, nt = blockDim.x;
-    int flip[8];  // 0: e in e0, 1: e in e1 (per level)
-    for (int q = 0; q < n; q++) flip[q] = 0;
+    unsigned flips = 0u;  // bit q: level q's correction is in e1 (else e0)
+    auto flip = [&](int q) { return (flips >> q) & 1u; };
     auto smooth = [&](int q, int ns, bool zero) {
         const TailLevel& L = tl[q];
         float omega = 1.0f;
         for (int k = 0; k < ns; k++) {
             omega = 4 / (4 - L.rho * L.rho * omega);
-            float4* e = flip[q] ? L.e1 : L.e0;
-            float4* ep = flip[q] ? L.e0 : L.e1;
+            float4* e = flip(q) ? L.e1 : L.e0;
+            float4* ep = flip(q) ? L.e0 : L.e1;
             for (int i = tid; i < L.V; i += nt)
                 jacobi_cheb_row(i, L.ap, L.aj, L.av, e, ep, L.r, L.vcl, omega, relax, zero && k == 0);
             __syncthreads();
-            flip[q] ^= 1;
+            flips ^= 1u << q;
         }
     };
     // down
     for (int q = 0; q + 1 < n; q++) {
         const TailLevel& L = tl[q];
         smooth(q, sweeps, true);
-        const float4* e = flip[q] ? L.e1 : L.e0;
+        const float4* e = flip(q) ? L.e1 : L.e0;
         for (int i = tid; i < L.V; i += nt) {
             const float3 Ae = spmv_row(i, L.ap, L.aj, L.av, e);
             const float4 rv = L.r[i];
@@ -1072,40 +1203,13 @@ __global__ void __launch_bounds__(1024) k_coarse_tail(const TailLevel* __restric
                 }
             }
             __syncthreads();
-            const int m = L.nfree;
-            for (int row = tid; row < m; row += nt) {
-                double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-                const double* wr = L.W + (size_t)row * m;
-                for (int k = 0; k <= row; k++) {
-                    const double w = wr[k];
-                    a0 += w * rb[0][k];
-                    a1 += w * rb[1][k];
-                    a2 += w * rb[2][k];
-                }
-                yb[0][row] = a0;
-                yb[1][row] = a1;
-                yb[2][row] = a2;
-            }
-            __syncthreads();
-            for (int row = tid; row < m; row += nt) {
-                double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-                for (int k = row; k < m; k++) {
-                    const double w = L.W[(size_t)k * m + row];
-                    a0 += w * yb[0][k];
-                    a1 += w * yb[1][k];
-                    a2 += w * yb[2][k];
-                }
-                rb[0][row] = a0;
-                rb[1][row] = a1;
-                rb[2][row] = a2;
-            }
-            __syncthreads();
+            dense_solve_block(L.nfree, L.W, L.Wt, rb, yb);
             for (int i = tid; i < L.V; i += nt) {
                 const int d = L.dense[i];
                 L.e0[i] = d >= 0 ? make_float4((float)rb[0][d], (float)rb[1][d], (float)rb[2][d], 0.f)
                                  : make_float4(0.f, 0.f, 0.f, 0.f);
             }
-            flip[n - 1] = 0;
+            flips &= ~(1u << (n - 1));
             __syncthreads();
         } else {
             smooth(n - 1, sweeps > 0 ? sweeps : 1, true);
@@ -1115,9 +1219,9 @@ __global__ void __launch_bounds__(1024) k_coarse_tail(const TailLevel* __restric
     for (int q = n - 2; q >= 0; q--) {
         const TailLevel& L = tl[q];
         const TailLevel& C = tl[q + 1];
-        const float4* ec = flip[q + 1] ? C.e1 : C.e0;
-        float4* e = flip[q] ? L.e1 : L.e0;
-        float4* ep = flip[q] ? L.e0 : L.e1;
+        const float4* ec = flip(q + 1) ? C.e1 : C.e0;
+        float4* e = flip(q) ? L.e1 : L.e0;
+        float4* ep = flip(q) ? L.e0 : L.e1;
         for (int i = tid; i < L.V; i += nt) {
             const int4 id = L.iids[i];
             const float4 ww = L.iw[i];
@@ -1137,7 +1241,7 @@ __global__ void __launch_bounds__(1024) k_coarse_tail(const TailLevel* __restric
         smooth(q, sweeps, false);
     }
     // leave the first level's correction in e0
-    if (flip[0]) {
+    if (flip(0)) {
         const TailLevel& L = tl[0];
         for (int i = tid; i < L.V; i += nt) L.e0[i] = L.e1[i];
     }
@@ -1183,9 +1287,10 @@ struct DevLevel {
     float* av = nullptr;
     std::vector<float> adiag;          // its diagonal
     float* Fs = nullptr;               // F(s) per tet (9 SoA streams), split-position steps
+    float4 *U1 = nullptr, *U2 = nullptr;  // corrections u = x - s of the last two steps
     int cur = 0;                       // which of e / ep holds the correction
     int* dense = nullptr;
-    double* W = nullptr;
+    double *W = nullptr, *Wt = nullptr;  // coarsest factor W = C^-1 and its transpose
     int nfree = -1;
     std::vector<float> vfix;
     double rho = 0.0;
@@ -1225,6 +1330,19 @@ struct mgsb_solver {
     mgsb_step_stats last{};
     VState* vstate = nullptr;
     double* vpart = nullptr;  // partials of the level-0 residual norm
+    unsigned* vcount = nullptr;  // arrival counter of the fused residual check
+    int* hmirror = nullptr;      // host-mapped (converged, cycles, halves, fallback, accepted)
+    // V-cycle steps run as pieces (start, pre-smoothing half, coarse + post
+    // half, finish), each captured once; the host launches only the halves a
+    // step needs, reading the convergence mirror after a guessed count.
+    hipGraph_t pgraph[5] = {};
+    hipGraphExec_t pexec[5] = {};
+    bool pvalid[5] = {};
+    int halves_guess = 4;
+    // learned schedule: after kGateAfter consecutive steps whose learned
+    // correction the judge rejected, steps start without the MLP stage for
+    // gate_skip steps (doubling up to kGateMax after each further rejection)
+    int gate_rejects = 0, gate_skip = 0, gate_left = 0;
     TailLevel* tail = nullptr; // descriptors of the one-workgroup coarse tail
     int tail_from = -1;        // first level handled by the tail kernel (-1: none)
     // learned coarse correction (level 1)
@@ -1237,18 +1355,26 @@ struct mgsb_solver {
     std::vector<void*> owned;
 
     ~mgsb_solver() {
-        if (stream) hipStreamSynchronize(stream);
-        if (gexec) hipGraphExecDestroy(gexec);
-        if (graph) hipGraphDestroy(graph);
-        for (auto& e : ev_pool) {
-            hipEventDestroy(e.first);
-            hipEventDestroy(e.second);
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (int q = 0; q < 5; q++) {
+            if (pexec[q]) (void)hipGraphExecDestroy(pexec[q]);
+            if (pgraph[q]) (void)hipGraphDestroy(pgraph[q]);
         }
-        for (auto& lv : L)
-            for (void* q : lv.owned) hipFree(q);
-        for (void* q : owned) hipFree(q);
+        if (hmirror) (void)hipHostFree(hmirror);
+        for (auto& e : ev_pool) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        for (auto& lv : L) {
+            for (void* q : lv.owned) (void)hipFree(q);
+            if (lv.W) (void)hipFree(lv.W);
+            if (lv.Wt) (void)hipFree(lv.Wt);
+        }
+        for (void* q : owned) (void)hipFree(q);
         if (mlp) mlp_destroy(mlp);
-        if (stream) hipStreamDestroy(stream);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -1341,7 +1467,7 @@ int pull_colliders(mgsb_solver* s) {
 int push_vstate(mgsb_solver* s) {
     if (!s->vstate) return MGSB_OK;
     VState v{};
-    v.check = s->p.check_every != 0 ? 1 : 0;
+    v.check = s->p.check_every;
     v.tol = s->p.tol;
     HIPCHK(hipMemcpyAsync(s->vstate, &v, sizeof(VState), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -1524,7 +1650,7 @@ int vcycle_lin(mgsb_solver* s, int l, const int* skip) {
     d.cur = 0;
     if (l == nl - 1) {
         if (s->p.coarse_direct && d.nfree >= 0) {
-            k_coarse_solve<<<1, 256, 0, s->stream>>>(d.V, d.nfree, d.dense, d.W, d.r, d.e, skip);
+            k_coarse_solve<<<1, 256, 0, s->stream>>>(d.V, d.nfree, d.dense, d.W, d.Wt, d.r, d.e, skip);
             HIPCHK(hipGetLastError());
             return MGSB_OK;
         }
@@ -1544,8 +1670,9 @@ int vcycle_lin(mgsb_solver* s, int l, const int* skip) {
 }
 
 // Fine-level residual on split positions (x holds u = x - s).  first: the
-// step's first evaluation, which also stores F(s).
-int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, bool norm, bool restart, const int* skip,
+// step's first evaluation, which also stores F(s).  chk: the check that the
+// residual kernel's last block applies (kChkNone: no norm).
+int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool restart, const int* skip,
                   bool first = false) {
     DevLevel& d = s->L[0];
     int rc;
@@ -1554,28 +1681,46 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, bool norm, bool r
         return rc;
     const int nb = nblk(d.V, 256);
     k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, write_r ? d.r : nullptr,
-                                       norm ? s->vpart : nullptr, restart ? d.xprev : nullptr, skip, 1);
+                                       chk != kChkNone ? s->vpart : nullptr, restart ? d.xprev : nullptr, skip, 1,
+                                       s->vstate, s->vcount, chk, s->hmirror);
     HIPCHK(hipGetLastError());
     return MGSB_OK;
 }
 
-// Residual-correction V-cycle step (see include/mgsb.h, MGSB_SCHED_VCYCLE);
-// with learned = true the MLP's coarse correction warm-starts the cycles.
-int enqueue_step_vcycle(mgsb_solver* s, bool learned) {
+// A residual-correction V-cycle step (include/mgsb.h, MGSB_SCHED_VCYCLE) is
+// enqueued as four pieces.  Every kernel after the start returns at once when
+// the step's convergence flag is set.
+enum { kPieceStart = 0, kPiecePre = 1, kPiecePost = 2, kPieceFinish = 3, kPieceStartGated = 4 };
+constexpr int kGateAfter = 8, kGateFirst = 8, kGateMax = 256;
+
+bool temporal_warm(const mgsb_solver* s) {
+    return s->p.schedule == MGSB_SCHED_LEARNED || s->p.warm_start == 1;
+}
+
+// Inertial prediction, g(s) and res0, then the warm start: the temporal
+// extrapolation of the last corrections (warm_start = 1 or learned), and with
+// learned = true the MLP's coarse delta on top.  Each stage is judged by the
+// residual it leaves (vstate_apply) and undone by k_revert when it is worse.
+int enqueue_vc_start(mgsb_solver* s, bool learned) {
     DevLevel& f = s->L[0];
     DevLevel& c1 = s->L[1];
     const int* skip = &s->vstate->converged;
     const int nbv = nblk(f.V, 256);
     const int bs = block_vert(s);
     const bool features = learned || s->recording;
+    const bool temporal = temporal_warm(s);
     float* recX = s->recording ? s->recX + (size_t)s->rec_steps * c1.V * kFeat : nullptr;
-    float* recT = s->recording ? s->recT + (size_t)s->rec_steps * c1.V * 3 : nullptr;
     int rc;
     if ((rc = launch_st(s, f, 1, 1))) return rc;
-    if ((rc = fine_residual(s, true, features, true, true, nullptr, true))) return rc;
-    k_check<<<1, 256, 0, s->stream>>>(nbv, s->vpart, s->vstate, 0);
-    HIPCHK(hipGetLastError());
-    if (features) {
+    if ((rc = fine_residual(s, true, features && !temporal, kChkStart, true, nullptr, true))) return rc;
+    if (temporal) {
+        k_warm_temporal<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, skip);
+        HIPCHK(hipGetLastError());
+        if ((rc = fine_residual(s, true, features, kChkTemporal, true, skip))) return rc;
+        k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate);
+        HIPCHK(hipGetLastError());
+    }
+    if (features) {  // restricted residual at the warm start, features (recorded even if converged)
         k_restrict<<<nblk(c1.V * 64, 256), 256, 0, s->stream>>>(c1.V, c1.rtp, c1.rti, c1.rtw, f.r, c1.vcl, c1.r,
                                                                 nullptr);
         k_features<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, c1.r, f.v, s->M[0].aids, s->M[0].aw, s->X1, s->eprev,
@@ -1588,60 +1733,160 @@ int enqueue_step_vcycle(mgsb_solver* s, bool learned) {
         k_prolong_add<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, s->epred, s->M[0].iids, s->M[0].iw,
                                                            f.vc, skip);
         HIPCHK(hipGetLastError());
-        if ((rc = fine_residual(s, true, false, true, true, skip))) return rc;
-        k_check<<<1, 256, 0, s->stream>>>(nbv, s->vpart, s->vstate, 2);
-        k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.s, s->vstate);
+        if ((rc = fine_residual(s, true, false, kChkWarm, true, skip))) return rc;
+        k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate);
         HIPCHK(hipGetLastError());
-        // cf may hold forces at the rejected warm start: recompute at x
-        if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, 2))) return rc;
     }
+    // after a revert the corner forces belong to the rejected x: recompute
+    if (temporal && (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, &s->vstate->cf_ok, 2)))
+        return rc;
+    return MGSB_OK;
+}
+
+// First half of a cycle: Chebyshev-accelerated nonlinear Jacobi sweeps, then
+// the residual that the coarse correction restricts (checked when
+// check_every == 2).
+int enqueue_vc_pre(mgsb_solver* s) {
+    DevLevel& f = s->L[0];
+    const int* skip = &s->vstate->converged;
     const float rs = s->p.rho_smooth;
-    for (int c = 0; c < s->p.max_cycles; c++) {
-        float omega = 1.0f;
-        for (int k = 0; k < s->p.pre_sweeps; k++) {
-            omega = cheb_next(omega, rs);
-            if (k > 0 && (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, 2))) return rc;
-            if ((rc = launch_vert(s, f, omega, kVertPoint, skip, 1))) return rc;
-        }
-        if ((rc = fine_residual(s, s->p.pre_sweeps > 0, true, false, false, skip))) return rc;
-        DevLevel& c1 = s->L[1];
-        k_restrict<<<nblk(c1.V * 64, 256), 256, 0, s->stream>>>(c1.V, c1.rtp, c1.rti, c1.rtw, f.r, c1.vcl, c1.r,
-                                                                skip);
-        HIPCHK(hipGetLastError());
-        if ((rc = vcycle_lin(s, 1, skip))) return rc;
-        k_prolong_add<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, cur_e(c1), s->M[0].iids, s->M[0].iw,
-                                                           f.vc, skip);
-        HIPCHK(hipGetLastError());
-        omega = 1.0f;
-        for (int k = 0; k < s->p.post_sweeps; k++) {
-            omega = cheb_next(omega, rs);
-            if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, 2))) return rc;
-            if ((rc = launch_vert(s, f, omega, kVertPoint | (k == s->p.post_sweeps - 1 ? kVertWriteCol : 0), skip,
-                                  1)))
-                return rc;
-        }
-        if ((rc = fine_residual(s, true, false, true, true, skip))) return rc;
-        k_check<<<1, 256, 0, s->stream>>>(nbv, s->vpart, s->vstate, 1);
-        HIPCHK(hipGetLastError());
+    int rc;
+    float omega = 1.0f;
+    for (int k = 0; k < s->p.pre_sweeps; k++) {
+        omega = cheb_next(omega, rs);
+        if (k > 0 && (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, 2))) return rc;
+        if ((rc = launch_vert(s, f, omega, kVertPoint, skip, 1))) return rc;
     }
-    k_step_done<<<1, 64, 0, s->stream>>>(s->vstate);
+    return fine_residual(s, s->p.pre_sweeps > 0, true, s->p.check_every == 2 ? kChkMid : kChkNone, false, skip);
+}
+
+// Second half: restriction, the linear V-cycle on the coarse levels,
+// prolongation, post-smoothing and the end-of-cycle residual check.
+int enqueue_vc_post(mgsb_solver* s) {
+    DevLevel& f = s->L[0];
+    DevLevel& c1 = s->L[1];
+    const int* skip = &s->vstate->converged;
+    const int bs = block_vert(s);
+    const float rs = s->p.rho_smooth;
+    int rc;
+    k_restrict<<<nblk(c1.V * 64, 256), 256, 0, s->stream>>>(c1.V, c1.rtp, c1.rti, c1.rtw, f.r, c1.vcl, c1.r, skip);
     HIPCHK(hipGetLastError());
-    if (features || s->eprev) {
-        k_target<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, f.x, nullptr, s->M[0].aids, s->M[0].aw, s->eprev, recT);
+    if ((rc = vcycle_lin(s, 1, skip))) return rc;
+    k_prolong_add<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, cur_e(c1), s->M[0].iids, s->M[0].iw, f.vc,
+                                                       skip);
+    HIPCHK(hipGetLastError());
+    float omega = 1.0f;
+    for (int k = 0; k < s->p.post_sweeps; k++) {
+        omega = cheb_next(omega, rs);
+        if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, 2))) return rc;
+        if ((rc = launch_vert(s, f, omega, kVertPoint | (k == s->p.post_sweeps - 1 ? kVertWriteCol : 0), skip, 1)))
+            return rc;
+    }
+    return fine_residual(s, true, false, kChkCycle, true, skip);
+}
+
+// Coarse correction achieved (features of the next step / training target),
+// x = s + u, velocities, run aggregates.
+int enqueue_vc_finish(mgsb_solver* s, bool learned) {
+    DevLevel& f = s->L[0];
+    DevLevel& c1 = s->L[1];
+    const int bs = block_vert(s);
+    const bool features = learned || s->recording;
+    float* recT = s->recording ? s->recT + (size_t)s->rec_steps * c1.V * 3 : nullptr;
+    if (features || temporal_warm(s)) {  // coarse delta: next step's feature / training target
+        const bool t = temporal_warm(s);
+        k_target<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, f.x, t ? f.U1 : nullptr, t ? f.U2 : nullptr, f.vc,
+                                                       s->M[0].aids, s->M[0].aw, s->eprev, recT);
         HIPCHK(hipGetLastError());
     }
     if (s->recording) s->rec_steps++;
-    k_finish_split<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, f.s, f.xlast, f.v, s->p.dt);
+    k_finish_split<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, f.s, f.xlast, f.v, s->p.dt, s->vstate,
+                                                        f.U1, f.U2);
     HIPCHK(hipGetLastError());
     return MGSB_OK;
+}
+
+int enqueue_piece(mgsb_solver* s, int piece) {
+    const bool learned = s->p.schedule == MGSB_SCHED_LEARNED;
+    switch (piece) {
+        case kPieceStart: return enqueue_vc_start(s, learned);
+        case kPieceStartGated: return enqueue_vc_start(s, false);
+        case kPiecePre: return enqueue_vc_pre(s);
+        case kPiecePost: return enqueue_vc_post(s);
+        default: return enqueue_vc_finish(s, learned);
+    }
+}
+
+int run_piece(mgsb_solver* s, int piece, bool graphable) {
+    if (!graphable) return enqueue_piece(s, piece);
+    if (!s->pvalid[piece]) {
+        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_piece(s, piece);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(s->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        HIPCHK(ce);
+        s->pgraph[piece] = g;
+        HIPCHK(hipGraphInstantiate(&s->pexec[piece], g, nullptr, nullptr, 0));
+        s->pvalid[piece] = true;
+    }
+    HIPCHK(hipGraphLaunch(s->pexec[piece], s->stream));
+    return MGSB_OK;
+}
+
+// One V-cycle step.  Without checks all max_cycles cycles are enqueued.  With
+// checks the host enqueues the number of half-cycles the previous step
+// needed, then reads the convergence mirror (one stream synchronisation) and
+// adds halves one check point at a time until the step has converged or
+// max_cycles cycles have run, so no launches are spent on a converged step
+// beyond a wrong guess.
+int step_vcycle(mgsb_solver* s, bool graphable) {
+    int rc;
+    const bool learned = s->p.schedule == MGSB_SCHED_LEARNED;
+    const bool gated = learned && s->p.learn_gate && !s->recording && s->gate_left > 0;
+    if (gated) s->gate_left--;
+    if ((rc = run_piece(s, gated ? kPieceStartGated : kPieceStart, graphable))) return rc;
+    const int check = s->p.check_every;
+    const int maxh = 2 * s->p.max_cycles;
+    int h = 0;
+    int target = check ? std::min(maxh, std::max(0, s->halves_guess)) : maxh;
+    for (;;) {
+        for (; h < target; h++)
+            if ((rc = run_piece(s, (h & 1) ? kPiecePost : kPiecePre, graphable))) return rc;
+        if (!check) break;
+        if (check == 1 && (h & 1)) {  // a pre half carries no check: finish the cycle
+            target = h + 1;
+            continue;
+        }
+        HIPCHK(hipStreamSynchronize(s->stream));
+        if (s->hmirror[0] || h >= maxh) {
+            s->halves_guess = s->hmirror[0] ? s->hmirror[2] : maxh;
+            break;
+        }
+        target = h + (check == 2 ? 1 : 2);
+    }
+    if (learned && !gated && check) {  // the mirror holds this step's judge verdict
+        if (s->hmirror[3] & 2) {
+            if (++s->gate_rejects >= kGateAfter) {
+                s->gate_skip = s->gate_skip ? std::min(2 * s->gate_skip, kGateMax) : kGateFirst;
+                s->gate_left = s->gate_skip;
+                s->gate_rejects = 0;
+            }
+        } else if (s->hmirror[4] == 2) {  // accepted (else the stage was skipped: no verdict)
+            s->gate_rejects = 0;
+            s->gate_skip = 0;
+        }
+    }
+    return run_piece(s, kPieceFinish, graphable);
 }
 
 int enqueue_step(mgsb_solver* s) {
     switch (s->p.schedule) {
         case MGSB_SCHED_SINGLE: return enqueue_step_single(s);
         case MGSB_SCHED_TWOLEVEL: return enqueue_step_twolevel(s);
-        case MGSB_SCHED_VCYCLE: return enqueue_step_vcycle(s, false);
-        case MGSB_SCHED_LEARNED: return enqueue_step_vcycle(s, true);
         default: set_error("schedule not available"); return MGSB_ERR_ARG;
     }
 }
@@ -1687,12 +1932,21 @@ int setup_vcycle(mgsb_solver* s) {
         HIPCHK(hipFree(cc.W));
         cc.W = nullptr;
     }
+    if (cc.Wt) {
+        HIPCHK(hipFree(cc.Wt));
+        cc.Wt = nullptr;
+    }
     int nfree_est = 0;
     for (float v : cc.vfix) nfree_est += v != 0.f;
     if (nfree_est <= kMaxDense && coarse_cholesky(cc.h, cc.vfix, dt, s->p.stiffness, dense, W, n) == 0 && n > 0) {
         int rc = dalloc(&cc.W, W.size());
         if (rc) return rc;
+        if ((rc = dalloc(&cc.Wt, W.size()))) return rc;
+        std::vector<double> Wt(W.size());
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) Wt[(size_t)i * n + j] = W[(size_t)j * n + i];
         HIPCHK(hipMemcpyAsync(cc.W, W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(cc.Wt, Wt.data(), sizeof(double) * Wt.size(), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipMemcpyAsync(cc.dense, dense.data(), sizeof(int) * dense.size(), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
         cc.nfree = n;
@@ -1722,6 +1976,7 @@ int setup_vcycle(mgsb_solver* s) {
             t.iw = l + 1 < nl ? s->M[l].iw : nullptr;
             t.dense = d.dense;
             t.W = d.W;
+            t.Wt = d.Wt;
             t.rho = (float)d.rho;
             tl.push_back(t);
         }
@@ -1744,6 +1999,9 @@ int alloc_vcycle(mgsb_solver* s) {
         if ((rc = up<float4>(s, d.owned, &d.r, nullptr, d.V))) return rc;
         if (l == 0) {
             if ((rc = up<float>(s, d.owned, &d.Fs, nullptr, (size_t)d.T * 9))) return rc;
+            std::vector<float4> z(d.V, make_float4(0.f, 0.f, 0.f, 0.f));
+            if ((rc = up(s, d.owned, &d.U1, z.data(), d.V))) return rc;
+            if ((rc = up(s, d.owned, &d.U2, z.data(), d.V))) return rc;
             continue;
         }
         if ((rc = up<float4>(s, d.owned, &d.e, nullptr, d.V))) return rc;
@@ -1777,6 +2035,11 @@ int alloc_vcycle(mgsb_solver* s) {
     s->owned.push_back(s->vstate);
     if ((rc = dalloc(&s->vpart, (size_t)nblk(s->L[0].V, 256)))) return rc;
     s->owned.push_back(s->vpart);
+    if ((rc = dalloc(&s->vcount, 1))) return rc;
+    s->owned.push_back(s->vcount);
+    HIPCHK(hipMemsetAsync(s->vcount, 0, sizeof(unsigned), s->stream));
+    HIPCHK(hipHostMalloc((void**)&s->hmirror, 8 * sizeof(int), hipHostMallocDefault));
+    std::memset(s->hmirror, 0, 8 * sizeof(int));
     return push_vstate(s);
 }
 
@@ -1809,7 +2072,7 @@ int residual_norm(mgsb_solver* s, double* out) {
     const int nb = nblk(d.V, 256);
     if ((rc = ensure_partials(s, (size_t)nb * 4))) return rc;
     k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, nullptr, s->dpart,
-                                       nullptr, nullptr, 0);
+                                       nullptr, nullptr, 0, nullptr, nullptr, kChkNone, nullptr);
     HIPCHK(hipGetLastError());
     double sq = 0;
     if ((rc = reduce_finish(s, nb, 1, &sq))) return rc;
@@ -1866,6 +2129,14 @@ int validate_params(const mgsb_params* p, int nlevels) {
         set_error("sweep counts must be >= 0 (coarse_sweeps >= 1), max_cycles >= 0");
         return MGSB_ERR_ARG;
     }
+    if (p->warm_start < 0 || p->warm_start > 1 || p->learn_gate < 0 || p->learn_gate > 1) {
+        set_error("warm_start and learn_gate must be 0 or 1");
+        return MGSB_ERR_ARG;
+    }
+    if (p->check_every < 0 || p->check_every > 2) {
+        set_error("check_every must be 0 (no checks), 1 (after each cycle) or 2 (also after pre-smoothing)");
+        return MGSB_ERR_ARG;
+    }
     if (nlevels > 8) {
         set_error("at most 8 levels");
         return MGSB_ERR_ARG;
@@ -1893,6 +2164,14 @@ int invalidate_graph(mgsb_solver* s) {
     s->gexec = nullptr;
     s->graph = nullptr;
     s->graph_valid = false;
+    s->gate_rejects = s->gate_skip = s->gate_left = 0;
+    for (int q = 0; q < 5; q++) {
+        if (s->pexec[q]) HIPCHK(hipGraphExecDestroy(s->pexec[q]));
+        if (s->pgraph[q]) HIPCHK(hipGraphDestroy(s->pgraph[q]));
+        s->pexec[q] = nullptr;
+        s->pgraph[q] = nullptr;
+        s->pvalid[q] = false;
+    }
     return MGSB_OK;
 }
 
@@ -1982,8 +2261,8 @@ int mgsb_create(const mgsb_mesh* levels, int32_t nlevels, const mgsb_params* p, 
 
 void mgsb_destroy(mgsb_solver* s) {
     if (!s) return;
-    hipSetDevice(s->device);
-    if (s->dpart) hipFree(s->dpart);
+    (void)hipSetDevice(s->device);
+    if (s->dpart) (void)hipFree(s->dpart);
     delete s;
 }
 
@@ -2115,6 +2394,10 @@ int mgsb_set_positions(mgsb_solver* s, int32_t level, const float* x) {
         return MGSB_ERR_ARG;
     }
     HIPCHK(hipSetDevice(s->device));
+    if (level == 0 && s->L[0].U1) {  // the temporal warm start's history no longer applies
+        HIPCHK(hipMemsetAsync(s->L[0].U1, 0, sizeof(float4) * s->L[0].V, s->stream));
+        HIPCHK(hipMemsetAsync(s->L[0].U2, 0, sizeof(float4) * s->L[0].V, s->stream));
+    }
     return write_f4(s, s->L[level].x, s->L[level].V, x);
 }
 
@@ -2125,17 +2408,20 @@ int mgsb_step(mgsb_solver* s, int32_t nsteps, mgsb_step_stats* stats) {
     }
     HIPCHK(hipSetDevice(s->device));
     int rc;
-    const bool graphable = s->p.use_graph && !s->timing && !s->recording;
+    const bool vcycle = s->p.schedule == MGSB_SCHED_VCYCLE || s->p.schedule == MGSB_SCHED_LEARNED;
     for (int st = 0; st < nsteps; st++) {
         if (s->recording && s->rec_steps >= s->rec_cap) s->recording = false;
-        if (graphable) {
+        const bool graphable = s->p.use_graph && !s->timing && !s->recording;
+        if (vcycle) {
+            if ((rc = step_vcycle(s, graphable))) return rc;
+        } else if (graphable) {
             if (!s->graph_valid) {
                 HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
                 rc = enqueue_step(s);
                 hipGraph_t g = nullptr;
                 hipError_t ce = hipStreamEndCapture(s->stream, &g);
                 if (rc) {
-                    if (g) hipGraphDestroy(g);
+                    if (g) (void)hipGraphDestroy(g);
                     return rc;
                 }
                 HIPCHK(ce);
@@ -2157,6 +2443,8 @@ int mgsb_step(mgsb_solver* s, int32_t nsteps, mgsb_step_stats* stats) {
         s->last.res = v.res;
         s->last.rel = v.res0 > 0 ? v.res / v.res0 : 0.0;
         s->last.cycles = v.cycles;
+        s->last.halves = v.halves;
+        s->last.fallback = v.fallback;
     }
     if (stats) *stats = s->last;
     HIPCHK(hipGetLastError());
@@ -2312,9 +2600,11 @@ int mgsb_run_stats(mgsb_solver* s, mgsb_run_stats_t* out, int32_t reset) {
     out->steps = v.nsteps;
     out->failed_steps = v.nfail;
     out->fallbacks = v.nfallback;
+    out->total_halves = v.total_halves;
     if (reset) {
         v.max_rel = 0.0;
         v.total_cycles = 0;
+        v.total_halves = 0;
         v.nsteps = 0;
         v.nfail = 0;
         v.nfallback = 0;
@@ -2476,6 +2766,17 @@ int mgsb_learn_norm(mgsb_solver* s, float* norm, int32_t set) {
         HIPCHK(hipMemcpyAsync(norm, s->norm, nb, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     return MGSB_OK;
+}
+
+int mgsb_learn_last(mgsb_solver* s, float* e1) {
+    int rc = need_mlp(s, "mgsb_learn_last");
+    if (rc) return rc;
+    if (!e1) {
+        set_error("mgsb_learn_last: null array");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    return read_f4(s, s->epred, s->L[1].V, e1);
 }
 
 int mgsb_mlp_forward(mgsb_solver* s, const float* x, int32_t n, float* y, int32_t path) {

@@ -88,6 +88,7 @@ _SIGS = {
     "ml_nfree": (C.c_int, [_vp]),
     "ml_get_vfix": (None, [_vp, C.c_int, _fp]),
     "ml_stats": (None, [_vp, _dp]),
+    "ml_set_warm": (None, [_vp, _fp, _fp]),
     "step_vcycle": (None, [_vp, C.POINTER(orc_params), C.POINTER(orc_vparams), C.c_int]),
     "ml_coarse_solve": (None, [_vp, _fp, _fp]),
     "mlp_forward": (None, [_fp, C.c_int, C.c_int, _fp, C.c_int, _dp, C.c_int]),
@@ -313,9 +314,19 @@ class MultiLevel:
         return e.reshape(-1, 3)
 
     def stats(self):
-        out = np.zeros(3, np.float64)
+        out = np.zeros(7, np.float64)
         self._f("ml_stats")(self._h, out.ctypes.data_as(_dp))
-        return {"res0": out[0], "res": out[1], "cycles": int(out[2]), "rel": out[1] / out[0] if out[0] else 0.0}
+        return {"res0": out[0], "res": out[1], "cycles": int(out[2]), "halves": int(out[3]),
+                "fallback": int(out[4]), "res_ws": out[5], "res_t": out[6],
+                "rel": out[1] / out[0] if out[0] else 0.0}
+
+    def set_warm(self, e1=None, u0=None):
+        """Warm start of the next step: x = s + fix u0 + P e1 (level-1
+        correction e1 (V1, 3), fine correction u0 (V0, 3)); None omits a part."""
+        a = None if e1 is None else np.ascontiguousarray(e1, np.float32).reshape(-1)
+        b = None if u0 is None else np.ascontiguousarray(u0, np.float32).reshape(-1)
+        self._f("ml_set_warm")(self._h, None if a is None else a.ctypes.data_as(_fp),
+                               None if b is None else b.ctypes.data_as(_fp))
 
 
 MLP_H = 128

@@ -1135,8 +1135,13 @@ typedef struct orc_ml {
     int* dense;          /* coarsest vertex -> dense index or -1 */
     double* chol;        /* lower Cholesky factor, nfree^2       */
     double res0, res;
-    int cycles;
+    int cycles, halves;
     float relax_coarse, dt;
+    real* warm;          /* pending warm start: level-1 correction (V1 x 3) */
+    real* warm0;         /* ... and a fine-level correction (V0 x 3)        */
+    int has_warm0;
+    int has_warm, fallback;
+    double res_ws, res_t;
 } orc_ml;
 
 /* q = A e = m/h^2 e - f_lin(e) on level l (f_lin from the linear tet pass). */
@@ -1431,6 +1436,8 @@ void PFX(ml_free)(orc_ml* M) {
     }
     free(M->dense);
     free(M->chol);
+    free(M->warm);
+    free(M->warm0);
     free(M);
 }
 
@@ -1443,8 +1450,36 @@ void PFX(ml_stats)(const orc_ml* M, double* out) {
     out[0] = M->res0;
     out[1] = M->res;
     out[2] = M->cycles;
+    out[3] = M->halves;
+    out[4] = M->fallback;
+    out[5] = M->res_ws;
+    out[6] = M->res_t;
 }
 
+/* Warm start of the next step: x = s + fix * u0 + P e1, with e1 the
+ * learned schedule's coarse correction (already masked by the level-1
+ * Dirichlet flags) and u0 a fine-level correction; either may be NULL. */
+void PFX(ml_set_warm)(orc_ml* M, const float* e1, const float* u0) {
+    int V1 = M->L[1]->V, V0 = M->L[0]->V;
+    M->has_warm = 0;
+    M->has_warm0 = 0;
+    if (e1) {
+        if (!M->warm) M->warm = (real*)xcalloc((size_t)V1 * 3, sizeof(real));
+        for (int i = 0; i < V1 * 3; i++) M->warm[i] = e1[i];
+        M->has_warm = 1;
+    }
+    if (u0) {
+        if (!M->warm0) M->warm0 = (real*)xcalloc((size_t)V0 * 3, sizeof(real));
+        for (int i = 0; i < V0 * 3; i++) M->warm0[i] = u0[i];
+        M->has_warm0 = 1;
+    }
+}
+
+/* One residual-correction V-cycle step (the device schedule MGSB_SCHED_VCYCLE).
+ * check: 0 no checks (max_cycles cycles), 1 after every cycle, 2 also after
+ * the pre-smoothing half (the residual the restriction uses).  cycles counts
+ * completed cycles, halves the half-cycles run (pre-smoothing half, coarse
+ * correction + post-smoothing half). */
 void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nsteps) {
     orc_level* L0 = M->L[0];
     real rs = vp->rho_smooth;
@@ -1454,14 +1489,60 @@ void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nstep
         M->res0 = ml_fine_residual(M, p);
         M->res = M->res0;
         M->cycles = 0;
-        for (int c = 0; c < vp->max_cycles; c++) {
-            if (vp->check && M->res <= (double)vp->tol * M->res0) break;
+        M->halves = 0;
+        M->fallback = 0;
+        M->res_ws = 0.0;
+        M->res_t = 0.0;
+        int conv = vp->check && M->res0 == 0.0;
+        /* warm start in two stages, each kept only if it lowers the best
+         * residual so far: x = s + fix u0 (temporal), then x += P e1 (learned) */
+        int accepted = 0;
+        if (M->has_warm0 && !conv) {
+            for (int i = 0; i < L0->V; i++)
+                for (int c = 0; c < 3; c++) L0->x[i * 3 + c] = L0->x[i * 3 + c] + L0->fix[i] * M->warm0[i * 3 + c];
+            double rt = ml_fine_residual(M, p);
+            M->res_t = rt;
+            if (rt > M->res) {
+                M->fallback |= 1;
+                memcpy(L0->x, L0->s, sizeof(real) * 3 * (size_t)L0->V);
+                ml_fine_residual(M, p);
+            } else {
+                M->res = rt;
+                accepted = 1;
+                conv = vp->check && rt <= (double)vp->tol * M->res0;
+            }
+        }
+        if (M->has_warm && !conv) {
+            ml_prolong_add(M, 0, M->warm, L0->x, L0->fix);
+            double rw = ml_fine_residual(M, p);
+            M->res_ws = rw;
+            if (rw > M->res) {
+                M->fallback |= 2;
+                memcpy(L0->x, L0->s, sizeof(real) * 3 * (size_t)L0->V);
+                if (accepted == 1)
+                    for (int i = 0; i < L0->V; i++)
+                        for (int c = 0; c < 3; c++)
+                            L0->x[i * 3 + c] = L0->x[i * 3 + c] + L0->fix[i] * M->warm0[i * 3 + c];
+                ml_fine_residual(M, p);
+            } else {
+                M->res = rw;
+                conv = vp->check && rw <= (double)vp->tol * M->res0;
+            }
+        }
+        M->has_warm = 0;
+        M->has_warm0 = 0;
+        for (int c = 0; c < vp->max_cycles && !conv; c++) {
             real omega = 1;
             for (int k = 0; k < vp->pre; k++) {
                 omega = 4 / (4 - rs * rs * omega);
                 pd_iteration(L0, p, omega, 0);
             }
-            ml_fine_residual(M, p);
+            double rm = ml_fine_residual(M, p);
+            if (vp->check == 2) {
+                M->res = rm;
+                M->halves++;
+                if (rm <= (double)vp->tol * M->res0) break;
+            }
             ml_restrict(M, 0, M->r[0], M->r[1]);
             ml_vcycle_lin(M, 1, vp);
             ml_prolong_add(M, 0, M->e[1], L0->x, L0->fix);
@@ -1473,6 +1554,8 @@ void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nstep
             }
             M->res = ml_fine_residual(M, p);
             M->cycles++;
+            M->halves += vp->check == 2 ? 1 : 2;
+            conv = vp->check && M->res <= (double)vp->tol * M->res0;
         }
         calc_v(L0, p);
     }

@@ -68,28 +68,137 @@ def test_mlp_gradients_match_oracle():
         o += n
 
 
-def test_learned_schedule_record_train_ab():
-    plain = small_solver(M.SCHED_VCYCLE)
-    plain.learn_record(150)
-    plain.step(150)
-    assert plain.learn_samples() == 150 * 44
-    hist = plain.learn_train(epochs=30, batch=512, lr=2e-3, seed=0)
-    print(f"[learned] loss first {hist[0]:.4f} last {hist[-1]:.4f}")
-    assert hist[-1] < 0.5 * hist[0]
-    w, norm = plain.learn_weights(), plain.learn_norm()
-    # A/B from the same state: continue the plain run vs a learned copy
-    learned = small_solver(M.SCHED_LEARNED)
-    learned.learn_weights(w)
-    learned.learn_norm(norm)
-    learned.learn_record(150)  # to carry the previous-correction feature
-    learned.step(150)
-    learned.learn_record(0)
-    for sol in (plain, learned):
-        sol.run_stats(reset=True)
-    plain.step(60)
-    learned.step(60)
-    a, b = plain.run_stats(), learned.run_stats()
-    print(f"[learned A/B] plain cycles/step {a['total_cycles'] / a['steps']:.2f} learned "
-          f"{b['total_cycles'] / b['steps']:.2f} fallbacks {b['fallbacks']} max rel {b['max_rel']:.2e}")
-    assert b["failed_steps"] == 0 and b["max_rel"] <= 1e-3
-    assert a["failed_steps"] == 0
+def oracle_pair(ns, fp64=False):
+    lv = W.beam_hierarchy(ns)
+    phys = W.physics()
+    fix = W.top_fixflag(lv[0])
+    Ls = [O.Level(m.pos, m.tets, m.tris, stiffness=phys["stiffness"], density=phys["density"], fp64=fp64)
+          for m in lv]
+    Ls[0].set_fix(fix)
+    Ls[0].set_velocity(W.seeded_velocity(lv[0], fix))
+    ML = O.MultiLevel(Ls, relax_coarse=0.6, dt=phys["dt"])
+    p = O.params(dt=phys["dt"], stiffness=phys["stiffness"], density=phys["density"], gravity=phys["gravity"],
+                 if_mode=3)
+    return lv, Ls, ML, p, fix
+
+
+VP = dict(pre=5, post=5, coarse_sweeps=2, max_cycles=12, tol=1e-3, rho_smooth=0.9)
+
+
+def rel_err(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+class Temporal:
+    """Temporal extrapolation u_t = 2 u_{n-1} - u_{n-2} of an oracle run."""
+
+    def __init__(self, L0, fix):
+        self.L0, self.fix = L0, fix[:, None]
+        self.u1 = self.u2 = np.zeros((L0.nv, 3), np.float32)
+
+    def warm(self):
+        return (self.fix * (2 * self.u1 - self.u2)).astype(np.float32)
+
+    def push(self):
+        u = self.L0.positions() - self.L0.get(O.S).reshape(-1, 3)
+        self.u2, self.u1 = self.u1, u.astype(np.float32)
+
+
+@pytest.mark.parametrize("check", [1, 2])
+def test_temporal_warm_start_matches_oracle(check):
+    lv, Ls, ML, p, fix = oracle_pair((4, 1))
+    g = small_solver(M.SCHED_VCYCLE, warm_start=1, check_every=check)
+    vp = O.vparams(check=check, **VP)
+    hist = Temporal(Ls[0], fix)
+    agree = 0
+    for k in range(60):
+        st = g.step(1)
+        ML.set_warm(None, hist.warm())
+        ML.step(p, vp, 1)
+        hist.push()
+        o = ML.stats()
+        agree += (st.halves, st.fallback) == (o["halves"], o["fallback"])
+        assert st.rel <= 1e-3
+    err = rel_err(g.positions(), Ls[0].positions())
+    print(f"[temporal check={check}] rel(gpu,oracle)={err:.2e} agree {agree / 60:.2f} last halves {st.halves}")
+    assert err < 1e-4 and agree >= 54
+
+
+def record_and_train(ns, steps, epochs, seed=0):
+    rec = small_solver(M.SCHED_VCYCLE, warm_start=1, check_every=2) if ns == (4, 1) else None
+    if rec is None:
+        rec = medium_solver(M.SCHED_VCYCLE, warm_start=1)
+    rec.learn_record(steps)
+    rec.step(steps)
+    assert rec.learn_samples() == steps * rec.levels[1].nverts
+    hist = rec.learn_train(epochs=epochs, batch=512, lr=2e-3, seed=seed)
+    return rec, hist
+
+
+def medium_solver(schedule, **kw):
+    lv = W.beam_hierarchy((12, 4, 1))
+    base = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, check_every=2, tol=1e-3,
+                rho_smooth=0.9, use_graph=1)
+    base.update(kw)
+    s = M.Solver(lv, schedule=schedule, if_mode=M.IF_STABLE, **base, **W.physics())
+    s.fix_plane(1, 10.0, 1e-5)
+    s.set_velocity(W.seeded_velocity(lv[0], W.top_fixflag(lv[0])))
+    return s
+
+
+def test_learned_schedule_matches_oracle_given_prediction():
+    """The learned schedule's machinery (temporal stage, prolongated coarse
+    prediction, two-stage judge, cycles) against the oracle fed the same
+    coarse prediction; the MLP itself is checked against the oracle above."""
+    rec, hist = record_and_train((4, 1), 100, 20)
+    w, norm = rec.learn_weights(), rec.learn_norm()
+    lv, Ls, ML, p, fix = oracle_pair((4, 1))
+    g = small_solver(M.SCHED_LEARNED, check_every=2, learn_gate=0)
+    g.learn_weights(w)
+    g.learn_norm(norm)
+    vp = O.vparams(check=2, **VP)
+    tmp = Temporal(Ls[0], fix)
+    agree = used = 0
+    for k in range(60):
+        st = g.step(1)
+        ML.set_warm(g.learn_last(), tmp.warm())
+        ML.step(p, vp, 1)
+        tmp.push()
+        o = ML.stats()
+        agree += (st.halves, st.fallback) == (o["halves"], o["fallback"])
+        used += (st.fallback & 2) == 0
+        assert st.rel <= 1e-3
+    err = rel_err(g.positions(), Ls[0].positions())
+    print(f"[learned vs oracle] rel={err:.2e} agree {agree / 60:.2f} prediction kept in {used}/60 steps")
+    assert err < 1e-4 and agree >= 54
+    assert used > 0
+
+
+def test_learned_ab_medium_box():
+    """What the trained network gains on the 103,680-tet box: half-cycles
+    per step to rel. residual 1e-3 of the plain V-cycle (start from s), the
+    V-cycle with the temporal warm start alone, and the learned schedule
+    (temporal warm start + MLP coarse delta), after the same 200 steps."""
+    rec, hist = record_and_train((12, 4, 1), 200, 20)
+    print(f"[learned] training loss first {hist[0]:.4f} last {hist[-1]:.4f}")
+    assert hist[-1] < 0.7 * hist[0]
+    w, norm = rec.learn_weights(), rec.learn_norm()
+    sols = {"plain": medium_solver(M.SCHED_VCYCLE, check_every=1),
+            "temporal": medium_solver(M.SCHED_VCYCLE, warm_start=1),
+            "learned": medium_solver(M.SCHED_LEARNED)}
+    sols["learned"].learn_weights(w)
+    sols["learned"].learn_norm(norm)
+    res = {}
+    for name, s in sols.items():
+        s.step(200)
+        s.run_stats(reset=True)
+        s.step(100)
+        r = s.run_stats()
+        res[name] = r["total_halves"] / r["steps"]
+        assert r["failed_steps"] == 0 and r["max_rel"] <= 1e-3, (name, r)
+        print(f"[A/B medium] {name:9s} half-cycles/step {res[name]:.2f} fallbacks {r['fallbacks']} "
+              f"max rel {r['max_rel']:.2e}")
+    # the temporal stage carries the schedule on this smooth workload; the
+    # network's correction must not cost half-cycles (the judge undoes it)
+    assert res["temporal"] < 0.5 * res["plain"]
+    assert res["learned"] <= res["temporal"] + 0.05

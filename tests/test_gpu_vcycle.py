@@ -61,37 +61,50 @@ def test_vcycle_fixed_cycles_matches_oracle(ns, rs):
     assert st.rel <= max(2 * r64, 1e-5), (st.rel, r64)
 
 
-def test_vcycle_adaptive_meets_target_each_step():
-    kw = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, check_every=1, tol=1e-3, rho_smooth=0.9)
+@pytest.mark.parametrize("check", [1, 2])
+def test_vcycle_adaptive_meets_target_each_step(check):
+    kw = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, check_every=check, tol=1e-3,
+              rho_smooth=0.9)
     lv, s, ML, Ls, p = make_pair((4, 1), None, **kw)
-    vp = O.vparams(pre=5, post=5, coarse_sweeps=2, max_cycles=12, check=1, tol=1e-3, rho_smooth=0.9)
+    vp = O.vparams(pre=5, post=5, coarse_sweeps=2, max_cycles=12, check=check, tol=1e-3, rho_smooth=0.9)
     gc, oc = [], []
     for _ in range(80):
         st = s.step(1)
         ML.step(p, vp, 1)
-        gc.append(st.cycles)
-        oc.append(ML.stats()["cycles"])
+        o = ML.stats()
+        gc.append((st.cycles, st.halves))
+        oc.append((o["cycles"], o["halves"]))
         assert st.rel <= 1e-3, st.rel
-    agree = np.mean(np.array(gc) == np.array(oc))
-    print(f"[vcycle adaptive] cycles gpu mean {np.mean(gc):.2f} oracle mean {np.mean(oc):.2f} agree {agree:.2f}")
+        if check == 1:
+            assert st.halves == 2 * st.cycles
+    agree = np.mean([a == b for a, b in zip(gc, oc)])
+    print(f"[vcycle adaptive check={check}] halves gpu mean {np.mean([h for _, h in gc]):.2f} "
+          f"oracle mean {np.mean([h for _, h in oc]):.2f} agree {agree:.2f}")
     assert agree >= 0.9
     assert rel_err(s.positions(), Ls[0].positions()) < REL_TOL
+    run = s.run_stats()
+    assert run["steps"] == 80 and run["failed_steps"] == 0
+    assert run["total_halves"] == sum(h for _, h in gc)
+    assert run["total_cycles"] == sum(c for c, _ in gc)
 
 
-def test_vcycle_graph_replay_matches_eager():
+@pytest.mark.parametrize("check", [1, 2])
+def test_vcycle_graph_replay_matches_eager(check):
+    """Captured pieces (start, half-cycles, finish) replayed by the host's
+    cycle loop give the eager trajectory bit for bit."""
     lv = W.beam_hierarchy((4, 1))
     fix = W.top_fixflag(lv[0])
     v0 = W.seeded_velocity(lv[0], fix)
     res = []
     for g in (0, 1):
-        s = M.Solver(lv, schedule=M.SCHED_VCYCLE, if_mode=M.IF_STABLE, max_cycles=8, check_every=1, tol=1e-3,
+        s = M.Solver(lv, schedule=M.SCHED_VCYCLE, if_mode=M.IF_STABLE, max_cycles=8, check_every=check, tol=1e-3,
                      use_graph=g, **W.physics())
         s.fix_plane(1, 10.0, 1e-5)
         s.set_velocity(v0)
         for _ in range(4):
             st = s.step(5)
             s.sync()
-            res.append((s.positions(), st.cycles, st.rel))
+            res.append((s.positions(), st.cycles, st.rel, st.halves))
     n = len(res) // 2
     for a, b in zip(res[:n], res[n:]):
-        assert np.array_equal(a[0], b[0]) and a[1] == b[1] and a[2] == b[2]
+        assert np.array_equal(a[0], b[0]) and a[1:] == b[1:]

@@ -104,3 +104,54 @@ def test_vcycle_reaches_tolerance_every_step(fp64):
         worst = max(worst, st["rel"])
         assert st["cycles"] <= 6
     assert worst <= 1e-3
+
+
+def _small_ml():
+    lv = W.beam_hierarchy((4, 1))
+    phys = W.physics()
+    Ls = [O.Level(m.pos, m.tets, m.tris, stiffness=phys["stiffness"], density=phys["density"]) for m in lv]
+    fix = W.top_fixflag(lv[0])
+    Ls[0].set_fix(fix)
+    Ls[0].set_velocity(W.seeded_velocity(lv[0], fix))
+    ML = O.MultiLevel(Ls, dt=phys["dt"])
+    p = O.params(dt=phys["dt"], stiffness=phys["stiffness"], density=phys["density"], gravity=phys["gravity"],
+                 if_mode=3)
+    return lv, Ls, ML, p
+
+
+def test_warm_start_judge_rejects_and_restores():
+    """A warm-start stage that raises the residual is undone exactly: the step
+    then equals the step without that stage, bit for bit; a stage that helps
+    is kept."""
+    lv, Ls, ML, p = _small_ml()
+    vp = O.vparams(pre=3, post=3, max_cycles=8, check=2, tol=1e-3, rho_smooth=0.9)
+    ML.step(p, vp, 10)
+    x0, v0 = Ls[0].positions().copy(), Ls[0].get(O.V).copy()
+    V1 = lv[1].nverts
+
+    def run(e1=None, u0=None):
+        Ls[0].set_positions(x0)
+        Ls[0].set_velocity(v0)
+        ML.set_warm(e1, u0)
+        ML.step(p, vp, 1)
+        return Ls[0].positions().copy(), ML.stats()
+
+    xp, sp = run()
+    assert sp["fallback"] == 0
+    # a wild fine correction: rejected, identical to the plain step
+    bad_u = np.full((lv[0].nverts, 3), 0.5, np.float32)
+    xb, sb = run(None, bad_u)
+    assert sb["fallback"] == 1 and sb["res_t"] > sb["res0"]
+    assert np.array_equal(xb, xp) and sb["halves"] == sp["halves"]
+    # the exact step correction as fine warm start: converged with no cycles
+    s = Ls[0].get(O.S).reshape(-1, 3)
+    u_exact = xp - s
+    xe, se = run(None, u_exact)
+    assert se["fallback"] == 0 and se["halves"] == 0 and se["res_t"] <= 1e-3 * se["res0"]
+    # good temporal stage, then a wild coarse stage: the coarse stage is undone
+    half = 0.5 * u_exact
+    xh, sh = run(None, half)
+    bad_e = np.full((V1, 3), 0.5, np.float32)
+    xhb, shb = run(bad_e, half)
+    assert shb["fallback"] == 2 and shb["res_ws"] > shb["res_t"]
+    assert np.array_equal(xhb, xh) and shb["halves"] == sh["halves"]
