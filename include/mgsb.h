@@ -285,6 +285,11 @@ int mgsb_level_info(mgsb_solver* s, int32_t level, int32_t* nverts, int32_t* nte
  * timing was switched on. */
 int mgsb_set_timing(mgsb_solver* s, int32_t on);
 int mgsb_kernel_time(mgsb_solver* s, int32_t kernel, double* ms, int64_t* launches);
+/* Throughput timing of kernel 0 as the current schedule launches it (split
+ * positions for the V-cycle schedules): reps back-to-back launches on the
+ * solver's stream between two HIP events; avg_us = elapsed / reps.  The
+ * launches overwrite the corner forces only (the next step recomputes them). */
+int mgsb_time_kernel(mgsb_solver* s, int32_t kernel, int32_t reps, double* avg_us);
 
 const char* mgsb_last_error(void);
 const char* mgsb_version(void);

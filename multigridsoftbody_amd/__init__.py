@@ -147,6 +147,7 @@ SIGNATURES = {
     "mgsb_coarse_nfree": (C.c_int, [_vp, _ip]),
     "mgsb_set_timing": (C.c_int, [_vp, C.c_int32]),
     "mgsb_kernel_time": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
+    "mgsb_time_kernel": (C.c_int, [_vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "mgsb_last_error": (C.c_char_p, []),
     "mgsb_version": (C.c_char_p, []),
 }
@@ -509,6 +510,12 @@ class Solver:
         n = C.c_int64()
         _check(lib.mgsb_kernel_time(self._h, kernel, C.byref(ms), C.byref(n)), "mgsb_kernel_time")
         return ms.value, n.value
+
+    def time_kernel(self, reps=200, kernel=0):
+        """Average microseconds of `reps` back-to-back launches of kernel 0."""
+        us = C.c_double()
+        _check(lib.mgsb_time_kernel(self._h, kernel, reps, C.byref(us)), "mgsb_time_kernel")
+        return us.value
 
 
 class PDSolver(Solver):

@@ -57,11 +57,11 @@ constexpr int kVertWriteCol = 4;  // write collision flags / depth / energy
 // before the adds that consume them, so a vertex waits on memory about once
 // per eight entries instead of once per entry; the summation order (and the
 // result, bit for bit) is that of the plain loop.
-__device__ __forceinline__ void gather_sum(const float4* __restrict__ cf, int e0, int e1, float& fx, float& fy,
+__device__ __forceinline__ void gather_sum(const float3* __restrict__ cf, int e0, int e1, float& fx, float& fy,
                                            float& fz) {
     int e = e0;
     for (; e + 8 <= e1; e += 8) {
-        float4 q[8];
+        float3 q[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) q[k] = cf[e + k];
 #pragma unroll
@@ -72,9 +72,9 @@ __device__ __forceinline__ void gather_sum(const float4* __restrict__ cf, int e0
         }
     }
     const int rem = e1 - e;
-    float4 q[8];
+    float3 q[8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) q[k] = k < rem ? cf[e + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k = 0; k < 8; k++) q[k] = k < rem ? cf[e + k] : make_float3(0.f, 0.f, 0.f);
 #pragma unroll
     for (int k = 0; k < 8; k++)
         if (k < rem) {
@@ -213,10 +213,10 @@ __device__ __forceinline__ bool tri_sphere_hit(float3 A, float3 B, float3 C, flo
 template <int MODE>
 __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x, const int4* __restrict__ tet,
                                              const float* __restrict__ B, const float* __restrict__ vol,
-                                             const int4* __restrict__ slot, float4* __restrict__ cf, float k,
+                                             const int4* __restrict__ slot, float3* __restrict__ cf, float k,
                                              int ntri, const int4* __restrict__ tri, const int4* __restrict__ tslot,
                                              float4* __restrict__ tcf, const DevColliders* __restrict__ col,
-                                             const int* __restrict__ skip, float* __restrict__ Fs, int fs_mode) {
+                                             const int* __restrict__ skip, const float4* __restrict__ sref) {
     if (skip && *skip) return;
     const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (g >= T) {
@@ -259,14 +259,21 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
     for (int r = 0; r < 3; r++)
 #pragma unroll
         for (int c = 0; c < 3; c++) F[r * 3 + c] = D[r * 3 + 0] * b[c] + D[r * 3 + 1] * b[3 + c] + D[r * 3 + 2] * b[6 + c];
-    // Split positions x = s + u (V-cycle steps): fs_mode 1 -- x holds s, F(s)
-    // is stored; fs_mode 2 -- x holds u and F = F(s) + Ds(u) Dm^-1.
-    if (fs_mode == 1) {
+    // Split positions x = s + u (V-cycle steps, sref = s): x holds u and
+    // F = F(s) + Ds(u) Dm^-1, with F(s) evaluated by the same operations as
+    // for any other position field (so it is the same value in every pass of
+    // the step) from s gathered at the corners.
+    if (sref) {
+        const float4 s0 = sref[id.x], s1 = sref[id.y], s2 = sref[id.z], s3 = sref[id.w];
+        float E[9];
+        E[0] = s1.x - s0.x; E[1] = s2.x - s0.x; E[2] = s3.x - s0.x;
+        E[3] = s1.y - s0.y; E[4] = s2.y - s0.y; E[5] = s3.y - s0.y;
+        E[6] = s1.z - s0.z; E[7] = s2.z - s0.z; E[8] = s3.z - s0.z;
 #pragma unroll
-        for (int q = 0; q < 9; q++) Fs[(size_t)q * T + g] = F[q];
-    } else if (fs_mode == 2) {
+        for (int r = 0; r < 3; r++)
 #pragma unroll
-        for (int q = 0; q < 9; q++) F[q] = Fs[(size_t)q * T + g] + F[q];
+            for (int c = 0; c < 3; c++)
+                F[r * 3 + c] = (E[r * 3 + 0] * b[c] + E[r * 3 + 1] * b[3 + c] + E[r * 3 + 2] * b[6 + c]) + F[r * 3 + c];
     }
     float RF[9];
     if (MODE == kTetModeLinear) {
@@ -305,7 +312,7 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
             if (MODE == kTetModeAc && fabsf(r) <= 1e-5f) r = 0.f;
             f[i] = r;
         }
-        cf[sls[j]] = make_float4(f[0], f[1], f[2], 0.f);
+        cf[sls[j]] = make_float3(f[0], f[1], f[2]);
     }
 }
 
@@ -315,7 +322,7 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
 // DCDByPoint_sphere (PDSolver.cu:277-331) including its 1.5x radius and t/2.
 __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, float4* __restrict__ xprev,
                                               const float4* __restrict__ s, const float4* __restrict__ vc,
-                                              const int* __restrict__ rp, const float4* __restrict__ cf,
+                                              const int* __restrict__ rp, const float3* __restrict__ cf,
                                               const int* __restrict__ trp, const float4* __restrict__ tcf,
                                               const DevColliders* __restrict__ col, float omega, float relax,
                                               int flags, int* __restrict__ cflag, float* __restrict__ cdepth,
@@ -700,7 +707,7 @@ __device__ void last_block_check(const double* part, unsigned* count, VState* st
 // mode != kChkNone the last block completes the norm and applies the check.
 __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__ x, const float4* __restrict__ s,
                                                const float4* __restrict__ vc, const int* __restrict__ rp,
-                                               const float4* __restrict__ cf, const DevColliders* __restrict__ col,
+                                               const float3* __restrict__ cf, const DevColliders* __restrict__ col,
                                                int flags, float4* __restrict__ r, double* __restrict__ part,
                                                float4* __restrict__ xprev, const int* __restrict__ skip, int split,
                                                VState* st, unsigned* count, int mode, int* hm) {
@@ -1271,7 +1278,7 @@ struct DevLevel {
     int4 *tet = nullptr, *slot = nullptr;
     float *B = nullptr, *vol = nullptr;
     int* rp = nullptr;
-    float4* cf = nullptr;
+    float3* cf = nullptr;  // corner forces in vertex-sorted (CSR) order
     int4 *tri = nullptr, *tslot = nullptr;
     int *trp = nullptr, *trid = nullptr;
     float4 *tcf = nullptr, *tn = nullptr, *vn = nullptr;
@@ -1286,7 +1293,6 @@ struct DevLevel {
     int *ap = nullptr, *aj = nullptr;  // scalar level matrix A = m/h^2 + L (coarse levels)
     float* av = nullptr;
     std::vector<float> adiag;          // its diagonal
-    float* Fs = nullptr;               // F(s) per tet (9 SoA streams), split-position steps
     float4 *U1 = nullptr, *U2 = nullptr;  // corrections u = x - s of the last two steps
     int cur = 0;                       // which of e / ep holds the correction
     int* dense = nullptr;
@@ -1426,7 +1432,7 @@ int upload_level(mgsb_solver* s, DevLevel& d) {
     if ((rc = up(s, d.owned, &d.B, bs.data(), bs.size()))) return rc;
     if ((rc = up(s, d.owned, &d.vol, h.vol.data(), h.T))) return rc;
     if ((rc = up(s, d.owned, &d.rp, h.rp.data(), h.rp.size()))) return rc;
-    if ((rc = up<float4>(s, d.owned, &d.cf, nullptr, (size_t)h.T * 4))) return rc;
+    if ((rc = up<float3>(s, d.owned, &d.cf, nullptr, (size_t)h.T * 4))) return rc;
     std::vector<int4> tr4(std::max(1, h.S)), ts4(std::max(1, h.S));
     for (int t = 0; t < h.S; t++) {
         tr4[t] = make_int4((int)h.tris[t * 3], (int)h.tris[t * 3 + 1], (int)h.tris[t * 3 + 2], 0);
@@ -1496,8 +1502,10 @@ int timed_end(mgsb_solver* s, int kernel) {
     return MGSB_OK;
 }
 
+// split: x holds the in-step correction u and the kernel adds F(s) (V-cycle steps).
 int launch_tet(mgsb_solver* s, DevLevel& d, int mode, bool with_tri, const float4* xsrc = nullptr,
-               const int* skip = nullptr, int fs_mode = 0) {
+               const int* skip = nullptr, bool split = false) {
+    const float4* sref = split ? d.s : nullptr;
     const int bs = block_tet(s);
     const int ntri = with_tri ? d.S : 0;
     const int n = d.T + ntri;
@@ -1510,19 +1518,19 @@ int launch_tet(mgsb_solver* s, DevLevel& d, int mode, bool with_tri, const float
     switch (mode) {
         case kTetModePolar:
             k_tet<kTetModePolar><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness,
-                                                           ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, d.Fs, fs_mode);
+                                                           ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, sref);
             break;
         case kTetModeAc:
             k_tet<kTetModeAc><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness, ntri,
-                                                        d.tri, d.tslot, d.tcf, s->dcol, skip, d.Fs, fs_mode);
+                                                        d.tri, d.tslot, d.tcf, s->dcol, skip, sref);
             break;
         case kTetModeStable:
             k_tet<kTetModeStable><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness,
-                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, d.Fs, fs_mode);
+                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, sref);
             break;
         default:
             k_tet<kTetModeLinear><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness,
-                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, d.Fs, fs_mode);
+                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, sref);
     }
     HIPCHK(hipGetLastError());
     return timed_end(s, kid);
@@ -1676,8 +1684,7 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool res
                   bool first = false) {
     DevLevel& d = s->L[0];
     int rc;
-    if (need_tet && (rc = launch_tet(s, d, tet_mode_of(s->p.if_mode), false, first ? d.s : d.x, skip,
-                                     first ? 1 : 2)))
+    if (need_tet && (rc = launch_tet(s, d, tet_mode_of(s->p.if_mode), false, first ? d.s : d.x, skip, !first)))
         return rc;
     const int nb = nblk(d.V, 256);
     k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, write_r ? d.r : nullptr,
@@ -1738,7 +1745,7 @@ int enqueue_vc_start(mgsb_solver* s, bool learned) {
         HIPCHK(hipGetLastError());
     }
     // after a revert the corner forces belong to the rejected x: recompute
-    if (temporal && (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, &s->vstate->cf_ok, 2)))
+    if (temporal && (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, &s->vstate->cf_ok, true)))
         return rc;
     return MGSB_OK;
 }
@@ -1754,7 +1761,7 @@ int enqueue_vc_pre(mgsb_solver* s) {
     float omega = 1.0f;
     for (int k = 0; k < s->p.pre_sweeps; k++) {
         omega = cheb_next(omega, rs);
-        if (k > 0 && (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, 2))) return rc;
+        if (k > 0 && (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, true))) return rc;
         if ((rc = launch_vert(s, f, omega, kVertPoint, skip, 1))) return rc;
     }
     return fine_residual(s, s->p.pre_sweeps > 0, true, s->p.check_every == 2 ? kChkMid : kChkNone, false, skip);
@@ -1778,7 +1785,7 @@ int enqueue_vc_post(mgsb_solver* s) {
     float omega = 1.0f;
     for (int k = 0; k < s->p.post_sweeps; k++) {
         omega = cheb_next(omega, rs);
-        if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, 2))) return rc;
+        if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, true))) return rc;
         if ((rc = launch_vert(s, f, omega, kVertPoint | (k == s->p.post_sweeps - 1 ? kVertWriteCol : 0), skip, 1)))
             return rc;
     }
@@ -1998,7 +2005,6 @@ int alloc_vcycle(mgsb_solver* s) {
         DevLevel& d = s->L[l];
         if ((rc = up<float4>(s, d.owned, &d.r, nullptr, d.V))) return rc;
         if (l == 0) {
-            if ((rc = up<float>(s, d.owned, &d.Fs, nullptr, (size_t)d.T * 9))) return rc;
             std::vector<float4> z(d.V, make_float4(0.f, 0.f, 0.f, 0.f));
             if ((rc = up(s, d.owned, &d.U1, z.data(), d.V))) return rc;
             if ((rc = up(s, d.owned, &d.U2, z.data(), d.V))) return rc;
@@ -2864,6 +2870,36 @@ int mgsb_kernel_time(mgsb_solver* s, int32_t kernel, double* ms, int64_t* launch
     s->ev_used = 0;
     if (ms) *ms = s->time_ms;
     if (launches) *launches = s->time_launches;
+    return MGSB_OK;
+}
+
+int mgsb_time_kernel(mgsb_solver* s, int32_t kernel, int32_t reps, double* avg_us) {
+    if (!s || kernel != 0 || reps < 1 || !avg_us) {
+        set_error("mgsb_time_kernel: kernel must be 0, reps >= 1, avg_us non-null");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    const bool was = s->timing;
+    s->timing = false;
+    const bool split = s->p.schedule == MGSB_SCHED_VCYCLE || s->p.schedule == MGSB_SCHED_LEARNED;
+    DevLevel& d = s->L[0];
+    int rc = launch_tet(s, d, tet_mode_of(s->p.if_mode), false, nullptr, nullptr, split);  // warm-up
+    if (rc) return rc;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, s->stream));
+    for (int r = 0; r < reps && !rc; r++) rc = launch_tet(s, d, tet_mode_of(s->p.if_mode), false, nullptr, nullptr, split);
+    hipError_t e = hipEventRecord(b, s->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    s->timing = was;
+    if (rc) return rc;
+    HIPCHK(e);
+    *avg_us = 1e3 * (double)ms / reps;
     return MGSB_OK;
 }
 
