@@ -37,6 +37,10 @@ HBM_PEAK_GBS = 8000.0
 VCYCLE = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, tol=1e-3, relax_coarse=0.6,
               rho_smooth=0.9, coarse_direct=1)
 TRAIN = dict(record_steps=2000, epochs=10, batch=4096, lr=2e-3, seed=0, velocity_seed=1)
+# HBM bytes per k_tet launch on the medium box (V-cycle split-position mode):
+# rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy),
+# profiles/k_tet_learned_ab_medium_box.md
+TRAFFIC = {"medium_box_12x120x12": 13.21e6}
 
 
 def log(*a):
@@ -253,7 +257,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": TRAFFIC.get(cfg["name"]) if head in ("vcycle", "temporal", "learned") else None,
             "kernel": "k_tet (finest-level local step)",
             "avg_us": round(avg_us, 3),
             "timing": f"{reps} back-to-back launches between two HIP events on the solver's stream",

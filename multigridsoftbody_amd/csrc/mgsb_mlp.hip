@@ -101,13 +101,32 @@ __global__ void __launch_bounds__(256) k_relu_bwd(int n, const float* __restrict
     if (i < n && !(Z[i] > 0.f)) dA[i] = 0.f;
 }
 
-// db[j] = sum_rows dZ[r][j], rows in order (deterministic).
-__global__ void __launch_bounds__(256) k_colsum(int rows, int cols, const float* __restrict__ dZ,
-                                                float* __restrict__ db) {
+// db[j] = sum_rows dZ[r][j] in two deterministic stages.  Stage 1: block
+// (cx, ry) sums rows [ry*kColRows, +kColRows) of columns [cx*64, +64): each of
+// the 4 row lanes of a column sums every 4th row in order, the 4 lane sums
+// are added in lane order, and the chunk sum goes to part[ry][j].  Stage 2
+// adds the chunk sums of each column in chunk order.
+constexpr int kColRows = 128;
+__global__ void __launch_bounds__(256) k_colsum_part(int rows, int cols, const float* __restrict__ dZ,
+                                                     float* __restrict__ part) {
+    __shared__ float sh[4][64];
+    const int c = threadIdx.x & 63, lane = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + c;
+    const int r0 = blockIdx.y * kColRows, r1 = min(rows, r0 + kColRows);
+    float a = 0.f;
+    if (j < cols)
+        for (int r = r0 + lane; r < r1; r += 4) a += dZ[(long)r * cols + j];
+    sh[lane][c] = a;
+    __syncthreads();
+    if (lane == 0 && j < cols) part[(long)blockIdx.y * cols + j] = ((sh[0][c] + sh[1][c]) + sh[2][c]) + sh[3][c];
+}
+
+__global__ void __launch_bounds__(256) k_colsum_final(int nchunks, int cols, const float* __restrict__ part,
+                                                      float* __restrict__ db) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= cols) return;
     float a = 0.f;
-    for (int r = 0; r < rows; r++) a += dZ[(long)r * cols + j];
+    for (int q = 0; q < nchunks; q++) a += part[(long)q * cols + j];
     db[j] = a;
 }
 
@@ -236,6 +255,7 @@ struct Mlp {
           *H3 = nullptr, *Y = nullptr, *dY = nullptr, *dA = nullptr, *dB = nullptr;
     int* idx = nullptr;
     double* part = nullptr;
+    float* colpart = nullptr;  // column-sum chunk partials
     std::vector<void*> owned;
 };
 
@@ -362,6 +382,12 @@ int mlp_infer(Mlp* p, const float* X, int n, float* Y, int ldy, hipStream_t st, 
     return MGSB_OK;
 }
 
+static void colsum(Mlp* p, int rows, int cols, const float* dZ, float* db, hipStream_t st) {
+    const int nch = (rows + kColRows - 1) / kColRows;
+    k_colsum_part<<<dim3((cols + 63) / 64, nch), 256, 0, st>>>(rows, cols, dZ, p->colpart);
+    k_colsum_final<<<(cols + 255) / 256, 256, 0, st>>>(nch, cols, p->colpart, db);
+}
+
 static int ensure_cap(Mlp* p, int cap) {
     if (p->cap >= cap) return MGSB_OK;
     // scratch is only ever grown; old buffers stay owned until destroy
@@ -376,7 +402,8 @@ static int ensure_cap(Mlp* p, int cap) {
         (rc = mlp_alloc(p, (void**)&p->dY, sizeof(float) * (size_t)cap * p->out)) ||
         (rc = mlp_alloc(p, (void**)&p->dA, fh)) || (rc = mlp_alloc(p, (void**)&p->dB, fh)) ||
         (rc = mlp_alloc(p, (void**)&p->idx, sizeof(int) * (size_t)cap)) ||
-        (rc = mlp_alloc(p, (void**)&p->part, sizeof(double) * ((size_t)cap * p->out / 256 + 2))))
+        (rc = mlp_alloc(p, (void**)&p->part, sizeof(double) * ((size_t)cap * p->out / 256 + 2))) ||
+        (rc = mlp_alloc(p, (void**)&p->colpart, sizeof(float) * ((size_t)cap / kColRows + 1) * kH)))
         return rc;
     p->cap = cap;
     return MGSB_OK;
@@ -410,23 +437,23 @@ static int fwd_bwd(Mlp* p, int nb, hipStream_t st, double* loss) {
     float* g = p->g;
     // layer 4: dW4 = H3^T dY, db4 = colsum dY, dA3 = dY W4^T
     if ((rc = gemm(kH, out, nb, p->H3, 1, kH, p->dY, out, 1, nullptr, g + p->off[6], nullptr, out, 0, st))) return rc;
-    k_colsum<<<1, 256, 0, st>>>(nb, out, p->dY, g + p->off[7]);
+    colsum(p, nb, out, p->dY, g + p->off[7], st);
     if ((rc = gemm(nb, kH, out, p->dY, out, 1, W4, 1, out, nullptr, p->dA, nullptr, kH, 0, st))) return rc;
     const int nh = nb * kH;
     k_relu_bwd<<<(nh + 255) / 256, 256, 0, st>>>(nh, p->Z3, p->dA);
     // layer 3
     if ((rc = gemm(kH, kH, nb, p->H2, 1, kH, p->dA, kH, 1, nullptr, g + p->off[4], nullptr, kH, 0, st))) return rc;
-    k_colsum<<<1, 256, 0, st>>>(nb, kH, p->dA, g + p->off[5]);
+    colsum(p, nb, kH, p->dA, g + p->off[5], st);
     if ((rc = gemm(nb, kH, kH, p->dA, kH, 1, W3, 1, kH, nullptr, p->dB, nullptr, kH, 0, st))) return rc;
     k_relu_bwd<<<(nh + 255) / 256, 256, 0, st>>>(nh, p->Z2, p->dB);
     // layer 2
     if ((rc = gemm(kH, kH, nb, p->H1, 1, kH, p->dB, kH, 1, nullptr, g + p->off[2], nullptr, kH, 0, st))) return rc;
-    k_colsum<<<1, 256, 0, st>>>(nb, kH, p->dB, g + p->off[3]);
+    colsum(p, nb, kH, p->dB, g + p->off[3], st);
     if ((rc = gemm(nb, kH, kH, p->dB, kH, 1, W2, 1, kH, nullptr, p->dA, nullptr, kH, 0, st))) return rc;
     k_relu_bwd<<<(nh + 255) / 256, 256, 0, st>>>(nh, p->Z1, p->dA);
     // layer 1
     if ((rc = gemm(in, kH, nb, p->Xb, 1, in, p->dA, kH, 1, nullptr, g + p->off[0], nullptr, kH, 0, st))) return rc;
-    k_colsum<<<1, 256, 0, st>>>(nb, kH, p->dA, g + p->off[1]);
+    colsum(p, nb, kH, p->dA, g + p->off[1], st);
     MLPCHK(hipGetLastError());
     if (loss) {
         std::vector<double> part(nbl);

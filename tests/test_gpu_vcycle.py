@@ -108,3 +108,47 @@ def test_vcycle_graph_replay_matches_eager(check):
     n = len(res) // 2
     for a, b in zip(res[:n], res[n:]):
         assert np.array_equal(a[0], b[0]) and a[1:] == b[1:]
+
+
+@pytest.mark.parametrize("warm", [0, 1])
+def test_vcycle_collision_matches_oracle(warm):
+    """The large-box collider set on the small box: no gravity, a sphere
+    (r 1.5, moving -x at 2 m/s through y = 0) and the ground plane y = -12,
+    penalty 1e6; adaptive V-cycles, with and without the temporal start."""
+    lv = W.beam_hierarchy((4, 1))
+    phys = W.physics(gravity=(0, 0, 0))
+    c = W.large_colliders()
+    sp = c["sphere"]
+    fix = W.top_fixflag(lv[0])
+    v0 = W.seeded_velocity(lv[0], fix)
+    kw = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, check_every=2, tol=1e-3, rho_smooth=0.9)
+    s = M.Solver(lv, schedule=M.SCHED_VCYCLE, if_mode=M.IF_STABLE, warm_start=warm, kcol=c["k"], **kw, **phys)
+    s.fix_plane(1, 10.0, 1e-5)
+    s.set_velocity(v0)
+    s.set_sphere(0, sp["center"], sp["radius"], sp["vel"])
+    s.set_plane(c["plane"]["normal"], c["plane"]["d"], c["k"])
+    Ls = [O.Level(m.pos, m.tets, m.tris, stiffness=phys["stiffness"], density=phys["density"]) for m in lv]
+    Ls[0].set_fix(fix)
+    Ls[0].set_velocity(v0)
+    ML = O.MultiLevel(Ls, relax_coarse=0.6, dt=phys["dt"])
+    p = O.params(dt=phys["dt"], stiffness=phys["stiffness"], density=phys["density"], gravity=(0, 0, 0), if_mode=3,
+                 kcol=c["k"], spheres=[(*sp["center"], sp["radius"], 1, *sp["vel"])],
+                 plane=(*c["plane"]["normal"], c["plane"]["d"], c["k"]))
+    vp = O.vparams(pre=5, post=5, coarse_sweeps=2, max_cycles=12, check=2, tol=1e-3, rho_smooth=0.9)
+    u1 = u2 = np.zeros((lv[0].nverts, 3), np.float32)
+    agree = 0
+    for k in range(90):
+        st = s.step(1)
+        if warm:
+            ML.set_warm(None, (fix[:, None] * (2 * u1 - u2)).astype(np.float32))
+        ML.step(p, vp, 1)
+        if warm:
+            u2, u1 = u1, (Ls[0].positions() - Ls[0].get(O.S).reshape(-1, 3)).astype(np.float32)
+        o = ML.stats()
+        agree += (st.halves, st.fallback) == (o["halves"], o["fallback"])
+        assert st.rel <= 1e-3, (k, st.rel)
+    g, x = s.positions(), Ls[0].positions()
+    moved = np.abs(x - lv[0].pos).max()
+    print(f"[vcycle collision warm={warm}] rel={rel_err(g, x):.2e} agree {agree / 90:.2f} max displacement {moved:.2f}")
+    assert moved > 1.0, "the sphere never pushed the beam"
+    assert rel_err(g, x) < REL_TOL and agree >= 81
