@@ -50,6 +50,12 @@ def log(*a):
 def build_workload(name):
     from multigridsoftbody_amd import workloads as W
 
+    if name == "batched":
+        b = W.batched()
+        cfg = dict(W.BATCHED, E=b["E"], ns=b["ns"])
+        return cfg, b["levels"]
+    if name == "blob":
+        return dict(W.BLOB, fix=W.blob_fixflag), W.blob_hierarchy()
     cfg = {"small": W.SMALL, "medium": W.MEDIUM, "large": W.LARGE}[name]
     return cfg, W.beam_hierarchy(cfg["ns"])
 
@@ -61,6 +67,8 @@ def make_solver(levels, schedule, cfg, use_graph=1, fixed_cycles=0, check_every=
     phys = W.physics(gravity=(0.0, -9.8, 0.0) if cfg["gravity"] else (0.0, 0.0, 0.0))
     if schedule in ("vcycle", "temporal", "learned"):
         kw = dict(VCYCLE)
+        if "max_cycles" in cfg:
+            kw.update(max_cycles=cfg["max_cycles"])
         if fixed_cycles:
             kw.update(max_cycles=fixed_cycles, check_every=0)
         else:
@@ -74,9 +82,11 @@ def make_solver(levels, schedule, cfg, use_graph=1, fixed_cycles=0, check_every=
     else:
         s = M.Solver(levels[:1], schedule=M.SCHED_SINGLE, if_mode=M.IF_AC, iters=cfg["iters"], use_graph=use_graph,
                      **phys)
-    s.fix_plane(1, 10.0, 1e-5)
-    fix = W.top_fixflag(levels[0])
+    fix = cfg["fix"](levels[0]) if "fix" in cfg else W.top_fixflag(levels[0])
+    s.set_fixed(fix)
     s.set_velocity(W.seeded_velocity(levels[0], fix, seed=velocity_seed))
+    if "E" in cfg:  # several bodies, each with its own stiffness
+        s.set_body_stiffness(cfg["E"])
     if cfg["name"].startswith("large"):
         c = W.large_colliders()
         s.set_sphere(0, c["sphere"]["center"], c["sphere"]["radius"], c["sphere"]["vel"])
@@ -121,25 +131,43 @@ def cpu_baseline(levels, cfg, schedule, budget_s=15.0):
 
     threads = min(16, os.cpu_count() or 1)
     O.set_threads(threads)
+    if schedule == "vcycle" and levels[0].ntets > 200_000:
+        # the oracle's level bindings follow the reference's brute-force point
+        # location (O(V_fine * T_coarse)); past the medium box the baseline is
+        # the reference's own single-level schedule on the same mesh
+        schedule = "single"
     phys = W.physics(gravity=(0.0, -9.8, 0.0) if cfg["gravity"] else (0.0, 0.0, 0.0))
     lv = levels if schedule == "vcycle" else levels[:1]
+    cfg = dict(cfg, iters=cfg.get("iters", 10))
     Ls = [O.Level(m.pos, m.tets, m.tris, stiffness=phys["stiffness"], density=phys["density"]) for m in lv]
-    fix = W.top_fixflag(levels[0])
+    if "E" in cfg:  # per-body stiffness on every level (60 n^3 tets per body)
+        import numpy as np
+
+        for l, L in enumerate(Ls[:len(lv)]):
+            L.set_tet_stiffness(np.concatenate([np.full(60 * W.BATCH_LEVELS[int(n)][l] ** 3, e, np.float32)
+                                                for n, e in zip(cfg["ns"], cfg["E"])]))
+    fix = cfg["fix"](levels[0]) if "fix" in cfg else W.top_fixflag(levels[0])
     Ls[0].set_fix(fix)
     Ls[0].set_velocity(W.seeded_velocity(levels[0], fix))
+    col = {}
+    if cfg["name"].startswith("large"):
+        c = W.large_colliders()
+        sp = c["sphere"]
+        col = dict(kcol=c["k"], spheres=[(*sp["center"], sp["radius"], 1, *sp["vel"])],
+                   plane=(*c["plane"]["normal"], c["plane"]["d"], c["k"]))
     if schedule == "vcycle":
         t0 = time.perf_counter()
         ML = O.MultiLevel(Ls, relax_coarse=VCYCLE["relax_coarse"], dt=phys["dt"])
         log(f"[bench] oracle multilevel set-up {time.perf_counter() - t0:.1f}s (not timed)")
         p = O.params(dt=phys["dt"], stiffness=phys["stiffness"], density=phys["density"], gravity=phys["gravity"],
-                     if_mode=3)
+                     if_mode=3, **col)
         vp = O.vparams(pre=VCYCLE["pre_sweeps"], post=VCYCLE["post_sweeps"], coarse_sweeps=VCYCLE["coarse_sweeps"],
                        max_cycles=VCYCLE["max_cycles"], tol=VCYCLE["tol"], check=1, rho_smooth=VCYCLE["rho_smooth"])
         step = lambda: ML.step(p, vp, 1)  # noqa: E731
         what = "V-cycle schedule (same parameters, adaptive cycles to rel. residual 1e-3)"
     else:
         p = O.params(dt=phys["dt"], stiffness=phys["stiffness"], density=phys["density"], gravity=phys["gravity"],
-                     iters=cfg["iters"], if_mode=1)
+                     iters=cfg["iters"], if_mode=1, **col)
         step = lambda: Ls[0].step(p, 1)  # noqa: E731
         what = f"single-level reference schedule ({cfg['iters']} PD iterations/step)"
     step()
@@ -158,12 +186,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="medium", choices=["small", "medium", "large"])
+    ap.add_argument("--workload", default="medium", choices=["small", "medium", "large", "batched", "blob"])
     ap.add_argument("--schedule", default="ab", choices=["ab", "vcycle", "temporal", "learned", "single", "twolevel"])
     ap.add_argument("--fixed-cycles", type=int, default=0,
                     help="V-cycles per step without the convergence check (profiling)")
     ap.add_argument("--check-every", type=int, default=2, choices=[1, 2],
                     help="warm-started schedules: residual checks after each cycle (1) or half-cycle (2)")
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="trial steps per candidate of mgsb_autotune before timing (0: off)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     args = ap.parse_args()
@@ -175,6 +205,8 @@ def main():
     log(f"[bench] workload={args.workload} schedule={args.schedule} steps={args.steps} warmup={args.warmup}")
     cfg, levels = build_workload(args.workload)
     names = ["vcycle", "temporal", "learned"] if args.schedule == "ab" else [args.schedule]
+    if args.schedule == "ab" and args.workload in ("large", "batched"):
+        names = ["vcycle", "temporal"]  # the network is trained and measured on the medium box
     head = names[-1]
     weights = None
     losses = None
@@ -193,10 +225,17 @@ def main():
         log(f"[bench] {name}: solver created in {time.perf_counter() - t0:.1f}s; tets per level "
             f"{[m.ntets for m in levels]}")
         vc = name in ("vcycle", "temporal", "learned")
+        tuned = None
+        if args.autotune:
+            s.step(args.warmup)
+            tuned = s.autotune(args.autotune)
+            log(f"[bench] {name}: autotune {tuned} (not timed)")
         elapsed, run = timed_run(s, args.steps, args.warmup, vc)
         sps = args.steps / elapsed
         log(f"[bench] {name}: timed {args.steps} steps in {elapsed:.3f}s -> {sps:.1f} steps/s; run {run}")
         results[name] = dict(steps_per_s=round(sps, 3), ms_per_step=round(1e3 * elapsed / args.steps, 4))
+        if tuned:
+            results[name]["autotuned"] = {k: tuned[k] for k in ("block_tet", "block_vert", "pre_sweeps", "post_sweeps")}
         if run:
             results[name].update(mean_cycles=round(run["total_cycles"] / max(1, run["steps"]), 3),
                                  mean_half_cycles=round(run["total_halves"] / max(1, run["steps"]), 3),

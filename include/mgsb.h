@@ -149,6 +149,11 @@ int mgsb_make_box(int32_t nx, int32_t ny, int32_t nz, const float lo[3], const f
  * h voxel size, 6 tets per voxel; voxels sit on the lattice h*Z^3, so blobs
  * at h, 3h, 9h nest.  No reference counterpart (organ-like workload). */
 int mgsb_make_blob(float a, float b, float c, float p, float amp, uint32_t seed, float h, mgsb_mesh* out);
+/* Level `coarsen` (1, 3, 9, ...) of a nested blob hierarchy on the same
+ * lattice: cells of size coarsen * h, kept iff any of their cells of size h
+ * is inside the shape, so every level covers the finer ones. */
+int mgsb_make_blob_level(float a, float b, float c, float p, float amp, uint32_t seed, float h, int32_t coarsen,
+                         mgsb_mesh* out);
 
 void mgsb_mesh_free(mgsb_mesh* m);
 
@@ -274,6 +279,36 @@ int mgsb_mlp_grad(mgsb_solver* s, const float* x, const float* t, int32_t n, flo
  * vertices of the coarsest level's dense FP64 Cholesky factor (-1: none). */
 int mgsb_level_rho(mgsb_solver* s, int32_t level, double* rho);
 int mgsb_coarse_nfree(mgsb_solver* s, int32_t* nfree);
+
+/* Autotuner (no reference counterpart; the reference hard-codes its launch
+ * shapes).  Times trial_steps steps per candidate from a snapshot of the
+ * solver state, restores the state, and keeps the fastest candidate whose
+ * steps all met the residual target: pre/post sweeps in {3,4,5,6} (V-cycle
+ * schedules), then block_tet and block_vert in {64,128,256}, one coordinate
+ * at a time.  The chosen values are written into the solver's parameters. */
+typedef struct mgsb_tune_result {
+    int32_t block_tet, block_vert, pre_sweeps, post_sweeps;
+    double ms_per_step;          /* the chosen candidate                       */
+    double ms_per_step_default;  /* the parameters before tuning               */
+    int32_t trials;
+    int32_t reserved;
+} mgsb_tune_result;
+int mgsb_autotune(mgsb_solver* s, int32_t trial_steps, mgsb_tune_result* out);
+
+/* Bodies: the connected components of level 0, numbered by their smallest
+ * vertex (several soft bodies in one mesh, e.g. a disjoint union of boxes,
+ * advance in the same launches).  Each body may have its own stiffness (ARAP
+ * weight per unit volume, E); it applies to the body's tets on every level
+ * (coarse vertices follow their coarse-in-fine binding) and rebuilds the
+ * Jacobi diagonals and the V-cycle's coarse operators.  When every body is a
+ * contiguous vertex range, the V-cycle checks each body's own relative
+ * residual ||g_b(x)|| / ||g_b(s)|| and a step converges when the largest is at
+ * most tol (per_body_checks = 1); mgsb_body_residuals gives res0_b and res_b
+ * of the last check.  Batching has no reference counterpart (the reference
+ * simulates one body, Simulator.cpp:76-87). */
+int mgsb_body_count(mgsb_solver* s, int32_t* nbodies, int32_t* per_body_checks);
+int mgsb_set_body_stiffness(mgsb_solver* s, const float* E, int32_t n);
+int mgsb_body_residuals(mgsb_solver* s, double* res0, double* res, int32_t n);
 
 /* Sizes of a level. */
 int mgsb_level_info(mgsb_solver* s, int32_t level, int32_t* nverts, int32_t* ntets, int32_t* ntris);

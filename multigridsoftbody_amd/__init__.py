@@ -92,6 +92,12 @@ class mgsb_run_stats_t(C.Structure):
                 ("failed_steps", C.c_int32), ("fallbacks", C.c_int32), ("total_halves", C.c_int32)]
 
 
+class mgsb_tune_result(C.Structure):
+    _fields_ = [("block_tet", C.c_int32), ("block_vert", C.c_int32), ("pre_sweeps", C.c_int32),
+                ("post_sweeps", C.c_int32), ("ms_per_step", C.c_double), ("ms_per_step_default", C.c_double),
+                ("trials", C.c_int32), ("reserved", C.c_int32)]
+
+
 class mgsb_energy_t(C.Structure):
     _fields_ = [("Ek", C.c_double), ("Ep", C.c_double), ("Ec", C.c_double), ("Nc", C.c_double), ("dX", C.c_double)]
 
@@ -108,6 +114,8 @@ SIGNATURES = {
     "mgsb_make_box": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _fp, _fp, C.POINTER(mgsb_mesh)]),
     "mgsb_make_blob": (C.c_int, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_float,
                                  C.POINTER(mgsb_mesh)]),
+    "mgsb_make_blob_level": (C.c_int, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_float,
+                                       C.c_int32, C.POINTER(mgsb_mesh)]),
     "mgsb_mesh_free": (None, [C.POINTER(mgsb_mesh)]),
     "mgsb_precompute": (C.c_int, [C.POINTER(mgsb_mesh), C.c_float, C.c_float, _fp, _fp, _fp, _fp, _fp]),
     "mgsb_bind_points": (C.c_int, [C.POINTER(mgsb_mesh), _fp, C.c_int32, C.c_int32, _ip, _fp]),
@@ -147,6 +155,10 @@ SIGNATURES = {
     "mgsb_coarse_nfree": (C.c_int, [_vp, _ip]),
     "mgsb_set_timing": (C.c_int, [_vp, C.c_int32]),
     "mgsb_kernel_time": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
+    "mgsb_autotune": (C.c_int, [_vp, C.c_int32, C.POINTER(mgsb_tune_result)]),
+    "mgsb_body_count": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "mgsb_set_body_stiffness": (C.c_int, [_vp, _fp, C.c_int32]),
+    "mgsb_body_residuals": (C.c_int, [_vp, _dp, _dp, C.c_int32]),
     "mgsb_time_kernel": (C.c_int, [_vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "mgsb_last_error": (C.c_char_p, []),
     "mgsb_version": (C.c_char_p, []),
@@ -265,10 +277,11 @@ class Mesh:
         return cls(*cls._take(m))
 
     @classmethod
-    def blob(cls, a=6.0, b=3.0, c=4.0, p=2.5, amp=0.15, seed=0, h=0.25):
-        """Voxelised superquadric blob, 6 tets per voxel."""
+    def blob(cls, a=6.0, b=3.0, c=4.0, p=2.5, amp=0.15, seed=0, h=0.25, coarsen=1):
+        """Voxelised superquadric blob, 6 tets per voxel; coarsen > 1 gives the
+        covering level of cell size coarsen * h of a nested hierarchy."""
         m = mgsb_mesh()
-        _check(lib.mgsb_make_blob(a, b, c, p, amp, seed, h, C.byref(m)), "mgsb_make_blob")
+        _check(lib.mgsb_make_blob_level(a, b, c, p, amp, seed, h, coarsen, C.byref(m)), "mgsb_make_blob_level")
         return cls(*cls._take(m))
 
     def precompute(self, stiffness, density=1.0):
@@ -510,6 +523,31 @@ class Solver:
         n = C.c_int64()
         _check(lib.mgsb_kernel_time(self._h, kernel, C.byref(ms), C.byref(n)), "mgsb_kernel_time")
         return ms.value, n.value
+
+    def autotune(self, trial_steps=20):
+        """Pick sweeps and block sizes by timing trial steps (state restored)."""
+        r = mgsb_tune_result()
+        _check(lib.mgsb_autotune(self._h, trial_steps, C.byref(r)), "mgsb_autotune")
+        for k in ("block_tet", "block_vert", "pre_sweeps", "post_sweeps"):  # keep the host copy in step
+            setattr(self.params, k, getattr(r, k))
+        return {k: getattr(r, k) for k, _ in r._fields_ if k != "reserved"}
+
+    def body_count(self):
+        """(number of bodies, per-body residual checks on)."""
+        n, c = C.c_int32(), C.c_int32()
+        _check(lib.mgsb_body_count(self._h, C.byref(n), C.byref(c)), "mgsb_body_count")
+        return n.value, bool(c.value)
+
+    def set_body_stiffness(self, E):
+        a = _f32(E)
+        _check(lib.mgsb_set_body_stiffness(self._h, _fptr(a), a.size), "mgsb_set_body_stiffness")
+
+    def body_residuals(self):
+        n = self.body_count()[0]
+        r0, r = np.zeros(n), np.zeros(n)
+        _check(lib.mgsb_body_residuals(self._h, r0.ctypes.data_as(_dp), r.ctypes.data_as(_dp), n),
+               "mgsb_body_residuals")
+        return r0, r
 
     def time_kernel(self, reps=200, kernel=0):
         """Average microseconds of `reps` back-to-back launches of kernel 0."""

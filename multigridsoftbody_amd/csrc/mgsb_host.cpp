@@ -259,6 +259,40 @@ int build_level(const mgsb_mesh& m, float stiffness, float density, HostLevel& L
     return MGSB_OK;
 }
 
+void recompute_diag(HostLevel& L, float k) {
+    L.diag.assign(L.V, 0.f);
+    for (int i = 0; i < L.T; i++) {
+        const float kt = L.tk.empty() ? k : L.tk[(size_t)i];
+        const float* ac = &L.AC[(size_t)i * 12];
+        for (int j = 0; j < 4; j++)
+            for (int r = 0; r < 3; r++)
+                L.diag[L.tets[(size_t)i * 4 + j]] += ac[r * 4 + j] * ac[r * 4 + j] * L.vol[i] * kt;
+    }
+}
+
+int mesh_components(int V, int T, const int32_t* tets, std::vector<int32_t>& comp) {
+    std::vector<int32_t> par(V);
+    for (int i = 0; i < V; i++) par[i] = i;
+    auto find = [&](int a) {
+        while (par[a] != a) a = par[a] = par[par[a]];
+        return a;
+    };
+    for (int t = 0; t < T; t++)
+        for (int c = 1; c < 4; c++) {
+            int a = find(tets[(size_t)t * 4]), b = find(tets[(size_t)t * 4 + c]);
+            if (a != b) par[std::max(a, b)] = std::min(a, b);  // root = smallest vertex
+        }
+    comp.assign(V, -1);
+    std::vector<int32_t> id(V, -1);
+    int n = 0;
+    for (int i = 0; i < V; i++) {
+        int r = find(i);
+        if (id[r] < 0) id[r] = n++;
+        comp[i] = id[r];
+    }
+    return n;
+}
+
 // ---------------------------------------------------------------------------
 // Barycentric bindings between levels (PDSolver_MG.cpp:5-58, 133-168).
 // ---------------------------------------------------------------------------
@@ -651,7 +685,16 @@ int mgsb_make_box(int32_t nx, int32_t ny, int32_t nz, const float lo[3], const f
 // multiples of h from the origin, so blobs at h, 3h, 9h are nested lattices.
 // Vertices are deduplicated lattice points; 6 Kuhn tets per voxel.
 int mgsb_make_blob(float a, float b, float c, float p, float amp, uint32_t seed, float h, mgsb_mesh* out) {
-    if (!(a > 0 && b > 0 && c > 0 && p > 0 && h > 0) || !out) {
+    return mgsb_make_blob_level(a, b, c, p, amp, seed, h, 1, out);
+}
+
+// Level of a nested hierarchy: cells of size coarsen * h on the same lattice
+// origin; a cell is kept iff any of its coarsen^3 cells of size h is inside
+// the shape, so each level covers every finer one (its vertices lie in the
+// coarser level's tets, as the V-cycle's prolongation needs).
+int mgsb_make_blob_level(float a, float b, float c, float p, float amp, uint32_t seed, float h, int32_t coarsen,
+                         mgsb_mesh* out) {
+    if (!(a > 0 && b > 0 && c > 0 && p > 0 && h > 0) || !out || coarsen < 1 || coarsen > 64) {
         set_error("mgsb_make_blob: bad arguments");
         return MGSB_ERR_ARG;
     }
@@ -674,7 +717,7 @@ int mgsb_make_blob(float a, float b, float c, float p, float amp, uint32_t seed,
     const double rmax = 1 + std::fabs(amp);
     const int ni = (int)std::ceil(a * rmax / h) + 1, nj = (int)std::ceil(b * rmax / h) + 1,
               nk = (int)std::ceil(c * rmax / h) + 1;
-    auto inside = [&](int i, int j, int k) {
+    auto inside_fine = [&](int i, int j, int k) {
         double x = (i + 0.5) * h, y = (j + 0.5) * h, z = (k + 0.5) * h;
         double rr = std::sqrt(x * x + y * y + z * z);
         double d[3] = {0, 0, 0};
@@ -687,26 +730,37 @@ int mgsb_make_blob(float a, float b, float c, float p, float amp, uint32_t seed,
         double lhs = std::pow(std::fabs(x / a), p) + std::pow(std::fabs(y / b), p) + std::pow(std::fabs(z / c), p);
         return lhs <= std::pow(r, (double)p);
     };
-    const long W = 2L * ni + 1, H = 2L * nj + 1;  // lattice points from -ni..ni
+    // coarse lattice: cell I covers fine cells coarsen*I .. coarsen*I + coarsen - 1
+    const int C = coarsen;
+    const int ci = (ni + C - 1) / C + 1, cj = (nj + C - 1) / C + 1, ck = (nk + C - 1) / C + 1;
+    auto inside = [&](int I, int J, int K) {
+        for (int dj = 0; dj < C; dj++)
+            for (int dk = 0; dk < C; dk++)
+                for (int di = 0; di < C; di++)
+                    if (inside_fine(I * C + di, J * C + dj, K * C + dk)) return true;
+        return false;
+    };
+    const double hc = (double)h * C;
+    const long W = 2L * ci + 1, H = 2L * cj + 1;  // lattice points from -ci..ci
     std::unordered_map<long, int32_t> vmap;
     std::vector<float> pos;
     std::vector<int32_t> tets;
     auto vert = [&](int i, int j, int k) {
-        long key = ((long)(k + nk) * H + (j + nj)) * W + (i + ni);
+        long key = ((long)(k + ck) * H + (j + cj)) * W + (i + ci);
         auto it = vmap.find(key);
         if (it != vmap.end()) return it->second;
         int32_t id = (int32_t)(pos.size() / 3);
         vmap.emplace(key, id);
-        pos.push_back((float)(i * (double)h));
-        pos.push_back((float)(j * (double)h));
-        pos.push_back((float)(k * (double)h));
+        pos.push_back((float)(i * hc));
+        pos.push_back((float)(j * hc));
+        pos.push_back((float)(k * hc));
         return id;
     };
     static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
     static const int odd[6] = {0, 1, 1, 0, 0, 1};
-    for (int j = -nj; j < nj; j++)
-        for (int k = -nk; k < nk; k++)
-            for (int i = -ni; i < ni; i++) {
+    for (int j = -cj; j < cj; j++)
+        for (int k = -ck; k < ck; k++)
+            for (int i = -ci; i < ci; i++) {
                 if (!inside(i, j, k)) continue;
                 for (int pq = 0; pq < 6; pq++) {
                     int cc[3] = {0, 0, 0};

@@ -35,8 +35,19 @@ struct HostLevel {
     // vertex -> incident (surface tri, corner), tris ascending.
     std::vector<int32_t> trp;     // V+1
     std::vector<int32_t> tslot;   // 3S
+    std::vector<float> tk;        // per-tet ARAP weight per unit volume (empty: the scalar stiffness)
     int bad_tet = -1;
 };
+
+inline double tet_k(const HostLevel& L, int t, double k) { return L.tk.empty() ? k : (double)L.tk[(size_t)t]; }
+
+// Jacobi diagonal sum_t k_t vol_t AC^2 recomputed in the precompute's tet
+// order (after per-tet stiffness changes).
+void recompute_diag(HostLevel& L, float k);
+
+// Connected components of a tet mesh (union of tet corners), numbered in
+// order of their smallest vertex index; returns the component count.
+int mesh_components(int V, int T, const int32_t* tets, std::vector<int32_t>& comp);
 
 // Builds a level: copies the mesh, precomputes, builds gather structures.
 // stiffness = ARAP weight per unit volume, density scales the lumped mass.
@@ -66,8 +77,12 @@ void build_restriction(int Vf, int Vc, const std::vector<int32_t>& iids, const s
 void derive_vfix(const std::vector<float>& vfix_fine, const std::vector<int32_t>& aids, const std::vector<float>& aw,
                  int Vc, std::vector<float>& vfix_coarse);
 double power_rho(const HostLevel& L, const std::vector<float>& vfix, double dt, double relax, double k);
+// Block-diagonal FP64 Cholesky of the coarsest level, one dense block per
+// connected component: dense[i] = row of vertex i (-1 pinned), rows grouped
+// by component; W holds W_b = C_b^-1 (row-major n_b x n_b) back to back and
+// rowinfo[4 r .. 4 r + 3] = (first row of r's block, n_b, offset of W_b, 0).
 int coarse_cholesky(const HostLevel& L, const std::vector<float>& vfix, double dt, double k,
-                    std::vector<int32_t>& dense, std::vector<double>& W, int& n);
+                    std::vector<int32_t>& dense, std::vector<double>& W, std::vector<int32_t>& rowinfo, int& n);
 void build_level_matrix(const HostLevel& L, double k, double dt, std::vector<int32_t>& ap,
                         std::vector<int32_t>& aj, std::vector<float>& av);
 

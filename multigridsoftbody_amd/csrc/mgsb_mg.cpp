@@ -5,6 +5,7 @@
 // coarsest level.  All of it runs once per create (and on dt changes).
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <vector>
 
 #include "mgsb_internal.h"
@@ -53,7 +54,7 @@ static void apply_stiffness(const HostLevel& L, double k, const std::vector<doub
         double d[3] = {w[id[1]] - w[id[0]], w[id[2]] - w[id[0]], w[id[3]] - w[id[0]]};
         double f[3];
         for (int c = 0; c < 3; c++) f[c] = d[0] * B[c] + d[1] * B[3 + c] + d[2] * B[6 + c];
-        double kv = k * L.vol[t];
+        double kv = tet_k(L, t, k) * L.vol[t];
         for (int a = 0; a < 4; a++) out[id[a]] += kv * (f[0] * AC[a] + f[1] * AC[4 + a] + f[2] * AC[8 + a]);
     }
 }
@@ -85,54 +86,85 @@ double power_rho(const HostLevel& L, const std::vector<float>& vfix, double dt, 
     return lam;
 }
 
-// Dense A = m/h^2 + L on the free vertices, FP64 Cholesky A = C C^T, and
-// W = C^-1 (lower triangular), so that A^-1 r = W^T (W r).
+// Dense A = m/h^2 + L on the free vertices of each connected component,
+// FP64 Cholesky A_b = C_b C_b^T, and W_b = C_b^-1 (lower triangular), so that
+// A_b^-1 r = W_b^T (W_b r).  Components do not couple, so the blocks are the
+// whole factor.
 int coarse_cholesky(const HostLevel& L, const std::vector<float>& vfix, double dt, double k,
-                    std::vector<int32_t>& dense, std::vector<double>& W, int& n) {
+                    std::vector<int32_t>& dense, std::vector<double>& W, std::vector<int32_t>& rowinfo, int& n) {
     const int V = L.V;
+    std::vector<int32_t> comp;
+    const int nc = mesh_components(V, L.T, L.tets.data(), comp);
+    // rows grouped by component, vertices ascending within a component
+    std::vector<int32_t> cstart(nc + 1, 0);
+    for (int i = 0; i < V; i++)
+        if (vfix[i] != 0.f) cstart[comp[i] + 1]++;
+    for (int c = 0; c < nc; c++) cstart[c + 1] += cstart[c];
+    n = cstart[nc];
     dense.assign(V, -1);
-    n = 0;
-    for (int i = 0; i < V; i++)
-        if (vfix[i] != 0.f) dense[i] = n++;
-    if (n == 0) {
-        W.clear();
-        return 0;
+    {
+        std::vector<int32_t> fill(cstart.begin(), cstart.end() - 1);
+        for (int i = 0; i < V; i++)
+            if (vfix[i] != 0.f) dense[i] = fill[comp[i]]++;
     }
-    std::vector<double> A((size_t)n * n, 0.0);
-    for (int i = 0; i < V; i++)
-        if (dense[i] >= 0) A[(size_t)dense[i] * n + dense[i]] += (double)L.mass[i] / (dt * dt);
-    for (int t = 0; t < L.T; t++) {
-        const float* AC = &L.AC[(size_t)t * 12];
-        double w = (double)L.vol[t] * k;
-        for (int a = 0; a < 4; a++)
-            for (int b = 0; b < 4; b++) {
-                int ia = dense[L.tets[(size_t)t * 4 + a]], ib = dense[L.tets[(size_t)t * 4 + b]];
-                if (ia < 0 || ib < 0) continue;
-                double s = 0;
-                for (int r = 0; r < 3; r++) s += (double)AC[r * 4 + a] * (double)AC[r * 4 + b];
-                A[(size_t)ia * n + ib] += w * s;
+    W.clear();
+    rowinfo.assign((size_t)n * 4, 0);
+    if (n == 0) return 0;
+    std::vector<size_t> woff(nc + 1, 0);
+    for (int c = 0; c < nc; c++) {
+        const size_t nb = (size_t)(cstart[c + 1] - cstart[c]);
+        woff[c + 1] = woff[c] + nb * nb;
+    }
+    if (woff[nc] > (size_t)INT32_MAX) return -1;
+    for (int c = 0; c < nc; c++)  // O(n_b^3) host factorisation: bounded block size
+        if (cstart[c + 1] - cstart[c] > 2048) return -2;
+    W.assign(woff[nc], 0.0);
+    std::vector<std::vector<int32_t>> ctets(nc);
+    for (int t = 0; t < L.T; t++) ctets[comp[L.tets[(size_t)t * 4]]].push_back(t);
+    for (int c = 0; c < nc; c++) {
+        const int r0 = cstart[c], nb = cstart[c + 1] - cstart[c];
+        for (int r = r0; r < r0 + nb; r++) {
+            rowinfo[(size_t)r * 4 + 0] = r0;
+            rowinfo[(size_t)r * 4 + 1] = nb;
+            rowinfo[(size_t)r * 4 + 2] = (int32_t)woff[c];
+        }
+        if (nb == 0) continue;
+        std::vector<double> A((size_t)nb * nb, 0.0);
+        for (int i = 0; i < V; i++)
+            if (comp[i] == c && dense[i] >= 0)
+                A[(size_t)(dense[i] - r0) * nb + (dense[i] - r0)] += (double)L.mass[i] / (dt * dt);
+        for (int t : ctets[c]) {
+            const float* AC = &L.AC[(size_t)t * 12];
+            double w = (double)L.vol[t] * tet_k(L, t, k);
+            for (int a = 0; a < 4; a++)
+                for (int b = 0; b < 4; b++) {
+                    int ia = dense[L.tets[(size_t)t * 4 + a]], ib = dense[L.tets[(size_t)t * 4 + b]];
+                    if (ia < 0 || ib < 0) continue;
+                    double s = 0;
+                    for (int r = 0; r < 3; r++) s += (double)AC[r * 4 + a] * (double)AC[r * 4 + b];
+                    A[(size_t)(ia - r0) * nb + (ib - r0)] += w * s;
+                }
+        }
+        for (int j = 0; j < nb; j++) {
+            double d = A[(size_t)j * nb + j];
+            for (int p = 0; p < j; p++) d -= A[(size_t)j * nb + p] * A[(size_t)j * nb + p];
+            if (!(d > 0)) return -1;
+            d = std::sqrt(d);
+            A[(size_t)j * nb + j] = d;
+            for (int i = j + 1; i < nb; i++) {
+                double s = A[(size_t)i * nb + j];
+                for (int p = 0; p < j; p++) s -= A[(size_t)i * nb + p] * A[(size_t)j * nb + p];
+                A[(size_t)i * nb + j] = s / d;
             }
-    }
-    for (int j = 0; j < n; j++) {
-        double d = A[(size_t)j * n + j];
-        for (int p = 0; p < j; p++) d -= A[(size_t)j * n + p] * A[(size_t)j * n + p];
-        if (!(d > 0)) return -1;
-        d = std::sqrt(d);
-        A[(size_t)j * n + j] = d;
-        for (int i = j + 1; i < n; i++) {
-            double s = A[(size_t)i * n + j];
-            for (int p = 0; p < j; p++) s -= A[(size_t)i * n + p] * A[(size_t)j * n + p];
-            A[(size_t)i * n + j] = s / d;
         }
-    }
-    // W = C^-1 by forward substitution on the identity, column by column
-    W.assign((size_t)n * n, 0.0);
-    for (int c = 0; c < n; c++) {
-        for (int i = c; i < n; i++) {
-            double s = (i == c) ? 1.0 : 0.0;
-            for (int p = c; p < i; p++) s -= A[(size_t)i * n + p] * W[(size_t)p * n + c];
-            W[(size_t)i * n + c] = s / A[(size_t)i * n + i];
-        }
+        // W_b = C_b^-1 by forward substitution on the identity, column by column
+        double* Wb = W.data() + woff[c];
+        for (int col = 0; col < nb; col++)
+            for (int i = col; i < nb; i++) {
+                double s = (i == col) ? 1.0 : 0.0;
+                for (int p = col; p < i; p++) s -= A[(size_t)i * nb + p] * Wb[(size_t)p * nb + col];
+                Wb[(size_t)i * nb + col] = s / A[(size_t)i * nb + i];
+            }
     }
     return 0;
 }
@@ -150,7 +182,7 @@ void build_level_matrix(const HostLevel& L, double k, double dt, std::vector<int
     std::vector<std::vector<std::pair<int, double>>> rows(V);
     for (int t = 0; t < L.T; t++) {
         const float* AC = &L.AC[(size_t)t * 12];
-        const double w = (double)L.vol[t] * k;
+        const double w = (double)L.vol[t] * tet_k(L, t, k);
         for (int a = 0; a < 4; a++) {
             const int va = L.tets[(size_t)t * 4 + a];
             for (int b = 0; b < 4; b++) {

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -216,7 +217,8 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
                                              const int4* __restrict__ slot, float3* __restrict__ cf, float k,
                                              int ntri, const int4* __restrict__ tri, const int4* __restrict__ tslot,
                                              float4* __restrict__ tcf, const DevColliders* __restrict__ col,
-                                             const int* __restrict__ skip, const float4* __restrict__ sref) {
+                                             const int* __restrict__ skip, const float4* __restrict__ sref,
+                                             const float* __restrict__ tk) {
     if (skip && *skip) return;
     const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (g >= T) {
@@ -294,6 +296,7 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
         AC[r * 4 + 3] = b[6 + r];
     }
     const float vt = vol[g];
+    const float kk = tk ? tk[g] : k;  // per-body stiffness when several bodies have their own E
     const int4 sl = slot[g];
     const int sls[4] = {sl.x, sl.y, sl.z, sl.w};
 #pragma unroll
@@ -307,7 +310,7 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
                 acc = acc > 10.f ? 10.f : acc;
                 acc = acc < -10.f ? -10.f : acc;
             }
-            float r = acc * vt * k;
+            float r = acc * vt * kk;
             // |r| < 1e-5 against the double literal is exactly |r| <= 1e-5f
             if (MODE == kTetModeAc && fabsf(r) <= 1e-5f) r = 0.f;
             f[i] = r;
@@ -362,7 +365,9 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
             if (col->vel[q].w == 0.f) continue;
             const float4 sp = col->sph[q];
             const float3 center = make_float3(sp.x, sp.y, sp.z);
-            const float3 cp = sub3(P, center);
+            // split: offsets from the centre as (s - c) + u, so the in-step
+            // correction keeps its low bits against the 1e6-scale penalty
+            const float3 cp = split ? add3(sub3(f3(st), center), f3(p)) : sub3(P, center);
             const float3 dir = div3(cp, len3(cp));
             const float distance = len3(cp);
             const float oldR = sp.w;
@@ -377,7 +382,7 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
                 }
                 t = t / 2.0f;
                 const float3 colP = add3(P, mul3(dir, t));
-                const float3 ccp = sub3(colP, center);
+                const float3 ccp = split ? add3(cp, mul3(dir, t)) : sub3(colP, center);
                 const float3 n = div3(ccp, len3(ccp));
                 const float nd = dot3(n, dir);
                 cdx = cdx + kc * n.x * n.x;
@@ -386,14 +391,14 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
                 cfx = cfx + kc * t * nd * n.x;
                 cfy = cfy + kc * t * nd * n.y;
                 cfz = cfz + kc * t * nd * n.z;
-                const float ee = dot3(sub3(P, colP), n);
+                const float ee = split ? -t * nd : dot3(sub3(P, colP), n);
                 energy = energy + kc * (ee * ee);
             }
         }
         if (col->plane_on) {
             const float4 pl = col->plane;
             const float3 n = make_float3(pl.x, pl.y, pl.z);
-            const float sd = dot3(n, P) - pl.w;
+            const float sd = split ? (dot3(n, f3(st)) - pl.w) + dot3(n, f3(p)) : dot3(n, P) - pl.w;
             if (sd < 0.f) {
                 const float dp = -sd;
                 const float kp = col->kplane;
@@ -559,7 +564,8 @@ __global__ void __launch_bounds__(256) k_energy_vert(int V, const float4* __rest
 // Per-tet part: k vol / 2 |F - R|^2.
 __global__ void __launch_bounds__(256) k_energy_tet(int T, const float4* __restrict__ x,
                                                     const int4* __restrict__ tet, const float* __restrict__ B,
-                                                    const float* __restrict__ vol, float k, double* __restrict__ part) {
+                                                    const float* __restrict__ vol, float k, const float* __restrict__ tk,
+                                                    double* __restrict__ part) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
     if (g < T) {
@@ -575,7 +581,7 @@ __global__ void __launch_bounds__(256) k_energy_tet(int T, const float4* __restr
         polar_rf(F, RF);
         double e = 0.0;
         for (int q = 0; q < 9; q++) e += (double)RF[q] * (double)RF[q];
-        v[0] = e * (double)vol[g] * (double)k / 2.0;
+        v[0] = e * (double)vol[g] * (double)(tk ? tk[g] : k) / 2.0;
     }
     block_reduce_store<1>(v, part);
 }
@@ -601,6 +607,8 @@ struct VState {
     int accepted;   // warm-start stage x holds: 0 s, 1 temporal, 2 temporal + learned
     int revert;     // pending revert of x: 0 none, 1 to s, 2 to the temporal extrapolation
     int cf_ok;      // corner forces are those of the current x (or the step has converged)
+    double rel;     // convergence measure at the last check: res / res0, or with several
+                    // bodies the largest per-body res_b / res0_b
 };
 
 // Check modes of the fused residual reduction.
@@ -612,12 +620,20 @@ constexpr int kChkMid = 3;    // after pre-smoothing
 constexpr int kChkTemporal = 4;  // after the temporal extrapolation
 
 // Convergence bookkeeping from the squared residual norm t.  hm (host-mapped,
-// optional) mirrors (converged, cycles, halves) for the host's cycle loop.
-__device__ void vstate_apply(VState* st, double t, int mode, int* hm) {
+// optional) mirrors (converged, cycles, halves, fallback, accepted) for the
+// host's cycle loop.  rel_in >= 0: the convergence measure of several bodies
+// (largest per-body relative residual), tested as rel_in <= tol; otherwise
+// the test is res <= tol * res0 on the whole-mesh norms.
+__device__ void vstate_apply(VState* st, double t, int mode, int* hm, double rel_in = -1.0) {
     const double r = sqrt(t);
+    auto ok = [&](double res) {
+        return rel_in >= 0.0 ? rel_in <= (double)st->tol : res <= (double)st->tol * st->res0;
+    };
+    auto rel_of = [&](double res) { return rel_in >= 0.0 ? rel_in : (st->res0 > 0.0 ? res / st->res0 : 0.0); };
     if (mode == kChkStart) {
         st->res0 = r;
         st->res = r;
+        st->rel = r > 0.0 ? 1.0 : 0.0;
         st->cycles = 0;
         st->halves = 0;
         st->fallback = 0;
@@ -639,20 +655,22 @@ __device__ void vstate_apply(VState* st, double t, int mode, int* hm) {
             st->cf_ok = 0;
         } else {
             st->res = r;
+            st->rel = rel_of(r);
             st->accepted = mode == kChkTemporal ? 1 : 2;
             st->revert = 0;
             st->cf_ok = 1;
-            if (st->check && r <= (double)st->tol * st->res0) st->converged = 1;
+            if (st->check && ok(r)) st->converged = 1;
         }
     } else {
         st->res = r;
+        st->rel = rel_of(r);
         if (mode == kChkCycle) {
             st->cycles += 1;
             st->halves += st->check == 2 ? 1 : 2;
         } else {
             st->halves += 1;
         }
-        if (st->check && r <= (double)st->tol * st->res0) st->converged = 1;
+        if (st->check && ok(r)) st->converged = 1;
     }
     if (hm) {
         hm[0] = st->converged;
@@ -665,7 +683,7 @@ __device__ void vstate_apply(VState* st, double t, int mode, int* hm) {
 
 // End of a step: fold the step's outcome into the run aggregates.
 __device__ void vstate_step_done(VState* st) {
-    const double rel = st->res0 > 0.0 ? st->res / st->res0 : 0.0;
+    const double rel = st->rel;
     if (rel > st->max_rel) st->max_rel = rel;
     st->total_cycles += st->cycles;
     st->total_halves += st->halves;
@@ -710,7 +728,8 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
                                                const float3* __restrict__ cf, const DevColliders* __restrict__ col,
                                                int flags, float4* __restrict__ r, double* __restrict__ part,
                                                float4* __restrict__ xprev, const int* __restrict__ skip, int split,
-                                               VState* st, unsigned* count, int mode, int* hm) {
+                                               VState* st, unsigned* count, int mode, int* hm,
+                                               double* __restrict__ g2, float* __restrict__ cmask) {
     if (skip && *skip) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
@@ -722,6 +741,7 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
         const float4 pa = split ? make_float4(st.x + p.x, st.y + p.y, st.z + p.z, 0.f) : p;
         const float sx = split ? -p.x : st.x - p.x, sy = split ? -p.y : st.y - p.y, sz = split ? -p.z : st.z - p.z;
         float cfx = 0.f, cfy = 0.f, cfz = 0.f;
+        bool contact = false;  // a collider acts on this vertex (active set)
         if (flags & kVertPoint) {
             const float kc = col->kcol;
             const float3 P = f3(pa);
@@ -729,17 +749,18 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
                 if (col->vel[q].w == 0.f) continue;
                 const float4 sp = col->sph[q];
                 const float3 center = make_float3(sp.x, sp.y, sp.z);
-                const float3 cp = sub3(P, center);
+                const float3 cp = split ? add3(sub3(f3(st), center), f3(p)) : sub3(P, center);
                 const float3 dir = div3(cp, len3(cp));
                 const float distance = len3(cp);
                 const float rr = sp.w * 1.5f;
                 if (distance < rr) {
+                    contact = true;
                     const float bq = 2 * dot3(cp, dir);
                     const float cq = dot3(cp, cp) - rr * rr;
                     float t = (-bq + sqrtf(bq * bq - 4.f * cq)) / 2.f;
                     t = t / 2.0f;
                     const float3 colP = add3(P, mul3(dir, t));
-                    const float3 ccp = sub3(colP, center);
+                    const float3 ccp = split ? add3(cp, mul3(dir, t)) : sub3(colP, center);
                     const float3 n = div3(ccp, len3(ccp));
                     const float nd = dot3(n, dir);
                     cfx = cfx + kc * t * nd * n.x;
@@ -749,8 +770,11 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
             }
             if (col->plane_on) {
                 const float4 pl = col->plane;
-                const float sd = pl.x * pa.x + pl.y * pa.y + pl.z * pa.z - pl.w;
+                const float sd = split ? ((pl.x * st.x + pl.y * st.y + pl.z * st.z) - pl.w) +
+                                             (pl.x * p.x + pl.y * p.y + pl.z * p.z)
+                                       : pl.x * pa.x + pl.y * pa.y + pl.z * pa.z - pl.w;
                 if (sd < 0.f) {
+                    contact = true;
                     cfx = cfx + col->kplane * (-sd) * pl.x;
                     cfy = cfy + col->kplane * (-sd) * pl.y;
                     cfz = cfz + col->kplane * (-sd) * pl.z;
@@ -762,14 +786,66 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
         const float gx = (c.x * sx + (fx + cfx)) * fix;
         const float gy = (c.x * sy + (fy + cfy)) * fix;
         const float gz = (c.x * sz + (fz + cfz)) * fix;
-        if (r) r[i] = make_float4(gx, gy, gz, 0.f);
+        // the active set is left out of the restriction: the coarse operators
+        // carry no penalty stiffness (cmask also masks the coarse corrections)
+        const float am = contact ? 0.f : 1.f;
+        if (cmask) cmask[i] = am;
+        if (r) r[i] = cmask ? make_float4(gx * am, gy * am, gz * am, 0.f) : make_float4(gx, gy, gz, 0.f);
         if (xprev) xprev[i] = p;
         v[0] = (double)gx * gx + (double)gy * gy + (double)gz * gz;
+        if (g2) g2[i] = v[0];
     }
     if (part) {
         block_reduce_store<1>(v, part);
         if (mode != kChkNone) last_block_check(part, count, st, mode, hm);
     }
+}
+
+// Per-body residual norms of several bodies (each a contiguous vertex range
+// vb[b] .. vb[b+1] of level 0): block b sums g2 over its body in a fixed order;
+// the block that arrives last (device-scope counter) adds the bodies in body
+// order, forms the largest per-body relative residual (res0b holds the
+// per-body norms of g(s), written at the step's start check) and applies the
+// check through vstate_apply.
+__global__ void __launch_bounds__(256) k_body_check(const int* __restrict__ vb, const double* __restrict__ g2,
+                                                    double* __restrict__ bodysq, double* __restrict__ res0b,
+                                                    unsigned* count, VState* st, int mode, int* hm,
+                                                    const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ double sh[256];
+    __shared__ int s_last;
+    const int b = blockIdx.x;
+    double a = 0.0;
+    for (int i = vb[b] + threadIdx.x; i < vb[b + 1]; i += blockDim.x) a += g2[i];
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        bodysq[b] = sh[0];
+        __threadfence();
+        s_last = atomicAdd(count, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last || threadIdx.x != 0) return;
+    __threadfence();
+    double tot = 0.0, worst = 0.0;
+    for (int q = 0; q < (int)gridDim.x; q++) {
+        const double sq = __hip_atomic_load(bodysq + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tot += sq;
+        const double r = sqrt(sq);
+        if (mode == kChkStart) {
+            res0b[q] = r;
+        } else {
+            const double r0 = res0b[q];
+            const double rel = r0 > 0.0 ? r / r0 : (r > 0.0 ? 1e300 : 0.0);
+            worst = rel > worst ? rel : worst;
+        }
+    }
+    *count = 0u;
+    vstate_apply(st, tot, mode, hm, worst);
 }
 
 // ---------------------------------------------------------------------------
@@ -923,7 +999,7 @@ __global__ void __launch_bounds__(256) k_restrict(int Vc, const int* __restrict_
 __global__ void __launch_bounds__(256) k_prolong_add(int N, float4* __restrict__ dst, float4* __restrict__ dst2,
                                                      const float4* __restrict__ src, const int4* __restrict__ ids,
                                                      const float4* __restrict__ w, const float4* __restrict__ maskv,
-                                                     const int* __restrict__ skip) {
+                                                     const int* __restrict__ skip, const float* __restrict__ amask) {
     if (skip && *skip) return;
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -933,7 +1009,7 @@ __global__ void __launch_bounds__(256) k_prolong_add(int N, float4* __restrict__
     const float vx = ww.x * a.x + ww.y * b.x + ww.z * c.x + ww.w * d.x;
     const float vy = ww.x * a.y + ww.y * b.y + ww.z * c.y + ww.w * d.y;
     const float vz = ww.x * a.z + ww.y * b.z + ww.z * c.z + ww.w * d.z;
-    const float m = maskv[i].z;
+    const float m = amask ? maskv[i].z * amask[i] : maskv[i].z;
     float4 o = dst[i];
     o.x = o.x + m * vx;
     o.y = o.y + m * vy;
@@ -1033,7 +1109,8 @@ __global__ void __launch_bounds__(256) k_zero4(int n, float4* __restrict__ a, co
     if (i < n) a[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-constexpr int kMaxDense = 1024;
+constexpr int kMaxDense = 1024;          // dense rows kept in LDS
+constexpr size_t kMaxDenseEntries = 1u << 25;  // 256 MiB of block factors at most
 
 __device__ __forceinline__ void wave_sum3(double& a0, double& a1, double& a2) {
 #pragma unroll
@@ -1044,75 +1121,95 @@ __device__ __forceinline__ void wave_sum3(double& a0, double& a1, double& a2) {
     }
 }
 
-// Coarsest solve e = W^T (W r) with W = C^-1, A = C C^T (FP64 Cholesky), on
-// LDS vectors: rb[c][0..m) holds r on entry and e on exit, yb is scratch.
-// One wave per row with the lanes across the row (W and Wt = W^T both
-// row-major), so all loads of a row are in flight at once; each row is summed
-// by a fixed shuffle tree.
-__device__ __forceinline__ void dense_solve_block(int m, const double* __restrict__ W, const double* __restrict__ Wt,
-                                                  double (*rb)[kMaxDense], double (*yb)[kMaxDense]) {
+// Coarsest solve e = W^T (W r), block by block (one block per connected
+// component; W_b = C_b^-1 with A_b = C_b C_b^T, FP64 Cholesky).  r is in
+// rb[c * ld + row] on entry and e on exit, yb is scratch (LDS, or device
+// memory when the rows do not fit); rowinfo[row] = (first row of the block,
+// block size, offset of W_b).  One wave per row with the lanes across the
+// row (W_b and Wt_b = W_b^T both row-major), so all loads of a row are in
+// flight at once; each row is summed by a fixed shuffle tree.
+__device__ __forceinline__ void dense_solve_block(int m, const int4* __restrict__ rowinfo,
+                                                  const double* __restrict__ W, const double* __restrict__ Wt,
+                                                  double* rb, double* yb, int ld) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     for (int row = wave; row < m; row += nw) {
-        const double* wr = W + (size_t)row * m;
+        const int4 ri = rowinfo[row];
+        const int i = row - ri.x;
+        const double* wr = W + ri.z + (size_t)i * ri.y;
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        for (int k = lane; k <= row; k += 64) {
+        for (int k = lane; k <= i; k += 64) {
             const double w = wr[k];
-            a0 += w * rb[0][k];
-            a1 += w * rb[1][k];
-            a2 += w * rb[2][k];
+            a0 += w * rb[ri.x + k];
+            a1 += w * rb[ld + ri.x + k];
+            a2 += w * rb[2 * ld + ri.x + k];
         }
         wave_sum3(a0, a1, a2);
         if (lane == 0) {
-            yb[0][row] = a0;
-            yb[1][row] = a1;
-            yb[2][row] = a2;
+            yb[row] = a0;
+            yb[ld + row] = a1;
+            yb[2 * ld + row] = a2;
         }
     }
     __syncthreads();
     for (int row = wave; row < m; row += nw) {
-        const double* wr = Wt + (size_t)row * m;
+        const int4 ri = rowinfo[row];
+        const int i = row - ri.x;
+        const double* wr = Wt + ri.z + (size_t)i * ri.y;
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-        for (int k = row + lane; k < m; k += 64) {
+        for (int k = i + lane; k < ri.y; k += 64) {
             const double w = wr[k];
-            a0 += w * yb[0][k];
-            a1 += w * yb[1][k];
-            a2 += w * yb[2][k];
+            a0 += w * yb[ri.x + k];
+            a1 += w * yb[ld + ri.x + k];
+            a2 += w * yb[2 * ld + ri.x + k];
         }
         wave_sum3(a0, a1, a2);
         if (lane == 0) {
-            rb[0][row] = a0;
-            rb[1][row] = a1;
-            rb[2][row] = a2;
+            rb[row] = a0;
+            rb[ld + row] = a1;
+            rb[2 * ld + row] = a2;
         }
     }
     __syncthreads();
 }
 
-// Coarsest level on its own (when it is not part of the one-workgroup tail):
-// dense[i] maps vertices to rows of the n x n factor (-1: pinned).
-__global__ void __launch_bounds__(256) k_coarse_solve(int Vc, int n, const int* __restrict__ dense,
-                                                      const double* __restrict__ W, const double* __restrict__ Wt,
-                                                      const float4* __restrict__ r, float4* __restrict__ e,
-                                                      const int* __restrict__ skip) {
-    if (skip && *skip) return;
-    __shared__ double rb[3][kMaxDense];
-    __shared__ double yb[3][kMaxDense];
-    for (int i = threadIdx.x; i < Vc; i += blockDim.x) {
+// The coarsest level's direct solve in one workgroup: gather r into dense
+// rows, dense_solve_block, scatter e (pinned vertices get 0).  big: device
+// scratch of 6 n doubles used when n > kMaxDense.
+__device__ __forceinline__ void coarsest_direct(int V, int n, const int* __restrict__ dense,
+                                                const int4* __restrict__ rowinfo, const double* __restrict__ W,
+                                                const double* __restrict__ Wt, const float4* __restrict__ r,
+                                                float4* __restrict__ e, double* big) {
+    __shared__ double lds[6 * kMaxDense];
+    double* rb = n <= kMaxDense ? lds : big;
+    const int ld = n <= kMaxDense ? kMaxDense : n;
+    double* yb = rb + 3 * ld;
+    for (int i = threadIdx.x; i < V; i += blockDim.x) {
         const int d = dense[i];
         if (d >= 0) {
             const float4 q = r[i];
-            rb[0][d] = q.x;
-            rb[1][d] = q.y;
-            rb[2][d] = q.z;
+            rb[d] = q.x;
+            rb[ld + d] = q.y;
+            rb[2 * ld + d] = q.z;
         }
     }
     __syncthreads();
-    dense_solve_block(n, W, Wt, rb, yb);
-    for (int i = threadIdx.x; i < Vc; i += blockDim.x) {
+    dense_solve_block(n, rowinfo, W, Wt, rb, yb, ld);
+    for (int i = threadIdx.x; i < V; i += blockDim.x) {
         const int d = dense[i];
-        e[i] = d >= 0 ? make_float4((float)rb[0][d], (float)rb[1][d], (float)rb[2][d], 0.f)
+        e[i] = d >= 0 ? make_float4((float)rb[d], (float)rb[ld + d], (float)rb[2 * ld + d], 0.f)
                       : make_float4(0.f, 0.f, 0.f, 0.f);
     }
+    __syncthreads();
+}
+
+// Coarsest level on its own (when it is not part of the one-workgroup tail).
+__global__ void __launch_bounds__(256) k_coarse_solve(int Vc, int n, const int* __restrict__ dense,
+                                                      const int4* __restrict__ rowinfo, const double* __restrict__ W,
+                                                      const double* __restrict__ Wt, const float4* __restrict__ r,
+                                                      float4* __restrict__ e, double* big,
+                                                      const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    coarsest_direct(Vc, n, dense, rowinfo, W, Wt, r, e, big);
 }
 
 // Small coarse levels of the linear V-cycle in ONE workgroup: level 0 of the
@@ -1132,7 +1229,9 @@ struct TailLevel {
     const int4* iids;       // prolongation from the next level into this one
     const float4* iw;
     const int* dense;
+    const int4* rowinfo;
     const double *W, *Wt;
+    double* big;  // dense-vector scratch when nfree > kMaxDense
     float rho;
     int pad;
 };
@@ -1140,8 +1239,6 @@ struct TailLevel {
 __global__ void __launch_bounds__(1024) k_coarse_tail(const TailLevel* __restrict__ tl, int n, int sweeps,
                                                       float relax, int direct, const int* __restrict__ skip) {
     if (skip && *skip) return;
-    __shared__ double rb[3][kMaxDense];
-    __shared__ double yb[3][kMaxDense];
     const int tid = threadIdx.x, nt = blockDim.x;
     unsigned flips = 0u;  // bit q: level q's correction is in e1 (else e0)
     auto flip = [&](int q) { return (flips >> q) & 1u; };
@@ -1200,22 +1297,7 @@ __global__ void __launch_bounds__(1024) k_coarse_tail(const TailLevel* __restric
     {
         const TailLevel& L = tl[n - 1];
         if (direct && L.nfree >= 0) {
-            for (int i = tid; i < L.V; i += nt) {
-                const int d = L.dense[i];
-                if (d >= 0) {
-                    const float4 v = L.r[i];
-                    rb[0][d] = v.x;
-                    rb[1][d] = v.y;
-                    rb[2][d] = v.z;
-                }
-            }
-            __syncthreads();
-            dense_solve_block(L.nfree, L.W, L.Wt, rb, yb);
-            for (int i = tid; i < L.V; i += nt) {
-                const int d = L.dense[i];
-                L.e0[i] = d >= 0 ? make_float4((float)rb[0][d], (float)rb[1][d], (float)rb[2][d], 0.f)
-                                 : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
+            coarsest_direct(L.V, L.nfree, L.dense, L.rowinfo, L.W, L.Wt, L.r, L.e0, L.big);
             flips &= ~(1u << (n - 1));
             __syncthreads();
         } else {
@@ -1294,9 +1376,13 @@ struct DevLevel {
     float* av = nullptr;
     std::vector<float> adiag;          // its diagonal
     float4 *U1 = nullptr, *U2 = nullptr;  // corrections u = x - s of the last two steps
+    float* cmask = nullptr;               // 0 where a collider acts (active set) at the last residual
     int cur = 0;                       // which of e / ep holds the correction
     int* dense = nullptr;
-    double *W = nullptr, *Wt = nullptr;  // coarsest factor W = C^-1 and its transpose
+    double *W = nullptr, *Wt = nullptr;  // coarsest block factors W_b = C_b^-1 and transposes
+    int4* rowinfo = nullptr;             // per dense row: (block start, block size, W offset, 0)
+    double* big = nullptr;               // dense-vector scratch (6 n) when n > kMaxDense
+    float* tk = nullptr;                 // per-tet stiffness (nullptr: p.stiffness)
     int nfree = -1;
     std::vector<float> vfix;
     double rho = 0.0;
@@ -1349,6 +1435,16 @@ struct mgsb_solver {
     // correction the judge rejected, steps start without the MLP stage for
     // gate_skip steps (doubling up to kGateMax after each further rejection)
     int gate_rejects = 0, gate_skip = 0, gate_left = 0;
+    // bodies: connected components of level 0 (body_of[l][v] on every level,
+    // through the coarse-in-fine bindings); when they are contiguous vertex
+    // ranges the V-cycle checks each body's own relative residual
+    int nbodies = 1;
+    bool body_checks = false;
+    std::vector<std::vector<int32_t>> body_of;
+    std::vector<int32_t> h_vb;
+    int* vb = nullptr;
+    double *g2 = nullptr, *bodysq = nullptr, *res0b = nullptr;
+    unsigned* bcount = nullptr;
     TailLevel* tail = nullptr; // descriptors of the one-workgroup coarse tail
     int tail_from = -1;        // first level handled by the tail kernel (-1: none)
     // learned coarse correction (level 1)
@@ -1377,6 +1473,8 @@ struct mgsb_solver {
             for (void* q : lv.owned) (void)hipFree(q);
             if (lv.W) (void)hipFree(lv.W);
             if (lv.Wt) (void)hipFree(lv.Wt);
+            if (lv.rowinfo) (void)hipFree(lv.rowinfo);
+            if (lv.big) (void)hipFree(lv.big);
         }
         for (void* q : owned) (void)hipFree(q);
         if (mlp) mlp_destroy(mlp);
@@ -1518,19 +1616,19 @@ int launch_tet(mgsb_solver* s, DevLevel& d, int mode, bool with_tri, const float
     switch (mode) {
         case kTetModePolar:
             k_tet<kTetModePolar><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness,
-                                                           ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, sref);
+                                                           ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, sref, d.tk);
             break;
         case kTetModeAc:
             k_tet<kTetModeAc><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness, ntri,
-                                                        d.tri, d.tslot, d.tcf, s->dcol, skip, sref);
+                                                        d.tri, d.tslot, d.tcf, s->dcol, skip, sref, d.tk);
             break;
         case kTetModeStable:
             k_tet<kTetModeStable><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness,
-                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, sref);
+                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, sref, d.tk);
             break;
         default:
             k_tet<kTetModeLinear><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness,
-                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, sref);
+                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, sref, d.tk);
     }
     HIPCHK(hipGetLastError());
     return timed_end(s, kid);
@@ -1658,7 +1756,8 @@ int vcycle_lin(mgsb_solver* s, int l, const int* skip) {
     d.cur = 0;
     if (l == nl - 1) {
         if (s->p.coarse_direct && d.nfree >= 0) {
-            k_coarse_solve<<<1, 256, 0, s->stream>>>(d.V, d.nfree, d.dense, d.W, d.Wt, d.r, d.e, skip);
+            k_coarse_solve<<<1, 256, 0, s->stream>>>(d.V, d.nfree, d.dense, d.rowinfo, d.W, d.Wt, d.r, d.e, d.big,
+                                                     skip);
             HIPCHK(hipGetLastError());
             return MGSB_OK;
         }
@@ -1672,7 +1771,8 @@ int vcycle_lin(mgsb_solver* s, int l, const int* skip) {
     HIPCHK(hipGetLastError());
     if ((rc = vcycle_lin(s, l + 1, skip))) return rc;
     LevelMap& m = s->M[l];
-    k_prolong_add<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, cur_e(d), cur_ep(d), cur_e(c), m.iids, m.iw, d.vcl, skip);
+    k_prolong_add<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, cur_e(d), cur_ep(d), cur_e(c), m.iids, m.iw, d.vcl, skip,
+                                                       nullptr);
     HIPCHK(hipGetLastError());
     return lin_smooth(s, l, s->p.coarse_sweeps, false, skip);
 }
@@ -1687,9 +1787,19 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool res
     if (need_tet && (rc = launch_tet(s, d, tet_mode_of(s->p.if_mode), false, first ? d.s : d.x, skip, !first)))
         return rc;
     const int nb = nblk(d.V, 256);
+    if (s->body_checks && chk != kChkNone) {  // per-body norms, then the check
+        k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint,
+                                           write_r ? d.r : nullptr, nullptr, restart ? d.xprev : nullptr, skip, 1,
+                                           nullptr, nullptr, kChkNone, nullptr, s->g2, d.cmask);
+        HIPCHK(hipGetLastError());
+        k_body_check<<<s->nbodies, 256, 0, s->stream>>>(s->vb, s->g2, s->bodysq, s->res0b, s->bcount, s->vstate, chk,
+                                                        s->hmirror, skip);
+        HIPCHK(hipGetLastError());
+        return MGSB_OK;
+    }
     k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, write_r ? d.r : nullptr,
                                        chk != kChkNone ? s->vpart : nullptr, restart ? d.xprev : nullptr, skip, 1,
-                                       s->vstate, s->vcount, chk, s->hmirror);
+                                       s->vstate, s->vcount, chk, s->hmirror, nullptr, d.cmask);
     HIPCHK(hipGetLastError());
     return MGSB_OK;
 }
@@ -1738,7 +1848,7 @@ int enqueue_vc_start(mgsb_solver* s, bool learned) {
         if ((rc = mlp_infer(s->mlp, s->feat, c1.V, s->pred, 3, s->stream, skip))) return rc;
         k_apply_pred<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, s->pred, s->norm, c1.vcl, s->epred);
         k_prolong_add<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, s->epred, s->M[0].iids, s->M[0].iw,
-                                                           f.vc, skip);
+                                                           f.vc, skip, f.cmask);
         HIPCHK(hipGetLastError());
         if ((rc = fine_residual(s, true, false, kChkWarm, true, skip))) return rc;
         k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate);
@@ -1780,7 +1890,7 @@ int enqueue_vc_post(mgsb_solver* s) {
     HIPCHK(hipGetLastError());
     if ((rc = vcycle_lin(s, 1, skip))) return rc;
     k_prolong_add<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, cur_e(c1), s->M[0].iids, s->M[0].iw, f.vc,
-                                                       skip);
+                                                       skip, f.cmask);
     HIPCHK(hipGetLastError());
     float omega = 1.0f;
     for (int k = 0; k < s->p.post_sweeps; k++) {
@@ -1900,9 +2010,13 @@ int enqueue_step(mgsb_solver* s) {
 
 // Coarse Dirichlet masks, vcl, rho and the coarsest factor from the current
 // level-0 fixflags (V-cycle set-up; rerun when fixflags or dt change).
+int invalidate_graph(mgsb_solver* s);
+
 int setup_vcycle(mgsb_solver* s) {
     const int nl = (int)s->L.size();
     if (nl < 2) return MGSB_OK;
+    // captured pieces hold the device pointers rebuilt below
+    if (int rc = invalidate_graph(s)) return rc;
     s->L[0].vfix = s->L[0].h.fix;
     const float dt = s->p.dt;
     for (int l = 1; l < nl; l++) {
@@ -1931,29 +2045,33 @@ int setup_vcycle(mgsb_solver* s) {
         HIPCHK(hipStreamSynchronize(s->stream));
     }
     DevLevel& cc = s->L[nl - 1];
-    std::vector<int32_t> dense;
+    std::vector<int32_t> dense, rowinfo;
     std::vector<double> W;
     int n = 0;
     cc.nfree = -1;
-    if (cc.W) {
-        HIPCHK(hipFree(cc.W));
-        cc.W = nullptr;
+    for (double** q : {&cc.W, &cc.Wt, &cc.big}) {
+        if (*q) HIPCHK(hipFree(*q));
+        *q = nullptr;
     }
-    if (cc.Wt) {
-        HIPCHK(hipFree(cc.Wt));
-        cc.Wt = nullptr;
-    }
-    int nfree_est = 0;
-    for (float v : cc.vfix) nfree_est += v != 0.f;
-    if (nfree_est <= kMaxDense && coarse_cholesky(cc.h, cc.vfix, dt, s->p.stiffness, dense, W, n) == 0 && n > 0) {
+    if (cc.rowinfo) HIPCHK(hipFree(cc.rowinfo));
+    cc.rowinfo = nullptr;
+    if (coarse_cholesky(cc.h, cc.vfix, dt, s->p.stiffness, dense, W, rowinfo, n) == 0 && n > 0 &&
+        W.size() <= kMaxDenseEntries) {
         int rc = dalloc(&cc.W, W.size());
         if (rc) return rc;
         if ((rc = dalloc(&cc.Wt, W.size()))) return rc;
+        if ((rc = dalloc(&cc.rowinfo, (size_t)n))) return rc;
+        if (n > kMaxDense && (rc = dalloc(&cc.big, (size_t)6 * n))) return rc;
         std::vector<double> Wt(W.size());
-        for (int i = 0; i < n; i++)
-            for (int j = 0; j < n; j++) Wt[(size_t)i * n + j] = W[(size_t)j * n + i];
+        for (int r = 0; r < n; r++) {  // transpose block by block
+            const int nb = rowinfo[(size_t)r * 4 + 1], off = rowinfo[(size_t)r * 4 + 2];
+            const int i = r - rowinfo[(size_t)r * 4 + 0];
+            for (int j = 0; j < nb; j++) Wt[(size_t)off + (size_t)i * nb + j] = W[(size_t)off + (size_t)j * nb + i];
+        }
         HIPCHK(hipMemcpyAsync(cc.W, W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipMemcpyAsync(cc.Wt, Wt.data(), sizeof(double) * Wt.size(), hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(cc.rowinfo, rowinfo.data(), sizeof(int32_t) * rowinfo.size(), hipMemcpyHostToDevice,
+                              s->stream));
         HIPCHK(hipMemcpyAsync(cc.dense, dense.data(), sizeof(int) * dense.size(), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
         cc.nfree = n;
@@ -1984,6 +2102,8 @@ int setup_vcycle(mgsb_solver* s) {
             t.dense = d.dense;
             t.W = d.W;
             t.Wt = d.Wt;
+            t.rowinfo = d.rowinfo;
+            t.big = d.big;
             t.rho = (float)d.rho;
             tl.push_back(t);
         }
@@ -2008,6 +2128,8 @@ int alloc_vcycle(mgsb_solver* s) {
             std::vector<float4> z(d.V, make_float4(0.f, 0.f, 0.f, 0.f));
             if ((rc = up(s, d.owned, &d.U1, z.data(), d.V))) return rc;
             if ((rc = up(s, d.owned, &d.U2, z.data(), d.V))) return rc;
+            std::vector<float> one(d.V, 1.f);
+            if ((rc = up(s, d.owned, &d.cmask, one.data(), d.V))) return rc;
             continue;
         }
         if ((rc = up<float4>(s, d.owned, &d.e, nullptr, d.V))) return rc;
@@ -2078,7 +2200,7 @@ int residual_norm(mgsb_solver* s, double* out) {
     const int nb = nblk(d.V, 256);
     if ((rc = ensure_partials(s, (size_t)nb * 4))) return rc;
     k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, nullptr, s->dpart,
-                                       nullptr, nullptr, 0, nullptr, nullptr, kChkNone, nullptr);
+                                       nullptr, nullptr, 0, nullptr, nullptr, kChkNone, nullptr, nullptr, nullptr);
     HIPCHK(hipGetLastError());
     double sq = 0;
     if ((rc = reduce_finish(s, nb, 1, &sq))) return rc;
@@ -2222,10 +2344,46 @@ int mgsb_create(const mgsb_mesh* levels, int32_t nlevels, const mgsb_params* p, 
             }
             int uc = bind_points_average(fine, coarse.X.data(), coarse.V, m.h_aids, m.h_aw);
             if (uc) {
-                set_error("mgsb_create: " + std::to_string(uc) + " vertices of level " + std::to_string(l + 1) +
-                          " lie in no tet of level " + std::to_string(l));
-                return MGSB_ERR_MESH;
+                // a covering coarse level (e.g. nested voxel levels) has vertices outside
+                // the finer mesh: they follow their nearest finer vertex (weight 1);
+                // the reference leaves such points at weight 0 (PDSolver_MG.cpp:133-168)
+                for (int j = 0; j < coarse.V; j++) {
+                    const float* w = &m.h_aw[(size_t)j * 4];
+                    if (w[0] != 0.f || w[1] != 0.f || w[2] != 0.f || w[3] != 0.f) continue;
+                    const float* P = &coarse.X[(size_t)j * 3];
+                    int best = 0;
+                    double bd = 1e300;
+                    for (int i = 0; i < fine.V; i++) {
+                        const double dx = fine.X[(size_t)i * 3] - P[0], dy = fine.X[(size_t)i * 3 + 1] - P[1],
+                                     dz = fine.X[(size_t)i * 3 + 2] - P[2];
+                        const double d = dx * dx + dy * dy + dz * dz;
+                        if (d < bd) {
+                            bd = d;
+                            best = i;
+                        }
+                    }
+                    for (int q = 0; q < 4; q++) m.h_aids[(size_t)j * 4 + q] = best;
+                    m.h_aw[(size_t)j * 4] = 1.f;
+                }
             }
+        }
+        {  // bodies
+            std::vector<int32_t> comp;
+            const HostLevel& h0 = s->L[0].h;
+            s->nbodies = mesh_components(h0.V, h0.T, h0.tets.data(), comp);
+            s->body_of.assign(nlevels, {});
+            s->body_of[0] = comp;
+            for (int l = 1; l < nlevels; l++) {
+                const auto& aids = s->M[l - 1].h_aids;
+                auto& bo = s->body_of[l];
+                bo.resize(s->L[l].h.V);
+                for (int j = 0; j < s->L[l].h.V; j++) bo[j] = s->body_of[l - 1][aids[(size_t)j * 4]];
+            }
+            bool contiguous = true;
+            for (int i = 1; i < h0.V; i++) contiguous = contiguous && comp[i] >= comp[i - 1];
+            s->body_checks = s->nbodies > 1 && contiguous && nlevels >= 2;
+            s->h_vb.assign(s->nbodies + 1, h0.V);
+            for (int i = h0.V - 1; i >= 0; i--) s->h_vb[comp[i]] = i;
         }
         for (auto& d : s->L)
             if ((rc = upload_level(s.get(), d))) return rc;
@@ -2255,6 +2413,15 @@ int mgsb_create(const mgsb_mesh* levels, int32_t nlevels, const mgsb_params* p, 
         if (nlevels >= 2) {
             if ((rc = alloc_vcycle(s.get()))) return rc;
             if ((rc = setup_vcycle(s.get()))) return rc;
+        }
+        if (s->body_checks) {
+            const int nb = s->nbodies;
+            if ((rc = up(s.get(), s->owned, &s->vb, s->h_vb.data(), s->h_vb.size()))) return rc;
+            if ((rc = up<double>(s.get(), s->owned, &s->g2, nullptr, (size_t)s->L[0].V))) return rc;
+            if ((rc = up<double>(s.get(), s->owned, &s->bodysq, nullptr, (size_t)nb))) return rc;
+            if ((rc = up<double>(s.get(), s->owned, &s->res0b, nullptr, (size_t)nb))) return rc;
+            std::vector<unsigned> z(1, 0u);
+            if ((rc = up(s.get(), s->owned, &s->bcount, z.data(), 1))) return rc;
         }
         HIPCHK(hipStreamSynchronize(s->stream));
         *out = s.release();
@@ -2447,7 +2614,7 @@ int mgsb_step(mgsb_solver* s, int32_t nsteps, mgsb_step_stats* stats) {
         HIPCHK(hipStreamSynchronize(s->stream));
         s->last.res0 = v.res0;
         s->last.res = v.res;
-        s->last.rel = v.res0 > 0 ? v.res / v.res0 : 0.0;
+        s->last.rel = v.rel;
         s->last.cycles = v.cycles;
         s->last.halves = v.halves;
         s->last.fallback = v.fallback;
@@ -2559,7 +2726,7 @@ int mgsb_energy(mgsb_solver* s, mgsb_energy_t* e) {
     HIPCHK(hipGetLastError());
     double v4[4];
     if ((rc = reduce_finish(s, nbv, 4, v4))) return rc;
-    k_energy_tet<<<nbt, 256, 0, s->stream>>>(d.T, d.x, d.tet, d.B, d.vol, s->p.stiffness, s->dpart);
+    k_energy_tet<<<nbt, 256, 0, s->stream>>>(d.T, d.x, d.tet, d.B, d.vol, s->p.stiffness, d.tk, s->dpart);
     HIPCHK(hipGetLastError());
     double ep = 0;
     if ((rc = reduce_finish(s, nbt, 1, &ep))) return rc;
@@ -2870,6 +3037,205 @@ int mgsb_kernel_time(mgsb_solver* s, int32_t kernel, double* ms, int64_t* launch
     s->ev_used = 0;
     if (ms) *ms = s->time_ms;
     if (launches) *launches = s->time_launches;
+    return MGSB_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Device state a step reads and writes (positions, velocities, the temporal
+// history, the learned features' previous delta, colliders, convergence
+// state), copied out so that trial steps can be undone.
+struct StateSnap {
+    std::vector<std::pair<float4*, size_t>> where;
+    std::vector<std::vector<float4>> bufs;
+    VState v{};
+    DevColliders col{};
+    int halves_guess = 0, gate_rejects = 0, gate_skip = 0, gate_left = 0;
+};
+
+int snap_take(mgsb_solver* s, StateSnap& S) {
+    HIPCHK(hipStreamSynchronize(s->stream));
+    S.where.clear();
+    for (auto& d : s->L)
+        for (float4* q : {d.x, d.xprev, d.s, d.xlast, d.v}) S.where.push_back({q, (size_t)d.V});
+    DevLevel& f = s->L[0];
+    if (f.U1) {
+        S.where.push_back({f.U1, (size_t)f.V});
+        S.where.push_back({f.U2, (size_t)f.V});
+    }
+    if (s->eprev) S.where.push_back({s->eprev, (size_t)s->L[1].V});
+    S.bufs.resize(S.where.size());
+    for (size_t i = 0; i < S.where.size(); i++) {
+        S.bufs[i].resize(S.where[i].second);
+        HIPCHK(hipMemcpy(S.bufs[i].data(), S.where[i].first, sizeof(float4) * S.where[i].second,
+                         hipMemcpyDeviceToHost));
+    }
+    if (s->vstate) HIPCHK(hipMemcpy(&S.v, s->vstate, sizeof(VState), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&S.col, s->dcol, sizeof(DevColliders), hipMemcpyDeviceToHost));
+    S.halves_guess = s->halves_guess;
+    S.gate_rejects = s->gate_rejects;
+    S.gate_skip = s->gate_skip;
+    S.gate_left = s->gate_left;
+    return MGSB_OK;
+}
+
+int snap_restore(mgsb_solver* s, const StateSnap& S) {
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < S.where.size(); i++)
+        HIPCHK(hipMemcpy(S.where[i].first, S.bufs[i].data(), sizeof(float4) * S.where[i].second,
+                         hipMemcpyHostToDevice));
+    if (s->vstate) HIPCHK(hipMemcpy(s->vstate, &S.v, sizeof(VState), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->dcol, &S.col, sizeof(DevColliders), hipMemcpyHostToDevice));
+    s->hcol = S.col;
+    s->halves_guess = S.halves_guess;
+    s->gate_rejects = S.gate_rejects;
+    s->gate_skip = S.gate_skip;
+    s->gate_left = S.gate_left;
+    return MGSB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mgsb_autotune(mgsb_solver* s, int32_t trial_steps, mgsb_tune_result* out) {
+    if (!s || trial_steps < 1 || !out) {
+        set_error("mgsb_autotune: bad arguments (trial_steps >= 1)");
+        return MGSB_ERR_ARG;
+    }
+    if (s->recording || s->timing) {
+        set_error("mgsb_autotune: stop recording / kernel timing first");
+        return MGSB_ERR_STATE;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    StateSnap S;
+    int rc = snap_take(s, S);
+    if (rc) return rc;
+    const mgsb_params p0 = s->p;
+    const bool vcycle = s->p.schedule == MGSB_SCHED_VCYCLE || s->p.schedule == MGSB_SCHED_LEARNED;
+    int trials = 0;
+    // ms per step of the current parameters from the saved state; ok = no step over tolerance
+    auto measure = [&](double& ms, bool& ok) -> int {
+        int r;
+        if ((r = snap_restore(s, S)) || (r = invalidate_graph(s))) return r;
+        if ((r = mgsb_step(s, 2, nullptr))) return r;  // graph capture, warm caches
+        HIPCHK(hipStreamSynchronize(s->stream));
+        VState a{}, b{};
+        if (s->vstate) HIPCHK(hipMemcpy(&a, s->vstate, sizeof(VState), hipMemcpyDeviceToHost));
+        const auto t0 = std::chrono::steady_clock::now();
+        if ((r = mgsb_step(s, trial_steps, nullptr))) return r;
+        HIPCHK(hipStreamSynchronize(s->stream));
+        const auto t1 = std::chrono::steady_clock::now();
+        if (s->vstate) HIPCHK(hipMemcpy(&b, s->vstate, sizeof(VState), hipMemcpyDeviceToHost));
+        ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / trial_steps;
+        ok = b.nfail == a.nfail;
+        trials++;
+        return MGSB_OK;
+    };
+    double base = 0.0, best = 0.0, ms = 0.0;
+    bool ok = false;
+    if ((rc = measure(base, ok))) return rc;
+    best = ok ? base : 1e300;
+    mgsb_params bestp = s->p;
+    auto consider = [&]() -> int {
+        int r = measure(ms, ok);
+        if (r) return r;
+        if (ok && ms < best) {
+            best = ms;
+            bestp = s->p;
+        }
+        s->p = bestp;
+        return MGSB_OK;
+    };
+    if (vcycle)
+        for (int sw : {3, 4, 5, 6}) {
+            s->p.pre_sweeps = s->p.post_sweeps = sw;
+            if ((rc = consider())) return rc;
+        }
+    for (int b : {64, 128, 256}) {
+        s->p.block_tet = b;
+        if ((rc = consider())) return rc;
+    }
+    for (int b : {64, 128, 256}) {
+        s->p.block_vert = b;
+        if ((rc = consider())) return rc;
+    }
+    s->p = best < 1e300 ? bestp : p0;
+    if ((rc = snap_restore(s, S)) || (rc = invalidate_graph(s))) return rc;
+    out->block_tet = block_tet(s);
+    out->block_vert = block_vert(s);
+    out->pre_sweeps = s->p.pre_sweeps;
+    out->post_sweeps = s->p.post_sweeps;
+    out->ms_per_step = best < 1e300 ? best : base;
+    out->ms_per_step_default = base;
+    out->trials = trials;
+    out->reserved = 0;
+    return MGSB_OK;
+}
+
+int mgsb_body_count(mgsb_solver* s, int32_t* nbodies, int32_t* per_body_checks) {
+    if (!s || !nbodies) {
+        set_error("mgsb_body_count: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    *nbodies = s->nbodies;
+    if (per_body_checks) *per_body_checks = s->body_checks ? 1 : 0;
+    return MGSB_OK;
+}
+
+int mgsb_set_body_stiffness(mgsb_solver* s, const float* E, int32_t n) {
+    if (!s || !E || n != s->nbodies) {
+        set_error("mgsb_set_body_stiffness: need one stiffness per body (mgsb_body_count)");
+        return MGSB_ERR_ARG;
+    }
+    for (int b = 0; b < n; b++)
+        if (!(E[b] > 0.f)) {
+            set_error("mgsb_set_body_stiffness: stiffness must be > 0");
+            return MGSB_ERR_ARG;
+        }
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    int rc;
+    for (size_t l = 0; l < s->L.size(); l++) {
+        DevLevel& d = s->L[l];
+        HostLevel& h = d.h;
+        h.tk.resize(h.T);
+        for (int t = 0; t < h.T; t++) h.tk[t] = E[s->body_of[l][h.tets[(size_t)t * 4]]];
+        recompute_diag(h, s->p.stiffness);
+        if (!d.tk && (rc = up<float>(s, d.owned, &d.tk, nullptr, (size_t)h.T))) return rc;
+        HIPCHK(hipMemcpyAsync(d.tk, h.tk.data(), sizeof(float) * h.T, hipMemcpyHostToDevice, s->stream));
+        if ((rc = upload_vc(s, d))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (s->L.size() >= 2 && (rc = setup_vcycle(s))) return rc;
+    return invalidate_graph(s);
+}
+
+int mgsb_body_residuals(mgsb_solver* s, double* res0, double* res, int32_t n) {
+    if (!s || n != s->nbodies || !res0 || !res) {
+        set_error("mgsb_body_residuals: need arrays of mgsb_body_count entries");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    if (!s->body_checks) {
+        if (n != 1 || !s->vstate) {
+            set_error("mgsb_body_residuals: per-body residuals need contiguous bodies and a V-cycle schedule");
+            return MGSB_ERR_STATE;
+        }
+        VState v;
+        HIPCHK(hipMemcpyAsync(&v, s->vstate, sizeof(VState), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        res0[0] = v.res0;
+        res[0] = v.res;
+        return MGSB_OK;
+    }
+    std::vector<double> sq(n);
+    HIPCHK(hipMemcpyAsync(res0, s->res0b, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(sq.data(), s->bodysq, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (int b = 0; b < n; b++) res[b] = std::sqrt(sq[b]);
     return MGSB_OK;
 }
 

@@ -67,6 +67,7 @@ _SIGS = {
     "level_set_fix": (None, [_vp, _fp]),
     "level_set_velocity": (None, [_vp, _fp]),
     "level_set_positions": (None, [_vp, _fp]),
+    "level_set_tet_k": (None, [_vp, _fp]),
     "level_get": (C.c_int, [_vp, C.c_int, _dp]),
     "step_single": (None, [_vp, C.POINTER(orc_params), C.c_int]),
     "mg_create": (_vp, [_vp, _vp]),
@@ -176,6 +177,12 @@ class Level:
     def set_positions(self, x):
         a = np.ascontiguousarray(x, np.float32).reshape(-1)
         self._f("level_set_positions")(self._h, a.ctypes.data_as(_fp))
+
+    def set_tet_stiffness(self, k):
+        """Per-tet ARAP weight per unit volume (bodies with their own E)."""
+        a = np.ascontiguousarray(k, np.float32).reshape(-1)
+        assert a.size == self.nt
+        self._f("level_set_tet_k")(self._h, a.ctypes.data_as(_fp))
 
     def get(self, which):
         sizes = {X: 3 * self.nv, V: 3 * self.nv, S: 3 * self.nv, XPREV: 3 * self.nv, XLAST: 3 * self.nv,

@@ -248,8 +248,12 @@ typedef struct orc_level {
     real *trin;      /* tri normals                                           */
     real *vnormal;   /* vertex normals                                        */
     float stiffness, density;
+    real* tk;        /* per-tet stiffness (bodies of their own E) or NULL     */
     int bad_tet;     /* index of first degenerate tet or -1                   */
 } orc_level;
+
+/* ARAP weight per unit volume of tet t. */
+static inline real tet_k(const orc_level* L, int t) { return L->tk ? L->tk[t] : (real)L->stiffness; }
 
 static void* xcalloc(size_t n, size_t s) { return calloc(n ? n : 1, s); }
 
@@ -367,9 +371,23 @@ void PFX(level_free)(orc_level* L) {
     void* ps[] = {L->tets, L->tris, L->x0, L->x, L->v, L->fext, L->xlast, L->s, L->xprev, L->B, L->AC, L->vol,
                   L->mass, L->diag, L->fix, L->center, L->faceN, L->faceA, L->force, L->cdiag, L->cforce,
                   L->cenergy, L->cdepth, L->cflag, L->cf, L->rp, L->ri, L->trp, L->tri, L->tc, L->trin,
-                  L->vnormal};
+                  L->vnormal, L->tk};
     for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
     free(L);
+}
+
+/* Per-tet stiffness (several bodies, each with its own E); recomputes the
+ * Jacobi diagonal in the precompute's tet order. */
+void PFX(level_set_tet_k)(orc_level* L, const float* k) {
+    if (!L->tk) L->tk = (real*)xcalloc((size_t)L->T, sizeof(real));
+    for (int t = 0; t < L->T; t++) L->tk[t] = (real)k[t];
+    for (int i = 0; i < L->V; i++) L->diag[i] = 0;
+    for (int t = 0; t < L->T; t++) {
+        const real* ac = &L->AC[t * 12];
+        for (int j = 0; j < 4; j++)
+            for (int r = 0; r < 3; r++)
+                L->diag[L->tets[t * 4 + j]] += ac[r * 4 + j] * ac[r * 4 + j] * L->vol[t] * L->tk[t];
+    }
 }
 
 int PFX(level_bad_tet)(const orc_level* L) { return L->bad_tet; }
@@ -469,9 +487,9 @@ static void calc_v(orc_level* L, const orc_params* p) {
  * getTetFRV2 + calTetIFV1 (PDSolverAc.cu:447-662, mode 1: float constants and
  * contributions below 1e-5 in magnitude dropped). */
 static void tet_forces_on(orc_level* L, const real* Xsrc, int mode) {
-    real k = (real)L->stiffness;
 #pragma omp parallel for schedule(static)
     for (int t = 0; t < L->T; t++) {
+        const real k = tet_k(L, t);
         const int* id = &L->tets[t * 4];
         const real* X = Xsrc;
         real D[9];
@@ -1022,7 +1040,7 @@ void PFX(energy)(const orc_level* L, const orc_params* p, double* out) {
         polar(F, R, 0);
         double e = 0;
         for (int i = 0; i < 9; i++) e += ((double)F[i] - R[i]) * ((double)F[i] - R[i]);
-        Ep += e * (double)L->vol[t] * (double)L->stiffness / 2;
+        Ep += e * (double)L->vol[t] * (double)tet_k(L, t) / 2;
     }
     for (int i = 0; i < L->V; i++) {
         Ec += L->cenergy[i];
@@ -1138,6 +1156,8 @@ typedef struct orc_ml {
     int cycles, halves;
     float relax_coarse, dt;
     real* warm;          /* pending warm start: level-1 correction (V1 x 3) */
+    real* cmask;         /* level 0: 0 where a collider acts (active set), else 1 */
+    real* fmask;         /* level 0: fix * cmask, the coarse corrections' mask      */
     real* warm0;         /* ... and a fine-level correction (V0 x 3)        */
     int has_warm0;
     int has_warm, fallback;
@@ -1200,12 +1220,12 @@ static int ml_factor_coarsest(orc_ml* M) {
     for (int i = 0; i < V; i++) M->dense[i] = M->vfix[l][i] != 0 ? n++ : -1;
     M->nfree = n;
     double* A = (double*)xcalloc((size_t)n * n, sizeof(double));
-    double dt = M->dt, k = L->stiffness;
+    double dt = M->dt;
     for (int i = 0; i < V; i++)
         if (M->dense[i] >= 0) A[(size_t)M->dense[i] * n + M->dense[i]] += (double)L->mass[i] / (dt * dt);
     for (int t = 0; t < L->T; t++) {
         const real* AC = &L->AC[t * 12];
-        double w = (double)L->vol[t] * k;
+        double w = (double)L->vol[t] * (double)tet_k(L, t);
         for (int a = 0; a < 4; a++)
             for (int b = 0; b < 4; b++) {
                 int ia = M->dense[L->tets[t * 4 + a]], ib = M->dense[L->tets[t * 4 + b]];
@@ -1360,25 +1380,40 @@ static void ml_vcycle_lin(orc_ml* M, int l, const orc_vparams* vp) {
 
 /* g(x) on level 0 into r[0] (free DOFs), returns ||g|| (FP64); also restarts
  * the fine Chebyshev history (x_prev = x). */
+/* Nonlinear residual on level 0 (stored in r[0]) and its norm; also restarts
+ * x_prev.  Vertices a collider acts on form the active set: their residual
+ * is left out of the restriction and the coarse corrections skip them
+ * (cmask = 0), since the coarse operators carry no penalty stiffness. */
 static double ml_fine_residual(orc_ml* M, const orc_params* p) {
     orc_level* L = M->L[0];
     dcd_point(L, p);
     tet_forces(L, p->if_mode);
     gather_forces(L);
+    if (!M->cmask) M->cmask = (real*)xcalloc((size_t)L->V, sizeof(real));
     double acc = 0;
     real dt = p->dt;
     for (int i = 0; i < L->V; i++) {
         real dc = L->mass[i] / (dt * dt);
         real fx = L->fix[i];
+        M->cmask[i] = (L->cdiag[i * 3] + L->cdiag[i * 3 + 1] + L->cdiag[i * 3 + 2] > 0) ? 0 : 1;
         for (int c = 0; c < 3; c++) {
             int q = i * 3 + c;
             real g = (dc * (L->s[q] - L->x[q]) + (L->force[q] + L->cforce[q])) * fx;
-            M->r[0][q] = g;
+            M->r[0][q] = g * M->cmask[i];
             acc += (double)g * (double)g;
             L->xprev[q] = L->x[q];
         }
     }
     return sqrt(acc);
+}
+
+/* Mask of the fine-level coarse corrections: Dirichlet flags times the
+ * active-set mask of the last residual evaluation. */
+static const real* ml_fine_mask(orc_ml* M) {
+    orc_level* L = M->L[0];
+    if (!M->fmask) M->fmask = (real*)xcalloc((size_t)L->V, sizeof(real));
+    for (int i = 0; i < L->V; i++) M->fmask[i] = L->fix[i] * (M->cmask ? M->cmask[i] : 1);
+    return M->fmask;
 }
 
 orc_ml* PFX(ml_create)(orc_level** levels, int nl, float relax_coarse, float dt, float rho_coarse) {
@@ -1438,6 +1473,8 @@ void PFX(ml_free)(orc_ml* M) {
     free(M->chol);
     free(M->warm);
     free(M->warm0);
+    free(M->cmask);
+    free(M->fmask);
     free(M);
 }
 
@@ -1513,7 +1550,7 @@ void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nstep
             }
         }
         if (M->has_warm && !conv) {
-            ml_prolong_add(M, 0, M->warm, L0->x, L0->fix);
+            ml_prolong_add(M, 0, M->warm, L0->x, ml_fine_mask(M));
             double rw = ml_fine_residual(M, p);
             M->res_ws = rw;
             if (rw > M->res) {
@@ -1545,7 +1582,7 @@ void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nstep
             }
             ml_restrict(M, 0, M->r[0], M->r[1]);
             ml_vcycle_lin(M, 1, vp);
-            ml_prolong_add(M, 0, M->e[1], L0->x, L0->fix);
+            ml_prolong_add(M, 0, M->e[1], L0->x, ml_fine_mask(M));
             memcpy(L0->xprev, L0->x, sizeof(real) * 3 * (size_t)L0->V);
             omega = 1;
             for (int k = 0; k < vp->post; k++) {

@@ -1,0 +1,51 @@
+"""Organ-like blob on nested covering voxel levels (h, 3h, 9h), pinned at
+x < -5: the V-cycle meets the residual target every step, and the learned
+schedule, trained on recorded steps of the same blob, stays within the
+target and does not cost half-cycles against the temporal warm start."""
+import numpy as np
+import pytest
+
+import multigridsoftbody_amd as M
+from multigridsoftbody_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def blob_solver(lv, schedule, **kw):
+    base = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, check_every=2, tol=1e-3,
+                rho_smooth=0.9, use_graph=1)
+    base.update(kw)
+    s = M.Solver(lv, schedule=schedule, if_mode=M.IF_STABLE, **base, **W.physics())
+    fix = W.blob_fixflag(lv[0])
+    s.set_fixed(fix)
+    s.set_velocity(W.seeded_velocity(lv[0], fix))
+    return s
+
+
+def test_blob_vcycle_and_learned():
+    lv = W.blob_hierarchy(h=0.45)  # a coarser copy of the workload keeps the test short
+    assert lv[0].ntets > 20000
+    rec = blob_solver(lv, M.SCHED_VCYCLE, warm_start=1)
+    rec.learn_record(150)
+    rec.step(150)
+    hist = rec.learn_train(epochs=15, batch=1024, lr=2e-3, seed=0)
+    print(f"[blob] training loss {hist[0]:.3f} -> {hist[-1]:.3f}")
+    res = {}
+    for name, sched, kw in (("plain", M.SCHED_VCYCLE, dict(check_every=1)),
+                            ("temporal", M.SCHED_VCYCLE, dict(warm_start=1)), ("learned", M.SCHED_LEARNED, {})):
+        s = blob_solver(lv, sched, **kw)
+        if name == "learned":
+            s.learn_weights(rec.learn_weights())
+            s.learn_norm(rec.learn_norm())
+        s.step(100)
+        run = s.run_stats()
+        res[name] = run["total_halves"] / run["steps"]
+        print(f"[blob] {name:9s} half-cycles/step {res[name]:.2f} fallbacks {run['fallbacks']} "
+              f"max rel {run['max_rel']:.2e}")
+        assert run["failed_steps"] == 0 and run["max_rel"] <= 1e-3
+        x = s.positions()
+        assert np.all(np.isfinite(x))
+        pinned = W.blob_fixflag(lv[0]) == 0
+        assert np.array_equal(x[pinned], lv[0].pos[pinned])
+    assert res["temporal"] < res["plain"]
+    assert res["learned"] <= res["temporal"] + 0.1
