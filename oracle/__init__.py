@@ -89,6 +89,7 @@ _SIGS = {
     "ml_nfree": (C.c_int, [_vp]),
     "ml_get_vfix": (None, [_vp, C.c_int, _fp]),
     "ml_stats": (None, [_vp, _dp]),
+    "ml_get_mask": (None, [_vp, _fp]),
     "ml_set_warm": (None, [_vp, _fp, _fp]),
     "step_vcycle": (None, [_vp, C.POINTER(orc_params), C.POINTER(orc_vparams), C.c_int]),
     "ml_coarse_solve": (None, [_vp, _fp, _fp]),
@@ -309,6 +310,13 @@ class MultiLevel:
     def vfix(self, level):
         out = np.zeros(self.levels[level].nv, np.float32)
         self._f("ml_get_vfix")(self._h, level, out.ctypes.data_as(_fp))
+        return out
+
+    def active_mask(self):
+        """Active-set mask of the last fine residual evaluation (0 where a
+        collider acts)."""
+        out = np.zeros(self.levels[0].nv, np.float32)
+        self._f("ml_get_mask")(self._h, out.ctypes.data_as(_fp))
         return out
 
     def step(self, p, vp, nsteps=1):

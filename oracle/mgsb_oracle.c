@@ -1483,6 +1483,11 @@ int PFX(ml_nfree)(const orc_ml* M) { return M->chol ? M->nfree : -1; }
 void PFX(ml_get_vfix)(const orc_ml* M, int l, float* out) {
     for (int i = 0; i < M->L[l]->V; i++) out[i] = (float)M->vfix[l][i];
 }
+/* Active-set mask of the last fine residual evaluation (1 where no collider
+ * acts, 0 where one does); all ones before the first evaluation. */
+void PFX(ml_get_mask)(const orc_ml* M, float* out) {
+    for (int i = 0; i < M->L[0]->V; i++) out[i] = M->cmask ? (float)M->cmask[i] : 1.0f;
+}
 void PFX(ml_stats)(const orc_ml* M, double* out) {
     out[0] = M->res0;
     out[1] = M->res;

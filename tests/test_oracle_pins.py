@@ -1,0 +1,237 @@
+"""Closed-form pins of the oracle's contact, normal, energy and active-set
+functions (oracle/mgsb_oracle.c: dcd_point_plane, dcd_tri_sphere +
+gather_tri_collision, tri_normals / vertex_normals, energy, ml_fine_residual's
+active-set mask).  Every expected value comes from geometry fixed by
+construction (hand-placed spheres and planes, axis-aligned faces, uniform
+stretches), not from the oracle's own formulas.  CPU only."""
+import numpy as np
+import pytest
+
+import oracle as O
+from multigridsoftbody_amd import Mesh
+from multigridsoftbody_amd import workloads as W
+
+# tet A=(0,0,0) B=(4,0,0) C=(0,4,0) D=(0,0,4)
+_TET_POS = np.array([[0, 0, 0], [4, 0, 0], [0, 4, 0], [0, 0, 4]], np.float32)
+_TET = np.array([[0, 1, 2, 3]], np.int32)
+# faces z = 0 (outward -z) and y = 0 (outward -y), both containing edge AB
+_FACE_Z0 = [0, 2, 1]
+_FACE_Y0 = [0, 1, 3]
+
+
+def _tri_level(tris):
+    return O.Level(_TET_POS, _TET, np.array(tris, np.uint32), fp64=True)
+
+
+def _push(center, closest, r, k):
+    """Sphere-triangle contact: the triangle's vertices are pushed by
+    k (r - |closest - c|) along unit(closest - c), the move that takes the
+    triangle's closest point out to the sphere surface."""
+    d = np.asarray(closest, float) - np.asarray(center, float)
+    dist = np.linalg.norm(d)
+    assert dist < r
+    return k * (r - dist) * d / dist
+
+
+@pytest.mark.parametrize("case", ["face", "edge", "vertex"])
+def test_tri_sphere_contact_closed_form(case):
+    """DCDByTriangle_sphere_V2 + _response (PDSolver.cu:612-680, :462-501):
+    the closest feature of the triangle (interior, edge, vertex) to the sphere
+    centre decides the hit point; each of the triangle's three vertices gets
+    diag += k and force += k (c - hit)."""
+    r, k = 0.5, 1000.0
+    center, closest = {
+        "face": ((1.0, 1.0, -0.3), (1.0, 1.0, 0.0)),     # projection inside the triangle
+        "edge": ((2.0, -0.3, -0.2), (2.0, 0.0, 0.0)),    # beyond edge AB, within r of it
+        "vertex": ((-0.2, -0.2, -0.1), (0.0, 0.0, 0.0)),  # beyond A, off both edges' extents
+    }[case]
+    L = _tri_level([_FACE_Z0])
+    L.collide(O.params(kcol=k, spheres=[(*center, r, 1)]), tri=True)
+    f = L.get(O.CFORCE).reshape(-1, 3)
+    cd = L.get(O.CDIAG).reshape(-1, 3)
+    flags = L.get(O.CFLAG)
+    push = _push(center, closest, r, k)
+    for v in (0, 1, 2):
+        assert np.allclose(f[v], push, rtol=1e-6, atol=1e-9), (case, v, f[v], push)
+        assert np.allclose(cd[v], k)
+        assert flags[v] == 1
+    assert np.allclose(f[3], 0) and np.allclose(cd[3], 0) and flags[3] == 0
+
+
+def test_tri_sphere_misses():
+    """Spheres farther than r from the triangle touch nothing: off the plane,
+    and within r of the plane but beyond every edge and vertex."""
+    L = _tri_level([_FACE_Z0])
+    for c in ((1.0, 1.0, -0.6), (5.0, 5.0, -0.1), (-1.0, 2.0, 0.0)):
+        L.collide(O.params(kcol=1000.0, spheres=[(*c, 0.5, 1)]), tri=True)
+        assert np.allclose(L.get(O.CFORCE), 0) and np.allclose(L.get(O.CFLAG), 0), c
+
+
+def test_tri_sphere_shared_edge_counts_both_triangles():
+    """Two faces meeting at edge AB, sphere centre off that edge: both
+    triangles hit at the same edge point, so A and B collect two
+    contributions (flag 2) and C, D one each -- the per-vertex count the
+    reference accumulates with atomics (PDSolver.cu:495-500)."""
+    r, k = 0.5, 1000.0
+    c = (2.0, -0.2, -0.2)
+    L = _tri_level([_FACE_Z0, _FACE_Y0])
+    L.collide(O.params(kcol=k, spheres=[(*c, r, 1)]), tri=True)
+    push = _push(c, (2.0, 0.0, 0.0), r, k)
+    f = L.get(O.CFORCE).reshape(-1, 3)
+    cd = L.get(O.CDIAG).reshape(-1, 3)
+    flags = L.get(O.CFLAG)
+    for v, n in ((0, 2), (1, 2), (2, 1), (3, 1)):
+        assert flags[v] == n
+        assert np.allclose(f[v], n * push, rtol=1e-6)
+        assert np.allclose(cd[v], n * k)
+
+
+def test_point_plane_closed_form():
+    """Ground plane n.x = d as a penalty k((x - q).n)^2: a vertex at signed
+    distance -depth gets force k depth n, diag k n*n, energy k depth^2;
+    vertices on or above the plane get nothing.  Checked with an oblique
+    plane so that a swapped component or a missing n factor fails."""
+    n = np.array([1.0, 2.0, 2.0]) / 3.0
+    d = 0.5
+    k = 1e4
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0.5, 0.25, 0.0]], np.float32)
+    tets = np.array([[0, 1, 2, 3], [4, 1, 2, 3]], np.int32)
+    L = O.Level(pos, tets, None, fp64=True)
+    L.collide(O.params(kcol=k, plane=(*n, d, k)))
+    sd = pos.astype(np.float64) @ n - d
+    f = L.get(O.CFORCE).reshape(-1, 3)
+    cd = L.get(O.CDIAG).reshape(-1, 3)
+    e = L.get(O.CENERGY)
+    flags = L.get(O.CFLAG)
+    depth = L.get(O.CDEPTH)
+    for i, s in enumerate(sd):
+        if s < 0:
+            assert np.allclose(f[i], k * (-s) * n, rtol=1e-5), i
+            assert np.allclose(cd[i], k * n * n, rtol=1e-6)
+            assert np.isclose(e[i], k * s * s, rtol=1e-5)
+            assert flags[i] == 1 and np.isclose(depth[i], -s, rtol=1e-5)
+        else:
+            assert np.allclose(f[i], 0) and np.allclose(cd[i], 0) and e[i] == 0 and flags[i] == 0
+    assert (sd < 0).sum() == 3 and (sd >= 0).sum() == 2  # (0,1,0), (0,0,1): n.p = 2/3 > d
+
+
+def test_plane_and_sphere_depth_is_the_larger():
+    """A vertex touched by a sphere and the plane keeps the deeper
+    penetration as its depth and the sum of both responses."""
+    pos = np.array([[0, -0.2, 0], [3, 0, 0], [0, 3, 0], [0, 0, 3]], np.float32)
+    L = O.Level(pos, np.array([[0, 1, 2, 3]], np.int32), None, fp64=True)
+    k = 1000.0
+    L.collide(O.params(kcol=k, spheres=[(0.5, -0.2, 0.0, 1.0, 1)], plane=(0, 1, 0, 0.0, k)))
+    # sphere: d = 0.5 < 1.5, t = 1.0 > 0.5 r -> depth 0.5, push t/2 = 0.5 along -x
+    # plane: depth 0.2 along +y
+    assert np.isclose(L.get(O.CDEPTH)[0], 0.5)
+    f = L.get(O.CFORCE).reshape(-1, 3)[0]
+    assert np.allclose(f, [-k * 0.5, k * 0.2, 0.0])
+    assert np.allclose(L.get(O.CDIAG).reshape(-1, 3)[0], [k, k, 0.0])
+
+
+def test_triangle_and_vertex_normals_on_a_cube():
+    """UpdateTriNormal / runUpdateOutsideTetVertNormal (PDSolver.cu:394-460)
+    on a 3x3x3-cell cube: every surface triangle's normal is the outward
+    axis normal of the face it lies on, every face-interior vertex's normal
+    is that axis normal, and every surface vertex's normal points outward
+    (the cube is convex)."""
+    m = Mesh.box(3, 3, 3, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    L = O.Level(m.pos, m.tets, m.tris, fp64=True)
+    vn = L.vertex_normals()
+    tn = L.get(O.TRIN).reshape(-1, 3)
+    P = m.pos.astype(np.float64)
+    cen = P[m.tris].mean(axis=1)
+    for t in range(m.ntris):
+        c = cen[t]
+        ax = int(np.argmin(np.minimum(c, 1 - c)))  # the face the triangle lies on
+        want = np.zeros(3)
+        want[ax] = 1.0 if c[ax] > 0.5 else -1.0
+        assert np.allclose(tn[t], want, atol=1e-12), (t, tn[t], want)
+    surf = np.unique(m.tris)
+    on_face = np.isclose(P, 0) | np.isclose(P, 1)
+    interior = surf[on_face[surf].sum(axis=1) == 1]
+    assert len(interior) == 6 * 4
+    for v in interior:
+        ax = int(np.flatnonzero(on_face[v])[0])
+        want = np.zeros(3)
+        want[ax] = 1.0 if P[v, ax] > 0.5 else -1.0
+        assert np.allclose(vn[v], want, atol=1e-12), (v, vn[v])
+    assert np.all(np.einsum("ij,ij->i", vn[surf], P[surf] - 0.5) > 0)
+    assert np.allclose(np.linalg.norm(vn[surf], axis=1), 1.0)
+    inner = np.setdiff1d(np.arange(m.nverts), surf)
+    assert np.allclose(vn[inner], 0)  # not on any surface triangle
+
+
+def test_vertex_normals_single_tet_closed_form():
+    """Unit tet: the corner where three orthogonal faces meet gets
+    -(1,1,1)/sqrt(3); vertex (1,0,0) gets the normalised sum of the unit
+    normals -y, -z and (1,1,1)/sqrt(3)."""
+    m = Mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32), np.array([[0, 1, 2, 3]], np.int32))
+    L = O.Level(m.pos, m.tets, m.tris, fp64=True)
+    vn = L.vertex_normals()
+    s = 1 / np.sqrt(3)
+    assert np.allclose(vn[0], [-s, -s, -s], atol=1e-12)
+    want = np.array([s, -1 + s, -1 + s])
+    assert np.allclose(vn[1], want / np.linalg.norm(want), atol=1e-12)
+
+
+def test_energy_closed_forms():
+    """runCalEnergy (PDSolver.cu:816-913) terms against closed forms:
+    Ep = 1/2 k sum(vol) sum_i (sigma_i - 1)^2 for x = Q diag(sigma) X (F = Q
+    diag(sigma), R = Q); Ek = sum(m) |delta|^2 / (2 h^2) and
+    dX = sqrt(V) |delta| for x = s + delta; Ec = sum k depth^2 and
+    Nc = number of vertices below a plane."""
+    m = Mesh.box(2, 4, 2, (-1.0, -2.0, -1.0), (1.0, 2.0, 1.0))
+    vol_total, k, rho, h = 16.0, 5e3, 7.0, 1 / 60
+    L = O.Level(m.pos, m.tets, m.tris, stiffness=k, density=rho, fp64=True)
+    p = O.params(dt=h, stiffness=k, density=rho, gravity=(0, 0, 0), iters=0)
+    L.set_velocity(np.zeros((m.nverts, 3), np.float32))
+    L.step(p, 1)  # s = x = X, x_prev = X
+    sig = np.array([1.1, 0.9, 1.25])
+    th = 0.6
+    Q = np.array([[np.cos(th), 0, np.sin(th)], [0, 1, 0], [-np.sin(th), 0, np.cos(th)]])
+    X = m.pos.astype(np.float64)
+    L.set_positions((X @ (Q @ np.diag(sig)).T).astype(np.float32))
+    e = L.energy(p)
+    assert np.isclose(e["Ep"], 0.5 * k * vol_total * ((sig - 1) ** 2).sum(), rtol=1e-5)
+
+    delta = np.array([0.01, -0.02, 0.005])
+    L.set_positions((X + delta).astype(np.float32))
+    e = L.energy(p)
+    assert np.isclose(e["Ep"], 0.0, atol=1e-6)  # rigid translation
+    assert np.isclose(e["Ek"], rho * vol_total * (delta ** 2).sum() / (2 * h * h), rtol=1e-5)
+    assert np.isclose(e["dX"], np.sqrt(m.nverts * (delta ** 2).sum()), rtol=1e-5)
+
+    L.set_positions(X.astype(np.float32))
+    kp, y0 = 1e4, -1.5
+    L.collide(O.params(kcol=kp, plane=(0, 1, 0, y0, kp)))
+    e = L.energy(p)
+    below = X[:, 1] < y0  # the y = -2 layer, 0.5 below the plane
+    assert below.sum() == 9
+    assert e["Nc"] == below.sum()
+    assert np.isclose(e["Ec"], kp * ((y0 - X[below, 1]) ** 2).sum(), rtol=1e-6)
+
+
+def test_active_set_mask_is_where_a_collider_acts():
+    """The V-cycle's active set (ml_fine_residual): after a step, the mask is
+    0 exactly on vertices within 1.5 r of the sphere or below the plane at
+    the final positions, and 1 elsewhere."""
+    lv = [W.beam(2), W.beam(1)]
+    fix = W.top_fixflag(lv[0])
+    Ls = [O.Level(mm.pos, mm.tets, mm.tris, stiffness=1e5, density=1000.0, fp64=True) for mm in lv]
+    Ls[0].set_fix(fix)
+    Ls[0].set_velocity(W.seeded_velocity(lv[0], fix))
+    ML = O.MultiLevel(Ls)
+    sphere = (1.2, 0.0, 0.0, 0.5)
+    y0, k = -9.9, 1e4
+    p = O.params(stiffness=1e5, density=1000.0, gravity=(0, -9.8, 0), kcol=k, if_mode=3,
+                 spheres=[(*sphere, 1)], plane=(0, 1, 0, y0, k))
+    ML.step(p, O.vparams(pre=3, post=3, max_cycles=3, check=1), 3)
+    x = Ls[0].positions()
+    mask = ML.active_mask()
+    in_sphere = np.linalg.norm(x - np.array(sphere[:3]), axis=1) < 1.5 * sphere[3]
+    below = x[:, 1] < y0
+    want = np.where(in_sphere | below, 0.0, 1.0)
+    assert in_sphere.sum() > 0 and below.sum() > 0
+    assert np.array_equal(mask, want)
