@@ -692,43 +692,34 @@ __device__ void vstate_step_done(VState* st) {
     st->nfallback += st->fallback != 0;
 }
 
-// Second stage of a block-partial reduction, fused into the kernel that
-// produced the partials: each block has stored part[blockIdx.x]; the block
-// that arrives last (device-scope counter) sums all partials in block order
-// -- the same order on every run -- and applies the check.  It resets the
-// counter for the next launch.
-__device__ void last_block_check(const double* part, unsigned* count, VState* st, int mode, int* hm) {
-    __shared__ int s_last;
+// Second stage of the residual-norm reduction: one workgroup sums the nb
+// block partials in block order (the same order on every run) and applies
+// the check.  A separate launch: completing the norm inside the producing
+// kernel (the last block to arrive, found with a device-scope counter) costs
+// far more on this GPU, since the blocks' atomics on one counter serialise.
+__global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict__ part, VState* st, int mode,
+                                               int* hm, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     __shared__ double sh[256];
-    if (threadIdx.x == 0) {
-        __threadfence();
-        s_last = atomicAdd(count, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
     double a = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-        a += __hip_atomic_load(part + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) a += part[b];
     sh[threadIdx.x] = a;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0.0;
         for (int q = 0; q < (int)blockDim.x; q++) t += sh[q];
-        *count = 0u;
         vstate_apply(st, t, mode, hm);
     }
 }
 
 // Nonlinear residual g = m/h^2 (s - x) + f_el + f_col over free DOFs; writes
-// g (float4) when r != nullptr and the FP64 partial of |g|^2.  With
-// mode != kChkNone the last block completes the norm and applies the check.
+// g (float4) when r != nullptr and the FP64 block partials of |g|^2 when
+// part != nullptr (k_check completes the norm).
 __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__ x, const float4* __restrict__ s,
                                                const float4* __restrict__ vc, const int* __restrict__ rp,
                                                const float3* __restrict__ cf, const DevColliders* __restrict__ col,
                                                int flags, float4* __restrict__ r, double* __restrict__ part,
                                                float4* __restrict__ xprev, const int* __restrict__ skip, int split,
-                                               VState* st, unsigned* count, int mode, int* hm,
                                                double* __restrict__ g2, float* __restrict__ cmask) {
     if (skip && *skip) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -795,10 +786,7 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
         v[0] = (double)gx * gx + (double)gy * gy + (double)gz * gz;
         if (g2) g2[i] = v[0];
     }
-    if (part) {
-        block_reduce_store<1>(v, part);
-        if (mode != kChkNone) last_block_check(part, count, st, mode, hm);
-    }
+    if (part) block_reduce_store<1>(v, part);
 }
 
 // Per-body residual norms of several bodies (each a contiguous vertex range
@@ -1422,7 +1410,6 @@ struct mgsb_solver {
     mgsb_step_stats last{};
     VState* vstate = nullptr;
     double* vpart = nullptr;  // partials of the level-0 residual norm
-    unsigned* vcount = nullptr;  // arrival counter of the fused residual check
     int* hmirror = nullptr;      // host-mapped (converged, cycles, halves, fallback, accepted)
     // V-cycle steps run as pieces (start, pre-smoothing half, coarse + post
     // half, finish), each captured once; the host launches only the halves a
@@ -1790,7 +1777,7 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool res
     if (s->body_checks && chk != kChkNone) {  // per-body norms, then the check
         k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint,
                                            write_r ? d.r : nullptr, nullptr, restart ? d.xprev : nullptr, skip, 1,
-                                           nullptr, nullptr, kChkNone, nullptr, s->g2, d.cmask);
+                                           s->g2, d.cmask);
         HIPCHK(hipGetLastError());
         k_body_check<<<s->nbodies, 256, 0, s->stream>>>(s->vb, s->g2, s->bodysq, s->res0b, s->bcount, s->vstate, chk,
                                                         s->hmirror, skip);
@@ -1799,8 +1786,12 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool res
     }
     k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, write_r ? d.r : nullptr,
                                        chk != kChkNone ? s->vpart : nullptr, restart ? d.xprev : nullptr, skip, 1,
-                                       s->vstate, s->vcount, chk, s->hmirror, nullptr, d.cmask);
+                                       nullptr, d.cmask);
     HIPCHK(hipGetLastError());
+    if (chk != kChkNone) {
+        k_check<<<1, 256, 0, s->stream>>>(nb, s->vpart, s->vstate, chk, s->hmirror, skip);
+        HIPCHK(hipGetLastError());
+    }
     return MGSB_OK;
 }
 
@@ -2163,9 +2154,6 @@ int alloc_vcycle(mgsb_solver* s) {
     s->owned.push_back(s->vstate);
     if ((rc = dalloc(&s->vpart, (size_t)nblk(s->L[0].V, 256)))) return rc;
     s->owned.push_back(s->vpart);
-    if ((rc = dalloc(&s->vcount, 1))) return rc;
-    s->owned.push_back(s->vcount);
-    HIPCHK(hipMemsetAsync(s->vcount, 0, sizeof(unsigned), s->stream));
     HIPCHK(hipHostMalloc((void**)&s->hmirror, 8 * sizeof(int), hipHostMallocDefault));
     std::memset(s->hmirror, 0, 8 * sizeof(int));
     return push_vstate(s);
@@ -2200,7 +2188,7 @@ int residual_norm(mgsb_solver* s, double* out) {
     const int nb = nblk(d.V, 256);
     if ((rc = ensure_partials(s, (size_t)nb * 4))) return rc;
     k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, nullptr, s->dpart,
-                                       nullptr, nullptr, 0, nullptr, nullptr, kChkNone, nullptr, nullptr, nullptr);
+                                       nullptr, nullptr, 0, nullptr, nullptr);
     HIPCHK(hipGetLastError());
     double sq = 0;
     if ((rc = reduce_finish(s, nb, 1, &sq))) return rc;
