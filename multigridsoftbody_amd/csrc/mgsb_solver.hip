@@ -87,11 +87,13 @@ __device__ __forceinline__ void gather_sum(const float3* __restrict__ cf, int e0
 
 // calculateST (PDSolver.cu:72-116).  Block 0 / lane 0 first advances sphere
 // centres by vel*dt (UpdateCollider before Step, Simulator.cpp:146-149).
+// split: x and x_prev take the in-step correction u (0, or with U1/U2 the
+// temporal extrapolation of the last two steps' corrections).
 __global__ void __launch_bounds__(256) k_st(int V, float4* __restrict__ x, float4* __restrict__ v,
                                             float4* __restrict__ xlast, float4* __restrict__ s,
                                             float4* __restrict__ xprev, const float4* __restrict__ vc, float dt,
                                             float damp, float gx, float gy, float gz, DevColliders* col, int advance,
-                                            int split) {
+                                            int split, const float4* __restrict__ U1, const float4* __restrict__ U2) {
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (advance && blockIdx.x == 0 && threadIdx.x == 0) {
         for (int q = 0; q < col->nsph; q++) {
@@ -120,8 +122,14 @@ __global__ void __launch_bounds__(256) k_st(int V, float4* __restrict__ x, float
     v[i] = u;
     s[i] = p;
     if (split) {  // x and x_prev hold the in-step correction u = x - s
-        x[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        xprev[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        // U1 != nullptr: temporal warm start u_t = fix (2 u_{n-1} - u_{n-2})
+        float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (U1) {
+            const float4 a = U1[i], b = U2[i];
+            q = make_float4(fix * (2.f * a.x - b.x), fix * (2.f * a.y - b.y), fix * (2.f * a.z - b.z), 0.f);
+        }
+        x[i] = q;
+        xprev[i] = q;
     } else {
         x[i] = p;
         xprev[i] = p;
@@ -205,48 +213,17 @@ __device__ __forceinline__ bool tri_sphere_hit(float3 A, float3 B, float3 C, flo
     return false;
 }
 
-// Per-tet local step, one tet per lane: F = Ds Dm^-1, R - F by the Franca
-// polar decomposition, P = (R - F) AC with AC rebuilt from Dm^-1, NaN -> 0,
-// clamp to [-10, 10], times vol * k (calculateIF, PDSolver.cu:128-183;
-// calTetIFV1's |f| < 1e-5 drop in Ac mode, PDSolverAc.cu:630-662).  Corner j's
-// force goes to cf[slot_j].  Lanes T..T+S-1 (when ntri > 0) run the triangle-
-// sphere DCD (PDSolver.cu:612-680) and write (k*(centre-hit), k) per corner.
+// Per-tet local step of tet g: F = Ds Dm^-1, R - F by the Franca polar
+// decomposition, P = (R - F) AC with AC rebuilt from Dm^-1, NaN -> 0, clamp to
+// [-10, 10], times vol * k (calculateIF, PDSolver.cu:128-183; calTetIFV1's
+// |f| < 1e-5 drop in Ac mode, PDSolverAc.cu:630-662).  Corner j's force goes
+// to cf[slot_j].  sref != nullptr: split positions (x holds u, see below).
 template <int MODE>
-__global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x, const int4* __restrict__ tet,
-                                             const float* __restrict__ B, const float* __restrict__ vol,
-                                             const int4* __restrict__ slot, float3* __restrict__ cf, float k,
-                                             int ntri, const int4* __restrict__ tri, const int4* __restrict__ tslot,
-                                             float4* __restrict__ tcf, const DevColliders* __restrict__ col,
-                                             const int* __restrict__ skip, const float4* __restrict__ sref,
-                                             const float* __restrict__ tk) {
-    if (skip && *skip) return;
-    const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (g >= T) {
-        const int t = g - T;
-        if (t >= ntri) return;
-        const int4 id = tri[t];
-        const float3 A = f3(x[id.x]), Bq = f3(x[id.y]), Cq = f3(x[id.z]);
-        float3 n = cross3(sub3(Bq, A), sub3(Cq, A));
-        n = div3(n, len3(n));
-        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-        const float kc = col->kcol;
-        for (int q = 0; q < col->nsph; q++) {
-            if (col->vel[q].w == 0.f) continue;
-            const float4 sp = col->sph[q];
-            float3 mv;
-            if (tri_sphere_hit(A, Bq, Cq, n, make_float3(sp.x, sp.y, sp.z), sp.w, mv)) {
-                acc.x = acc.x + kc * mv.x;
-                acc.y = acc.y + kc * mv.y;
-                acc.z = acc.z + kc * mv.z;
-                acc.w = acc.w + kc;
-            }
-        }
-        const int4 sl = tslot[t];
-        tcf[sl.x] = acc;
-        tcf[sl.y] = acc;
-        tcf[sl.z] = acc;
-        return;
-    }
+__device__ __forceinline__ void tet_local_step(int g, int T, const float4* __restrict__ x,
+                                               const int4* __restrict__ tet, const float* __restrict__ B,
+                                               const float* __restrict__ vol, const int4* __restrict__ slot,
+                                               float3* __restrict__ cf, float k, const float4* __restrict__ sref,
+                                               const float* __restrict__ tk) {
     const int4 id = tet[g];
     const float4 p0 = x[id.x], p1 = x[id.y], p2 = x[id.z], p3 = x[id.w];
     float b[9];
@@ -317,6 +294,65 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
         }
         cf[sls[j]] = make_float3(f[0], f[1], f[2]);
     }
+}
+
+// Per-tet local step kernel, one tet per lane (tet_local_step).  Lanes
+// T..T+S-1 (when ntri > 0) run the triangle-sphere DCD (PDSolver.cu:612-680)
+// and write (k*(centre-hit), k) per corner.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x, const int4* __restrict__ tet,
+                                             const float* __restrict__ B, const float* __restrict__ vol,
+                                             const int4* __restrict__ slot, float3* __restrict__ cf, float k,
+                                             int ntri, const int4* __restrict__ tri, const int4* __restrict__ tslot,
+                                             float4* __restrict__ tcf, const DevColliders* __restrict__ col,
+                                             const int* __restrict__ skip, const float4* __restrict__ sref,
+                                             const float* __restrict__ tk) {
+    if (skip && *skip) return;
+    const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (g >= T) {
+        const int t = g - T;
+        if (t >= ntri) return;
+        const int4 id = tri[t];
+        const float3 A = f3(x[id.x]), Bq = f3(x[id.y]), Cq = f3(x[id.z]);
+        float3 n = cross3(sub3(Bq, A), sub3(Cq, A));
+        n = div3(n, len3(n));
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        const float kc = col->kcol;
+        for (int q = 0; q < col->nsph; q++) {
+            if (col->vel[q].w == 0.f) continue;
+            const float4 sp = col->sph[q];
+            float3 mv;
+            if (tri_sphere_hit(A, Bq, Cq, n, make_float3(sp.x, sp.y, sp.z), sp.w, mv)) {
+                acc.x = acc.x + kc * mv.x;
+                acc.y = acc.y + kc * mv.y;
+                acc.z = acc.z + kc * mv.z;
+                acc.w = acc.w + kc;
+            }
+        }
+        const int4 sl = tslot[t];
+        tcf[sl.x] = acc;
+        tcf[sl.y] = acc;
+        tcf[sl.z] = acc;
+        return;
+    }
+    tet_local_step<MODE>(g, T, x, tet, B, vol, slot, cf, k, sref, tk);
+}
+
+// Start of a temporally warm-started V-cycle step: lanes 0..T-1 evaluate the
+// local step at the extrapolation (split, u = x, forces to cf), lanes
+// T..2T-1 at the inertial prediction s itself (forces to cf2), so that one
+// launch serves both residual norms the start needs.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_tet_dual(int T, const float4* __restrict__ x, const float4* __restrict__ s,
+                                                  const int4* __restrict__ tet, const float* __restrict__ B,
+                                                  const float* __restrict__ vol, const int4* __restrict__ slot,
+                                                  float3* __restrict__ cf, float3* __restrict__ cf2, float k,
+                                                  const float* __restrict__ tk) {
+    const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (g < T)
+        tet_local_step<MODE>(g, T, x, tet, B, vol, slot, cf, k, s, tk);
+    else if (g < 2 * T)
+        tet_local_step<MODE>(g - T, T, s, tet, B, vol, slot, cf2, k, nullptr, tk);
 }
 
 // Per-vertex update: gather the CSR range of corner forces (tet order), add
@@ -712,9 +748,97 @@ __global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict_
     }
 }
 
-// Nonlinear residual g = m/h^2 (s - x) + f_el + f_col over free DOFs; writes
-// g (float4) when r != nullptr and the FP64 block partials of |g|^2 when
-// part != nullptr (k_check completes the norm).
+// Both checks of a temporally warm-started start (k_resid_dual's partials,
+// each summed in block order like k_check): res0 = |g(s)|, then the judge of
+// the extrapolation.  Converged already at s: x (which holds the
+// extrapolation) must return to s, so revert is set without a verdict.
+__global__ void __launch_bounds__(256) k_check2(int nb, const double* __restrict__ part, VState* st, int* hm) {
+    __shared__ double sh[2][256];
+    double a = 0.0, b = 0.0;
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) {
+        a += part[2 * q];
+        b += part[2 * q + 1];
+    }
+    sh[0][threadIdx.x] = a;
+    sh[1][threadIdx.x] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int q = 0; q < (int)blockDim.x; q++) {
+            t0 += sh[0][q];
+            t1 += sh[1][q];
+        }
+        vstate_apply(st, t0, kChkStart, hm);
+        if (!st->converged)
+            vstate_apply(st, t1, kChkTemporal, hm);
+        else
+            st->revert = 1;
+    }
+}
+
+// Nonlinear residual g = m/h^2 (s - x) + f_el + f_col of vertex i on free
+// DOFs from its gathered corner forces cf: p is x (or the correction u when
+// split), st = s, c = (m/h^2, diag, fix, m).  contact: a collider acts on the
+// vertex (the active set).
+__device__ __forceinline__ float3 vertex_residual(int i, float4 p, float4 st, float4 c, const int* __restrict__ rp,
+                                                  const float3* __restrict__ cf,
+                                                  const DevColliders* __restrict__ col, int flags, int split,
+                                                  bool& contact) {
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    gather_sum(cf, rp[i], rp[i + 1], fx, fy, fz);
+    const float4 pa = split ? make_float4(st.x + p.x, st.y + p.y, st.z + p.z, 0.f) : p;
+    const float sx = split ? -p.x : st.x - p.x, sy = split ? -p.y : st.y - p.y, sz = split ? -p.z : st.z - p.z;
+    float cfx = 0.f, cfy = 0.f, cfz = 0.f;
+    contact = false;
+    if (flags & kVertPoint) {
+        const float kc = col->kcol;
+        const float3 P = f3(pa);
+        for (int q = 0; q < col->nsph; q++) {
+            if (col->vel[q].w == 0.f) continue;
+            const float4 sp = col->sph[q];
+            const float3 center = make_float3(sp.x, sp.y, sp.z);
+            const float3 cp = split ? add3(sub3(f3(st), center), f3(p)) : sub3(P, center);
+            const float3 dir = div3(cp, len3(cp));
+            const float distance = len3(cp);
+            const float rr = sp.w * 1.5f;
+            if (distance < rr) {
+                contact = true;
+                const float bq = 2 * dot3(cp, dir);
+                const float cq = dot3(cp, cp) - rr * rr;
+                float t = (-bq + sqrtf(bq * bq - 4.f * cq)) / 2.f;
+                t = t / 2.0f;
+                const float3 colP = add3(P, mul3(dir, t));
+                const float3 ccp = split ? add3(cp, mul3(dir, t)) : sub3(colP, center);
+                const float3 n = div3(ccp, len3(ccp));
+                const float nd = dot3(n, dir);
+                cfx = cfx + kc * t * nd * n.x;
+                cfy = cfy + kc * t * nd * n.y;
+                cfz = cfz + kc * t * nd * n.z;
+            }
+        }
+        if (col->plane_on) {
+            const float4 pl = col->plane;
+            const float sd = split ? ((pl.x * st.x + pl.y * st.y + pl.z * st.z) - pl.w) +
+                                         (pl.x * p.x + pl.y * p.y + pl.z * p.z)
+                                   : pl.x * pa.x + pl.y * pa.y + pl.z * pa.z - pl.w;
+            if (sd < 0.f) {
+                contact = true;
+                cfx = cfx + col->kplane * (-sd) * pl.x;
+                cfy = cfy + col->kplane * (-sd) * pl.y;
+                cfz = cfz + col->kplane * (-sd) * pl.z;
+            }
+        }
+    }
+    const float fix = c.z;
+    const float gx = (c.x * sx + (fx + cfx)) * fix;
+    const float gy = (c.x * sy + (fy + cfy)) * fix;
+    const float gz = (c.x * sz + (fz + cfz)) * fix;
+    return make_float3(gx, gy, gz);
+}
+
+// Nonlinear residual over free DOFs (vertex_residual); writes g (float4) when
+// r != nullptr and the FP64 block partials of |g|^2 when part != nullptr
+// (k_check completes the norm).
 __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__ x, const float4* __restrict__ s,
                                                const float4* __restrict__ vc, const int* __restrict__ rp,
                                                const float3* __restrict__ cf, const DevColliders* __restrict__ col,
@@ -725,68 +849,47 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
     if (i < V) {
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-        gather_sum(cf, rp[i], rp[i + 1], fx, fy, fz);
         const float4 p = x[i];
-        const float4 st = s[i];
-        const float4 pa = split ? make_float4(st.x + p.x, st.y + p.y, st.z + p.z, 0.f) : p;
-        const float sx = split ? -p.x : st.x - p.x, sy = split ? -p.y : st.y - p.y, sz = split ? -p.z : st.z - p.z;
-        float cfx = 0.f, cfy = 0.f, cfz = 0.f;
-        bool contact = false;  // a collider acts on this vertex (active set)
-        if (flags & kVertPoint) {
-            const float kc = col->kcol;
-            const float3 P = f3(pa);
-            for (int q = 0; q < col->nsph; q++) {
-                if (col->vel[q].w == 0.f) continue;
-                const float4 sp = col->sph[q];
-                const float3 center = make_float3(sp.x, sp.y, sp.z);
-                const float3 cp = split ? add3(sub3(f3(st), center), f3(p)) : sub3(P, center);
-                const float3 dir = div3(cp, len3(cp));
-                const float distance = len3(cp);
-                const float rr = sp.w * 1.5f;
-                if (distance < rr) {
-                    contact = true;
-                    const float bq = 2 * dot3(cp, dir);
-                    const float cq = dot3(cp, cp) - rr * rr;
-                    float t = (-bq + sqrtf(bq * bq - 4.f * cq)) / 2.f;
-                    t = t / 2.0f;
-                    const float3 colP = add3(P, mul3(dir, t));
-                    const float3 ccp = split ? add3(cp, mul3(dir, t)) : sub3(colP, center);
-                    const float3 n = div3(ccp, len3(ccp));
-                    const float nd = dot3(n, dir);
-                    cfx = cfx + kc * t * nd * n.x;
-                    cfy = cfy + kc * t * nd * n.y;
-                    cfz = cfz + kc * t * nd * n.z;
-                }
-            }
-            if (col->plane_on) {
-                const float4 pl = col->plane;
-                const float sd = split ? ((pl.x * st.x + pl.y * st.y + pl.z * st.z) - pl.w) +
-                                             (pl.x * p.x + pl.y * p.y + pl.z * p.z)
-                                       : pl.x * pa.x + pl.y * pa.y + pl.z * pa.z - pl.w;
-                if (sd < 0.f) {
-                    contact = true;
-                    cfx = cfx + col->kplane * (-sd) * pl.x;
-                    cfy = cfy + col->kplane * (-sd) * pl.y;
-                    cfz = cfz + col->kplane * (-sd) * pl.z;
-                }
-            }
-        }
-        const float4 c = vc[i];
-        const float fix = c.z;
-        const float gx = (c.x * sx + (fx + cfx)) * fix;
-        const float gy = (c.x * sy + (fy + cfy)) * fix;
-        const float gz = (c.x * sz + (fz + cfz)) * fix;
+        bool contact;
+        const float3 g = vertex_residual(i, p, s[i], vc[i], rp, cf, col, flags, split, contact);
         // the active set is left out of the restriction: the coarse operators
         // carry no penalty stiffness (cmask also masks the coarse corrections)
         const float am = contact ? 0.f : 1.f;
         if (cmask) cmask[i] = am;
-        if (r) r[i] = cmask ? make_float4(gx * am, gy * am, gz * am, 0.f) : make_float4(gx, gy, gz, 0.f);
+        if (r) r[i] = cmask ? make_float4(g.x * am, g.y * am, g.z * am, 0.f) : make_float4(g.x, g.y, g.z, 0.f);
         if (xprev) xprev[i] = p;
-        v[0] = (double)gx * gx + (double)gy * gy + (double)gz * gz;
+        v[0] = (double)g.x * g.x + (double)g.y * g.y + (double)g.z * g.z;
         if (g2) g2[i] = v[0];
     }
     if (part) block_reduce_store<1>(v, part);
+}
+
+// Both residuals of a temporally warm-started start on split positions: at
+// the extrapolation u = x with the forces cf (g written to r when
+// r != nullptr, active set to cmask) and at s itself (u = 0) with the forces
+// cf2.  part[2b] holds block b's partial of |g(s)|^2, part[2b + 1] that of
+// |g(s + u)|^2.
+__global__ void __launch_bounds__(256) k_resid_dual(int V, const float4* __restrict__ x,
+                                                    const float4* __restrict__ s, const float4* __restrict__ vc,
+                                                    const int* __restrict__ rp, const float3* __restrict__ cf,
+                                                    const float3* __restrict__ cf2,
+                                                    const DevColliders* __restrict__ col, int flags,
+                                                    float4* __restrict__ r, double* __restrict__ part,
+                                                    float* __restrict__ cmask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (i < V) {
+        const float4 p = x[i], st = s[i], c = vc[i];
+        bool contact;
+        const float3 g0 = vertex_residual(i, make_float4(0.f, 0.f, 0.f, 0.f), st, c, rp, cf2, col, flags, 1, contact);
+        v[0] = (double)g0.x * g0.x + (double)g0.y * g0.y + (double)g0.z * g0.z;
+        const float3 g = vertex_residual(i, p, st, c, rp, cf, col, flags, 1, contact);
+        const float am = contact ? 0.f : 1.f;
+        if (cmask) cmask[i] = am;
+        if (r) r[i] = cmask ? make_float4(g.x * am, g.y * am, g.z * am, 0.f) : make_float4(g.x, g.y, g.z, 0.f);
+        v[1] = (double)g.x * g.x + (double)g.y * g.y + (double)g.z * g.z;
+    }
+    block_reduce_store<2>(v, part);
 }
 
 // Per-body residual norms of several bodies (each a contiguous vertex range
@@ -857,14 +960,20 @@ __global__ void __launch_bounds__(256) k_warm_temporal(int V, float4* __restrict
 }
 
 // Judge of a rejected warm-start stage: x returns to s (u = 0) or to the
-// temporal extrapolation.
+// temporal extrapolation.  cf2 != nullptr holds the corner forces at s (the
+// dual start): on a return to s each vertex copies its CSR range of them
+// into cf, so the forces match x again without another local step.
 __global__ void __launch_bounds__(256) k_revert(int V, float4* __restrict__ x, float4* __restrict__ xprev,
                                                 const float4* __restrict__ U1, const float4* __restrict__ U2,
-                                                const float4* __restrict__ vc, const VState* st) {
+                                                const float4* __restrict__ vc, const VState* st,
+                                                const int* __restrict__ rp, const float3* __restrict__ cf2,
+                                                float3* __restrict__ cf) {
     const int rv = st->revert;
     if (!rv) return;
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= V) return;
+    if (cf2 && rv == 1)
+        for (int e = rp[i]; e < rp[i + 1]; e++) cf[e] = cf2[e];
     float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
     if (rv == 2) {
         const float4 a = U1[i], b = U2[i];
@@ -1365,6 +1474,7 @@ struct DevLevel {
     std::vector<float> adiag;          // its diagonal
     float4 *U1 = nullptr, *U2 = nullptr;  // corrections u = x - s of the last two steps
     float* cmask = nullptr;               // 0 where a collider acts (active set) at the last residual
+    float3* cf2 = nullptr;                // level 0: corner forces at s (temporally warm-started start)
     int cur = 0;                       // which of e / ep holds the correction
     int* dense = nullptr;
     double *W = nullptr, *Wt = nullptr;  // coarsest block factors W_b = C_b^-1 and transposes
@@ -1626,6 +1736,28 @@ int tet_mode_of(int if_mode) {
     return if_mode == MGSB_IF_AC ? kTetModeAc : (if_mode == MGSB_IF_STABLE ? kTetModeStable : kTetModePolar);
 }
 
+// Local step at the temporal extrapolation (forces to cf) and at s (cf2) in
+// one launch of 2T lanes.
+int launch_tet_dual(mgsb_solver* s, DevLevel& d) {
+    const int bs = block_tet(s);
+    const int nb = nblk(2 * d.T, bs);
+    switch (tet_mode_of(s->p.if_mode)) {
+        case kTetModePolar:
+            k_tet_dual<kTetModePolar><<<nb, bs, 0, s->stream>>>(d.T, d.x, d.s, d.tet, d.B, d.vol, d.slot, d.cf, d.cf2,
+                                                                s->p.stiffness, d.tk);
+            break;
+        case kTetModeAc:
+            k_tet_dual<kTetModeAc><<<nb, bs, 0, s->stream>>>(d.T, d.x, d.s, d.tet, d.B, d.vol, d.slot, d.cf, d.cf2,
+                                                             s->p.stiffness, d.tk);
+            break;
+        default:
+            k_tet_dual<kTetModeStable><<<nb, bs, 0, s->stream>>>(d.T, d.x, d.s, d.tet, d.B, d.vol, d.slot, d.cf,
+                                                                 d.cf2, s->p.stiffness, d.tk);
+    }
+    HIPCHK(hipGetLastError());
+    return MGSB_OK;
+}
+
 int launch_vert(mgsb_solver* s, DevLevel& d, float omega, int flags, const int* skip = nullptr, int split = 0) {
     const int bs = block_vert(s);
     k_vert<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.x, d.xprev, d.s, d.vc, d.rp, d.cf, d.trp, d.tcf, s->dcol, omega,
@@ -1634,11 +1766,12 @@ int launch_vert(mgsb_solver* s, DevLevel& d, float omega, int flags, const int* 
     return MGSB_OK;
 }
 
-int launch_st(mgsb_solver* s, DevLevel& d, int advance, int split = 0) {
+// warm: split positions start from the temporal extrapolation (U1, U2)
+int launch_st(mgsb_solver* s, DevLevel& d, int advance, int split = 0, bool warm = false) {
     const int bs = block_vert(s);
     k_st<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.x, d.v, d.xlast, d.s, d.xprev, d.vc, s->p.dt, s->p.damping,
                                                s->p.gravity[0], s->p.gravity[1], s->p.gravity[2], s->dcol, advance,
-                                               split);
+                                               split, warm ? d.U1 : nullptr, warm ? d.U2 : nullptr);
     HIPCHK(hipGetLastError());
     return MGSB_OK;
 }
@@ -1817,16 +1950,32 @@ int enqueue_vc_start(mgsb_solver* s, bool learned) {
     const int bs = block_vert(s);
     const bool features = learned || s->recording;
     const bool temporal = temporal_warm(s);
+    // temporal start: g(s) and g(s + u_t) from one local-step launch, one
+    // residual launch and one check (per-body checks keep the sequential
+    // form, which reduces each residual per body)
+    const bool dual = temporal && !s->body_checks;
     float* recX = s->recording ? s->recX + (size_t)s->rec_steps * c1.V * kFeat : nullptr;
     int rc;
-    if ((rc = launch_st(s, f, 1, 1))) return rc;
-    if ((rc = fine_residual(s, true, features && !temporal, kChkStart, true, nullptr, true))) return rc;
-    if (temporal) {
-        k_warm_temporal<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, skip);
+    if ((rc = launch_st(s, f, 1, 1, dual))) return rc;
+    if (dual) {
+        if ((rc = launch_tet_dual(s, f))) return rc;
+        k_resid_dual<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.s, f.vc, f.rp, f.cf, f.cf2, s->dcol, kVertPoint,
+                                                 features ? f.r : nullptr, s->vpart, f.cmask);
         HIPCHK(hipGetLastError());
-        if ((rc = fine_residual(s, true, features, kChkTemporal, true, skip))) return rc;
-        k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate);
+        k_check2<<<1, 256, 0, s->stream>>>(nbv, s->vpart, s->vstate, s->hmirror);
         HIPCHK(hipGetLastError());
+        k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, f.rp, f.cf2, f.cf);
+        HIPCHK(hipGetLastError());
+    } else {
+        if ((rc = fine_residual(s, true, features && !temporal, kChkStart, true, nullptr, true))) return rc;
+        if (temporal) {
+            k_warm_temporal<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, skip);
+            HIPCHK(hipGetLastError());
+            if ((rc = fine_residual(s, true, features, kChkTemporal, true, skip))) return rc;
+            k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, f.rp, nullptr,
+                                                 f.cf);
+            HIPCHK(hipGetLastError());
+        }
     }
     if (features) {  // restricted residual at the warm start, features (recorded even if converged)
         k_restrict<<<nblk(c1.V * 64, 256), 256, 0, s->stream>>>(c1.V, c1.rtp, c1.rti, c1.rtw, f.r, c1.vcl, c1.r,
@@ -1842,11 +1991,14 @@ int enqueue_vc_start(mgsb_solver* s, bool learned) {
                                                            f.vc, skip, f.cmask);
         HIPCHK(hipGetLastError());
         if ((rc = fine_residual(s, true, false, kChkWarm, true, skip))) return rc;
-        k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate);
+        k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, f.rp,
+                                             dual ? f.cf2 : nullptr, f.cf);
         HIPCHK(hipGetLastError());
     }
     // after a revert the corner forces belong to the rejected x: recompute
-    if (temporal && (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, &s->vstate->cf_ok, true)))
+    // (the dual start's k_revert restores the forces at s by itself)
+    if (temporal && (learned || !dual) &&
+        (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, &s->vstate->cf_ok, true)))
         return rc;
     return MGSB_OK;
 }
@@ -2121,6 +2273,7 @@ int alloc_vcycle(mgsb_solver* s) {
             if ((rc = up(s, d.owned, &d.U2, z.data(), d.V))) return rc;
             std::vector<float> one(d.V, 1.f);
             if ((rc = up(s, d.owned, &d.cmask, one.data(), d.V))) return rc;
+            if ((rc = up<float3>(s, d.owned, &d.cf2, nullptr, (size_t)d.T * 4))) return rc;
             continue;
         }
         if ((rc = up<float4>(s, d.owned, &d.e, nullptr, d.V))) return rc;
@@ -2152,7 +2305,7 @@ int alloc_vcycle(mgsb_solver* s) {
     }
     if ((rc = dalloc(&s->vstate, 1))) return rc;
     s->owned.push_back(s->vstate);
-    if ((rc = dalloc(&s->vpart, (size_t)nblk(s->L[0].V, 256)))) return rc;
+    if ((rc = dalloc(&s->vpart, 2 * (size_t)nblk(s->L[0].V, 256)))) return rc;  // two norms (dual start)
     s->owned.push_back(s->vpart);
     HIPCHK(hipHostMalloc((void**)&s->hmirror, 8 * sizeof(int), hipHostMallocDefault));
     std::memset(s->hmirror, 0, 8 * sizeof(int));
