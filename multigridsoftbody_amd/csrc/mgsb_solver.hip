@@ -57,7 +57,9 @@ constexpr int kVertWriteCol = 4;  // write collision flags / depth / energy
 // Sum of the xyz of cf[e0..e1) in index order.  Eight loads are issued
 // before the adds that consume them, so a vertex waits on memory about once
 // per eight entries instead of once per entry; the summation order (and the
-// result, bit for bit) is that of the plain loop.
+// result, bit for bit) is that of the plain loop.  (Measured on the medium
+// box: issuing a whole 24-entry row at once, or staging a block's rows in
+// LDS with coalesced loads, made the vertex kernels slower.)
 __device__ __forceinline__ void gather_sum(const float3* __restrict__ cf, int e0, int e1, float& fx, float& fy,
                                            float& fz) {
     int e = e0;
@@ -224,11 +226,23 @@ __device__ __forceinline__ void tet_local_step(int g, int T, const float4* __res
                                                const float* __restrict__ vol, const int4* __restrict__ slot,
                                                float3* __restrict__ cf, float k, const float4* __restrict__ sref,
                                                const float* __restrict__ tk) {
+    // every load is issued up front: the per-tet streams and the index in the
+    // first round trip, the corner gathers of x (and s) in the second
     const int4 id = tet[g];
-    const float4 p0 = x[id.x], p1 = x[id.y], p2 = x[id.z], p3 = x[id.w];
     float b[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) b[q] = B[(size_t)q * T + g];
+    const float vt = vol[g];
+    const float kk = tk ? tk[g] : k;  // per-body stiffness when several bodies have their own E
+    const int4 sl = slot[g];
+    const float4 p0 = x[id.x], p1 = x[id.y], p2 = x[id.z], p3 = x[id.w];
+    float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0, s2 = s0, s3 = s0;
+    if (sref) {
+        s0 = sref[id.x];
+        s1 = sref[id.y];
+        s2 = sref[id.z];
+        s3 = sref[id.w];
+    }
     float D[9];
     D[0] = p1.x - p0.x; D[1] = p2.x - p0.x; D[2] = p3.x - p0.x;
     D[3] = p1.y - p0.y; D[4] = p2.y - p0.y; D[5] = p3.y - p0.y;
@@ -243,7 +257,6 @@ __device__ __forceinline__ void tet_local_step(int g, int T, const float4* __res
     // for any other position field (so it is the same value in every pass of
     // the step) from s gathered at the corners.
     if (sref) {
-        const float4 s0 = sref[id.x], s1 = sref[id.y], s2 = sref[id.z], s3 = sref[id.w];
         float E[9];
         E[0] = s1.x - s0.x; E[1] = s2.x - s0.x; E[2] = s3.x - s0.x;
         E[3] = s1.y - s0.y; E[4] = s2.y - s0.y; E[5] = s3.y - s0.y;
@@ -272,9 +285,6 @@ __device__ __forceinline__ void tet_local_step(int g, int T, const float4* __res
         AC[r * 4 + 2] = b[3 + r];
         AC[r * 4 + 3] = b[6 + r];
     }
-    const float vt = vol[g];
-    const float kk = tk ? tk[g] : k;  // per-body stiffness when several bodies have their own E
-    const int4 sl = slot[g];
     const int sls[4] = {sl.x, sl.y, sl.z, sl.w};
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -369,10 +379,13 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
     if (skip && *skip) return;
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= V) return;
-    float fx = 0.f, fy = 0.f, fz = 0.f;
-    gather_sum(cf, rp[i], rp[i + 1], fx, fy, fz);
+    // per-vertex streams in the same round trip as the row bounds
     const float4 p = x[i];  // x, or the correction u when split
     const float4 st = s[i];
+    const float4 c = vc[i];  // (m/h^2, diag, fix, m)
+    const float4 pp = xprev[i];
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    gather_sum(cf, rp[i], rp[i + 1], fx, fy, fz);
     // position seen by the colliders, and s - x of the inertial term
     const float4 pa = split ? make_float4(st.x + p.x, st.y + p.y, st.z + p.z, 0.f) : p;
     const float sx = split ? -p.x : st.x - p.x, sy = split ? -p.y : st.y - p.y, sz = split ? -p.z : st.z - p.z;
@@ -455,9 +468,7 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
         cdepth[i] = depth;
         cenergy[i] = energy;
     }
-    const float4 c = vc[i];  // (m/h^2, diag, fix, m)
     const float dc = c.x, dg = c.y, fix = c.z;
-    const float4 pp = xprev[i];
     float nx = (dc * sx + (fx + cfx)) / (dg + cdx + dc) * fix + p.x;
     float ny = (dc * sy + (fy + cfy)) / (dg + cdy + dc) * fix + p.y;
     float nz = (dc * sz + (fz + cfz)) / (dg + cdz + dc) * fix + p.z;
@@ -776,16 +787,13 @@ __global__ void __launch_bounds__(256) k_check2(int nb, const double* __restrict
     }
 }
 
-// Nonlinear residual g = m/h^2 (s - x) + f_el + f_col of vertex i on free
-// DOFs from its gathered corner forces cf: p is x (or the correction u when
+// Nonlinear residual g = m/h^2 (s - x) + f_el + f_col of a vertex on free
+// DOFs from its summed corner forces f: p is x (or the correction u when
 // split), st = s, c = (m/h^2, diag, fix, m).  contact: a collider acts on the
 // vertex (the active set).
-__device__ __forceinline__ float3 vertex_residual(int i, float4 p, float4 st, float4 c, const int* __restrict__ rp,
-                                                  const float3* __restrict__ cf,
+__device__ __forceinline__ float3 vertex_residual(float4 p, float4 st, float4 c, float fx, float fy, float fz,
                                                   const DevColliders* __restrict__ col, int flags, int split,
                                                   bool& contact) {
-    float fx = 0.f, fy = 0.f, fz = 0.f;
-    gather_sum(cf, rp[i], rp[i + 1], fx, fy, fz);
     const float4 pa = split ? make_float4(st.x + p.x, st.y + p.y, st.z + p.z, 0.f) : p;
     const float sx = split ? -p.x : st.x - p.x, sy = split ? -p.y : st.y - p.y, sz = split ? -p.z : st.z - p.z;
     float cfx = 0.f, cfy = 0.f, cfz = 0.f;
@@ -850,8 +858,10 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
     double v[1] = {0.0};
     if (i < V) {
         const float4 p = x[i];
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        gather_sum(cf, rp[i], rp[i + 1], fx, fy, fz);
         bool contact;
-        const float3 g = vertex_residual(i, p, s[i], vc[i], rp, cf, col, flags, split, contact);
+        const float3 g = vertex_residual(p, s[i], vc[i], fx, fy, fz, col, flags, split, contact);
         // the active set is left out of the restriction: the coarse operators
         // carry no penalty stiffness (cmask also masks the coarse corrections)
         const float am = contact ? 0.f : 1.f;
@@ -880,10 +890,14 @@ __global__ void __launch_bounds__(256) k_resid_dual(int V, const float4* __restr
     double v[2] = {0.0, 0.0};
     if (i < V) {
         const float4 p = x[i], st = s[i], c = vc[i];
+        const int r0 = rp[i], r1 = rp[i + 1];
+        float ax = 0.f, ay = 0.f, az = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
+        gather_sum(cf2, r0, r1, ax, ay, az);  // forces at s
+        gather_sum(cf, r0, r1, bx, by, bz);   // forces at the extrapolation
         bool contact;
-        const float3 g0 = vertex_residual(i, make_float4(0.f, 0.f, 0.f, 0.f), st, c, rp, cf2, col, flags, 1, contact);
+        const float3 g0 = vertex_residual(make_float4(0.f, 0.f, 0.f, 0.f), st, c, ax, ay, az, col, flags, 1, contact);
         v[0] = (double)g0.x * g0.x + (double)g0.y * g0.y + (double)g0.z * g0.z;
-        const float3 g = vertex_residual(i, p, st, c, rp, cf, col, flags, 1, contact);
+        const float3 g = vertex_residual(p, st, c, bx, by, bz, col, flags, 1, contact);
         const float am = contact ? 0.f : 1.f;
         if (cmask) cmask[i] = am;
         if (r) r[i] = cmask ? make_float4(g.x * am, g.y * am, g.z * am, 0.f) : make_float4(g.x, g.y, g.z, 0.f);
