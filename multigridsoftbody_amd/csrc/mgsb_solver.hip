@@ -739,14 +739,30 @@ __device__ void vstate_step_done(VState* st) {
     st->nfallback += st->fallback != 0;
 }
 
+// Every check kernel, skipped or not, counts itself in seqc and mirrors the
+// count to hm[kHmSeq] after a system-scope fence that makes its other mirror
+// writes visible first: the host waits for the count of the checks it has
+// enqueued instead of for the whole stream.
+constexpr int kHmSeq = 5;
+__device__ __forceinline__ void bump_seq(unsigned* seqc, int* hm) {
+    const unsigned n = atomicAdd(seqc, 1u) + 1u;
+    if (hm) {
+        __threadfence_system();
+        __hip_atomic_store(hm + kHmSeq, (int)n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 // Second stage of the residual-norm reduction: one workgroup sums the nb
 // block partials in block order (the same order on every run) and applies
 // the check.  A separate launch: completing the norm inside the producing
 // kernel (the last block to arrive, found with a device-scope counter) costs
 // far more on this GPU, since the blocks' atomics on one counter serialise.
 __global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict__ part, VState* st, int mode,
-                                               int* hm, const int* __restrict__ skip) {
-    if (skip && *skip) return;
+                                               int* hm, const int* __restrict__ skip, unsigned* seqc) {
+    if (skip && *skip) {
+        if (threadIdx.x == 0) bump_seq(seqc, hm);
+        return;
+    }
     __shared__ double sh[256];
     double a = 0.0;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) a += part[b];
@@ -756,6 +772,7 @@ __global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict_
         double t = 0.0;
         for (int q = 0; q < (int)blockDim.x; q++) t += sh[q];
         vstate_apply(st, t, mode, hm);
+        bump_seq(seqc, hm);
     }
 }
 
@@ -763,7 +780,8 @@ __global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict_
 // each summed in block order like k_check): res0 = |g(s)|, then the judge of
 // the extrapolation.  Converged already at s: x (which holds the
 // extrapolation) must return to s, so revert is set without a verdict.
-__global__ void __launch_bounds__(256) k_check2(int nb, const double* __restrict__ part, VState* st, int* hm) {
+__global__ void __launch_bounds__(256) k_check2(int nb, const double* __restrict__ part, VState* st, int* hm,
+                                                unsigned* seqc) {
     __shared__ double sh[2][256];
     double a = 0.0, b = 0.0;
     for (int q = threadIdx.x; q < nb; q += blockDim.x) {
@@ -784,6 +802,7 @@ __global__ void __launch_bounds__(256) k_check2(int nb, const double* __restrict
             vstate_apply(st, t1, kChkTemporal, hm);
         else
             st->revert = 1;
+        bump_seq(seqc, hm);
     }
 }
 
@@ -915,8 +934,11 @@ __global__ void __launch_bounds__(256) k_resid_dual(int V, const float4* __restr
 __global__ void __launch_bounds__(256) k_body_check(const int* __restrict__ vb, const double* __restrict__ g2,
                                                     double* __restrict__ bodysq, double* __restrict__ res0b,
                                                     unsigned* count, VState* st, int mode, int* hm,
-                                                    const int* __restrict__ skip) {
-    if (skip && *skip) return;
+                                                    const int* __restrict__ skip, unsigned* seqc) {
+    if (skip && *skip) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) bump_seq(seqc, hm);
+        return;
+    }
     __shared__ double sh[256];
     __shared__ int s_last;
     const int b = blockIdx.x;
@@ -951,6 +973,7 @@ __global__ void __launch_bounds__(256) k_body_check(const int* __restrict__ vb, 
     }
     *count = 0u;
     vstate_apply(st, tot, mode, hm, worst);
+    bump_seq(seqc, hm);
 }
 
 // ---------------------------------------------------------------------------
@@ -1534,7 +1557,10 @@ struct mgsb_solver {
     mgsb_step_stats last{};
     VState* vstate = nullptr;
     double* vpart = nullptr;  // partials of the level-0 residual norm
-    int* hmirror = nullptr;      // host-mapped (converged, cycles, halves, fallback, accepted)
+    int* hmirror = nullptr;      // host-mapped (converged, cycles, halves, fallback, accepted, check count)
+    unsigned* seqc = nullptr;    // device count of check kernels run
+    unsigned chk_launched = 0;   // check kernels enqueued (graph replays included)
+    unsigned pchecks[5] = {};    // check kernels in each captured piece
     // V-cycle steps run as pieces (start, pre-smoothing half, coarse + post
     // half, finish), each captured once; the host launches only the halves a
     // step needs, reading the convergence mirror after a guessed count.
@@ -1927,7 +1953,8 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool res
                                            s->g2, d.cmask);
         HIPCHK(hipGetLastError());
         k_body_check<<<s->nbodies, 256, 0, s->stream>>>(s->vb, s->g2, s->bodysq, s->res0b, s->bcount, s->vstate, chk,
-                                                        s->hmirror, skip);
+                                                        s->hmirror, skip, s->seqc);
+        s->chk_launched++;
         HIPCHK(hipGetLastError());
         return MGSB_OK;
     }
@@ -1936,7 +1963,8 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool res
                                        nullptr, d.cmask);
     HIPCHK(hipGetLastError());
     if (chk != kChkNone) {
-        k_check<<<1, 256, 0, s->stream>>>(nb, s->vpart, s->vstate, chk, s->hmirror, skip);
+        k_check<<<1, 256, 0, s->stream>>>(nb, s->vpart, s->vstate, chk, s->hmirror, skip, s->seqc);
+        s->chk_launched++;
         HIPCHK(hipGetLastError());
     }
     return MGSB_OK;
@@ -1976,7 +2004,8 @@ int enqueue_vc_start(mgsb_solver* s, bool learned) {
         k_resid_dual<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.s, f.vc, f.rp, f.cf, f.cf2, s->dcol, kVertPoint,
                                                  features ? f.r : nullptr, s->vpart, f.cmask);
         HIPCHK(hipGetLastError());
-        k_check2<<<1, 256, 0, s->stream>>>(nbv, s->vpart, s->vstate, s->hmirror);
+        k_check2<<<1, 256, 0, s->stream>>>(nbv, s->vpart, s->vstate, s->hmirror, s->seqc);
+        s->chk_launched++;
         HIPCHK(hipGetLastError());
         k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, f.rp, f.cf2, f.cf);
         HIPCHK(hipGetLastError());
@@ -2095,7 +2124,10 @@ int run_piece(mgsb_solver* s, int piece, bool graphable) {
     if (!graphable) return enqueue_piece(s, piece);
     if (!s->pvalid[piece]) {
         HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        const unsigned before = s->chk_launched;
         const int rc = enqueue_piece(s, piece);
+        s->pchecks[piece] = s->chk_launched - before;  // captured, not run yet
+        s->chk_launched = before;
         hipGraph_t g = nullptr;
         const hipError_t ce = hipStreamEndCapture(s->stream, &g);
         if (rc) {
@@ -2108,7 +2140,23 @@ int run_piece(mgsb_solver* s, int piece, bool graphable) {
         s->pvalid[piece] = true;
     }
     HIPCHK(hipGraphLaunch(s->pexec[piece], s->stream));
+    s->chk_launched += s->pchecks[piece];
     return MGSB_OK;
+}
+
+// Waits until every check kernel enqueued so far has run (the host mirror's
+// count reaches chk_launched), spinning on the coherent mirror; the stream
+// itself may still be busy.  Falls back to the stream's completion.
+int wait_checks(mgsb_solver* s) {
+    const int want = (int)s->chk_launched;
+    for (long it = 1;; it++) {
+        if (__atomic_load_n(s->hmirror + kHmSeq, __ATOMIC_ACQUIRE) == want) return MGSB_OK;
+        if ((it & 4095) == 0) {
+            const hipError_t q = hipStreamQuery(s->stream);
+            if (q == hipSuccess) return MGSB_OK;  // all work done: the mirror is final
+            if (q != hipErrorNotReady) HIPCHK(q);
+        }
+    }
 }
 
 // One V-cycle step.  Without checks all max_cycles cycles are enqueued.  With
@@ -2135,7 +2183,7 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
             target = h + 1;
             continue;
         }
-        HIPCHK(hipStreamSynchronize(s->stream));
+        if ((rc = wait_checks(s))) return rc;
         if (s->hmirror[0] || h >= maxh) {
             s->halves_guess = s->hmirror[0] ? s->hmirror[2] : maxh;
             break;
@@ -2321,7 +2369,11 @@ int alloc_vcycle(mgsb_solver* s) {
     s->owned.push_back(s->vstate);
     if ((rc = dalloc(&s->vpart, 2 * (size_t)nblk(s->L[0].V, 256)))) return rc;  // two norms (dual start)
     s->owned.push_back(s->vpart);
-    HIPCHK(hipHostMalloc((void**)&s->hmirror, 8 * sizeof(int), hipHostMallocDefault));
+    if ((rc = dalloc(&s->seqc, 1))) return rc;
+    s->owned.push_back(s->seqc);
+    HIPCHK(hipMemsetAsync(s->seqc, 0, sizeof(unsigned), s->stream));
+    s->chk_launched = 0;
+    HIPCHK(hipHostMalloc((void**)&s->hmirror, 8 * sizeof(int), hipHostMallocCoherent));
     std::memset(s->hmirror, 0, 8 * sizeof(int));
     return push_vstate(s);
 }
