@@ -146,7 +146,9 @@ __device__ void vstate_step_done(VState* st);
 __global__ void __launch_bounds__(256) k_finish_split(int V, float4* __restrict__ x, float4* __restrict__ xprev,
                                                       const float4* __restrict__ s, const float4* __restrict__ xlast,
                                                       float4* __restrict__ v, float dt, VState* vs,
-                                                      float4* __restrict__ U1, float4* __restrict__ U2) {
+                                                      float4* __restrict__ U1, float4* __restrict__ U2,
+                                                      const int* __restrict__ skip) {
+    if (skip && *skip) return;
     if (vs && blockIdx.x == 0 && threadIdx.x == 0) vstate_step_done(vs);
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= V) return;
@@ -656,6 +658,9 @@ struct VState {
     int cf_ok;      // corner forces are those of the current x (or the step has converged)
     double rel;     // convergence measure at the last check: res / res0, or with several
                     // bodies the largest per-body res_b / res0_b
+    int maxh;       // half-cycle cap of a step (2 max_cycles)
+    int notdone;    // the step has neither converged nor reached the cap (no checks: 0);
+                    // the finish piece, enqueued before the host reads the verdict, skips
 };
 
 // Check modes of the fused residual reduction.
@@ -719,6 +724,7 @@ __device__ void vstate_apply(VState* st, double t, int mode, int* hm, double rel
         }
         if (st->check && ok(r)) st->converged = 1;
     }
+    st->notdone = (st->check == 0 || st->converged || (st->maxh > 0 && st->halves >= st->maxh)) ? 0 : 1;
     if (hm) {
         hm[0] = st->converged;
         hm[1] = st->cycles;
@@ -1069,7 +1075,9 @@ __global__ void __launch_bounds__(256) k_apply_pred(int Vc, const float* __restr
 __global__ void __launch_bounds__(256) k_target(int Vc, const float4* __restrict__ x, const float4* __restrict__ U1,
                                                 const float4* __restrict__ U2, const float4* __restrict__ vc,
                                                 const int4* __restrict__ aids, const float4* __restrict__ aw,
-                                                float4* __restrict__ eprev, float* __restrict__ rec) {
+                                                float4* __restrict__ eprev, float* __restrict__ rec,
+                                                const int* __restrict__ skip) {
+    if (skip && *skip) return;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= Vc) return;
     const int4 id = aids[j];
@@ -1710,6 +1718,7 @@ int push_vstate(mgsb_solver* s) {
     VState v{};
     v.check = s->p.check_every;
     v.tol = s->p.tol;
+    v.maxh = 2 * s->p.max_cycles;
     HIPCHK(hipMemcpyAsync(s->vstate, &v, sizeof(VState), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     return MGSB_OK;
@@ -2099,12 +2108,12 @@ int enqueue_vc_finish(mgsb_solver* s, bool learned) {
     if (features || temporal_warm(s)) {  // coarse delta: next step's feature / training target
         const bool t = temporal_warm(s);
         k_target<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, f.x, t ? f.U1 : nullptr, t ? f.U2 : nullptr, f.vc,
-                                                       s->M[0].aids, s->M[0].aw, s->eprev, recT);
+                                                       s->M[0].aids, s->M[0].aw, s->eprev, recT,
+                                                       &s->vstate->notdone);
         HIPCHK(hipGetLastError());
     }
-    if (s->recording) s->rec_steps++;
     k_finish_split<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, f.s, f.xlast, f.v, s->p.dt, s->vstate,
-                                                        f.U1, f.U2);
+                                                        f.U1, f.U2, &s->vstate->notdone);
     HIPCHK(hipGetLastError());
     return MGSB_OK;
 }
@@ -2161,10 +2170,12 @@ int wait_checks(mgsb_solver* s) {
 
 // One V-cycle step.  Without checks all max_cycles cycles are enqueued.  With
 // checks the host enqueues the number of half-cycles the previous step
-// needed, then reads the convergence mirror (one stream synchronisation) and
-// adds halves one check point at a time until the step has converged or
-// max_cycles cycles have run, so no launches are spent on a converged step
-// beyond a wrong guess.
+// needed and the finish piece behind them, then waits for the checks' verdict
+// on the host mirror and adds halves (and another finish) one check point at
+// a time until the step has converged or max_cycles cycles have run.  The
+// finish kernels run only once the device has marked the step done, so the
+// finish overlaps the host's wait, and no launches are spent on a converged
+// step beyond a wrong guess.
 int step_vcycle(mgsb_solver* s, bool graphable) {
     int rc;
     const bool learned = s->p.schedule == MGSB_SCHED_LEARNED;
@@ -2175,6 +2186,7 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
     const int maxh = 2 * s->p.max_cycles;
     int h = 0;
     int target = check ? std::min(maxh, std::max(0, s->halves_guess)) : maxh;
+    bool finished = false;
     for (;;) {
         for (; h < target; h++)
             if ((rc = run_piece(s, (h & 1) ? kPiecePost : kPiecePre, graphable))) return rc;
@@ -2183,9 +2195,11 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
             target = h + 1;
             continue;
         }
+        if ((rc = run_piece(s, kPieceFinish, graphable))) return rc;  // runs only if the step is done
         if ((rc = wait_checks(s))) return rc;
         if (s->hmirror[0] || h >= maxh) {
             s->halves_guess = s->hmirror[0] ? s->hmirror[2] : maxh;
+            finished = true;
             break;
         }
         target = h + (check == 2 ? 1 : 2);
@@ -2202,7 +2216,9 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
             s->gate_skip = 0;
         }
     }
-    return run_piece(s, kPieceFinish, graphable);
+    if (!finished && (rc = run_piece(s, kPieceFinish, graphable))) return rc;
+    if (s->recording) s->rec_steps++;  // the finish enqueued above recorded this step's target
+    return MGSB_OK;
 }
 
 int enqueue_step(mgsb_solver* s) {
