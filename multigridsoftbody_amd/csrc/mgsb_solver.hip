@@ -1543,6 +1543,12 @@ struct LevelMap {  // between level l (fine) and l+1 (coarse)
 
 }  // namespace
 
+// Pieces of a V-cycle step (captured once each): the start (with or without
+// the learned stage), the two half-cycles, and the finish (with or without the
+// coarse delta that only the learned stage reads as a feature).
+enum { kPieceStart = 0, kPiecePre = 1, kPiecePost = 2, kPieceFinish = 3, kPieceStartGated = 4, kPieceFinishLite = 5,
+       kPieces = 6 };
+
 struct mgsb_solver {
     mgsb_params p{};
     int device = 0;
@@ -1568,13 +1574,13 @@ struct mgsb_solver {
     int* hmirror = nullptr;      // host-mapped (converged, cycles, halves, fallback, accepted, check count)
     unsigned* seqc = nullptr;    // device count of check kernels run
     unsigned chk_launched = 0;   // check kernels enqueued (graph replays included)
-    unsigned pchecks[5] = {};    // check kernels in each captured piece
+    unsigned pchecks[kPieces] = {};  // check kernels in each captured piece
     // V-cycle steps run as pieces (start, pre-smoothing half, coarse + post
     // half, finish), each captured once; the host launches only the halves a
     // step needs, reading the convergence mirror after a guessed count.
-    hipGraph_t pgraph[5] = {};
-    hipGraphExec_t pexec[5] = {};
-    bool pvalid[5] = {};
+    hipGraph_t pgraph[kPieces] = {};
+    hipGraphExec_t pexec[kPieces] = {};
+    bool pvalid[kPieces] = {};
     int halves_guess = 4;
     // learned schedule: after kGateAfter consecutive steps whose learned
     // correction the judge rejected, steps start without the MLP stage for
@@ -1605,7 +1611,7 @@ struct mgsb_solver {
         if (stream) (void)hipStreamSynchronize(stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (int q = 0; q < 5; q++) {
+        for (int q = 0; q < kPieces; q++) {
             if (pexec[q]) (void)hipGraphExecDestroy(pexec[q]);
             if (pgraph[q]) (void)hipGraphDestroy(pgraph[q]);
         }
@@ -1982,7 +1988,6 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool res
 // A residual-correction V-cycle step (include/mgsb.h, MGSB_SCHED_VCYCLE) is
 // enqueued as four pieces.  Every kernel after the start returns at once when
 // the step's convergence flag is set.
-enum { kPieceStart = 0, kPiecePre = 1, kPiecePost = 2, kPieceFinish = 3, kPieceStartGated = 4 };
 constexpr int kGateAfter = 8, kGateFirst = 8, kGateMax = 256;
 
 bool temporal_warm(const mgsb_solver* s) {
@@ -2097,15 +2102,16 @@ int enqueue_vc_post(mgsb_solver* s) {
     return fine_residual(s, true, false, kChkCycle, true, skip);
 }
 
-// Coarse correction achieved (features of the next step / training target),
-// x = s + u, velocities, run aggregates.
-int enqueue_vc_finish(mgsb_solver* s, bool learned) {
+// Coarse correction achieved (features of the next step / training target;
+// left out when lite: no learned stage reads it next step), x = s + u,
+// velocities, run aggregates.
+int enqueue_vc_finish(mgsb_solver* s, bool learned, bool lite) {
     DevLevel& f = s->L[0];
     DevLevel& c1 = s->L[1];
     const int bs = block_vert(s);
     const bool features = learned || s->recording;
     float* recT = s->recording ? s->recT + (size_t)s->rec_steps * c1.V * 3 : nullptr;
-    if (features || temporal_warm(s)) {  // coarse delta: next step's feature / training target
+    if (!lite && (features || temporal_warm(s))) {  // coarse delta: next step's feature / training target
         const bool t = temporal_warm(s);
         k_target<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, f.x, t ? f.U1 : nullptr, t ? f.U2 : nullptr, f.vc,
                                                        s->M[0].aids, s->M[0].aw, s->eprev, recT,
@@ -2123,9 +2129,10 @@ int enqueue_piece(mgsb_solver* s, int piece) {
     switch (piece) {
         case kPieceStart: return enqueue_vc_start(s, learned);
         case kPieceStartGated: return enqueue_vc_start(s, false);
+        case kPieceFinishLite: return enqueue_vc_finish(s, learned, true);
         case kPiecePre: return enqueue_vc_pre(s);
         case kPiecePost: return enqueue_vc_post(s);
-        default: return enqueue_vc_finish(s, learned);
+        default: return enqueue_vc_finish(s, learned, false);
     }
 }
 
@@ -2184,6 +2191,9 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
     if ((rc = run_piece(s, gated ? kPieceStartGated : kPieceStart, graphable))) return rc;
     const int check = s->p.check_every;
     const int maxh = 2 * s->p.max_cycles;
+    // the next step reads this step's coarse delta only if it runs the learned
+    // stage: not under the plain schedules, nor while the gate stays shut
+    const int fin = !s->recording && (!learned || (gated && s->gate_left > 0)) ? kPieceFinishLite : kPieceFinish;
     int h = 0;
     int target = check ? std::min(maxh, std::max(0, s->halves_guess)) : maxh;
     bool finished = false;
@@ -2195,7 +2205,7 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
             target = h + 1;
             continue;
         }
-        if ((rc = run_piece(s, kPieceFinish, graphable))) return rc;  // runs only if the step is done
+        if ((rc = run_piece(s, fin, graphable))) return rc;  // runs only if the step is done
         if ((rc = wait_checks(s))) return rc;
         if (s->hmirror[0] || h >= maxh) {
             s->halves_guess = s->hmirror[0] ? s->hmirror[2] : maxh;
@@ -2216,7 +2226,7 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
             s->gate_skip = 0;
         }
     }
-    if (!finished && (rc = run_piece(s, kPieceFinish, graphable))) return rc;
+    if (!finished && (rc = run_piece(s, fin, graphable))) return rc;
     if (s->recording) s->rec_steps++;  // the finish enqueued above recorded this step's target
     return MGSB_OK;
 }
@@ -2516,7 +2526,7 @@ int invalidate_graph(mgsb_solver* s) {
     s->graph = nullptr;
     s->graph_valid = false;
     s->gate_rejects = s->gate_skip = s->gate_left = 0;
-    for (int q = 0; q < 5; q++) {
+    for (int q = 0; q < kPieces; q++) {
         if (s->pexec[q]) HIPCHK(hipGraphExecDestroy(s->pexec[q]));
         if (s->pgraph[q]) HIPCHK(hipGraphDestroy(s->pgraph[q]));
         s->pexec[q] = nullptr;
