@@ -12,11 +12,13 @@ by stream synchronisations.  Every timed step's final residual is folded into
 device-side run statistics and reported (max_rel, steps over tolerance).
 
 Schedules: vcycle = plain MG, each step starting from the inertial prediction
-s; temporal = the same V-cycle started from the temporal extrapolation of the
-last two steps' corrections; learned = temporal start plus the MLP's coarse
-correction (trained first, untimed, on 2,000 recorded steps of a trajectory
-with a different initial velocity).  The default "ab" times all three on the
-same workload and reports the learned schedule as the value.
+s; learned_s = the MLP's coarse correction applied to s alone (the network's
+own gain over plain MG); temporal = the V-cycle started from the temporal
+extrapolation of the last two steps' corrections; learned = temporal start
+plus the MLP's coarse correction.  Each network is trained first, untimed, on
+2,000 recorded steps (plain and temporal schedule respectively) of a
+trajectory with a different initial velocity.  The default "ab" times all
+four on the same workload and reports the learned schedule as the value.
 
 Prints ONE JSON line.  The CPU baseline is the oracle (C restatement, FP32,
 OpenMP) running the plain V-cycle on the same workload on this host, timed
@@ -65,17 +67,18 @@ def make_solver(levels, schedule, cfg, use_graph=1, fixed_cycles=0, check_every=
     from multigridsoftbody_amd import workloads as W
 
     phys = W.physics(gravity=(0.0, -9.8, 0.0) if cfg["gravity"] else (0.0, 0.0, 0.0))
-    if schedule in ("vcycle", "temporal", "learned"):
+    if schedule in ("vcycle", "temporal", "learned", "learned_s"):
         kw = dict(VCYCLE)
         if "max_cycles" in cfg:
             kw.update(max_cycles=cfg["max_cycles"])
         if fixed_cycles:
             kw.update(max_cycles=fixed_cycles, check_every=0)
-        else:
+        else:  # plain MG needs whole cycles: checks after each (the faster of 1 and 2 here)
             kw.update(check_every=check_every if schedule != "vcycle" else 1)
-        sched = M.SCHED_LEARNED if schedule == "learned" else M.SCHED_VCYCLE
-        s = M.Solver(levels, schedule=sched, if_mode=M.IF_STABLE, use_graph=use_graph,
-                     warm_start=0 if schedule == "vcycle" else 1, **kw, **phys)
+        sched = M.SCHED_LEARNED if schedule in ("learned", "learned_s") else M.SCHED_VCYCLE
+        warm = {"vcycle": 0, "temporal": 1, "learned": 1, "learned_s": 2}[schedule]
+        s = M.Solver(levels, schedule=sched, if_mode=M.IF_STABLE, use_graph=use_graph, warm_start=warm, **kw,
+                     **phys)
     elif schedule == "twolevel":
         s = M.Solver([levels[0], levels[-1]], schedule=M.SCHED_TWOLEVEL, if_mode=M.IF_PLAIN, iters=cfg["iters"],
                      iters_coarse=cfg["iters"], use_graph=use_graph, **phys)
@@ -95,10 +98,12 @@ def make_solver(levels, schedule, cfg, use_graph=1, fixed_cycles=0, check_every=
     return s
 
 
-def train_network(levels, cfg, check_every):
-    """Record TRAIN['record_steps'] steps of the temporal-start V-cycle on a
-    trajectory with another initial velocity, train the MLP on the device."""
-    s = make_solver(levels, "temporal", cfg, check_every=check_every, velocity_seed=TRAIN["velocity_seed"])
+def train_network(levels, cfg, check_every, on_s=False):
+    """Record TRAIN['record_steps'] steps of the temporal-start V-cycle (or,
+    on_s, of plain MG: targets relative to s) on a trajectory with another
+    initial velocity, train the MLP on the device."""
+    s = make_solver(levels, "vcycle" if on_s else "temporal", cfg, check_every=check_every,
+                    velocity_seed=TRAIN["velocity_seed"])
     s.learn_record(TRAIN["record_steps"])
     t0 = time.perf_counter()
     s.step(TRAIN["record_steps"])
@@ -106,8 +111,8 @@ def train_network(levels, cfg, check_every):
     t1 = time.perf_counter()
     hist = s.learn_train(epochs=TRAIN["epochs"], batch=TRAIN["batch"], lr=TRAIN["lr"], seed=TRAIN["seed"])
     t2 = time.perf_counter()
-    log(f"[bench] recorded {TRAIN['record_steps']} steps in {t1 - t0:.1f}s, trained {TRAIN['epochs']} epochs in "
-        f"{t2 - t1:.1f}s, loss {hist[0]:.4f} -> {hist[-1]:.4f} (not timed)")
+    log(f"[bench] {'plain' if on_s else 'temporal'} recording: {TRAIN['record_steps']} steps in {t1 - t0:.1f}s, "
+        f"trained {TRAIN['epochs']} epochs in {t2 - t1:.1f}s, loss {hist[0]:.4f} -> {hist[-1]:.4f} (not timed)")
     return s.learn_weights(), s.learn_norm(), [float(hist[0]), float(hist[-1])]
 
 
@@ -187,7 +192,8 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="medium", choices=["small", "medium", "large", "batched", "blob"])
-    ap.add_argument("--schedule", default="ab", choices=["ab", "vcycle", "temporal", "learned", "single", "twolevel"])
+    ap.add_argument("--schedule", default="ab",
+                    choices=["ab", "vcycle", "learned_s", "temporal", "learned", "single", "twolevel"])
     ap.add_argument("--fixed-cycles", type=int, default=0,
                     help="V-cycles per step without the convergence check (profiling)")
     ap.add_argument("--check-every", type=int, default=2, choices=[1, 2],
@@ -204,27 +210,27 @@ def main():
 
     log(f"[bench] workload={args.workload} schedule={args.schedule} steps={args.steps} warmup={args.warmup}")
     cfg, levels = build_workload(args.workload)
-    names = ["vcycle", "temporal", "learned"] if args.schedule == "ab" else [args.schedule]
+    names = ["vcycle", "learned_s", "temporal", "learned"] if args.schedule == "ab" else [args.schedule]
     if args.schedule == "ab" and args.workload in ("large", "batched"):
         names = ["vcycle", "temporal"]  # the network is trained and measured on the medium box
     head = names[-1]
-    weights = None
-    losses = None
+    nets = {}
     if "learned" in names:
-        weights = train_network(levels, cfg, args.check_every)
-        losses = weights[2]
+        nets["learned"] = train_network(levels, cfg, args.check_every)
+    if "learned_s" in names:
+        nets["learned_s"] = train_network(levels, cfg, args.check_every, on_s=True)
     results = {}
     solvers = {}
     for name in names:
         t0 = time.perf_counter()
         s = make_solver(levels, name, cfg, use_graph=0 if args.no_graph else 1, fixed_cycles=args.fixed_cycles,
                         check_every=args.check_every)
-        if name == "learned":
-            s.learn_weights(weights[0])
-            s.learn_norm(weights[1])
+        if name in nets:
+            s.learn_weights(nets[name][0])
+            s.learn_norm(nets[name][1])
         log(f"[bench] {name}: solver created in {time.perf_counter() - t0:.1f}s; tets per level "
             f"{[m.ntets for m in levels]}")
-        vc = name in ("vcycle", "temporal", "learned")
+        vc = name in ("vcycle", "temporal", "learned", "learned_s")
         tuned = None
         if args.autotune:
             s.step(args.warmup)
@@ -258,24 +264,27 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
-        cpu = cpu_baseline(levels, cfg, "vcycle" if head in ("temporal", "learned") else head)
+        cpu = cpu_baseline(levels, cfg, "vcycle" if head in ("temporal", "learned", "learned_s") else head)
         log(f"[bench] cpu baseline {cpu['value']} steps/s")
 
     config = {
         "workload": cfg["name"],
-        "levels_tets": [m.ntets for m in levels] if head in ("vcycle", "temporal", "learned") else
+        "levels_tets": [m.ntets for m in levels] if head in ("vcycle", "temporal", "learned", "learned_s") else
         [levels[0].ntets],
         "schedule": head,
-        "tol": VCYCLE["tol"] if head in ("vcycle", "temporal", "learned") else None,
+        "tol": VCYCLE["tol"] if head in ("vcycle", "temporal", "learned", "learned_s") else None,
         "graph": not args.no_graph,
         "schedules": results,
     }
-    if "vcycle" in results and "learned" in results:
-        config["learned_vs_plain_mg"] = round(results["learned"]["steps_per_s"] / results["vcycle"]["steps_per_s"], 3)
-        config["learned_vs_temporal_only"] = round(results["learned"]["steps_per_s"] /
-                                                   results["temporal"]["steps_per_s"], 3)
-    if losses:
-        config["training"] = dict(TRAIN, loss_first=round(losses[0], 4), loss_last=round(losses[1], 4))
+    sps = {k: v["steps_per_s"] for k, v in results.items()}
+    if "vcycle" in sps and "learned" in sps:
+        config["learned_vs_plain_mg"] = round(sps["learned"] / sps["vcycle"], 3)
+    if "temporal" in sps and "learned" in sps:
+        config["learned_vs_temporal_only"] = round(sps["learned"] / sps["temporal"], 3)
+    if "vcycle" in sps and "learned_s" in sps:
+        config["learned_on_s_vs_plain_mg"] = round(sps["learned_s"] / sps["vcycle"], 3)
+    if nets:
+        config["training"] = dict(TRAIN, loss={k: [round(v[2][0], 4), round(v[2][1], 4)] for k, v in nets.items()})
     out = {
         "metric": METRIC,
         "value": steps_per_s,
@@ -296,7 +305,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": TRAFFIC.get(cfg["name"]) if head in ("vcycle", "temporal", "learned") else None,
+            "traffic": TRAFFIC.get(cfg["name"]) if head in ("vcycle", "temporal", "learned", "learned_s") else None,
             "kernel": "k_tet (finest-level local step)",
             "avg_us": round(avg_us, 3),
             "timing": f"{reps} back-to-back launches between two HIP events on the solver's stream",

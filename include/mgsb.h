@@ -96,7 +96,9 @@ typedef struct mgsb_params {
     int32_t warm_start;   /* V-cycle: 0 start from s (the reference's x = s), 1 from the
                            * temporal extrapolation s + 2 u_{n-1} - u_{n-2} of the
                            * previous steps' corrections u = x - s (kept only if it
-                           * lowers the residual).  The learned schedule always uses 1. */
+                           * lowers the residual).  The learned schedule uses the
+                           * temporal stage for 0 and 1; 2 runs its MLP stage on s
+                           * alone (no temporal stage, the network's own gain).   */
     int32_t learn_gate;   /* learned schedule: after 8 consecutive steps whose learned
                            * correction was rejected, skip the MLP stage for 8, 16, ...
                            * 256 steps before trying it again (1, default) or never (0) */
@@ -243,8 +245,9 @@ int mgsb_run_stats(mgsb_solver* s, mgsb_run_stats_t* out, int32_t reset);
 
 /* Learned coarse correction (MGSB_SCHED_LEARNED; no reference counterpart).
  * The step starts from the temporal extrapolation x_t = s + 2 u_{n-1} -
- * u_{n-2} (warm_start = 1).  A 4-layer MLP (12 -> 128 -> 128 -> 128 -> 3,
- * ReLU, BF16 MFMA with FP32 accumulation, FP32 master weights) maps features
+ * u_{n-2} (warm_start 0 or 1; with warm_start = 2, x_t = s).  A 4-layer MLP
+ * (12 -> 128 -> 128 -> 128 -> 3, ReLU, BF16 MFMA with FP32 accumulation, FP32
+ * master weights) maps features
  * of each level-1 vertex (restricted residual at x_t, injected velocity, rest
  * position, previous step's coarse delta) to the coarse correction that x_t
  * still misses; x = x_t + P pred warm-starts the V-cycles.  Each stage is kept

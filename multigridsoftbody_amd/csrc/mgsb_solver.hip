@@ -1220,6 +1220,61 @@ __device__ __forceinline__ void jacobi_cheb_row(int i, const int* __restrict__ a
     ep[i] = make_float4(yx, yy, yz, 0.f);
 }
 
+// Row i of A e for a level stored as column-major ELL (w entries per row in
+// the CSR row's order, rows shorter than w padded with (i, 0)): the row's
+// loads are coalesced across lanes and e is read from LDS.  Same summation
+// order as spmv_row (a padded entry adds a zero product).
+__device__ __forceinline__ float3 spmv_row_ell(int i, int V, int w, const int* __restrict__ ajE,
+                                               const float* __restrict__ avE, const float4* e) {
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    for (int k0 = 0; k0 < w; k0 += 8) {
+        int j[8];
+        float a[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const bool on = k0 + k < w;
+            j[k] = on ? ajE[(size_t)(k0 + k) * V + i] : i;
+            a[k] = on ? avE[(size_t)(k0 + k) * V + i] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float4 x = e[j[k]];
+            sx = sx + a[k] * x.x;
+            sy = sy + a[k] * x.y;
+            sz = sz + a[k] * x.z;
+        }
+    }
+    return make_float3(sx, sy, sz);
+}
+
+// jacobi_cheb_row on an ELL level whose e / ep live in LDS.
+__device__ __forceinline__ void jacobi_cheb_row_ell(int i, int V, int w, const int* __restrict__ ajE,
+                                                    const float* __restrict__ avE, float4* e, float4* ep,
+                                                    const float4* __restrict__ r, const float4* __restrict__ vcl,
+                                                    float omega, float relax, int first_zero) {
+    const float4 c = vcl[i];
+    const float d = c.x;
+    const float fx = c.z;
+    const float4 rr = r[i];
+    if (first_zero) {
+        float yx = fx * (relax * (rr.x / d));
+        float yy = fx * (relax * (rr.y / d));
+        float yz = fx * (relax * (rr.z / d));
+        ep[i] = make_float4(omega * yx, omega * yy, omega * yz, 0.f);
+        e[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const float3 Ae = spmv_row_ell(i, V, w, ajE, avE, e);
+    const float4 ec = e[i], pc = ep[i];
+    float yx = ec.x + fx * (relax * ((rr.x - Ae.x) / d));
+    float yy = ec.y + fx * (relax * ((rr.y - Ae.y) / d));
+    float yz = ec.z + fx * (relax * ((rr.z - Ae.z) / d));
+    yx = omega * (yx - pc.x) + pc.x;
+    yy = omega * (yy - pc.y) + pc.y;
+    yz = omega * (yz - pc.z) + pc.z;
+    ep[i] = make_float4(yx, yy, yz, 0.f);
+}
+
 __global__ void __launch_bounds__(256) k_spmv_smooth(int V, const int* __restrict__ ap, const int* __restrict__ aj,
                                                      const float* __restrict__ av, float4* __restrict__ e,
                                                      float4* __restrict__ ep, const float4* __restrict__ r,
@@ -1375,8 +1430,14 @@ struct TailLevel {
     const double *W, *Wt;
     double* big;  // dense-vector scratch when nfree > kMaxDense
     float rho;
-    int pad;
+    int ellw;     // first tail level only: ELL width of ajE / avE (0: CSR, vectors in device memory)
+    const int* ajE;
+    const float* avE;
 };
+
+// The first tail level keeps its correction (and the Chebyshev partner) in
+// LDS up to this many vertices.
+constexpr int kTailLdsV = 2048;
 
 __global__ void __launch_bounds__(1024) k_coarse_tail(const TailLevel* __restrict__ tl, int n, int sweeps,
                                                       float relax, int direct, const int* __restrict__ skip) {
@@ -1384,15 +1445,33 @@ __global__ void __launch_bounds__(1024) k_coarse_tail(const TailLevel* __restric
     const int tid = threadIdx.x, nt = blockDim.x;
     unsigned flips = 0u;  // bit q: level q's correction is in e1 (else e0)
     auto flip = [&](int q) { return (flips >> q) & 1u; };
+    // level 0 with an ELL matrix: its e0 / e1 are the two LDS buffers
+    __shared__ float4 e_lds[2][kTailLdsV];
+    const bool lds0 = tl[0].ellw > 0;
+    auto ecur = [&](int q) -> float4* {
+        const TailLevel& L = tl[q];
+        if (q == 0 && lds0) return e_lds[flip(0)];
+        return flip(q) ? L.e1 : L.e0;
+    };
+    auto eprv = [&](int q) -> float4* {
+        const TailLevel& L = tl[q];
+        if (q == 0 && lds0) return e_lds[flip(0) ^ 1u];
+        return flip(q) ? L.e0 : L.e1;
+    };
     auto smooth = [&](int q, int ns, bool zero) {
         const TailLevel& L = tl[q];
+        const bool ell = q == 0 && lds0;
         float omega = 1.0f;
         for (int k = 0; k < ns; k++) {
             omega = 4 / (4 - L.rho * L.rho * omega);
-            float4* e = flip(q) ? L.e1 : L.e0;
-            float4* ep = flip(q) ? L.e0 : L.e1;
-            for (int i = tid; i < L.V; i += nt)
-                jacobi_cheb_row(i, L.ap, L.aj, L.av, e, ep, L.r, L.vcl, omega, relax, zero && k == 0);
+            float4* e = ecur(q);
+            float4* ep = eprv(q);
+            for (int i = tid; i < L.V; i += nt) {
+                if (ell)
+                    jacobi_cheb_row_ell(i, L.V, L.ellw, L.ajE, L.avE, e, ep, L.r, L.vcl, omega, relax, zero && k == 0);
+                else
+                    jacobi_cheb_row(i, L.ap, L.aj, L.av, e, ep, L.r, L.vcl, omega, relax, zero && k == 0);
+            }
             __syncthreads();
             flips ^= 1u << q;
         }
@@ -1401,9 +1480,10 @@ __global__ void __launch_bounds__(1024) k_coarse_tail(const TailLevel* __restric
     for (int q = 0; q + 1 < n; q++) {
         const TailLevel& L = tl[q];
         smooth(q, sweeps, true);
-        const float4* e = flip(q) ? L.e1 : L.e0;
+        const float4* e = ecur(q);
+        const bool ell = q == 0 && lds0;
         for (int i = tid; i < L.V; i += nt) {
-            const float3 Ae = spmv_row(i, L.ap, L.aj, L.av, e);
+            const float3 Ae = ell ? spmv_row_ell(i, L.V, L.ellw, L.ajE, L.avE, e) : spmv_row(i, L.ap, L.aj, L.av, e);
             const float4 rv = L.r[i];
             const float fx = L.vcl[i].z;
             L.rr[i] = make_float4(fx * (rv.x - Ae.x), fx * (rv.y - Ae.y), fx * (rv.z - Ae.z), 0.f);
@@ -1449,10 +1529,9 @@ __global__ void __launch_bounds__(1024) k_coarse_tail(const TailLevel* __restric
     // up
     for (int q = n - 2; q >= 0; q--) {
         const TailLevel& L = tl[q];
-        const TailLevel& C = tl[q + 1];
-        const float4* ec = flip(q + 1) ? C.e1 : C.e0;
-        float4* e = flip(q) ? L.e1 : L.e0;
-        float4* ep = flip(q) ? L.e0 : L.e1;
+        const float4* ec = ecur(q + 1);
+        float4* e = ecur(q);
+        float4* ep = eprv(q);
         for (int i = tid; i < L.V; i += nt) {
             const int4 id = L.iids[i];
             const float4 ww = L.iw[i];
@@ -1472,9 +1551,10 @@ __global__ void __launch_bounds__(1024) k_coarse_tail(const TailLevel* __restric
         smooth(q, sweeps, false);
     }
     // leave the first level's correction in e0
-    if (flip(0)) {
+    if (lds0 || flip(0)) {
         const TailLevel& L = tl[0];
-        for (int i = tid; i < L.V; i += nt) L.e0[i] = L.e1[i];
+        const float4* e = ecur(0);
+        for (int i = tid; i < L.V; i += nt) L.e0[i] = e[i];
     }
 }
 
@@ -1516,6 +1596,9 @@ struct DevLevel {
     float* rtw = nullptr;
     int *ap = nullptr, *aj = nullptr;  // scalar level matrix A = m/h^2 + L (coarse levels)
     float* av = nullptr;
+    int* ajE = nullptr;                // ... as column-major ELL (first level of the coarse tail)
+    float* avE = nullptr;
+    int ellw = 0;
     std::vector<float> adiag;          // its diagonal
     float4 *U1 = nullptr, *U2 = nullptr;  // corrections u = x - s of the last two steps
     float* cmask = nullptr;               // 0 where a collider acts (active set) at the last residual
@@ -1991,7 +2074,7 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool res
 constexpr int kGateAfter = 8, kGateFirst = 8, kGateMax = 256;
 
 bool temporal_warm(const mgsb_solver* s) {
-    return s->p.schedule == MGSB_SCHED_LEARNED || s->p.warm_start == 1;
+    return s->p.warm_start == 1 || (s->p.schedule == MGSB_SCHED_LEARNED && s->p.warm_start != 2);
 }
 
 // Inertial prediction, g(s) and res0, then the warm start: the temporal
@@ -2054,7 +2137,7 @@ int enqueue_vc_start(mgsb_solver* s, bool learned) {
     }
     // after a revert the corner forces belong to the rejected x: recompute
     // (the dual start's k_revert restores the forces at s by itself)
-    if (temporal && (learned || !dual) &&
+    if ((learned || (temporal && !dual)) &&
         (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, &s->vstate->cf_ok, true)))
         return rc;
     return MGSB_OK;
@@ -2250,6 +2333,9 @@ int setup_vcycle(mgsb_solver* s) {
     if (int rc = invalidate_graph(s)) return rc;
     s->L[0].vfix = s->L[0].h.fix;
     const float dt = s->p.dt;
+    // one-workgroup tail: the longest suffix of levels (>= 1) with V <= kTailMaxV
+    s->tail_from = -1;
+    for (int l = nl - 1; l >= 1 && s->L[l].V <= kTailMaxV; l--) s->tail_from = l;
     for (int l = 1; l < nl; l++) {
         DevLevel& c = s->L[l];
         derive_vfix(s->L[l - 1].vfix, s->M[l - 1].h_aids, s->M[l - 1].h_aw, c.V, c.vfix);
@@ -2267,6 +2353,28 @@ int setup_vcycle(mgsb_solver* s) {
             if ((rc = up(s, c.owned, &c.av, av.data(), av.size()))) return rc;
         } else {
             HIPCHK(hipMemcpyAsync(c.av, av.data(), sizeof(float) * av.size(), hipMemcpyHostToDevice, s->stream));
+        }
+        if (l == s->tail_from && l < nl - 1 && c.V <= kTailLdsV) {  // the tail's first level: ELL copy
+            int w = 0;
+            for (int i = 0; i < c.V; i++) w = std::max(w, ap[i + 1] - ap[i]);
+            std::vector<int32_t> je((size_t)w * c.V);
+            std::vector<float> ve((size_t)w * c.V);
+            for (int i = 0; i < c.V; i++)
+                for (int k = 0; k < w; k++) {
+                    const int q = ap[i] + k;
+                    const bool on = q < ap[i + 1];
+                    je[(size_t)k * c.V + i] = on ? aj[q] : i;
+                    ve[(size_t)k * c.V + i] = on ? av[q] : 0.f;
+                }
+            int rc = MGSB_OK;
+            if (!c.ajE) {
+                if ((rc = up(s, c.owned, &c.ajE, je.data(), je.size()))) return rc;
+                if ((rc = up(s, c.owned, &c.avE, ve.data(), ve.size()))) return rc;
+            } else {
+                HIPCHK(hipMemcpyAsync(c.avE, ve.data(), sizeof(float) * ve.size(), hipMemcpyHostToDevice, s->stream));
+            }
+            c.ellw = w;
+            HIPCHK(hipStreamSynchronize(s->stream));
         }
         std::vector<float4> vcl(c.V);
         for (int i = 0; i < c.V; i++) vcl[i] = make_float4(c.adiag[i], 0.f, c.vfix[i], c.h.mass[i]);
@@ -2307,9 +2415,6 @@ int setup_vcycle(mgsb_solver* s) {
         HIPCHK(hipStreamSynchronize(s->stream));
         cc.nfree = n;
     }
-    // one-workgroup tail: the longest suffix of levels (>= 1) with V <= kTailMaxV
-    s->tail_from = -1;
-    for (int l = nl - 1; l >= 1 && s->L[l].V <= kTailMaxV; l--) s->tail_from = l;
     if (s->tail_from >= 1) {
         std::vector<TailLevel> tl;
         for (int l = s->tail_from; l < nl; l++) {
@@ -2336,6 +2441,9 @@ int setup_vcycle(mgsb_solver* s) {
             t.rowinfo = d.rowinfo;
             t.big = d.big;
             t.rho = (float)d.rho;
+            t.ellw = l == s->tail_from ? d.ellw : 0;
+            t.ajE = d.ajE;
+            t.avE = d.avE;
             tl.push_back(t);
         }
         if (!s->tail) {
@@ -2490,8 +2598,8 @@ int validate_params(const mgsb_params* p, int nlevels) {
         set_error("sweep counts must be >= 0 (coarse_sweeps >= 1), max_cycles >= 0");
         return MGSB_ERR_ARG;
     }
-    if (p->warm_start < 0 || p->warm_start > 1 || p->learn_gate < 0 || p->learn_gate > 1) {
-        set_error("warm_start and learn_gate must be 0 or 1");
+    if (p->warm_start < 0 || p->warm_start > 2 || p->learn_gate < 0 || p->learn_gate > 1) {
+        set_error("warm_start must be 0, 1 or 2 and learn_gate 0 or 1");
         return MGSB_ERR_ARG;
     }
     if (p->check_every < 0 || p->check_every > 2) {

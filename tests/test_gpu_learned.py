@@ -124,10 +124,12 @@ def test_temporal_warm_start_matches_oracle(check):
     assert err < 1e-4 and agree >= 54
 
 
-def record_and_train(ns, steps, epochs, seed=0):
-    rec = small_solver(M.SCHED_VCYCLE, warm_start=1, check_every=2) if ns == (4, 1) else None
+def record_and_train(ns, steps, epochs, seed=0, warm=1):
+    """Record `steps` steps of the V-cycle (warm = 1: temporal start, targets
+    u - u_t; warm = 0: start from s, targets u) and train the MLP on them."""
+    rec = small_solver(M.SCHED_VCYCLE, warm_start=warm, check_every=2) if ns == (4, 1) else None
     if rec is None:
-        rec = medium_solver(M.SCHED_VCYCLE, warm_start=1)
+        rec = medium_solver(M.SCHED_VCYCLE, warm_start=warm)
     rec.learn_record(steps)
     rec.step(steps)
     assert rec.learn_samples() == steps * rec.levels[1].nverts
@@ -174,20 +176,53 @@ def test_learned_schedule_matches_oracle_given_prediction():
     assert used > 0
 
 
+def test_learned_only_matches_oracle_given_prediction():
+    """warm_start = 2: the MLP's coarse correction applied to s alone (no
+    temporal stage), judged against g(s), against the oracle fed the same
+    prediction."""
+    rec, hist = record_and_train((4, 1), 100, 20, warm=0)
+    w, norm = rec.learn_weights(), rec.learn_norm()
+    lv, Ls, ML, p, fix = oracle_pair((4, 1))
+    g = small_solver(M.SCHED_LEARNED, check_every=2, learn_gate=0, warm_start=2)
+    g.learn_weights(w)
+    g.learn_norm(norm)
+    vp = O.vparams(check=2, **VP)
+    agree = used = 0
+    for k in range(60):
+        st = g.step(1)
+        ML.set_warm(g.learn_last(), None)
+        ML.step(p, vp, 1)
+        o = ML.stats()
+        agree += (st.halves, st.fallback) == (o["halves"], o["fallback"])
+        used += (st.fallback & 2) == 0
+        assert st.rel <= 1e-3
+    err = rel_err(g.positions(), Ls[0].positions())
+    print(f"[learned on s vs oracle] rel={err:.2e} agree {agree / 60:.2f} prediction kept in {used}/60 steps")
+    assert err < 1e-4 and agree >= 54
+    assert used > 0
+
+
 def test_learned_ab_medium_box():
     """What the trained network gains on the 103,680-tet box: half-cycles
     per step to rel. residual 1e-3 of the plain V-cycle (start from s), the
-    V-cycle with the temporal warm start alone, and the learned schedule
-    (temporal warm start + MLP coarse delta), after the same 200 steps."""
+    network's coarse correction on s alone (trained on plain-schedule
+    recordings), the V-cycle with the temporal warm start alone, and the
+    learned schedule (temporal warm start + MLP coarse delta), after the same
+    200 steps."""
     rec, hist = record_and_train((12, 4, 1), 200, 20)
     print(f"[learned] training loss first {hist[0]:.4f} last {hist[-1]:.4f}")
     assert hist[-1] < 0.7 * hist[0]
     w, norm = rec.learn_weights(), rec.learn_norm()
+    rec_s, hist_s = record_and_train((12, 4, 1), 200, 20, warm=0)
+    print(f"[learned on s] training loss first {hist_s[0]:.4f} last {hist_s[-1]:.4f}")
     sols = {"plain": medium_solver(M.SCHED_VCYCLE, check_every=1),
+            "learned_s": medium_solver(M.SCHED_LEARNED, warm_start=2, check_every=2),
             "temporal": medium_solver(M.SCHED_VCYCLE, warm_start=1),
             "learned": medium_solver(M.SCHED_LEARNED)}
     sols["learned"].learn_weights(w)
     sols["learned"].learn_norm(norm)
+    sols["learned_s"].learn_weights(rec_s.learn_weights())
+    sols["learned_s"].learn_norm(rec_s.learn_norm())
     res = {}
     for name, s in sols.items():
         s.step(200)
@@ -202,3 +237,6 @@ def test_learned_ab_medium_box():
     # network's correction must not cost half-cycles (the judge undoes it)
     assert res["temporal"] < 0.5 * res["plain"]
     assert res["learned"] <= res["temporal"] + 0.05
+    # on its own (start from s) the network's coarse correction saves
+    # half-cycles: steps stop after the pre-smoothing half of the second cycle
+    assert res["learned_s"] < res["plain"] - 0.5
