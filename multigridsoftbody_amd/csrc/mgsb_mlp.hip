@@ -76,6 +76,59 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(int M, int N, int K, const fl
     }
 }
 
+// Split-K form of k_gemm_bf16 for products with few output tiles and a long
+// K (the weight gradients, K = batch rows): block (x, c) runs four tiles over
+// the K chunk [c*kc, min(K, (c+1)*kc)) and stores each lane's four MFMA
+// accumulators as part[(c*tiles + tile)*256 + lane*4 + q].
+__global__ void __launch_bounds__(256) k_gemm_bf16_part(int M, int N, int K, int kc, const float* __restrict__ A,
+                                                        long sam, long sak, const float* __restrict__ B, long sbk,
+                                                        long sbn, float* __restrict__ part) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tiles_n = (N + 15) >> 4, tiles = ((M + 15) >> 4) * tiles_n;
+    const int tile = blockIdx.x * 4 + wave;
+    if (tile >= tiles) return;
+    const int tm = tile / tiles_n, tn = tile % tiles_n;
+    const int row = tm * 16 + (lane & 15), col = tn * 16 + (lane & 15), kq = (lane >> 4) * 4;
+    const int kb = blockIdx.y * kc, ke = min(K, kb + kc);
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = kb; k0 < ke; k0 += 16) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int k = k0 + kq + q;
+            a[q] = f2bf((row < M && k < ke) ? A[row * sam + k * sak] : 0.f);
+            b[q] = f2bf((col < N && k < ke) ? B[k * sbk + col * sbn] : 0.f);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    *reinterpret_cast<f32x4*>(part + ((size_t)blockIdx.y * tiles + tile) * 256 + lane * 4) = acc;
+}
+
+// Second stage of the split-K product: each lane adds its accumulators over
+// the S chunks in chunk order (the same sum on every run), then the output
+// mapping, bias and activation of k_gemm_bf16.
+__global__ void __launch_bounds__(256) k_gemm_splitk_final(int M, int N, int S, const float* __restrict__ part,
+                                                           const float* __restrict__ bias, float* __restrict__ Z,
+                                                           float* __restrict__ H, int ldc, int act) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tiles_n = (N + 15) >> 4, tiles = ((M + 15) >> 4) * tiles_n;
+    const int tile = blockIdx.x * 4 + wave;
+    if (tile >= tiles) return;
+    const int tm = tile / tiles_n, tn = tile % tiles_n;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < S; c++) acc += *reinterpret_cast<const f32x4*>(part + ((size_t)c * tiles + tile) * 256 + lane * 4);
+    const int oc = tn * 16 + (lane & 15);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int orow = tm * 16 + (lane >> 4) * 4 + q;
+        if (orow < M && oc < N) {
+            float v = acc[q] + (bias ? bias[oc] : 0.f);
+            Z[(long)orow * ldc + oc] = v;
+            if (act == 1) H[(long)orow * ldc + oc] = v > 0.f ? v : 0.f;
+        }
+    }
+}
+
 // dY = 2 (Y - T) / (n*out); per-block partial sums of (Y - T)^2.
 __global__ void __launch_bounds__(256) k_mse(int n, const float* __restrict__ Y, const float* __restrict__ T,
                                              float* __restrict__ dY, float scale, double* __restrict__ part) {
@@ -256,6 +309,8 @@ struct Mlp {
     int* idx = nullptr;
     double* part = nullptr;
     float* colpart = nullptr;  // column-sum chunk partials
+    float* kpart = nullptr;    // split-K partial tiles
+    size_t kpart_cap = 0;      // floats
     std::vector<void*> owned;
 };
 
@@ -340,6 +395,7 @@ int mlp_create(Mlp** out, int in, int outd, uint32_t seed, hipStream_t st) {
 void mlp_destroy(Mlp* p) {
     if (!p) return;
     for (void* q : p->owned) (void)hipFree(q);
+    if (p->kpart) (void)hipFree(p->kpart);
     delete p;
 }
 
@@ -409,9 +465,33 @@ static int ensure_cap(Mlp* p, int cap) {
     return MGSB_OK;
 }
 
-static int gemm(int M, int N, int K, const float* A, long sam, long sak, const float* B, long sbk, long sbn,
+// Products with few output tiles and a long K (fewer than kSplitTiles tiles
+// of 16x16, K of at least 2 kSplitChunk) are split over K into chunks of
+// kSplitChunk, at most kSplitMax of them, and reduced in a second launch.
+constexpr int kSplitTiles = 256, kSplitChunk = 256, kSplitMax = 64;
+
+static int gemm(Mlp* p, int M, int N, int K, const float* A, long sam, long sak, const float* B, long sbk, long sbn,
                 const float* bias, float* Z, float* H, int ldc, int act, hipStream_t st) {
     const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
+    if (tiles < kSplitTiles && K >= 2 * kSplitChunk) {
+        int S = (K + kSplitChunk - 1) / kSplitChunk;
+        S = S < kSplitMax ? S : kSplitMax;
+        const int kc = ((K + S - 1) / S + 15) / 16 * 16;  // chunks of whole 16-steps
+        S = (K + kc - 1) / kc;
+        const size_t need = (size_t)S * tiles * 256;
+        if (need > p->kpart_cap) {
+            if (p->kpart) MLPCHK(hipFree(p->kpart));
+            p->kpart = nullptr;
+            p->kpart_cap = 0;
+            MLPCHK(hipMalloc(&p->kpart, need * sizeof(float)));
+            p->kpart_cap = need;
+        }
+        k_gemm_bf16_part<<<dim3((tiles + 3) / 4, S), 256, 0, st>>>(M, N, K, kc, A, sam, sak, B, sbk, sbn, p->kpart);
+        MLPCHK(hipGetLastError());
+        k_gemm_splitk_final<<<(tiles + 3) / 4, 256, 0, st>>>(M, N, S, p->kpart, bias, Z, H, ldc, act);
+        MLPCHK(hipGetLastError());
+        return MGSB_OK;
+    }
     k_gemm_bf16<<<(tiles + 3) / 4, 256, 0, st>>>(M, N, K, A, sam, sak, B, sbk, sbn, bias, Z, H, ldc, act);
     MLPCHK(hipGetLastError());
     return MGSB_OK;
@@ -426,33 +506,33 @@ static int fwd_bwd(Mlp* p, int nb, hipStream_t st, double* loss) {
     float* W3 = p->w + p->off[4];
     float* W4 = p->w + p->off[6];
     int rc;
-    if ((rc = gemm(nb, kH, in, p->Xb, in, 1, W1, kH, 1, p->w + p->off[1], p->Z1, p->H1, kH, 1, st))) return rc;
-    if ((rc = gemm(nb, kH, kH, p->H1, kH, 1, W2, kH, 1, p->w + p->off[3], p->Z2, p->H2, kH, 1, st))) return rc;
-    if ((rc = gemm(nb, kH, kH, p->H2, kH, 1, W3, kH, 1, p->w + p->off[5], p->Z3, p->H3, kH, 1, st))) return rc;
-    if ((rc = gemm(nb, out, kH, p->H3, kH, 1, W4, out, 1, p->w + p->off[7], p->Y, nullptr, out, 0, st))) return rc;
+    if ((rc = gemm(p, nb, kH, in, p->Xb, in, 1, W1, kH, 1, p->w + p->off[1], p->Z1, p->H1, kH, 1, st))) return rc;
+    if ((rc = gemm(p, nb, kH, kH, p->H1, kH, 1, W2, kH, 1, p->w + p->off[3], p->Z2, p->H2, kH, 1, st))) return rc;
+    if ((rc = gemm(p, nb, kH, kH, p->H2, kH, 1, W3, kH, 1, p->w + p->off[5], p->Z3, p->H3, kH, 1, st))) return rc;
+    if ((rc = gemm(p, nb, out, kH, p->H3, kH, 1, W4, out, 1, p->w + p->off[7], p->Y, nullptr, out, 0, st))) return rc;
     const int ny = nb * out;
     const int nbl = (ny + 255) / 256;
     k_mse<<<nbl, 256, 0, st>>>(ny, p->Y, p->Tb, p->dY, 1.f / (float)ny, p->part);
     MLPCHK(hipGetLastError());
     float* g = p->g;
     // layer 4: dW4 = H3^T dY, db4 = colsum dY, dA3 = dY W4^T
-    if ((rc = gemm(kH, out, nb, p->H3, 1, kH, p->dY, out, 1, nullptr, g + p->off[6], nullptr, out, 0, st))) return rc;
+    if ((rc = gemm(p, kH, out, nb, p->H3, 1, kH, p->dY, out, 1, nullptr, g + p->off[6], nullptr, out, 0, st))) return rc;
     colsum(p, nb, out, p->dY, g + p->off[7], st);
-    if ((rc = gemm(nb, kH, out, p->dY, out, 1, W4, 1, out, nullptr, p->dA, nullptr, kH, 0, st))) return rc;
+    if ((rc = gemm(p, nb, kH, out, p->dY, out, 1, W4, 1, out, nullptr, p->dA, nullptr, kH, 0, st))) return rc;
     const int nh = nb * kH;
     k_relu_bwd<<<(nh + 255) / 256, 256, 0, st>>>(nh, p->Z3, p->dA);
     // layer 3
-    if ((rc = gemm(kH, kH, nb, p->H2, 1, kH, p->dA, kH, 1, nullptr, g + p->off[4], nullptr, kH, 0, st))) return rc;
+    if ((rc = gemm(p, kH, kH, nb, p->H2, 1, kH, p->dA, kH, 1, nullptr, g + p->off[4], nullptr, kH, 0, st))) return rc;
     colsum(p, nb, kH, p->dA, g + p->off[5], st);
-    if ((rc = gemm(nb, kH, kH, p->dA, kH, 1, W3, 1, kH, nullptr, p->dB, nullptr, kH, 0, st))) return rc;
+    if ((rc = gemm(p, nb, kH, kH, p->dA, kH, 1, W3, 1, kH, nullptr, p->dB, nullptr, kH, 0, st))) return rc;
     k_relu_bwd<<<(nh + 255) / 256, 256, 0, st>>>(nh, p->Z2, p->dB);
     // layer 2
-    if ((rc = gemm(kH, kH, nb, p->H1, 1, kH, p->dB, kH, 1, nullptr, g + p->off[2], nullptr, kH, 0, st))) return rc;
+    if ((rc = gemm(p, kH, kH, nb, p->H1, 1, kH, p->dB, kH, 1, nullptr, g + p->off[2], nullptr, kH, 0, st))) return rc;
     colsum(p, nb, kH, p->dB, g + p->off[3], st);
-    if ((rc = gemm(nb, kH, kH, p->dB, kH, 1, W2, 1, kH, nullptr, p->dA, nullptr, kH, 0, st))) return rc;
+    if ((rc = gemm(p, nb, kH, kH, p->dB, kH, 1, W2, 1, kH, nullptr, p->dA, nullptr, kH, 0, st))) return rc;
     k_relu_bwd<<<(nh + 255) / 256, 256, 0, st>>>(nh, p->Z1, p->dA);
     // layer 1
-    if ((rc = gemm(in, kH, nb, p->Xb, 1, in, p->dA, kH, 1, nullptr, g + p->off[0], nullptr, kH, 0, st))) return rc;
+    if ((rc = gemm(p, in, kH, nb, p->Xb, 1, in, p->dA, kH, 1, nullptr, g + p->off[0], nullptr, kH, 0, st))) return rc;
     colsum(p, nb, kH, p->dA, g + p->off[1], st);
     MLPCHK(hipGetLastError());
     if (loss) {
