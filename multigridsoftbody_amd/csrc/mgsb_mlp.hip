@@ -216,8 +216,12 @@ __global__ void __launch_bounds__(256) k_pack_bf16(int K, int N, int Kp, int Np,
     Wb[i] = f2bf((k < K && n < N) ? W[k * N + n] : 0.f);
 }
 
-// Fused inference: one wave per 16 samples through all four layers; the
-// hidden activations of a wave stay in LDS as BF16.  Y = out_scale * mlp(X).
+// Fused inference: one workgroup of four waves per 16 samples through all
+// four layers.  Each wave computes two of a hidden layer's eight 16-column
+// output tiles; the layer's activations stay in LDS as BF16 and a barrier
+// separates the layers.  Wave 0 computes the output layer (one tile).  Every
+// tile accumulates its K blocks in order, as a one-wave-per-tile product
+// would.  Y = mlp(X).
 constexpr int kH = MLP_HIDDEN;
 __global__ void __launch_bounds__(256) k_mlp_infer(int n, int in, int out, const float* __restrict__ X,
                                                    const short* __restrict__ W1, const short* __restrict__ W2,
@@ -226,10 +230,11 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int in, int out, const
                                                    const float* __restrict__ b3, const float* __restrict__ b4,
                                                    float* __restrict__ Y, int ldy, const int* __restrict__ skip) {
     if (skip && *skip) return;
-    __shared__ short hs[4][2][16][kH];
+    __shared__ short hs[2][16][kH];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int base = (blockIdx.x * 4 + wave) * 16;
+    const int base = blockIdx.x * 16;
     const int r = lane & 15, kq = (lane >> 4) * 4;
+    constexpr int kTilesPerWave = kH / 16 / 4;
     // layer 1: K = 16 (inputs zero-padded)
     {
         bf16x4 a;
@@ -238,7 +243,8 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int in, int out, const
             const int k = kq + q, row = base + r;
             a[q] = f2bf((row < n && k < in) ? X[(long)row * in + k] : 0.f);
         }
-        for (int t = 0; t < kH / 16; t++) {
+        for (int tt = 0; tt < kTilesPerWave; tt++) {
+            const int t = wave * kTilesPerWave + tt;
             bf16x4 b;
 #pragma unroll
             for (int q = 0; q < 4; q++) b[q] = W1[(kq + q) * kH + t * 16 + r];
@@ -247,7 +253,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int in, int out, const
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const float v = acc[q] + b1[t * 16 + r];
-                hs[wave][0][kq + q][t * 16 + r] = f2bf(v > 0.f ? v : 0.f);
+                hs[0][kq + q][t * 16 + r] = f2bf(v > 0.f ? v : 0.f);
             }
         }
     }
@@ -257,13 +263,14 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int in, int out, const
     const float* bs[2] = {b2, b3};
     for (int L = 0; L < 2; L++) {
         const int src = L & 1, dst = src ^ 1;
-        for (int t = 0; t < kH / 16; t++) {
+        for (int tt = 0; tt < kTilesPerWave; tt++) {
+            const int t = wave * kTilesPerWave + tt;
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
             for (int kb = 0; kb < kH / 16; kb++) {
                 bf16x4 a, b;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    a[q] = hs[wave][src][r][kb * 16 + kq + q];
+                    a[q] = hs[src][r][kb * 16 + kq + q];
                     b[q] = Ws[L][(kb * 16 + kq + q) * kH + t * 16 + r];
                 }
                 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
@@ -271,18 +278,19 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int in, int out, const
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const float v = acc[q] + bs[L][t * 16 + r];
-                hs[wave][dst][kq + q][t * 16 + r] = f2bf(v > 0.f ? v : 0.f);
+                hs[dst][kq + q][t * 16 + r] = f2bf(v > 0.f ? v : 0.f);
             }
         }
         __syncthreads();
     }
     // layer 4: N = 16 (outputs zero-padded); source buffer is 0 after 2 layers
+    if (wave != 0) return;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int kb = 0; kb < kH / 16; kb++) {
         bf16x4 a, b;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            a[q] = hs[wave][0][r][kb * 16 + kq + q];
+            a[q] = hs[0][r][kb * 16 + kq + q];
             b[q] = W4[(kb * 16 + kq + q) * 16 + r];
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
@@ -429,8 +437,7 @@ int mlp_get_params(Mlp* p, float* w, int n, hipStream_t st) {
 }
 
 int mlp_infer(Mlp* p, const float* X, int n, float* Y, int ldy, hipStream_t st, const int* skip) {
-    const int waves = (n + 15) / 16;
-    const int blocks = (waves + 3) / 4;
+    const int blocks = (n + 15) / 16;  // one workgroup per 16 samples
     if (blocks == 0) return MGSB_OK;
     k_mlp_infer<<<blocks, 256, 0, st>>>(n, p->in, p->out, X, p->W1b, p->W2b, p->W3b, p->W4b, p->w + p->off[1],
                                         p->w + p->off[3], p->w + p->off[5], p->w + p->off[7], Y, ldy, skip);
