@@ -49,6 +49,7 @@ constexpr int kTetModeStable = 3; // calculateIF with the deviatoric-invariant p
 constexpr int kVertPoint = 1;     // sphere point DCD + plane
 constexpr int kVertTri = 2;       // gather triangle DCD contributions
 constexpr int kVertWriteCol = 4;  // write collision flags / depth / energy
+constexpr int kVertContactJacobi = 8;  // V-cycle: plain Jacobi (omega 1) where a collider acts
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -477,9 +478,13 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
     nx = (nx - p.x) * relax + p.x;
     ny = (ny - p.y) * relax + p.y;
     nz = (nz - p.z) * relax + p.z;
-    nx = omega * (nx - pp.x) + pp.x;
-    ny = omega * (ny - pp.y) + pp.y;
-    nz = omega * (nz - pp.z) + pp.z;
+    // V-cycle sweeps: the Chebyshev extrapolation of a contact vertex's stiff
+    // penalty direction keeps the contact rim from converging; those vertices
+    // take the plain Jacobi step
+    const float om = ((flags & kVertContactJacobi) && (cdx + cdy + cdz) > 0.f) ? 1.f : omega;
+    nx = om * (nx - pp.x) + pp.x;
+    ny = om * (ny - pp.y) + pp.y;
+    nz = om * (nz - pp.z) + pp.z;
     xprev[i] = p;
     x[i] = make_float4(nx, ny, nz, 0.f);
 }
@@ -2155,7 +2160,7 @@ int enqueue_vc_pre(mgsb_solver* s) {
     for (int k = 0; k < s->p.pre_sweeps; k++) {
         omega = cheb_next(omega, rs);
         if (k > 0 && (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, true))) return rc;
-        if ((rc = launch_vert(s, f, omega, kVertPoint, skip, 1))) return rc;
+        if ((rc = launch_vert(s, f, omega, kVertPoint | kVertContactJacobi, skip, 1))) return rc;
     }
     return fine_residual(s, s->p.pre_sweeps > 0, true, s->p.check_every == 2 ? kChkMid : kChkNone, false, skip);
 }
@@ -2179,7 +2184,9 @@ int enqueue_vc_post(mgsb_solver* s) {
     for (int k = 0; k < s->p.post_sweeps; k++) {
         omega = cheb_next(omega, rs);
         if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, true))) return rc;
-        if ((rc = launch_vert(s, f, omega, kVertPoint | (k == s->p.post_sweeps - 1 ? kVertWriteCol : 0), skip, 1)))
+        if ((rc = launch_vert(s, f, omega,
+                              kVertPoint | kVertContactJacobi | (k == s->p.post_sweeps - 1 ? kVertWriteCol : 0), skip,
+                              1)))
             return rc;
     }
     return fine_residual(s, true, false, kChkCycle, true, skip);

@@ -751,7 +751,10 @@ static void dcd_triangle(orc_level* L, const orc_params* p) {
 
 /* calculatePOS (PDSolver.cu:192-235).  The 0.6 under-relaxation is a double
  * literal in the reference, so the FP32 oracle evaluates that line in double. */
-static void calc_pos(orc_level* L, const orc_params* p, real omega) {
+/* contact_jacobi (V-cycle only, not the reference): vertices a collider acts
+ * on take the plain Jacobi step (omega 1); the Chebyshev extrapolation of
+ * their stiff penalty direction keeps the contact rim from converging. */
+static void calc_pos(orc_level* L, const orc_params* p, real omega, int contact_jacobi) {
     real dt = p->dt;
     double relax = p->relax;
 #pragma omp parallel for schedule(static)
@@ -764,7 +767,9 @@ static void calc_pos(orc_level* L, const orc_params* p, real omega) {
             real elem = L->force[q] + L->cforce[q];
             real next = (dc * (L->s[q] - pos) + elem) / (L->diag[i] + L->cdiag[q] + dc) * fix + pos;
             next = (real)((double)(next - pos) * relax + (double)pos);
-            next = omega * (next - L->xprev[q]) + L->xprev[q];
+            const real om =
+                contact_jacobi && (L->cdiag[i * 3] + L->cdiag[i * 3 + 1] + L->cdiag[i * 3 + 2]) > 0 ? 1 : omega;
+            next = om * (next - L->xprev[q]) + L->xprev[q];
             L->xprev[q] = pos;
             L->x[q] = next;
         }
@@ -781,14 +786,14 @@ static void move_spheres(orc_params* p) {
 }
 
 /* One PD iteration on a level: runClearTemp; DCD; IF; omega; POS. */
-static void pd_iteration(orc_level* L, const orc_params* p, real omega, int tri_dcd) {
+static void pd_iteration(orc_level* L, const orc_params* p, real omega, int tri_dcd, int contact_jacobi) {
     if (tri_dcd)
         dcd_triangle(L, p);
     else
         dcd_point(L, p);
     tet_forces(L, p->if_mode);
     gather_forces(L);
-    calc_pos(L, p, omega);
+    calc_pos(L, p, omega, contact_jacobi);
 }
 
 /* PDSolver::Step (PDSolver.cpp:243-293); sphere colliders move first, as
@@ -802,7 +807,7 @@ void PFX(step_single)(orc_level* L, orc_params* p, int nsteps) {
         real rho = p->rho;
         for (int i = 0; i < p->iters; i++) {
             omega = 4 / (4 - rho * rho * omega);
-            pd_iteration(L, p, omega, 0);
+            pd_iteration(L, p, omega, 0, 0);
         }
         calc_v(L, p);
     }
@@ -992,13 +997,13 @@ void PFX(step_mg)(orc_mg* M, orc_params* p, int nsteps) {
         pc.if_mode = mode;
         for (int i = 0; i < p->iters_coarse; i++) {
             omega = 4 / (4 - rho * rho * omega);
-            pd_iteration(c, &pc, omega, p->tri_dcd_coarse);
+            pd_iteration(c, &pc, omega, p->tri_dcd_coarse, 0);
         }
         update_point_in_tet(f, c, M->iids, M->iw);
         omega = 1;
         for (int i = 0; i < p->iters; i++) {
             omega = 4 / (4 - rho * rho * omega);
-            pd_iteration(f, p, omega, 0);
+            pd_iteration(f, p, omega, 0, 0);
         }
         update_point_in_tet(c, f, M->aids, M->aw);
         update_mapping(M);
@@ -1577,7 +1582,7 @@ void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nstep
             real omega = 1;
             for (int k = 0; k < vp->pre; k++) {
                 omega = 4 / (4 - rs * rs * omega);
-                pd_iteration(L0, p, omega, 0);
+                pd_iteration(L0, p, omega, 0, 1);
             }
             double rm = ml_fine_residual(M, p);
             if (vp->check == 2) {
@@ -1592,7 +1597,7 @@ void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nstep
             omega = 1;
             for (int k = 0; k < vp->post; k++) {
                 omega = 4 / (4 - rs * rs * omega);
-                pd_iteration(L0, p, omega, 0);
+                pd_iteration(L0, p, omega, 0, 1);
             }
             M->res = ml_fine_residual(M, p);
             M->cycles++;
