@@ -102,7 +102,9 @@ typedef struct mgsb_params {
     int32_t learn_gate;   /* learned schedule: after 8 consecutive steps whose learned
                            * correction was rejected, skip the MLP stage for 8, 16, ...
                            * 256 steps before trying it again (1, default) or never (0) */
-    int32_t reserved[2];
+    int32_t vert_lanes;   /* lanes that gather one vertex's row of corner forces in the
+                           * sweep and residual kernels: 1, 2, 4 or 8 (0: 4; tuned)   */
+    int32_t reserved[1];
 } mgsb_params;
 
 typedef struct mgsb_step_stats {
@@ -287,14 +289,15 @@ int mgsb_coarse_nfree(mgsb_solver* s, int32_t* nfree);
  * shapes).  Times trial_steps steps per candidate from a snapshot of the
  * solver state, restores the state, and keeps the fastest candidate whose
  * steps all met the residual target: pre/post sweeps in {3,4,5,6} (V-cycle
- * schedules), then block_tet and block_vert in {64,128,256}, one coordinate
- * at a time.  The chosen values are written into the solver's parameters. */
+ * schedules), then block_tet and block_vert in {64,128,256}, then vert_lanes
+ * in {1,2,4,8}, one coordinate at a time.  The chosen values are written into
+ * the solver's parameters. */
 typedef struct mgsb_tune_result {
     int32_t block_tet, block_vert, pre_sweeps, post_sweeps;
     double ms_per_step;          /* the chosen candidate                       */
     double ms_per_step_default;  /* the parameters before tuning               */
     int32_t trials;
-    int32_t reserved;
+    int32_t vert_lanes;
 } mgsb_tune_result;
 int mgsb_autotune(mgsb_solver* s, int32_t trial_steps, mgsb_tune_result* out);
 

@@ -71,7 +71,8 @@ class mgsb_params(C.Structure):
         ("coarse_direct", C.c_int32),
         ("warm_start", C.c_int32),
         ("learn_gate", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("vert_lanes", C.c_int32),
+        ("reserved", C.c_int32 * 1),
     ]
 
 
@@ -95,7 +96,7 @@ class mgsb_run_stats_t(C.Structure):
 class mgsb_tune_result(C.Structure):
     _fields_ = [("block_tet", C.c_int32), ("block_vert", C.c_int32), ("pre_sweeps", C.c_int32),
                 ("post_sweeps", C.c_int32), ("ms_per_step", C.c_double), ("ms_per_step_default", C.c_double),
-                ("trials", C.c_int32), ("reserved", C.c_int32)]
+                ("trials", C.c_int32), ("vert_lanes", C.c_int32)]
 
 
 class mgsb_energy_t(C.Structure):
@@ -528,9 +529,9 @@ class Solver:
         """Pick sweeps and block sizes by timing trial steps (state restored)."""
         r = mgsb_tune_result()
         _check(lib.mgsb_autotune(self._h, trial_steps, C.byref(r)), "mgsb_autotune")
-        for k in ("block_tet", "block_vert", "pre_sweeps", "post_sweeps"):  # keep the host copy in step
+        for k in ("block_tet", "block_vert", "pre_sweeps", "post_sweeps", "vert_lanes"):  # keep the host copy in step
             setattr(self.params, k, getattr(r, k))
-        return {k: getattr(r, k) for k, _ in r._fields_ if k != "reserved"}
+        return {k: getattr(r, k) for k, _ in r._fields_}
 
     def body_count(self):
         """(number of bodies, per-body residual checks on)."""

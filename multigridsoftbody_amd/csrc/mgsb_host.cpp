@@ -862,6 +862,8 @@ void mgsb_default_params(mgsb_params* p) {
     p->coarse_direct = 1;
     p->warm_start = 0;
     p->learn_gate = 1;
+    p->vert_lanes = 0;
+    p->reserved[0] = 0;
 }
 
 const char* mgsb_version(void) { return "mgsb 0.1 (gfx950)"; }

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -86,6 +87,51 @@ __device__ __forceinline__ void gather_sum(const float3* __restrict__ cf, int e0
             fy = fy + q[k].y;
             fz = fz + q[k].z;
         }
+}
+
+// Sum of the xyz of cf[e0..e1) by a group of 2^lsh adjacent lanes (sub = the
+// lane's index in the group): lane sub adds entries e0 + sub, e0 + sub + L, ...
+// (up to eight loads in flight), then the group combines its partials by an
+// xor butterfly, which leaves the identical total in every lane of the group
+// (a + b == b + a).  lsh = 0 is gather_sum.  A row is usually read in one
+// round trip instead of three, and a vertex kernel launches L times the
+// waves, which the medium box's 20k vertices need to cover 256 CUs.
+__device__ __forceinline__ float3 gather_lanes(const float3* __restrict__ cf, int e0, int e1, int sub, int lsh) {
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    if (lsh == 0) {
+        gather_sum(cf, e0, e1, fx, fy, fz);
+        return make_float3(fx, fy, fz);
+    }
+    const int L = 1 << lsh;
+    int e = e0 + sub;
+    for (; e + 7 * L < e1; e += 8 * L) {
+        float3 q[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) q[k] = cf[e + k * L];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            fx = fx + q[k].x;
+            fy = fy + q[k].y;
+            fz = fz + q[k].z;
+        }
+    }
+    const int rem = e < e1 ? (e1 - e + L - 1) >> lsh : 0;
+    float3 q[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) q[k] = k < rem ? cf[e + k * L] : make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if (k < rem) {
+            fx = fx + q[k].x;
+            fy = fy + q[k].y;
+            fz = fz + q[k].z;
+        }
+    for (int m = 1; m < L; m <<= 1) {
+        fx = fx + __shfl_xor(fx, m, 64);
+        fy = fy + __shfl_xor(fy, m, 64);
+        fz = fz + __shfl_xor(fz, m, 64);
+    }
+    return make_float3(fx, fy, fz);
 }
 
 // calculateST (PDSolver.cu:72-116).  Block 0 / lane 0 first advances sphere
@@ -378,17 +424,21 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
                                               const int* __restrict__ trp, const float4* __restrict__ tcf,
                                               const DevColliders* __restrict__ col, float omega, float relax,
                                               int flags, int* __restrict__ cflag, float* __restrict__ cdepth,
-                                              float* __restrict__ cenergy, const int* __restrict__ skip, int split) {
+                                              float* __restrict__ cenergy, const int* __restrict__ skip, int split,
+                                              int lsh) {
     if (skip && *skip) return;
-    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    // 2^lsh lanes per vertex gather its row (gather_lanes); lane 0 finishes
+    const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int i = g >> lsh, sub = g & ((1 << lsh) - 1);
     if (i >= V) return;
     // per-vertex streams in the same round trip as the row bounds
     const float4 p = x[i];  // x, or the correction u when split
     const float4 st = s[i];
     const float4 c = vc[i];  // (m/h^2, diag, fix, m)
     const float4 pp = xprev[i];
-    float fx = 0.f, fy = 0.f, fz = 0.f;
-    gather_sum(cf, rp[i], rp[i + 1], fx, fy, fz);
+    const float3 fg = gather_lanes(cf, rp[i], rp[i + 1], sub, lsh);
+    if (sub) return;
+    const float fx = fg.x, fy = fg.y, fz = fg.z;
     // position seen by the colliders, and s - x of the inertial term
     const float4 pa = split ? make_float4(st.x + p.x, st.y + p.y, st.z + p.z, 0.f) : p;
     const float sx = split ? -p.x : st.x - p.x, sy = split ? -p.y : st.y - p.y, sz = split ? -p.z : st.z - p.z;
@@ -882,24 +932,26 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
                                                const float3* __restrict__ cf, const DevColliders* __restrict__ col,
                                                int flags, float4* __restrict__ r, double* __restrict__ part,
                                                float4* __restrict__ xprev, const int* __restrict__ skip, int split,
-                                               double* __restrict__ g2, float* __restrict__ cmask) {
+                                               double* __restrict__ g2, float* __restrict__ cmask, int lsh) {
     if (skip && *skip) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gi >> lsh, sub = gi & ((1 << lsh) - 1);  // 2^lsh lanes per vertex
     double v[1] = {0.0};
     if (i < V) {
-        const float4 p = x[i];
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-        gather_sum(cf, rp[i], rp[i + 1], fx, fy, fz);
-        bool contact;
-        const float3 g = vertex_residual(p, s[i], vc[i], fx, fy, fz, col, flags, split, contact);
-        // the active set is left out of the restriction: the coarse operators
-        // carry no penalty stiffness (cmask also masks the coarse corrections)
-        const float am = contact ? 0.f : 1.f;
-        if (cmask) cmask[i] = am;
-        if (r) r[i] = cmask ? make_float4(g.x * am, g.y * am, g.z * am, 0.f) : make_float4(g.x, g.y, g.z, 0.f);
-        if (xprev) xprev[i] = p;
-        v[0] = (double)g.x * g.x + (double)g.y * g.y + (double)g.z * g.z;
-        if (g2) g2[i] = v[0];
+        const float4 p = x[i], st = s[i], c = vc[i];
+        const float3 f = gather_lanes(cf, rp[i], rp[i + 1], sub, lsh);
+        if (sub == 0) {
+            bool contact;
+            const float3 g = vertex_residual(p, st, c, f.x, f.y, f.z, col, flags, split, contact);
+            // the active set is left out of the restriction: the coarse operators
+            // carry no penalty stiffness (cmask also masks the coarse corrections)
+            const float am = contact ? 0.f : 1.f;
+            if (cmask) cmask[i] = am;
+            if (r) r[i] = cmask ? make_float4(g.x * am, g.y * am, g.z * am, 0.f) : make_float4(g.x, g.y, g.z, 0.f);
+            if (xprev) xprev[i] = p;
+            v[0] = (double)g.x * g.x + (double)g.y * g.y + (double)g.z * g.z;
+            if (g2) g2[i] = v[0];
+        }
     }
     if (part) block_reduce_store<1>(v, part);
 }
@@ -915,23 +967,26 @@ __global__ void __launch_bounds__(256) k_resid_dual(int V, const float4* __restr
                                                     const float3* __restrict__ cf2,
                                                     const DevColliders* __restrict__ col, int flags,
                                                     float4* __restrict__ r, double* __restrict__ part,
-                                                    float* __restrict__ cmask) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                    float* __restrict__ cmask, int lsh) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gi >> lsh, sub = gi & ((1 << lsh) - 1);  // 2^lsh lanes per vertex
     double v[2] = {0.0, 0.0};
     if (i < V) {
         const float4 p = x[i], st = s[i], c = vc[i];
         const int r0 = rp[i], r1 = rp[i + 1];
-        float ax = 0.f, ay = 0.f, az = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
-        gather_sum(cf2, r0, r1, ax, ay, az);  // forces at s
-        gather_sum(cf, r0, r1, bx, by, bz);   // forces at the extrapolation
-        bool contact;
-        const float3 g0 = vertex_residual(make_float4(0.f, 0.f, 0.f, 0.f), st, c, ax, ay, az, col, flags, 1, contact);
-        v[0] = (double)g0.x * g0.x + (double)g0.y * g0.y + (double)g0.z * g0.z;
-        const float3 g = vertex_residual(p, st, c, bx, by, bz, col, flags, 1, contact);
-        const float am = contact ? 0.f : 1.f;
-        if (cmask) cmask[i] = am;
-        if (r) r[i] = cmask ? make_float4(g.x * am, g.y * am, g.z * am, 0.f) : make_float4(g.x, g.y, g.z, 0.f);
-        v[1] = (double)g.x * g.x + (double)g.y * g.y + (double)g.z * g.z;
+        const float3 a = gather_lanes(cf2, r0, r1, sub, lsh);  // forces at s
+        const float3 b = gather_lanes(cf, r0, r1, sub, lsh);   // forces at the extrapolation
+        if (sub == 0) {
+            bool contact;
+            const float3 g0 =
+                vertex_residual(make_float4(0.f, 0.f, 0.f, 0.f), st, c, a.x, a.y, a.z, col, flags, 1, contact);
+            v[0] = (double)g0.x * g0.x + (double)g0.y * g0.y + (double)g0.z * g0.z;
+            const float3 g = vertex_residual(p, st, c, b.x, b.y, b.z, col, flags, 1, contact);
+            const float am = contact ? 0.f : 1.f;
+            if (cmask) cmask[i] = am;
+            if (r) r[i] = cmask ? make_float4(g.x * am, g.y * am, g.z * am, 0.f) : make_float4(g.x, g.y, g.z, 0.f);
+            v[1] = (double)g.x * g.x + (double)g.y * g.y + (double)g.z * g.z;
+        }
     }
     block_reduce_store<2>(v, part);
 }
@@ -1631,11 +1686,10 @@ struct LevelMap {  // between level l (fine) and l+1 (coarse)
 
 }  // namespace
 
-// Pieces of a V-cycle step (captured once each): the start (with or without
-// the learned stage), the two half-cycles, and the finish (with or without the
-// coarse delta that only the learned stage reads as a feature).
-enum { kPieceStart = 0, kPiecePre = 1, kPiecePost = 2, kPieceFinish = 3, kPieceStartGated = 4, kPieceFinishLite = 5,
-       kPieces = 6 };
+// Pieces of a V-cycle step: the start (with or without the learned stage),
+// the two half-cycles, and the finish (with or without the coarse delta that
+// only the learned stage reads as a feature).
+enum { kPieceStart = 0, kPiecePre = 1, kPiecePost = 2, kPieceFinish = 3, kPieceStartGated = 4, kPieceFinishLite = 5 };
 
 struct mgsb_solver {
     mgsb_params p{};
@@ -1662,13 +1716,18 @@ struct mgsb_solver {
     int* hmirror = nullptr;      // host-mapped (converged, cycles, halves, fallback, accepted, check count)
     unsigned* seqc = nullptr;    // device count of check kernels run
     unsigned chk_launched = 0;   // check kernels enqueued (graph replays included)
-    unsigned pchecks[kPieces] = {};  // check kernels in each captured piece
     // V-cycle steps run as pieces (start, pre-smoothing half, coarse + post
-    // half, finish), each captured once; the host launches only the halves a
-    // step needs, reading the convergence mirror after a guessed count.
-    hipGraph_t pgraph[kPieces] = {};
-    hipGraphExec_t pexec[kPieces] = {};
-    bool pvalid[kPieces] = {};
+    // half, finish).  The host launches the start, the halves the previous
+    // step needed and the finish as one graph, then reads the convergence
+    // mirror and adds halves (and a finish) only when the step needs them.
+    // Each distinct piece sequence is captured once: a graph launch costs
+    // about 5 us of idle GPU against about 1 us between kernels inside one.
+    struct PieceGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        unsigned checks = 0;  // check kernels in the sequence
+    };
+    std::map<std::string, PieceGraph> pseq;
     int halves_guess = 4;
     // learned schedule: after kGateAfter consecutive steps whose learned
     // correction the judge rejected, steps start without the MLP stage for
@@ -1699,9 +1758,9 @@ struct mgsb_solver {
         if (stream) (void)hipStreamSynchronize(stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (int q = 0; q < kPieces; q++) {
-            if (pexec[q]) (void)hipGraphExecDestroy(pexec[q]);
-            if (pgraph[q]) (void)hipGraphDestroy(pgraph[q]);
+        for (auto& kv : pseq) {
+            if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+            if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
         }
         if (hmirror) (void)hipHostFree(hmirror);
         for (auto& e : ev_pool) {
@@ -1820,6 +1879,13 @@ int push_vstate(mgsb_solver* s) {
 
 int block_tet(const mgsb_solver* s) { return s->p.block_tet > 0 ? s->p.block_tet : 256; }
 int block_vert(const mgsb_solver* s) { return s->p.block_vert > 0 ? s->p.block_vert : 256; }
+// lanes per vertex of the gathering vertex kernels (k_vert, k_resid,
+// k_resid_dual) as a shift: vert_lanes 1, 2, 4 or 8 (0: 4)
+int vert_lsh(const mgsb_solver* s) {
+    const int l = s->p.vert_lanes > 0 ? s->p.vert_lanes : 4;
+    return l >= 8 ? 3 : l >= 4 ? 2 : l >= 2 ? 1 : 0;
+}
+constexpr int kMaxVertLanes = 8;
 
 // --- launch helpers (all on s->stream) --------------------------------------
 int timed_begin(mgsb_solver* s, int kernel) {
@@ -1902,9 +1968,10 @@ int launch_tet_dual(mgsb_solver* s, DevLevel& d) {
 }
 
 int launch_vert(mgsb_solver* s, DevLevel& d, float omega, int flags, const int* skip = nullptr, int split = 0) {
-    const int bs = block_vert(s);
-    k_vert<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.x, d.xprev, d.s, d.vc, d.rp, d.cf, d.trp, d.tcf, s->dcol, omega,
-                                                 s->p.relax, flags, d.cflag, d.cdepth, d.cenergy, skip, split);
+    const int bs = block_vert(s), lsh = vert_lsh(s);
+    k_vert<<<nblk(d.V << lsh, bs), bs, 0, s->stream>>>(d.V, d.x, d.xprev, d.s, d.vc, d.rp, d.cf, d.trp, d.tcf, s->dcol,
+                                                        omega, s->p.relax, flags, d.cflag, d.cdepth, d.cenergy, skip,
+                                                        split, lsh);
     HIPCHK(hipGetLastError());
     return MGSB_OK;
 }
@@ -2049,11 +2116,11 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool res
     int rc;
     if (need_tet && (rc = launch_tet(s, d, tet_mode_of(s->p.if_mode), false, first ? d.s : d.x, skip, !first)))
         return rc;
-    const int nb = nblk(d.V, 256);
+    const int lsh = vert_lsh(s), nb = nblk(d.V << lsh, 256);
     if (s->body_checks && chk != kChkNone) {  // per-body norms, then the check
         k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint,
                                            write_r ? d.r : nullptr, nullptr, restart ? d.xprev : nullptr, skip, 1,
-                                           s->g2, d.cmask);
+                                           s->g2, d.cmask, lsh);
         HIPCHK(hipGetLastError());
         k_body_check<<<s->nbodies, 256, 0, s->stream>>>(s->vb, s->g2, s->bodysq, s->res0b, s->bcount, s->vstate, chk,
                                                         s->hmirror, skip, s->seqc);
@@ -2063,7 +2130,7 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool res
     }
     k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, write_r ? d.r : nullptr,
                                        chk != kChkNone ? s->vpart : nullptr, restart ? d.xprev : nullptr, skip, 1,
-                                       nullptr, d.cmask);
+                                       nullptr, d.cmask, lsh);
     HIPCHK(hipGetLastError());
     if (chk != kChkNone) {
         k_check<<<1, 256, 0, s->stream>>>(nb, s->vpart, s->vstate, chk, s->hmirror, skip, s->seqc);
@@ -2103,10 +2170,11 @@ int enqueue_vc_start(mgsb_solver* s, bool learned) {
     if ((rc = launch_st(s, f, 1, 1, dual))) return rc;
     if (dual) {
         if ((rc = launch_tet_dual(s, f))) return rc;
-        k_resid_dual<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.s, f.vc, f.rp, f.cf, f.cf2, s->dcol, kVertPoint,
-                                                 features ? f.r : nullptr, s->vpart, f.cmask);
+        const int lsh = vert_lsh(s), nbr = nblk(f.V << lsh, 256);
+        k_resid_dual<<<nbr, 256, 0, s->stream>>>(f.V, f.x, f.s, f.vc, f.rp, f.cf, f.cf2, s->dcol, kVertPoint,
+                                                 features ? f.r : nullptr, s->vpart, f.cmask, lsh);
         HIPCHK(hipGetLastError());
-        k_check2<<<1, 256, 0, s->stream>>>(nbv, s->vpart, s->vstate, s->hmirror, s->seqc);
+        k_check2<<<1, 256, 0, s->stream>>>(nbr, s->vpart, s->vstate, s->hmirror, s->seqc);
         s->chk_launched++;
         HIPCHK(hipGetLastError());
         k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, f.rp, f.cf2, f.cf);
@@ -2226,27 +2294,39 @@ int enqueue_piece(mgsb_solver* s, int piece) {
     }
 }
 
-int run_piece(mgsb_solver* s, int piece, bool graphable) {
-    if (!graphable) return enqueue_piece(s, piece);
-    if (!s->pvalid[piece]) {
+// Enqueues the pieces seq (kPiece* codes, one char each) in order: eagerly,
+// or as one graph captured at the sequence's first use.
+int run_pieces(mgsb_solver* s, const std::string& seq, bool graphable) {
+    if (!graphable) {
+        for (const char q : seq)
+            if (int rc = enqueue_piece(s, q)) return rc;
+        return MGSB_OK;
+    }
+    auto it = s->pseq.find(seq);
+    if (it == s->pseq.end()) {
         HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         const unsigned before = s->chk_launched;
-        const int rc = enqueue_piece(s, piece);
-        s->pchecks[piece] = s->chk_launched - before;  // captured, not run yet
+        int rc = MGSB_OK;
+        for (const char q : seq)
+            if ((rc = enqueue_piece(s, q))) break;
+        mgsb_solver::PieceGraph pg;
+        pg.checks = s->chk_launched - before;  // captured, not run yet
         s->chk_launched = before;
-        hipGraph_t g = nullptr;
-        const hipError_t ce = hipStreamEndCapture(s->stream, &g);
-        if (rc) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
+        const hipError_t ce = hipStreamEndCapture(s->stream, &pg.graph);
+        if (rc || ce != hipSuccess) {
+            if (pg.graph) (void)hipGraphDestroy(pg.graph);
+            if (rc) return rc;
+            HIPCHK(ce);
         }
-        HIPCHK(ce);
-        s->pgraph[piece] = g;
-        HIPCHK(hipGraphInstantiate(&s->pexec[piece], g, nullptr, nullptr, 0));
-        s->pvalid[piece] = true;
+        const hipError_t ie = hipGraphInstantiate(&pg.exec, pg.graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(pg.graph);
+            HIPCHK(ie);
+        }
+        it = s->pseq.emplace(seq, pg).first;
     }
-    HIPCHK(hipGraphLaunch(s->pexec[piece], s->stream));
-    s->chk_launched += s->pchecks[piece];
+    HIPCHK(hipGraphLaunch(it->second.exec, s->stream));
+    s->chk_launched += it->second.checks;
     return MGSB_OK;
 }
 
@@ -2278,7 +2358,7 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
     const bool learned = s->p.schedule == MGSB_SCHED_LEARNED;
     const bool gated = learned && s->p.learn_gate && !s->recording && s->gate_left > 0;
     if (gated) s->gate_left--;
-    if ((rc = run_piece(s, gated ? kPieceStartGated : kPieceStart, graphable))) return rc;
+    std::string seq(1, (char)(gated ? kPieceStartGated : kPieceStart));
     const int check = s->p.check_every;
     const int maxh = 2 * s->p.max_cycles;
     // the next step reads this step's coarse delta only if it runs the learned
@@ -2288,14 +2368,15 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
     int target = check ? std::min(maxh, std::max(0, s->halves_guess)) : maxh;
     bool finished = false;
     for (;;) {
-        for (; h < target; h++)
-            if ((rc = run_piece(s, (h & 1) ? kPiecePost : kPiecePre, graphable))) return rc;
+        for (; h < target; h++) seq.push_back((char)((h & 1) ? kPiecePost : kPiecePre));
         if (!check) break;
         if (check == 1 && (h & 1)) {  // a pre half carries no check: finish the cycle
             target = h + 1;
             continue;
         }
-        if ((rc = run_piece(s, fin, graphable))) return rc;  // runs only if the step is done
+        seq.push_back((char)fin);  // the finish runs only if the step is done
+        if ((rc = run_pieces(s, seq, graphable))) return rc;
+        seq.clear();
         if ((rc = wait_checks(s))) return rc;
         if (s->hmirror[0] || h >= maxh) {
             s->halves_guess = s->hmirror[0] ? s->hmirror[2] : maxh;
@@ -2316,7 +2397,8 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
             s->gate_skip = 0;
         }
     }
-    if (!finished && (rc = run_piece(s, fin, graphable))) return rc;
+    if (!finished) seq.push_back((char)fin);
+    if (!seq.empty() && (rc = run_pieces(s, seq, graphable))) return rc;
     if (s->recording) s->rec_steps++;  // the finish enqueued above recorded this step's target
     return MGSB_OK;
 }
@@ -2508,7 +2590,8 @@ int alloc_vcycle(mgsb_solver* s) {
     }
     if ((rc = dalloc(&s->vstate, 1))) return rc;
     s->owned.push_back(s->vstate);
-    if ((rc = dalloc(&s->vpart, 2 * (size_t)nblk(s->L[0].V, 256)))) return rc;  // two norms (dual start)
+    // two norms (dual start) per block of the residual kernels at any lane count
+    if ((rc = dalloc(&s->vpart, 2 * (size_t)nblk(s->L[0].V * kMaxVertLanes, 256)))) return rc;
     s->owned.push_back(s->vpart);
     if ((rc = dalloc(&s->seqc, 1))) return rc;
     s->owned.push_back(s->seqc);
@@ -2545,10 +2628,10 @@ int residual_norm(mgsb_solver* s, double* out) {
     DevLevel& d = s->L[0];
     int rc = launch_tet(s, d, tet_mode_of(s->p.if_mode), false);
     if (rc) return rc;
-    const int nb = nblk(d.V, 256);
+    const int lsh = vert_lsh(s), nb = nblk(d.V << lsh, 256);
     if ((rc = ensure_partials(s, (size_t)nb * 4))) return rc;
     k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, nullptr, s->dpart,
-                                       nullptr, nullptr, 0, nullptr, nullptr);
+                                       nullptr, nullptr, 0, nullptr, nullptr, lsh);
     HIPCHK(hipGetLastError());
     double sq = 0;
     if ((rc = reduce_finish(s, nb, 1, &sq))) return rc;
@@ -2631,6 +2714,11 @@ int validate_params(const mgsb_params* p, int nlevels) {
         set_error("block sizes must be 0 or a multiple of 64 in [64, 256]");
         return MGSB_ERR_ARG;
     }
+    const int vl = p->vert_lanes;
+    if (vl != 0 && vl != 1 && vl != 2 && vl != 4 && vl != 8) {
+        set_error("vert_lanes must be 0, 1, 2, 4 or 8");
+        return MGSB_ERR_ARG;
+    }
     return MGSB_OK;
 }
 
@@ -2641,13 +2729,11 @@ int invalidate_graph(mgsb_solver* s) {
     s->graph = nullptr;
     s->graph_valid = false;
     s->gate_rejects = s->gate_skip = s->gate_left = 0;
-    for (int q = 0; q < kPieces; q++) {
-        if (s->pexec[q]) HIPCHK(hipGraphExecDestroy(s->pexec[q]));
-        if (s->pgraph[q]) HIPCHK(hipGraphDestroy(s->pgraph[q]));
-        s->pexec[q] = nullptr;
-        s->pgraph[q] = nullptr;
-        s->pvalid[q] = false;
+    for (auto& kv : s->pseq) {
+        if (kv.second.exec) HIPCHK(hipGraphExecDestroy(kv.second.exec));
+        if (kv.second.graph) HIPCHK(hipGraphDestroy(kv.second.graph));
     }
+    s->pseq.clear();
     return MGSB_OK;
 }
 
@@ -3510,6 +3596,10 @@ int mgsb_autotune(mgsb_solver* s, int32_t trial_steps, mgsb_tune_result* out) {
         s->p.block_vert = b;
         if ((rc = consider())) return rc;
     }
+    for (int l : {1, 2, 4, 8}) {
+        s->p.vert_lanes = l;
+        if ((rc = consider())) return rc;
+    }
     s->p = best < 1e300 ? bestp : p0;
     if ((rc = snap_restore(s, S)) || (rc = invalidate_graph(s))) return rc;
     out->block_tet = block_tet(s);
@@ -3519,7 +3609,7 @@ int mgsb_autotune(mgsb_solver* s, int32_t trial_steps, mgsb_tune_result* out) {
     out->ms_per_step = best < 1e300 ? best : base;
     out->ms_per_step_default = base;
     out->trials = trials;
-    out->reserved = 0;
+    out->vert_lanes = 1 << vert_lsh(s);
     return MGSB_OK;
 }
 

@@ -30,14 +30,15 @@ def test_autotune_restores_state_and_picks_valid_parameters():
     assert np.array_equal(a.positions(), x0) and np.array_equal(a.velocity(), v0)
     assert r["block_tet"] in (64, 128, 256) and r["block_vert"] in (64, 128, 256)
     assert r["pre_sweeps"] in (3, 4, 5, 6) and r["pre_sweeps"] == r["post_sweeps"]
-    assert r["trials"] == 11 and r["ms_per_step"] <= r["ms_per_step_default"]
+    assert r["vert_lanes"] in (1, 2, 4, 8)
+    assert r["trials"] == 15 and r["ms_per_step"] <= r["ms_per_step_default"]
     assert a.params.pre_sweeps == r["pre_sweeps"]
     # with the original parameters restored the run continues as an untuned copy
     b.step(1)
     c = make()
     c.step(30)
     c.autotune(trial_steps=4)
-    c.set_params(pre_sweeps=5, post_sweeps=5, block_tet=0, block_vert=0)
+    c.set_params(pre_sweeps=5, post_sweeps=5, block_tet=0, block_vert=0, vert_lanes=0)
     c.step(1)
     assert np.array_equal(c.positions(), b.positions())
     # tuned steps meet the target
