@@ -241,7 +241,8 @@ def main():
         log(f"[bench] {name}: timed {args.steps} steps in {elapsed:.3f}s -> {sps:.1f} steps/s; run {run}")
         results[name] = dict(steps_per_s=round(sps, 3), ms_per_step=round(1e3 * elapsed / args.steps, 4))
         if tuned:
-            results[name]["autotuned"] = {k: tuned[k] for k in ("block_tet", "block_vert", "pre_sweeps", "post_sweeps")}
+            results[name]["autotuned"] = {k: tuned[k] for k in ("block_tet", "block_vert", "pre_sweeps", "post_sweeps",
+                                                                 "vert_lanes")}
         if run:
             results[name].update(mean_cycles=round(run["total_cycles"] / max(1, run["steps"]), 3),
                                  mean_half_cycles=round(run["total_halves"] / max(1, run["steps"]), 3),

@@ -100,8 +100,10 @@ typedef struct mgsb_params {
                            * temporal stage for 0 and 1; 2 runs its MLP stage on s
                            * alone (no temporal stage, the network's own gain).   */
     int32_t learn_gate;   /* learned schedule: after 8 consecutive steps whose learned
-                           * correction was rejected, skip the MLP stage for 8, 16, ...
-                           * 256 steps before trying it again (1, default) or never (0) */
+                           * correction was rejected, or kept without saving a half-cycle
+                           * against the last step run without it (one such step in 64
+                           * renews that count), skip the MLP stage for 8, 16, ... 256
+                           * steps before trying it again (1, default) or never (0)    */
     int32_t vert_lanes;   /* lanes that gather one vertex's row of corner forces in the
                            * sweep and residual kernels: 1, 2, 4 or 8 (0: 4; tuned)   */
     int32_t reserved[1];

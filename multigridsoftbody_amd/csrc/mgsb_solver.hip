@@ -1730,9 +1730,15 @@ struct mgsb_solver {
     std::map<std::string, PieceGraph> pseq;
     int halves_guess = 4;
     // learned schedule: after kGateAfter consecutive steps whose learned
-    // correction the judge rejected, steps start without the MLP stage for
-    // gate_skip steps (doubling up to kGateMax after each further rejection)
+    // correction the judge rejected, or kept while the step still needed as
+    // many half-cycles as the last step run without it, steps start without
+    // the MLP stage for gate_skip steps (doubling up to kGateMax after each
+    // further such run); every kGateProbe steps with the stage, one step runs
+    // without it to renew that baseline
     int gate_rejects = 0, gate_skip = 0, gate_left = 0;
+    // halves of the last step run without the stage (-1: none yet), and the
+    // steps with the stage since
+    int base_halves = -1, learned_run = 0;
     // bodies: connected components of level 0 (body_of[l][v] on every level,
     // through the coarse-in-fine bindings); when they are contiguous vertex
     // ranges the V-cycle checks each body's own relative residual
@@ -2143,7 +2149,7 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool res
 // A residual-correction V-cycle step (include/mgsb.h, MGSB_SCHED_VCYCLE) is
 // enqueued as four pieces.  Every kernel after the start returns at once when
 // the step's convergence flag is set.
-constexpr int kGateAfter = 8, kGateFirst = 8, kGateMax = 256;
+constexpr int kGateAfter = 8, kGateFirst = 8, kGateMax = 256, kGateProbe = 64;
 
 bool temporal_warm(const mgsb_solver* s) {
     return s->p.warm_start == 1 || (s->p.schedule == MGSB_SCHED_LEARNED && s->p.warm_start != 2);
@@ -2385,16 +2391,28 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
         }
         target = h + (check == 2 ? 1 : 2);
     }
-    if (learned && !gated && check) {  // the mirror holds this step's judge verdict
-        if (s->hmirror[3] & 2) {
-            if (++s->gate_rejects >= kGateAfter) {
-                s->gate_skip = s->gate_skip ? std::min(2 * s->gate_skip, kGateMax) : kGateFirst;
-                s->gate_left = s->gate_skip;
+    if (learned && check && finished && !s->recording) {  // the mirror holds this step's verdicts
+        const int halves = s->hmirror[2];
+        if (gated) {  // a step without the stage: the baseline it has to beat
+            s->base_halves = halves;
+            s->learned_run = 0;
+        } else {
+            const bool rejected = (s->hmirror[3] & 2) != 0, accepted = s->hmirror[4] == 2;
+            // kept, but the step needed as many halves as without the stage
+            const bool idle = accepted && s->base_halves >= 0 && halves >= s->base_halves;
+            if (rejected || idle) {
+                if (++s->gate_rejects >= kGateAfter) {
+                    s->gate_skip = s->gate_skip ? std::min(2 * s->gate_skip, kGateMax) : kGateFirst;
+                    s->gate_left = s->gate_skip;
+                    s->gate_rejects = 0;
+                }
+            } else if (accepted) {  // (else the stage was skipped: no verdict)
                 s->gate_rejects = 0;
+                s->gate_skip = 0;
             }
-        } else if (s->hmirror[4] == 2) {  // accepted (else the stage was skipped: no verdict)
-            s->gate_rejects = 0;
-            s->gate_skip = 0;
+            // no baseline yet, or an old one: the next step runs without the stage
+            if (s->p.learn_gate && s->gate_left == 0 && (s->base_halves < 0 || ++s->learned_run >= kGateProbe))
+                s->gate_left = 1;
         }
     }
     if (!finished) seq.push_back((char)fin);
@@ -2729,6 +2747,8 @@ int invalidate_graph(mgsb_solver* s) {
     s->graph = nullptr;
     s->graph_valid = false;
     s->gate_rejects = s->gate_skip = s->gate_left = 0;
+    s->base_halves = -1;
+    s->learned_run = 0;
     for (auto& kv : s->pseq) {
         if (kv.second.exec) HIPCHK(hipGraphExecDestroy(kv.second.exec));
         if (kv.second.graph) HIPCHK(hipGraphDestroy(kv.second.graph));
@@ -3486,7 +3506,7 @@ struct StateSnap {
     std::vector<std::vector<float4>> bufs;
     VState v{};
     DevColliders col{};
-    int halves_guess = 0, gate_rejects = 0, gate_skip = 0, gate_left = 0;
+    int halves_guess = 0, gate_rejects = 0, gate_skip = 0, gate_left = 0, base_halves = -1, learned_run = 0;
 };
 
 int snap_take(mgsb_solver* s, StateSnap& S) {
@@ -3512,6 +3532,8 @@ int snap_take(mgsb_solver* s, StateSnap& S) {
     S.gate_rejects = s->gate_rejects;
     S.gate_skip = s->gate_skip;
     S.gate_left = s->gate_left;
+    S.base_halves = s->base_halves;
+    S.learned_run = s->learned_run;
     return MGSB_OK;
 }
 
@@ -3527,6 +3549,8 @@ int snap_restore(mgsb_solver* s, const StateSnap& S) {
     s->gate_rejects = S.gate_rejects;
     s->gate_skip = S.gate_skip;
     s->gate_left = S.gate_left;
+    s->base_halves = S.base_halves;
+    s->learned_run = S.learned_run;
     return MGSB_OK;
 }
 

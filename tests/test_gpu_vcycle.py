@@ -61,6 +61,29 @@ def test_vcycle_fixed_cycles_matches_oracle(ns, rs):
     assert st.rel <= max(2 * r64, 1e-5), (st.rel, r64)
 
 
+@pytest.mark.parametrize("lanes", [1, 2, 8])
+def test_vcycle_gather_lanes_match_oracle(lanes):
+    """Rows of corner forces gathered by 1, 2 or 8 lanes per vertex (the
+    default, 4, runs in every other test): the same schedule, the same
+    oracle trajectory, and the same residual-norm partials consumed by the
+    adaptive checks (a wider launch gives more partials)."""
+    kw = dict(pre_sweeps=2, post_sweeps=2, coarse_sweeps=2, max_cycles=8, check_every=2, rho_smooth=0.9,
+              tol=1e-3, vert_lanes=lanes)
+    lv, s, ML, Ls, p = make_pair((8, 4, 1), None, **kw)
+    vp = O.vparams(pre=2, post=2, coarse_sweeps=2, max_cycles=8, check=2, rho_smooth=0.9, tol=1e-3)
+    halves_g, halves_o = [], []
+    for _ in range(30):
+        st = s.step(1)
+        ML.step(p, vp, 1)
+        halves_g.append(st.halves)
+        halves_o.append(ML.stats()["halves"])
+        assert st.rel <= 1e-3
+    err = rel_err(s.positions(), Ls[0].positions())
+    print(f"[lanes {lanes}] rel(gpu,oracle)={err:.2e} halves gpu {sum(halves_g)} oracle {sum(halves_o)}")
+    assert err < REL_TOL
+    assert sum(a == b for a, b in zip(halves_g, halves_o)) >= 27
+
+
 @pytest.mark.parametrize("check", [1, 2])
 def test_vcycle_adaptive_meets_target_each_step(check):
     kw = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, check_every=check, tol=1e-3,
