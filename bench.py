@@ -129,8 +129,9 @@ def timed_run(s, steps, warmup, vcycle):
     return elapsed, run
 
 
-def cpu_baseline(levels, cfg, schedule, budget_s=15.0):
-    """The oracle on the same workload and schedule, FP32, OpenMP threads."""
+def cpu_baseline(levels, cfg, schedule, budget_s=12.0):
+    """The oracle on the same workload and schedule, FP32: budget_s seconds
+    on up to 16 OpenMP threads, then half that on one thread."""
     import oracle as O
     from multigridsoftbody_amd import workloads as W
 
@@ -175,15 +176,22 @@ def cpu_baseline(levels, cfg, schedule, budget_s=15.0):
                      iters=cfg["iters"], if_mode=1, **col)
         step = lambda: Ls[0].step(p, 1)  # noqa: E731
         what = f"single-level reference schedule ({cfg['iters']} PD iterations/step)"
+    def rate(budget):
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget:
+            step()
+            n += 1
+        return n, n / (time.perf_counter() - t0)
+
     step()
-    n, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        step()
-        n += 1
-    dt = time.perf_counter() - t0
-    return {"value": round(n / dt, 4), "unit": "steps/s", "cores": threads, "kind": "port",
+    n, sps = rate(budget_s)
+    O.set_threads(1)  # the same trajectory continued on one thread
+    n1, sps1 = rate(budget_s / 2)
+    O.set_threads(threads)
+    return {"value": round(sps, 4), "unit": "steps/s", "cores": threads, "kind": "port",
             "sample": f"{n} steps (after 1 warm-up) of the {what} on the {levels[0].ntets}-tet workload, "
-                      f"FP32 oracle, {threads} OpenMP threads"}
+                      f"FP32 oracle, {threads} OpenMP threads",
+            "value_1_thread": round(sps1, 4), "sample_1_thread": f"the next {n1} steps on one thread"}
 
 
 def main():
@@ -266,7 +274,7 @@ def main():
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         cpu = cpu_baseline(levels, cfg, "vcycle" if head in ("temporal", "learned", "learned_s") else head)
-        log(f"[bench] cpu baseline {cpu['value']} steps/s")
+        log(f"[bench] cpu baseline {cpu['value']} steps/s ({cpu['cores']} threads), {cpu['value_1_thread']} on one")
 
     config = {
         "workload": cfg["name"],
