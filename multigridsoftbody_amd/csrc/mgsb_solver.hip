@@ -50,7 +50,7 @@ constexpr int kTetModeStable = 3; // calculateIF with the deviatoric-invariant p
 constexpr int kVertPoint = 1;     // sphere point DCD + plane
 constexpr int kVertTri = 2;       // gather triangle DCD contributions
 constexpr int kVertWriteCol = 4;  // write collision flags / depth / energy
-constexpr int kVertContactJacobi = 8;  // V-cycle: plain Jacobi (omega 1) where a collider acts
+constexpr int kVertContactJacobi = 8;  // V-cycle: exact 3x3 block step, no Chebyshev, where a collider acts
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -443,6 +443,7 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
     const float4 pa = split ? make_float4(st.x + p.x, st.y + p.y, st.z + p.z, 0.f) : p;
     const float sx = split ? -p.x : st.x - p.x, sy = split ? -p.y : st.y - p.y, sz = split ? -p.z : st.z - p.z;
     float cdx = 0.f, cdy = 0.f, cdz = 0.f, cfx = 0.f, cfy = 0.f, cfz = 0.f;
+    float cxy = 0.f, cxz = 0.f, cyz = 0.f;  // off-diagonals of the contacts' k n n^T
     int flag = 0;
     float depth = 0.f, energy = 0.f;
     if (flags & kVertTri) {
@@ -490,6 +491,9 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
                 cdx = cdx + kc * n.x * n.x;
                 cdy = cdy + kc * n.y * n.y;
                 cdz = cdz + kc * n.z * n.z;
+                cxy = cxy + kc * n.x * n.y;
+                cxz = cxz + kc * n.x * n.z;
+                cyz = cyz + kc * n.y * n.z;
                 cfx = cfx + kc * t * nd * n.x;
                 cfy = cfy + kc * t * nd * n.y;
                 cfz = cfz + kc * t * nd * n.z;
@@ -509,6 +513,9 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
                 cdx = cdx + kp * n.x * n.x;
                 cdy = cdy + kp * n.y * n.y;
                 cdz = cdz + kp * n.z * n.z;
+                cxy = cxy + kp * n.x * n.y;
+                cxz = cxz + kp * n.x * n.z;
+                cyz = cyz + kp * n.y * n.z;
                 cfx = cfx + kp * dp * n.x;
                 cfy = cfy + kp * dp * n.y;
                 cfz = cfz + kp * dp * n.z;
@@ -522,19 +529,38 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
         cenergy[i] = energy;
     }
     const float dc = c.x, dg = c.y, fix = c.z;
+    if ((flags & kVertContactJacobi) && (cdx + cdy + cdz) > 0.f) {
+        // V-cycle sweeps, a vertex a collider acts on: the Jacobi step solves
+        // its 3x3 block (a I + K) d = g exactly, K = sum of the contacts'
+        // k n n^T (the per-axis diagonal k n_c^2 also pins the directions
+        // tangent to an oblique contact), with no Chebyshev extrapolation
+        // (that of the stiff penalty direction kept the contact rim from
+        // converging).  Adjugate inverse of the SPD block.
+        const float a = dg + dc;
+        const float gx = dc * sx + (fx + cfx), gy = dc * sy + (fy + cfy), gz = dc * sz + (fz + cfz);
+        const float m00 = a + cdx, m11 = a + cdy, m22 = a + cdz;
+        const float c00 = m11 * m22 - cyz * cyz, c01 = cxz * cyz - cxy * m22, c02 = cxy * cyz - cxz * m11;
+        const float c11 = m00 * m22 - cxz * cxz, c12 = cxy * cxz - m00 * cyz, c22 = m00 * m11 - cxy * cxy;
+        const float det = m00 * c00 + cxy * c01 + cxz * c02;
+        const float dx = (c00 * gx + c01 * gy + c02 * gz) / det;
+        const float dy = (c01 * gx + c11 * gy + c12 * gz) / det;
+        const float dz = (c02 * gx + c12 * gy + c22 * gz) / det;
+        const float bx = (dx * fix + p.x - p.x) * relax + p.x;
+        const float by = (dy * fix + p.y - p.y) * relax + p.y;
+        const float bz = (dz * fix + p.z - p.z) * relax + p.z;
+        xprev[i] = p;
+        x[i] = make_float4(bx, by, bz, 0.f);
+        return;
+    }
     float nx = (dc * sx + (fx + cfx)) / (dg + cdx + dc) * fix + p.x;
     float ny = (dc * sy + (fy + cfy)) / (dg + cdy + dc) * fix + p.y;
     float nz = (dc * sz + (fz + cfz)) / (dg + cdz + dc) * fix + p.z;
     nx = (nx - p.x) * relax + p.x;
     ny = (ny - p.y) * relax + p.y;
     nz = (nz - p.z) * relax + p.z;
-    // V-cycle sweeps: the Chebyshev extrapolation of a contact vertex's stiff
-    // penalty direction keeps the contact rim from converging; those vertices
-    // take the plain Jacobi step
-    const float om = ((flags & kVertContactJacobi) && (cdx + cdy + cdz) > 0.f) ? 1.f : omega;
-    nx = om * (nx - pp.x) + pp.x;
-    ny = om * (ny - pp.y) + pp.y;
-    nz = om * (nz - pp.z) + pp.z;
+    nx = omega * (nx - pp.x) + pp.x;
+    ny = omega * (ny - pp.y) + pp.y;
+    nz = omega * (nz - pp.z) + pp.z;
     xprev[i] = p;
     x[i] = make_float4(nx, ny, nz, 0.f);
 }

@@ -240,6 +240,7 @@ typedef struct orc_level {
     real *x0, *x, *v, *fext, *xlast, *s, *xprev;
     real *B, *AC, *vol, *mass, *diag, *fix, *center, *faceN, *faceA;
     real *force, *cdiag, *cforce, *cenergy, *cdepth;
+    real* coff;      /* off-diagonals (xy, xz, yz) of the contacts' k n n^T  */
     int* cflag;
     real* cf;        /* per tet 12: corner j component i at [t*12 + j*3 + i] */
     int *rp, *ri;    /* vertex -> incident (t*4+c), t ascending               */
@@ -341,7 +342,7 @@ orc_level* PFX(level_create)(int V, int T, int S, const float* pos, const int* t
     if (S) memcpy(L->tris, tris, sizeof(unsigned) * 3 * (size_t)S);
     size_t v3n = (size_t)V * 3;
     real** vec3s[] = {&L->x0, &L->x, &L->v, &L->fext, &L->xlast, &L->s, &L->xprev, &L->force, &L->cdiag,
-                      &L->cforce, &L->vnormal};
+                      &L->cforce, &L->vnormal, &L->coff};
     for (size_t q = 0; q < sizeof(vec3s) / sizeof(vec3s[0]); q++) *vec3s[q] = (real*)xcalloc(v3n, sizeof(real));
     L->mass = (real*)xcalloc(V, sizeof(real));
     L->diag = (real*)xcalloc(V, sizeof(real));
@@ -371,7 +372,7 @@ void PFX(level_free)(orc_level* L) {
     void* ps[] = {L->tets, L->tris, L->x0, L->x, L->v, L->fext, L->xlast, L->s, L->xprev, L->B, L->AC, L->vol,
                   L->mass, L->diag, L->fix, L->center, L->faceN, L->faceA, L->force, L->cdiag, L->cforce,
                   L->cenergy, L->cdepth, L->cflag, L->cf, L->rp, L->ri, L->trp, L->tri, L->tc, L->trin,
-                  L->vnormal, L->tk};
+                  L->vnormal, L->tk, L->coff};
     for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
     free(L);
 }
@@ -557,6 +558,7 @@ static void gather_forces(orc_level* L) {
 /* runClearCollision (PDSolver.cu:691-698). */
 static void clear_collision(orc_level* L) {
     memset(L->cdiag, 0, sizeof(real) * 3 * (size_t)L->V);
+    memset(L->coff, 0, sizeof(real) * 3 * (size_t)L->V);
     memset(L->cforce, 0, sizeof(real) * 3 * (size_t)L->V);
     memset(L->cenergy, 0, sizeof(real) * (size_t)L->V);
     memset(L->cdepth, 0, sizeof(real) * (size_t)L->V);
@@ -590,6 +592,9 @@ static void dcd_point_sphere(orc_level* L, v3 center, real radius, real kc) {
             L->cdiag[i * 3 + 0] += kc * n.x * n.x;
             L->cdiag[i * 3 + 1] += kc * n.y * n.y;
             L->cdiag[i * 3 + 2] += kc * n.z * n.z;
+            L->coff[i * 3 + 0] += kc * n.x * n.y;
+            L->coff[i * 3 + 1] += kc * n.x * n.z;
+            L->coff[i * 3 + 2] += kc * n.y * n.z;
             L->cforce[i * 3 + 0] += kc * t * nd * n.x;
             L->cforce[i * 3 + 1] += kc * t * nd * n.y;
             L->cforce[i * 3 + 2] += kc * t * nd * n.z;
@@ -614,6 +619,9 @@ static void dcd_point_plane(orc_level* L, v3 n, real d, real kp) {
             L->cdiag[i * 3 + 0] += kp * n.x * n.x;
             L->cdiag[i * 3 + 1] += kp * n.y * n.y;
             L->cdiag[i * 3 + 2] += kp * n.z * n.z;
+            L->coff[i * 3 + 0] += kp * n.x * n.y;
+            L->coff[i * 3 + 1] += kp * n.x * n.z;
+            L->coff[i * 3 + 2] += kp * n.y * n.z;
             L->cforce[i * 3 + 0] += kp * depth * n.x;
             L->cforce[i * 3 + 1] += kp * depth * n.y;
             L->cforce[i * 3 + 2] += kp * depth * n.z;
@@ -752,8 +760,9 @@ static void dcd_triangle(orc_level* L, const orc_params* p) {
 /* calculatePOS (PDSolver.cu:192-235).  The 0.6 under-relaxation is a double
  * literal in the reference, so the FP32 oracle evaluates that line in double. */
 /* contact_jacobi (V-cycle only, not the reference): vertices a collider acts
- * on take the plain Jacobi step (omega 1); the Chebyshev extrapolation of
- * their stiff penalty direction keeps the contact rim from converging. */
+ * on take an exact 3x3 block Jacobi step without Chebyshev extrapolation;
+ * the extrapolation of their stiff penalty direction kept the contact rim
+ * from converging, and the per-axis diagonal pinned their tangential motion. */
 static void calc_pos(orc_level* L, const orc_params* p, real omega, int contact_jacobi) {
     real dt = p->dt;
     double relax = p->relax;
@@ -761,15 +770,41 @@ static void calc_pos(orc_level* L, const orc_params* p, real omega, int contact_
     for (int i = 0; i < L->V; i++) {
         real fix = L->fix[i];
         real dc = L->mass[i] / (dt * dt);
+        const real* cd = L->cdiag + i * 3;
+        if (contact_jacobi && cd[0] + cd[1] + cd[2] > 0) {
+            /* V-cycle sweeps, a vertex a collider acts on: the Jacobi step
+             * solves its 3x3 block (a I + K) d = g exactly, K = sum of the
+             * contacts' k n n^T (the per-axis diagonal k n_c^2 of calculatePOS
+             * also pins the directions tangent to an oblique contact), with
+             * no Chebyshev extrapolation.  Adjugate inverse of the SPD block. */
+            const real* co = L->coff + i * 3;
+            real a = L->diag[i] + dc;
+            real g[3];
+            for (int c = 0; c < 3; c++)
+                g[c] = dc * (L->s[i * 3 + c] - L->x[i * 3 + c]) + (L->force[i * 3 + c] + L->cforce[i * 3 + c]);
+            real m00 = a + cd[0], m11 = a + cd[1], m22 = a + cd[2], m01 = co[0], m02 = co[1], m12 = co[2];
+            real c00 = m11 * m22 - m12 * m12, c01 = m02 * m12 - m01 * m22, c02 = m01 * m12 - m02 * m11;
+            real c11 = m00 * m22 - m02 * m02, c12 = m01 * m02 - m00 * m12, c22 = m00 * m11 - m01 * m01;
+            real det = m00 * c00 + m01 * c01 + m02 * c02;
+            real d[3] = {(c00 * g[0] + c01 * g[1] + c02 * g[2]) / det, (c01 * g[0] + c11 * g[1] + c12 * g[2]) / det,
+                         (c02 * g[0] + c12 * g[1] + c22 * g[2]) / det};
+            for (int c = 0; c < 3; c++) {
+                int q = i * 3 + c;
+                real pos = L->x[q];
+                real next = d[c] * fix + pos;
+                next = (real)((double)(next - pos) * relax + (double)pos);
+                L->xprev[q] = pos;
+                L->x[q] = next;
+            }
+            continue;
+        }
         for (int c = 0; c < 3; c++) {
             int q = i * 3 + c;
             real pos = L->x[q];
             real elem = L->force[q] + L->cforce[q];
             real next = (dc * (L->s[q] - pos) + elem) / (L->diag[i] + L->cdiag[q] + dc) * fix + pos;
             next = (real)((double)(next - pos) * relax + (double)pos);
-            const real om =
-                contact_jacobi && (L->cdiag[i * 3] + L->cdiag[i * 3 + 1] + L->cdiag[i * 3 + 2]) > 0 ? 1 : omega;
-            next = om * (next - L->xprev[q]) + L->xprev[q];
+            next = omega * (next - L->xprev[q]) + L->xprev[q];
             L->xprev[q] = pos;
             L->x[q] = next;
         }
