@@ -67,6 +67,8 @@ _SIGS = {
     "level_set_fix": (None, [_vp, _fp]),
     "level_set_velocity": (None, [_vp, _fp]),
     "level_set_positions": (None, [_vp, _fp]),
+    "level_set_s": (None, [_vp, _fp]),
+    "sweep": (None, [_vp, C.POINTER(orc_params), C.c_double, C.c_int]),
     "level_set_tet_k": (None, [_vp, _fp]),
     "level_get": (C.c_int, [_vp, C.c_int, _dp]),
     "step_single": (None, [_vp, C.POINTER(orc_params), C.c_int]),
@@ -178,6 +180,15 @@ class Level:
     def set_positions(self, x):
         a = np.ascontiguousarray(x, np.float32).reshape(-1)
         self._f("level_set_positions")(self._h, a.ctypes.data_as(_fp))
+
+    def set_s(self, s):
+        a = np.ascontiguousarray(s, np.float32).reshape(-1)
+        self._f("level_set_s")(self._h, a.ctypes.data_as(_fp))
+
+    def sweep(self, p, omega=1.0, contact_block=False):
+        """One PD sweep (point DCD, local step, gather, Jacobi step); with
+        contact_block the V-cycle's exact 3x3 step on contact vertices."""
+        self._f("sweep")(self._h, C.byref(p), float(omega), int(contact_block))
 
     def set_tet_stiffness(self, k):
         """Per-tet ARAP weight per unit volume (bodies with their own E)."""

@@ -411,6 +411,9 @@ void PFX(level_set_velocity)(orc_level* L, const float* v) {
 void PFX(level_set_positions)(orc_level* L, const float* x) {
     for (int i = 0; i < L->V * 3; i++) L->x[i] = (real)x[i];
 }
+void PFX(level_set_s)(orc_level* L, const float* s) {
+    for (int i = 0; i < L->V * 3; i++) L->s[i] = (real)s[i];
+}
 
 /* which: 0 x, 1 v, 2 s, 3 xprev, 4 xlast, 5 mass, 6 diag, 7 vol, 8 B(9T), 9 AC(12T),
  *        10 cdiag, 11 cforce, 12 cflag(as real), 13 cdepth, 14 cenergy, 15 vnormal,
@@ -1149,6 +1152,13 @@ float PFX(inverse3_f)(const float* A, float* R) {
 void PFX(tet_corner_forces)(orc_level* L, int mode, double* out) {
     tet_forces(L, mode);
     for (size_t i = 0; i < (size_t)L->T * 12; i++) out[i] = (double)L->cf[i];
+}
+
+/* One PD sweep on the current state: point DCD, local step, gather and the
+ * Jacobi step (contact_jacobi: the V-cycle's block step on contact
+ * vertices). */
+void PFX(sweep)(orc_level* L, const orc_params* p, double omega, int contact_jacobi) {
+    pd_iteration(L, p, (real)omega, 0, contact_jacobi);
 }
 
 /* One collision pass on the current positions (point DCD or triangle DCD). */

@@ -1,5 +1,5 @@
 """Closed-form pins of the oracle's contact, normal, energy and active-set
-functions (oracle/mgsb_oracle.c: dcd_point_plane, dcd_tri_sphere +
+functions (oracle/mgsb_oracle.c: dcd_point_plane, calc_pos's contact step, dcd_tri_sphere +
 gather_tri_collision, tri_normals / vertex_normals, energy, ml_fine_residual's
 active-set mask).  Every expected value comes from geometry fixed by
 construction (hand-placed spheres and planes, axis-aligned faces, uniform
@@ -235,3 +235,64 @@ def test_active_set_mask_is_where_a_collider_acts():
     want = np.where(in_sphere | below, 0.0, 1.0)
     assert in_sphere.sum() > 0 and below.sum() > 0
     assert np.array_equal(mask, want)
+
+
+RELAX = float(np.float32(0.6))  # the under-relaxation, an FP32 parameter
+
+
+def _contact_sweep(block):
+    """One sweep of the unit-scale tet at rest with s offset from x, a sphere
+    whose inflated radius (1.5 r) holds vertex A only; omega 1."""
+    L = O.Level(_TET_POS, _TET, None, fp64=True)
+    L.set_fix(np.ones(4, np.float32))
+    x = _TET_POS.astype(np.float32)
+    off = np.array([[0.01, -0.02, 0.015], [0.0, 0.005, 0.0], [-0.004, 0.0, 0.0], [0.0, 0.0, 0.003]], np.float32)
+    L.set_positions(x)
+    L.set_s(x + off)
+    c = np.array([-0.6, -0.8, -1.0])
+    r, k = 1.2, 1e4
+    p = O.params(kcol=k, spheres=[(*c, r, 1, 0, 0, 0)], if_mode=3)
+    L.sweep(p, omega=1.0, contact_block=block)
+    x = x.astype(np.float64)
+    return L, p, x, L.get(O.S).reshape(-1, 3) - x, c, k  # s - x as stored (FP32 inputs)
+
+
+def test_contact_block_step_solves_the_vertex_block():
+    """The V-cycle's Jacobi step on a vertex a collider acts on solves the
+    3x3 block (a I + k n n^T) d = g, with a = elastic diagonal + m/dt^2 and
+    n the contact normal, for a sphere unit(p - c) by construction (the
+    contact point lies on the ray from the centre through p).  At rest there
+    is no elastic force, so g = m/dt^2 (s - x) + f_col; the offset s - x
+    gives g parts tangent to the sphere.  Free vertices away from the sphere
+    take the plain Jacobi step g / a; every step is under-relaxed by 0.6.
+    Tolerance 1e-6: the sphere and dt are FP32 parameters of the FP64 oracle."""
+    L, p, x, off, c, k = _contact_sweep(True)
+    xn = L.get(O.X).reshape(-1, 3)
+    dc = L.get(O.MASS) / (p.dt * p.dt)
+    a = L.get(O.DIAG) + dc
+    fcol = L.get(O.CFORCE).reshape(-1, 3)
+    assert np.abs(fcol[1:]).max() == 0 and np.abs(fcol[0]).max() > 0  # only A is in contact
+    g = dc[:, None] * off + fcol
+    n = (x[0] - c) / np.linalg.norm(x[0] - c)
+    d0 = np.linalg.solve(a[0] * np.eye(3) + k * np.outer(n, n), g[0])
+    assert np.allclose(xn[0] - x[0], RELAX * d0, rtol=1e-6, atol=1e-15)
+    for i in range(1, 4):
+        assert np.allclose(xn[i] - x[i], RELAX * g[i] / a[i], rtol=1e-6, atol=1e-15)
+    # the tangential part of g moves A as if the sphere were not there
+    t = g[0] - (g[0] @ n) * n
+    assert np.isclose((xn[0] - x[0]) @ t / np.linalg.norm(t), RELAX * np.linalg.norm(t) / a[0], rtol=1e-6)
+
+
+def test_reference_step_divides_each_axis_by_its_diagonal():
+    """calculatePOS (the reference schedules): each component of a contact
+    vertex's step is g_c / (a + k n_c^2).  For the oblique normal here that
+    also damps the tangential part, which the block step does not."""
+    L, p, x, off, c, k = _contact_sweep(False)
+    xn = L.get(O.X).reshape(-1, 3)
+    dc = L.get(O.MASS) / (p.dt * p.dt)
+    a = L.get(O.DIAG) + dc
+    g = dc[:, None] * off + L.get(O.CFORCE).reshape(-1, 3)
+    n = (x[0] - c) / np.linalg.norm(x[0] - c)
+    assert np.allclose(xn[0] - x[0], RELAX * g[0] / (a[0] + k * n * n), rtol=1e-6)
+    Lb = _contact_sweep(True)[0]
+    assert not np.allclose(Lb.get(O.X).reshape(-1, 3)[0], xn[0], rtol=1e-3)
