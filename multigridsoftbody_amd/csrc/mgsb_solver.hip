@@ -1746,8 +1746,9 @@ struct mgsb_solver {
     // half, finish).  The host launches the start, the halves the previous
     // step needed and the finish as one graph, then reads the convergence
     // mirror and adds halves (and a finish) only when the step needs them.
-    // Each distinct piece sequence is captured once: a graph launch costs
-    // about 5 us of idle GPU against about 1 us between kernels inside one.
+    // Each distinct piece sequence is captured once.  Fewer, longer graphs:
+    // on MI355X a dependent kernel inside a graph costs about 2 us, while
+    // back-to-back launches of a one-kernel graph run at about 16 us each.
     struct PieceGraph {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
