@@ -1410,10 +1410,53 @@ __device__ __forceinline__ void wave_sum3(double& a0, double& a1, double& a2) {
 // memory when the rows do not fit); rowinfo[row] = (first row of the block,
 // block size, offset of W_b).  One wave per row with the lanes across the
 // row (W_b and Wt_b = W_b^T both row-major), so all loads of a row are in
-// flight at once; each row is summed by a fixed shuffle tree.
+// flight at once; each row is summed by a fixed shuffle tree.  When no block
+// has more than 64 rows (many small bodies) one LANE takes each row instead
+// and sums it in index order: the rows of a 64-body batch then take three
+// passes of the workgroup instead of 160 dependent row loads per wave.
+constexpr int kLaneRowMax = 64;
 __device__ __forceinline__ void dense_solve_block(int m, const int4* __restrict__ rowinfo,
                                                   const double* __restrict__ W, const double* __restrict__ Wt,
-                                                  double* rb, double* yb, int ld) {
+                                                  double* rb, double* yb, int ld, int maxblk) {
+    if (maxblk <= kLaneRowMax) {
+        for (int row = threadIdx.x; row < m; row += blockDim.x) {
+            const int4 ri = rowinfo[row];
+            const int i = row - ri.x;
+            const double* wr = W + ri.z + (size_t)i * ri.y;
+            const double* r0 = rb + ri.x;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll 8
+            for (int k = 0; k <= i; k++) {
+                const double w = wr[k];
+                a0 += w * r0[k];
+                a1 += w * r0[ld + k];
+                a2 += w * r0[2 * ld + k];
+            }
+            yb[row] = a0;
+            yb[ld + row] = a1;
+            yb[2 * ld + row] = a2;
+        }
+        __syncthreads();
+        for (int row = threadIdx.x; row < m; row += blockDim.x) {
+            const int4 ri = rowinfo[row];
+            const int i = row - ri.x;
+            const double* wr = Wt + ri.z + (size_t)i * ri.y;
+            const double* y0 = yb + ri.x;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll 8
+            for (int k = i; k < ri.y; k++) {
+                const double w = wr[k];
+                a0 += w * y0[k];
+                a1 += w * y0[ld + k];
+                a2 += w * y0[2 * ld + k];
+            }
+            rb[row] = a0;
+            rb[ld + row] = a1;
+            rb[2 * ld + row] = a2;
+        }
+        __syncthreads();
+        return;
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     for (int row = wave; row < m; row += nw) {
         const int4 ri = rowinfo[row];
@@ -1461,7 +1504,7 @@ __device__ __forceinline__ void dense_solve_block(int m, const int4* __restrict_
 __device__ __forceinline__ void coarsest_direct(int V, int n, const int* __restrict__ dense,
                                                 const int4* __restrict__ rowinfo, const double* __restrict__ W,
                                                 const double* __restrict__ Wt, const float4* __restrict__ r,
-                                                float4* __restrict__ e, double* big) {
+                                                float4* __restrict__ e, double* big, int maxblk) {
     __shared__ double lds[6 * kMaxDense];
     double* rb = n <= kMaxDense ? lds : big;
     const int ld = n <= kMaxDense ? kMaxDense : n;
@@ -1476,7 +1519,7 @@ __device__ __forceinline__ void coarsest_direct(int V, int n, const int* __restr
         }
     }
     __syncthreads();
-    dense_solve_block(n, rowinfo, W, Wt, rb, yb, ld);
+    dense_solve_block(n, rowinfo, W, Wt, rb, yb, ld, maxblk);
     for (int i = threadIdx.x; i < V; i += blockDim.x) {
         const int d = dense[i];
         e[i] = d >= 0 ? make_float4((float)rb[d], (float)rb[ld + d], (float)rb[2 * ld + d], 0.f)
@@ -1489,10 +1532,10 @@ __device__ __forceinline__ void coarsest_direct(int V, int n, const int* __restr
 __global__ void __launch_bounds__(256) k_coarse_solve(int Vc, int n, const int* __restrict__ dense,
                                                       const int4* __restrict__ rowinfo, const double* __restrict__ W,
                                                       const double* __restrict__ Wt, const float4* __restrict__ r,
-                                                      float4* __restrict__ e, double* big,
+                                                      float4* __restrict__ e, double* big, int maxblk,
                                                       const int* __restrict__ skip) {
     if (skip && *skip) return;
-    coarsest_direct(Vc, n, dense, rowinfo, W, Wt, r, e, big);
+    coarsest_direct(Vc, n, dense, rowinfo, W, Wt, r, e, big, maxblk);
 }
 
 // Small coarse levels of the linear V-cycle in ONE workgroup: level 0 of the
@@ -1503,6 +1546,7 @@ __global__ void __launch_bounds__(256) k_coarse_solve(int Vc, int n, const int* 
 // the first level's correction is in e0.
 struct TailLevel {
     int V, nfree;
+    int maxblk;  // largest block of the coarsest factor
     const int *ap, *aj;
     const float* av;
     const float4* vcl;
@@ -1605,7 +1649,7 @@ __global__ void __launch_bounds__(1024) k_coarse_tail(const TailLevel* __restric
     {
         const TailLevel& L = tl[n - 1];
         if (direct && L.nfree >= 0) {
-            coarsest_direct(L.V, L.nfree, L.dense, L.rowinfo, L.W, L.Wt, L.r, L.e0, L.big);
+            coarsest_direct(L.V, L.nfree, L.dense, L.rowinfo, L.W, L.Wt, L.r, L.e0, L.big, L.maxblk);
             flips &= ~(1u << (n - 1));
             __syncthreads();
         } else {
@@ -1696,6 +1740,7 @@ struct DevLevel {
     double* big = nullptr;               // dense-vector scratch (6 n) when n > kMaxDense
     float* tk = nullptr;                 // per-tet stiffness (nullptr: p.stiffness)
     int nfree = -1;
+    int maxblk = 0;                      // largest block of the coarsest factor (rows)
     std::vector<float> vfix;
     double rho = 0.0;
     std::vector<void*> owned;
@@ -2120,7 +2165,7 @@ int vcycle_lin(mgsb_solver* s, int l, const int* skip) {
     if (l == nl - 1) {
         if (s->p.coarse_direct && d.nfree >= 0) {
             k_coarse_solve<<<1, 256, 0, s->stream>>>(d.V, d.nfree, d.dense, d.rowinfo, d.W, d.Wt, d.r, d.e, d.big,
-                                                     skip);
+                                                     d.maxblk, skip);
             HIPCHK(hipGetLastError());
             return MGSB_OK;
         }
@@ -2548,6 +2593,8 @@ int setup_vcycle(mgsb_solver* s) {
         HIPCHK(hipMemcpyAsync(cc.dense, dense.data(), sizeof(int) * dense.size(), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
         cc.nfree = n;
+        cc.maxblk = 0;
+        for (int r = 0; r < n; r++) cc.maxblk = std::max(cc.maxblk, (int)rowinfo[(size_t)r * 4 + 1]);
     }
     if (s->tail_from >= 1) {
         std::vector<TailLevel> tl;
@@ -2556,6 +2603,7 @@ int setup_vcycle(mgsb_solver* s) {
             TailLevel t{};
             t.V = d.V;
             t.nfree = d.nfree;
+            t.maxblk = d.maxblk;
             t.ap = d.ap;
             t.aj = d.aj;
             t.av = d.av;
