@@ -1023,7 +1023,8 @@ __global__ void __launch_bounds__(256) k_resid_dual(int V, const float4* __restr
 // order, forms the largest per-body relative residual (res0b holds the
 // per-body norms of g(s), written at the step's start check) and applies the
 // check through vstate_apply.
-__global__ void __launch_bounds__(256) k_body_check(const int* __restrict__ vb, const double* __restrict__ g2,
+constexpr int kBodyCheckThreads = 1024;
+__global__ void __launch_bounds__(kBodyCheckThreads) k_body_check(const int* __restrict__ vb, const double* __restrict__ g2,
                                                     double* __restrict__ bodysq, double* __restrict__ res0b,
                                                     unsigned* count, VState* st, int mode, int* hm,
                                                     const int* __restrict__ skip, unsigned* seqc) {
@@ -1031,11 +1032,21 @@ __global__ void __launch_bounds__(256) k_body_check(const int* __restrict__ vb, 
         if (blockIdx.x == 0 && threadIdx.x == 0) bump_seq(seqc, hm);
         return;
     }
-    __shared__ double sh[256];
+    __shared__ double sh[kBodyCheckThreads];
     __shared__ int s_last;
     const int b = blockIdx.x;
+    // a body has up to ~50k vertices: eight loads in flight per lane
+    const int e1 = vb[b + 1], stride = blockDim.x;
     double a = 0.0;
-    for (int i = vb[b] + threadIdx.x; i < vb[b + 1]; i += blockDim.x) a += g2[i];
+    int i = vb[b] + threadIdx.x;
+    for (; i + 7 * stride < e1; i += 8 * stride) {
+        double q[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) q[k] = g2[i + k * stride];
+#pragma unroll
+        for (int k = 0; k < 8; k++) a += q[k];
+    }
+    for (; i < e1; i += stride) a += g2[i];
     sh[threadIdx.x] = a;
     __syncthreads();
     for (int w = blockDim.x >> 1; w > 0; w >>= 1) {
@@ -2200,8 +2211,8 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool res
                                            write_r ? d.r : nullptr, nullptr, restart ? d.xprev : nullptr, skip, 1,
                                            s->g2, d.cmask, lsh);
         HIPCHK(hipGetLastError());
-        k_body_check<<<s->nbodies, 256, 0, s->stream>>>(s->vb, s->g2, s->bodysq, s->res0b, s->bcount, s->vstate, chk,
-                                                        s->hmirror, skip, s->seqc);
+        k_body_check<<<s->nbodies, kBodyCheckThreads, 0, s->stream>>>(s->vb, s->g2, s->bodysq, s->res0b, s->bcount,
+                                                                      s->vstate, chk, s->hmirror, skip, s->seqc);
         s->chk_launched++;
         HIPCHK(hipGetLastError());
         return MGSB_OK;
