@@ -772,12 +772,13 @@ __device__ void vstate_apply(VState* st, double t, int mode, int* hm, double rel
         st->cf_ok = 1;
         st->converged = (st->check && r == 0.0) ? 1 : 0;
     } else if (mode == kChkTemporal || mode == kChkWarm) {
-        // a warm-start stage is kept only if it lowers the best residual so far
+        // a warm-start stage is kept only if it lowers the best residual so
+        // far; a non-finite residual (a diverged network) is never kept
         if (mode == kChkTemporal)
             st->res_t = r;
         else
             st->res_ws = r;
-        if (r > st->res) {
+        if (!(r <= st->res)) {
             st->revert = st->accepted == 1 ? 2 : 1;
             st->fallback |= mode == kChkTemporal ? 1 : 2;
             st->cf_ok = 0;

@@ -1587,14 +1587,15 @@ void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nstep
         M->res_t = 0.0;
         int conv = vp->check && M->res0 == 0.0;
         /* warm start in two stages, each kept only if it lowers the best
-         * residual so far: x = s + fix u0 (temporal), then x += P e1 (learned) */
+         * residual so far: x = s + fix u0 (temporal), then x += P e1 (learned);
+         * a non-finite residual (a diverged network) is never kept */
         int accepted = 0;
         if (M->has_warm0 && !conv) {
             for (int i = 0; i < L0->V; i++)
                 for (int c = 0; c < 3; c++) L0->x[i * 3 + c] = L0->x[i * 3 + c] + L0->fix[i] * M->warm0[i * 3 + c];
             double rt = ml_fine_residual(M, p);
             M->res_t = rt;
-            if (rt > M->res) {
+            if (!(rt <= M->res)) {
                 M->fallback |= 1;
                 memcpy(L0->x, L0->s, sizeof(real) * 3 * (size_t)L0->V);
                 ml_fine_residual(M, p);
@@ -1608,7 +1609,7 @@ void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nstep
             ml_prolong_add(M, 0, M->warm, L0->x, ml_fine_mask(M));
             double rw = ml_fine_residual(M, p);
             M->res_ws = rw;
-            if (rw > M->res) {
+            if (!(rw <= M->res)) {
                 M->fallback |= 2;
                 memcpy(L0->x, L0->s, sizeof(real) * 3 * (size_t)L0->V);
                 if (accepted == 1)

@@ -244,3 +244,25 @@ def test_learned_ab_medium_box():
     # on its own (start from s) the network's coarse correction saves
     # half-cycles: steps stop after the pre-smoothing half of the second cycle
     assert res["learned_s"] < res["plain"] - 0.5
+
+
+def test_learn_gate_bounds_rejected_stages():
+    """A network whose every correction the judge rejects (weights far too
+    large: the prediction overflows): each step still meets the target from
+    the temporal start, nothing non-finite is kept, and the gate runs the MLP
+    stage only in shrinking windows (8 misses, then shut for 8, 16, 32, ...
+    steps), against every step with the gate off."""
+    res = {}
+    for gate in (1, 0):
+        s = small_solver(M.SCHED_LEARNED, check_every=2, learn_gate=gate)
+        s.learn_weights(rand_weights(s.learn_nparams(), seed=5, scale=50.0))
+        s.step(120)
+        run = s.run_stats()
+        x = s.positions()
+        print(f"[gate {gate}] steps {run['steps']} rejected stages {run['fallbacks']} "
+              f"half-cycles/step {run['total_halves'] / run['steps']:.2f} max rel {run['max_rel']:.2e}")
+        assert np.all(np.isfinite(x))
+        assert run["failed_steps"] == 0 and run["max_rel"] <= 1e-3
+        res[gate] = run["fallbacks"]
+    # shut windows after 8 misses: at most 8 + 8 + 8 + 8 (+ renewal probes) in 120 steps
+    assert res[1] <= 45 and res[0] >= 60 and res[0] >= 2.5 * res[1]
