@@ -813,7 +813,8 @@ __device__ void vstate_apply(VState* st, double t, int mode, int* hm, double rel
 
 // End of a step: fold the step's outcome into the run aggregates.
 __device__ void vstate_step_done(VState* st) {
-    const double rel = st->rel;
+    // a non-finite residual counts as an infinitely bad step, not as none
+    const double rel = st->rel == st->rel ? st->rel : __builtin_huge_val();
     if (rel > st->max_rel) st->max_rel = rel;
     st->total_cycles += st->cycles;
     st->total_halves += st->halves;
