@@ -99,11 +99,12 @@ typedef struct mgsb_params {
                            * lowers the residual).  The learned schedule uses the
                            * temporal stage for 0 and 1; 2 runs its MLP stage on s
                            * alone (no temporal stage, the network's own gain).   */
-    int32_t learn_gate;   /* learned schedule: after 8 consecutive steps whose learned
-                           * correction was rejected, or kept without saving a half-cycle
-                           * against the last step run without it (one such step in 64
-                           * renews that count), skip the MLP stage for 8, 16, ... 256
-                           * steps before trying it again (1, default) or never (0)    */
+    int32_t learn_gate;   /* learned schedule (1, default; 0: the stage runs every step):
+                           * skip the MLP stage after a step that needed at most one
+                           * half-cycle (nothing left for it to save), and after 8
+                           * consecutive steps whose learned correction was rejected, or
+                           * kept without saving a half-cycle against the last step run
+                           * without it, for 8, 16, ... 256 steps before trying again */
     int32_t vert_lanes;   /* lanes that gather one vertex's row of corner forces in the
                            * sweep and residual kernels: 1, 2, 4 or 8 (0: 4; tuned)   */
     int32_t reserved[1];

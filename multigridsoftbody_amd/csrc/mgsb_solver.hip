@@ -1809,10 +1809,11 @@ struct mgsb_solver {
     };
     std::map<std::string, PieceGraph> pseq;
     int halves_guess = 4;
-    // learned schedule: after kGateAfter consecutive steps whose learned
-    // correction the judge rejected, or kept while the step still needed as
-    // many half-cycles as the last step run without it, steps start without
-    // the MLP stage for gate_skip steps (doubling up to kGateMax after each
+    // learned schedule: a step after one that needed at most one half-cycle
+    // starts without the MLP stage; after kGateAfter consecutive steps whose
+    // learned correction the judge rejected, or kept while the step still
+    // needed as many half-cycles as the last step run without it, steps start
+    // without it for gate_skip steps (doubling up to kGateMax after each
     // further such run); every kGateProbe steps with the stage, one step runs
     // without it to renew that baseline
     int gate_rejects = 0, gate_skip = 0, gate_left = 0;
@@ -2442,14 +2443,23 @@ int wait_checks(mgsb_solver* s) {
 int step_vcycle(mgsb_solver* s, bool graphable) {
     int rc;
     const bool learned = s->p.schedule == MGSB_SCHED_LEARNED;
-    const bool gated = learned && s->p.learn_gate && !s->recording && s->gate_left > 0;
-    if (gated) s->gate_left--;
+    const bool gate_on = learned && s->p.learn_gate && !s->recording;
+    const bool shut = gate_on && s->gate_left > 0;
+    if (shut) s->gate_left--;
+    // the MLP stage can only save half-cycles beyond the first: after a step
+    // that needed at most one, this step starts without it
+    const bool gated = shut || (gate_on && s->halves_guess <= 1);
     std::string seq(1, (char)(gated ? kPieceStartGated : kPieceStart));
     const int check = s->p.check_every;
     const int maxh = 2 * s->p.max_cycles;
     // the next step reads this step's coarse delta only if it runs the learned
-    // stage: not under the plain schedules, nor while the gate stays shut
-    const int fin = !s->recording && (!learned || (gated && s->gate_left > 0)) ? kPieceFinishLite : kPieceFinish;
+    // stage: not under the plain schedules, nor while the gate stays shut, nor
+    // after a step of at most one half-cycle (hend: halves by the end of the
+    // round whose finish this is)
+    auto fin_for = [&](int hend) {
+        const bool lite = !s->recording && (!learned || (gate_on && (s->gate_left > 0 || hend <= 1)));
+        return lite ? kPieceFinishLite : kPieceFinish;
+    };
     int h = 0;
     int target = check ? std::min(maxh, std::max(0, s->halves_guess)) : maxh;
     bool finished = false;
@@ -2460,7 +2470,7 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
             target = h + 1;
             continue;
         }
-        seq.push_back((char)fin);  // the finish runs only if the step is done
+        seq.push_back((char)fin_for(h));  // the finish runs only if the step is done
         if ((rc = run_pieces(s, seq, graphable))) return rc;
         seq.clear();
         if ((rc = wait_checks(s))) return rc;
@@ -2495,7 +2505,7 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
                 s->gate_left = 1;
         }
     }
-    if (!finished) seq.push_back((char)fin);
+    if (!finished) seq.push_back((char)fin_for(h));
     if (!seq.empty() && (rc = run_pieces(s, seq, graphable))) return rc;
     if (s->recording) s->rec_steps++;  // the finish enqueued above recorded this step's target
     return MGSB_OK;
