@@ -1546,6 +1546,60 @@ __global__ void __launch_bounds__(256) k_coarse_solve(int Vc, int n, const int* 
     coarsest_direct(Vc, n, dense, rowinfo, W, Wt, r, e, big, maxblk);
 }
 
+// The coarsest solve of many small bodies on its own: one wave per block
+// (blk = first row, rows <= 64, W offset), one lane per row, the same sums in
+// the same order as dense_solve_block's lane path, r and e addressed through
+// the row's vertex.  Pinned vertices' e is zeroed beforehand.  Replaces the
+// one-workgroup solve when the coarsest level is the whole tail: for the
+// 64-body batch that workgroup ran 2,560 rows through device-memory vectors.
+constexpr int kBlockRowsMax = 64;
+__global__ void __launch_bounds__(kBlockRowsMax) k_coarse_solve_blocks(const int4* __restrict__ blk,
+                                                                       const int* __restrict__ rowvert,
+                                                                       const double* __restrict__ W,
+                                                                       const double* __restrict__ Wt,
+                                                                       const float4* __restrict__ r,
+                                                                       float4* __restrict__ e,
+                                                                       const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ double rb[3][kBlockRowsMax], yb[3][kBlockRowsMax];
+    const int4 b = blk[blockIdx.x];
+    const int i = threadIdx.x;
+    const bool on = i < b.y;
+    const int v = on ? rowvert[b.x + i] : 0;
+    if (on) {
+        const float4 q = r[v];
+        rb[0][i] = q.x;
+        rb[1][i] = q.y;
+        rb[2][i] = q.z;
+    }
+    __syncthreads();
+    if (on) {
+        const double* wr = W + b.z + (size_t)i * b.y;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int k = 0; k <= i; k++) {
+            const double w = wr[k];
+            a0 += w * rb[0][k];
+            a1 += w * rb[1][k];
+            a2 += w * rb[2][k];
+        }
+        yb[0][i] = a0;
+        yb[1][i] = a1;
+        yb[2][i] = a2;
+    }
+    __syncthreads();
+    if (on) {
+        const double* wr = Wt + b.z + (size_t)i * b.y;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int k = i; k < b.y; k++) {
+            const double w = wr[k];
+            a0 += w * yb[0][k];
+            a1 += w * yb[1][k];
+            a2 += w * yb[2][k];
+        }
+        e[v] = make_float4((float)a0, (float)a1, (float)a2, 0.f);
+    }
+}
+
 // Small coarse levels of the linear V-cycle in ONE workgroup: level 0 of the
 // descriptor array already holds its right-hand side r; the workgroup runs
 // pre-smoothing, residual, restriction, ... down to the coarsest level, the
@@ -1749,6 +1803,9 @@ struct DevLevel {
     float* tk = nullptr;                 // per-tet stiffness (nullptr: p.stiffness)
     int nfree = -1;
     int maxblk = 0;                      // largest block of the coarsest factor (rows)
+    int4* blk = nullptr;                 // per block: (first row, rows, W offset, 0)
+    int* rowvert = nullptr;              // vertex of each dense row
+    int nblocks = 0;
     std::vector<float> vfix;
     double rho = 0.0;
     std::vector<void*> owned;
@@ -1860,6 +1917,8 @@ struct mgsb_solver {
             if (lv.Wt) (void)hipFree(lv.Wt);
             if (lv.rowinfo) (void)hipFree(lv.rowinfo);
             if (lv.big) (void)hipFree(lv.big);
+            if (lv.blk) (void)hipFree(lv.blk);
+            if (lv.rowvert) (void)hipFree(lv.rowvert);
         }
         for (void* q : owned) (void)hipFree(q);
         if (mlp) mlp_destroy(mlp);
@@ -2163,6 +2222,15 @@ int vcycle_lin(mgsb_solver* s, int l, const int* skip) {
     const int nl = (int)s->L.size();
     const int bs = block_vert(s);
     int rc;
+    // the coarsest level alone, many small blocks: one wave per block
+    if (l == nl - 1 && s->p.coarse_direct && d.nfree >= 0 && d.maxblk <= kBlockRowsMax && d.nblocks > 1) {
+        d.cur = 0;
+        k_zero4<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.e, skip);
+        k_coarse_solve_blocks<<<d.nblocks, kBlockRowsMax, 0, s->stream>>>(d.blk, d.rowvert, d.W, d.Wt, d.r, d.e,
+                                                                          skip);
+        HIPCHK(hipGetLastError());
+        return MGSB_OK;
+    }
     if (l == s->tail_from) {
         k_coarse_tail<<<1, 1024, 0, s->stream>>>(s->tail, nl - l, s->p.coarse_sweeps, s->p.relax_coarse,
                                                  s->p.coarse_direct, skip);
@@ -2195,8 +2263,8 @@ int vcycle_lin(mgsb_solver* s, int l, const int* skip) {
 }
 
 // Fine-level residual on split positions (x holds u = x - s).  first: the
-// step's first evaluation, which also stores F(s).  chk: the check that the
-// residual kernel's last block applies (kChkNone: no norm).
+// step's first evaluation, at s itself.  chk: the check applied to the norm
+// (kChkNone: no norm).
 int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool restart, const int* skip,
                   bool first = false) {
     DevLevel& d = s->L[0];
@@ -2591,6 +2659,11 @@ int setup_vcycle(mgsb_solver* s) {
     }
     if (cc.rowinfo) HIPCHK(hipFree(cc.rowinfo));
     cc.rowinfo = nullptr;
+    if (cc.blk) HIPCHK(hipFree(cc.blk));
+    if (cc.rowvert) HIPCHK(hipFree(cc.rowvert));
+    cc.blk = nullptr;
+    cc.rowvert = nullptr;
+    cc.nblocks = 0;
     if (coarse_cholesky(cc.h, cc.vfix, dt, s->p.stiffness, dense, W, rowinfo, n) == 0 && n > 0 &&
         W.size() <= kMaxDenseEntries) {
         int rc = dalloc(&cc.W, W.size());
@@ -2613,6 +2686,18 @@ int setup_vcycle(mgsb_solver* s) {
         cc.nfree = n;
         cc.maxblk = 0;
         for (int r = 0; r < n; r++) cc.maxblk = std::max(cc.maxblk, (int)rowinfo[(size_t)r * 4 + 1]);
+        // per-block table and row -> vertex map of the one-wave-per-block solve
+        std::vector<int32_t> blk, rowvert(n, 0);
+        for (int r = 0; r < n; r++)
+            if (rowinfo[(size_t)r * 4 + 0] == r)
+                blk.insert(blk.end(), {r, rowinfo[(size_t)r * 4 + 1], rowinfo[(size_t)r * 4 + 2], 0});
+        for (int i = 0; i < cc.V; i++)
+            if (dense[i] >= 0) rowvert[dense[i]] = i;
+        cc.nblocks = (int)(blk.size() / 4);
+        if ((rc = dalloc(&cc.blk, (size_t)cc.nblocks))) return rc;
+        if ((rc = dalloc(&cc.rowvert, (size_t)n))) return rc;
+        HIPCHK(hipMemcpy(cc.blk, blk.data(), sizeof(int32_t) * blk.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(cc.rowvert, rowvert.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     }
     if (s->tail_from >= 1) {
         std::vector<TailLevel> tl;
