@@ -802,13 +802,7 @@ __device__ void vstate_apply(VState* st, double t, int mode, int* hm, double rel
         if (st->check && ok(r)) st->converged = 1;
     }
     st->notdone = (st->check == 0 || st->converged || (st->maxh > 0 && st->halves >= st->maxh)) ? 0 : 1;
-    if (hm) {
-        hm[0] = st->converged;
-        hm[1] = st->cycles;
-        hm[2] = st->halves;
-        hm[3] = st->fallback;
-        hm[4] = st->accepted;
-    }
+    (void)hm;  // the check kernel publishes the verdict with its count (bump_seq)
 }
 
 // End of a step: fold the step's outcome into the run aggregates.
@@ -823,16 +817,22 @@ __device__ void vstate_step_done(VState* st) {
     st->nfallback += st->fallback != 0;
 }
 
-// Every check kernel, skipped or not, counts itself in seqc and mirrors the
-// count to hm[kHmSeq] after a system-scope fence that makes its other mirror
-// writes visible first: the host waits for the count of the checks it has
-// enqueued instead of for the whole stream.
-constexpr int kHmSeq = 5;
-__device__ __forceinline__ void bump_seq(unsigned* seqc, int* hm) {
+// Every check kernel, skipped or not, counts itself in seqc and publishes the
+// count with the step's verdict as ONE 64-bit store to host-mapped memory
+// (low word: the count; high word: converged, fallback bits, accepted stage,
+// half-cycles, cycles).  The host waits for the count of the checks it has
+// enqueued, instead of for the whole stream, and reads the verdict from the
+// same word, so no fence has to order separate mirror writes.
+__device__ __forceinline__ unsigned pack_verdict(const VState* st) {
+    const unsigned h = (unsigned)min(st->halves, 255), c = (unsigned)min(st->cycles, 255);
+    return (st->converged ? 1u : 0u) | ((unsigned)st->fallback & 3u) << 1 | ((unsigned)st->accepted & 3u) << 3 |
+           h << 8 | c << 16;
+}
+__device__ __forceinline__ void bump_seq(unsigned* seqc, int* hm, const VState* st) {
     const unsigned n = atomicAdd(seqc, 1u) + 1u;
     if (hm) {
-        __threadfence_system();
-        __hip_atomic_store(hm + kHmSeq, (int)n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const unsigned long long w = (unsigned long long)pack_verdict(st) << 32 | n;
+        __hip_atomic_store(reinterpret_cast<unsigned long long*>(hm), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -844,7 +844,7 @@ __device__ __forceinline__ void bump_seq(unsigned* seqc, int* hm) {
 __global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict__ part, VState* st, int mode,
                                                int* hm, const int* __restrict__ skip, unsigned* seqc) {
     if (skip && *skip) {
-        if (threadIdx.x == 0) bump_seq(seqc, hm);
+        if (threadIdx.x == 0) bump_seq(seqc, hm, st);
         return;
     }
     __shared__ double sh[256];
@@ -856,7 +856,7 @@ __global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict_
         double t = 0.0;
         for (int q = 0; q < (int)blockDim.x; q++) t += sh[q];
         vstate_apply(st, t, mode, hm);
-        bump_seq(seqc, hm);
+        bump_seq(seqc, hm, st);
     }
 }
 
@@ -886,7 +886,7 @@ __global__ void __launch_bounds__(256) k_check2(int nb, const double* __restrict
             vstate_apply(st, t1, kChkTemporal, hm);
         else
             st->revert = 1;
-        bump_seq(seqc, hm);
+        bump_seq(seqc, hm, st);
     }
 }
 
@@ -1026,7 +1026,7 @@ __global__ void __launch_bounds__(kBodyCheckThreads) k_body_check(const int* __r
                                                     unsigned* count, VState* st, int mode, int* hm,
                                                     const int* __restrict__ skip, unsigned* seqc) {
     if (skip && *skip) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) bump_seq(seqc, hm);
+        if (blockIdx.x == 0 && threadIdx.x == 0) bump_seq(seqc, hm, st);
         return;
     }
     __shared__ double sh[kBodyCheckThreads];
@@ -1073,7 +1073,7 @@ __global__ void __launch_bounds__(kBodyCheckThreads) k_body_check(const int* __r
     }
     *count = 0u;
     vstate_apply(st, tot, mode, hm, worst);
-    bump_seq(seqc, hm);
+    bump_seq(seqc, hm, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1849,7 +1849,7 @@ struct mgsb_solver {
     mgsb_step_stats last{};
     VState* vstate = nullptr;
     double* vpart = nullptr;  // partials of the level-0 residual norm
-    int* hmirror = nullptr;      // host-mapped (converged, cycles, halves, fallback, accepted, check count)
+    int* hmirror = nullptr;      // host-mapped 64-bit word: checks run (low), last verdict (high)
     unsigned* seqc = nullptr;    // device count of check kernels run
     unsigned chk_launched = 0;   // check kernels enqueued (graph replays included)
     // V-cycle steps run as pieces (start, pre-smoothing half, coarse + post
@@ -2488,10 +2488,24 @@ int run_pieces(mgsb_solver* s, const std::string& seq, bool graphable) {
 // Waits until every check kernel enqueued so far has run (the host mirror's
 // count reaches chk_launched), spinning on the coherent mirror; the stream
 // itself may still be busy.  Falls back to the stream's completion.
+unsigned long long mirror_word(const mgsb_solver* s) {
+    return __atomic_load_n(reinterpret_cast<const unsigned long long*>(s->hmirror), __ATOMIC_ACQUIRE);
+}
+// The verdict published with the last check that ran (bump_seq).
+struct Verdict {
+    bool converged;
+    int fallback, accepted, halves, cycles;
+};
+Verdict mirror_verdict(const mgsb_solver* s) {
+    const unsigned v = (unsigned)(mirror_word(s) >> 32);
+    return Verdict{(v & 1u) != 0, (int)((v >> 1) & 3u), (int)((v >> 3) & 3u), (int)((v >> 8) & 255u),
+                   (int)((v >> 16) & 255u)};
+}
+
 int wait_checks(mgsb_solver* s) {
-    const int want = (int)s->chk_launched;
+    const unsigned want = s->chk_launched;
     for (long it = 1;; it++) {
-        if (__atomic_load_n(s->hmirror + kHmSeq, __ATOMIC_ACQUIRE) == want) return MGSB_OK;
+        if ((unsigned)mirror_word(s) == want) return MGSB_OK;
         if ((it & 4095) == 0) {
             const hipError_t q = hipStreamQuery(s->stream);
             if (q == hipSuccess) return MGSB_OK;  // all work done: the mirror is final
@@ -2542,20 +2556,22 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
         if ((rc = run_pieces(s, seq, graphable))) return rc;
         seq.clear();
         if ((rc = wait_checks(s))) return rc;
-        if (s->hmirror[0] || h >= maxh) {
-            s->halves_guess = s->hmirror[0] ? s->hmirror[2] : maxh;
+        const Verdict vd = mirror_verdict(s);
+        if (vd.converged || h >= maxh) {
+            s->halves_guess = vd.converged ? vd.halves : maxh;
             finished = true;
             break;
         }
         target = h + (check == 2 ? 1 : 2);
     }
     if (learned && check && finished && !s->recording) {  // the mirror holds this step's verdicts
-        const int halves = s->hmirror[2];
+        const Verdict vd = mirror_verdict(s);
+        const int halves = vd.halves;
         if (gated) {  // a step without the stage: the baseline it has to beat
             s->base_halves = halves;
             s->learned_run = 0;
         } else {
-            const bool rejected = (s->hmirror[3] & 2) != 0, accepted = s->hmirror[4] == 2;
+            const bool rejected = (vd.fallback & 2) != 0, accepted = vd.accepted == 2;
             // kept, but the step needed as many halves as without the stage
             const bool idle = accepted && s->base_halves >= 0 && halves >= s->base_halves;
             if (rejected || idle) {
