@@ -817,6 +817,28 @@ __device__ void vstate_step_done(VState* st) {
     st->nfallback += st->fallback != 0;
 }
 
+// Workgroup sum of v in a fixed order: a shuffle tree in each wave, then the
+// waves' sums in wave order; thread 0 holds the totals.
+template <int NV>
+__device__ __forceinline__ void wg_sum_fixed(double (&v)[NV]) {
+    __shared__ double sh[NV][16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
+        if (lane == 0) sh[q][wid] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            double t = 0.0;
+            for (int w = 0; w < nw; w++) t += sh[q][w];
+            v[q] = t;
+        }
+}
+
 // Every check kernel, skipped or not, counts itself in seqc and publishes the
 // count with the step's verdict as ONE 64-bit store to host-mapped memory
 // (low word: the count; high word: converged, fallback bits, accepted stage,
@@ -837,8 +859,8 @@ __device__ __forceinline__ void bump_seq(unsigned* seqc, int* hm, const VState* 
 }
 
 // Second stage of the residual-norm reduction: one workgroup sums the nb
-// block partials in block order (the same order on every run) and applies
-// the check.  A separate launch: completing the norm inside the producing
+// block partials in a fixed order (lane-strided, then wg_sum_fixed: the same
+// order on every run) and applies the check.  A separate launch: completing the norm inside the producing
 // kernel (the last block to arrive, found with a device-scope counter) costs
 // far more on this GPU, since the blocks' atomics on one counter serialise.
 __global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict__ part, VState* st, int mode,
@@ -847,15 +869,11 @@ __global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict_
         if (threadIdx.x == 0) bump_seq(seqc, hm, st);
         return;
     }
-    __shared__ double sh[256];
-    double a = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) a += part[b];
-    sh[threadIdx.x] = a;
-    __syncthreads();
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += part[b];
+    wg_sum_fixed<1>(v);
     if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int q = 0; q < (int)blockDim.x; q++) t += sh[q];
-        vstate_apply(st, t, mode, hm);
+        vstate_apply(st, v[0], mode, hm);
         bump_seq(seqc, hm, st);
     }
 }
@@ -866,24 +884,16 @@ __global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict_
 // extrapolation) must return to s, so revert is set without a verdict.
 __global__ void __launch_bounds__(256) k_check2(int nb, const double* __restrict__ part, VState* st, int* hm,
                                                 unsigned* seqc) {
-    __shared__ double sh[2][256];
-    double a = 0.0, b = 0.0;
+    double v[2] = {0.0, 0.0};
     for (int q = threadIdx.x; q < nb; q += blockDim.x) {
-        a += part[2 * q];
-        b += part[2 * q + 1];
+        v[0] += part[2 * q];
+        v[1] += part[2 * q + 1];
     }
-    sh[0][threadIdx.x] = a;
-    sh[1][threadIdx.x] = b;
-    __syncthreads();
+    wg_sum_fixed<2>(v);
     if (threadIdx.x == 0) {
-        double t0 = 0.0, t1 = 0.0;
-        for (int q = 0; q < (int)blockDim.x; q++) {
-            t0 += sh[0][q];
-            t1 += sh[1][q];
-        }
-        vstate_apply(st, t0, kChkStart, hm);
+        vstate_apply(st, v[0], kChkStart, hm);
         if (!st->converged)
-            vstate_apply(st, t1, kChkTemporal, hm);
+            vstate_apply(st, v[1], kChkTemporal, hm);
         else
             st->revert = 1;
         bump_seq(seqc, hm, st);
