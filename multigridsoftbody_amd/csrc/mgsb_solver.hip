@@ -1066,12 +1066,15 @@ __global__ void __launch_bounds__(kBodyCheckThreads) k_body_check(const int* __r
         s_last = atomicAdd(count, 1u) == gridDim.x - 1;
     }
     __syncthreads();
-    if (!s_last || threadIdx.x != 0) return;
+    if (!s_last) return;  // the last workgroup to arrive combines the bodies
     __threadfence();
-    double tot = 0.0, worst = 0.0;
-    for (int q = 0; q < (int)gridDim.x; q++) {
+    // the bodies' norms in parallel (one lane per body, strided), the total in
+    // a fixed order and the largest relative residual
+    double v[1] = {0.0};
+    double worst = 0.0;
+    for (int q = threadIdx.x; q < (int)gridDim.x; q += blockDim.x) {
         const double sq = __hip_atomic_load(bodysq + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tot += sq;
+        v[0] += sq;
         const double r = sqrt(sq);
         if (mode == kChkStart) {
             res0b[q] = r;
@@ -1081,8 +1084,19 @@ __global__ void __launch_bounds__(kBodyCheckThreads) k_body_check(const int* __r
             worst = rel > worst ? rel : worst;
         }
     }
+    wg_sum_fixed<1>(v);
+    __shared__ double s_worst[kBodyCheckThreads / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(worst, off, 64);
+        worst = o > worst ? o : worst;
+    }
+    if ((threadIdx.x & 63) == 0) s_worst[threadIdx.x >> 6] = worst;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) worst = s_worst[w] > worst ? s_worst[w] : worst;
     *count = 0u;
-    vstate_apply(st, tot, mode, hm, worst);
+    vstate_apply(st, v[0], mode, hm, worst);
     bump_seq(seqc, hm, st);
 }
 
