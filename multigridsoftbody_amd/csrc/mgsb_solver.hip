@@ -337,9 +337,8 @@ __device__ __forceinline__ void tet_local_step(int g, int T, const float4* __res
         for (int i = 0; i < 3; i++) {
             float acc = RF[i * 3 + 0] * AC[0 * 4 + j] + RF[i * 3 + 1] * AC[1 * 4 + j] + RF[i * 3 + 2] * AC[2 * 4 + j];
             if (MODE != kTetModeLinear) {
-                if (isnan(acc)) acc = 0.f;
-                acc = acc > 10.f ? 10.f : acc;
-                acc = acc < -10.f ? -10.f : acc;
+                // NaN -> 0, then clamp to [-10, 10] (one v_med3_f32)
+                acc = isnan(acc) ? 0.f : fminf(fmaxf(acc, -10.f), 10.f);
             }
             float r = acc * vt * kk;
             // |r| < 1e-5 against the double literal is exactly |r| <= 1e-5f
@@ -350,10 +349,11 @@ __device__ __forceinline__ void tet_local_step(int g, int T, const float4* __res
     }
 }
 
-// Per-tet local step kernel, one tet per lane (tet_local_step).  Lanes
+// Per-tet local step kernel, one tet per lane (tet_local_step).  TRI: lanes
 // T..T+S-1 (when ntri > 0) run the triangle-sphere DCD (PDSolver.cu:612-680)
-// and write (k*(centre-hit), k) per corner.
-template <int MODE>
+// and write (k*(centre-hit), k) per corner.  The instances without it hold
+// 55 VGPRs instead of 78: 8 waves per SIMD instead of 6.
+template <int MODE, bool TRI>
 __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x, const int4* __restrict__ tet,
                                              const float* __restrict__ B, const float* __restrict__ vol,
                                              const int4* __restrict__ slot, float3* __restrict__ cf, float k,
@@ -364,6 +364,7 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
     if (skip && *skip) return;
     const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (g >= T) {
+        if (!TRI) return;
         const int t = g - T;
         if (t >= ntri) return;
         const int4 id = tri[t];
@@ -2077,6 +2078,14 @@ int timed_end(mgsb_solver* s, int kernel) {
 }
 
 // split: x holds the in-step correction u and the kernel adds F(s) (V-cycle steps).
+template <int MODE>
+void launch_tet_mode(mgsb_solver* s, DevLevel& d, int nb, int bs, const float4* xx, int ntri, const int* skip,
+                     const float4* sref) {
+    auto k = ntri ? k_tet<MODE, true> : k_tet<MODE, false>;
+    k<<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness, ntri, d.tri, d.tslot,
+                                d.tcf, s->dcol, skip, sref, d.tk);
+}
+
 int launch_tet(mgsb_solver* s, DevLevel& d, int mode, bool with_tri, const float4* xsrc = nullptr,
                const int* skip = nullptr, bool split = false) {
     const float4* sref = split ? d.s : nullptr;
@@ -2090,21 +2099,10 @@ int launch_tet(mgsb_solver* s, DevLevel& d, int mode, bool with_tri, const float
     if (rc) return rc;
     const int nb = nblk(n, bs);
     switch (mode) {
-        case kTetModePolar:
-            k_tet<kTetModePolar><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness,
-                                                           ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, sref, d.tk);
-            break;
-        case kTetModeAc:
-            k_tet<kTetModeAc><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness, ntri,
-                                                        d.tri, d.tslot, d.tcf, s->dcol, skip, sref, d.tk);
-            break;
-        case kTetModeStable:
-            k_tet<kTetModeStable><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness,
-                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, sref, d.tk);
-            break;
-        default:
-            k_tet<kTetModeLinear><<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness,
-                                                            ntri, d.tri, d.tslot, d.tcf, s->dcol, skip, sref, d.tk);
+        case kTetModePolar: launch_tet_mode<kTetModePolar>(s, d, nb, bs, xx, ntri, skip, sref); break;
+        case kTetModeAc: launch_tet_mode<kTetModeAc>(s, d, nb, bs, xx, ntri, skip, sref); break;
+        case kTetModeStable: launch_tet_mode<kTetModeStable>(s, d, nb, bs, xx, ntri, skip, sref); break;
+        default: launch_tet_mode<kTetModeLinear>(s, d, nb, bs, xx, ntri, skip, sref);
     }
     HIPCHK(hipGetLastError());
     return timed_end(s, kid);
