@@ -44,7 +44,7 @@ TRAIN = dict(record_steps=2000, epochs=10, batch=4096, lr=2e-3, seed=0, velocity
 # HBM bytes per k_tet launch on the medium box (V-cycle split-position mode):
 # rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy),
 # profiles/k_tet_learned_ab_medium_box.md
-TRAFFIC = {"medium_box_12x120x12": 12.93e6}
+TRAFFIC = {"medium_box_12x120x12": 12.90e6}
 
 
 def log(*a):
