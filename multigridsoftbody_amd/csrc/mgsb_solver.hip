@@ -2530,7 +2530,11 @@ int wait_checks(mgsb_solver* s) {
         if ((unsigned)mirror_word(s) == want) return MGSB_OK;
         if ((it & 4095) == 0) {
             const hipError_t q = hipStreamQuery(s->stream);
-            if (q == hipSuccess) return MGSB_OK;  // all work done: the mirror is final
+            if (q == hipSuccess) {  // all work done: the mirror is final
+                if ((unsigned)mirror_word(s) == want) return MGSB_OK;
+                set_error("step: the stream drained with a published check count other than the one enqueued");
+                return MGSB_ERR_STATE;
+            }
             if (q != hipErrorNotReady) HIPCHK(q);
         }
     }
