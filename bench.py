@@ -216,8 +216,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(sharding across GPUs is described in DESIGN.md, not built).", file=sys.stderr)
+        print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding of the bodies is not built).", file=sys.stderr)
         sys.exit(2)
 
     log(f"[bench] workload={args.workload} schedule={args.schedule} steps={args.steps} warmup={args.warmup}")
