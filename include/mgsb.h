@@ -47,6 +47,9 @@ extern "C" {
                             small anisotropic strains the reference's FP32 k < 1e-6 branch   \
                             drops; the V-cycle's default                                     */
 
+#define MGSB_IF_LINEAR 3 /* the V-cycle's linear operator on corrections, -F(e) AC k vol with no clamp \
+                            (mgsb_tet_forces only; no reference counterpart)                     */
+
 #define MGSB_MAX_SPHERES 8
 
 /* Tet mesh (SoftObject's arrays, SoftObject.h:22-27).  Arrays returned by the
@@ -202,11 +205,16 @@ int mgsb_set_params(mgsb_solver* s, const mgsb_params* p);
  * level, 1 = free, 0 = pinned. */
 int mgsb_set_fixed(mgsb_solver* s, int32_t level, const float* fixflag);
 /* SphereFixer rule (PDSolver.cpp:145-161): pin vertices strictly inside any of
- * n spheres (cx,cy,cz,r) on every level. */
+ * n spheres (cx,cy,cz,r) on every level (|X - c| < r in FP32, rest positions),
+ * adding to the pins already set. */
 int mgsb_fix_spheres(mgsb_solver* s, const float* spheres, int32_t n);
 /* Pin vertices with |pos[axis]-value| < tol on every level (the y=10 rule,
  * PDSolver.cpp:137-143). */
 int mgsb_fix_plane(mgsb_solver* s, int32_t axis, float value, float tol);
+
+/* The fixflags of a level (V floats, 1 free / 0 pinned): PDSolver's
+ * m_tetVertFixed (PDSolver.h:95), host copy. */
+int mgsb_read_fixed(mgsb_solver* s, int32_t level, float* fixflag);
 
 /* SphereCollider (Collider.h:5-20): centre, radius, velocity (the centre
  * advances by vel*dt at the start of each step, as UpdateCollider moves it
@@ -334,6 +342,14 @@ int mgsb_kernel_time(mgsb_solver* s, int32_t kernel, double* ms, int64_t* launch
  * solver's stream between two HIP events; avg_us = elapsed / reps.  The
  * launches overwrite the corner forces only (the next step recomputes them). */
 int mgsb_time_kernel(mgsb_solver* s, int32_t kernel, int32_t reps, double* avg_us);
+
+/* Test entry point: the per-tet local step of level `level` evaluated at
+ * positions x (3*V) in mode if_mode (MGSB_IF_*), the forces each tet adds to
+ * its four corners before the gather: cf[12*t + 3*j + i] is component i at
+ * corner j of tet t -- calculateIF's temp[] times vol*k (PDSolver.cu:160-181)
+ * or calTetIFV1's (PDSolverAc.cu:630-662).  Overwrites the level's corner
+ * forces only (the next step recomputes them). */
+int mgsb_tet_forces(mgsb_solver* s, int32_t level, int32_t if_mode, const float* x, float* cf);
 
 const char* mgsb_last_error(void);
 const char* mgsb_version(void);

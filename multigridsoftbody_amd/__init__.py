@@ -20,7 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmgsb.so")
 
 SCHED_SINGLE, SCHED_TWOLEVEL, SCHED_VCYCLE, SCHED_LEARNED = 0, 1, 2, 3
-IF_PLAIN, IF_AC, IF_STABLE = 0, 1, 2
+IF_PLAIN, IF_AC, IF_STABLE, IF_LINEAR = 0, 1, 2, 3
 MAX_SPHERES = 8
 
 
@@ -127,6 +127,7 @@ SIGNATURES = {
     "mgsb_set_fixed": (C.c_int, [_vp, C.c_int32, _fp]),
     "mgsb_fix_spheres": (C.c_int, [_vp, _fp, C.c_int32]),
     "mgsb_fix_plane": (C.c_int, [_vp, C.c_int32, C.c_float, C.c_float]),
+    "mgsb_read_fixed": (C.c_int, [_vp, C.c_int32, _fp]),
     "mgsb_set_sphere": (C.c_int, [_vp, C.c_int32, _fp, C.c_float, _fp, C.c_int32]),
     "mgsb_set_plane": (C.c_int, [_vp, _fp, C.c_float, C.c_float, C.c_int32]),
     "mgsb_set_velocity": (C.c_int, [_vp, _fp]),
@@ -161,6 +162,7 @@ SIGNATURES = {
     "mgsb_set_body_stiffness": (C.c_int, [_vp, _fp, C.c_int32]),
     "mgsb_body_residuals": (C.c_int, [_vp, _dp, _dp, C.c_int32]),
     "mgsb_time_kernel": (C.c_int, [_vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+    "mgsb_tet_forces": (C.c_int, [_vp, C.c_int32, C.c_int32, _fp, _fp]),
     "mgsb_last_error": (C.c_char_p, []),
     "mgsb_version": (C.c_char_p, []),
 }
@@ -376,6 +378,12 @@ class Solver:
         s = _f32(spheres).reshape(-1, 4)
         _check(lib.mgsb_fix_spheres(self._h, _fptr(s), s.shape[0]), "mgsb_fix_spheres")
 
+    def fixed(self, level=0):
+        """fixflag per vertex of a level (1 free, 0 pinned)."""
+        out = np.zeros(self.levels[level].nverts, np.float32)
+        _check(lib.mgsb_read_fixed(self._h, level, _fptr(out)), "mgsb_read_fixed")
+        return out
+
     def fix_plane(self, axis, value, tol=1e-5):
         _check(lib.mgsb_fix_plane(self._h, axis, value, tol), "mgsb_fix_plane")
 
@@ -550,6 +558,14 @@ class Solver:
                "mgsb_body_residuals")
         return r0, r
 
+    def tet_forces(self, x, if_mode, level=0):
+        """Per-tet corner forces (T, 4, 3) of the local step at positions x."""
+        m = self.levels[level]
+        a = _f32(x, m.nverts * 3)
+        out = np.zeros(m.ntets * 12, np.float32)
+        _check(lib.mgsb_tet_forces(self._h, level, if_mode, _fptr(a), _fptr(out)), "mgsb_tet_forces")
+        return out.reshape(-1, 4, 3)
+
     def time_kernel(self, reps=200, kernel=0):
         """Average microseconds of `reps` back-to-back launches of kernel 0."""
         us = C.c_double()
@@ -586,4 +602,4 @@ class PDSolver_MG(Solver):  # noqa: N801  (reference name)
 
 
 __all__ = ["lib", "Mesh", "Solver", "PDSolver", "PDSolver_MG", "default_params", "MgsbError", "SCHED_SINGLE",
-           "SCHED_TWOLEVEL", "SCHED_VCYCLE", "SCHED_LEARNED", "IF_PLAIN", "IF_AC", "IF_STABLE"]
+           "SCHED_TWOLEVEL", "SCHED_VCYCLE", "SCHED_LEARNED", "IF_PLAIN", "IF_AC", "IF_STABLE", "IF_LINEAR"]

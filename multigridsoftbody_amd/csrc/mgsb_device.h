@@ -44,9 +44,70 @@ __device__ __forceinline__ float len3(float3 a) { return sqrtf(a.x * a.x + a.y *
 // x that is exactly x <= 1e-6f (no float lies in (1e-6f, 1e-6]).
 __device__ __forceinline__ bool lt_1e6(float x) { return x <= 1e-6f; }
 
+// Error-free transformations in FP32: a + b = s + e exactly (Knuth's
+// TwoSum, TAOCP vol. 2, 4.2.2) and a * b = p + e exactly (Dekker's
+// TwoProduct, "A floating-point technique for extending the available
+// precision", 1971, with the product's error taken by one fused
+// multiply-add).
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void two_prod(float a, float b, float& p, float& e) {
+    p = a * b;
+    e = __builtin_fmaf(a, b, -p);
+}
+
+// l = I_c (I_c^2 - 4.5 II_c) + 13.5 III_c as GetRotation_D evaluates it
+// (utils.cu:161): I_c^2 is a float product, the double literals 4.5 and 13.5
+// carry everything else into double, and the double result is rounded to
+// float.  l is the cancelling numerator of cos(3 phi) (three O(27) terms,
+// result O(k^1.5)), so an FP32 evaluation loses it at small strains (at
+// strain 1e-3 its error is hundreds of times k^1.5).  Here every term is
+// kept exactly as a sum of floats: 4.5 II_c and 13.5 III_c as TwoProducts,
+// I_c^2 - 4.5 II_c as dh + dl (exact: the difference needs about 30 bits),
+// I_c (dh + dl) as four TwoProduct parts.  After the leading parts cancel
+// (TwoSum, exact), the small parts are summed with their rounding errors
+// carried (cascaded TwoSum, Ogita-Rump-Oishi "Accurate sum and dot
+// product", 2005), and the result is rounded once.  Over 1.4M random F
+// (strains 1e-4 to 20, half of them inverted) this equals the
+// double-evaluated l rounded to float in all but one case (one ulp).
+__device__ __forceinline__ float franca_l_double(float I_c, float I_c2, float II_c, float III_c) {
+    float ph, pl;
+    two_prod(4.5f, II_c, ph, pl);
+    float dh, e1;
+    two_sum(I_c2, -ph, dh, e1);
+    const float dl = e1 - pl;
+    float mh, m1, m2h, m2l;
+    two_prod(I_c, dh, mh, m1);
+    two_prod(I_c, dl, m2h, m2l);
+    float qh, ql;
+    two_prod(13.5f, III_c, qh, ql);
+    float sh, e2;
+    two_sum(mh, qh, sh, e2);
+    float t1, c1, t2, c2, t3, c3;
+    two_sum(e2, m1, t1, c1);
+    two_sum(t1, m2h, t2, c2);
+    two_sum(t2, ql, t3, c3);
+    float th, tl;
+    two_sum(t3, ((c1 + c2) + c3) + m2l, th, tl);
+    float r, cr;
+    two_sum(sh, th, r, cr);
+    return r + (cr + tl);
+}
+
 // Franca closed-form polar decomposition R = F U^-1 from the invariants of
-// C = F^T F (GetRotation_D, utils.cu:123-226; getTetFRV2, PDSolverAc.cu:494-595).
-// Fallbacks return R = I where the reference does.  Writes R - F into RF.
+// C = F^T F.  REF_D selects the variant:
+//   true : GetRotation_D (utils.cu:123-226), the local step of calculateIF
+//          and of runCalEnergy: l to double accuracy (franca_l_double) and
+//          III_u = +-sqrt(III_c) with the sign of det F (utils.cu:171-172);
+//   false: getTetFRV2 (PDSolverAc.cu:494-595): l in FP32 with float
+//          constants and III_u = det F.
+// (In FP32, sqrtf(det * det) == |det| barring underflow, so the two III_u
+// agree wherever the guards below let the formula run.)  Fallbacks return
+// R = I where the reference does.  Writes R - F into RF.
+template <bool REF_D>
 __device__ __forceinline__ void polar_rf(const float F[9], float RF[9]) {
     float C[9], C2[9];
 #pragma unroll
@@ -77,7 +138,7 @@ __device__ __forceinline__ void polar_rf(const float F[9], float RF[9]) {
             iu[0] = iu[4] = iu[8] = inv_lambda;
         }
     } else {
-        const float l = I_c * (I_c * I_c - 4.5f * II_c) + 13.5f * III_c;
+        const float l = REF_D ? franca_l_double(I_c, I_c2, II_c, III_c) : I_c * (I_c * I_c - 4.5f * II_c) + 13.5f * III_c;
         const float k_root = sqrtf(k);
         float value = l / (k * k_root);
         value = value < -1.0f ? -1.0f : value;
@@ -85,7 +146,7 @@ __device__ __forceinline__ void polar_rf(const float F[9], float RF[9]) {
         const float phi = acosf(value);
         const float lambda2 = (I_c + 2 * k_root * cosf(phi / 3)) / 3;
         const float lambda = sqrtf(lambda2);
-        const float III_u = det;
+        const float III_u = REF_D ? copysignf(sqrtf(III_c), det < 0.f ? -1.f : 1.f) : det;
         float I_u = 0.f, II_u = 0.f;
         if (lt_1e6(fabsf(lambda))) ok = false;
         if (ok && lt_1e6(-lambda2 + I_c + 2 * III_u / lambda)) ok = false;

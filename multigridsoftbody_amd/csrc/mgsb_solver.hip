@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -318,7 +319,7 @@ __device__ __forceinline__ void tet_local_step(int g, int T, const float4* __res
     } else if (MODE == kTetModeStable) {
         polar_rf_stable(F, RF);
     } else {
-        polar_rf(F, RF);
+        polar_rf<MODE == kTetModePolar>(F, RF);
     }
     // AC[r][0] = -B[0][r] - B[1][r] - B[2][r], AC[r][j] = B[j-1][r]
     float AC[12];
@@ -410,6 +411,22 @@ __global__ void __launch_bounds__(256) k_tet_dual(int T, const float4* __restric
         tet_local_step<MODE>(g - T, T, s, tet, B, vol, slot, cf2, k, nullptr, tk);
 }
 
+// calculatePOS's under-relaxation (next - pos) * 0.6 + pos (PDSolver.cu:
+// 219-221): the float difference times the double literal 0.6 plus the float
+// position, evaluated in double and rounded to float once.  Here the
+// constant is the double word relax_hi + relax_lo (two floats, 48 bits), the
+// product is a TwoProduct and the sum a TwoSum, so the result is the
+// correctly rounded float of the double expression but for a tie within
+// about 2^-48 |next - pos| of a rounding boundary.
+__device__ __forceinline__ float relax_step(float d, float pos, float relax_hi, float relax_lo) {
+    float ph, pe;
+    two_prod(d, relax_hi, ph, pe);
+    pe = __builtin_fmaf(d, relax_lo, pe);
+    float sh, se;
+    two_sum(ph, pos, sh, se);
+    return sh + (se + pe);
+}
+
 // Per-vertex update: gather the CSR range of corner forces (tet order), add
 // collision response, then calculatePOS (PDSolver.cu:192-235): Jacobi step,
 // under-relaxation, Chebyshev extrapolation with omega.  Sphere point DCD is
@@ -418,8 +435,8 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
                                               const float4* __restrict__ s, const float4* __restrict__ vc,
                                               const int* __restrict__ rp, const float3* __restrict__ cf,
                                               const int* __restrict__ trp, const float4* __restrict__ tcf,
-                                              const DevColliders* __restrict__ col, float omega, float relax,
-                                              int flags, int* __restrict__ cflag, float* __restrict__ cdepth,
+                                              const DevColliders* __restrict__ col, float omega, float relax_hi,
+                                              float relax_lo, int flags, int* __restrict__ cflag, float* __restrict__ cdepth,
                                               float* __restrict__ cenergy, const int* __restrict__ skip, int split,
                                               int lsh) {
     if (skip && *skip) return;
@@ -541,9 +558,9 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
         const float dx = (c00 * gx + c01 * gy + c02 * gz) / det;
         const float dy = (c01 * gx + c11 * gy + c12 * gz) / det;
         const float dz = (c02 * gx + c12 * gy + c22 * gz) / det;
-        const float bx = (dx * fix + p.x - p.x) * relax + p.x;
-        const float by = (dy * fix + p.y - p.y) * relax + p.y;
-        const float bz = (dz * fix + p.z - p.z) * relax + p.z;
+        const float bx = relax_step(dx * fix + p.x - p.x, p.x, relax_hi, relax_lo);
+        const float by = relax_step(dy * fix + p.y - p.y, p.y, relax_hi, relax_lo);
+        const float bz = relax_step(dz * fix + p.z - p.z, p.z, relax_hi, relax_lo);
         xprev[i] = p;
         x[i] = make_float4(bx, by, bz, 0.f);
         return;
@@ -551,9 +568,9 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
     float nx = (dc * sx + (fx + cfx)) / (dg + cdx + dc) * fix + p.x;
     float ny = (dc * sy + (fy + cfy)) / (dg + cdy + dc) * fix + p.y;
     float nz = (dc * sz + (fz + cfz)) / (dg + cdz + dc) * fix + p.z;
-    nx = (nx - p.x) * relax + p.x;
-    ny = (ny - p.y) * relax + p.y;
-    nz = (nz - p.z) * relax + p.z;
+    nx = relax_step(nx - p.x, p.x, relax_hi, relax_lo);
+    ny = relax_step(ny - p.y, p.y, relax_hi, relax_lo);
+    nz = relax_step(nz - p.z, p.z, relax_hi, relax_lo);
     nx = omega * (nx - pp.x) + pp.x;
     ny = omega * (ny - pp.y) + pp.y;
     nz = omega * (nz - pp.z) + pp.z;
@@ -704,7 +721,7 @@ __global__ void __launch_bounds__(256) k_energy_tet(int T, const float4* __restr
         float F[9], RF[9];
         for (int r = 0; r < 3; r++)
             for (int c = 0; c < 3; c++) F[r * 3 + c] = D[r * 3 + 0] * b[c] + D[r * 3 + 1] * b[3 + c] + D[r * 3 + 2] * b[6 + c];
-        polar_rf(F, RF);
+        polar_rf<true>(F, RF);  // runCalEnergy calls GetRotation_D (PDSolver.cu:883)
         double e = 0.0;
         for (int q = 0; q < 9; q++) e += (double)RF[q] * (double)RF[q];
         v[0] = e * (double)vol[g] * (double)(tk ? tk[g] : k) / 2.0;
@@ -1854,6 +1871,7 @@ enum { kPieceStart = 0, kPiecePre = 1, kPiecePost = 2, kPieceFinish = 3, kPieceS
 
 struct mgsb_solver {
     mgsb_params p{};
+    float relax_key = NAN, relax_hi = 0.f, relax_lo = 0.f;  // p.relax as a double word (launch_vert)
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<DevLevel> L;
@@ -2135,11 +2153,30 @@ int launch_tet_dual(mgsb_solver* s, DevLevel& d) {
     return MGSB_OK;
 }
 
+// The double a float parameter stands for: the double nearest to the
+// shortest decimal that rounds to it (0.6f -> 0.6), as the constants the
+// reference writes in its kernels are double literals (PDSolver.cu:219).
+double decimal_double(float f) {
+    char buf[32];
+    for (int prec = 1; prec <= 9; prec++) {
+        std::snprintf(buf, sizeof buf, "%.*g", prec, (double)f);
+        const double d = std::strtod(buf, nullptr);
+        if ((float)d == f) return d;
+    }
+    return (double)f;
+}
+
 int launch_vert(mgsb_solver* s, DevLevel& d, float omega, int flags, const int* skip = nullptr, int split = 0) {
     const int bs = block_vert(s), lsh = vert_lsh(s);
+    if (!(s->p.relax == s->relax_key)) {  // the relaxation constant as a double word
+        s->relax_key = s->p.relax;
+        const double r = decimal_double(s->p.relax);
+        s->relax_hi = (float)r;
+        s->relax_lo = (float)(r - (double)s->relax_hi);
+    }
     k_vert<<<nblk(d.V << lsh, bs), bs, 0, s->stream>>>(d.V, d.x, d.xprev, d.s, d.vc, d.rp, d.cf, d.trp, d.tcf, s->dcol,
-                                                        omega, s->p.relax, flags, d.cflag, d.cdepth, d.cenergy, skip,
-                                                        split, lsh);
+                                                        omega, s->relax_hi, s->relax_lo, flags, d.cflag, d.cdepth,
+                                                        d.cenergy, skip, split, lsh);
     HIPCHK(hipGetLastError());
     return MGSB_OK;
 }
@@ -3330,6 +3367,18 @@ int mgsb_read_velocity(mgsb_solver* s, int32_t level, float* v) {
     return read_f4(s, s->L[level].v, s->L[level].V, v);
 }
 
+int mgsb_read_fixed(mgsb_solver* s, int32_t level, float* fixflag) {
+    int rc = check_level(s, level);
+    if (rc) return rc;
+    if (!fixflag) {
+        set_error("mgsb_read_fixed: null array");
+        return MGSB_ERR_ARG;
+    }
+    const std::vector<float>& f = s->L[level].h.fix;
+    std::copy(f.begin(), f.end(), fixflag);
+    return MGSB_OK;
+}
+
 static int compute_normals(mgsb_solver* s) {
     DevLevel& d = s->L[0];
     const int bs = 256;
@@ -3948,6 +3997,40 @@ int mgsb_time_kernel(mgsb_solver* s, int32_t kernel, int32_t reps, double* avg_u
     if (rc) return rc;
     HIPCHK(e);
     *avg_us = 1e3 * (double)ms / reps;
+    return MGSB_OK;
+}
+
+int mgsb_tet_forces(mgsb_solver* s, int32_t level, int32_t if_mode, const float* x, float* cf) {
+    if (!s || level < 0 || level >= (int)s->L.size() || if_mode < 0 || if_mode > MGSB_IF_LINEAR || !x || !cf) {
+        set_error("mgsb_tet_forces: bad level, if_mode (0..3) or null array");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    DevLevel& d = s->L[level];
+    float4* xd = nullptr;
+    int rc = dalloc(&xd, (size_t)d.V);
+    if (rc) return rc;
+    const int mode = if_mode == MGSB_IF_LINEAR ? kTetModeLinear : tet_mode_of(if_mode);
+    const bool was = s->timing;
+    s->timing = false;
+    rc = write_f4(s, xd, d.V, x);
+    if (!rc) rc = launch_tet(s, d, mode, false, xd);
+    s->timing = was;
+    std::vector<float3> h((size_t)d.T * 4);
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMemcpyAsync(h.data(), d.cf, sizeof(float3) * h.size(), hipMemcpyDeviceToHost, s->stream);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(xd);
+    if (rc) return rc;
+    HIPCHK(e);
+    for (int t = 0; t < d.T; t++)
+        for (int j = 0; j < 4; j++) {
+            const float3 q = h[(size_t)d.h.slot[(size_t)t * 4 + j]];
+            float* o = cf + (size_t)t * 12 + j * 3;
+            o[0] = q.x;
+            o[1] = q.y;
+            o[2] = q.z;
+        }
     return MGSB_OK;
 }
 
