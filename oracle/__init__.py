@@ -95,6 +95,7 @@ _SIGS = {
     "ml_set_warm": (None, [_vp, _fp, _fp]),
     "step_vcycle": (None, [_vp, C.POINTER(orc_params), C.POINTER(orc_vparams), C.c_int]),
     "ml_coarse_solve": (None, [_vp, _fp, _fp]),
+    "ml_lin_smooth": (None, [_vp, C.c_int, C.c_int, _fp, _fp]),
     "mlp_forward": (None, [_fp, C.c_int, C.c_int, _fp, C.c_int, _dp, C.c_int]),
     "mlp_grad": (C.c_double, [_fp, C.c_int, C.c_int, _fp, _fp, C.c_int, _dp, C.c_int]),
 }
@@ -337,6 +338,13 @@ class MultiLevel:
         r = np.ascontiguousarray(r, np.float32).reshape(-1)
         e = np.zeros_like(r)
         self._f("ml_coarse_solve")(self._h, r.ctypes.data_as(_fp), e.ctypes.data_as(_fp))
+        return e.reshape(-1, 3)
+
+    def lin_smooth(self, level, sweeps, r):
+        """`sweeps` coarse-level smoothing sweeps on A_l e = r from e = 0."""
+        r = np.ascontiguousarray(r, np.float32).reshape(-1)
+        e = np.zeros_like(r)
+        self._f("ml_lin_smooth")(self._h, level, sweeps, r.ctypes.data_as(_fp), e.ctypes.data_as(_fp))
         return e.reshape(-1, 3)
 
     def stats(self):

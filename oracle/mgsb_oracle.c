@@ -1654,6 +1654,15 @@ void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nstep
     }
 }
 
+/* `sweeps` damped-Jacobi + Chebyshev sweeps on A_l e = r from e = 0 on level
+ * l (ml_smooth; test entry point). */
+void PFX(ml_lin_smooth)(orc_ml* M, int l, int sweeps, const float* r, float* e) {
+    int V = M->L[l]->V;
+    for (int i = 0; i < V * 3; i++) M->r[l][i] = r[i];
+    ml_smooth(M, l, sweeps, 1);
+    for (int i = 0; i < V * 3; i++) e[i] = (float)M->e[l][i];
+}
+
 /* Coarsest-level solve on a given right-hand side (test entry point). */
 void PFX(ml_coarse_solve)(orc_ml* M, const float* r, float* e) {
     int l = M->nl - 1, V = M->L[l]->V;

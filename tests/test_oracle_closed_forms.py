@@ -157,8 +157,8 @@ def test_default_params_are_the_references():
     (PDSolver.cpp:166-176: 20 iterations, dt 1/60, damping 0.95, stiffness
     8000, collision stiffness 1000, rho 0.9992, gravity (0, -9.8, 0)), the
     0.6 under-relaxation of calculatePOS (PDSolver.cu:219) and the 20 coarse
-    iterations of PDSolver_MG::Step (PDSolver_MG.cpp:201); unit density (the
-    reference's lumped mass is vol/4, PDSolver.cpp:51-52)."""
+    iterations of PDSolver_MG::Step (PDSolver_MG.cpp:202); unit density (the
+    reference's lumped mass is vol/4, PDSolver.cpp:46-49)."""
     p = M.default_params()
     assert p.iters == 20 and p.iters_coarse == 20
     assert p.dt == F32(1.0 / 60.0)
@@ -166,4 +166,35 @@ def test_default_params_are_the_references():
     assert list(p.gravity) == [0.0, F32(-9.8), 0.0]
     assert p.relax == F32(0.6) and p.density == 1.0
     assert p.schedule == M.SCHED_SINGLE and p.if_mode == M.IF_AC  # PDSolver::Step runs runCalculateIFAc
-    assert p.tri_dcd_coarse == 1  # PDSolver_MG's coarse level uses DCDByTriangle (PDSolver_MG.cpp:203)
+    assert p.tri_dcd_coarse == 1  # PDSolver_MG's coarse level uses DCDByTriangle (PDSolver_MG.cpp:204)
+
+
+def test_two_level_schedule_free_fall_closed_form():
+    """PDSolver_MG::Step (PDSolver_MG.cpp:183-253) on an unpinned nested pair
+    translating rigidly: no elastic force on either level, and the barycentric
+    transfers reproduce affine fields, so both levels follow the free-fall
+    closed form of calculateST + calculateV (a level missing its ST or V
+    update, or a transfer that is not affine-exact, breaks it)."""
+    from multigridsoftbody_amd import workloads as W
+
+    h, damp = F32(1 / 60), F32(0.95)
+    g = np.array([0.0, F32(-9.8), F32(1.1)])
+    v0 = np.array([0.2, 0.5, -0.3], np.float32)
+    fine, coarse = W.beam(2), W.beam(1)
+    Lf = O.Level(fine.pos, fine.tets, fine.tris, stiffness=1e5, density=1000.0, fp64=True)
+    Lc = O.Level(coarse.pos, coarse.tets, coarse.tris, stiffness=1e5, density=1000.0, fp64=True)
+    for L, m in ((Lf, fine), (Lc, coarse)):
+        L.set_fix(np.ones(m.nverts, np.float32))
+        L.set_velocity(np.tile(v0, (m.nverts, 1)))
+    mg = O.TwoLevel(Lc, Lf)
+    n = 12
+    mg.step(O.params(dt=h, damping=damp, stiffness=1e5, density=1000.0, gravity=g, iters=5, iters_coarse=5,
+                     if_mode=0), n)
+    v = v0.astype(np.float64)
+    d = np.zeros(3)
+    for _ in range(n):
+        v = damp * v + g * h
+        d = d + h * v
+    for L, m in ((Lf, fine), (Lc, coarse)):
+        assert np.allclose(L.positions(), m.pos.astype(np.float64) + d, rtol=0, atol=1e-9)
+        assert np.allclose(L.get(O.V).reshape(-1, 3), v, rtol=0, atol=1e-7)

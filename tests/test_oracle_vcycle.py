@@ -65,6 +65,44 @@ def test_power_iteration_rho_matches_dense_eigenvalues():
     assert abs(ML.rho(1) - rho) < 1e-3 * rho, (ML.rho(1), rho)
 
 
+def test_coarse_smoother_against_the_assembled_system():
+    """ml_smooth (damped Jacobi with Chebyshev on A_l e = r, the V-cycle's
+    coarse-level smoother) on level 1 of a 3-level beam, against the system
+    assembled from the mesh alone (dense_system): k sweeps from e = 0 follow
+        y = e + relax D^-1 (r - A e),  e_new = w_k (y - e_prev) + e_prev,
+        w_k = 4 / (4 - rho^2 w_{k-1}),  w_0 = 1,
+    on the free DOFs (pinned rows stay 0), and many sweeps converge to the
+    solution of A_ff e = r_f.  A wrong assembly, sign or omega seed fails one
+    of the two."""
+    lv = W.beam_hierarchy((4, 2, 1))
+    phys = W.physics()
+    Ls = [O.Level(m.pos, m.tets, m.tris, stiffness=phys["stiffness"], density=phys["density"], fp64=True)
+          for m in lv]
+    Ls[0].set_fix(W.top_fixflag(lv[0]))
+    ML = O.MultiLevel(Ls, relax_coarse=0.6, dt=phys["dt"])
+    vfix = ML.vfix(1)
+    A, free, _ = dense_system(lv[1], vfix, phys["stiffness"], phys["density"], phys["dt"])
+    Af = A[np.ix_(free, free)]
+    d = np.diag(Af)
+    rho, relax = ML.rho(1), float(np.float32(0.6))
+    r = np.random.default_rng(4).normal(size=(lv[1].nverts, 3)).astype(np.float32)
+    r[vfix == 0] = 0
+    rf = r[free].astype(np.float64)
+    for k in (1, 2, 5):
+        e, ep, w = np.zeros_like(rf), np.zeros_like(rf), 1.0
+        for _ in range(k):
+            w = 4 / (4 - rho * rho * w)
+            y = e + relax * (rf - Af @ e) / d[:, None]
+            e, ep = w * (y - ep) + ep, e
+        got = ML.lin_smooth(1, k, r)
+        # the entry point returns FP32: agreement to its rounding, 6e-8 of the max
+        assert np.abs(got[free] - e).max() <= 3e-7 * np.abs(e).max(), k
+        assert np.all(got[vfix == 0] == 0)
+    got = ML.lin_smooth(1, 400, r)
+    exact = np.linalg.solve(Af, rf)
+    assert np.abs(got[free] - exact).max() <= 1e-5 * np.abs(exact).max()
+
+
 def test_stable_polar_keeps_small_anisotropic_strain():
     """At strains ~1e-4 the reference's FP32 branch (k < 1e-6) returns the
     isotropic factor; the deviatoric-invariant form (variant 2) still matches
