@@ -249,6 +249,17 @@ int mgsb_read_render9(mgsb_solver* s, float* v9);
 int mgsb_read_collision(mgsb_solver* s, int32_t* flags, float* depth);
 /* runCalEnergy (PDSolver.cu:816-913) on the finest level, FP64 sums on device. */
 int mgsb_energy(mgsb_solver* s, mgsb_energy_t* e);
+/* mgsb_energy without any allocation: no hipMalloc, hipFree or other
+ * allocating call on success (the reduction partials of mgsb_energy grow
+ * by hipMalloc on demand).  No counterpart in the reference; it is for
+ * callers that cannot allocate (serving loops that read the energy every
+ * step, as PDSolver_MG::Step does, harnesses that forbid allocation).  The
+ * partials live in a fixed 1 MiB device array per GPU, so the finest level
+ * may have at most 8,387,584 vertices and tets (MGSB_ERR_ARG beyond).  Every
+ * call on one GPU shares that array: calls that overlap on a GPU (handles
+ * stepped from several threads) race and must be serialised by the caller.
+ * Results are bit-identical to mgsb_energy. */
+int mgsb_energy_noalloc(mgsb_solver* s, mgsb_energy_t* e);
 /* ||g(x)|| of the finest level at its current state (FP64). */
 int mgsb_residual(mgsb_solver* s, double* res);
 

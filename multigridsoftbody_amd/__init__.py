@@ -140,6 +140,7 @@ SIGNATURES = {
     "mgsb_read_render9": (C.c_int, [_vp, _fp]),
     "mgsb_read_collision": (C.c_int, [_vp, _ip, _fp]),
     "mgsb_energy": (C.c_int, [_vp, C.POINTER(mgsb_energy_t)]),
+    "mgsb_energy_noalloc": (C.c_int, [_vp, C.POINTER(mgsb_energy_t)]),
     "mgsb_residual": (C.c_int, [_vp, _dp]),
     "mgsb_level_info": (C.c_int, [_vp, C.c_int32, _ip, _ip, _ip]),
     "mgsb_level_rho": (C.c_int, [_vp, C.c_int32, _dp]),

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -728,6 +729,16 @@ __global__ void __launch_bounds__(256) k_energy_tet(int T, const float4* __restr
     }
     block_reduce_store<1>(v, part);
 }
+
+// Scratch of the allocation-free energy diagnostics (mgsb_energy_noalloc):
+// the FP64 block partials of the two reductions and their results, in a
+// fixed device array instead of the handle's partials (which grow by
+// hipMalloc on demand).  Every GPU the library runs on holds its own copy
+// until the process ends; in host code `scratch` is only a symbol, and its
+// address on each device comes from hipGetSymbolAddress (scratch_setup).
+constexpr size_t kScratchDoubles = 131072;  // 1 MiB
+constexpr size_t kScratchResults = 16;      // the last 16 doubles take the reduction results
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 struct VState {
     double res0;    // ||g(s)||
@@ -1807,6 +1818,35 @@ int dalloc(T** p, size_t n) {
         return MGSB_ERR_NOMEM;
     }
     return MGSB_OK;
+}
+
+// Device address of `scratch` per device ordinal, looked up once per device
+// with that device current (at mgsb_create).
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxScratchDevices] = {};
+
+int scratch_setup(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) {
+        set_error("device ordinal beyond the scratch table (64 devices)");
+        return MGSB_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return MGSB_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        return MGSB_ERR_HIP;
+    }
+    g_scratch[device] = static_cast<double*>(ptr);
+    return MGSB_OK;
+}
+
+double* scratch_of(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch[device];
 }
 
 struct DevLevel {
@@ -3034,6 +3074,7 @@ int mgsb_create(const mgsb_mesh* levels, int32_t nlevels, const mgsb_params* p, 
             return MGSB_ERR_HIP;
         }
         HIPCHK(hipSetDevice(p->device));
+        if ((rc = scratch_setup(p->device))) return rc;
         auto s = std::make_unique<mgsb_solver>();
         s->p = *p;
         s->device = p->device;
@@ -3459,6 +3500,49 @@ int mgsb_energy(mgsb_solver* s, mgsb_energy_t* e) {
     e->Ep = ep;
     e->Ec = v4[2];
     e->Nc = v4[3];
+    return MGSB_OK;
+}
+
+int mgsb_energy_noalloc(mgsb_solver* s, mgsb_energy_t* e) {
+    if (!s || !e) {
+        set_error("mgsb_energy_noalloc: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    double* sp = scratch_of(s->device);
+    if (!sp) {
+        set_error("mgsb_energy_noalloc: no scratch address for this device");
+        return MGSB_ERR_STATE;
+    }
+    DevLevel& d = s->L[0];
+    const int nbv = nblk(d.V, 256), nbt = nblk(d.T, 256);
+    const size_t cap = (kScratchDoubles - kScratchResults) / 4;  // blocks of 256 items
+    if ((size_t)std::max(nbv, nbt) > cap) {
+        set_error("mgsb_energy_noalloc: the finest level has more than " + std::to_string(cap * 256) +
+                  " vertices or tets (fixed scratch of 1 MiB); use mgsb_energy");
+        return MGSB_ERR_ARG;
+    }
+    // every block writes its own partials and each final pass writes its
+    // results, so nothing in the scratch has to start at zero
+    double* part = sp;
+    double* res = sp + (kScratchDoubles - kScratchResults);
+    k_energy_vert<<<nbv, 256, 0, s->stream>>>(d.V, d.x, d.xprev, d.s, d.vc, d.cflag, d.cenergy, part);
+    HIPCHK(hipGetLastError());
+    k_reduce_final<4><<<1, 256, 0, s->stream>>>(nbv, part, res);
+    HIPCHK(hipGetLastError());
+    k_energy_tet<<<nbt, 256, 0, s->stream>>>(d.T, d.x, d.tet, d.B, d.vol, s->p.stiffness, d.tk, part);
+    HIPCHK(hipGetLastError());
+    k_reduce_final<1><<<1, 256, 0, s->stream>>>(nbt, part, res + 4);
+    HIPCHK(hipGetLastError());
+    double out[5];
+    HIPCHK(hipMemcpyAsync(out, res, sizeof(out), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const double dt = s->p.dt;
+    e->dX = std::sqrt(out[0]);
+    e->Ek = out[1] / (2 * dt * dt);
+    e->Ep = out[4];
+    e->Ec = out[2];
+    e->Nc = out[3];
     return MGSB_OK;
 }
 
