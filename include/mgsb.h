@@ -249,6 +249,29 @@ int mgsb_read_render9(mgsb_solver* s, float* v9);
 int mgsb_read_collision(mgsb_solver* s, int32_t* flags, float* depth);
 /* runCalEnergy (PDSolver.cu:816-913) on the finest level, FP64 sums on device. */
 int mgsb_energy(mgsb_solver* s, mgsb_energy_t* e);
+/* PDSolver::StepForConvergence (PDSolver.cpp:202-221) on the finest level:
+ * from the current state, calculateST (the colliders where the next step
+ * will see them), `iters` reference iterations with calculateIF's local step
+ * (the reference uses m_iterNumCvg = 128, PDSolver.cpp:167), and
+ * runCalEnergy after each; then positions and velocity are restored
+ * (runSaveVel / runResetPosVel, PDSolver.cu:270-275), so the next mgsb_step
+ * runs from the same state.  E receives iters + 1 totals Ek + Ep + Ec: E[0]
+ * at s (after a DCD at s, as PDSolver::Step's energy log does), E[iters]
+ * the converged energy (g_conEnergy_V2); last (may be NULL) the five terms
+ * of the final iterate. */
+int mgsb_step_convergence(mgsb_solver* s, int32_t iters, double* E, mgsb_energy_t* last);
+/* The reference's convergence metric for one step (the error of its slides
+ * and of PDSolver::Step's energy log, PDSolver.cpp:252-255): runs
+ * mgsb_step_convergence(iters), then one mgsb_step of the current schedule,
+ * and returns err = (E - E_conv) / (E0 - E_conv) with E = Ek + Ep + Ec at the
+ * step's end (collision energy by a DCD at the final positions).  E (may be
+ * NULL) receives E0, E_conv, E. */
+int mgsb_step_error(mgsb_solver* s, int32_t iters, double* err, double* E);
+/* runCalEnergy's per-vertex Ep density (calEveryVertEp, PDSolver.cu:893-924)
+ * at the current positions of the finest level: per vertex, the sum over
+ * incident tets of k vol/2 |F - R|^2 (negated where det F < 1) divided by the
+ * sum of their current volumes (V floats; Simulator::m_tetVertEpDensity). */
+int mgsb_read_ep_density(mgsb_solver* s, float* density);
 /* mgsb_energy without any allocation: no hipMalloc, hipFree or other
  * allocating call on success (the reduction partials of mgsb_energy grow
  * by hipMalloc on demand).  No counterpart in the reference; it is for

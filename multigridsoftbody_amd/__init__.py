@@ -141,6 +141,9 @@ SIGNATURES = {
     "mgsb_read_collision": (C.c_int, [_vp, _ip, _fp]),
     "mgsb_energy": (C.c_int, [_vp, C.POINTER(mgsb_energy_t)]),
     "mgsb_energy_noalloc": (C.c_int, [_vp, C.POINTER(mgsb_energy_t)]),
+    "mgsb_step_convergence": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(mgsb_energy_t)]),
+    "mgsb_step_error": (C.c_int, [_vp, C.c_int32, _dp, _dp]),
+    "mgsb_read_ep_density": (C.c_int, [_vp, _fp]),
     "mgsb_residual": (C.c_int, [_vp, _dp]),
     "mgsb_level_info": (C.c_int, [_vp, C.c_int32, _ip, _ip, _ip]),
     "mgsb_level_rho": (C.c_int, [_vp, C.c_int32, _dp]),
@@ -445,6 +448,29 @@ class Solver:
         e = mgsb_energy_t()
         _check(lib.mgsb_energy(self._h, C.byref(e)), "mgsb_energy")
         return {"Ek": e.Ek, "Ep": e.Ep, "Ec": e.Ec, "Nc": e.Nc, "dX": e.dX}
+
+    def step_convergence(self, iters=128):
+        """PDSolver::StepForConvergence: energies Ek + Ep + Ec after each of
+        `iters` reference iterations (index 0: at s) and the final terms; the
+        state is restored."""
+        E = np.zeros(iters + 1)
+        t = mgsb_energy_t()
+        _check(lib.mgsb_step_convergence(self._h, iters, E.ctypes.data_as(_dp), C.byref(t)),
+               "mgsb_step_convergence")
+        return E, {"Ek": t.Ek, "Ep": t.Ep, "Ec": t.Ec, "Nc": t.Nc, "dX": t.dX}
+
+    def step_error(self, iters=128):
+        """One step with the reference's energy error (E - E_conv) / (E0 - E_conv)."""
+        err = C.c_double()
+        E = np.zeros(3)
+        _check(lib.mgsb_step_error(self._h, iters, C.byref(err), E.ctypes.data_as(_dp)), "mgsb_step_error")
+        return err.value, {"E0": E[0], "E_conv": E[1], "E": E[2]}
+
+    def ep_density(self):
+        """Per-vertex Ep density of the finest level (calEveryVertEp)."""
+        out = np.zeros(self.levels[0].nverts, np.float32)
+        _check(lib.mgsb_read_ep_density(self._h, _fptr(out)), "mgsb_read_ep_density")
+        return out
 
     def residual(self):
         r = C.c_double()

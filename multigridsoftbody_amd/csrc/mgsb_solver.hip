@@ -53,6 +53,7 @@ constexpr int kVertPoint = 1;     // sphere point DCD + plane
 constexpr int kVertTri = 2;       // gather triangle DCD contributions
 constexpr int kVertWriteCol = 4;  // write collision flags / depth / energy
 constexpr int kVertContactJacobi = 8;  // V-cycle: exact 3x3 block step, no Chebyshev, where a collider acts
+constexpr int kVertDcdOnly = 16;       // collision detection only (with kVertWriteCol): positions untouched
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -542,6 +543,7 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
         cdepth[i] = depth;
         cenergy[i] = energy;
     }
+    if (flags & kVertDcdOnly) return;
     const float dc = c.x, dg = c.y, fix = c.z;
     if ((flags & kVertContactJacobi) && (cdx + cdy + cdz) > 0.f) {
         // V-cycle sweeps, a vertex a collider acts on: the Jacobi step solves
@@ -728,6 +730,59 @@ __global__ void __launch_bounds__(256) k_energy_tet(int T, const float4* __restr
         v[0] = e * (double)vol[g] * (double)(tk ? tk[g] : k) / 2.0;
     }
     block_reduce_store<1>(v, part);
+}
+
+// runCalEnergy's per-vertex Ep density (calEveryVertEp, PDSolver.cu:893-924),
+// per-tet part: e = k vol / 2 |F - R|^2, negated where det F < 1, and the
+// current signed volume (GetVolumn, utils.cu:254-275), written to the tet's
+// four corner slots of the CSR corner array so that each vertex sums its
+// incident tets in ascending tet order, the reference's loop order.
+__global__ void __launch_bounds__(256) k_ep_tet(int T, const float4* __restrict__ x, const int4* __restrict__ tet,
+                                                const float* __restrict__ B, const float* __restrict__ vol, float k,
+                                                const float* __restrict__ tk, const int4* __restrict__ slot,
+                                                float3* __restrict__ cf) {
+    const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (g >= T) return;
+    const int4 id = tet[g];
+    const float4 p0 = x[id.x], p1 = x[id.y], p2 = x[id.z], p3 = x[id.w];
+    float b[9];
+    for (int q = 0; q < 9; q++) b[q] = B[(size_t)q * T + g];
+    float D[9] = {p1.x - p0.x, p2.x - p0.x, p3.x - p0.x, p1.y - p0.y, p2.y - p0.y,
+                  p3.y - p0.y, p1.z - p0.z, p2.z - p0.z, p3.z - p0.z};
+    float F[9], RF[9];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) F[r * 3 + c] = D[r * 3 + 0] * b[c] + D[r * 3 + 1] * b[3 + c] + D[r * 3 + 2] * b[6 + c];
+    polar_rf<true>(F, RF);
+    float e = 0.f;
+    for (int q = 0; q < 9; q++) e = e + RF[q] * RF[q];
+    e = e * vol[g] * (tk ? tk[g] : k) / 2;
+    const float det = F[0] * F[4] * F[8] + F[1] * F[5] * F[6] + F[3] * F[7] * F[2] - F[2] * F[4] * F[6] -
+                      F[1] * F[3] * F[8] - F[0] * F[5] * F[7];
+    if (det < 1.0f) e = -e;
+    // GetVolumn: (AB, AC, AD) as columns, det by the first column's cofactors
+    const float r0 = D[4] * D[8] - D[7] * D[5], r1 = D[7] * D[2] - D[1] * D[8], r2 = D[1] * D[5] - D[4] * D[2];
+    const float v = (D[0] * r0 + D[3] * r1 + D[6] * r2) / 6.0f;
+    const int4 sl = slot[g];
+    const float3 q = make_float3(e, v, 0.f);
+    cf[sl.x] = q;
+    cf[sl.y] = q;
+    cf[sl.z] = q;
+    cf[sl.w] = q;
+}
+
+// Per-vertex part: density = sum of the incident tets' signed e / sum of
+// their current volumes, both summed in ascending tet order.
+__global__ void __launch_bounds__(256) k_ep_vert(int V, const int* __restrict__ rp, const float3* __restrict__ cf,
+                                                 float* __restrict__ dens) {
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    float es = 0.f, vs = 0.f;
+    for (int e = rp[i]; e < rp[i + 1]; e++) {
+        const float3 q = cf[e];
+        es = es + q.x;
+        vs = vs + q.y;
+    }
+    dens[i] = es / vs;
 }
 
 // Scratch of the allocation-free energy diagnostics (mgsb_energy_noalloc):
@@ -3543,6 +3598,179 @@ int mgsb_energy_noalloc(mgsb_solver* s, mgsb_energy_t* e) {
     e->Ep = out[4];
     e->Ec = out[2];
     e->Nc = out[3];
+    return MGSB_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The five energy terms of level 0 into out[0..5) on the device
+// (dX^2, m |x - s|^2, Ec, Nc, Ep), enqueued without a host wait.
+int energy_enqueue(mgsb_solver* s, double* out) {
+    DevLevel& d = s->L[0];
+    const int nbv = nblk(d.V, 256), nbt = nblk(d.T, 256);
+    int rc = ensure_partials(s, (size_t)std::max(nbv, nbt) * 4);
+    if (rc) return rc;
+    k_energy_vert<<<nbv, 256, 0, s->stream>>>(d.V, d.x, d.xprev, d.s, d.vc, d.cflag, d.cenergy, s->dpart);
+    HIPCHK(hipGetLastError());
+    k_reduce_final<4><<<1, 256, 0, s->stream>>>(nbv, s->dpart, out);
+    HIPCHK(hipGetLastError());
+    k_energy_tet<<<nbt, 256, 0, s->stream>>>(d.T, d.x, d.tet, d.B, d.vol, s->p.stiffness, d.tk, s->dpart);
+    HIPCHK(hipGetLastError());
+    k_reduce_final<1><<<1, 256, 0, s->stream>>>(nbt, s->dpart, out + 4);
+    HIPCHK(hipGetLastError());
+    return MGSB_OK;
+}
+
+void energy_terms(const double* r, double dt, mgsb_energy_t* e) {
+    e->dX = std::sqrt(r[0]);
+    e->Ek = r[1] / (2 * dt * dt);
+    e->Ep = r[4];
+    e->Ec = r[2];
+    e->Nc = r[3];
+}
+
+// Point DCD (spheres and plane) at the current positions of level 0, writing
+// the collision flags, depths and energies (DCDByPoint, PDSolver.cpp:228-233).
+int dcd_refresh(mgsb_solver* s) {
+    DevLevel& d = s->L[0];
+    return launch_vert(s, d, 1.0f, kVertPoint | kVertWriteCol | kVertDcdOnly);
+}
+
+// PDSolver::StepForConvergence (PDSolver.cpp:202-221) on level 0: save the
+// velocity, calculateST with the colliders where the next step will see
+// them, `iters` reference iterations with calculateIF's local step and the
+// energy after each, then the positions and velocity restored
+// (runResetPosVel, PDSolver.cu:270-273).  E receives iters + 1 totals
+// Ek + Ep + Ec (E[0] at s, after a DCD at s), last the terms of the final
+// iterate.  Nothing else of the solver's state is changed that the next
+// step does not rebuild.
+int convergence_reference(mgsb_solver* s, int iters, double* E, mgsb_energy_t* last) {
+    DevLevel& d = s->L[0];
+    int rc = pull_colliders(s);
+    if (rc) return rc;
+    const DevColliders saved = s->hcol;
+    for (int q = 0; q < s->hcol.nsph; q++) {  // the advance the next step's calculateST applies
+        float4& c = s->hcol.sph[q];
+        const float4 w = s->hcol.vel[q];
+        c.x = c.x + w.x * s->p.dt;
+        c.y = c.y + w.y * s->p.dt;
+        c.z = c.z + w.z * s->p.dt;
+    }
+    if ((rc = push_colliders(s))) return rc;
+    float4* vbak = nullptr;
+    double* er = nullptr;
+    if ((rc = dalloc(&vbak, (size_t)d.V))) return rc;
+    if ((rc = dalloc(&er, (size_t)(iters + 1) * 5))) {
+        (void)hipFree(vbak);
+        return rc;
+    }
+    std::vector<double> h((size_t)(iters + 1) * 5);
+    auto body = [&]() -> int {
+        int r;
+        HIPCHK(hipMemcpyAsync(vbak, d.v, sizeof(float4) * d.V, hipMemcpyDeviceToDevice, s->stream));
+        if ((r = launch_st(s, d, 0))) return r;
+        if ((r = dcd_refresh(s))) return r;
+        if ((r = energy_enqueue(s, er))) return r;
+        float omega = 1.0f;
+        const float rho = s->p.rho;
+        for (int i = 0; i < iters; i++) {
+            omega = 4 / (4 - rho * rho * omega);
+            if ((r = pd_iteration(s, d, MGSB_IF_PLAIN, omega, false, true))) return r;
+            if ((r = energy_enqueue(s, er + (size_t)(i + 1) * 5))) return r;
+        }
+        HIPCHK(hipMemcpyAsync(d.x, d.xlast, sizeof(float4) * d.V, hipMemcpyDeviceToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(d.v, vbak, sizeof(float4) * d.V, hipMemcpyDeviceToDevice, s->stream));
+        HIPCHK(hipMemcpyAsync(h.data(), er, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        return MGSB_OK;
+    };
+    const bool was = s->timing;
+    s->timing = false;
+    rc = body();
+    s->timing = was;
+    (void)hipFree(vbak);
+    (void)hipFree(er);
+    if (rc) return rc;
+    s->hcol = saved;
+    if ((rc = push_colliders(s))) return rc;
+    for (int i = 0; i <= iters; i++) {
+        mgsb_energy_t t;
+        energy_terms(&h[(size_t)i * 5], s->p.dt, &t);
+        if (E) E[i] = t.Ek + t.Ep + t.Ec;
+        if (last && i == iters) *last = t;
+    }
+    return MGSB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mgsb_step_convergence(mgsb_solver* s, int32_t iters, double* E, mgsb_energy_t* last) {
+    if (!s || iters < 1 || !E) {
+        set_error("mgsb_step_convergence: iters >= 1 and E (iters + 1 doubles) required");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    return convergence_reference(s, iters, E, last);
+}
+
+int mgsb_step_error(mgsb_solver* s, int32_t iters, double* err, double* E) {
+    if (!s || iters < 1 || !err) {
+        set_error("mgsb_step_error: iters >= 1 and err required");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    std::vector<double> Eh((size_t)iters + 1);
+    int rc = convergence_reference(s, iters, Eh.data(), nullptr);
+    if (rc) return rc;
+    if ((rc = mgsb_step(s, 1, nullptr))) return rc;
+    if ((rc = dcd_refresh(s))) return rc;
+    double* er = nullptr;
+    if ((rc = dalloc(&er, 5))) return rc;
+    double r[5];
+    rc = energy_enqueue(s, er);
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMemcpyAsync(r, er, sizeof(r), hipMemcpyDeviceToHost, s->stream);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(er);
+    if (rc) return rc;
+    HIPCHK(e);
+    mgsb_energy_t t;
+    energy_terms(r, s->p.dt, &t);
+    const double E0 = Eh[0], Ec = Eh[(size_t)iters], Ee = t.Ek + t.Ep + t.Ec;
+    *err = (Ee - Ec) / (E0 - Ec);
+    if (E) {
+        E[0] = E0;
+        E[1] = Ec;
+        E[2] = Ee;
+    }
+    return MGSB_OK;
+}
+
+int mgsb_read_ep_density(mgsb_solver* s, float* density) {
+    if (!s || !density) {
+        set_error("mgsb_read_ep_density: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(s->device));
+    DevLevel& d = s->L[0];
+    float* dd = nullptr;
+    int rc = dalloc(&dd, (size_t)d.V);
+    if (rc) return rc;
+    const int bs = 256;
+    k_ep_tet<<<nblk(d.T, bs), bs, 0, s->stream>>>(d.T, d.x, d.tet, d.B, d.vol, s->p.stiffness, d.tk, d.slot, d.cf);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        k_ep_vert<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.rp, d.cf, dd);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(density, dd, sizeof(float) * d.V, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(dd);
+    HIPCHK(e);
     return MGSB_OK;
 }
 

@@ -79,6 +79,8 @@ _SIGS = {
     "step_mg": (None, [_vp, C.POINTER(orc_params), C.c_int]),
     "energy": (None, [_vp, C.POINTER(orc_params), _dp]),
     "vertex_normals": (None, [_vp]),
+    "step_convergence": (None, [_vp, C.POINTER(orc_params), C.c_int, _dp]),
+    "ep_density": (None, [_vp, _fp]),
     "residual_norm": (C.c_double, [_vp, C.POINTER(orc_params)]),
     "polar_f": (C.c_float, [_fp, _fp, C.c_int]),
     "inverse3_f": (C.c_float, [_fp, _fp]),
@@ -217,6 +219,18 @@ class Level:
         out = np.zeros(5, np.float64)
         self._f("energy")(self._h, C.byref(p), out.ctypes.data_as(_dp))
         return dict(zip(("Ek", "Ep", "Ec", "Nc", "dX"), out.tolist()))
+
+    def step_convergence(self, p, iters=128):
+        """PDSolver::StepForConvergence: Ek + Ep + Ec at s and after each of
+        `iters` iterations; positions and velocity restored."""
+        E = np.zeros(iters + 1)
+        self._f("step_convergence")(self._h, C.byref(p), iters, E.ctypes.data_as(_dp))
+        return E
+
+    def ep_density(self):
+        out = np.zeros(self.nv, np.float32)
+        self._f("ep_density")(self._h, out.ctypes.data_as(_fp))
+        return out
 
     def vertex_normals(self):
         self._f("vertex_normals")(self._h)

@@ -1092,6 +1092,75 @@ void PFX(energy)(const orc_level* L, const orc_params* p, double* out) {
     out[0] = Ek; out[1] = Ep; out[2] = Ec; out[3] = Nc; out[4] = dX;
 }
 
+/* PDSolver::StepForConvergence (PDSolver.cpp:202-221): save the velocity,
+ * calculateST (colliders advanced as the next step will see them), `iters`
+ * iterations of DCDByPoint + calculateIF (mode 0) + omega + calculatePOS
+ * with runCalEnergy after each, then x = x_last and v restored
+ * (runResetPosVel, PDSolver.cu:270-273).  E[0] = Ek + Ep + Ec at s after a
+ * DCD at s (PDSolver::Step's log, PDSolver.cpp:252-255), E[i] after
+ * iteration i.  `p` is not changed. */
+void PFX(step_convergence)(orc_level* L, const orc_params* p0, int iters, double* E) {
+    orc_params p = *p0;
+    move_spheres(&p);
+    size_t n3 = (size_t)L->V * 3;
+    real* vbak = (real*)xcalloc(n3, sizeof(real));
+    memcpy(vbak, L->v, sizeof(real) * n3);
+    calc_st(L, &p);
+    dcd_point(L, &p);
+    double e[5];
+    PFX(energy)(L, &p, e);
+    E[0] = e[0] + e[1] + e[2];
+    orc_params pp = p;
+    pp.if_mode = 0;
+    real omega = 1, rho = p.rho;
+    for (int i = 0; i < iters; i++) {
+        omega = 4 / (4 - rho * rho * omega);
+        pd_iteration(L, &pp, omega, 0, 0);
+        PFX(energy)(L, &pp, e);
+        E[i + 1] = e[0] + e[1] + e[2];
+    }
+    memcpy(L->x, L->xlast, sizeof(real) * n3);
+    memcpy(L->v, vbak, sizeof(real) * n3);
+    free(vbak);
+}
+
+/* runCalEnergy's per-vertex Ep density (calEveryVertEp, PDSolver.cu:893-924):
+ * per tet e = sum (F - R)^2 (float differences squared in double) * vol * k / 2,
+ * negated where det F < 1; EpSum and VSum (current signed volume, GetVolumn)
+ * are float accumulators summed in tet order; density = EpSum / VSum. */
+void PFX(ep_density)(const orc_level* L, float* out) {
+    float* es = (float*)xcalloc((size_t)L->V, sizeof(float));
+    float* vs = (float*)xcalloc((size_t)L->V, sizeof(float));
+    for (int t = 0; t < L->T; t++) {
+        const int* id = &L->tets[t * 4];
+        const real* X = L->x;
+        real D[9];
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) D[r * 3 + q] = X[id[q + 1] * 3 + r] - X[id[0] * 3 + r];
+        const real* B = &L->B[t * 9];
+        real F[9], R[9];
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) F[r * 3 + q] = D[r * 3 + 0] * B[q] + D[r * 3 + 1] * B[3 + q] + D[r * 3 + 2] * B[6 + q];
+        real det = polar(F, R, 0);
+        double e = 0;
+        for (int i = 0; i < 9; i++) {
+            real d = F[i] - R[i];
+            e += (double)d * (double)d;
+        }
+        e = e * (double)L->vol[t] * (double)tet_k(L, t) / 2;
+        if (det < 1.0f) e = -e;
+        v3 a = vload(X + id[0] * 3), b = vload(X + id[1] * 3), c = vload(X + id[2] * 3), d4 = vload(X + id[3] * 3);
+        float vol = (float)get_volume(a, b, c, d4);
+        for (int j = 0; j < 4; j++) {
+            es[id[j]] = (float)((double)es[id[j]] + e);
+            vs[id[j]] += vol;
+        }
+    }
+    for (int i = 0; i < L->V; i++) out[i] = es[i] / vs[i];
+    free(es);
+    free(vs);
+}
+
 /* Vertex normals, runUpdateOutsideTetVertNormal (PDSolver.cu:394-460):
  * unit triangle normals summed per vertex then normalised if |n| > FLT_EPSILON. */
 void PFX(vertex_normals)(orc_level* L) {

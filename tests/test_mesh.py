@@ -35,6 +35,7 @@ MESHES = [
     ("Y_4_40_4.msh", 817, 2606, 1344, 80.0),
     ("cube10_1_1.msh", None, None, None, None),
     ("cube40_4_4.msh", None, None, None, None),
+    ("cube120_12_12.msh", 10937, 48087, 12096, 80.0),
     ("liver700_3500.msh", None, None, None, None),
 ]
 
@@ -110,7 +111,8 @@ def test_degenerate_tet_is_an_error_not_an_exit():
         m.precompute(1.0)
 
 
-@pytest.mark.parametrize("pair", [("Y_4_40_4.msh", "Y_1_10_1.msh"), ("cube40_4_4.msh", "cube10_1_1.msh")])
+@pytest.mark.parametrize("pair", [("Y_4_40_4.msh", "Y_1_10_1.msh"), ("cube40_4_4.msh", "cube10_1_1.msh"),
+                                  ("cube120_12_12.msh", "cube40_4_4.msh")])
 def test_bindings_match_brute_force_oracle(golden, pair):
     """Spatial-hash binding == PDSolver_MG's brute-force scans, bit for bit."""
     fine = Mesh.load_msh(os.path.join(golden, pair[0]))

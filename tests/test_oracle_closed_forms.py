@@ -198,3 +198,65 @@ def test_two_level_schedule_free_fall_closed_form():
     for L, m in ((Lf, fine), (Lc, coarse)):
         assert np.allclose(L.positions(), m.pos.astype(np.float64) + d, rtol=0, atol=1e-9)
         assert np.allclose(L.get(O.V).reshape(-1, 3), v, rtol=0, atol=1e-7)
+
+
+@pytest.mark.parametrize("sig", [(1.1, 0.9, 1.25), (0.9, 0.95, 1.05)])
+def test_ep_density_uniform_stretch_closed_form(sig):
+    """calEveryVertEp (PDSolver.cu:893-924) under x = Q diag(sig) X: every
+    tet has |F - R|^2 = sum (sig - 1)^2 and current volume det(F) vol, so
+    every vertex's density is +-(k/2) sum (sig - 1)^2 / prod(sig), negative
+    when det F < 1."""
+    from multigridsoftbody_amd import Mesh
+
+    m = Mesh.box(2, 4, 2, (-1.0, -2.0, -1.0), (1.0, 2.0, 1.0))
+    k = 5e3
+    sig = np.array(sig)
+    th = 0.7
+    Q = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    x = (m.pos.astype(np.float64) @ (Q @ np.diag(sig)).T).astype(np.float32)
+    want = 0.5 * k * ((sig - 1) ** 2).sum() / sig.prod() * (1 if sig.prod() >= 1 else -1)
+    # the reference sums the densities in float vectors (Simulator.h:32-34),
+    # so the FP64 oracle carries their rounding too
+    for fp64, rtol in ((True, 3e-6), (False, 3e-5)):
+        L = O.Level(m.pos, m.tets, m.tris, stiffness=k, fp64=fp64)
+        L.set_positions(x)
+        d = L.ep_density()
+        assert np.allclose(d, want, rtol=rtol), (fp64, d[:4], want)
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_convergence_reference_step_one_free_vertex(fp64):
+    """PDSolver::StepForConvergence (PDSolver.cpp:202-221) on the one-free-vertex
+    tet: the energy after each iteration is E(x) = m (x - s)^2 / (2 h^2) +
+    K (x - 1)^2 / 2 (Ek + Ep; no collider) at the iterates of the
+    hand-derived Chebyshev-Jacobi sequence, E[0] = E(s), the last tends to
+    the minimum E(x*), and positions and velocity come back unchanged."""
+    h, damp, rho = F32(1 / 60), F32(0.95), F32(0.9992)
+    k, dens = 1e5, 1000.0
+    L = O.Level(_POS, _TET, None, stiffness=k, density=dens, fp64=fp64)
+    L.set_fix(np.array([0, 1, 0, 0], np.float32))
+    vel = np.zeros((4, 3), np.float32)
+    vel[1, 0] = 3.0
+    L.set_velocity(vel)
+    p = O.params(dt=h, damping=damp, stiffness=k, density=dens, rho=rho, gravity=(0, 0, 0), iters=10, if_mode=1)
+    x_before, v_before = L.positions().copy(), L.get(O.V).copy()
+    iters = 128
+    E = L.step_convergence(p, iters)
+    assert np.array_equal(L.positions(), x_before) and np.array_equal(L.get(O.V), v_before)
+    K, m = k / 6, dens / 24
+    dc = m / (h * h)
+    s = 1.0 + h * (damp * 3.0)
+    xs = (dc * s + K) / (dc + K)
+    Ef = lambda x: m * (x - s) ** 2 / (2 * h * h) + K * (x - 1) ** 2 / 2  # noqa: E731
+    e = ep = s - xs
+    w = 1.0
+    want = [Ef(s)]
+    for _ in range(iters):
+        w = 4 / (4 - rho * rho * w)
+        e, ep = w * (0.4 * e - ep) + ep, e
+        want.append(Ef(xs + e))
+    want = np.array(want)
+    rtol = 1e-9 if fp64 else 2e-4
+    assert np.allclose(E, want, rtol=rtol, atol=rtol * want[0])
+    assert abs(E[-1] - Ef(xs)) <= (1e-6 if fp64 else 1e-4) * (want[0] - Ef(xs))  # converged to the minimum
+    assert want[0] > want[1] and E[-1] < E[1]
