@@ -16,8 +16,9 @@ s; learned_s = the MLP's coarse correction applied to s alone (the network's
 own gain over plain MG); temporal = the V-cycle started from the temporal
 extrapolation of the last two steps' corrections; learned = temporal start
 plus the MLP's coarse correction.  Each network is trained first, untimed, on
-2,000 recorded steps (plain and temporal schedule respectively) of a
-trajectory with a different initial velocity.  The default "ab" times all
+recorded steps (plain and temporal schedule respectively) of 33 trajectories
+with other initial velocities: the first 60 steps of 32 of them and 600 steps
+of one more.  The default "ab" times all
 four on the same workload (5 pre- and post-sweeps each) and reports the
 learned schedule as the value; it also times plain MG with 3, 4 and 6 sweeps
 and reports the learned schedule against the fastest plain MG as well.
@@ -40,7 +41,10 @@ METRIC = "sim steps/sec at rel. residual ≤1e-3, learned vs plain MG; achieved 
 HBM_PEAK_GBS = 8000.0
 VCYCLE = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, tol=1e-3, relax_coarse=0.6,
               rho_smooth=0.9, coarse_direct=1)
-TRAIN = dict(record_steps=2000, epochs=10, batch=4096, lr=2e-3, seed=0, velocity_seed=1)
+# training data: the first 60 steps of 32 trajectories from other initial
+# velocities (seeds 1..32), where the extrapolation's error is largest, plus
+# 600 steps of one more (seed 40)
+TRAIN = dict(episodes=[(sd, 60) for sd in range(1, 33)] + [(40, 600)], epochs=30, batch=4096, lr=1e-3, seed=0)
 # HBM bytes per k_tet launch on the medium box (V-cycle split-position mode):
 # rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy),
 # profiles/k_tet_learned_ab_medium_box.md
@@ -103,20 +107,28 @@ def make_solver(levels, schedule, cfg, use_graph=1, fixed_cycles=0, check_every=
 
 
 def train_network(levels, cfg, check_every, on_s=False):
-    """Record TRAIN['record_steps'] steps of the temporal-start V-cycle (or,
-    on_s, of plain MG: targets relative to s) on a trajectory with another
-    initial velocity, train the MLP on the device."""
-    s = make_solver(levels, "vcycle" if on_s else "temporal", cfg, check_every=check_every,
-                    velocity_seed=TRAIN["velocity_seed"])
-    s.learn_record(TRAIN["record_steps"])
+    """Record the episodes of TRAIN (trajectories from other initial
+    velocities) under the temporal-start V-cycle (or, on_s, plain MG: targets
+    relative to s), train the MLP on the device."""
+    from multigridsoftbody_amd import workloads as W
+
+    eps = TRAIN["episodes"]
+    s = make_solver(levels, "vcycle" if on_s else "temporal", cfg, check_every=check_every, velocity_seed=eps[0][0])
+    fix = cfg["fix"](levels[0]) if "fix" in cfg else W.top_fixflag(levels[0])
+    s.learn_record(sum(n for _, n in eps))
     t0 = time.perf_counter()
-    s.step(TRAIN["record_steps"])
+    for i, (seed, n) in enumerate(eps):
+        if i:  # a new trajectory from rest
+            s.set_positions(levels[0].pos)
+            s.set_velocity(W.seeded_velocity(levels[0], fix, seed=seed))
+        s.step(n)
     s.sync()
     t1 = time.perf_counter()
     hist = s.learn_train(epochs=TRAIN["epochs"], batch=TRAIN["batch"], lr=TRAIN["lr"], seed=TRAIN["seed"])
     t2 = time.perf_counter()
-    log(f"[bench] {'plain' if on_s else 'temporal'} recording: {TRAIN['record_steps']} steps in {t1 - t0:.1f}s, "
-        f"trained {TRAIN['epochs']} epochs in {t2 - t1:.1f}s, loss {hist[0]:.4f} -> {hist[-1]:.4f} (not timed)")
+    log(f"[bench] {'plain' if on_s else 'temporal'} recording: {sum(n for _, n in eps)} steps of {len(eps)} "
+        f"trajectories in {t1 - t0:.1f}s, trained {TRAIN['epochs']} epochs in {t2 - t1:.1f}s, "
+        f"loss {hist[0]:.4f} -> {hist[-1]:.4f} (not timed)")
     return s.learn_weights(), s.learn_norm(), [float(hist[0]), float(hist[-1])]
 
 
@@ -312,7 +324,11 @@ def main():
     if "vcycle" in sps and "learned_s" in sps:
         config["learned_on_s_vs_plain_mg"] = round(sps["learned_s"] / sps["vcycle"], 3)
     if nets:
-        config["training"] = dict(TRAIN, loss={k: [round(v[2][0], 4), round(v[2][1], 4)] for k, v in nets.items()})
+        eps = TRAIN["episodes"]
+        config["training"] = dict(epochs=TRAIN["epochs"], batch=TRAIN["batch"], lr=TRAIN["lr"], seed=TRAIN["seed"],
+                                  trajectories=len(eps), steps=sum(n for _, n in eps),
+                                  velocity_seeds=f"{eps[0][0]}..{eps[-1][0]}",
+                                  loss={k: [round(v[2][0], 4), round(v[2][1], 4)] for k, v in nets.items()})
     out = {
         "metric": METRIC,
         "value": steps_per_s,

@@ -110,7 +110,14 @@ typedef struct mgsb_params {
                            * without it, for 8, 16, ... 256 steps before trying again */
     int32_t vert_lanes;   /* lanes that gather one vertex's row of corner forces in the
                            * sweep and residual kernels: 1, 2, 4 or 8 (0: 4; tuned)   */
-    int32_t reserved[1];
+    float learn_slack;    /* learned schedule: the MLP's coarse correction is kept when the
+                           * residual it leaves is at most learn_slack times the best one so
+                           * far (1, default: it must lower the residual; 0 reads as 1).  A
+                           * coarse correction is normally followed by smoothing before it is
+                           * judged: right after it, the high-frequency error its interpolation
+                           * adds can raise the residual while the smooth error it removes is
+                           * what the next half-cycle needs gone.  A non-finite residual is
+                           * never kept. */
 } mgsb_params;
 
 typedef struct mgsb_step_stats {

@@ -72,7 +72,7 @@ class mgsb_params(C.Structure):
         ("warm_start", C.c_int32),
         ("learn_gate", C.c_int32),
         ("vert_lanes", C.c_int32),
-        ("reserved", C.c_int32 * 1),
+        ("learn_slack", C.c_float),
     ]
 
 

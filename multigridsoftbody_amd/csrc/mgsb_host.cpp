@@ -863,7 +863,7 @@ void mgsb_default_params(mgsb_params* p) {
     p->warm_start = 0;
     p->learn_gate = 1;
     p->vert_lanes = 0;
-    p->reserved[0] = 0;
+    p->learn_slack = 1.0f;
 }
 
 const char* mgsb_version(void) { return "mgsb 0.1 (gfx950)"; }

@@ -21,6 +21,23 @@ int mlp_set_params(Mlp* p, const float* w, int n, hipStream_t st);
 int mlp_get_params(Mlp* p, float* w, int n, hipStream_t st);
 // Fused BF16-MFMA inference: X (n x in, device) -> Y (row stride ldy, device).
 int mlp_infer(Mlp* p, const float* X, int n, float* Y, int ldy, hipStream_t st, const int* skip);
+// The learned coarse correction in one launch (features, network, prior and
+// mask): epred[j] = vcl[j].z * (e1[j] + y_j * out_sd + out_mu), y = mlp of the
+// standardised features (injected fine velocity vf through (aids, aw), rest
+// position X1, last coarse deltas e1, e2) of each of the n level-1 vertices.
+constexpr int kCoarseFeat = 12;
+struct CoarseIO {
+    const float4* vf;
+    const int4* aids;
+    const float4* aw;
+    const float4* X1;
+    const float4* e1;
+    const float4* e2;
+    const float* norm;  // in_mu[12] in_sd[12] out_mu[3] out_sd[3]
+    const float4* vcl;
+    float4* epred;
+};
+int mlp_infer_coarse(Mlp* p, int n, const CoarseIO& io, hipStream_t st);
 // Test paths: gradients of the mean squared error, and the GEMM-path forward.
 int mlp_grad(Mlp* p, const float* X_host, const float* T_host, int n, float* grad_host, double* loss, hipStream_t st);
 int mlp_forward_train_path(Mlp* p, const float* X_host, int n, float* Y_host, hipStream_t st);

@@ -223,12 +223,30 @@ __global__ void __launch_bounds__(256) k_pack_bf16(int K, int N, int Kp, int Np,
 // tile accumulates its K blocks in order, as a one-wave-per-tile product
 // would.  Y = mlp(X).
 constexpr int kH = MLP_HIDDEN;
-__global__ void __launch_bounds__(256) k_mlp_infer(int n, int in, int out, const float* __restrict__ X,
-                                                   const short* __restrict__ W1, const short* __restrict__ W2,
-                                                   const short* __restrict__ W3, const short* __restrict__ W4,
-                                                   const float* __restrict__ b1, const float* __restrict__ b2,
-                                                   const float* __restrict__ b3, const float* __restrict__ b4,
-                                                   float* __restrict__ Y, int ldy, const int* __restrict__ skip) {
+
+// Row-major FP32 inputs X (n x in) and outputs Y (row stride ldy).
+struct DenseIn {
+    const float* X;
+    int n, in;
+    __device__ void load4(int row, int k0, float (&v)[4]) const {
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = (row < n && k0 + q < in) ? X[(long)row * in + k0 + q] : 0.f;
+    }
+};
+struct DenseOut {
+    float* Y;
+    int n, out, ldy;
+    __device__ void store(int row, int col, float v) const {
+        if (row < n && col < out) Y[(long)row * ldy + col] = v;
+    }
+};
+
+template <class In, class Out>
+__global__ void __launch_bounds__(256) k_mlp_infer(int n, int nout, In in, Out out, const short* __restrict__ W1,
+                                                   const short* __restrict__ W2, const short* __restrict__ W3,
+                                                   const short* __restrict__ W4, const float* __restrict__ b1,
+                                                   const float* __restrict__ b2, const float* __restrict__ b3,
+                                                   const float* __restrict__ b4, const int* __restrict__ skip) {
     if (skip && *skip) return;
     __shared__ short hs[2][16][kH];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -238,11 +256,10 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int in, int out, const
     // layer 1: K = 16 (inputs zero-padded)
     {
         bf16x4 a;
+        float x4[4];
+        in.load4(base + r, kq, x4);
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int k = kq + q, row = base + r;
-            a[q] = f2bf((row < n && k < in) ? X[(long)row * in + k] : 0.f);
-        }
+        for (int q = 0; q < 4; q++) a[q] = f2bf(x4[q]);
         for (int tt = 0; tt < kTilesPerWave; tt++) {
             const int t = wave * kTilesPerWave + tt;
             bf16x4 b;
@@ -296,10 +313,8 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int in, int out, const
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int row = base + kq + q;
-        if (row < n && r < out) Y[(long)row * ldy + r] = acc[q] + b4[r];
-    }
+    for (int q = 0; q < 4; q++)
+        if (r < nout) out.store(base + kq + q, r, acc[q] + b4[r]);
 }
 
 // ---------------------------------------------------------------------------
@@ -439,8 +454,76 @@ int mlp_get_params(Mlp* p, float* w, int n, hipStream_t st) {
 int mlp_infer(Mlp* p, const float* X, int n, float* Y, int ldy, hipStream_t st, const int* skip) {
     const int blocks = (n + 15) / 16;  // one workgroup per 16 samples
     if (blocks == 0) return MGSB_OK;
-    k_mlp_infer<<<blocks, 256, 0, st>>>(n, p->in, p->out, X, p->W1b, p->W2b, p->W3b, p->W4b, p->w + p->off[1],
-                                        p->w + p->off[3], p->w + p->off[5], p->w + p->off[7], Y, ldy, skip);
+    k_mlp_infer<<<blocks, 256, 0, st>>>(n, p->out, DenseIn{X, n, p->in}, DenseOut{Y, n, p->out, ldy}, p->W1b, p->W2b, p->W3b,
+                                        p->W4b, p->w + p->off[1], p->w + p->off[3], p->w + p->off[5],
+                                        p->w + p->off[7], skip);
+    MLPCHK(hipGetLastError());
+    return MGSB_OK;
+}
+
+// The learned coarse correction as one launch: the features of each level-1
+// vertex (fine velocity injected through the coarse-in-fine binding, rest
+// position, the last two coarse deltas), standardised, through the network,
+// and its output de-standardised onto the last delta and masked by the
+// level's Dirichlet flags (the same values as k_features, mlp_infer and
+// k_apply_pred in sequence).
+struct CoarseIn {
+    CoarseIO io;
+    int n;
+    __device__ void load4(int row, int k0, float (&v)[4]) const {
+        float raw[4] = {0.f, 0.f, 0.f, 0.f};
+        if (row < n) {
+            if (k0 == 0) {
+                const int4 id = io.aids[row];
+                const float4 w = io.aw[row];
+                const float4 a = io.vf[id.x], b = io.vf[id.y], c = io.vf[id.z], d = io.vf[id.w];
+                raw[0] = w.x * a.x + w.y * b.x + w.z * c.x + w.w * d.x;
+                raw[1] = w.x * a.y + w.y * b.y + w.z * c.y + w.w * d.y;
+                raw[2] = w.x * a.z + w.y * b.z + w.z * c.z + w.w * d.z;
+                raw[3] = io.X1[row].x;
+            } else if (k0 == 4) {
+                const float4 X = io.X1[row], e = io.e1[row];
+                raw[0] = X.y;
+                raw[1] = X.z;
+                raw[2] = e.x;
+                raw[3] = e.y;
+            } else if (k0 == 8) {
+                const float4 e = io.e1[row], f = io.e2[row];
+                raw[0] = e.z;
+                raw[1] = f.x;
+                raw[2] = f.y;
+                raw[3] = f.z;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int k = k0 + q;
+            v[q] = (row < n && k < kCoarseFeat) ? (raw[q] - io.norm[k]) / io.norm[kCoarseFeat + k] : 0.f;
+        }
+    }
+};
+struct CoarseOut {
+    CoarseIO io;
+    int n;
+    __device__ void store(int row, int col, float v) const {
+        if (row >= n || col >= 3) return;
+        const float mu = io.norm[2 * kCoarseFeat + col], sd = io.norm[2 * kCoarseFeat + 3 + col];
+        const float4 e = io.e1[row];
+        const float prior = col == 0 ? e.x : (col == 1 ? e.y : e.z);
+        reinterpret_cast<float*>(io.epred + row)[col] = io.vcl[row].z * (prior + (v * sd + mu));
+    }
+};
+
+int mlp_infer_coarse(Mlp* p, int n, const CoarseIO& io, hipStream_t st) {
+    if (p->in != kCoarseFeat || p->out != 3) {
+        set_error("mlp_infer_coarse: the network must map 12 features to 3 outputs");
+        return MGSB_ERR_STATE;
+    }
+    const int blocks = (n + 15) / 16;
+    if (blocks == 0) return MGSB_OK;
+    k_mlp_infer<<<blocks, 256, 0, st>>>(n, 3, CoarseIn{io, n}, CoarseOut{io, n}, p->W1b, p->W2b, p->W3b, p->W4b,
+                                        p->w + p->off[1], p->w + p->off[3], p->w + p->off[5], p->w + p->off[7],
+                                        nullptr);
     MLPCHK(hipGetLastError());
     return MGSB_OK;
 }

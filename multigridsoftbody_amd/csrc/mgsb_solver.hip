@@ -137,6 +137,17 @@ __device__ __forceinline__ float3 gather_lanes(const float3* __restrict__ cf, in
     return make_float3(fx, fy, fz);
 }
 
+// (P e)_i: the coarse vector e interpolated at fine vertex i through its
+// barycentric binding in a coarse tet (ids, weights).
+__device__ __forceinline__ float3 prolong_at(int i, const float4* __restrict__ e, const int4* __restrict__ ids,
+                                             const float4* __restrict__ w) {
+    const int4 id = ids[i];
+    const float4 ww = w[i];
+    const float4 a = e[id.x], b = e[id.y], c = e[id.z], d = e[id.w];
+    return make_float3(ww.x * a.x + ww.y * b.x + ww.z * c.x + ww.w * d.x, ww.x * a.y + ww.y * b.y + ww.z * c.y + ww.w * d.y,
+                       ww.x * a.z + ww.y * b.z + ww.z * c.z + ww.w * d.z);
+}
+
 // calculateST (PDSolver.cu:72-116).  Block 0 / lane 0 first advances sphere
 // centres by vel*dt (UpdateCollider before Step, Simulator.cpp:146-149).
 // split: x and x_prev take the in-step correction u (0, or with U1/U2 the
@@ -145,7 +156,9 @@ __global__ void __launch_bounds__(256) k_st(int V, float4* __restrict__ x, float
                                             float4* __restrict__ xlast, float4* __restrict__ s,
                                             float4* __restrict__ xprev, const float4* __restrict__ vc, float dt,
                                             float damp, float gx, float gy, float gz, DevColliders* col, int advance,
-                                            int split, const float4* __restrict__ U1, const float4* __restrict__ U2) {
+                                            int split, const float4* __restrict__ U1, const float4* __restrict__ U2,
+                                            const float4* __restrict__ epred, const int4* __restrict__ iids,
+                                            const float4* __restrict__ iw) {
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (advance && blockIdx.x == 0 && threadIdx.x == 0) {
         for (int q = 0; q < col->nsph; q++) {
@@ -174,11 +187,16 @@ __global__ void __launch_bounds__(256) k_st(int V, float4* __restrict__ x, float
     v[i] = u;
     s[i] = p;
     if (split) {  // x and x_prev hold the in-step correction u = x - s
-        // U1 != nullptr: temporal warm start u_t = fix (2 u_{n-1} - u_{n-2})
+        // U1 != nullptr: temporal warm start u_t = fix (2 u_{n-1} - u_{n-2});
+        // epred != nullptr: plus the prolongated learned coarse correction
         float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
         if (U1) {
             const float4 a = U1[i], b = U2[i];
             q = make_float4(fix * (2.f * a.x - b.x), fix * (2.f * a.y - b.y), fix * (2.f * a.z - b.z), 0.f);
+        }
+        if (epred) {
+            const float3 e = prolong_at(i, epred, iids, iw);
+            q = make_float4(q.x + fix * e.x, q.y + fix * e.y, q.z + fix * e.z, 0.f);
         }
         x[i] = q;
         xprev[i] = q;
@@ -821,6 +839,7 @@ struct VState {
     int maxh;       // half-cycle cap of a step (2 max_cycles)
     int notdone;    // the step has neither converged nor reached the cap (no checks: 0);
                     // the finish piece, enqueued before the host reads the verdict, skips
+    double slack;   // the learned stage is kept while its residual <= slack * res
 };
 
 // Check modes of the fused residual reduction.
@@ -862,7 +881,8 @@ __device__ void vstate_apply(VState* st, double t, int mode, int* hm, double rel
             st->res_t = r;
         else
             st->res_ws = r;
-        if (!(r <= st->res)) {
+        const double bound = mode == kChkWarm ? st->slack * st->res : st->res;
+        if (!(r <= bound)) {
             st->revert = st->accepted == 1 ? 2 : 1;
             st->fallback |= mode == kChkTemporal ? 1 : 2;
             st->cf_ok = 0;
@@ -967,7 +987,7 @@ __global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict_
 // the extrapolation.  Converged already at s: x (which holds the
 // extrapolation) must return to s, so revert is set without a verdict.
 __global__ void __launch_bounds__(256) k_check2(int nb, const double* __restrict__ part, VState* st, int* hm,
-                                                unsigned* seqc) {
+                                                unsigned* seqc, int judge) {
     double v[2] = {0.0, 0.0};
     for (int q = threadIdx.x; q < nb; q += blockDim.x) {
         v[0] += part[2 * q];
@@ -977,7 +997,7 @@ __global__ void __launch_bounds__(256) k_check2(int nb, const double* __restrict
     if (threadIdx.x == 0) {
         vstate_apply(st, v[0], kChkStart, hm);
         if (!st->converged)
-            vstate_apply(st, v[1], kChkTemporal, hm);
+            vstate_apply(st, v[1], judge, hm);
         else
             st->revert = 1;
         bump_seq(seqc, hm, st);
@@ -1193,13 +1213,22 @@ __global__ void __launch_bounds__(kBodyCheckThreads) k_body_check(const int* __r
 // of the last two steps' corrections (split positions: x holds u = x - s).
 __global__ void __launch_bounds__(256) k_warm_temporal(int V, float4* __restrict__ x, float4* __restrict__ xprev,
                                                        const float4* __restrict__ U1, const float4* __restrict__ U2,
-                                                       const float4* __restrict__ vc, const int* __restrict__ skip) {
+                                                       const float4* __restrict__ vc, const int* __restrict__ skip,
+                                                       const float4* __restrict__ epred, const int4* __restrict__ iids,
+                                                       const float4* __restrict__ iw) {
     if (skip && *skip) return;
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= V) return;
-    const float4 a = U1[i], b = U2[i];
     const float f = vc[i].z;
-    const float4 q = make_float4(f * (2.f * a.x - b.x), f * (2.f * a.y - b.y), f * (2.f * a.z - b.z), 0.f);
+    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (U1) {
+        const float4 a = U1[i], b = U2[i];
+        q = make_float4(f * (2.f * a.x - b.x), f * (2.f * a.y - b.y), f * (2.f * a.z - b.z), 0.f);
+    }
+    if (epred) {
+        const float3 e = prolong_at(i, epred, iids, iw);
+        q = make_float4(q.x + f * e.x, q.y + f * e.y, q.z + f * e.z, 0.f);
+    }
     x[i] = q;
     xprev[i] = q;
 }
@@ -1229,14 +1258,15 @@ __global__ void __launch_bounds__(256) k_revert(int V, float4* __restrict__ x, f
     xprev[i] = q;
 }
 
-// Features of the learned coarse correction at each vertex of level 1:
-// restricted residual g(s) (3), fine velocity injected through the
-// coarse-in-fine binding (3), rest position (3), the previous step's coarse
-// correction (3); normalised with (mu, sd).  Raw values optionally recorded.
+// Features of the learned coarse correction at each vertex of level 1, all
+// known before the step starts: the fine velocity injected through the
+// coarse-in-fine binding (3), the rest position (3), and the coarse deltas
+// achieved by the last two steps (3 + 3, k_target); normalised with
+// (mu, sd).  Raw values optionally recorded.
 constexpr int kFeat = 12;
-__global__ void __launch_bounds__(256) k_features(int Vc, const float4* __restrict__ r1, const float4* __restrict__ vf,
-                                                  const int4* __restrict__ aids, const float4* __restrict__ aw,
-                                                  const float4* __restrict__ X0, const float4* __restrict__ eprev,
+__global__ void __launch_bounds__(256) k_features(int Vc, const float4* __restrict__ vf, const int4* __restrict__ aids,
+                                                  const float4* __restrict__ aw, const float4* __restrict__ X0,
+                                                  const float4* __restrict__ eprev, const float4* __restrict__ eprev2,
                                                   const float* __restrict__ norm, float* __restrict__ feat,
                                                   float* __restrict__ rec) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1244,12 +1274,11 @@ __global__ void __launch_bounds__(256) k_features(int Vc, const float4* __restri
     const int4 id = aids[j];
     const float4 w = aw[j];
     const float4 a = vf[id.x], b = vf[id.y], c = vf[id.z], d = vf[id.w];
-    const float4 r = r1[j], X = X0[j], ep = eprev[j];
-    const float raw[kFeat] = {r.x, r.y, r.z,
-                              w.x * a.x + w.y * b.x + w.z * c.x + w.w * d.x,
+    const float4 X = X0[j], ep = eprev[j], ep2 = eprev2[j];
+    const float raw[kFeat] = {w.x * a.x + w.y * b.x + w.z * c.x + w.w * d.x,
                               w.x * a.y + w.y * b.y + w.z * c.y + w.w * d.y,
                               w.x * a.z + w.y * b.z + w.z * c.z + w.w * d.z,
-                              X.x, X.y, X.z, ep.x, ep.y, ep.z};
+                              X.x, X.y, X.z, ep.x, ep.y, ep.z, ep2.x, ep2.y, ep2.z};
 #pragma unroll
     for (int q = 0; q < kFeat; q++) {
         if (rec) rec[(long)j * kFeat + q] = raw[q];
@@ -1257,28 +1286,31 @@ __global__ void __launch_bounds__(256) k_features(int Vc, const float4* __restri
     }
 }
 
-// epred = vfix * (pred * out_sd + out_mu)
+// epred = vfix * (eprev + pred * out_sd + out_mu): the network predicts the
+// change of the coarse delta from the last step's (which is its prior)
 __global__ void __launch_bounds__(256) k_apply_pred(int Vc, const float* __restrict__ pred,
                                                     const float* __restrict__ norm, const float4* __restrict__ vcl,
-                                                    float4* __restrict__ epred) {
+                                                    const float4* __restrict__ eprev, float4* __restrict__ epred) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= Vc) return;
     const float* mu = norm + 2 * kFeat;
     const float* sd = norm + 2 * kFeat + 3;
     const float m = vcl[j].z;
-    epred[j] = make_float4(m * (pred[j * 3 + 0] * sd[0] + mu[0]), m * (pred[j * 3 + 1] * sd[1] + mu[1]),
-                           m * (pred[j * 3 + 2] * sd[2] + mu[2]), 0.f);
+    const float4 e1 = eprev[j];
+    epred[j] = make_float4(m * (e1.x + (pred[j * 3 + 0] * sd[0] + mu[0])), m * (e1.y + (pred[j * 3 + 1] * sd[1] + mu[1])),
+                           m * (e1.z + (pred[j * 3 + 2] * sd[2] + mu[2])), 0.f);
 }
 
 // Coarse delta achieved this step: (u - u_t) injected at each level-1 vertex
 // through its coarse-in-fine binding, u = x - s the step's correction (x
 // holds u) and u_t its temporal extrapolation (U1 == nullptr: u_t = 0).  Kept
-// for the next step's features and optionally recorded as training target.
+// with the previous one for the next step's features (eprev, eprev2) and
+// optionally recorded as training target.
 __global__ void __launch_bounds__(256) k_target(int Vc, const float4* __restrict__ x, const float4* __restrict__ U1,
                                                 const float4* __restrict__ U2, const float4* __restrict__ vc,
                                                 const int4* __restrict__ aids, const float4* __restrict__ aw,
-                                                float4* __restrict__ eprev, float* __restrict__ rec,
-                                                const int* __restrict__ skip) {
+                                                float4* __restrict__ eprev, float4* __restrict__ eprev2,
+                                                float* __restrict__ rec, const int* __restrict__ skip) {
     if (skip && *skip) return;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= Vc) return;
@@ -1300,11 +1332,13 @@ __global__ void __launch_bounds__(256) k_target(int Vc, const float4* __restrict
         dy = dy + ws[q] * a.y;
         dz = dz + ws[q] * a.z;
     }
+    const float4 e1 = eprev[j];
+    eprev2[j] = e1;
     eprev[j] = make_float4(dx, dy, dz, 0.f);
-    if (rec) {
-        rec[(long)j * 3 + 0] = dx;
-        rec[(long)j * 3 + 1] = dy;
-        rec[(long)j * 3 + 2] = dz;
+    if (rec) {  // the network's target: the change from the last step's delta
+        rec[(long)j * 3 + 0] = dx - e1.x;
+        rec[(long)j * 3 + 1] = dy - e1.y;
+        rec[(long)j * 3 + 2] = dz - e1.z;
     }
 }
 
@@ -2003,18 +2037,20 @@ struct mgsb_solver {
         unsigned checks = 0;  // check kernels in the sequence
     };
     std::map<std::string, PieceGraph> pseq;
+    bool precaptured = false;  // the finite set of first rounds and follow-ups is in pseq
     int halves_guess = 4;
-    // learned schedule: a step after one that needed at most one half-cycle
-    // starts without the MLP stage; after kGateAfter consecutive steps whose
-    // learned correction the judge rejected, or kept while the step still
-    // needed as many half-cycles as the last step run without it, steps start
-    // without it for gate_skip steps (doubling up to kGateMax after each
-    // further such run); every kGateProbe steps with the stage, one step runs
-    // without it to renew that baseline
+    // learned schedule: after kGateAfter consecutive steps whose learned
+    // warm start the judge undid, steps start without the MLP stage for
+    // gate_skip steps (kGateFirst, doubling up to kGateMax after each further
+    // such run; a kept stage resets both)
     int gate_rejects = 0, gate_skip = 0, gate_left = 0;
-    // halves of the last step run without the stage (-1: none yet), and the
-    // steps with the stage since
-    int base_halves = -1, learned_run = 0;
+    // the stage runs while its steps end after the first half-cycle or need
+    // fewer half-cycles than the last step without it (off_halves, -1: none
+    // yet); otherwise, and every kGateProbe steps with it, the next step runs
+    // without it, and the stage resumes after a step without it that needed
+    // two or more
+    bool stage_on = true;
+    int off_halves = -1, since_probe = 0;
     // bodies: connected components of level 0 (body_of[l][v] on every level,
     // through the coarse-in-fine bindings); when they are contiguous vertex
     // ranges the V-cycle checks each body's own relative residual
@@ -2030,7 +2066,7 @@ struct mgsb_solver {
     // learned coarse correction (level 1)
     Mlp* mlp = nullptr;
     float *feat = nullptr, *pred = nullptr, *norm = nullptr;
-    float4 *epred = nullptr, *eprev = nullptr, *X1 = nullptr;
+    float4 *epred = nullptr, *eprev = nullptr, *eprev2 = nullptr, *X1 = nullptr;
     float *recX = nullptr, *recT = nullptr;  // recorded samples (raw)
     int rec_cap = 0, rec_steps = 0;           // in steps of V1 samples
     bool recording = false;
@@ -2156,6 +2192,7 @@ int push_vstate(mgsb_solver* s) {
     v.check = s->p.check_every;
     v.tol = s->p.tol;
     v.maxh = 2 * s->p.max_cycles;
+    v.slack = s->p.learn_slack > 0.f ? (double)s->p.learn_slack : 1.0;
     HIPCHK(hipMemcpyAsync(s->vstate, &v, sizeof(VState), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     return MGSB_OK;
@@ -2276,12 +2313,15 @@ int launch_vert(mgsb_solver* s, DevLevel& d, float omega, int flags, const int* 
     return MGSB_OK;
 }
 
-// warm: split positions start from the temporal extrapolation (U1, U2)
-int launch_st(mgsb_solver* s, DevLevel& d, int advance, int split = 0, bool warm = false) {
+// warm: split positions start from the temporal extrapolation (U1, U2);
+// epred: plus the prolongated learned coarse correction (level 0 only)
+int launch_st(mgsb_solver* s, DevLevel& d, int advance, int split = 0, bool warm = false,
+              const float4* epred = nullptr) {
     const int bs = block_vert(s);
     k_st<<<nblk(d.V, bs), bs, 0, s->stream>>>(d.V, d.x, d.v, d.xlast, d.s, d.xprev, d.vc, s->p.dt, s->p.damping,
                                                s->p.gravity[0], s->p.gravity[1], s->p.gravity[2], s->dcol, advance,
-                                               split, warm ? d.U1 : nullptr, warm ? d.U2 : nullptr);
+                                               split, warm ? d.U1 : nullptr, warm ? d.U2 : nullptr, epred,
+                                               epred ? s->M[0].iids : nullptr, epred ? s->M[0].iw : nullptr);
     HIPCHK(hipGetLastError());
     return MGSB_OK;
 }
@@ -2460,68 +2500,66 @@ bool temporal_warm(const mgsb_solver* s) {
 
 // Inertial prediction, g(s) and res0, then the warm start: the temporal
 // extrapolation of the last corrections (warm_start = 1 or learned), and with
-// learned = true the MLP's coarse delta on top.  Each stage is judged by the
-// residual it leaves (vstate_apply) and undone by k_revert when it is worse.
+// learned = true the MLP's coarse correction on top.  The network reads only
+// what is known before the step (velocity, rest position, the last two
+// steps' coarse deltas), so it runs before the inertial prediction and its
+// prolongated correction enters the warm start with the extrapolation:
+// the stage costs the feature and inference launches, and the one judge of
+// the warm start (vstate_apply: kept if its residual is at most learn_slack
+// times g(s)'s, else undone by k_revert) covers both.
 int enqueue_vc_start(mgsb_solver* s, bool learned) {
     DevLevel& f = s->L[0];
     DevLevel& c1 = s->L[1];
     const int* skip = &s->vstate->converged;
     const int nbv = nblk(f.V, 256);
     const int bs = block_vert(s);
-    const bool features = learned || s->recording;
+    const bool features = s->recording;  // (the learned stage computes its own)
     const bool temporal = temporal_warm(s);
-    // temporal start: g(s) and g(s + u_t) from one local-step launch, one
-    // residual launch and one check (per-body checks keep the sequential
-    // form, which reduces each residual per body)
-    const bool dual = temporal && !s->body_checks;
+    const bool warm = temporal || learned;
+    // g(s) and g(warm start) from one local-step launch, one residual launch
+    // and one check (per-body checks keep the sequential form, which reduces
+    // each residual per body)
+    const bool dual = warm && !s->body_checks;
+    const int judge = learned ? kChkWarm : kChkTemporal;
+    const float4* ep = learned ? s->epred : nullptr;
     float* recX = s->recording ? s->recX + (size_t)s->rec_steps * c1.V * kFeat : nullptr;
     int rc;
-    if ((rc = launch_st(s, f, 1, 1, dual))) return rc;
+    if (features) {  // features at the step's start (recorded even if the step converges at once)
+        k_features<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, f.v, s->M[0].aids, s->M[0].aw, s->X1, s->eprev,
+                                                         s->eprev2, s->norm, s->feat, recX);
+        HIPCHK(hipGetLastError());
+    }
+    if (learned) {  // features, network, prior and mask in one launch
+        const CoarseIO io{f.v, s->M[0].aids, s->M[0].aw, s->X1, s->eprev, s->eprev2, s->norm, c1.vcl, s->epred};
+        if ((rc = mlp_infer_coarse(s->mlp, c1.V, io, s->stream))) return rc;
+    }
     if (dual) {
+        if ((rc = launch_st(s, f, 1, 1, temporal, ep))) return rc;
         if ((rc = launch_tet_dual(s, f))) return rc;
         const int lsh = vert_lsh(s), nbr = nblk(f.V << lsh, 256);
-        k_resid_dual<<<nbr, 256, 0, s->stream>>>(f.V, f.x, f.s, f.vc, f.rp, f.cf, f.cf2, s->dcol, kVertPoint,
-                                                 features ? f.r : nullptr, s->vpart, f.cmask, lsh);
+        k_resid_dual<<<nbr, 256, 0, s->stream>>>(f.V, f.x, f.s, f.vc, f.rp, f.cf, f.cf2, s->dcol, kVertPoint, nullptr,
+                                                 s->vpart, f.cmask, lsh);
         HIPCHK(hipGetLastError());
-        k_check2<<<1, 256, 0, s->stream>>>(nbr, s->vpart, s->vstate, s->hmirror, s->seqc);
+        k_check2<<<1, 256, 0, s->stream>>>(nbr, s->vpart, s->vstate, s->hmirror, s->seqc, judge);
         s->chk_launched++;
         HIPCHK(hipGetLastError());
         k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, f.rp, f.cf2, f.cf);
         HIPCHK(hipGetLastError());
-    } else {
-        if ((rc = fine_residual(s, true, features && !temporal, kChkStart, true, nullptr, true))) return rc;
-        if (temporal) {
-            k_warm_temporal<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, skip);
-            HIPCHK(hipGetLastError());
-            if ((rc = fine_residual(s, true, features, kChkTemporal, true, skip))) return rc;
-            k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, f.rp, nullptr,
-                                                 f.cf);
-            HIPCHK(hipGetLastError());
-        }
+        return MGSB_OK;
     }
-    if (features) {  // restricted residual at the warm start, features (recorded even if converged)
-        k_restrict<<<nblk(c1.V * 64, 256), 256, 0, s->stream>>>(c1.V, c1.rtp, c1.rti, c1.rtw, f.r, c1.vcl, c1.r,
-                                                                nullptr);
-        k_features<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, c1.r, f.v, s->M[0].aids, s->M[0].aw, s->X1, s->eprev,
-                                                         s->norm, s->feat, recX);
+    if ((rc = launch_st(s, f, 1, 1, false))) return rc;
+    if ((rc = fine_residual(s, true, false, kChkStart, true, nullptr, true))) return rc;
+    if (warm) {
+        k_warm_temporal<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, temporal ? f.U1 : nullptr,
+                                                    temporal ? f.U2 : nullptr, f.vc, skip, ep,
+                                                    ep ? s->M[0].iids : nullptr, ep ? s->M[0].iw : nullptr);
         HIPCHK(hipGetLastError());
+        if ((rc = fine_residual(s, true, false, judge, true, skip))) return rc;
+        k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, f.rp, nullptr, f.cf);
+        HIPCHK(hipGetLastError());
+        // after a revert the corner forces belong to the rejected x: recompute
+        if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, &s->vstate->cf_ok, true))) return rc;
     }
-    if (learned) {
-        if ((rc = mlp_infer(s->mlp, s->feat, c1.V, s->pred, 3, s->stream, skip))) return rc;
-        k_apply_pred<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, s->pred, s->norm, c1.vcl, s->epred);
-        k_prolong_add<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, s->epred, s->M[0].iids, s->M[0].iw,
-                                                           f.vc, skip, f.cmask);
-        HIPCHK(hipGetLastError());
-        if ((rc = fine_residual(s, true, false, kChkWarm, true, skip))) return rc;
-        k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, f.rp,
-                                             dual ? f.cf2 : nullptr, f.cf);
-        HIPCHK(hipGetLastError());
-    }
-    // after a revert the corner forces belong to the rejected x: recompute
-    // (the dual start's k_revert restores the forces at s by itself)
-    if ((learned || (temporal && !dual)) &&
-        (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, &s->vstate->cf_ok, true)))
-        return rc;
     return MGSB_OK;
 }
 
@@ -2581,7 +2619,7 @@ int enqueue_vc_finish(mgsb_solver* s, bool learned, bool lite) {
     if (!lite && (features || temporal_warm(s))) {  // coarse delta: next step's feature / training target
         const bool t = temporal_warm(s);
         k_target<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, f.x, t ? f.U1 : nullptr, t ? f.U2 : nullptr, f.vc,
-                                                       s->M[0].aids, s->M[0].aw, s->eprev, recT,
+                                                       s->M[0].aids, s->M[0].aw, s->eprev, s->eprev2, recT,
                                                        &s->vstate->notdone);
         HIPCHK(hipGetLastError());
     }
@@ -2603,15 +2641,10 @@ int enqueue_piece(mgsb_solver* s, int piece) {
     }
 }
 
-// Enqueues the pieces seq (kPiece* codes, one char each) in order: eagerly,
-// or as one graph captured at the sequence's first use.
-int run_pieces(mgsb_solver* s, const std::string& seq, bool graphable) {
-    if (!graphable) {
-        for (const char q : seq)
-            if (int rc = enqueue_piece(s, q)) return rc;
-        return MGSB_OK;
-    }
-    auto it = s->pseq.find(seq);
+// Captures the pieces seq (kPiece* codes, one char each) as one graph and
+// keeps it (no launch).
+int capture_pieces(mgsb_solver* s, const std::string& seq, std::map<std::string, mgsb_solver::PieceGraph>::iterator& it) {
+    it = s->pseq.find(seq);
     if (it == s->pseq.end()) {
         HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         const unsigned before = s->chk_launched;
@@ -2634,8 +2667,73 @@ int run_pieces(mgsb_solver* s, const std::string& seq, bool graphable) {
         }
         it = s->pseq.emplace(seq, pg).first;
     }
+    return MGSB_OK;
+}
+
+// Enqueues the pieces seq in order: eagerly, or as one graph (captured at
+// the sequence's first use unless pre-captured).
+int run_pieces(mgsb_solver* s, const std::string& seq, bool graphable) {
+    if (!graphable) {
+        for (const char q : seq)
+            if (int rc = enqueue_piece(s, q)) return rc;
+        return MGSB_OK;
+    }
+    std::map<std::string, mgsb_solver::PieceGraph>::iterator it;
+    if (int rc = capture_pieces(s, seq, it)) return rc;
     HIPCHK(hipGraphLaunch(it->second.exec, s->stream));
     s->chk_launched += it->second.checks;
+    return MGSB_OK;
+}
+
+// The piece sequences step_vcycle can launch form a finite set: the first
+// round of a step (start or gated start, the halves guessed from the last
+// step, finish or lite finish) and the rounds after a verdict (one half, or
+// a cycle, plus a finish).  They are captured before the first step runs, so
+// that no step pays for a capture and instantiation inside a timed loop
+// (one capture costs far more than a step).  First rounds are captured up to
+// kPrecaptureHalves halves; longer guesses are captured at first use.
+constexpr int kPrecaptureHalves = 8;
+
+int precapture_pieces(mgsb_solver* s) {
+    const bool learned = s->p.schedule == MGSB_SCHED_LEARNED;
+    const int check = s->p.check_every;
+    const int maxh = 2 * s->p.max_cycles;
+    const std::string starts = learned ? std::string{(char)kPieceStart, (char)kPieceStartGated}
+                                       : std::string{(char)kPieceStart};
+    // the learned schedule (or a recording, which runs eagerly) takes the
+    // full finish, the others the lite one (fin_for)
+    const std::string fins(1, (char)(learned ? kPieceFinish : kPieceFinishLite));
+    auto halves = [](char st, int h) {
+        std::string q(1, st);
+        for (int k = 0; k < h; k++) q.push_back((char)((k & 1) ? kPiecePost : kPiecePre));
+        return q;
+    };
+    std::vector<std::string> seqs;
+    for (const char st : starts) {
+        if (!check) {  // no checks: start, max_cycles cycles, finish
+            for (const char f : fins) seqs.push_back(halves(st, maxh) + f);
+            continue;
+        }
+        for (int h = 0; h <= std::min(maxh, kPrecaptureHalves); h++) {
+            if (check == 1 && (h & 1)) continue;
+            for (const char f : fins) seqs.push_back(halves(st, h) + f);
+        }
+    }
+    if (check)
+        for (const char f : fins) {
+            if (check == 2) {
+                seqs.push_back(std::string{(char)kPiecePre, f});
+                seqs.push_back(std::string{(char)kPiecePost, f});
+            } else {
+                seqs.push_back(std::string{(char)kPiecePre, (char)kPiecePost, f});
+            }
+        }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (const std::string& q : seqs) {
+        std::map<std::string, mgsb_solver::PieceGraph>::iterator it;
+        if (int rc = capture_pieces(s, q, it)) return rc;
+    }
+    s->precaptured = true;
     return MGSB_OK;
 }
 
@@ -2682,24 +2780,20 @@ int wait_checks(mgsb_solver* s) {
 // step beyond a wrong guess.
 int step_vcycle(mgsb_solver* s, bool graphable) {
     int rc;
+    if (graphable && !s->precaptured && (rc = precapture_pieces(s))) return rc;
     const bool learned = s->p.schedule == MGSB_SCHED_LEARNED;
     const bool gate_on = learned && s->p.learn_gate && !s->recording;
-    const bool shut = gate_on && s->gate_left > 0;
-    if (shut) s->gate_left--;
-    // the MLP stage can only save half-cycles beyond the first: after a step
-    // that needed at most one, this step starts without it
-    const bool gated = shut || (gate_on && s->halves_guess <= 1);
+    // no MLP stage this step: the gate is shut after rejections, or the
+    // stage is idle (below)
+    const bool gated = gate_on && (s->gate_left > 0 || !s->stage_on);
+    if (gate_on && s->gate_left > 0) s->gate_left--;
     std::string seq(1, (char)(gated ? kPieceStartGated : kPieceStart));
     const int check = s->p.check_every;
     const int maxh = 2 * s->p.max_cycles;
-    // the next step reads this step's coarse delta only if it runs the learned
-    // stage: not under the plain schedules, nor while the gate stays shut, nor
-    // after a step of at most one half-cycle (hend: halves by the end of the
-    // round whose finish this is)
-    auto fin_for = [&](int hend) {
-        const bool lite = !s->recording && (!learned || (gate_on && (s->gate_left > 0 || hend <= 1)));
-        return lite ? kPieceFinishLite : kPieceFinish;
-    };
+    // the learned stage reads the coarse deltas of the last two steps: the
+    // learned schedule (and a recording) keeps them every step, the others
+    // take the lite finish
+    auto fin_for = [&](int) { return (s->recording || learned) ? kPieceFinish : kPieceFinishLite; };
     int h = 0;
     int target = check ? std::min(maxh, std::max(0, s->halves_guess)) : maxh;
     bool finished = false;
@@ -2722,29 +2816,29 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
         }
         target = h + (check == 2 ? 1 : 2);
     }
-    if (learned && check && finished && !s->recording) {  // the mirror holds this step's verdicts
+    if (gate_on && check && finished && gated && s->gate_left == 0) {  // a step without the stage
+        // the warm start without it needed more than the first half-cycle:
+        // what is left may be the smooth error a coarse correction removes
+        s->off_halves = mirror_verdict(s).halves;
+        s->stage_on = s->off_halves >= 2;
+        s->since_probe = 0;
+    }
+    if (gate_on && check && finished && !gated) {  // the mirror holds this step's verdict
         const Verdict vd = mirror_verdict(s);
-        const int halves = vd.halves;
-        if (gated) {  // a step without the stage: the baseline it has to beat
-            s->base_halves = halves;
-            s->learned_run = 0;
-        } else {
-            const bool rejected = (vd.fallback & 2) != 0, accepted = vd.accepted == 2;
-            // kept, but the step needed as many halves as without the stage
-            const bool idle = accepted && s->base_halves >= 0 && halves >= s->base_halves;
-            if (rejected || idle) {
-                if (++s->gate_rejects >= kGateAfter) {
-                    s->gate_skip = s->gate_skip ? std::min(2 * s->gate_skip, kGateMax) : kGateFirst;
-                    s->gate_left = s->gate_skip;
-                    s->gate_rejects = 0;
-                }
-            } else if (accepted) {  // (else the stage was skipped: no verdict)
+        // the stage keeps running while its steps end after the first
+        // half-cycle or need fewer than the last step without it; otherwise
+        // (and every kGateProbe steps) the next step runs without it
+        s->stage_on = (vd.halves <= 1 || (s->off_halves >= 0 && vd.halves < s->off_halves)) &&
+                      ++s->since_probe < kGateProbe;
+        if (vd.fallback & 2) {  // the judge undid the stage's correction
+            if (++s->gate_rejects >= kGateAfter) {
+                s->gate_skip = s->gate_skip ? std::min(2 * s->gate_skip, kGateMax) : kGateFirst;
+                s->gate_left = s->gate_skip;
                 s->gate_rejects = 0;
-                s->gate_skip = 0;
             }
-            // no baseline yet, or an old one: the next step runs without the stage
-            if (s->p.learn_gate && s->gate_left == 0 && (s->base_halves < 0 || ++s->learned_run >= kGateProbe))
-                s->gate_left = 1;
+        } else if (vd.accepted == 2) {
+            s->gate_rejects = 0;
+            s->gate_skip = 0;
         }
     }
     if (!finished) seq.push_back((char)fin_for(h));
@@ -2955,6 +3049,7 @@ int alloc_vcycle(mgsb_solver* s) {
         for (int i = 0; i < c1.V; i++) x1[i] = make_float4(c1.h.X[i * 3], c1.h.X[i * 3 + 1], c1.h.X[i * 3 + 2], 0.f);
         if ((rc = up(s, s->owned, &s->epred, z.data(), c1.V))) return rc;
         if ((rc = up(s, s->owned, &s->eprev, z.data(), c1.V))) return rc;
+        if ((rc = up(s, s->owned, &s->eprev2, z.data(), c1.V))) return rc;
         if ((rc = up(s, s->owned, &s->X1, x1.data(), c1.V))) return rc;
         if ((rc = mlp_create(&s->mlp, kFeat, 3, 1234u, s->stream))) return rc;
     }
@@ -3084,6 +3179,10 @@ int validate_params(const mgsb_params* p, int nlevels) {
         set_error("block sizes must be 0 or a multiple of 64 in [64, 256]");
         return MGSB_ERR_ARG;
     }
+    if (!(p->learn_slack >= 0.f) || (p->learn_slack > 0.f && p->learn_slack < 1.f)) {
+        set_error("learn_slack must be 0 (reads as 1) or >= 1");
+        return MGSB_ERR_ARG;
+    }
     const int vl = p->vert_lanes;
     if (vl != 0 && vl != 1 && vl != 2 && vl != 4 && vl != 8) {
         set_error("vert_lanes must be 0, 1, 2, 4 or 8");
@@ -3099,13 +3198,15 @@ int invalidate_graph(mgsb_solver* s) {
     s->graph = nullptr;
     s->graph_valid = false;
     s->gate_rejects = s->gate_skip = s->gate_left = 0;
-    s->base_halves = -1;
-    s->learned_run = 0;
+    s->stage_on = true;
+    s->off_halves = -1;
+    s->since_probe = 0;
     for (auto& kv : s->pseq) {
         if (kv.second.exec) HIPCHK(hipGraphExecDestroy(kv.second.exec));
         if (kv.second.graph) HIPCHK(hipGraphDestroy(kv.second.graph));
     }
     s->pseq.clear();
+    s->precaptured = false;
     return MGSB_OK;
 }
 
@@ -3374,9 +3475,13 @@ int mgsb_set_positions(mgsb_solver* s, int32_t level, const float* x) {
         return MGSB_ERR_ARG;
     }
     HIPCHK(hipSetDevice(s->device));
-    if (level == 0 && s->L[0].U1) {  // the temporal warm start's history no longer applies
+    if (level == 0 && s->L[0].U1) {  // the warm starts' history no longer applies
         HIPCHK(hipMemsetAsync(s->L[0].U1, 0, sizeof(float4) * s->L[0].V, s->stream));
         HIPCHK(hipMemsetAsync(s->L[0].U2, 0, sizeof(float4) * s->L[0].V, s->stream));
+    }
+    if (level == 0 && s->eprev) {
+        HIPCHK(hipMemsetAsync(s->eprev, 0, sizeof(float4) * s->L[1].V, s->stream));
+        HIPCHK(hipMemsetAsync(s->eprev2, 0, sizeof(float4) * s->L[1].V, s->stream));
     }
     return write_f4(s, s->L[level].x, s->L[level].V, x);
 }
@@ -4087,7 +4192,8 @@ struct StateSnap {
     std::vector<std::vector<float4>> bufs;
     VState v{};
     DevColliders col{};
-    int halves_guess = 0, gate_rejects = 0, gate_skip = 0, gate_left = 0, base_halves = -1, learned_run = 0;
+    int halves_guess = 0, gate_rejects = 0, gate_skip = 0, gate_left = 0, off_halves = -1, since_probe = 0;
+    bool stage_on = true;
 };
 
 int snap_take(mgsb_solver* s, StateSnap& S) {
@@ -4101,6 +4207,7 @@ int snap_take(mgsb_solver* s, StateSnap& S) {
         S.where.push_back({f.U2, (size_t)f.V});
     }
     if (s->eprev) S.where.push_back({s->eprev, (size_t)s->L[1].V});
+    if (s->eprev2) S.where.push_back({s->eprev2, (size_t)s->L[1].V});
     S.bufs.resize(S.where.size());
     for (size_t i = 0; i < S.where.size(); i++) {
         S.bufs[i].resize(S.where[i].second);
@@ -4113,8 +4220,9 @@ int snap_take(mgsb_solver* s, StateSnap& S) {
     S.gate_rejects = s->gate_rejects;
     S.gate_skip = s->gate_skip;
     S.gate_left = s->gate_left;
-    S.base_halves = s->base_halves;
-    S.learned_run = s->learned_run;
+    S.stage_on = s->stage_on;
+    S.off_halves = s->off_halves;
+    S.since_probe = s->since_probe;
     return MGSB_OK;
 }
 
@@ -4130,8 +4238,9 @@ int snap_restore(mgsb_solver* s, const StateSnap& S) {
     s->gate_rejects = S.gate_rejects;
     s->gate_skip = S.gate_skip;
     s->gate_left = S.gate_left;
-    s->base_halves = S.base_halves;
-    s->learned_run = S.learned_run;
+    s->stage_on = S.stage_on;
+    s->off_halves = S.off_halves;
+    s->since_probe = S.since_probe;
     return MGSB_OK;
 }
 
