@@ -1892,6 +1892,59 @@ __global__ void __launch_bounds__(1024) k_coarse_tail(const TailLevel* __restric
     }
 }
 
+// The coarse tail is linear in the residual of its first level (fixed
+// Chebyshev weights, fixed transfers, the direct coarsest solve), and the
+// three components go through the same scalar operator: e = Bt r with one
+// V x V matrix Bt.  For a first tail level of at most kTailDenseMaxV
+// vertices Bt is formed at set-up, column by column, by running the tail on
+// unit right-hand sides (k_unit_rhs, k_store_cols), and each coarse
+// correction is then one matrix-vector product: one wave per row, the row
+// read coalesced, a fixed shuffle tree combining the lanes.  The tail's
+// dozen dependent phases in one workgroup become one launch over the whole
+// GPU; the product rounds differently from the sequence of sparse steps.
+constexpr int kTailDenseMaxV = 2048;
+
+__global__ void __launch_bounds__(256) k_unit_rhs(int V, float4* __restrict__ r, int c0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    r[i] = make_float4(i == c0 ? 1.f : 0.f, i == c0 + 1 ? 1.f : 0.f, i == c0 + 2 ? 1.f : 0.f, 0.f);
+}
+
+__global__ void __launch_bounds__(256) k_store_cols(int V, const float4* __restrict__ e, float* __restrict__ Bt,
+                                                    int c0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const float4 v = e[i];
+    float* row = Bt + (size_t)i * V;
+    row[c0] = v.x;
+    if (c0 + 1 < V) row[c0 + 1] = v.y;
+    if (c0 + 2 < V) row[c0 + 2] = v.z;
+}
+
+__global__ void __launch_bounds__(256) k_tail_gemv(int V, const float* __restrict__ Bt, const float4* __restrict__ r,
+                                                   float4* __restrict__ e, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    const int row = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (row >= V) return;
+    const float* b = Bt + (size_t)row * V;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    for (int j = lane; j < V; j += 64) {
+        const float w = b[j];
+        const float4 q = r[j];
+        ax = ax + w * q.x;
+        ay = ay + w * q.y;
+        az = az + w * q.z;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        ax += __shfl_down(ax, off, 64);
+        ay += __shfl_down(ay, off, 64);
+        az += __shfl_down(az, off, 64);
+    }
+    if (lane == 0) e[row] = make_float4(ax, ay, az, 0.f);
+}
+
 inline int nblk(int n, int bs) { return (n + bs - 1) / bs; }
 
 // ---------------------------------------------------------------------------
@@ -1959,6 +2012,7 @@ struct DevLevel {
     float* rtw = nullptr;
     int *ap = nullptr, *aj = nullptr;  // scalar level matrix A = m/h^2 + L (coarse levels)
     float* av = nullptr;
+    float* tailB = nullptr;            // first tail level: the tail as a dense V x V matrix (or nullptr)
     int* ajE = nullptr;                // ... as column-major ELL (first level of the coarse tail)
     float* avE = nullptr;
     int ellw = 0;
@@ -2426,8 +2480,11 @@ int vcycle_lin(mgsb_solver* s, int l, const int* skip) {
         return MGSB_OK;
     }
     if (l == s->tail_from) {
-        k_coarse_tail<<<1, 1024, 0, s->stream>>>(s->tail, nl - l, s->p.coarse_sweeps, s->p.relax_coarse,
-                                                 s->p.coarse_direct, skip);
+        if (d.tailB)
+            k_tail_gemv<<<nblk(d.V * 64, 256), 256, 0, s->stream>>>(d.V, d.tailB, d.r, d.e, skip);
+        else
+            k_coarse_tail<<<1, 1024, 0, s->stream>>>(s->tail, nl - l, s->p.coarse_sweeps, s->p.relax_coarse,
+                                                     s->p.coarse_direct, skip);
         HIPCHK(hipGetLastError());
         d.cur = 0;
         return MGSB_OK;
@@ -3006,6 +3063,24 @@ int setup_vcycle(mgsb_solver* s) {
         }
         HIPCHK(hipMemcpyAsync(s->tail, tl.data(), sizeof(TailLevel) * tl.size(), hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
+        // the tail as a dense matrix, three columns per run on unit vectors
+        DevLevel& t0 = s->L[s->tail_from];
+        if (s->tail_from < nl - 1 && t0.V <= kTailDenseMaxV) {
+            if (!t0.tailB) {
+                int rc = dalloc(&t0.tailB, (size_t)t0.V * t0.V);
+                if (rc) return rc;
+                t0.owned.push_back(t0.tailB);
+            }
+            const int nb = nblk(t0.V, 256);
+            for (int c0 = 0; c0 < t0.V; c0 += 3) {
+                k_unit_rhs<<<nb, 256, 0, s->stream>>>(t0.V, t0.r, c0);
+                k_coarse_tail<<<1, 1024, 0, s->stream>>>(s->tail, nl - s->tail_from, s->p.coarse_sweeps,
+                                                         s->p.relax_coarse, s->p.coarse_direct, nullptr);
+                k_store_cols<<<nb, 256, 0, s->stream>>>(t0.V, t0.e, t0.tailB, c0);
+            }
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(s->stream));
+        }
     }
     return MGSB_OK;
 }
