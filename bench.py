@@ -145,13 +145,42 @@ def timed_run(s, steps, warmup, vcycle):
     return elapsed, run
 
 
+def host_cpu():
+    """nproc, the cores this process may run on, and the CPU model."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return os.cpu_count() or 1, allowed, model
+
+
 def cpu_baseline(levels, cfg, schedule, budget_s=12.0):
-    """The oracle on the same workload and schedule, FP32: budget_s seconds
-    on up to 16 OpenMP threads, then half that on one thread."""
+    """The oracle (OpenMP over tets and vertices, a fixed-order CSR gather)
+    on the same workload: the same schedule as the GPU line on every
+    core this process may use (capped by OMP_NUM_THREADS where the host sets
+    a CPU share), the median of 5 blocks after a warm-up step, then the same
+    trajectory on one thread, then the reference's own fixed-iteration
+    schedule (PDSolver_MG::Step, PDSolver_MG.cpp:183-253: 20 coarse + 20
+    fine PD iterations per step on the first two levels) for comparison."""
+    import statistics
+
     import oracle as O
     from multigridsoftbody_amd import workloads as W
 
-    threads = min(16, os.cpu_count() or 1)
+    nproc, allowed, model = host_cpu()
+    threads = allowed
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        threads = min(threads, int(share))
     O.set_threads(threads)
     if schedule == "vcycle" and levels[0].ntets > 200_000:
         # the oracle's level bindings follow the reference's brute-force point
@@ -161,16 +190,28 @@ def cpu_baseline(levels, cfg, schedule, budget_s=12.0):
     phys = W.physics(gravity=(0.0, -9.8, 0.0) if cfg["gravity"] else (0.0, 0.0, 0.0))
     lv = levels if schedule == "vcycle" else levels[:1]
     cfg = dict(cfg, iters=cfg.get("iters", 10))
-    Ls = [O.Level(m.pos, m.tets, m.tris, stiffness=phys["stiffness"], density=phys["density"]) for m in lv]
-    if "E" in cfg:  # per-body stiffness on every level (60 n^3 tets per body)
-        import numpy as np
 
-        for l, L in enumerate(Ls[:len(lv)]):
-            L.set_tet_stiffness(np.concatenate([np.full(60 * W.BATCH_LEVELS[int(n)][l] ** 3, e, np.float32)
-                                                for n, e in zip(cfg["ns"], cfg["E"])]))
-    fix = cfg["fix"](levels[0]) if "fix" in cfg else W.top_fixflag(levels[0])
-    Ls[0].set_fix(fix)
-    Ls[0].set_velocity(W.seeded_velocity(levels[0], fix))
+    def make_levels(meshes, fp64=False):
+        Ls = [O.Level(m.pos, m.tets, m.tris, stiffness=phys["stiffness"], density=phys["density"], fp64=fp64)
+              for m in meshes]
+        if "E" in cfg:  # per-body stiffness on every level (60 n^3 tets per body)
+            import numpy as np
+
+            for l, L in enumerate(Ls):
+                L.set_tet_stiffness(np.concatenate([np.full(60 * W.BATCH_LEVELS[int(n)][l] ** 3, e, np.float32)
+                                                    for n, e in zip(cfg["ns"], cfg["E"])]))
+        fix = cfg["fix"](levels[0]) if "fix" in cfg else W.top_fixflag(levels[0])
+        Ls[0].set_fix(fix)
+        Ls[0].set_velocity(W.seeded_velocity(levels[0], fix))
+        return Ls, fix
+
+    # The V-cycle runs in FP64: the oracle iterates on absolute positions,
+    # and in FP32 its residual stalls at that format's floor above the 1e-3
+    # target within a few steps of the medium box (every later step then
+    # runs all 12 cycles); the device iterates on x = s + u and converges in
+    # the same cycle counts as the FP64 schedule.
+    prec = "FP64" if schedule == "vcycle" else "FP32"
+    Ls, fix = make_levels(lv, fp64=schedule == "vcycle")
     col = {}
     if cfg["name"].startswith("large"):
         c = W.large_colliders()
@@ -185,29 +226,63 @@ def cpu_baseline(levels, cfg, schedule, budget_s=12.0):
                      if_mode=3, **col)
         vp = O.vparams(pre=VCYCLE["pre_sweeps"], post=VCYCLE["post_sweeps"], coarse_sweeps=VCYCLE["coarse_sweeps"],
                        max_cycles=VCYCLE["max_cycles"], tol=VCYCLE["tol"], check=1, rho_smooth=VCYCLE["rho_smooth"])
-        step = lambda: ML.step(p, vp, 1)  # noqa: E731
+        cyc = []
+
+        def step():
+            ML.step(p, vp, 1)
+            st = ML.stats()
+            cyc.append((int(st["cycles"]), bool(st["rel"] > VCYCLE["tol"])))
+
         what = "V-cycle schedule (same parameters, adaptive cycles to rel. residual 1e-3)"
     else:
         p = O.params(dt=phys["dt"], stiffness=phys["stiffness"], density=phys["density"], gravity=phys["gravity"],
                      iters=cfg["iters"], if_mode=1, **col)
+        cyc = None
         step = lambda: Ls[0].step(p, 1)  # noqa: E731
         what = f"single-level reference schedule ({cfg['iters']} PD iterations/step)"
-    def rate(budget):
-        n, t0 = 0, time.perf_counter()
-        while time.perf_counter() - t0 < budget:
-            step()
-            n += 1
-        return n, n / (time.perf_counter() - t0)
+
+    def rate(budget, fn, blocks=5):
+        """steps/s: the median over `blocks` blocks of about budget/blocks s"""
+        rates, n = [], 0
+        for _ in range(blocks):
+            k, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < budget / blocks:
+                fn()
+                k += 1
+            rates.append(k / (time.perf_counter() - t0))
+            n += k
+        return n, statistics.median(rates)
 
     step()
-    n, sps = rate(budget_s)
+    n, sps = rate(budget_s, step)
     O.set_threads(1)  # the same trajectory continued on one thread
-    n1, sps1 = rate(budget_s / 2)
+    n1, sps1 = rate(budget_s / 2, step)
     O.set_threads(threads)
-    return {"value": round(sps, 4), "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} steps (after 1 warm-up) of the {what} on the {levels[0].ntets}-tet workload, "
-                      f"FP32 oracle, {threads} OpenMP threads",
-            "value_1_thread": round(sps1, 4), "sample_1_thread": f"the next {n1} steps on one thread"}
+    out = {"value": round(sps, 4), "unit": "steps/s", "cores": threads, "kind": "port",
+           "sample": f"{n} steps (after 1 warm-up; median of 5 blocks) of the {what} on the {levels[0].ntets}-tet "
+                     f"workload, {prec} oracle, {threads} OpenMP threads",
+           "nproc": nproc, "cores_allowed": allowed, "cpu_model": model,
+           "value_1_thread": round(sps1, 4), "sample_1_thread": f"the next {n1} steps on one thread"}
+    if cyc:
+        out["cycles_per_step"] = round(sum(c for c, _ in cyc) / len(cyc), 3)
+        out["steps_over_tol"] = int(sum(f for _, f in cyc))
+    if schedule == "vcycle" and len(levels) >= 2:
+        # the reference's own schedule runs a fixed iteration count per step
+        # and does not reach the 1e-3 residual the metric asks for: reported
+        # beside the value, not as it
+        Lr, _ = make_levels(levels[:2])
+        t0 = time.perf_counter()
+        mg = O.TwoLevel(Lr[1], Lr[0])
+        log(f"[bench] oracle two-level set-up {time.perf_counter() - t0:.1f}s (not timed)")
+        pr = O.params(dt=phys["dt"], stiffness=phys["stiffness"], density=phys["density"], gravity=phys["gravity"],
+                      iters=20, iters_coarse=20, if_mode=0, **col)
+        mg.step(pr, 1)
+        nr, spsr = rate(budget_s / 2, lambda: mg.step(pr, 1))
+        out["reference_schedule"] = {
+            "value": round(spsr, 4), "unit": "steps/s", "cores": threads,
+            "sample": f"{nr} steps (median of 5 blocks) of PDSolver_MG::Step (20 coarse + 20 fine PD iterations, "
+                      f"no residual target) on the first two levels ({levels[1].ntets} + {levels[0].ntets} tets)"}
+    return out
 
 
 def main():
@@ -303,7 +378,8 @@ def main():
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         cpu = cpu_baseline(levels, cfg, "vcycle" if head in ("temporal", "learned", "learned_s") else head)
-        log(f"[bench] cpu baseline {cpu['value']} steps/s ({cpu['cores']} threads), {cpu['value_1_thread']} on one")
+        log(f"[bench] cpu baseline {cpu['value']} steps/s ({cpu['cores']} threads of {cpu['nproc']}, "
+            f"{cpu['cpu_model']}), {cpu['value_1_thread']} on one")
 
     config = {
         "workload": cfg["name"],

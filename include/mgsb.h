@@ -109,7 +109,8 @@ typedef struct mgsb_params {
                            * kept without saving a half-cycle against the last step run
                            * without it, for 8, 16, ... 256 steps before trying again */
     int32_t vert_lanes;   /* lanes that gather one vertex's row of corner forces in the
-                           * sweep and residual kernels: 1, 2, 4 or 8 (0: 4; tuned)   */
+                           * sweep and residual kernels: 1, 2, 4 or 8 (0: 1 with
+                           * tile_gather, 4 without; tuned)                          */
     float learn_slack;    /* learned schedule: the MLP's coarse correction is kept when the
                            * residual it leaves is at most learn_slack times the best one so
                            * far (1, default: it must lower the residual; 0 reads as 1).  A
@@ -118,6 +119,18 @@ typedef struct mgsb_params {
                            * adds can raise the residual while the smooth error it removes is
                            * what the next half-cycle needs gone.  A non-finite residual is
                            * never kept. */
+    int32_t coarse_dense; /* V-cycle: the coarse levels below the first (a fixed linear
+                           * map of that level's residual) applied as one dense
+                           * matrix-vector product, the matrix formed at set-up, when
+                           * that level has at most 2048 vertices (1, default); 0: the
+                           * sweeps, transfers and coarsest solve run in sequence.  The
+                           * two differ by FP32 rounding only. */
+    int32_t tile_gather;  /* per-tet kernels sum the corner forces of each tile of 256
+                           * consecutive tets per vertex in LDS and write one partial
+                           * per (tile, vertex), which the vertex kernels gather (1,
+                           * default); 0: one force per (tet, corner) in the vertex-
+                           * sorted CSR order.  Both are deterministic and differ by
+                           * the grouping of the same sums in FP32. */
 } mgsb_params;
 
 typedef struct mgsb_step_stats {
@@ -190,6 +203,18 @@ int mgsb_precompute(const mgsb_mesh* m, float stiffness, float density, float* B
  * ids/w receive 4 per point; returns the number of unbound points (>= 0) or
  * a negative error. */
 int mgsb_bind_points(const mgsb_mesh* m, const float* pts, int32_t n, int32_t mode, int32_t* ids, float* w);
+
+/* The tiled local step's gather layout of mesh m, host only (no counterpart
+ * in the reference, whose calculateIF scatters with atomicAdd,
+ * PDSolver.cu:170-181): tiles of 256 consecutive tets; for each tile its
+ * distinct vertices in ascending order, the position of each one's partial
+ * in the vertex-sorted (tile, vertex) order, and the tile's corner array in
+ * vertex-sorted (tet, corner) order.  sizes receives {ntiles, npart}
+ * (npart = sum of the tiles' vertex counts); each array may be null:
+ * prp V+1, tvo ntiles+1, tvs npart, tlr npart+ntiles, tls 2T (two 16-bit
+ * corner positions per entry). */
+int mgsb_mesh_tiles(const mgsb_mesh* m, int32_t sizes[2], int32_t* prp, int32_t* tvo, int32_t* tvs, int32_t* tlr,
+                    uint32_t* tls);
 
 /* PDSolver::Init constants (PDSolver.cpp:166-176) plus V-cycle defaults. */
 void mgsb_default_params(mgsb_params* p);

@@ -73,6 +73,8 @@ class mgsb_params(C.Structure):
         ("learn_gate", C.c_int32),
         ("vert_lanes", C.c_int32),
         ("learn_slack", C.c_float),
+        ("coarse_dense", C.c_int32),
+        ("tile_gather", C.c_int32),
     ]
 
 
@@ -120,6 +122,7 @@ SIGNATURES = {
     "mgsb_mesh_free": (None, [C.POINTER(mgsb_mesh)]),
     "mgsb_precompute": (C.c_int, [C.POINTER(mgsb_mesh), C.c_float, C.c_float, _fp, _fp, _fp, _fp, _fp]),
     "mgsb_bind_points": (C.c_int, [C.POINTER(mgsb_mesh), _fp, C.c_int32, C.c_int32, _ip, _fp]),
+    "mgsb_mesh_tiles": (C.c_int, [C.POINTER(mgsb_mesh), _ip, _ip, _ip, _ip, _ip, C.POINTER(C.c_uint32)]),
     "mgsb_default_params": (None, [C.POINTER(mgsb_params)]),
     "mgsb_create": (C.c_int, [C.POINTER(mgsb_mesh), C.c_int32, C.POINTER(mgsb_params), C.POINTER(_vp)]),
     "mgsb_destroy": (None, [_vp]),
@@ -303,6 +306,23 @@ class Mesh:
         _check(lib.mgsb_precompute(C.byref(m), stiffness, density, _fptr(B), _fptr(AC), _fptr(vol), _fptr(mass),
                                    _fptr(diag)), "mgsb_precompute")
         return {"B": B.reshape(T, 9), "AC": AC.reshape(T, 12), "vol": vol, "mass": mass, "diag": diag}
+
+    def tiles(self):
+        """The tiled local step's gather layout (tiles of 256 consecutive
+        tets): prp, tvo, tvs, tlr and the four corner positions per tet."""
+        m = self._as_c()
+        sz = np.zeros(2, np.int32)
+        _check(lib.mgsb_mesh_tiles(C.byref(m), _iptr(sz), None, None, None, None, None), "mgsb_mesh_tiles")
+        nt, npart = int(sz[0]), int(sz[1])
+        prp = np.zeros(self.nverts + 1, np.int32)
+        tvo = np.zeros(nt + 1, np.int32)
+        tvs = np.zeros(max(npart, 1), np.int32)
+        tlr = np.zeros(npart + nt, np.int32)
+        tls = np.zeros(2 * self.ntets, np.uint32)
+        _check(lib.mgsb_mesh_tiles(C.byref(m), _iptr(sz), _iptr(prp), _iptr(tvo), _iptr(tvs), _iptr(tlr),
+                                   tls.ctypes.data_as(C.POINTER(C.c_uint32))), "mgsb_mesh_tiles")
+        pos = np.stack([tls[0::2] & 0xFFFF, tls[0::2] >> 16, tls[1::2] & 0xFFFF, tls[1::2] >> 16], axis=1)
+        return {"prp": prp, "tvo": tvo, "tvs": tvs[:npart], "tlr": tlr, "pos": pos.astype(np.int64)}
 
     def bind(self, points, mode=0):
         """Barycentric binding of points into this mesh's tets

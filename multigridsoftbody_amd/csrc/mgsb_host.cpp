@@ -159,6 +159,48 @@ static void build_incidence(int V, int n_items, int corners, const int32_t* item
     }
 }
 
+void build_tiles(HostLevel& L) {
+    const int T = L.T, V = L.V, nt = (T + kTileTets - 1) / kTileTets;
+    L.tvo.assign((size_t)nt + 1, 0);
+    L.tlr.clear();
+    L.tls.assign((size_t)T * 2, 0u);
+    std::vector<int32_t> local((size_t)V, -1), cnt((size_t)V, 0), all, verts, fill;
+    for (int k = 0; k < nt; k++) {
+        const int t0 = k * kTileTets, t1 = std::min(T, t0 + kTileTets);
+        verts.clear();
+        for (int e = t0 * 4; e < t1 * 4; e++) {
+            const int v = L.tets[(size_t)e];
+            if (local[v] < 0) {
+                local[v] = 0;
+                verts.push_back(v);
+            }
+        }
+        std::sort(verts.begin(), verts.end());
+        const int nv = (int)verts.size();
+        for (int i = 0; i < nv; i++) local[verts[i]] = i;
+        fill.assign((size_t)nv + 1, 0);
+        for (int e = t0 * 4; e < t1 * 4; e++) fill[(size_t)local[L.tets[(size_t)e]] + 1]++;
+        for (int i = 0; i < nv; i++) fill[(size_t)i + 1] += fill[(size_t)i];
+        L.tlr.insert(L.tlr.end(), fill.begin(), fill.end());
+        for (int t = t0; t < t1; t++)
+            for (int j = 0; j < 4; j++) {
+                const uint32_t pos = (uint32_t)fill[(size_t)local[L.tets[(size_t)t * 4 + j]]]++;
+                L.tls[(size_t)t * 2 + (j >> 1)] |= pos << (16 * (j & 1));
+            }
+        for (int v : verts) {
+            cnt[v]++;
+            all.push_back(v);
+            local[v] = -1;
+        }
+        L.tvo[(size_t)k + 1] = L.tvo[(size_t)k] + nv;
+    }
+    L.prp.assign((size_t)V + 1, 0);
+    for (int v = 0; v < V; v++) L.prp[(size_t)v + 1] = L.prp[(size_t)v] + cnt[v];
+    std::vector<int32_t> at(L.prp.begin(), L.prp.end() - 1);
+    L.tvs.resize(all.size());
+    for (size_t q = 0; q < all.size(); q++) L.tvs[q] = at[all[q]]++;  // tiles ascending in each row
+}
+
 int build_level(const mgsb_mesh& m, float stiffness, float density, HostLevel& L) {
     if (m.nverts <= 0 || m.ntets <= 0 || !m.pos || !m.tets) {
         set_error("build_level: empty mesh");
@@ -256,6 +298,7 @@ int build_level(const mgsb_mesh& m, float stiffness, float density, HostLevel& L
     }
     build_incidence(V, T, 4, L.tets.data(), L.rp, L.slot);
     build_incidence(V, L.S, 3, reinterpret_cast<const int32_t*>(L.tris.data()), L.trp, L.tslot);
+    build_tiles(L);
     return MGSB_OK;
 }
 
@@ -807,6 +850,30 @@ int mgsb_precompute(const mgsb_mesh* m, float stiffness, float density, float* B
     }
 }
 
+int mgsb_mesh_tiles(const mgsb_mesh* m, int32_t sizes[2], int32_t* prp, int32_t* tvo, int32_t* tvs, int32_t* tlr,
+                    uint32_t* tls) {
+    if (!m || !sizes) {
+        set_error("mgsb_mesh_tiles: null mesh or sizes");
+        return MGSB_ERR_ARG;
+    }
+    try {
+        HostLevel L;
+        int rc = build_level(*m, 1.f, 1.f, L);
+        if (rc) return rc;
+        sizes[0] = (int32_t)L.tvo.size() - 1;
+        sizes[1] = L.tvo.back();
+        if (prp) std::copy(L.prp.begin(), L.prp.end(), prp);
+        if (tvo) std::copy(L.tvo.begin(), L.tvo.end(), tvo);
+        if (tvs) std::copy(L.tvs.begin(), L.tvs.end(), tvs);
+        if (tlr) std::copy(L.tlr.begin(), L.tlr.end(), tlr);
+        if (tls) std::copy(L.tls.begin(), L.tls.end(), tls);
+        return MGSB_OK;
+    } catch (const std::exception& e) {
+        set_error(std::string("mgsb_mesh_tiles: ") + e.what());
+        return MGSB_ERR_NOMEM;
+    }
+}
+
 int mgsb_bind_points(const mgsb_mesh* m, const float* pts, int32_t n, int32_t mode, int32_t* ids, float* w) {
     if (!m || (n > 0 && (!pts || !ids || !w)) || n < 0 || (mode != 0 && mode != 1)) {
         set_error("mgsb_bind_points: bad arguments");
@@ -864,6 +931,8 @@ void mgsb_default_params(mgsb_params* p) {
     p->learn_gate = 1;
     p->vert_lanes = 0;
     p->learn_slack = 1.0f;
+    p->coarse_dense = 1;
+    p->tile_gather = 1;
 }
 
 const char* mgsb_version(void) { return "mgsb 0.1 (gfx950)"; }

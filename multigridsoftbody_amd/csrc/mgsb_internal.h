@@ -35,6 +35,19 @@ struct HostLevel {
     // vertex -> incident (surface tri, corner), tris ascending.
     std::vector<int32_t> trp;     // V+1
     std::vector<int32_t> tslot;   // 3S
+    // Tiles of kTileTets consecutive tets, one workgroup each: the tile sums
+    // its corner forces per vertex and writes one partial per (tile, vertex).
+    // prp: vertex -> its partials, tiles ascending.  Tile k's vertices
+    // (ascending ids) are tvo[k] .. tvo[k+1]-1 of tvs (the position of each
+    // one's partial in prp order); tlr holds each tile's nv+1 row starts into
+    // its corner array (at tvo[k] + k), and tls the position of corner j of
+    // tet t in that array (16 bits each, corners 0,1 in tls[2t], 2,3 in
+    // tls[2t+1]); a row lists its corners in (tet, corner) order.
+    std::vector<int32_t> prp;     // V+1
+    std::vector<int32_t> tvo;     // ntiles+1
+    std::vector<int32_t> tvs;     // tvo[ntiles]
+    std::vector<int32_t> tlr;     // tvo[ntiles] + ntiles
+    std::vector<uint32_t> tls;    // 2T
     std::vector<float> tk;        // per-tet ARAP weight per unit volume (empty: the scalar stiffness)
     int bad_tet = -1;
 };
@@ -48,6 +61,10 @@ void recompute_diag(HostLevel& L, float k);
 // Connected components of a tet mesh (union of tet corners), numbered in
 // order of their smallest vertex index; returns the component count.
 int mesh_components(int V, int T, const int32_t* tets, std::vector<int32_t>& comp);
+
+constexpr int kTileTets = 256;
+// The tile structures of a level with tets and V set (see HostLevel).
+void build_tiles(HostLevel& L);
 
 // Builds a level: copies the mesh, precomputes, builds gather structures.
 // stiffness = ARAP weight per unit volume, density scales the lumped mass.

@@ -465,8 +465,7 @@ int mlp_infer(Mlp* p, const float* X, int n, float* Y, int ldy, hipStream_t st, 
 // vertex (fine velocity injected through the coarse-in-fine binding, rest
 // position, the last two coarse deltas), standardised, through the network,
 // and its output de-standardised onto the last delta and masked by the
-// level's Dirichlet flags (the same values as k_features, mlp_infer and
-// k_apply_pred in sequence).
+// level's Dirichlet flags (the features are k_features' values).
 struct CoarseIn {
     CoarseIO io;
     int n;
@@ -482,13 +481,16 @@ struct CoarseIn {
                 raw[2] = w.x * a.z + w.y * b.z + w.z * c.z + w.w * d.z;
                 raw[3] = io.X1[row].x;
             } else if (k0 == 4) {
-                const float4 X = io.X1[row], e = io.e1[row];
+                const float4 X = io.X1[row];
+                const float3 e = inject4(io.D1, io.aids[row], io.aw[row]);
                 raw[0] = X.y;
                 raw[1] = X.z;
                 raw[2] = e.x;
                 raw[3] = e.y;
             } else if (k0 == 8) {
-                const float4 e = io.e1[row], f = io.e2[row];
+                const int4 id = io.aids[row];
+                const float4 w = io.aw[row];
+                const float3 e = inject4(io.D1, id, w), f = inject4(io.D2, id, w);
                 raw[0] = e.z;
                 raw[1] = f.x;
                 raw[2] = f.y;
@@ -508,7 +510,7 @@ struct CoarseOut {
     __device__ void store(int row, int col, float v) const {
         if (row >= n || col >= 3) return;
         const float mu = io.norm[2 * kCoarseFeat + col], sd = io.norm[2 * kCoarseFeat + 3 + col];
-        const float4 e = io.e1[row];
+        const float3 e = inject4(io.D1, io.aids[row], io.aw[row]);
         const float prior = col == 0 ? e.x : (col == 1 ? e.y : e.z);
         reinterpret_cast<float*>(io.epred + row)[col] = io.vcl[row].z * (prior + (v * sd + mu));
     }

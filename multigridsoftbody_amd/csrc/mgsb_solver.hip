@@ -211,16 +211,34 @@ __device__ void vstate_step_done(VState* st);
 
 // End of a split-position step: x = s + u (and x_prev = s + u_prev), then
 // calculateV; block 0 folds the step's outcome into the run aggregates.
+// D1 != nullptr (learned schedule): D2 = D1, D1 = what the step's correction
+// added to its temporal extrapolation, u - vfix (2 u1 - u2) (u when
+// temporal == 0), per fine vertex; the learned stage injects both at the
+// coarse vertices as its last two coarse deltas.
 __global__ void __launch_bounds__(256) k_finish_split(int V, float4* __restrict__ x, float4* __restrict__ xprev,
                                                       const float4* __restrict__ s, const float4* __restrict__ xlast,
                                                       float4* __restrict__ v, float dt, VState* vs,
                                                       float4* __restrict__ U1, float4* __restrict__ U2,
+                                                      const float4* __restrict__ vc, float4* __restrict__ D1,
+                                                      float4* __restrict__ D2, int temporal,
                                                       const int* __restrict__ skip) {
     if (skip && *skip) return;
     if (vs && blockIdx.x == 0 && threadIdx.x == 0) vstate_step_done(vs);
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= V) return;
     const float4 u = x[i], up = xprev[i], st = s[i], q = xlast[i];
+    if (D1) {
+        float4 a = u;
+        if (temporal) {
+            const float4 u1 = U1[i], u2 = U2[i];
+            const float f = vc[i].z;
+            a.x = a.x - f * (2.f * u1.x - u2.x);
+            a.y = a.y - f * (2.f * u1.y - u2.y);
+            a.z = a.z - f * (2.f * u1.z - u2.z);
+        }
+        D2[i] = D1[i];
+        D1[i] = make_float4(a.x, a.y, a.z, 0.f);
+    }
     if (U1) {  // corrections of the last two steps (temporal warm start)
         U2[i] = U1[i];
         U1[i] = make_float4(u.x, u.y, u.z, 0.f);
@@ -289,13 +307,13 @@ __device__ __forceinline__ bool tri_sphere_hit(float3 A, float3 B, float3 C, flo
 // decomposition, P = (R - F) AC with AC rebuilt from Dm^-1, NaN -> 0, clamp to
 // [-10, 10], times vol * k (calculateIF, PDSolver.cu:128-183; calTetIFV1's
 // |f| < 1e-5 drop in Ac mode, PDSolverAc.cu:630-662).  Corner j's force goes
-// to cf[slot_j].  sref != nullptr: split positions (x holds u, see below).
+// to fo[3j..3j+2].  sref != nullptr: split positions (x holds u, see below).
 template <int MODE>
-__device__ __forceinline__ void tet_local_step(int g, int T, const float4* __restrict__ x,
-                                               const int4* __restrict__ tet, const float* __restrict__ B,
-                                               const float* __restrict__ vol, const int4* __restrict__ slot,
-                                               float3* __restrict__ cf, float k, const float4* __restrict__ sref,
-                                               const float* __restrict__ tk) {
+__device__ __forceinline__ void tet_local_forces(int g, int T, const float4* __restrict__ x,
+                                                 const int4* __restrict__ tet, const float* __restrict__ B,
+                                                 const float* __restrict__ vol, float k,
+                                                 const float4* __restrict__ sref, const float* __restrict__ tk,
+                                                 float (&fo)[12]) {
     // every load is issued up front: the per-tet streams and the index in the
     // first round trip, the corner gathers of x (and s) in the second
     const int4 id = tet[g];
@@ -304,7 +322,6 @@ __device__ __forceinline__ void tet_local_step(int g, int T, const float4* __res
     for (int q = 0; q < 9; q++) b[q] = B[(size_t)q * T + g];
     const float vt = vol[g];
     const float kk = tk ? tk[g] : k;  // per-body stiffness when several bodies have their own E
-    const int4 sl = slot[g];
     const float4 p0 = x[id.x], p1 = x[id.y], p2 = x[id.z], p3 = x[id.w];
     float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0, s2 = s0, s3 = s0;
     if (sref) {
@@ -350,10 +367,8 @@ __device__ __forceinline__ void tet_local_step(int g, int T, const float4* __res
         AC[r * 4 + 2] = b[3 + r];
         AC[r * 4 + 3] = b[6 + r];
     }
-    const int sls[4] = {sl.x, sl.y, sl.z, sl.w};
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        float f[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             float acc = RF[i * 3 + 0] * AC[0 * 4 + j] + RF[i * 3 + 1] * AC[1 * 4 + j] + RF[i * 3 + 2] * AC[2 * 4 + j];
@@ -364,10 +379,93 @@ __device__ __forceinline__ void tet_local_step(int g, int T, const float4* __res
             float r = acc * vt * kk;
             // |r| < 1e-5 against the double literal is exactly |r| <= 1e-5f
             if (MODE == kTetModeAc && fabsf(r) <= 1e-5f) r = 0.f;
-            f[i] = r;
+            fo[j * 3 + i] = r;
         }
-        cf[sls[j]] = make_float3(f[0], f[1], f[2]);
     }
+}
+
+// The local step of tet g with corner j's force stored to cf[slot_j] (the
+// per-corner CSR layout).
+template <int MODE>
+__device__ __forceinline__ void tet_local_step(int g, int T, const float4* __restrict__ x,
+                                               const int4* __restrict__ tet, const float* __restrict__ B,
+                                               const float* __restrict__ vol, const int4* __restrict__ slot,
+                                               float3* __restrict__ cf, float k, const float4* __restrict__ sref,
+                                               const float* __restrict__ tk) {
+    const int4 sl = slot[g];
+    float fo[12];
+    tet_local_forces<MODE>(g, T, x, tet, B, vol, k, sref, tk, fo);
+    cf[sl.x] = make_float3(fo[0], fo[1], fo[2]);
+    cf[sl.y] = make_float3(fo[3], fo[4], fo[5]);
+    cf[sl.z] = make_float3(fo[6], fo[7], fo[8]);
+    cf[sl.w] = make_float3(fo[9], fo[10], fo[11]);
+}
+
+// One tile of kTileTets consecutive tets per workgroup: the tile's corner
+// forces go to LDS at their positions in the tile's vertex-sorted corner
+// array (tls), then each of the tile's vertices sums its row there in (tet,
+// corner) order and writes one partial to its slot in the vertex-sorted
+// (tile, vertex) order (tvs).  The vertex kernels gather those partials, a
+// row of ~2 per vertex instead of ~20 corners, and the tet kernel's scattered
+// stores shrink by the same factor.  Deterministic: fixed sums, no atomics.
+struct TileLds {
+    float x[4 * kTileTets], y[4 * kTileTets], z[4 * kTileTets];
+};
+
+__device__ __forceinline__ void tile_put(TileLds& L, uint2 ls, const float (&fo)[12]) {
+    const int p[4] = {(int)(ls.x & 0xffffu), (int)(ls.x >> 16), (int)(ls.y & 0xffffu), (int)(ls.y >> 16)};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        L.x[p[j]] = fo[j * 3 + 0];
+        L.y[p[j]] = fo[j * 3 + 1];
+        L.z[p[j]] = fo[j * 3 + 2];
+    }
+}
+
+__device__ __forceinline__ void tile_sum(const TileLds& L, int tile, const int* __restrict__ tvo,
+                                         const int* __restrict__ tvs, const int* __restrict__ tlr,
+                                         float3* __restrict__ pcf) {
+    const int v0 = tvo[tile], nv = tvo[tile + 1] - v0;
+    const int* lr = tlr + v0 + tile;
+    for (int l = threadIdx.x; l < nv; l += kTileTets) {
+        const int r0 = lr[l], r1 = lr[l + 1];
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        for (int e = r0; e < r1; e++) {
+            fx = fx + L.x[e];
+            fy = fy + L.y[e];
+            fz = fz + L.z[e];
+        }
+        pcf[tvs[v0 + l]] = make_float3(fx, fy, fz);
+    }
+}
+
+// Triangle-sphere DCD of surface triangle t (PDSolver.cu:612-680): writes
+// (k*(centre-hit), k) summed over the spheres to each of its corners' slots.
+__device__ __forceinline__ void tri_lane(int t, int ntri, const float4* __restrict__ x, const int4* __restrict__ tri,
+                                         const int4* __restrict__ tslot, float4* __restrict__ tcf,
+                                         const DevColliders* __restrict__ col) {
+    if (t >= ntri) return;
+    const int4 id = tri[t];
+    const float3 A = f3(x[id.x]), Bq = f3(x[id.y]), Cq = f3(x[id.z]);
+    float3 n = cross3(sub3(Bq, A), sub3(Cq, A));
+    n = div3(n, len3(n));
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float kc = col->kcol;
+    for (int q = 0; q < col->nsph; q++) {
+        if (col->vel[q].w == 0.f) continue;
+        const float4 sp = col->sph[q];
+        float3 mv;
+        if (tri_sphere_hit(A, Bq, Cq, n, make_float3(sp.x, sp.y, sp.z), sp.w, mv)) {
+            acc.x = acc.x + kc * mv.x;
+            acc.y = acc.y + kc * mv.y;
+            acc.z = acc.z + kc * mv.z;
+            acc.w = acc.w + kc;
+        }
+    }
+    const int4 sl = tslot[t];
+    tcf[sl.x] = acc;
+    tcf[sl.y] = acc;
+    tcf[sl.z] = acc;
 }
 
 // Per-tet local step kernel, one tet per lane (tet_local_step).  TRI: lanes
@@ -385,33 +483,38 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
     if (skip && *skip) return;
     const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (g >= T) {
-        if (!TRI) return;
-        const int t = g - T;
-        if (t >= ntri) return;
-        const int4 id = tri[t];
-        const float3 A = f3(x[id.x]), Bq = f3(x[id.y]), Cq = f3(x[id.z]);
-        float3 n = cross3(sub3(Bq, A), sub3(Cq, A));
-        n = div3(n, len3(n));
-        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-        const float kc = col->kcol;
-        for (int q = 0; q < col->nsph; q++) {
-            if (col->vel[q].w == 0.f) continue;
-            const float4 sp = col->sph[q];
-            float3 mv;
-            if (tri_sphere_hit(A, Bq, Cq, n, make_float3(sp.x, sp.y, sp.z), sp.w, mv)) {
-                acc.x = acc.x + kc * mv.x;
-                acc.y = acc.y + kc * mv.y;
-                acc.z = acc.z + kc * mv.z;
-                acc.w = acc.w + kc;
-            }
-        }
-        const int4 sl = tslot[t];
-        tcf[sl.x] = acc;
-        tcf[sl.y] = acc;
-        tcf[sl.z] = acc;
+        if (TRI) tri_lane(g - T, ntri, x, tri, tslot, tcf, col);
         return;
     }
     tet_local_step<MODE>(g, T, x, tet, B, vol, slot, cf, k, sref, tk);
+}
+
+// The tiled form of k_tet (tile_put / tile_sum): workgroups 0..ntiles-1 run
+// one tile each (kTileTets lanes), the ones after them the triangle lanes.
+template <int MODE, bool TRI>
+__global__ void __launch_bounds__(kTileTets) k_tet_tiled(
+    int T, int ntiles, const float4* __restrict__ x, const int4* __restrict__ tet, const float* __restrict__ B,
+    const float* __restrict__ vol, const uint2* __restrict__ tls, const int* __restrict__ tvo,
+    const int* __restrict__ tvs, const int* __restrict__ tlr, float3* __restrict__ pcf, float k, int ntri,
+    const int4* __restrict__ tri, const int4* __restrict__ tslot, float4* __restrict__ tcf,
+    const DevColliders* __restrict__ col, const int* __restrict__ skip, const float4* __restrict__ sref,
+    const float* __restrict__ tk) {
+    if (skip && *skip) return;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    if (blk >= ntiles) {
+        if (TRI) tri_lane((blk - ntiles) * kTileTets + threadIdx.x, ntri, x, tri, tslot, tcf, col);
+        return;
+    }
+    __shared__ TileLds L;
+    const int g = blk * kTileTets + threadIdx.x;
+    if (g < T) {
+        const uint2 ls = tls[g];
+        float fo[12];
+        tet_local_forces<MODE>(g, T, x, tet, B, vol, k, sref, tk, fo);
+        tile_put(L, ls, fo);
+    }
+    __syncthreads();
+    tile_sum(L, blk, tvo, tvs, tlr, pcf);
 }
 
 // Start of a temporally warm-started V-cycle step: lanes 0..T-1 evaluate the
@@ -429,6 +532,30 @@ __global__ void __launch_bounds__(256) k_tet_dual(int T, const float4* __restric
         tet_local_step<MODE>(g, T, x, tet, B, vol, slot, cf, k, s, tk);
     else if (g < 2 * T)
         tet_local_step<MODE>(g - T, T, s, tet, B, vol, slot, cf2, k, nullptr, tk);
+}
+
+// The tiled form of k_tet_dual: workgroups 0..ntiles-1 take the tiles at the
+// extrapolation (partials to pcf), ntiles..2 ntiles-1 the same tiles at s
+// (pcf2).
+template <int MODE>
+__global__ void __launch_bounds__(kTileTets) k_tet_dual_tiled(
+    int T, int ntiles, const float4* __restrict__ x, const float4* __restrict__ s, const int4* __restrict__ tet,
+    const float* __restrict__ B, const float* __restrict__ vol, const uint2* __restrict__ tls,
+    const int* __restrict__ tvo, const int* __restrict__ tvs, const int* __restrict__ tlr, float3* __restrict__ pcf,
+    float3* __restrict__ pcf2, float k, const float* __restrict__ tk) {
+    const int b0 = xcd_block(blockIdx.x, gridDim.x);
+    const bool at_s = b0 >= ntiles;
+    const int blk = at_s ? b0 - ntiles : b0;
+    __shared__ TileLds L;
+    const int g = blk * kTileTets + threadIdx.x;
+    if (g < T) {
+        const uint2 ls = tls[g];
+        float fo[12];
+        tet_local_forces<MODE>(g, T, at_s ? s : x, tet, B, vol, k, at_s ? nullptr : s, tk, fo);
+        tile_put(L, ls, fo);
+    }
+    __syncthreads();
+    tile_sum(L, blk, tvo, tvs, tlr, at_s ? pcf2 : pcf);
 }
 
 // calculatePOS's under-relaxation (next - pos) * 0.6 + pos (PDSolver.cu:
@@ -1261,12 +1388,13 @@ __global__ void __launch_bounds__(256) k_revert(int V, float4* __restrict__ x, f
 // Features of the learned coarse correction at each vertex of level 1, all
 // known before the step starts: the fine velocity injected through the
 // coarse-in-fine binding (3), the rest position (3), and the coarse deltas
-// achieved by the last two steps (3 + 3, k_target); normalised with
-// (mu, sd).  Raw values optionally recorded.
+// of the last two steps (3 + 3): the fine corrections beyond the temporal
+// extrapolation (D1, D2, kept by k_finish_split) injected the same way;
+// normalised with (mu, sd).  Raw values optionally recorded.
 constexpr int kFeat = 12;
 __global__ void __launch_bounds__(256) k_features(int Vc, const float4* __restrict__ vf, const int4* __restrict__ aids,
                                                   const float4* __restrict__ aw, const float4* __restrict__ X0,
-                                                  const float4* __restrict__ eprev, const float4* __restrict__ eprev2,
+                                                  const float4* __restrict__ D1, const float4* __restrict__ D2,
                                                   const float* __restrict__ norm, float* __restrict__ feat,
                                                   float* __restrict__ rec) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1274,7 +1402,8 @@ __global__ void __launch_bounds__(256) k_features(int Vc, const float4* __restri
     const int4 id = aids[j];
     const float4 w = aw[j];
     const float4 a = vf[id.x], b = vf[id.y], c = vf[id.z], d = vf[id.w];
-    const float4 X = X0[j], ep = eprev[j], ep2 = eprev2[j];
+    const float4 X = X0[j];
+    const float3 ep = inject4(D1, id, w), ep2 = inject4(D2, id, w);
     const float raw[kFeat] = {w.x * a.x + w.y * b.x + w.z * c.x + w.w * d.x,
                               w.x * a.y + w.y * b.y + w.z * c.y + w.w * d.y,
                               w.x * a.z + w.y * b.z + w.z * c.z + w.w * d.z,
@@ -1286,60 +1415,21 @@ __global__ void __launch_bounds__(256) k_features(int Vc, const float4* __restri
     }
 }
 
-// epred = vfix * (eprev + pred * out_sd + out_mu): the network predicts the
-// change of the coarse delta from the last step's (which is its prior)
-__global__ void __launch_bounds__(256) k_apply_pred(int Vc, const float* __restrict__ pred,
-                                                    const float* __restrict__ norm, const float4* __restrict__ vcl,
-                                                    const float4* __restrict__ eprev, float4* __restrict__ epred) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= Vc) return;
-    const float* mu = norm + 2 * kFeat;
-    const float* sd = norm + 2 * kFeat + 3;
-    const float m = vcl[j].z;
-    const float4 e1 = eprev[j];
-    epred[j] = make_float4(m * (e1.x + (pred[j * 3 + 0] * sd[0] + mu[0])), m * (e1.y + (pred[j * 3 + 1] * sd[1] + mu[1])),
-                           m * (e1.z + (pred[j * 3 + 2] * sd[2] + mu[2])), 0.f);
-}
-
-// Coarse delta achieved this step: (u - u_t) injected at each level-1 vertex
-// through its coarse-in-fine binding, u = x - s the step's correction (x
-// holds u) and u_t its temporal extrapolation (U1 == nullptr: u_t = 0).  Kept
-// with the previous one for the next step's features (eprev, eprev2) and
-// optionally recorded as training target.
-__global__ void __launch_bounds__(256) k_target(int Vc, const float4* __restrict__ x, const float4* __restrict__ U1,
-                                                const float4* __restrict__ U2, const float4* __restrict__ vc,
+// Training target of a recorded step, after k_finish_split has shifted the
+// history: the change of the coarse delta, inject(D1) - inject(D2) (this
+// step's against the last step's, which is the network's prior).
+__global__ void __launch_bounds__(256) k_target(int Vc, const float4* __restrict__ D1, const float4* __restrict__ D2,
                                                 const int4* __restrict__ aids, const float4* __restrict__ aw,
-                                                float4* __restrict__ eprev, float4* __restrict__ eprev2,
                                                 float* __restrict__ rec, const int* __restrict__ skip) {
     if (skip && *skip) return;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= Vc) return;
     const int4 id = aids[j];
     const float4 w = aw[j];
-    const int ids[4] = {id.x, id.y, id.z, id.w};
-    const float ws[4] = {w.x, w.y, w.z, w.w};
-    float dx = 0.f, dy = 0.f, dz = 0.f;
-    for (int q = 0; q < 4; q++) {
-        float4 a = x[ids[q]];
-        if (U1) {
-            const float4 u1 = U1[ids[q]], u2 = U2[ids[q]];
-            const float f = vc[ids[q]].z;
-            a.x = a.x - f * (2.f * u1.x - u2.x);
-            a.y = a.y - f * (2.f * u1.y - u2.y);
-            a.z = a.z - f * (2.f * u1.z - u2.z);
-        }
-        dx = dx + ws[q] * a.x;
-        dy = dy + ws[q] * a.y;
-        dz = dz + ws[q] * a.z;
-    }
-    const float4 e1 = eprev[j];
-    eprev2[j] = e1;
-    eprev[j] = make_float4(dx, dy, dz, 0.f);
-    if (rec) {  // the network's target: the change from the last step's delta
-        rec[(long)j * 3 + 0] = dx - e1.x;
-        rec[(long)j * 3 + 1] = dy - e1.y;
-        rec[(long)j * 3 + 2] = dz - e1.z;
-    }
+    const float3 e = inject4(D1, id, w), e1 = inject4(D2, id, w);
+    rec[(long)j * 3 + 0] = e.x - e1.x;
+    rec[(long)j * 3 + 1] = e.y - e1.y;
+    rec[(long)j * 3 + 2] = e.z - e1.z;
 }
 
 // r_c[j] = vfix_j * sum_i w_ij r_f[i]  (P^T, no atomics).  One wave per
@@ -1999,6 +2089,11 @@ struct DevLevel {
     float *B = nullptr, *vol = nullptr;
     int* rp = nullptr;
     float3* cf = nullptr;  // corner forces in vertex-sorted (CSR) order
+    // tiled local step (HostLevel::prp ...): per-(tile, vertex) partials
+    int ntiles = 0, npart = 0;
+    int *prp = nullptr, *tvo = nullptr, *tvs = nullptr, *tlr = nullptr;
+    uint2* tls = nullptr;
+    float3 *pcf = nullptr, *pcf2 = nullptr;
     int4 *tri = nullptr, *tslot = nullptr;
     int *trp = nullptr, *trid = nullptr;
     float4 *tcf = nullptr, *tn = nullptr, *vn = nullptr;
@@ -2120,7 +2215,8 @@ struct mgsb_solver {
     // learned coarse correction (level 1)
     Mlp* mlp = nullptr;
     float *feat = nullptr, *pred = nullptr, *norm = nullptr;
-    float4 *epred = nullptr, *eprev = nullptr, *eprev2 = nullptr, *X1 = nullptr;
+    float4 *epred = nullptr, *X1 = nullptr;
+    float4 *D1 = nullptr, *D2 = nullptr;  // fine level: the last two steps' corrections beyond the extrapolation
     float *recX = nullptr, *recT = nullptr;  // recorded samples (raw)
     int rec_cap = 0, rec_steps = 0;           // in steps of V1 samples
     bool recording = false;
@@ -2203,6 +2299,14 @@ int upload_level(mgsb_solver* s, DevLevel& d) {
     if ((rc = up(s, d.owned, &d.vol, h.vol.data(), h.T))) return rc;
     if ((rc = up(s, d.owned, &d.rp, h.rp.data(), h.rp.size()))) return rc;
     if ((rc = up<float3>(s, d.owned, &d.cf, nullptr, (size_t)h.T * 4))) return rc;
+    d.ntiles = (int)h.tvo.size() - 1;
+    d.npart = h.tvo.back();
+    if ((rc = up(s, d.owned, &d.prp, h.prp.data(), h.prp.size()))) return rc;
+    if ((rc = up(s, d.owned, &d.tvo, h.tvo.data(), h.tvo.size()))) return rc;
+    if ((rc = up(s, d.owned, &d.tvs, h.tvs.data(), h.tvs.size()))) return rc;
+    if ((rc = up(s, d.owned, &d.tlr, h.tlr.data(), h.tlr.size()))) return rc;
+    if ((rc = up(s, d.owned, &d.tls, reinterpret_cast<const uint2*>(h.tls.data()), (size_t)h.T))) return rc;
+    if ((rc = up<float3>(s, d.owned, &d.pcf, nullptr, (size_t)d.npart))) return rc;
     std::vector<int4> tr4(std::max(1, h.S)), ts4(std::max(1, h.S));
     for (int t = 0; t < h.S; t++) {
         tr4[t] = make_int4((int)h.tris[t * 3], (int)h.tris[t * 3 + 1], (int)h.tris[t * 3 + 2], 0);
@@ -2257,7 +2361,9 @@ int block_vert(const mgsb_solver* s) { return s->p.block_vert > 0 ? s->p.block_v
 // lanes per vertex of the gathering vertex kernels (k_vert, k_resid,
 // k_resid_dual) as a shift: vert_lanes 1, 2, 4 or 8 (0: 4)
 int vert_lsh(const mgsb_solver* s) {
-    const int l = s->p.vert_lanes > 0 ? s->p.vert_lanes : 4;
+    // default: one lane for the ~2.5 partials a vertex has in the tiled
+    // layout, four for its ~20 corners in the per-corner one
+    const int l = s->p.vert_lanes > 0 ? s->p.vert_lanes : (s->p.tile_gather ? 1 : 4);
     return l >= 8 ? 3 : l >= 4 ? 2 : l >= 2 ? 1 : 0;
 }
 constexpr int kMaxVertLanes = 8;
@@ -2284,14 +2390,28 @@ int timed_end(mgsb_solver* s, int kernel) {
 // split: x holds the in-step correction u and the kernel adds F(s) (V-cycle steps).
 template <int MODE>
 void launch_tet_mode(mgsb_solver* s, DevLevel& d, int nb, int bs, const float4* xx, int ntri, const int* skip,
-                     const float4* sref) {
+                     const float4* sref, bool tiled) {
+    if (tiled) {
+        auto k = ntri ? k_tet_tiled<MODE, true> : k_tet_tiled<MODE, false>;
+        k<<<d.ntiles + nblk(ntri, kTileTets), kTileTets, 0, s->stream>>>(
+            d.T, d.ntiles, xx, d.tet, d.B, d.vol, d.tls, d.tvo, d.tvs, d.tlr, d.pcf, s->p.stiffness, ntri, d.tri,
+            d.tslot, d.tcf, s->dcol, skip, sref, d.tk);
+        return;
+    }
     auto k = ntri ? k_tet<MODE, true> : k_tet<MODE, false>;
     k<<<nb, bs, 0, s->stream>>>(d.T, xx, d.tet, d.B, d.vol, d.slot, d.cf, s->p.stiffness, ntri, d.tri, d.tslot,
                                 d.tcf, s->dcol, skip, sref, d.tk);
 }
 
+// The per-vertex gathers read the (tile, vertex) partials of the tiled local
+// step, or the per-corner forces.
+bool tiled(const mgsb_solver* s) { return s->p.tile_gather != 0; }
+const int* gath_rp(const mgsb_solver* s, const DevLevel& d) { return tiled(s) ? d.prp : d.rp; }
+float3* gath_cf(const mgsb_solver* s, const DevLevel& d) { return tiled(s) ? d.pcf : d.cf; }
+float3* gath_cf2(const mgsb_solver* s, const DevLevel& d) { return tiled(s) ? d.pcf2 : d.cf2; }
+
 int launch_tet(mgsb_solver* s, DevLevel& d, int mode, bool with_tri, const float4* xsrc = nullptr,
-               const int* skip = nullptr, bool split = false) {
+               const int* skip = nullptr, bool split = false, bool corners = false) {
     const float4* sref = split ? d.s : nullptr;
     const int bs = block_tet(s);
     const int ntri = with_tri ? d.S : 0;
@@ -2302,11 +2422,12 @@ int launch_tet(mgsb_solver* s, DevLevel& d, int mode, bool with_tri, const float
     int rc = timed_begin(s, kid);
     if (rc) return rc;
     const int nb = nblk(n, bs);
+    const bool til = tiled(s) && !corners;  // corners: the per-corner layout (test readback)
     switch (mode) {
-        case kTetModePolar: launch_tet_mode<kTetModePolar>(s, d, nb, bs, xx, ntri, skip, sref); break;
-        case kTetModeAc: launch_tet_mode<kTetModeAc>(s, d, nb, bs, xx, ntri, skip, sref); break;
-        case kTetModeStable: launch_tet_mode<kTetModeStable>(s, d, nb, bs, xx, ntri, skip, sref); break;
-        default: launch_tet_mode<kTetModeLinear>(s, d, nb, bs, xx, ntri, skip, sref);
+        case kTetModePolar: launch_tet_mode<kTetModePolar>(s, d, nb, bs, xx, ntri, skip, sref, til); break;
+        case kTetModeAc: launch_tet_mode<kTetModeAc>(s, d, nb, bs, xx, ntri, skip, sref, til); break;
+        case kTetModeStable: launch_tet_mode<kTetModeStable>(s, d, nb, bs, xx, ntri, skip, sref, til); break;
+        default: launch_tet_mode<kTetModeLinear>(s, d, nb, bs, xx, ntri, skip, sref, til);
     }
     HIPCHK(hipGetLastError());
     return timed_end(s, kid);
@@ -2320,6 +2441,17 @@ int tet_mode_of(int if_mode) {
 // Local step at the temporal extrapolation (forces to cf) and at s (cf2) in
 // one launch of 2T lanes.
 int launch_tet_dual(mgsb_solver* s, DevLevel& d) {
+    if (tiled(s)) {
+        const int nb = 2 * d.ntiles;
+        auto k = tet_mode_of(s->p.if_mode) == kTetModePolar
+                     ? k_tet_dual_tiled<kTetModePolar>
+                     : (tet_mode_of(s->p.if_mode) == kTetModeAc ? k_tet_dual_tiled<kTetModeAc>
+                                                                  : k_tet_dual_tiled<kTetModeStable>);
+        k<<<nb, kTileTets, 0, s->stream>>>(d.T, d.ntiles, d.x, d.s, d.tet, d.B, d.vol, d.tls, d.tvo, d.tvs, d.tlr,
+                                           d.pcf, d.pcf2, s->p.stiffness, d.tk);
+        HIPCHK(hipGetLastError());
+        return MGSB_OK;
+    }
     const int bs = block_tet(s);
     const int nb = nblk(2 * d.T, bs);
     switch (tet_mode_of(s->p.if_mode)) {
@@ -2360,7 +2492,8 @@ int launch_vert(mgsb_solver* s, DevLevel& d, float omega, int flags, const int* 
         s->relax_hi = (float)r;
         s->relax_lo = (float)(r - (double)s->relax_hi);
     }
-    k_vert<<<nblk(d.V << lsh, bs), bs, 0, s->stream>>>(d.V, d.x, d.xprev, d.s, d.vc, d.rp, d.cf, d.trp, d.tcf, s->dcol,
+    k_vert<<<nblk(d.V << lsh, bs), bs, 0, s->stream>>>(d.V, d.x, d.xprev, d.s, d.vc, gath_rp(s, d), gath_cf(s, d),
+                                                        d.trp, d.tcf, s->dcol,
                                                         omega, s->relax_hi, s->relax_lo, flags, d.cflag, d.cdepth,
                                                         d.cenergy, skip, split, lsh);
     HIPCHK(hipGetLastError());
@@ -2480,7 +2613,7 @@ int vcycle_lin(mgsb_solver* s, int l, const int* skip) {
         return MGSB_OK;
     }
     if (l == s->tail_from) {
-        if (d.tailB)
+        if (d.tailB && s->p.coarse_dense)
             k_tail_gemv<<<nblk(d.V * 64, 256), 256, 0, s->stream>>>(d.V, d.tailB, d.r, d.e, skip);
         else
             k_coarse_tail<<<1, 1024, 0, s->stream>>>(s->tail, nl - l, s->p.coarse_sweeps, s->p.relax_coarse,
@@ -2524,7 +2657,7 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool res
         return rc;
     const int lsh = vert_lsh(s), nb = nblk(d.V << lsh, 256);
     if (s->body_checks && chk != kChkNone) {  // per-body norms, then the check
-        k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint,
+        k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, gath_rp(s, d), gath_cf(s, d), s->dcol, kVertPoint,
                                            write_r ? d.r : nullptr, nullptr, restart ? d.xprev : nullptr, skip, 1,
                                            s->g2, d.cmask, lsh);
         HIPCHK(hipGetLastError());
@@ -2534,7 +2667,7 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool res
         HIPCHK(hipGetLastError());
         return MGSB_OK;
     }
-    k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, write_r ? d.r : nullptr,
+    k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, gath_rp(s, d), gath_cf(s, d), s->dcol, kVertPoint, write_r ? d.r : nullptr,
                                        chk != kChkNone ? s->vpart : nullptr, restart ? d.xprev : nullptr, skip, 1,
                                        nullptr, d.cmask, lsh);
     HIPCHK(hipGetLastError());
@@ -2582,25 +2715,26 @@ int enqueue_vc_start(mgsb_solver* s, bool learned) {
     float* recX = s->recording ? s->recX + (size_t)s->rec_steps * c1.V * kFeat : nullptr;
     int rc;
     if (features) {  // features at the step's start (recorded even if the step converges at once)
-        k_features<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, f.v, s->M[0].aids, s->M[0].aw, s->X1, s->eprev,
-                                                         s->eprev2, s->norm, s->feat, recX);
+        k_features<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, f.v, s->M[0].aids, s->M[0].aw, s->X1, s->D1, s->D2,
+                                                         s->norm, s->feat, recX);
         HIPCHK(hipGetLastError());
     }
     if (learned) {  // features, network, prior and mask in one launch
-        const CoarseIO io{f.v, s->M[0].aids, s->M[0].aw, s->X1, s->eprev, s->eprev2, s->norm, c1.vcl, s->epred};
+        const CoarseIO io{f.v, s->M[0].aids, s->M[0].aw, s->X1, s->D1, s->D2, s->norm, c1.vcl, s->epred};
         if ((rc = mlp_infer_coarse(s->mlp, c1.V, io, s->stream))) return rc;
     }
     if (dual) {
         if ((rc = launch_st(s, f, 1, 1, temporal, ep))) return rc;
         if ((rc = launch_tet_dual(s, f))) return rc;
         const int lsh = vert_lsh(s), nbr = nblk(f.V << lsh, 256);
-        k_resid_dual<<<nbr, 256, 0, s->stream>>>(f.V, f.x, f.s, f.vc, f.rp, f.cf, f.cf2, s->dcol, kVertPoint, nullptr,
+        k_resid_dual<<<nbr, 256, 0, s->stream>>>(f.V, f.x, f.s, f.vc, gath_rp(s, f), gath_cf(s, f), gath_cf2(s, f), s->dcol, kVertPoint, nullptr,
                                                  s->vpart, f.cmask, lsh);
         HIPCHK(hipGetLastError());
         k_check2<<<1, 256, 0, s->stream>>>(nbr, s->vpart, s->vstate, s->hmirror, s->seqc, judge);
         s->chk_launched++;
         HIPCHK(hipGetLastError());
-        k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, f.rp, f.cf2, f.cf);
+        k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, gath_rp(s, f),
+                                             gath_cf2(s, f), gath_cf(s, f));
         HIPCHK(hipGetLastError());
         return MGSB_OK;
     }
@@ -2612,7 +2746,8 @@ int enqueue_vc_start(mgsb_solver* s, bool learned) {
                                                     ep ? s->M[0].iids : nullptr, ep ? s->M[0].iw : nullptr);
         HIPCHK(hipGetLastError());
         if ((rc = fine_residual(s, true, false, judge, true, skip))) return rc;
-        k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, f.rp, nullptr, f.cf);
+        k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, gath_rp(s, f),
+                                             nullptr, gath_cf(s, f));
         HIPCHK(hipGetLastError());
         // after a revert the corner forces belong to the rejected x: recompute
         if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, &s->vstate->cf_ok, true))) return rc;
@@ -2665,24 +2800,23 @@ int enqueue_vc_post(mgsb_solver* s) {
 }
 
 // Coarse correction achieved (features of the next step / training target;
-// left out when lite: no learned stage reads it next step), x = s + u,
-// velocities, run aggregates.
-int enqueue_vc_finish(mgsb_solver* s, bool learned, bool lite) {
+// x = s + u, velocities, the delta history of the learned features, run
+// aggregates; then the training target when the step is recorded (left out
+// when lite).
+int enqueue_vc_finish(mgsb_solver* s, bool lite) {
     DevLevel& f = s->L[0];
     DevLevel& c1 = s->L[1];
     const int bs = block_vert(s);
-    const bool features = learned || s->recording;
-    float* recT = s->recording ? s->recT + (size_t)s->rec_steps * c1.V * 3 : nullptr;
-    if (!lite && (features || temporal_warm(s))) {  // coarse delta: next step's feature / training target
-        const bool t = temporal_warm(s);
-        k_target<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, f.x, t ? f.U1 : nullptr, t ? f.U2 : nullptr, f.vc,
-                                                       s->M[0].aids, s->M[0].aw, s->eprev, s->eprev2, recT,
+    k_finish_split<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, f.s, f.xlast, f.v, s->p.dt, s->vstate,
+                                                        f.U1, f.U2, f.vc, s->D1, s->D2, temporal_warm(s) ? 1 : 0,
+                                                        &s->vstate->notdone);
+    HIPCHK(hipGetLastError());
+    if (!lite && s->recording) {
+        k_target<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, s->D1, s->D2, s->M[0].aids, s->M[0].aw,
+                                                       s->recT + (size_t)s->rec_steps * c1.V * 3,
                                                        &s->vstate->notdone);
         HIPCHK(hipGetLastError());
     }
-    k_finish_split<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, f.s, f.xlast, f.v, s->p.dt, s->vstate,
-                                                        f.U1, f.U2, &s->vstate->notdone);
-    HIPCHK(hipGetLastError());
     return MGSB_OK;
 }
 
@@ -2691,10 +2825,10 @@ int enqueue_piece(mgsb_solver* s, int piece) {
     switch (piece) {
         case kPieceStart: return enqueue_vc_start(s, learned);
         case kPieceStartGated: return enqueue_vc_start(s, false);
-        case kPieceFinishLite: return enqueue_vc_finish(s, learned, true);
+        case kPieceFinishLite: return enqueue_vc_finish(s, true);
         case kPiecePre: return enqueue_vc_pre(s);
         case kPiecePost: return enqueue_vc_post(s);
-        default: return enqueue_vc_finish(s, learned, false);
+        default: return enqueue_vc_finish(s, false);
     }
 }
 
@@ -2757,9 +2891,8 @@ int precapture_pieces(mgsb_solver* s) {
     const int maxh = 2 * s->p.max_cycles;
     const std::string starts = learned ? std::string{(char)kPieceStart, (char)kPieceStartGated}
                                        : std::string{(char)kPieceStart};
-    // the learned schedule (or a recording, which runs eagerly) takes the
-    // full finish, the others the lite one (fin_for)
-    const std::string fins(1, (char)(learned ? kPieceFinish : kPieceFinishLite));
+    // only a recording (which runs eagerly) takes the full finish (fin_for)
+    const std::string fins(1, (char)kPieceFinishLite);
     auto halves = [](char st, int h) {
         std::string q(1, st);
         for (int k = 0; k < h; k++) q.push_back((char)((k & 1) ? kPiecePost : kPiecePre));
@@ -2847,10 +2980,9 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
     std::string seq(1, (char)(gated ? kPieceStartGated : kPieceStart));
     const int check = s->p.check_every;
     const int maxh = 2 * s->p.max_cycles;
-    // the learned stage reads the coarse deltas of the last two steps: the
-    // learned schedule (and a recording) keeps them every step, the others
-    // take the lite finish
-    auto fin_for = [&](int) { return (s->recording || learned) ? kPieceFinish : kPieceFinishLite; };
+    // every finish keeps the delta history of the learned features; only a
+    // recorded step also writes its training target
+    auto fin_for = [&](int) { return s->recording ? kPieceFinish : kPieceFinishLite; };
     int h = 0;
     int target = check ? std::min(maxh, std::max(0, s->halves_guess)) : maxh;
     bool finished = false;
@@ -3065,7 +3197,7 @@ int setup_vcycle(mgsb_solver* s) {
         HIPCHK(hipStreamSynchronize(s->stream));
         // the tail as a dense matrix, three columns per run on unit vectors
         DevLevel& t0 = s->L[s->tail_from];
-        if (s->tail_from < nl - 1 && t0.V <= kTailDenseMaxV) {
+        if (s->p.coarse_dense && s->tail_from < nl - 1 && t0.V <= kTailDenseMaxV) {
             if (!t0.tailB) {
                 int rc = dalloc(&t0.tailB, (size_t)t0.V * t0.V);
                 if (rc) return rc;
@@ -3098,6 +3230,7 @@ int alloc_vcycle(mgsb_solver* s) {
             std::vector<float> one(d.V, 1.f);
             if ((rc = up(s, d.owned, &d.cmask, one.data(), d.V))) return rc;
             if ((rc = up<float3>(s, d.owned, &d.cf2, nullptr, (size_t)d.T * 4))) return rc;
+            if ((rc = up<float3>(s, d.owned, &d.pcf2, nullptr, (size_t)d.npart))) return rc;
             continue;
         }
         if ((rc = up<float4>(s, d.owned, &d.e, nullptr, d.V))) return rc;
@@ -3123,9 +3256,10 @@ int alloc_vcycle(mgsb_solver* s) {
         std::vector<float4> z(c1.V, make_float4(0.f, 0.f, 0.f, 0.f)), x1(c1.V);
         for (int i = 0; i < c1.V; i++) x1[i] = make_float4(c1.h.X[i * 3], c1.h.X[i * 3 + 1], c1.h.X[i * 3 + 2], 0.f);
         if ((rc = up(s, s->owned, &s->epred, z.data(), c1.V))) return rc;
-        if ((rc = up(s, s->owned, &s->eprev, z.data(), c1.V))) return rc;
-        if ((rc = up(s, s->owned, &s->eprev2, z.data(), c1.V))) return rc;
         if ((rc = up(s, s->owned, &s->X1, x1.data(), c1.V))) return rc;
+        std::vector<float4> zf(s->L[0].V, make_float4(0.f, 0.f, 0.f, 0.f));
+        if ((rc = up(s, s->owned, &s->D1, zf.data(), zf.size()))) return rc;
+        if ((rc = up(s, s->owned, &s->D2, zf.data(), zf.size()))) return rc;
         if ((rc = mlp_create(&s->mlp, kFeat, 3, 1234u, s->stream))) return rc;
     }
     if ((rc = dalloc(&s->vstate, 1))) return rc;
@@ -3170,7 +3304,7 @@ int residual_norm(mgsb_solver* s, double* out) {
     if (rc) return rc;
     const int lsh = vert_lsh(s), nb = nblk(d.V << lsh, 256);
     if ((rc = ensure_partials(s, (size_t)nb * 4))) return rc;
-    k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, d.rp, d.cf, s->dcol, kVertPoint, nullptr, s->dpart,
+    k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, gath_rp(s, d), gath_cf(s, d), s->dcol, kVertPoint, nullptr, s->dpart,
                                        nullptr, nullptr, 0, nullptr, nullptr, lsh);
     HIPCHK(hipGetLastError());
     double sq = 0;
@@ -3554,9 +3688,9 @@ int mgsb_set_positions(mgsb_solver* s, int32_t level, const float* x) {
         HIPCHK(hipMemsetAsync(s->L[0].U1, 0, sizeof(float4) * s->L[0].V, s->stream));
         HIPCHK(hipMemsetAsync(s->L[0].U2, 0, sizeof(float4) * s->L[0].V, s->stream));
     }
-    if (level == 0 && s->eprev) {
-        HIPCHK(hipMemsetAsync(s->eprev, 0, sizeof(float4) * s->L[1].V, s->stream));
-        HIPCHK(hipMemsetAsync(s->eprev2, 0, sizeof(float4) * s->L[1].V, s->stream));
+    if (level == 0 && s->D1) {
+        HIPCHK(hipMemsetAsync(s->D1, 0, sizeof(float4) * s->L[0].V, s->stream));
+        HIPCHK(hipMemsetAsync(s->D2, 0, sizeof(float4) * s->L[0].V, s->stream));
     }
     return write_f4(s, s->L[level].x, s->L[level].V, x);
 }
@@ -4281,8 +4415,10 @@ int snap_take(mgsb_solver* s, StateSnap& S) {
         S.where.push_back({f.U1, (size_t)f.V});
         S.where.push_back({f.U2, (size_t)f.V});
     }
-    if (s->eprev) S.where.push_back({s->eprev, (size_t)s->L[1].V});
-    if (s->eprev2) S.where.push_back({s->eprev2, (size_t)s->L[1].V});
+    if (s->D1) {
+        S.where.push_back({s->D1, (size_t)f.V});
+        S.where.push_back({s->D2, (size_t)f.V});
+    }
     S.bufs.resize(S.where.size());
     for (size_t i = 0; i < S.where.size(); i++) {
         S.bufs[i].resize(S.where[i].second);
@@ -4510,7 +4646,7 @@ int mgsb_tet_forces(mgsb_solver* s, int32_t level, int32_t if_mode, const float*
     const bool was = s->timing;
     s->timing = false;
     rc = write_f4(s, xd, d.V, x);
-    if (!rc) rc = launch_tet(s, d, mode, false, xd);
+    if (!rc) rc = launch_tet(s, d, mode, false, xd, nullptr, false, true);
     s->timing = was;
     std::vector<float3> h((size_t)d.T * 4);
     hipError_t e = hipSuccess;

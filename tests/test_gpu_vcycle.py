@@ -111,6 +111,33 @@ def test_vcycle_adaptive_meets_target_each_step(check):
     assert run["total_cycles"] == sum(c for c, _ in gc)
 
 
+@pytest.mark.parametrize("ns", [(8, 4, 1), (12, 4, 1)])
+def test_vcycle_dense_tail_matches_sequential_tail(ns):
+    """coarse_dense = 1 applies the coarse levels below level 1 (sweeps,
+    restriction, the direct coarsest solve, prolongation, sweeps) as one
+    dense matrix formed at set-up; coarse_dense = 0 runs them in sequence.
+    The map is linear, so the two agree but for FP32 rounding: the same
+    half-cycle counts, positions within 1e-5 of each other, and the same
+    trajectory as the oracle; bitwise equality would mean the dense path
+    never ran."""
+    kw = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, check_every=2, tol=1e-3, rho_smooth=0.9)
+    runs = []
+    for dense in (1, 0):
+        lv, s, ML, Ls, p = make_pair(ns, None, coarse_dense=dense, **kw)
+        hs = [s.step(1).halves for _ in range(40)]
+        runs.append((s.positions(), hs, s.run_stats()))
+    (g1, h1, r1), (g0, h0, r0) = runs
+    vp = O.vparams(pre=5, post=5, coarse_sweeps=2, max_cycles=12, check=2, tol=1e-3, rho_smooth=0.9)
+    ML.step(p, vp, 40)
+    err, erro = rel_err(g1, g0), rel_err(g1, Ls[0].positions())
+    agree = sum(a == b for a, b in zip(h1, h0))
+    print(f"[dense tail {ns}] rel(dense, sequential)={err:.2e} rel(dense, oracle)={erro:.2e} halves agree {agree}/40")
+    assert not np.array_equal(g1, g0)
+    assert err < 1e-5 and erro < REL_TOL
+    assert agree >= 38
+    assert r1["failed_steps"] == 0 and r0["failed_steps"] == 0
+
+
 @pytest.mark.parametrize("check", [1, 2])
 def test_vcycle_graph_replay_matches_eager(check):
     """Captured pieces (start, half-cycles, finish) replayed by the host's

@@ -137,3 +137,66 @@ def test_nested_box_bindings():
     assert np.allclose(w.sum(1), 1, atol=1e-5)
     rec = (w[:, :, None] * c.pos[ids].astype(np.float64)).sum(1)
     assert np.allclose(rec, f.pos, atol=1e-5)
+
+
+def _tile_layout_checks(mesh):
+    """The tiled local step's layout against its definition: tiles of 256
+    consecutive tets; each tile's vertices are the distinct corners of its
+    tets; every (tile, vertex) partial has one slot, the slots of a vertex
+    are contiguous in ascending tile order; a tile's corner array holds each
+    of its corners once, in the row of the corner's vertex, in (tet, corner)
+    order, within the 1024 entries of the kernel's LDS arrays; and summing
+    corner values through the layout gives every vertex the sum over its
+    incident corners."""
+    L = mesh.tiles()
+    tets = np.asarray(mesh.tets).reshape(-1, 4)
+    T, V = tets.shape[0], mesh.nverts
+    prp, tvo, tvs, tlr, pos = L["prp"], L["tvo"], L["tvs"], L["tlr"], L["pos"]
+    nt = (T + 255) // 256
+    npart = int(tvo[-1])
+    assert len(tvo) == nt + 1 and tvo[0] == 0 and np.all(np.diff(tvo) > 0)
+    assert prp[0] == 0 and prp[-1] == npart and np.all(np.diff(prp) >= 0)
+    assert np.array_equal(np.sort(tvs), np.arange(npart))
+    assert pos.min() >= 0 and pos.max() < 1024
+    owner = np.searchsorted(prp, tvs, side="right") - 1  # vertex of each partial slot
+    tile_of_slot = np.empty(npart, np.int64)
+    rng = np.random.default_rng(0)
+    f = rng.standard_normal((T, 4))
+    part = np.zeros(npart)
+    for k in range(nt):
+        t0, t1 = 256 * k, min(T, 256 * k + 256)
+        vk = owner[tvo[k]:tvo[k + 1]]
+        assert np.array_equal(vk, np.unique(tets[t0:t1])), k
+        tile_of_slot[tvs[tvo[k]:tvo[k + 1]]] = k
+        rows = tlr[tvo[k] + k:tvo[k + 1] + k + 1]
+        assert rows[0] == 0 and rows[-1] == 4 * (t1 - t0) and np.all(np.diff(rows) > 0)
+        p = pos[t0:t1].ravel()
+        assert np.array_equal(np.sort(p), np.arange(4 * (t1 - t0)))
+        loc = np.searchsorted(vk, tets[t0:t1].ravel())
+        assert np.all(rows[loc] <= p) and np.all(p < rows[loc + 1])
+        order = np.argsort(p)
+        assert np.all(np.diff(loc[order]) >= 0)
+        for l in range(len(vk)):  # (tet, corner) order inside each row
+            assert np.all(np.diff(order[rows[l]:rows[l + 1]]) > 0)
+        lds = np.zeros(4 * (t1 - t0))
+        lds[p] = f[t0:t1].ravel()
+        part[tvs[tvo[k]:tvo[k + 1]]] = np.add.reduceat(lds, rows[:-1])
+    for v in range(0, V, max(1, V // 200)):
+        assert np.all(np.diff(tile_of_slot[prp[v]:prp[v + 1]]) > 0)
+    want = np.zeros(V)
+    np.add.at(want, tets.ravel(), f.ravel())
+    got = np.add.reduceat(np.append(part, 0.0), prp[:-1]) * (np.diff(prp) > 0)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+    return npart
+
+
+@pytest.mark.parametrize("which", ["box4", "box12", "liver", "blob"])
+def test_tile_layout(golden, which):
+    from multigridsoftbody_amd import workloads as W
+
+    mesh = {"box4": lambda: W.beam(4), "box12": lambda: W.beam(12),
+            "liver": lambda: Mesh.load_msh(os.path.join(golden, "liver700_3500.msh")),
+            "blob": lambda: Mesh.blob(h=0.45)}[which]()
+    npart = _tile_layout_checks(mesh)
+    print(f"[tiles {which}] {mesh.ntets} tets, {npart} partials: {npart / mesh.nverts:.2f} per vertex, "
+          f"{4 * mesh.ntets / npart:.1f} corners per partial")
