@@ -45,10 +45,10 @@ VCYCLE = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, tol=1
 # velocities (seeds 1..32), where the extrapolation's error is largest, plus
 # 600 steps of one more (seed 40)
 TRAIN = dict(episodes=[(sd, 60) for sd in range(1, 33)] + [(40, 600)], epochs=30, batch=4096, lr=1e-3, seed=0)
-# HBM bytes per k_tet launch on the medium box (V-cycle split-position mode):
-# rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB copy),
-# profiles/k_tet_learned_ab_medium_box.md
-TRAFFIC = {"medium_box_12x120x12": 12.90e6}
+# HBM bytes per launch of the tiled local step on the medium box (V-cycle
+# split-position mode): rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
+# (scales from a 256 MiB copy), profiles/k_tet_tiled_medium_large_batched.md
+TRAFFIC = {"medium_box_12x120x12": 7.68e6}
 
 
 def log(*a):
@@ -331,8 +331,7 @@ def main():
             f"{[m.ntets for m in levels]}")
         vc = name in ("vcycle", "temporal", "learned", "learned_s")
         tuned = None
-        if args.autotune:
-            s.step(args.warmup)
+        if args.autotune:  # from the initial state, which it restores: the same timed window as untuned
             tuned = s.autotune(args.autotune)
             log(f"[bench] {name}: autotune {tuned} (not timed)")
         elapsed, run = timed_run(s, args.steps, args.warmup, vc)

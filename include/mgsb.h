@@ -363,18 +363,27 @@ int mgsb_level_rho(mgsb_solver* s, int32_t level, double* rho);
 int mgsb_coarse_nfree(mgsb_solver* s, int32_t* nfree);
 
 /* Autotuner (no reference counterpart; the reference hard-codes its launch
- * shapes).  Times trial_steps steps per candidate from a snapshot of the
- * solver state, restores the state, and keeps the fastest candidate whose
- * steps all met the residual target: pre/post sweeps in {3,4,5,6} (V-cycle
- * schedules), then block_tet and block_vert in {64,128,256}, then vert_lanes
- * in {1,2,4,8}, one coordinate at a time.  The chosen values are written into
- * the solver's parameters. */
+ * shapes).  Works on a snapshot of the solver state and restores it.
+ * 1. Launch shapes by kernel time: 20 fine-level sweeps (local step +
+ *    vertex update, as the schedule runs them) between HIP events for the
+ *    current shape, the tiled layout with block_vert in {64,128,256} and
+ *    vert_lanes in {1,2}, and the per-corner layout with block_tet and
+ *    block_vert in {64,128,256} and vert_lanes in {2,4,8}; the fastest is
+ *    kept if it beats the current shape by 2 %.
+ * 2. On V-cycle schedules, pre/post sweeps in {3,4,5,6} by steps per second
+ *    over trial_steps steps that start trial_steps steps after the saved
+ *    state (best of two runs each); a count is kept only if every one of its
+ *    steps met the residual target and it beat the current one by 3 %.
+ * The chosen values are written into the solver's parameters. */
 typedef struct mgsb_tune_result {
     int32_t block_tet, block_vert, pre_sweeps, post_sweeps;
-    double ms_per_step;          /* the chosen candidate                       */
-    double ms_per_step_default;  /* the parameters before tuning               */
+    double ms_per_step;          /* the chosen parameters, over the window      */
+    double ms_per_step_default;  /* the parameters before tuning, same window   */
     int32_t trials;
     int32_t vert_lanes;
+    int32_t tile_gather;
+    double sweep_us;             /* one fine sweep, chosen shape               */
+    double sweep_us_default;     /* ... and the shape before tuning            */
 } mgsb_tune_result;
 int mgsb_autotune(mgsb_solver* s, int32_t trial_steps, mgsb_tune_result* out);
 

@@ -98,7 +98,8 @@ class mgsb_run_stats_t(C.Structure):
 class mgsb_tune_result(C.Structure):
     _fields_ = [("block_tet", C.c_int32), ("block_vert", C.c_int32), ("pre_sweeps", C.c_int32),
                 ("post_sweeps", C.c_int32), ("ms_per_step", C.c_double), ("ms_per_step_default", C.c_double),
-                ("trials", C.c_int32), ("vert_lanes", C.c_int32)]
+                ("trials", C.c_int32), ("vert_lanes", C.c_int32), ("tile_gather", C.c_int32),
+                ("sweep_us", C.c_double), ("sweep_us_default", C.c_double)]
 
 
 class mgsb_energy_t(C.Structure):
@@ -581,10 +582,12 @@ class Solver:
         return ms.value, n.value
 
     def autotune(self, trial_steps=20):
-        """Pick sweeps and block sizes by timing trial steps (state restored)."""
+        """Launch shapes by kernel time, then sweeps over a window of steps
+        (mgsb_autotune; the solver's state is restored)."""
         r = mgsb_tune_result()
         _check(lib.mgsb_autotune(self._h, trial_steps, C.byref(r)), "mgsb_autotune")
-        for k in ("block_tet", "block_vert", "pre_sweeps", "post_sweeps", "vert_lanes"):  # keep the host copy in step
+        for k in ("block_tet", "block_vert", "pre_sweeps", "post_sweeps", "vert_lanes",
+                  "tile_gather"):  # keep the host copy in step
             setattr(self.params, k, getattr(r, k))
         return {k: getattr(r, k) for k, _ in r._fields_}
 

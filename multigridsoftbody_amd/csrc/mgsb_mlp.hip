@@ -208,12 +208,23 @@ __global__ void __launch_bounds__(256) k_gather_rows(int nb, const int* __restri
 }
 
 // BF16 copies of the weights, zero-padded to multiples of 16, for inference.
+// BF16 copy of a K x N weight (row-major FP32), zero-padded to Kp x Np and
+// stored in the order the 16x16x16 MFMA reads its B operand: fragment
+// (tile t of 16 columns, K block kb) is 64 lanes x 4 consecutive values,
+// lane l holding W[kb*16 + (l/16)*4 + q][t*16 + l%16], q = 0..3, so a
+// wave loads one fragment as one coalesced 512-byte read.
 __global__ void __launch_bounds__(256) k_pack_bf16(int K, int N, int Kp, int Np, const float* __restrict__ W,
                                                    short* __restrict__ Wb) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Kp * Np) return;
-    const int k = i / Np, n = i % Np;
+    const int nkb = Kp / 16;
+    const int frag = i >> 8, lane = (i >> 2) & 63, q = i & 3;
+    const int t = frag / nkb, kb = frag % nkb;
+    const int k = kb * 16 + (lane >> 4) * 4 + q, n = t * 16 + (lane & 15);
     Wb[i] = f2bf((k < K && n < N) ? W[k * N + n] : 0.f);
+}
+__device__ __forceinline__ bf16x4 frag_b(const short* __restrict__ Wb, int nkb, int t, int kb, int lane) {
+    return *reinterpret_cast<const bf16x4*>(Wb + ((size_t)(t * nkb + kb) * 64 + lane) * 4);
 }
 
 // Fused inference: one workgroup of four waves per 16 samples through all
@@ -221,7 +232,9 @@ __global__ void __launch_bounds__(256) k_pack_bf16(int K, int N, int Kp, int Np,
 // output tiles; the layer's activations stay in LDS as BF16 and a barrier
 // separates the layers.  Wave 0 computes the output layer (one tile).  Every
 // tile accumulates its K blocks in order, as a one-wave-per-tile product
-// would.  Y = mlp(X).
+// would.  All of a wave's weight fragments (k_pack_bf16's order) are loaded
+// at the start, in one round trip, instead of one dependent load per
+// product.  Y = mlp(X).
 constexpr int kH = MLP_HIDDEN;
 
 // Row-major FP32 inputs X (n x in) and outputs Y (row stride ldy).
@@ -248,11 +261,27 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int nout, In in, Out o
                                                    const float* __restrict__ b2, const float* __restrict__ b3,
                                                    const float* __restrict__ b4, const int* __restrict__ skip) {
     if (skip && *skip) return;
-    __shared__ short hs[2][16][kH];
+    __shared__ __align__(16) short hs[2][16][kH];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int base = blockIdx.x * 16;
     const int r = lane & 15, kq = (lane >> 4) * 4;
     constexpr int kTilesPerWave = kH / 16 / 4;
+    constexpr int kKb = kH / 16;
+    // every weight fragment this wave multiplies by, issued up front
+    bf16x4 w1[kTilesPerWave], w23[2][kTilesPerWave][kKb], w4[kKb];
+#pragma unroll
+    for (int tt = 0; tt < kTilesPerWave; tt++) {
+        const int t = wave * kTilesPerWave + tt;
+        w1[tt] = frag_b(W1, 1, t, 0, lane);
+#pragma unroll
+        for (int kb = 0; kb < kKb; kb++) {
+            w23[0][tt][kb] = frag_b(W2, kKb, t, kb, lane);
+            w23[1][tt][kb] = frag_b(W3, kKb, t, kb, lane);
+        }
+    }
+    if (wave == 0)
+#pragma unroll
+        for (int kb = 0; kb < kKb; kb++) w4[kb] = frag_b(W4, kKb, 0, kb, lane);
     // layer 1: K = 16 (inputs zero-padded)
     {
         bf16x4 a;
@@ -260,13 +289,11 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int nout, In in, Out o
         in.load4(base + r, kq, x4);
 #pragma unroll
         for (int q = 0; q < 4; q++) a[q] = f2bf(x4[q]);
+#pragma unroll
         for (int tt = 0; tt < kTilesPerWave; tt++) {
             const int t = wave * kTilesPerWave + tt;
-            bf16x4 b;
-#pragma unroll
-            for (int q = 0; q < 4; q++) b[q] = W1[(kq + q) * kH + t * 16 + r];
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w1[tt], acc, 0, 0, 0);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const float v = acc[q] + b1[t * 16 + r];
@@ -276,21 +303,18 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int nout, In in, Out o
     }
     __syncthreads();
     // layers 2 and 3: K = H
-    const short* Ws[2] = {W2, W3};
     const float* bs[2] = {b2, b3};
+#pragma unroll
     for (int L = 0; L < 2; L++) {
         const int src = L & 1, dst = src ^ 1;
+#pragma unroll
         for (int tt = 0; tt < kTilesPerWave; tt++) {
             const int t = wave * kTilesPerWave + tt;
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            for (int kb = 0; kb < kH / 16; kb++) {
-                bf16x4 a, b;
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    a[q] = hs[src][r][kb * 16 + kq + q];
-                    b[q] = Ws[L][(kb * 16 + kq + q) * kH + t * 16 + r];
-                }
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            for (int kb = 0; kb < kKb; kb++) {
+                const bf16x4 a = *reinterpret_cast<const bf16x4*>(&hs[src][r][kb * 16 + kq]);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w23[L][tt][kb], acc, 0, 0, 0);
             }
 #pragma unroll
             for (int q = 0; q < 4; q++) {
@@ -303,14 +327,10 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int nout, In in, Out o
     // layer 4: N = 16 (outputs zero-padded); source buffer is 0 after 2 layers
     if (wave != 0) return;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int kb = 0; kb < kH / 16; kb++) {
-        bf16x4 a, b;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            a[q] = hs[0][r][kb * 16 + kq + q];
-            b[q] = W4[(kb * 16 + kq + q) * 16 + r];
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    for (int kb = 0; kb < kKb; kb++) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(&hs[0][r][kb * 16 + kq]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w4[kb], acc, 0, 0, 0);
     }
 #pragma unroll
     for (int q = 0; q < 4; q++)

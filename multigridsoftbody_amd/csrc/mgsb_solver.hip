@@ -309,14 +309,13 @@ __device__ __forceinline__ bool tri_sphere_hit(float3 A, float3 B, float3 C, flo
 // |f| < 1e-5 drop in Ac mode, PDSolverAc.cu:630-662).  Corner j's force goes
 // to fo[3j..3j+2].  sref != nullptr: split positions (x holds u, see below).
 template <int MODE>
-__device__ __forceinline__ void tet_local_forces(int g, int T, const float4* __restrict__ x,
-                                                 const int4* __restrict__ tet, const float* __restrict__ B,
-                                                 const float* __restrict__ vol, float k,
-                                                 const float4* __restrict__ sref, const float* __restrict__ tk,
-                                                 float (&fo)[12]) {
-    // every load is issued up front: the per-tet streams and the index in the
-    // first round trip, the corner gathers of x (and s) in the second
-    const int4 id = tet[g];
+__device__ __forceinline__ void tet_local_forces(int g, int T, int4 id, const float4* __restrict__ x,
+                                                 const float* __restrict__ B, const float* __restrict__ vol,
+                                                 float k, const float4* __restrict__ sref,
+                                                 const float* __restrict__ tk, float (&fo)[12]) {
+    // every load is issued up front: the per-tet streams and the index (id,
+    // loaded by the caller) in the first round trip, the corner gathers of x
+    // (and s) in the second
     float b[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) b[q] = B[(size_t)q * T + g];
@@ -392,9 +391,10 @@ __device__ __forceinline__ void tet_local_step(int g, int T, const float4* __res
                                                const float* __restrict__ vol, const int4* __restrict__ slot,
                                                float3* __restrict__ cf, float k, const float4* __restrict__ sref,
                                                const float* __restrict__ tk) {
+    const int4 id = tet[g];
     const int4 sl = slot[g];
     float fo[12];
-    tet_local_forces<MODE>(g, T, x, tet, B, vol, k, sref, tk, fo);
+    tet_local_forces<MODE>(g, T, id, x, B, vol, k, sref, tk, fo);
     cf[sl.x] = make_float3(fo[0], fo[1], fo[2]);
     cf[sl.y] = make_float3(fo[3], fo[4], fo[5]);
     cf[sl.z] = make_float3(fo[6], fo[7], fo[8]);
@@ -408,12 +408,22 @@ __device__ __forceinline__ void tet_local_step(int g, int T, const float4* __res
 // (tile, vertex) order (tvs).  The vertex kernels gather those partials, a
 // row of ~2 per vertex instead of ~20 corners, and the tet kernel's scattered
 // stores shrink by the same factor.  Deterministic: fixed sums, no atomics.
+// The tiled kernels read each tet as one int4 whose entries carry the corner's
+// vertex id in the low 22 bits and its position in the tile's corner array
+// in the high 10 (levels of at most 2^22 vertices run tiled).
 struct TileLds {
     float x[4 * kTileTets], y[4 * kTileTets], z[4 * kTileTets];
 };
+constexpr int kTileIdBits = 22;
+constexpr unsigned kTileIdMask = (1u << kTileIdBits) - 1u;
+__device__ __forceinline__ int4 tile_ids(int4 q) {
+    return make_int4((int)((unsigned)q.x & kTileIdMask), (int)((unsigned)q.y & kTileIdMask),
+                     (int)((unsigned)q.z & kTileIdMask), (int)((unsigned)q.w & kTileIdMask));
+}
 
-__device__ __forceinline__ void tile_put(TileLds& L, uint2 ls, const float (&fo)[12]) {
-    const int p[4] = {(int)(ls.x & 0xffffu), (int)(ls.x >> 16), (int)(ls.y & 0xffffu), (int)(ls.y >> 16)};
+__device__ __forceinline__ void tile_put(TileLds& L, int4 q, const float (&fo)[12]) {
+    const int p[4] = {(int)((unsigned)q.x >> kTileIdBits), (int)((unsigned)q.y >> kTileIdBits),
+                      (int)((unsigned)q.z >> kTileIdBits), (int)((unsigned)q.w >> kTileIdBits)};
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         L.x[p[j]] = fo[j * 3 + 0];
@@ -493,9 +503,9 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
 // one tile each (kTileTets lanes), the ones after them the triangle lanes.
 template <int MODE, bool TRI>
 __global__ void __launch_bounds__(kTileTets) k_tet_tiled(
-    int T, int ntiles, const float4* __restrict__ x, const int4* __restrict__ tet, const float* __restrict__ B,
-    const float* __restrict__ vol, const uint2* __restrict__ tls, const int* __restrict__ tvo,
-    const int* __restrict__ tvs, const int* __restrict__ tlr, float3* __restrict__ pcf, float k, int ntri,
+    int T, int ntiles, const float4* __restrict__ x, const int4* __restrict__ tetp, const float* __restrict__ B,
+    const float* __restrict__ vol, const int* __restrict__ tvo, const int* __restrict__ tvs,
+    const int* __restrict__ tlr, float3* __restrict__ pcf, float k, int ntri,
     const int4* __restrict__ tri, const int4* __restrict__ tslot, float4* __restrict__ tcf,
     const DevColliders* __restrict__ col, const int* __restrict__ skip, const float4* __restrict__ sref,
     const float* __restrict__ tk) {
@@ -508,10 +518,10 @@ __global__ void __launch_bounds__(kTileTets) k_tet_tiled(
     __shared__ TileLds L;
     const int g = blk * kTileTets + threadIdx.x;
     if (g < T) {
-        const uint2 ls = tls[g];
+        const int4 q = tetp[g];
         float fo[12];
-        tet_local_forces<MODE>(g, T, x, tet, B, vol, k, sref, tk, fo);
-        tile_put(L, ls, fo);
+        tet_local_forces<MODE>(g, T, tile_ids(q), x, B, vol, k, sref, tk, fo);
+        tile_put(L, q, fo);
     }
     __syncthreads();
     tile_sum(L, blk, tvo, tvs, tlr, pcf);
@@ -539,9 +549,9 @@ __global__ void __launch_bounds__(256) k_tet_dual(int T, const float4* __restric
 // (pcf2).
 template <int MODE>
 __global__ void __launch_bounds__(kTileTets) k_tet_dual_tiled(
-    int T, int ntiles, const float4* __restrict__ x, const float4* __restrict__ s, const int4* __restrict__ tet,
-    const float* __restrict__ B, const float* __restrict__ vol, const uint2* __restrict__ tls,
-    const int* __restrict__ tvo, const int* __restrict__ tvs, const int* __restrict__ tlr, float3* __restrict__ pcf,
+    int T, int ntiles, const float4* __restrict__ x, const float4* __restrict__ s, const int4* __restrict__ tetp,
+    const float* __restrict__ B, const float* __restrict__ vol, const int* __restrict__ tvo,
+    const int* __restrict__ tvs, const int* __restrict__ tlr, float3* __restrict__ pcf,
     float3* __restrict__ pcf2, float k, const float* __restrict__ tk) {
     const int b0 = xcd_block(blockIdx.x, gridDim.x);
     const bool at_s = b0 >= ntiles;
@@ -549,10 +559,10 @@ __global__ void __launch_bounds__(kTileTets) k_tet_dual_tiled(
     __shared__ TileLds L;
     const int g = blk * kTileTets + threadIdx.x;
     if (g < T) {
-        const uint2 ls = tls[g];
+        const int4 q = tetp[g];
         float fo[12];
-        tet_local_forces<MODE>(g, T, at_s ? s : x, tet, B, vol, k, at_s ? nullptr : s, tk, fo);
-        tile_put(L, ls, fo);
+        tet_local_forces<MODE>(g, T, tile_ids(q), at_s ? s : x, B, vol, k, at_s ? nullptr : s, tk, fo);
+        tile_put(L, q, fo);
     }
     __syncthreads();
     tile_sum(L, blk, tvo, tvs, tlr, at_s ? pcf2 : pcf);
@@ -2092,7 +2102,7 @@ struct DevLevel {
     // tiled local step (HostLevel::prp ...): per-(tile, vertex) partials
     int ntiles = 0, npart = 0;
     int *prp = nullptr, *tvo = nullptr, *tvs = nullptr, *tlr = nullptr;
-    uint2* tls = nullptr;
+    int4* tetp = nullptr;  // tets with each corner's position in its tile's corner array (nullptr: not tiled)
     float3 *pcf = nullptr, *pcf2 = nullptr;
     int4 *tri = nullptr, *tslot = nullptr;
     int *trp = nullptr, *trid = nullptr;
@@ -2305,8 +2315,19 @@ int upload_level(mgsb_solver* s, DevLevel& d) {
     if ((rc = up(s, d.owned, &d.tvo, h.tvo.data(), h.tvo.size()))) return rc;
     if ((rc = up(s, d.owned, &d.tvs, h.tvs.data(), h.tvs.size()))) return rc;
     if ((rc = up(s, d.owned, &d.tlr, h.tlr.data(), h.tlr.size()))) return rc;
-    if ((rc = up(s, d.owned, &d.tls, reinterpret_cast<const uint2*>(h.tls.data()), (size_t)h.T))) return rc;
     if ((rc = up<float3>(s, d.owned, &d.pcf, nullptr, (size_t)d.npart))) return rc;
+    if (h.V <= (1 << 22)) {
+        std::vector<int4> tp(h.T);
+        for (int t = 0; t < h.T; t++) {
+            int c[4];
+            for (int j = 0; j < 4; j++) {
+                const uint32_t pos = (h.tls[(size_t)t * 2 + (j >> 1)] >> (16 * (j & 1))) & 0xffffu;
+                c[j] = (int)((uint32_t)h.tets[(size_t)t * 4 + j] | (pos << 22));
+            }
+            tp[t] = make_int4(c[0], c[1], c[2], c[3]);
+        }
+        if ((rc = up(s, d.owned, &d.tetp, tp.data(), tp.size()))) return rc;
+    }
     std::vector<int4> tr4(std::max(1, h.S)), ts4(std::max(1, h.S));
     for (int t = 0; t < h.S; t++) {
         tr4[t] = make_int4((int)h.tris[t * 3], (int)h.tris[t * 3 + 1], (int)h.tris[t * 3 + 2], 0);
@@ -2394,7 +2415,7 @@ void launch_tet_mode(mgsb_solver* s, DevLevel& d, int nb, int bs, const float4* 
     if (tiled) {
         auto k = ntri ? k_tet_tiled<MODE, true> : k_tet_tiled<MODE, false>;
         k<<<d.ntiles + nblk(ntri, kTileTets), kTileTets, 0, s->stream>>>(
-            d.T, d.ntiles, xx, d.tet, d.B, d.vol, d.tls, d.tvo, d.tvs, d.tlr, d.pcf, s->p.stiffness, ntri, d.tri,
+            d.T, d.ntiles, xx, d.tetp, d.B, d.vol, d.tvo, d.tvs, d.tlr, d.pcf, s->p.stiffness, ntri, d.tri,
             d.tslot, d.tcf, s->dcol, skip, sref, d.tk);
         return;
     }
@@ -2405,10 +2426,10 @@ void launch_tet_mode(mgsb_solver* s, DevLevel& d, int nb, int bs, const float4* 
 
 // The per-vertex gathers read the (tile, vertex) partials of the tiled local
 // step, or the per-corner forces.
-bool tiled(const mgsb_solver* s) { return s->p.tile_gather != 0; }
-const int* gath_rp(const mgsb_solver* s, const DevLevel& d) { return tiled(s) ? d.prp : d.rp; }
-float3* gath_cf(const mgsb_solver* s, const DevLevel& d) { return tiled(s) ? d.pcf : d.cf; }
-float3* gath_cf2(const mgsb_solver* s, const DevLevel& d) { return tiled(s) ? d.pcf2 : d.cf2; }
+bool tiled(const mgsb_solver* s, const DevLevel& d) { return s->p.tile_gather != 0 && d.tetp != nullptr; }
+const int* gath_rp(const mgsb_solver* s, const DevLevel& d) { return tiled(s, d) ? d.prp : d.rp; }
+float3* gath_cf(const mgsb_solver* s, const DevLevel& d) { return tiled(s, d) ? d.pcf : d.cf; }
+float3* gath_cf2(const mgsb_solver* s, const DevLevel& d) { return tiled(s, d) ? d.pcf2 : d.cf2; }
 
 int launch_tet(mgsb_solver* s, DevLevel& d, int mode, bool with_tri, const float4* xsrc = nullptr,
                const int* skip = nullptr, bool split = false, bool corners = false) {
@@ -2422,7 +2443,7 @@ int launch_tet(mgsb_solver* s, DevLevel& d, int mode, bool with_tri, const float
     int rc = timed_begin(s, kid);
     if (rc) return rc;
     const int nb = nblk(n, bs);
-    const bool til = tiled(s) && !corners;  // corners: the per-corner layout (test readback)
+    const bool til = tiled(s, d) && !corners;  // corners: the per-corner layout (test readback)
     switch (mode) {
         case kTetModePolar: launch_tet_mode<kTetModePolar>(s, d, nb, bs, xx, ntri, skip, sref, til); break;
         case kTetModeAc: launch_tet_mode<kTetModeAc>(s, d, nb, bs, xx, ntri, skip, sref, til); break;
@@ -2441,13 +2462,13 @@ int tet_mode_of(int if_mode) {
 // Local step at the temporal extrapolation (forces to cf) and at s (cf2) in
 // one launch of 2T lanes.
 int launch_tet_dual(mgsb_solver* s, DevLevel& d) {
-    if (tiled(s)) {
+    if (tiled(s, d)) {
         const int nb = 2 * d.ntiles;
         auto k = tet_mode_of(s->p.if_mode) == kTetModePolar
                      ? k_tet_dual_tiled<kTetModePolar>
                      : (tet_mode_of(s->p.if_mode) == kTetModeAc ? k_tet_dual_tiled<kTetModeAc>
                                                                   : k_tet_dual_tiled<kTetModeStable>);
-        k<<<nb, kTileTets, 0, s->stream>>>(d.T, d.ntiles, d.x, d.s, d.tet, d.B, d.vol, d.tls, d.tvo, d.tvs, d.tlr,
+        k<<<nb, kTileTets, 0, s->stream>>>(d.T, d.ntiles, d.x, d.s, d.tetp, d.B, d.vol, d.tvo, d.tvs, d.tlr,
                                            d.pcf, d.pcf2, s->p.stiffness, d.tk);
         HIPCHK(hipGetLastError());
         return MGSB_OK;
@@ -4469,72 +4490,157 @@ int mgsb_autotune(mgsb_solver* s, int32_t trial_steps, mgsb_tune_result* out) {
         return MGSB_ERR_STATE;
     }
     HIPCHK(hipSetDevice(s->device));
-    StateSnap S;
-    int rc = snap_take(s, S);
+    StateSnap S0;
+    int rc = snap_take(s, S0);
     if (rc) return rc;
     const mgsb_params p0 = s->p;
     const bool vcycle = s->p.schedule == MGSB_SCHED_VCYCLE || s->p.schedule == MGSB_SCHED_LEARNED;
     int trials = 0;
-    // ms per step of the current parameters from the saved state; ok = no step over tolerance
-    auto measure = [&](double& ms, bool& ok) -> int {
-        int r;
-        if ((r = snap_restore(s, S)) || (r = invalidate_graph(s))) return r;
-        if ((r = mgsb_step(s, 2, nullptr))) return r;  // graph capture, warm caches
-        HIPCHK(hipStreamSynchronize(s->stream));
-        VState a{}, b{};
-        if (s->vstate) HIPCHK(hipMemcpy(&a, s->vstate, sizeof(VState), hipMemcpyDeviceToHost));
-        const auto t0 = std::chrono::steady_clock::now();
-        if ((r = mgsb_step(s, trial_steps, nullptr))) return r;
-        HIPCHK(hipStreamSynchronize(s->stream));
-        const auto t1 = std::chrono::steady_clock::now();
-        if (s->vstate) HIPCHK(hipMemcpy(&b, s->vstate, sizeof(VState), hipMemcpyDeviceToHost));
-        ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / trial_steps;
-        ok = b.nfail == a.nfail;
+    // 1. Launch shapes by kernel time: one fine-level sweep (local step and
+    // vertex update) as the schedule runs it, 20 back-to-back between HIP
+    // events.  A V-cycle sweep starts from u = 0 at the saved s, as the
+    // first sweep of a step does.
+    DevLevel& f = s->L[0];
+    const int tmode = tet_mode_of(s->p.if_mode);
+    const int vflags = vcycle ? (kVertPoint | kVertContactJacobi) : kVertPoint;
+    const int split = vcycle ? 1 : 0;
+    hipEvent_t ea, eb;
+    HIPCHK(hipEventCreate(&ea));
+    HIPCHK(hipEventCreate(&eb));
+    auto sweep_us = [&](double& us) -> int {
+        constexpr int reps = 20;
+        int r = MGSB_OK;
+        for (int pass = 0; pass < 2 && !r; pass++) {  // pass 0 warms up
+            if (vcycle) {
+                HIPCHK(hipMemsetAsync(f.x, 0, sizeof(float4) * f.V, s->stream));
+                HIPCHK(hipMemsetAsync(f.xprev, 0, sizeof(float4) * f.V, s->stream));
+            }
+            HIPCHK(hipEventRecord(ea, s->stream));
+            for (int k = 0; k < (pass ? reps : 1) && !r; k++) {
+                r = launch_tet(s, f, tmode, false, nullptr, nullptr, split != 0);
+                if (!r) r = launch_vert(s, f, 1.0f, vflags, nullptr, split);
+            }
+            HIPCHK(hipEventRecord(eb, s->stream));
+        }
+        if (r) return r;
+        HIPCHK(hipEventSynchronize(eb));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ea, eb));
+        us = 1e3 * (double)ms / reps;
         trials++;
         return MGSB_OK;
     };
-    double base = 0.0, best = 0.0, ms = 0.0;
-    bool ok = false;
-    if ((rc = measure(base, ok))) return rc;
-    best = ok ? base : 1e300;
-    mgsb_params bestp = s->p;
-    auto consider = [&]() -> int {
-        int r = measure(ms, ok);
-        if (r) return r;
-        if (ok && ms < best) {
-            best = ms;
-            bestp = s->p;
+    struct Shape {
+        int tg, bt, bv, vl;
+    };
+    std::vector<Shape> shapes{{p0.tile_gather ? 1 : 0, p0.block_tet, p0.block_vert, p0.vert_lanes}};
+    for (int bv : {64, 128, 256})
+        for (int vl : {1, 2}) shapes.push_back({1, p0.block_tet, bv, vl});
+    for (int bt : {64, 128, 256})
+        for (int bv : {64, 128, 256})
+            for (int vl : {2, 4, 8}) shapes.push_back({0, bt, bv, vl});
+    double base_us = 0.0, best_us = 1e300;
+    Shape best = shapes[0];
+    for (size_t q = 0; q < shapes.size(); q++) {
+        s->p.tile_gather = shapes[q].tg;
+        s->p.block_tet = shapes[q].bt;
+        s->p.block_vert = shapes[q].bv;
+        s->p.vert_lanes = shapes[q].vl;
+        double us = 0.0;
+        if ((rc = sweep_us(us))) break;
+        if (q == 0) base_us = us;
+        if (us < best_us) {
+            best_us = us;
+            best = shapes[q];
         }
-        s->p = bestp;
+    }
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
+    if (rc) {
+        s->p = p0;
+        (void)snap_restore(s, S0);
+        return rc;
+    }
+    if (!(best_us < 0.98 * base_us)) {  // a win within the timing noise keeps the current shape
+        best = shapes[0];
+        best_us = base_us;
+    }
+    s->p.tile_gather = best.tg;
+    s->p.block_tet = best.bt;
+    s->p.block_vert = best.bv;
+    s->p.vert_lanes = best.vl;
+    // 2. Steps per second over a window of trial_steps steps that starts
+    // trial_steps steps after the saved state (past the first steps, whose
+    // residual histories differ most from the run's), each candidate from
+    // the same state, best of two runs; on V-cycle schedules the sweep count
+    // (3, 4, 5, 6 pre- and post-sweeps) is chosen there, kept only if its
+    // steps all met the target and it beat the current count by 3 %.
+    if ((rc = snap_restore(s, S0)) || (rc = invalidate_graph(s))) return rc;
+    if ((rc = mgsb_step(s, trial_steps, nullptr))) return rc;
+    StateSnap S1;
+    if ((rc = snap_take(s, S1))) return rc;
+    auto measure = [&](double& ms, bool& ok) -> int {
+        ms = 1e300;
+        ok = true;
+        for (int rep = 0; rep < 2; rep++) {
+            int r;
+            if ((r = snap_restore(s, S1)) || (r = invalidate_graph(s))) return r;
+            if ((r = mgsb_step(s, 1, nullptr))) return r;  // graph capture, warm caches
+            HIPCHK(hipStreamSynchronize(s->stream));
+            VState a{}, b{};
+            if (s->vstate) HIPCHK(hipMemcpy(&a, s->vstate, sizeof(VState), hipMemcpyDeviceToHost));
+            const auto t0 = std::chrono::steady_clock::now();
+            if ((r = mgsb_step(s, trial_steps, nullptr))) return r;
+            HIPCHK(hipStreamSynchronize(s->stream));
+            const auto t1 = std::chrono::steady_clock::now();
+            if (s->vstate) HIPCHK(hipMemcpy(&b, s->vstate, sizeof(VState), hipMemcpyDeviceToHost));
+            ms = std::min(ms, std::chrono::duration<double, std::milli>(t1 - t0).count() / trial_steps);
+            ok = ok && b.nfail == a.nfail;
+        }
+        trials++;
         return MGSB_OK;
     };
+    const mgsb_params shaped = s->p;
+    double ms_shaped = 0.0, ms = 0.0, ms_default = 0.0;
+    bool ok = false;
+    s->p = p0;
+    if ((rc = measure(ms_default, ok))) return rc;
+    s->p = shaped;
+    if ((rc = measure(ms_shaped, ok))) return rc;
+    double best_ms = ms_shaped;
+    mgsb_params bestp = shaped;
     if (vcycle)
         for (int sw : {3, 4, 5, 6}) {
+            if (sw == shaped.pre_sweeps && sw == shaped.post_sweeps) continue;
+            s->p = shaped;
             s->p.pre_sweeps = s->p.post_sweeps = sw;
-            if ((rc = consider())) return rc;
+            if ((rc = measure(ms, ok))) return rc;
+            if (ok && ms < 0.97 * best_ms) {
+                best_ms = ms;
+                bestp = s->p;
+            }
         }
-    for (int b : {64, 128, 256}) {
-        s->p.block_tet = b;
-        if ((rc = consider())) return rc;
+    if (best_ms > ms_default) {  // the window disagrees with the kernel timing: keep what was there
+        best_ms = ms_default;
+        bestp = p0;
     }
-    for (int b : {64, 128, 256}) {
-        s->p.block_vert = b;
-        if ((rc = consider())) return rc;
-    }
-    for (int l : {1, 2, 4, 8}) {
-        s->p.vert_lanes = l;
-        if ((rc = consider())) return rc;
-    }
-    s->p = best < 1e300 ? bestp : p0;
-    if ((rc = snap_restore(s, S)) || (rc = invalidate_graph(s))) return rc;
+    s->p = bestp;
+    // the resolved values, so that the parameters read back are the ones in use
+    s->p.block_tet = block_tet(s);
+    s->p.block_vert = block_vert(s);
+    s->p.vert_lanes = 1 << vert_lsh(s);
+    if ((rc = snap_restore(s, S0)) || (rc = invalidate_graph(s))) return rc;
     out->block_tet = block_tet(s);
     out->block_vert = block_vert(s);
     out->pre_sweeps = s->p.pre_sweeps;
     out->post_sweeps = s->p.post_sweeps;
-    out->ms_per_step = best < 1e300 ? best : base;
-    out->ms_per_step_default = base;
+    out->ms_per_step = best_ms;
+    out->ms_per_step_default = ms_default;
     out->trials = trials;
     out->vert_lanes = 1 << vert_lsh(s);
+    out->tile_gather = s->p.tile_gather;
+    out->sweep_us = best_us;
+    out->sweep_us_default = base_us;
     return MGSB_OK;
 }
 
