@@ -194,8 +194,17 @@ __device__ __forceinline__ void polar_rf(const float F[9], float RF[9]) {
 // Algebraically identical to polar_rf, but k no longer comes from the
 // difference of two O(10) numbers, so small anisotropic strains survive in
 // FP32 (the reference form drops them below k < 1e-6).  Isotropic branch only
-// for k < 1e-30.  Writes R - F into RF.
+// for k < 1e-30.  Writes R - F into RF.  This form has no counterpart in the
+// reference (it serves the V-cycle schedules, whose residual iteration the
+// reference form stalls), so it is free of the reference's operation order:
+// its multiply-adds fuse, its divisions and square roots are the
+// hardware's reciprocal, reciprocal square root and square root (about 1
+// ulp each), and cos(phi / 3), phi / 3 in [0, pi / 3], is the hardware
+// cosine: the local step is bound by instruction issue on large meshes.
+__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ void polar_rf_stable(const float F[9], float RF[9]) {
+#pragma clang fp contract(fast)
     float C[9], C2[9];
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -209,7 +218,7 @@ __device__ __forceinline__ void polar_rf_stable(const float F[9], float RF[9]) {
     const float det = F[0] * F[4] * F[8] + F[1] * F[5] * F[6] + F[3] * F[7] * F[2] - F[2] * F[4] * F[6] -
                       F[1] * F[3] * F[8] - F[0] * F[5] * F[7];
     const float I_c = C[0] + C[4] + C[8];
-    const float m = I_c / 3;
+    const float m = I_c * (1.0f / 3);
     const float D0 = C[0] - m, D4 = C[4] - m, D8 = C[8] - m;
     const float k = 1.5f * (D0 * D0 + D4 * D4 + D8 * D8 + 2 * (C[1] * C[1] + C[2] * C[2] + C[5] * C[5]));
     float iu[9];
@@ -218,7 +227,7 @@ __device__ __forceinline__ void polar_rf_stable(const float F[9], float RF[9]) {
         if (lt_1e6(I_c)) {
             ok = false;
         } else {
-            const float inv_lambda = 1 / sqrtf(m);
+            const float inv_lambda = __builtin_amdgcn_rsqf(m);
 #pragma unroll
             for (int i = 0; i < 9; i++) iu[i] = 0.f;
             iu[0] = iu[4] = iu[8] = inv_lambda;
@@ -226,25 +235,26 @@ __device__ __forceinline__ void polar_rf_stable(const float F[9], float RF[9]) {
     } else {
         const float l = 13.5f * (D0 * (D4 * D8 - C[5] * C[5]) - C[1] * (C[1] * D8 - C[5] * C[2]) +
                                  C[2] * (C[1] * C[5] - D4 * C[2]));
-        const float k_root = sqrtf(k);
-        float value = l / (k * k_root);
+        const float k_root = sqrt_hw(k);
+        float value = l * rcp_hw(k * k_root);
         value = value < -1.0f ? -1.0f : value;
         value = value > 1.0f ? 1.0f : value;
         const float phi = acosf(value);
-        const float lambda2 = m + 2.0f / 3 * k_root * cosf(phi / 3);
-        const float lambda = sqrtf(lambda2);
+        const float lambda2 = m + 2.0f / 3 * k_root * __cosf(phi * (1.0f / 3));  // phi / 3 in [0, pi/3]
+        const float lambda = sqrt_hw(lambda2);
         const float III_u = det;
+        const float t2 = 2 * III_u * rcp_hw(lambda);
         float I_u = 0.f, II_u = 0.f;
         if (lt_1e6(fabsf(lambda))) ok = false;
-        if (ok && lt_1e6(-lambda2 + I_c + 2 * III_u / lambda)) ok = false;
+        if (ok && lt_1e6(-lambda2 + I_c + t2)) ok = false;
         if (ok) {
-            I_u = lambda + sqrtf(-lambda2 + I_c + 2 * III_u / lambda);
+            I_u = lambda + sqrt_hw(-lambda2 + I_c + t2);
             II_u = (I_u * I_u - I_c) * 0.5f;
             if (lt_1e6(fabsf(I_u * II_u - III_u))) ok = false;
         }
         if (ok && lt_1e6(fabsf(III_u))) ok = false;
         if (ok) {
-            float inv_rate = 1 / (I_u * II_u - III_u);
+            float inv_rate = rcp_hw(I_u * II_u - III_u);
             const float fdiag = I_u * III_u * inv_rate;
             const float fc = (I_u * I_u - II_u) * inv_rate;
             float U[9];
@@ -253,7 +263,7 @@ __device__ __forceinline__ void polar_rf_stable(const float F[9], float RF[9]) {
             U[0] += fdiag;
             U[4] += fdiag;
             U[8] += fdiag;
-            inv_rate = 1 / III_u;
+            inv_rate = rcp_hw(III_u);
             const float gdiag = II_u * inv_rate;
             const float gu = -I_u * inv_rate;
 #pragma unroll
