@@ -197,10 +197,11 @@ __device__ __forceinline__ void polar_rf(const float F[9], float RF[9]) {
 // for k < 1e-30.  Writes R - F into RF.  This form has no counterpart in the
 // reference (it serves the V-cycle schedules, whose residual iteration the
 // reference form stalls), so it is free of the reference's operation order:
-// its multiply-adds fuse, its divisions and square roots are the
+// its multiply-adds fuse, most divisions and square roots are the
 // hardware's reciprocal, reciprocal square root and square root (about 1
 // ulp each), and cos(phi / 3), phi / 3 in [0, pi / 3], is the hardware
 // cosine: the local step is bound by instruction issue on large meshes.
+// lambda and I_u stay IEEE (below).
 __device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ void polar_rf_stable(const float F[9], float RF[9]) {
@@ -241,14 +242,17 @@ __device__ __forceinline__ void polar_rf_stable(const float F[9], float RF[9]) {
         value = value > 1.0f ? 1.0f : value;
         const float phi = acosf(value);
         const float lambda2 = m + 2.0f / 3 * k_root * __cosf(phi * (1.0f / 3));  // phi / 3 in [0, pi/3]
-        const float lambda = sqrt_hw(lambda2);
+        // IEEE from here to I_u: on an inverted tet near a reflection the
+        // difference under the root cancels and 1 / (I_u II_u - III_u) below
+        // amplifies every error in lambda and in that root
+        const float lambda = sqrtf(lambda2);
         const float III_u = det;
-        const float t2 = 2 * III_u * rcp_hw(lambda);
+        const float t2 = 2 * III_u / lambda;
         float I_u = 0.f, II_u = 0.f;
         if (lt_1e6(fabsf(lambda))) ok = false;
         if (ok && lt_1e6(-lambda2 + I_c + t2)) ok = false;
         if (ok) {
-            I_u = lambda + sqrt_hw(-lambda2 + I_c + t2);
+            I_u = lambda + sqrtf(-lambda2 + I_c + t2);
             II_u = (I_u * I_u - I_c) * 0.5f;
             if (lt_1e6(fabsf(I_u * II_u - III_u))) ok = false;
         }

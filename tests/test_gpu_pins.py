@@ -64,7 +64,14 @@ def test_tet_forces_every_mode_match_oracle(if_mode):
     the closed form is ill-conditioned (sqrt(-lambda^2 + I_c + 2 III_u /
     lambda) of a small difference), so there an ulp changes R visibly; the
     device must then be as accurate as the FP32 oracle itself against the
-    FP64 oracle: max error vs FP64 within 1.5x the FP32 oracle's (+1e-5)."""
+    FP64 oracle: max error vs FP64 within 1.5x the FP32 oracle's (+1e-5).
+
+    The stable variant has no counterpart in the reference, and the device
+    evaluates it with fused multiply-adds and the hardware reciprocal, square
+    root and cosine, the oracle with IEEE operations: there the FP64 oracle is
+    the yardstick everywhere, the device within 1.5x the FP32 oracle's error
+    (+1e-6; +1e-5 inverted), and with det F > 0 the median difference of the
+    two FP32 evaluations below 3e-6 (a few ulps of the forces' scale)."""
     rng = np.random.default_rng(5)
     mesh = W.beam(4)
     k = 1e5
@@ -89,6 +96,11 @@ def test_tet_forces_every_mode_match_oracle(if_mode):
               f"median {np.median(e32):.2e}; gpu vs o64 max {e64.max():.2e}; o32 vs o64 max {o64.max():.2e}",
               flush=True)
         assert np.isfinite(gpu).all()
+        if if_mode == M.IF_STABLE:
+            assert inv or np.median(e32) < 3e-6
+            assert e64.max() <= 1.5 * o64.max() + (1e-5 if inv else 1e-6), (strain, e64.max(), o64.max())
+            assert (ninv > 0) == inv
+            continue
         assert np.median(e32) < 1e-6
         if inv:
             assert ninv > 0
