@@ -131,6 +131,12 @@ typedef struct mgsb_params {
                            * default); 0: one force per (tet, corner) in the vertex-
                            * sorted CSR order.  Both are deterministic and differ by
                            * the grouping of the same sums in FP32. */
+    int32_t coarse_first; /* V-cycle, checks after every half-cycle or cycle: a step's
+                           * first half-cycle is the coarse correction of the start's
+                           * residual with post-smoothing, then pre-smoothing halves and
+                           * coarse-correction halves alternate (0, default: pre-smoothing
+                           * first, the textbook V-cycle).  After a warm start the error
+                           * left is mostly smooth, so the coarse correction pays first. */
 } mgsb_params;
 
 typedef struct mgsb_step_stats {

@@ -75,6 +75,7 @@ class mgsb_params(C.Structure):
         ("learn_slack", C.c_float),
         ("coarse_dense", C.c_int32),
         ("tile_gather", C.c_int32),
+        ("coarse_first", C.c_int32),
     ]
 
 

@@ -933,6 +933,7 @@ void mgsb_default_params(mgsb_params* p) {
     p->learn_slack = 1.0f;
     p->coarse_dense = 1;
     p->tile_gather = 1;
+    p->coarse_first = 0;
 }
 
 const char* mgsb_version(void) { return "mgsb 0.1 (gfx950)"; }

@@ -973,10 +973,11 @@ struct VState {
     int cf_ok;      // corner forces are those of the current x (or the step has converged)
     double rel;     // convergence measure at the last check: res / res0, or with several
                     // bodies the largest per-body res_b / res0_b
-    int maxh;       // half-cycle cap of a step (2 max_cycles)
+    int maxh;       // half-cycle cap of a step (2 max_cycles, + 1 with coarse_first)
     int notdone;    // the step has neither converged nor reached the cap (no checks: 0);
                     // the finish piece, enqueued before the host reads the verdict, skips
     double slack;   // the learned stage is kept while its residual <= slack * res
+    int cfirst;     // the step's first half-cycle is a coarse correction (coarse_first)
 };
 
 // Check modes of the fused residual reduction.
@@ -1035,8 +1036,9 @@ __device__ void vstate_apply(VState* st, double t, int mode, int* hm, double rel
         st->res = r;
         st->rel = rel_of(r);
         if (mode == kChkCycle) {
+            // with coarse_first the first check follows the coarse half alone
+            st->halves += (st->check == 2 || (st->cfirst && st->cycles == 0)) ? 1 : 2;
             st->cycles += 1;
-            st->halves += st->check == 2 ? 1 : 2;
         } else {
             st->halves += 1;
         }
@@ -1233,7 +1235,7 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
 // Both residuals of a temporally warm-started start on split positions: at
 // the extrapolation u = x with the forces cf (g written to r when
 // r != nullptr, active set to cmask) and at s itself (u = 0) with the forces
-// cf2.  part[2b] holds block b's partial of |g(s)|^2, part[2b + 1] that of
+// cf2 (g(s) and its active-set mask written to rs when rs != nullptr).  part[2b] holds block b's partial of |g(s)|^2, part[2b + 1] that of
 // |g(s + u)|^2.
 __global__ void __launch_bounds__(256) k_resid_dual(int V, const float4* __restrict__ x,
                                                     const float4* __restrict__ s, const float4* __restrict__ vc,
@@ -1241,7 +1243,7 @@ __global__ void __launch_bounds__(256) k_resid_dual(int V, const float4* __restr
                                                     const float3* __restrict__ cf2,
                                                     const DevColliders* __restrict__ col, int flags,
                                                     float4* __restrict__ r, double* __restrict__ part,
-                                                    float* __restrict__ cmask, int lsh) {
+                                                    float* __restrict__ cmask, int lsh, float4* __restrict__ rs) {
     const int gi = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gi >> lsh, sub = gi & ((1 << lsh) - 1);  // 2^lsh lanes per vertex
     double v[2] = {0.0, 0.0};
@@ -1255,6 +1257,10 @@ __global__ void __launch_bounds__(256) k_resid_dual(int V, const float4* __restr
             const float3 g0 =
                 vertex_residual(make_float4(0.f, 0.f, 0.f, 0.f), st, c, a.x, a.y, a.z, col, flags, 1, contact);
             v[0] = (double)g0.x * g0.x + (double)g0.y * g0.y + (double)g0.z * g0.z;
+            if (rs) {  // g(s) masked by the active set at s, the mask in w (for k_revert)
+                const float am0 = contact ? 0.f : 1.f;
+                rs[i] = make_float4(g0.x * am0, g0.y * am0, g0.z * am0, am0);
+            }
             const float3 g = vertex_residual(p, st, c, b.x, b.y, b.z, col, flags, 1, contact);
             const float am = contact ? 0.f : 1.f;
             if (cmask) cmask[i] = am;
@@ -1373,18 +1379,25 @@ __global__ void __launch_bounds__(256) k_warm_temporal(int V, float4* __restrict
 // Judge of a rejected warm-start stage: x returns to s (u = 0) or to the
 // temporal extrapolation.  cf2 != nullptr holds the corner forces at s (the
 // dual start): on a return to s each vertex copies its CSR range of them
-// into cf, so the forces match x again without another local step.
+// into cf, so the forces match x again without another local step, and with
+// rs != nullptr the residual and active set at s into r and cmask.
 __global__ void __launch_bounds__(256) k_revert(int V, float4* __restrict__ x, float4* __restrict__ xprev,
                                                 const float4* __restrict__ U1, const float4* __restrict__ U2,
                                                 const float4* __restrict__ vc, const VState* st,
                                                 const int* __restrict__ rp, const float3* __restrict__ cf2,
-                                                float3* __restrict__ cf) {
+                                                float3* __restrict__ cf, float4* __restrict__ r,
+                                                const float4* __restrict__ rs, float* __restrict__ cmask) {
     const int rv = st->revert;
     if (!rv) return;
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= V) return;
     if (cf2 && rv == 1)
         for (int e = rp[i]; e < rp[i + 1]; e++) cf[e] = cf2[e];
+    if (rs && rv == 1) {  // the residual at s (and its active set) for a coarse-first half
+        const float4 g = rs[i];
+        r[i] = make_float4(g.x, g.y, g.z, 0.f);
+        cmask[i] = g.w;
+    }
     float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
     if (rv == 2) {
         const float4 a = U1[i], b = U2[i];
@@ -2365,12 +2378,19 @@ int pull_colliders(mgsb_solver* s) {
     return MGSB_OK;
 }
 
+// coarse_first applies to steps with checks: the first half-cycle is a
+// coarse-correction half, and the half-cycle cap is the first coarse half
+// plus max_cycles cycles
+bool coarse_first(const mgsb_solver* s) { return s->p.coarse_first != 0 && s->p.check_every > 0; }
+int max_halves(const mgsb_solver* s) { return 2 * s->p.max_cycles + (coarse_first(s) ? 1 : 0); }
+
 int push_vstate(mgsb_solver* s) {
     if (!s->vstate) return MGSB_OK;
     VState v{};
     v.check = s->p.check_every;
     v.tol = s->p.tol;
-    v.maxh = 2 * s->p.max_cycles;
+    v.maxh = max_halves(s);
+    v.cfirst = coarse_first(s) ? 1 : 0;
     v.slack = s->p.learn_slack > 0.f ? (double)s->p.learn_slack : 1.0;
     HIPCHK(hipMemcpyAsync(s->vstate, &v, sizeof(VState), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -2709,6 +2729,7 @@ bool temporal_warm(const mgsb_solver* s) {
     return s->p.warm_start == 1 || (s->p.schedule == MGSB_SCHED_LEARNED && s->p.warm_start != 2);
 }
 
+
 // Inertial prediction, g(s) and res0, then the warm start: the temporal
 // extrapolation of the last corrections (warm_start = 1 or learned), and with
 // learned = true the MLP's coarse correction on top.  The network reads only
@@ -2734,6 +2755,8 @@ int enqueue_vc_start(mgsb_solver* s, bool learned) {
     const int judge = learned ? kChkWarm : kChkTemporal;
     const float4* ep = learned ? s->epred : nullptr;
     float* recX = s->recording ? s->recX + (size_t)s->rec_steps * c1.V * kFeat : nullptr;
+    // a coarse-first step restricts the start's residual: keep it in r
+    const bool cf = coarse_first(s);
     int rc;
     if (features) {  // features at the step's start (recorded even if the step converges at once)
         k_features<<<nblk(c1.V, bs), bs, 0, s->stream>>>(c1.V, f.v, s->M[0].aids, s->M[0].aw, s->X1, s->D1, s->D2,
@@ -2748,30 +2771,38 @@ int enqueue_vc_start(mgsb_solver* s, bool learned) {
         if ((rc = launch_st(s, f, 1, 1, temporal, ep))) return rc;
         if ((rc = launch_tet_dual(s, f))) return rc;
         const int lsh = vert_lsh(s), nbr = nblk(f.V << lsh, 256);
-        k_resid_dual<<<nbr, 256, 0, s->stream>>>(f.V, f.x, f.s, f.vc, gath_rp(s, f), gath_cf(s, f), gath_cf2(s, f), s->dcol, kVertPoint, nullptr,
-                                                 s->vpart, f.cmask, lsh);
+        k_resid_dual<<<nbr, 256, 0, s->stream>>>(f.V, f.x, f.s, f.vc, gath_rp(s, f), gath_cf(s, f), gath_cf2(s, f),
+                                                 s->dcol, kVertPoint, cf ? f.r : nullptr, s->vpart, f.cmask, lsh,
+                                                 cf ? f.rr : nullptr);
         HIPCHK(hipGetLastError());
         k_check2<<<1, 256, 0, s->stream>>>(nbr, s->vpart, s->vstate, s->hmirror, s->seqc, judge);
         s->chk_launched++;
         HIPCHK(hipGetLastError());
         k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, gath_rp(s, f),
-                                             gath_cf2(s, f), gath_cf(s, f));
+                                             gath_cf2(s, f), gath_cf(s, f), f.r, cf ? f.rr : nullptr, f.cmask);
         HIPCHK(hipGetLastError());
         return MGSB_OK;
     }
     if ((rc = launch_st(s, f, 1, 1, false))) return rc;
-    if ((rc = fine_residual(s, true, false, kChkStart, true, nullptr, true))) return rc;
+    if ((rc = fine_residual(s, true, cf, kChkStart, true, nullptr, true))) return rc;
     if (warm) {
         k_warm_temporal<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, temporal ? f.U1 : nullptr,
                                                     temporal ? f.U2 : nullptr, f.vc, skip, ep,
                                                     ep ? s->M[0].iids : nullptr, ep ? s->M[0].iw : nullptr);
         HIPCHK(hipGetLastError());
-        if ((rc = fine_residual(s, true, false, judge, true, skip))) return rc;
+        if ((rc = fine_residual(s, true, cf, judge, true, skip))) return rc;
         k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, gath_rp(s, f),
-                                             nullptr, gath_cf(s, f));
+                                             nullptr, gath_cf(s, f), nullptr, nullptr, nullptr);
         HIPCHK(hipGetLastError());
         // after a revert the corner forces belong to the rejected x: recompute
         if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, &s->vstate->cf_ok, true))) return rc;
+        if (cf) {  // ... and so does the residual a coarse-first half restricts
+            const int lsh = vert_lsh(s);
+            k_resid<<<nblk(f.V << lsh, 256), 256, 0, s->stream>>>(
+                f.V, f.x, f.s, f.vc, gath_rp(s, f), gath_cf(s, f), s->dcol, kVertPoint, f.r, nullptr, nullptr,
+                &s->vstate->cf_ok, 1, nullptr, f.cmask, lsh);
+            HIPCHK(hipGetLastError());
+        }
     }
     return MGSB_OK;
 }
@@ -2906,17 +2937,28 @@ int run_pieces(mgsb_solver* s, const std::string& seq, bool graphable) {
 // kPrecaptureHalves halves; longer guesses are captured at first use.
 constexpr int kPrecaptureHalves = 8;
 
+// The piece of a step's half-cycle h (0-based): pre-smoothing and coarse-
+// correction halves alternate, the first a coarse one with coarse_first.
+char half_piece(const mgsb_solver* s, int h) {
+    return (char)(((h & 1) != 0) != coarse_first(s) ? kPiecePost : kPiecePre);
+}
+// With checks after whole cycles only, a sequence of h halves must not end
+// with a pre-smoothing half (it carries no check).
+bool ends_unchecked(const mgsb_solver* s, int h) {
+    return s->p.check_every == 1 && h > 0 && half_piece(s, h - 1) == kPiecePre;
+}
+
 int precapture_pieces(mgsb_solver* s) {
     const bool learned = s->p.schedule == MGSB_SCHED_LEARNED;
     const int check = s->p.check_every;
-    const int maxh = 2 * s->p.max_cycles;
+    const int maxh = max_halves(s);
     const std::string starts = learned ? std::string{(char)kPieceStart, (char)kPieceStartGated}
                                        : std::string{(char)kPieceStart};
     // only a recording (which runs eagerly) takes the full finish (fin_for)
     const std::string fins(1, (char)kPieceFinishLite);
-    auto halves = [](char st, int h) {
+    auto halves = [s](char st, int h) {
         std::string q(1, st);
-        for (int k = 0; k < h; k++) q.push_back((char)((k & 1) ? kPiecePost : kPiecePre));
+        for (int k = 0; k < h; k++) q.push_back(half_piece(s, k));
         return q;
     };
     std::vector<std::string> seqs;
@@ -2926,7 +2968,7 @@ int precapture_pieces(mgsb_solver* s) {
             continue;
         }
         for (int h = 0; h <= std::min(maxh, kPrecaptureHalves); h++) {
-            if (check == 1 && (h & 1)) continue;
+            if (ends_unchecked(s, h)) continue;
             for (const char f : fins) seqs.push_back(halves(st, h) + f);
         }
     }
@@ -3000,7 +3042,7 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
     if (gate_on && s->gate_left > 0) s->gate_left--;
     std::string seq(1, (char)(gated ? kPieceStartGated : kPieceStart));
     const int check = s->p.check_every;
-    const int maxh = 2 * s->p.max_cycles;
+    const int maxh = max_halves(s);
     // every finish keeps the delta history of the learned features; only a
     // recorded step also writes its training target
     auto fin_for = [&](int) { return s->recording ? kPieceFinish : kPieceFinishLite; };
@@ -3008,9 +3050,9 @@ int step_vcycle(mgsb_solver* s, bool graphable) {
     int target = check ? std::min(maxh, std::max(0, s->halves_guess)) : maxh;
     bool finished = false;
     for (;;) {
-        for (; h < target; h++) seq.push_back((char)((h & 1) ? kPiecePost : kPiecePre));
+        for (; h < target; h++) seq.push_back(half_piece(s, h));
         if (!check) break;
-        if (check == 1 && (h & 1)) {  // a pre half carries no check: finish the cycle
+        if (ends_unchecked(s, h)) {  // a pre half carries no check: finish the cycle
             target = h + 1;
             continue;
         }
@@ -3250,6 +3292,8 @@ int alloc_vcycle(mgsb_solver* s) {
             if ((rc = up(s, d.owned, &d.U2, z.data(), d.V))) return rc;
             std::vector<float> one(d.V, 1.f);
             if ((rc = up(s, d.owned, &d.cmask, one.data(), d.V))) return rc;
+            // the residual at s of a warm-started start, for a coarse-first revert
+            if ((rc = up<float4>(s, d.owned, &d.rr, nullptr, d.V))) return rc;
             if ((rc = up<float3>(s, d.owned, &d.cf2, nullptr, (size_t)d.T * 4))) return rc;
             if ((rc = up<float3>(s, d.owned, &d.pcf2, nullptr, (size_t)d.npart))) return rc;
             continue;
@@ -3389,6 +3433,10 @@ int validate_params(const mgsb_params* p, int nlevels) {
     }
     if (p->check_every < 0 || p->check_every > 2) {
         set_error("check_every must be 0 (no checks), 1 (after each cycle) or 2 (also after pre-smoothing)");
+        return MGSB_ERR_ARG;
+    }
+    if (p->coarse_first < 0 || p->coarse_first > 1) {
+        set_error("coarse_first must be 0 or 1");
         return MGSB_ERR_ARG;
     }
     if (nlevels > 8) {

@@ -1258,6 +1258,7 @@ void PFX(collide)(orc_level* L, const orc_params* p, int tri) {
 typedef struct orc_vparams {
     int pre, post, coarse_sweeps, max_cycles, coarse_direct, check;
     float tol, rho_smooth, relax_coarse, rho_coarse;
+    int coarse_first; /* with checks: the first half-cycle is the coarse correction */
 } orc_vparams;
 
 typedef struct orc_ml {
@@ -1636,14 +1637,35 @@ void PFX(ml_set_warm)(orc_ml* M, const float* e1, const float* u0) {
     }
 }
 
+/* Coarse-correction half of a cycle: the masked residual M->r[0] of the last
+ * fine evaluation restricted, the linear V-cycle on the coarse levels,
+ * prolongation, post-smoothing; returns the residual after it. */
+static double ml_coarse_half(orc_ml* M, orc_params* p, const orc_vparams* vp) {
+    orc_level* L0 = M->L[0];
+    real rs = vp->rho_smooth;
+    ml_restrict(M, 0, M->r[0], M->r[1]);
+    ml_vcycle_lin(M, 1, vp);
+    ml_prolong_add(M, 0, M->e[1], L0->x, ml_fine_mask(M));
+    memcpy(L0->xprev, L0->x, sizeof(real) * 3 * (size_t)L0->V);
+    real omega = 1;
+    for (int k = 0; k < vp->post; k++) {
+        omega = 4 / (4 - rs * rs * omega);
+        pd_iteration(L0, p, omega, 0, 1);
+    }
+    return ml_fine_residual(M, p);
+}
+
 /* One residual-correction V-cycle step (the device schedule MGSB_SCHED_VCYCLE).
  * check: 0 no checks (max_cycles cycles), 1 after every cycle, 2 also after
  * the pre-smoothing half (the residual the restriction uses).  cycles counts
  * completed cycles, halves the half-cycles run (pre-smoothing half, coarse
- * correction + post-smoothing half). */
+ * correction + post-smoothing half).  coarse_first (with checks): the first
+ * half is a coarse-correction half on the start's residual (counted as one
+ * half and one cycle), then max_cycles cycles at most. */
 void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nsteps) {
     orc_level* L0 = M->L[0];
     real rs = vp->rho_smooth;
+    const int cfirst = vp->coarse_first && vp->check;
     for (int st = 0; st < nsteps; st++) {
         move_spheres(p);
         calc_st(L0, p);
@@ -1693,6 +1715,12 @@ void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nstep
         }
         M->has_warm = 0;
         M->has_warm0 = 0;
+        if (cfirst && !conv) {
+            M->res = ml_coarse_half(M, p, vp);
+            M->cycles++;
+            M->halves++;
+            conv = M->res <= (double)vp->tol * M->res0;
+        }
         for (int c = 0; c < vp->max_cycles && !conv; c++) {
             real omega = 1;
             for (int k = 0; k < vp->pre; k++) {
@@ -1705,16 +1733,7 @@ void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nstep
                 M->halves++;
                 if (rm <= (double)vp->tol * M->res0) break;
             }
-            ml_restrict(M, 0, M->r[0], M->r[1]);
-            ml_vcycle_lin(M, 1, vp);
-            ml_prolong_add(M, 0, M->e[1], L0->x, ml_fine_mask(M));
-            memcpy(L0->xprev, L0->x, sizeof(real) * 3 * (size_t)L0->V);
-            omega = 1;
-            for (int k = 0; k < vp->post; k++) {
-                omega = 4 / (4 - rs * rs * omega);
-                pd_iteration(L0, p, omega, 0, 1);
-            }
-            M->res = ml_fine_residual(M, p);
+            M->res = ml_coarse_half(M, p, vp);
             M->cycles++;
             M->halves += vp->check == 2 ? 1 : 2;
             conv = vp->check && M->res <= (double)vp->tol * M->res0;

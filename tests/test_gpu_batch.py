@@ -64,9 +64,10 @@ def test_batch_fixed_cycles_matches_oracle_union():
     assert err < 1e-4
 
 
-def test_batch_every_body_meets_target():
+@pytest.mark.parametrize("cfirst", [0, 1])
+def test_batch_every_body_meets_target(cfirst):
     b = small_batch()
-    s = gpu_solver(b, max_cycles=12, check_every=2, warm_start=1, **KW)
+    s = gpu_solver(b, max_cycles=12, check_every=2, warm_start=1, coarse_first=cfirst, **KW)
     for _ in range(60):
         st = s.step(1)
         r0, r = s.body_residuals()

@@ -113,11 +113,11 @@ class Temporal:
         self.u2, self.u1 = self.u1, u.astype(np.float32)
 
 
-@pytest.mark.parametrize("check", [1, 2])
-def test_temporal_warm_start_matches_oracle(check):
+@pytest.mark.parametrize("check,cfirst", [(1, 0), (2, 0), (1, 1), (2, 1)])
+def test_temporal_warm_start_matches_oracle(check, cfirst):
     lv, Ls, ML, p, fix = oracle_pair((4, 1))
-    g = small_solver(M.SCHED_VCYCLE, warm_start=1, check_every=check)
-    vp = O.vparams(check=check, **VP)
+    g = small_solver(M.SCHED_VCYCLE, warm_start=1, check_every=check, coarse_first=cfirst)
+    vp = O.vparams(check=check, coarse_first=cfirst, **VP)
     hist = Temporal(Ls[0], fix)
     agree = 0
     for k in range(60):
@@ -129,7 +129,7 @@ def test_temporal_warm_start_matches_oracle(check):
         agree += (st.halves, st.fallback) == (o["halves"], o["fallback"])
         assert st.rel <= 1e-3
     err = rel_err(g.positions(), Ls[0].positions())
-    print(f"[temporal check={check}] rel(gpu,oracle)={err:.2e} agree {agree / 60:.2f} last halves {st.halves}")
+    print(f"[temporal check={check} coarse_first={cfirst}] rel(gpu,oracle)={err:.2e} agree {agree / 60:.2f} last halves {st.halves}")
     assert err < 1e-4 and agree >= 54
 
 
@@ -170,18 +170,19 @@ def prolong(lv, e1):
     return out
 
 
-def test_learned_schedule_matches_oracle_given_prediction():
+@pytest.mark.parametrize("cfirst", [0, 1])
+def test_learned_schedule_matches_oracle_given_prediction(cfirst):
     """The learned schedule's machinery (the MLP's coarse prediction
     prolongated into the temporal warm start, one judge of the warm start,
-    cycles) against the oracle fed the same warm start; the MLP itself is
-    checked against the oracle above."""
+    cycles, in either half-cycle order) against the oracle fed the same warm
+    start; the MLP itself is checked against the oracle above."""
     rec, hist = record_and_train((4, 1), 100, 20)
     w, norm = rec.learn_weights(), rec.learn_norm()
     lv, Ls, ML, p, fix = oracle_pair((4, 1))
-    g = small_solver(M.SCHED_LEARNED, check_every=2, learn_gate=0)
+    g = small_solver(M.SCHED_LEARNED, check_every=2, learn_gate=0, coarse_first=cfirst)
     g.learn_weights(w)
     g.learn_norm(norm)
-    vp = O.vparams(check=2, **VP)
+    vp = O.vparams(check=2, coarse_first=cfirst, **VP)
     tmp = Temporal(Ls[0], fix)
     agree = used = 0
     for k in range(60):

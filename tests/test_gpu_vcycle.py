@@ -84,12 +84,15 @@ def test_vcycle_gather_lanes_match_oracle(lanes):
     assert sum(a == b for a, b in zip(halves_g, halves_o)) >= 27
 
 
-@pytest.mark.parametrize("check", [1, 2])
-def test_vcycle_adaptive_meets_target_each_step(check):
+@pytest.mark.parametrize("check,cfirst", [(1, 0), (2, 0), (1, 1), (2, 1)])
+def test_vcycle_adaptive_meets_target_each_step(check, cfirst):
+    """Adaptive cycle counts against the oracle's, in the textbook order and
+    with coarse_first (the first half-cycle the coarse correction of g(s))."""
     kw = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, check_every=check, tol=1e-3,
-              rho_smooth=0.9)
+              rho_smooth=0.9, coarse_first=cfirst)
     lv, s, ML, Ls, p = make_pair((4, 1), None, **kw)
-    vp = O.vparams(pre=5, post=5, coarse_sweeps=2, max_cycles=12, check=check, tol=1e-3, rho_smooth=0.9)
+    vp = O.vparams(pre=5, post=5, coarse_sweeps=2, max_cycles=12, check=check, tol=1e-3, rho_smooth=0.9,
+                   coarse_first=cfirst)
     gc, oc = [], []
     for _ in range(80):
         st = s.step(1)
@@ -99,9 +102,9 @@ def test_vcycle_adaptive_meets_target_each_step(check):
         oc.append((o["cycles"], o["halves"]))
         assert st.rel <= 1e-3, st.rel
         if check == 1:
-            assert st.halves == 2 * st.cycles
+            assert st.halves == 2 * st.cycles - cfirst
     agree = np.mean([a == b for a, b in zip(gc, oc)])
-    print(f"[vcycle adaptive check={check}] halves gpu mean {np.mean([h for _, h in gc]):.2f} "
+    print(f"[vcycle adaptive check={check} coarse_first={cfirst}] halves gpu mean {np.mean([h for _, h in gc]):.2f} "
           f"oracle mean {np.mean([h for _, h in oc]):.2f} agree {agree:.2f}")
     assert agree >= 0.9
     assert rel_err(s.positions(), Ls[0].positions()) < REL_TOL

@@ -193,3 +193,69 @@ def test_warm_start_judge_rejects_and_restores():
     xhb, shb = run(bad_e, half)
     assert shb["fallback"] == 2 and shb["res_ws"] > shb["res_t"]
     assert np.array_equal(xhb, xh) and shb["halves"] == sh["halves"]
+
+
+def arap_forces(mesh, x, k):
+    """Per-vertex ARAP forces sum_t k vol_t (R_t - F_t) AC_t in FP64, R the
+    SVD polar factor (proper rotation), AC[:, 0] = -sum of B's rows,
+    AC[:, 1:] = B^T with B = Dm^-1."""
+    X = mesh.pos.astype(np.float64)
+    x = np.asarray(x, np.float64)
+    f = np.zeros_like(X)
+    for tet in mesh.tets:
+        Dm = np.stack([X[tet[q]] - X[tet[0]] for q in (1, 2, 3)], 1)
+        B = np.linalg.inv(Dm)
+        F = np.stack([x[tet[q]] - x[tet[0]] for q in (1, 2, 3)], 1) @ B
+        U, _, Vt = np.linalg.svd(F)
+        if np.linalg.det(U @ Vt) < 0:
+            U[:, -1] *= -1
+        AC = np.concatenate([-B.sum(axis=0)[:, None], B.T], axis=1)
+        f[tet] += (k * abs(np.linalg.det(Dm)) / 6 * (U @ Vt - F) @ AC).T
+    return f
+
+
+def test_coarse_first_half_is_the_coarse_correction_of_g_s():
+    """coarse_first: a step's first half-cycle is the coarse correction of
+    the start's residual.  Two levels, no warm start, no post-smoothing and
+    no further cycles: the step's correction u = x - s must equal
+    fix * P A_1^-1 (vfix_1 * P^T g(s)), assembled here from the oracle's
+    corner forces at s (summed per vertex in numpy), the binding weights and
+    the coarsest solve (pinned above against numpy).  A pre-smoothing-first
+    order, a residual taken at x instead of s, or a transfer without the
+    Dirichlet masks fails it."""
+    fine, coarse = W.beam(2), W.beam(1)
+    phys = W.physics()
+    fix = W.top_fixflag(fine)
+    v0 = W.seeded_velocity(fine, fix)
+    Ls = [O.Level(m.pos, m.tets, m.tris, stiffness=phys["stiffness"], density=phys["density"], fp64=True)
+          for m in (fine, coarse)]
+    Ls[0].set_fix(fix)
+    Ls[0].set_velocity(v0)
+    ML = O.MultiLevel(Ls, relax_coarse=0.6, dt=phys["dt"])
+    p = O.params(dt=phys["dt"], stiffness=phys["stiffness"], density=phys["density"], gravity=phys["gravity"],
+                 if_mode=3)
+    vp = O.vparams(pre=3, post=0, coarse_sweeps=2, max_cycles=0, check=2, tol=0.0, rho_smooth=0.9, coarse_first=1)
+    ML.step(p, vp, 1)
+    st = ML.stats()
+    assert st["halves"] == 1 and st["cycles"] == 1
+    s = Ls[0].get(O.S).reshape(-1, 3)
+    u = Ls[0].positions() - s
+    # g(s) = fix * f_el(s) (x = s: the inertial term vanishes; no colliders)
+    g = fix[:, None] * arap_forces(fine, s, phys["stiffness"])
+    ids, w, _ = coarse.bind(fine.pos, 0)
+    r1 = np.zeros((coarse.nverts, 3))
+    np.add.at(r1, ids.reshape(-1), (w[:, :, None] * g[:, None, :]).reshape(-1, 3))
+    r1 *= ML.vfix(1)[:, None]
+    e1 = ML.coarse_solve(r1.astype(np.float32)).astype(np.float64)
+    ref = fix[:, None] * np.einsum("nk,nkc->nc", w.astype(np.float64), e1[ids])
+    err = np.abs(u - ref).max() / np.abs(ref).max()
+    assert err < 1e-6, err
+    # the textbook order smooths first: a different correction
+    Ls2 = [O.Level(m.pos, m.tets, m.tris, stiffness=phys["stiffness"], density=phys["density"], fp64=True)
+           for m in (fine, coarse)]
+    Ls2[0].set_fix(fix)
+    Ls2[0].set_velocity(v0)
+    ML2 = O.MultiLevel(Ls2, relax_coarse=0.6, dt=phys["dt"])
+    ML2.step(p, O.vparams(pre=3, post=0, coarse_sweeps=2, max_cycles=1, check=2, tol=0.0, rho_smooth=0.9), 1)
+    u2 = Ls2[0].positions() - Ls2[0].get(O.S).reshape(-1, 3)
+    assert np.abs(u2 - ref).max() > 100 * err * np.abs(ref).max()
