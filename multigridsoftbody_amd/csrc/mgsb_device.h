@@ -201,41 +201,39 @@ __device__ __forceinline__ void polar_rf(const float F[9], float RF[9]) {
 // hardware's reciprocal, reciprocal square root and square root (about 1
 // ulp each), and cos(phi / 3), phi / 3 in [0, pi / 3], is the hardware
 // cosine: the local step is bound by instruction issue on large meshes.
-// lambda and I_u stay IEEE (below).
+// lambda and I_u stay IEEE (below).  C and C^2 are formed from their six
+// distinct entries, U^-1 = gu U + q C + gdiag I with U = fc C - r C^2 +
+// fdiag I is applied as one combination a C + b C^2 + g I, and R - F is
+// F (U^-1 - I): the same formulas in fewer instructions.
 __device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ void polar_rf_stable(const float F[9], float RF[9]) {
 #pragma clang fp contract(fast)
-    float C[9], C2[9];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) C[i * 3 + j] = F[0 * 3 + i] * F[0 * 3 + j] + F[1 * 3 + i] * F[1 * 3 + j] + F[2 * 3 + i] * F[2 * 3 + j];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-            C2[i * 3 + j] = C[i * 3 + 0] * C[j * 3 + 0] + C[i * 3 + 1] * C[j * 3 + 1] + C[i * 3 + 2] * C[j * 3 + 2];
-    const float det = F[0] * F[4] * F[8] + F[1] * F[5] * F[6] + F[3] * F[7] * F[2] - F[2] * F[4] * F[6] -
-                      F[1] * F[3] * F[8] - F[0] * F[5] * F[7];
-    const float I_c = C[0] + C[4] + C[8];
+    // C = F^T F is symmetric: its six distinct entries
+    const float c00 = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
+    const float c01 = F[0] * F[1] + F[3] * F[4] + F[6] * F[7];
+    const float c02 = F[0] * F[2] + F[3] * F[5] + F[6] * F[8];
+    const float c11 = F[1] * F[1] + F[4] * F[4] + F[7] * F[7];
+    const float c12 = F[1] * F[2] + F[4] * F[5] + F[7] * F[8];
+    const float c22 = F[2] * F[2] + F[5] * F[5] + F[8] * F[8];
+    // det F by cofactors of the first row
+    const float det = F[0] * (F[4] * F[8] - F[5] * F[7]) + F[1] * (F[5] * F[6] - F[3] * F[8]) +
+                      F[2] * (F[3] * F[7] - F[4] * F[6]);
+    const float I_c = c00 + c11 + c22;
     const float m = I_c * (1.0f / 3);
-    const float D0 = C[0] - m, D4 = C[4] - m, D8 = C[8] - m;
-    const float k = 1.5f * (D0 * D0 + D4 * D4 + D8 * D8 + 2 * (C[1] * C[1] + C[2] * C[2] + C[5] * C[5]));
-    float iu[9];
+    const float D0 = c00 - m, D4 = c11 - m, D8 = c22 - m;
+    const float k = 1.5f * (D0 * D0 + D4 * D4 + D8 * D8 + 2 * (c01 * c01 + c02 * c02 + c12 * c12));
+    // U^-1 = a C + b C^2 + g I (symmetric); RF = F (U^-1 - I) = R - F
+    float a = 0.f, b = 0.f, g = 0.f;
     bool ok = true;
     if (k < 1e-30f) {
-        if (lt_1e6(I_c)) {
+        if (lt_1e6(I_c))
             ok = false;
-        } else {
-            const float inv_lambda = __builtin_amdgcn_rsqf(m);
-#pragma unroll
-            for (int i = 0; i < 9; i++) iu[i] = 0.f;
-            iu[0] = iu[4] = iu[8] = inv_lambda;
-        }
+        else
+            g = __builtin_amdgcn_rsqf(m);
     } else {
-        const float l = 13.5f * (D0 * (D4 * D8 - C[5] * C[5]) - C[1] * (C[1] * D8 - C[5] * C[2]) +
-                                 C[2] * (C[1] * C[5] - D4 * C[2]));
+        const float l = 13.5f * (D0 * (D4 * D8 - c12 * c12) - c01 * (c01 * D8 - c12 * c02) +
+                                 c02 * (c01 * c12 - D4 * c02));
         const float k_root = sqrt_hw(k);
         float value = l * rcp_hw(k * k_root);
         value = value < -1.0f ? -1.0f : value;
@@ -258,36 +256,40 @@ __device__ __forceinline__ void polar_rf_stable(const float F[9], float RF[9]) {
         }
         if (ok && lt_1e6(fabsf(III_u))) ok = false;
         if (ok) {
-            float inv_rate = rcp_hw(I_u * II_u - III_u);
-            const float fdiag = I_u * III_u * inv_rate;
-            const float fc = (I_u * I_u - II_u) * inv_rate;
-            float U[9];
-#pragma unroll
-            for (int i = 0; i < 9; i++) U[i] = fc * C[i] - inv_rate * C2[i];
-            U[0] += fdiag;
-            U[4] += fdiag;
-            U[8] += fdiag;
-            inv_rate = rcp_hw(III_u);
-            const float gdiag = II_u * inv_rate;
-            const float gu = -I_u * inv_rate;
-#pragma unroll
-            for (int i = 0; i < 9; i++) iu[i] = gu * U[i] + inv_rate * C[i];
-            iu[0] += gdiag;
-            iu[4] += gdiag;
-            iu[8] += gdiag;
+            // U = fc C - r C^2 + fdiag I and U^-1 = gu U + q C + gdiag I,
+            // folded into one combination of C, C^2 and I
+            const float r = rcp_hw(I_u * II_u - III_u);
+            const float fdiag = I_u * III_u * r;
+            const float fc = (I_u * I_u - II_u) * r;
+            const float q = rcp_hw(III_u);
+            const float gdiag = II_u * q;
+            const float gu = -I_u * q;
+            a = gu * fc + q;
+            b = -(gu * r);
+            g = gu * fdiag + gdiag;
         }
     }
-    if (ok) {
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-                RF[i * 3 + j] = (F[i * 3 + 0] * iu[0 * 3 + j] + F[i * 3 + 1] * iu[1 * 3 + j] + F[i * 3 + 2] * iu[2 * 3 + j]) -
-                                F[i * 3 + j];
-    } else {
+    if (!ok) {
 #pragma unroll
         for (int i = 0; i < 9; i++) RF[i] = ((i % 4) == 0 ? 1.f : 0.f) - F[i];
+        return;
     }
+    // C^2 (symmetric) only where it enters
+    const float q00 = c00 * c00 + c01 * c01 + c02 * c02;
+    const float q01 = c00 * c01 + c01 * c11 + c02 * c12;
+    const float q02 = c00 * c02 + c01 * c12 + c02 * c22;
+    const float q11 = c01 * c01 + c11 * c11 + c12 * c12;
+    const float q12 = c01 * c02 + c11 * c12 + c12 * c22;
+    const float q22 = c02 * c02 + c12 * c12 + c22 * c22;
+    const float gm = g - 1.0f;
+    const float M[9] = {a * c00 + b * q00 + gm, a * c01 + b * q01,      a * c02 + b * q02,
+                        a * c01 + b * q01,      a * c11 + b * q11 + gm, a * c12 + b * q12,
+                        a * c02 + b * q02,      a * c12 + b * q12,      a * c22 + b * q22 + gm};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            RF[i * 3 + j] = F[i * 3 + 0] * M[0 * 3 + j] + F[i * 3 + 1] * M[1 * 3 + j] + F[i * 3 + 2] * M[2 * 3 + j];
 }
 
 // barycentricCoordinate on four points (utils.cu:335-364), GetVolumn inline.

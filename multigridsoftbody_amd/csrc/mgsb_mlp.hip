@@ -241,7 +241,7 @@ constexpr int kH = MLP_HIDDEN;
 struct DenseIn {
     const float* X;
     int n, in;
-    __device__ void load4(int row, int k0, float (&v)[4]) const {
+    __device__ void load4(int row, int k0, float (&v)[4], float* /*prior*/) const {
 #pragma unroll
         for (int q = 0; q < 4; q++) v[q] = (row < n && k0 + q < in) ? X[(long)row * in + k0 + q] : 0.f;
     }
@@ -249,7 +249,7 @@ struct DenseIn {
 struct DenseOut {
     float* Y;
     int n, out, ldy;
-    __device__ void store(int row, int col, float v) const {
+    __device__ void store(int row, int col, float v, const float* /*prior*/) const {
         if (row < n && col < out) Y[(long)row * ldy + col] = v;
     }
 };
@@ -262,6 +262,9 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int nout, In in, Out o
                                                    const float* __restrict__ b4, const int* __restrict__ skip) {
     if (skip && *skip) return;
     __shared__ __align__(16) short hs[2][16][kH];
+    // per-row values the input functor hands to the output functor (the
+    // coarse correction's prior), written by wave 0's loads
+    __shared__ float pri[16][4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int base = blockIdx.x * 16;
     const int r = lane & 15, kq = (lane >> 4) * 4;
@@ -286,7 +289,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int nout, In in, Out o
     {
         bf16x4 a;
         float x4[4];
-        in.load4(base + r, kq, x4);
+        in.load4(base + r, kq, x4, wave == 0 ? &pri[r][0] : nullptr);
 #pragma unroll
         for (int q = 0; q < 4; q++) a[q] = f2bf(x4[q]);
 #pragma unroll
@@ -334,7 +337,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int nout, In in, Out o
     }
 #pragma unroll
     for (int q = 0; q < 4; q++)
-        if (r < nout) out.store(base + kq + q, r, acc[q] + b4[r]);
+        if (r < nout) out.store(base + kq + q, r, acc[q] + b4[r], &pri[kq + q][0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -489,7 +492,8 @@ int mlp_infer(Mlp* p, const float* X, int n, float* Y, int ldy, hipStream_t st, 
 struct CoarseIn {
     CoarseIO io;
     int n;
-    __device__ void load4(int row, int k0, float (&v)[4]) const {
+    // prior (when not null): the row's injected last delta, kept for the output
+    __device__ void load4(int row, int k0, float (&v)[4], float* prior) const {
         float raw[4] = {0.f, 0.f, 0.f, 0.f};
         if (row < n) {
             if (k0 == 0) {
@@ -507,6 +511,10 @@ struct CoarseIn {
                 raw[1] = X.z;
                 raw[2] = e.x;
                 raw[3] = e.y;
+                if (prior) {
+                    prior[0] = e.x;
+                    prior[1] = e.y;
+                }
             } else if (k0 == 8) {
                 const int4 id = io.aids[row];
                 const float4 w = io.aw[row];
@@ -515,6 +523,7 @@ struct CoarseIn {
                 raw[1] = f.x;
                 raw[2] = f.y;
                 raw[3] = f.z;
+                if (prior) prior[2] = e.z;
             }
         }
 #pragma unroll
@@ -527,12 +536,11 @@ struct CoarseIn {
 struct CoarseOut {
     CoarseIO io;
     int n;
-    __device__ void store(int row, int col, float v) const {
+    // prior: the row's injected last delta, as load4 computed it
+    __device__ void store(int row, int col, float v, const float* prior) const {
         if (row >= n || col >= 3) return;
         const float mu = io.norm[2 * kCoarseFeat + col], sd = io.norm[2 * kCoarseFeat + 3 + col];
-        const float3 e = inject4(io.D1, io.aids[row], io.aw[row]);
-        const float prior = col == 0 ? e.x : (col == 1 ? e.y : e.z);
-        reinterpret_cast<float*>(io.epred + row)[col] = io.vcl[row].z * (prior + (v * sd + mu));
+        reinterpret_cast<float*>(io.epred + row)[col] = io.vcl[row].z * (prior[col] + (v * sd + mu));
     }
 };
 

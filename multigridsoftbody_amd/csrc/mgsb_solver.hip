@@ -303,6 +303,29 @@ __device__ __forceinline__ bool tri_sphere_hit(float3 A, float3 B, float3 C, flo
     return false;
 }
 
+// The stable mode's products (it has no counterpart in the reference, so it
+// keeps no reference operation order): F = D Dm^-1 and the corner forces
+// (R - F) AC with their multiply-adds fused, and the weight vol k applied as
+// one factor after the NaN -> 0 and [-10, 10] clamp.
+__device__ __forceinline__ void stable_products_F(const float (&D)[9], const float (&b)[9], float (&F)[9]) {
+#pragma clang fp contract(fast)
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) F[r * 3 + c] = D[r * 3 + 0] * b[c] + D[r * 3 + 1] * b[3 + c] + D[r * 3 + 2] * b[6 + c];
+}
+__device__ __forceinline__ void stable_forces(const float (&RF)[9], const float (&AC)[12], float vk, float (&fo)[12]) {
+#pragma clang fp contract(fast)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            float acc = RF[i * 3 + 0] * AC[0 * 4 + j] + RF[i * 3 + 1] * AC[1 * 4 + j] + RF[i * 3 + 2] * AC[2 * 4 + j];
+            acc = isnan(acc) ? 0.f : fminf(fmaxf(acc, -10.f), 10.f);
+            fo[j * 3 + i] = acc * vk;
+        }
+}
+
 // Per-tet local step of tet g: F = Ds Dm^-1, R - F by the Franca polar
 // decomposition, P = (R - F) AC with AC rebuilt from Dm^-1, NaN -> 0, clamp to
 // [-10, 10], times vol * k (calculateIF, PDSolver.cu:128-183; calTetIFV1's
@@ -344,10 +367,15 @@ __device__ __forceinline__ void tet_local_forces(int g, int T, int4 id, const fl
         D[6] = (s1.z - s0.z) + D[6]; D[7] = (s2.z - s0.z) + D[7]; D[8] = (s3.z - s0.z) + D[8];
     }
     float F[9];
+    if (MODE == kTetModeStable) {
+        stable_products_F(D, b, F);
+    } else {
 #pragma unroll
-    for (int r = 0; r < 3; r++)
+        for (int r = 0; r < 3; r++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) F[r * 3 + c] = D[r * 3 + 0] * b[c] + D[r * 3 + 1] * b[3 + c] + D[r * 3 + 2] * b[6 + c];
+            for (int c = 0; c < 3; c++)
+                F[r * 3 + c] = D[r * 3 + 0] * b[c] + D[r * 3 + 1] * b[3 + c] + D[r * 3 + 2] * b[6 + c];
+    }
     float RF[9];
     if (MODE == kTetModeLinear) {
 #pragma unroll
@@ -365,6 +393,10 @@ __device__ __forceinline__ void tet_local_forces(int g, int T, int4 id, const fl
         AC[r * 4 + 1] = b[r];
         AC[r * 4 + 2] = b[3 + r];
         AC[r * 4 + 3] = b[6 + r];
+    }
+    if (MODE == kTetModeStable) {
+        stable_forces(RF, AC, vt * kk, fo);
+        return;
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -1121,28 +1153,6 @@ __global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict_
     }
 }
 
-// Both checks of a temporally warm-started start (k_resid_dual's partials,
-// each summed in block order like k_check): res0 = |g(s)|, then the judge of
-// the extrapolation.  Converged already at s: x (which holds the
-// extrapolation) must return to s, so revert is set without a verdict.
-__global__ void __launch_bounds__(256) k_check2(int nb, const double* __restrict__ part, VState* st, int* hm,
-                                                unsigned* seqc, int judge) {
-    double v[2] = {0.0, 0.0};
-    for (int q = threadIdx.x; q < nb; q += blockDim.x) {
-        v[0] += part[2 * q];
-        v[1] += part[2 * q + 1];
-    }
-    wg_sum_fixed<2>(v);
-    if (threadIdx.x == 0) {
-        vstate_apply(st, v[0], kChkStart, hm);
-        if (!st->converged)
-            vstate_apply(st, v[1], judge, hm);
-        else
-            st->revert = 1;
-        bump_seq(seqc, hm, st);
-    }
-}
-
 // Nonlinear residual g = m/h^2 (s - x) + f_el + f_col of a vertex on free
 // DOFs from its summed corner forces f: p is x (or the correction u when
 // split), st = s, c = (m/h^2, diag, fix, m).  contact: a collider acts on the
@@ -1406,6 +1416,61 @@ __global__ void __launch_bounds__(256) k_revert(int V, float4* __restrict__ x, f
     }
     x[i] = q;
     xprev[i] = q;
+}
+
+// Both checks of a temporally warm-started start and the judge's revert in
+// one launch (the dual start): res0 = |g(s)| from k_resid_dual's partials
+// part[2b], then the judge of the warm start from part[2b + 1]; converged
+// already at s, x (which holds the warm start) returns to s without a
+// verdict.  Every workgroup sums the partials in the same fixed order
+// (lane-strided over blockDim 256, then wg_sum_fixed, as k_check) and takes
+// the same verdict on a fresh copy of the step's constants; workgroup 0
+// applies it to the step state and publishes it.  vstate_apply at the start check resets every field the
+// judge reads but check, tol and slack, so the copies decide alike.
+__global__ void __launch_bounds__(256) k_judge_revert(int nb, const double* __restrict__ part, VState* st, int* hm,
+                                                      unsigned* seqc, int judge, int V, float4* __restrict__ x,
+                                                      float4* __restrict__ xprev, const int* __restrict__ rp,
+                                                      const float3* __restrict__ cf2, float3* __restrict__ cf,
+                                                      float4* __restrict__ r, const float4* __restrict__ rs,
+                                                      float* __restrict__ cmask) {
+    double v[2] = {0.0, 0.0};
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) {
+        v[0] += part[2 * q];
+        v[1] += part[2 * q + 1];
+    }
+    wg_sum_fixed<2>(v);
+    __shared__ int s_rv;
+    if (threadIdx.x == 0) {
+        VState loc{};
+        VState* tgt = &loc;
+        if (blockIdx.x == 0) {
+            tgt = st;
+        } else {
+            loc.check = st->check;
+            loc.tol = st->tol;
+            loc.slack = st->slack;
+        }
+        vstate_apply(tgt, v[0], kChkStart, hm);
+        if (!tgt->converged)
+            vstate_apply(tgt, v[1], judge, hm);
+        else
+            tgt->revert = 1;
+        s_rv = tgt->revert;
+        if (blockIdx.x == 0) bump_seq(seqc, hm, st);
+    }
+    __syncthreads();
+    const int rv = s_rv;  // the dual start rejects to s only
+    if (!rv) return;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    for (int e = rp[i]; e < rp[i + 1]; e++) cf[e] = cf2[e];
+    if (rs) {
+        const float4 g = rs[i];
+        r[i] = make_float4(g.x, g.y, g.z, 0.f);
+        cmask[i] = g.w;
+    }
+    x[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    xprev[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Features of the learned coarse correction at each vertex of level 1, all
@@ -2775,11 +2840,11 @@ int enqueue_vc_start(mgsb_solver* s, bool learned) {
                                                  s->dcol, kVertPoint, cf ? f.r : nullptr, s->vpart, f.cmask, lsh,
                                                  cf ? f.rr : nullptr);
         HIPCHK(hipGetLastError());
-        k_check2<<<1, 256, 0, s->stream>>>(nbr, s->vpart, s->vstate, s->hmirror, s->seqc, judge);
+        // res0, the judge and its revert to s in one launch
+        k_judge_revert<<<nbv, 256, 0, s->stream>>>(nbr, s->vpart, s->vstate, s->hmirror, s->seqc, judge, f.V, f.x,
+                                                   f.xprev, gath_rp(s, f), gath_cf2(s, f), gath_cf(s, f), f.r,
+                                                   cf ? f.rr : nullptr, f.cmask);
         s->chk_launched++;
-        HIPCHK(hipGetLastError());
-        k_revert<<<nbv, 256, 0, s->stream>>>(f.V, f.x, f.xprev, f.U1, f.U2, f.vc, s->vstate, gath_rp(s, f),
-                                             gath_cf2(s, f), gath_cf(s, f), f.r, cf ? f.rr : nullptr, f.cmask);
         HIPCHK(hipGetLastError());
         return MGSB_OK;
     }
@@ -3470,6 +3535,9 @@ int validate_params(const mgsb_params* p, int nlevels) {
 }
 
 int invalidate_graph(mgsb_solver* s) {
+    // a step returns with its last graph possibly still running: no
+    // executable graph is destroyed before the stream has drained
+    if (s->stream && (s->gexec || !s->pseq.empty())) HIPCHK(hipStreamSynchronize(s->stream));
     if (s->gexec) HIPCHK(hipGraphExecDestroy(s->gexec));
     if (s->graph) HIPCHK(hipGraphDestroy(s->graph));
     s->gexec = nullptr;
