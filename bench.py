@@ -45,10 +45,10 @@ VCYCLE = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, tol=1
 # velocities (seeds 1..32), where the extrapolation's error is largest, plus
 # 600 steps of one more (seed 40)
 TRAIN = dict(episodes=[(sd, 60) for sd in range(1, 33)] + [(40, 600)], epochs=30, batch=4096, lr=1e-3, seed=0)
-# HBM bytes per launch of the tiled local step on the medium box (V-cycle
-# split-position mode): rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
-# (scales from a 256 MiB copy), profiles/k_tet_tiled_medium_large_batched.md
-TRAFFIC = {"medium_box_12x120x12": 7.68e6}
+# HBM bytes per launch of the dominant kernel, (workload, kernel) ->
+# rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB
+# copy), from profiles/fused_sweep_and_local_step_counters.md
+TRAFFIC = {}
 
 
 def log(*a):
@@ -346,7 +346,7 @@ def main():
         results[name] = dict(steps_per_s=round(sps, 3), ms_per_step=round(1e3 * elapsed / args.steps, 4))
         if tuned:
             results[name]["autotuned"] = {k: tuned[k] for k in ("block_tet", "block_vert", "pre_sweeps", "post_sweeps",
-                                                                 "vert_lanes")}
+                                                                 "vert_lanes", "tile_gather", "fuse_sweeps")}
         if run:
             results[name].update(mean_cycles=round(run["total_cycles"] / max(1, run["steps"]), 3),
                                  mean_half_cycles=round(run["total_halves"] / max(1, run["steps"]), 3),
@@ -374,15 +374,23 @@ def main():
     s = solvers[head]
     steps_per_s = results[head]["steps_per_s"]
 
-    # dominant kernel: the finest-level local step as the schedule launches
-    # it, timed over REPS back-to-back launches between two HIP events on the
-    # solver's stream
+    # dominant kernel, timed over REPS back-to-back launches between two HIP
+    # events on the solver's stream: the fused fine-level sweep (vertex update
+    # and local step in one launch; one PD sweep of the level, 56 B per tet +
+    # 68 B per vertex) where the schedule fuses its sweeps, else the
+    # finest-level local step (56 B per tet + 12 B per vertex)
+    import multigridsoftbody_amd as M
+
     T, V = levels[0].ntets, levels[0].nverts
     reps = 200
-    avg_us = s.time_kernel(reps)
-    alg_bytes = 56 * T + 12 * V
+    try:
+        avg_us = s.time_kernel(reps, kernel=1)
+        kname, alg_bytes = "k_sweep_fused (fused fine-level sweep: vertex update + local step)", 56 * T + 68 * V
+    except M.MgsbError:
+        avg_us = s.time_kernel(reps, kernel=0)
+        kname, alg_bytes = "k_tet (finest-level local step)", 56 * T + 12 * V
     achieved = alg_bytes / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
-    log(f"[bench] finest-level local step: {reps} back-to-back launches, avg {avg_us:.2f} us")
+    log(f"[bench] {kname.split()[0]}: {reps} back-to-back launches, avg {avg_us:.2f} us")
 
     cpu = None
     if not args.no_cpu:
@@ -435,8 +443,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": TRAFFIC.get(cfg["name"]) if head in ("vcycle", "temporal", "learned", "learned_s") else None,
-            "kernel": "k_tet (finest-level local step)",
+            "traffic": TRAFFIC.get((cfg["name"], kname.split()[0])),
+            "kernel": kname,
             "avg_us": round(avg_us, 3),
             "timing": f"{reps} back-to-back launches between two HIP events on the solver's stream",
             "alg_bytes_per_launch": alg_bytes,

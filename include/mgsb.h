@@ -137,6 +137,15 @@ typedef struct mgsb_params {
                            * coarse-correction halves alternate (0, default: pre-smoothing
                            * first, the textbook V-cycle).  After a warm start the error
                            * left is mostly smooth, so the coarse correction pays first. */
+    int32_t fuse_sweeps;  /* V-cycle fine-level sweeps with tile_gather: each sweep's vertex
+                           * update runs inside the next local-step launch, every tile
+                           * updating its own vertices; needs every vertex in a tile of at
+                           * most 256 vertices and vert_lanes <= 1.  0: separate launches;
+                           * 1 (default): fused on fine levels of at most 65,536 vertices,
+                           * where launch latency dominates (each tile repeats the updates
+                           * of its ~2.5x shared vertices, which costs more than a launch
+                           * on larger meshes); 2: fused at any size.  The results are
+                           * identical either way. */
 } mgsb_params;
 
 typedef struct mgsb_step_stats {
@@ -390,6 +399,7 @@ typedef struct mgsb_tune_result {
     int32_t tile_gather;
     double sweep_us;             /* one fine sweep, chosen shape               */
     double sweep_us_default;     /* ... and the shape before tuning            */
+    int32_t fuse_sweeps;         /* 1: the chosen shape fuses the fine sweeps  */
 } mgsb_tune_result;
 int mgsb_autotune(mgsb_solver* s, int32_t trial_steps, mgsb_tune_result* out);
 
@@ -419,9 +429,12 @@ int mgsb_level_info(mgsb_solver* s, int32_t level, int32_t* nverts, int32_t* nte
 int mgsb_set_timing(mgsb_solver* s, int32_t on);
 int mgsb_kernel_time(mgsb_solver* s, int32_t kernel, double* ms, int64_t* launches);
 /* Throughput timing of kernel 0 as the current schedule launches it (split
- * positions for the V-cycle schedules): reps back-to-back launches on the
+ * positions for the V-cycle schedules), or of kernel 1, the fused fine-level
+ * sweep (vertex update and local step in one launch; V-cycle schedules with
+ * fuse_sweeps on a level that fuses): reps back-to-back launches on the
  * solver's stream between two HIP events; avg_us = elapsed / reps.  The
- * launches overwrite the corner forces only (the next step recomputes them). */
+ * launches overwrite scratch only (corner forces or partials, the spare
+ * position buffer), which the next step recomputes. */
 int mgsb_time_kernel(mgsb_solver* s, int32_t kernel, int32_t reps, double* avg_us);
 
 /* Test entry point: the per-tet local step of level `level` evaluated at

@@ -76,6 +76,7 @@ class mgsb_params(C.Structure):
         ("coarse_dense", C.c_int32),
         ("tile_gather", C.c_int32),
         ("coarse_first", C.c_int32),
+        ("fuse_sweeps", C.c_int32),
     ]
 
 
@@ -100,7 +101,7 @@ class mgsb_tune_result(C.Structure):
     _fields_ = [("block_tet", C.c_int32), ("block_vert", C.c_int32), ("pre_sweeps", C.c_int32),
                 ("post_sweeps", C.c_int32), ("ms_per_step", C.c_double), ("ms_per_step_default", C.c_double),
                 ("trials", C.c_int32), ("vert_lanes", C.c_int32), ("tile_gather", C.c_int32),
-                ("sweep_us", C.c_double), ("sweep_us_default", C.c_double)]
+                ("sweep_us", C.c_double), ("sweep_us_default", C.c_double), ("fuse_sweeps", C.c_int32)]
 
 
 class mgsb_energy_t(C.Structure):
@@ -590,6 +591,7 @@ class Solver:
         for k in ("block_tet", "block_vert", "pre_sweeps", "post_sweeps", "vert_lanes",
                   "tile_gather"):  # keep the host copy in step
             setattr(self.params, k, getattr(r, k))
+        self.params.fuse_sweeps = 2 if r.fuse_sweeps else 0
         return {k: getattr(r, k) for k, _ in r._fields_}
 
     def body_count(self):
@@ -618,7 +620,8 @@ class Solver:
         return out.reshape(-1, 4, 3)
 
     def time_kernel(self, reps=200, kernel=0):
-        """Average microseconds of `reps` back-to-back launches of kernel 0."""
+        """Average microseconds of `reps` back-to-back launches of kernel 0 (the
+        finest local step) or 1 (the fused fine-level sweep)."""
         us = C.c_double()
         _check(lib.mgsb_time_kernel(self._h, kernel, reps, C.byref(us)), "mgsb_time_kernel")
         return us.value

@@ -164,6 +164,7 @@ void build_tiles(HostLevel& L) {
     L.tvo.assign((size_t)nt + 1, 0);
     L.tlr.clear();
     L.tls.assign((size_t)T * 2, 0u);
+    L.tlv.assign((size_t)T * 2, 0u);
     std::vector<int32_t> local((size_t)V, -1), cnt((size_t)V, 0), all, verts, fill;
     for (int k = 0; k < nt; k++) {
         const int t0 = k * kTileTets, t1 = std::min(T, t0 + kTileTets);
@@ -184,8 +185,10 @@ void build_tiles(HostLevel& L) {
         L.tlr.insert(L.tlr.end(), fill.begin(), fill.end());
         for (int t = t0; t < t1; t++)
             for (int j = 0; j < 4; j++) {
-                const uint32_t pos = (uint32_t)fill[(size_t)local[L.tets[(size_t)t * 4 + j]]]++;
+                const int lv = local[L.tets[(size_t)t * 4 + j]];
+                const uint32_t pos = (uint32_t)fill[(size_t)lv]++;
                 L.tls[(size_t)t * 2 + (j >> 1)] |= pos << (16 * (j & 1));
+                L.tlv[(size_t)t * 2 + (j >> 1)] |= (uint32_t)lv << (16 * (j & 1));
             }
         for (int v : verts) {
             cnt[v]++;
@@ -199,6 +202,7 @@ void build_tiles(HostLevel& L) {
     std::vector<int32_t> at(L.prp.begin(), L.prp.end() - 1);
     L.tvs.resize(all.size());
     for (size_t q = 0; q < all.size(); q++) L.tvs[q] = at[all[q]]++;  // tiles ascending in each row
+    L.tvid = all;
 }
 
 int build_level(const mgsb_mesh& m, float stiffness, float density, HostLevel& L) {
@@ -934,6 +938,7 @@ void mgsb_default_params(mgsb_params* p) {
     p->coarse_dense = 1;
     p->tile_gather = 1;
     p->coarse_first = 0;
+    p->fuse_sweeps = 1;
 }
 
 const char* mgsb_version(void) { return "mgsb 0.1 (gfx950)"; }

@@ -48,6 +48,10 @@ struct HostLevel {
     std::vector<int32_t> tvs;     // tvo[ntiles]
     std::vector<int32_t> tlr;     // tvo[ntiles] + ntiles
     std::vector<uint32_t> tls;    // 2T
+    // the tiles' vertex lists in tile order (tvo ranges), and each corner's
+    // index in its tile's list (16 bits each, packed as tls)
+    std::vector<int32_t> tvid;    // tvo[ntiles]
+    std::vector<uint32_t> tlv;    // 2T
     std::vector<float> tk;        // per-tet ARAP weight per unit volume (empty: the scalar stiffness)
     int bad_tet = -1;
 };

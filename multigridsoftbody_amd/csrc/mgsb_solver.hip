@@ -331,27 +331,12 @@ __device__ __forceinline__ void stable_forces(const float (&RF)[9], const float 
 // [-10, 10], times vol * k (calculateIF, PDSolver.cu:128-183; calTetIFV1's
 // |f| < 1e-5 drop in Ac mode, PDSolverAc.cu:630-662).  Corner j's force goes
 // to fo[3j..3j+2].  sref != nullptr: split positions (x holds u, see below).
+// The local step from the corners' positions p0..p3 (and, split, their s0..s3),
+// Dm^-1 in b, rest volume vt and stiffness kk.
 template <int MODE>
-__device__ __forceinline__ void tet_local_forces(int g, int T, int4 id, const float4* __restrict__ x,
-                                                 const float* __restrict__ B, const float* __restrict__ vol,
-                                                 float k, const float4* __restrict__ sref,
-                                                 const float* __restrict__ tk, float (&fo)[12]) {
-    // every load is issued up front: the per-tet streams and the index (id,
-    // loaded by the caller) in the first round trip, the corner gathers of x
-    // (and s) in the second
-    float b[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) b[q] = B[(size_t)q * T + g];
-    const float vt = vol[g];
-    const float kk = tk ? tk[g] : k;  // per-body stiffness when several bodies have their own E
-    const float4 p0 = x[id.x], p1 = x[id.y], p2 = x[id.z], p3 = x[id.w];
-    float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0, s2 = s0, s3 = s0;
-    if (sref) {
-        s0 = sref[id.x];
-        s1 = sref[id.y];
-        s2 = sref[id.z];
-        s3 = sref[id.w];
-    }
+__device__ __forceinline__ void tet_forces_at(const float (&b)[9], float vt, float kk, float4 p0, float4 p1,
+                                              float4 p2, float4 p3, bool split, float4 s0, float4 s1, float4 s2,
+                                              float4 s3, float (&fo)[12]) {
     float D[9];
     D[0] = p1.x - p0.x; D[1] = p2.x - p0.x; D[2] = p3.x - p0.x;
     D[3] = p1.y - p0.y; D[4] = p2.y - p0.y; D[5] = p3.y - p0.y;
@@ -361,7 +346,7 @@ __device__ __forceinline__ void tet_local_forces(int g, int T, int4 id, const fl
     // the small in-step correction keeps its low bits; at u = 0 it is Ds(s)
     // exactly, the same F(s) in every pass of the step.  One product with
     // Dm^-1 instead of F(s) + Ds(u) Dm^-1 (45 fewer instructions per tet).
-    if (sref) {
+    if (split) {
         D[0] = (s1.x - s0.x) + D[0]; D[1] = (s2.x - s0.x) + D[1]; D[2] = (s3.x - s0.x) + D[2];
         D[3] = (s1.y - s0.y) + D[3]; D[4] = (s2.y - s0.y) + D[4]; D[5] = (s3.y - s0.y) + D[5];
         D[6] = (s1.z - s0.z) + D[6]; D[7] = (s2.z - s0.z) + D[7]; D[8] = (s3.z - s0.z) + D[8];
@@ -413,6 +398,30 @@ __device__ __forceinline__ void tet_local_forces(int g, int T, int4 id, const fl
             fo[j * 3 + i] = r;
         }
     }
+}
+
+template <int MODE>
+__device__ __forceinline__ void tet_local_forces(int g, int T, int4 id, const float4* __restrict__ x,
+                                                 const float* __restrict__ B, const float* __restrict__ vol,
+                                                 float k, const float4* __restrict__ sref,
+                                                 const float* __restrict__ tk, float (&fo)[12]) {
+    // every load is issued up front: the per-tet streams and the index (id,
+    // loaded by the caller) in the first round trip, the corner gathers of x
+    // (and s) in the second
+    float b[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) b[q] = B[(size_t)q * T + g];
+    const float vt = vol[g];
+    const float kk = tk ? tk[g] : k;  // per-body stiffness when several bodies have their own E
+    const float4 p0 = x[id.x], p1 = x[id.y], p2 = x[id.z], p3 = x[id.w];
+    float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0, s2 = s0, s3 = s0;
+    if (sref) {
+        s0 = sref[id.x];
+        s1 = sref[id.y];
+        s2 = sref[id.z];
+        s3 = sref[id.w];
+    }
+    tet_forces_at<MODE>(b, vt, kk, p0, p1, p2, p3, sref != nullptr, s0, s1, s2, s3, fo);
 }
 
 // The local step of tet g with corner j's force stored to cf[slot_j] (the
@@ -616,38 +625,27 @@ __device__ __forceinline__ float relax_step(float d, float pos, float relax_hi, 
     return sh + (se + pe);
 }
 
-// Per-vertex update: gather the CSR range of corner forces (tet order), add
-// collision response, then calculatePOS (PDSolver.cu:192-235): Jacobi step,
+// Vertex update of vertex i from its summed corner forces fg: collision
+// response, then calculatePOS (PDSolver.cu:192-235): Jacobi step,
 // under-relaxation, Chebyshev extrapolation with omega.  Sphere point DCD is
 // DCDByPoint_sphere (PDSolver.cu:277-331) including its 1.5x radius and t/2.
-__global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, float4* __restrict__ xprev,
-                                              const float4* __restrict__ s, const float4* __restrict__ vc,
-                                              const int* __restrict__ rp, const float3* __restrict__ cf,
+// p = x (or the correction u when split), pp = x_prev, st = s, c = (m/h^2,
+// diag, fix, m).  Returns the new position (p itself with kVertDcdOnly) and
+// the vertex's collision flag, depth and energy.
+__device__ __forceinline__ float4 vertex_step(float4 p, float4 pp, float4 st, float4 c, float3 fg, int i,
                                               const int* __restrict__ trp, const float4* __restrict__ tcf,
                                               const DevColliders* __restrict__ col, float omega, float relax_hi,
-                                              float relax_lo, int flags, int* __restrict__ cflag, float* __restrict__ cdepth,
-                                              float* __restrict__ cenergy, const int* __restrict__ skip, int split,
-                                              int lsh) {
-    if (skip && *skip) return;
-    // 2^lsh lanes per vertex gather its row (gather_lanes); lane 0 finishes
-    const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const int i = g >> lsh, sub = g & ((1 << lsh) - 1);
-    if (i >= V) return;
-    // per-vertex streams in the same round trip as the row bounds
-    const float4 p = x[i];  // x, or the correction u when split
-    const float4 st = s[i];
-    const float4 c = vc[i];  // (m/h^2, diag, fix, m)
-    const float4 pp = xprev[i];
-    const float3 fg = gather_lanes(cf, rp[i], rp[i + 1], sub, lsh);
-    if (sub) return;
+                                              float relax_lo, int flags, int split, int& flag, float& depth,
+                                              float& energy) {
     const float fx = fg.x, fy = fg.y, fz = fg.z;
     // position seen by the colliders, and s - x of the inertial term
     const float4 pa = split ? make_float4(st.x + p.x, st.y + p.y, st.z + p.z, 0.f) : p;
     const float sx = split ? -p.x : st.x - p.x, sy = split ? -p.y : st.y - p.y, sz = split ? -p.z : st.z - p.z;
     float cdx = 0.f, cdy = 0.f, cdz = 0.f, cfx = 0.f, cfy = 0.f, cfz = 0.f;
     float cxy = 0.f, cxz = 0.f, cyz = 0.f;  // off-diagonals of the contacts' k n n^T
-    int flag = 0;
-    float depth = 0.f, energy = 0.f;
+    flag = 0;
+    depth = 0.f;
+    energy = 0.f;
     if (flags & kVertTri) {
         const int t0 = trp[i], t1 = trp[i + 1];
         for (int e = t0; e < t1; e++) {
@@ -725,12 +723,7 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
             }
         }
     }
-    if (flags & kVertWriteCol) {
-        cflag[i] = flag;
-        cdepth[i] = depth;
-        cenergy[i] = energy;
-    }
-    if (flags & kVertDcdOnly) return;
+    if (flags & kVertDcdOnly) return p;
     const float dc = c.x, dg = c.y, fix = c.z;
     if ((flags & kVertContactJacobi) && (cdx + cdy + cdz) > 0.f) {
         // V-cycle sweeps, a vertex a collider acts on: the Jacobi step solves
@@ -751,9 +744,7 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
         const float bx = relax_step(dx * fix + p.x - p.x, p.x, relax_hi, relax_lo);
         const float by = relax_step(dy * fix + p.y - p.y, p.y, relax_hi, relax_lo);
         const float bz = relax_step(dz * fix + p.z - p.z, p.z, relax_hi, relax_lo);
-        xprev[i] = p;
-        x[i] = make_float4(bx, by, bz, 0.f);
-        return;
+        return make_float4(bx, by, bz, 0.f);
     }
     float nx = (dc * sx + (fx + cfx)) / (dg + cdx + dc) * fix + p.x;
     float ny = (dc * sy + (fy + cfy)) / (dg + cdy + dc) * fix + p.y;
@@ -764,8 +755,136 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
     nx = omega * (nx - pp.x) + pp.x;
     ny = omega * (ny - pp.y) + pp.y;
     nz = omega * (nz - pp.z) + pp.z;
+    return make_float4(nx, ny, nz, 0.f);
+}
+
+
+// Per-vertex update kernel: gather the CSR range of corner forces (tet order,
+// or the tiles' partials), then vertex_step.
+__global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, float4* __restrict__ xprev,
+                                              const float4* __restrict__ s, const float4* __restrict__ vc,
+                                              const int* __restrict__ rp, const float3* __restrict__ cf,
+                                              const int* __restrict__ trp, const float4* __restrict__ tcf,
+                                              const DevColliders* __restrict__ col, float omega, float relax_hi,
+                                              float relax_lo, int flags, int* __restrict__ cflag, float* __restrict__ cdepth,
+                                              float* __restrict__ cenergy, const int* __restrict__ skip, int split,
+                                              int lsh) {
+    if (skip && *skip) return;
+    // 2^lsh lanes per vertex gather its row (gather_lanes); lane 0 finishes
+    const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int i = g >> lsh, sub = g & ((1 << lsh) - 1);
+    if (i >= V) return;
+    // per-vertex streams in the same round trip as the row bounds
+    const float4 p = x[i];  // x, or the correction u when split
+    const float4 st = s[i];
+    const float4 c = vc[i];  // (m/h^2, diag, fix, m)
+    const float4 pp = xprev[i];
+    const float3 fg = gather_lanes(cf, rp[i], rp[i + 1], sub, lsh);
+    if (sub) return;
+    int flag;
+    float depth, energy;
+    const float4 q = vertex_step(p, pp, st, c, fg, i, trp, tcf, col, omega, relax_hi, relax_lo, flags, split, flag,
+                                 depth, energy);
+    if (flags & kVertWriteCol) {
+        cflag[i] = flag;
+        cdepth[i] = depth;
+        cenergy[i] = energy;
+    }
+    if (flags & kVertDcdOnly) return;
     xprev[i] = p;
-    x[i] = make_float4(nx, ny, nz, 0.f);
+    x[i] = q;
+}
+
+// One sweep of the V-cycle's fine smoother in one launch (tiled layout,
+// split positions).  Phase 1, over the tile's vertices: the vertex update of
+// k_vert from the previous local step's partials pin (JACOBI), or the coarse
+// correction of k_prolong_add (PROLONG), into LDS with the vertex's s; the
+// vertex's first tile (its partial leads its row of prp) stores it to xout.
+// Phase 2, over the tile's tets: the local step at the new positions read
+// from LDS, its partials to pout as k_tet_tiled writes them.  A vertex shared
+// by several tiles is updated by each of them from the same inputs in the
+// same order, so every copy equals the owner's.  Positions rotate over three
+// buffers (xcur and xprv read, xout written) and partials over two, so no
+// launch reads what it writes.  tetq: per corner, its tile-local vertex index
+// (low 16 bits) and its position in the tile's corner array (high 16).
+constexpr int kFuseJacobi = 0, kFuseProlong = 1;
+constexpr int kFuseMaxNv = 256;  // vertices of a tile the fused sweep holds in LDS
+template <int MODE, int PHASE>
+__global__ void __launch_bounds__(kTileTets) k_sweep_fused(
+    int T, const float4* __restrict__ xcur, const float4* __restrict__ xprv, float4* __restrict__ xout,
+    const float4* __restrict__ s, const float4* __restrict__ vc, const int* __restrict__ prp,
+    const float3* __restrict__ pin, const int* __restrict__ tvo, const int* __restrict__ tvs,
+    const int* __restrict__ tlr, const int* __restrict__ tvid, const int4* __restrict__ tetq,
+    const float* __restrict__ B, const float* __restrict__ vol, float k, const float* __restrict__ tk,
+    float3* __restrict__ pout, const DevColliders* __restrict__ col, int flags, float omega, float relax_hi,
+    float relax_lo, int* __restrict__ cflag, float* __restrict__ cdepth, float* __restrict__ cenergy,
+    const float4* __restrict__ ec, const int4* __restrict__ iids, const float4* __restrict__ iw,
+    const float* __restrict__ amask, const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    __shared__ float4 lu[kFuseMaxNv], ls[kFuseMaxNv];  // the tile's new positions and s
+    __shared__ TileLds L;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int v0 = tvo[blk], nv = tvo[blk + 1] - v0;  // nv <= kFuseMaxNv (checked at set-up)
+    const int g = blk * kTileTets + threadIdx.x;
+    // the tet's streams first: in flight during phase 1
+    int4 q = make_int4(0, 0, 0, 0);
+    float b[9];
+    float vt = 0.f, kk = k;
+    if (g < T) {
+        q = tetq[g];
+#pragma unroll
+        for (int c = 0; c < 9; c++) b[c] = B[(size_t)c * T + g];
+        vt = vol[g];
+        if (tk) kk = tk[g];
+    }
+    for (int l = threadIdx.x; l < nv; l += kTileTets) {
+        const int v = tvid[v0 + l];
+        const float4 st = s[v];
+        const int r0 = prp[v];
+        const bool owner = tvs[v0 + l] == r0;
+        float4 pn;
+        if (PHASE == kFuseProlong) {
+            const float3 e = prolong_at(v, ec, iids, iw);
+            const float m = amask ? vc[v].z * amask[v] : vc[v].z;
+            pn = xcur[v];
+            pn.x = pn.x + m * e.x;
+            pn.y = pn.y + m * e.y;
+            pn.z = pn.z + m * e.z;
+        } else {
+            const float4 p = xcur[v], pp = xprv[v], c = vc[v];
+            float fx = 0.f, fy = 0.f, fz = 0.f;
+            gather_sum(pin, r0, prp[v + 1], fx, fy, fz);
+            int flag;
+            float depth, energy;
+            pn = vertex_step(p, pp, st, c, make_float3(fx, fy, fz), v, nullptr, nullptr, col, omega, relax_hi,
+                             relax_lo, flags & ~kVertTri, 1, flag, depth, energy);
+            if (owner && (flags & kVertWriteCol)) {
+                cflag[v] = flag;
+                cdepth[v] = depth;
+                cenergy[v] = energy;
+            }
+        }
+        lu[l] = pn;
+        ls[l] = st;
+        if (owner) xout[v] = pn;
+    }
+    __syncthreads();
+    if (g < T) {
+        const int lv[4] = {q.x & 0xffff, q.y & 0xffff, q.z & 0xffff, q.w & 0xffff};
+        float fo[12];
+        tet_forces_at<MODE>(b, vt, kk, lu[lv[0]], lu[lv[1]], lu[lv[2]], lu[lv[3]], true, ls[lv[0]], ls[lv[1]],
+                            ls[lv[2]], ls[lv[3]], fo);
+        const int pos[4] = {(int)((unsigned)q.x >> 16), (int)((unsigned)q.y >> 16), (int)((unsigned)q.z >> 16),
+                            (int)((unsigned)q.w >> 16)};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            L.x[pos[j]] = fo[j * 3 + 0];
+            L.y[pos[j]] = fo[j * 3 + 1];
+            L.z[pos[j]] = fo[j * 3 + 2];
+        }
+    }
+    __syncthreads();
+    tile_sum(L, blk, tvo, tvs, tlr, pout);
 }
 
 // updatePointInTet (PDSolver.cu:724-744): dst.x = dst.x_prev = sum w src.x.
@@ -1213,18 +1332,25 @@ __device__ __forceinline__ float3 vertex_residual(float4 p, float4 st, float4 c,
 // Nonlinear residual over free DOFs (vertex_residual); writes g (float4) when
 // r != nullptr and the FP64 block partials of |g|^2 when part != nullptr
 // (k_check completes the norm).
-__global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__ x, const float4* __restrict__ s,
+// After fused sweeps the positions sit in a rotated buffer: xcanon != nullptr
+// takes x (the level's x buffer), and xprev takes xpsrc[i] instead of x when
+// xpsrc != nullptr (the level's x_prev buffer keeps the smoother's history).
+// These buffers may alias each other and x: no __restrict__ on them, and every
+// vertex reads before it writes.
+__global__ void __launch_bounds__(256) k_resid(int V, const float4* x, const float4* __restrict__ s,
                                                const float4* __restrict__ vc, const int* __restrict__ rp,
                                                const float3* __restrict__ cf, const DevColliders* __restrict__ col,
                                                int flags, float4* __restrict__ r, double* __restrict__ part,
-                                               float4* __restrict__ xprev, const int* __restrict__ skip, int split,
-                                               double* __restrict__ g2, float* __restrict__ cmask, int lsh) {
+                                               float4* xprev, const int* __restrict__ skip, int split,
+                                               double* __restrict__ g2, float* __restrict__ cmask, int lsh,
+                                               float4* xcanon = nullptr, const float4* xpsrc = nullptr) {
     if (skip && *skip) return;
     const int gi = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gi >> lsh, sub = gi & ((1 << lsh) - 1);  // 2^lsh lanes per vertex
     double v[1] = {0.0};
     if (i < V) {
         const float4 p = x[i], st = s[i], c = vc[i];
+        const float4 pq = (xpsrc && sub == 0) ? xpsrc[i] : p;
         const float3 f = gather_lanes(cf, rp[i], rp[i + 1], sub, lsh);
         if (sub == 0) {
             bool contact;
@@ -1234,7 +1360,8 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* __restrict__
             const float am = contact ? 0.f : 1.f;
             if (cmask) cmask[i] = am;
             if (r) r[i] = cmask ? make_float4(g.x * am, g.y * am, g.z * am, 0.f) : make_float4(g.x, g.y, g.z, 0.f);
-            if (xprev) xprev[i] = p;
+            if (xcanon) xcanon[i] = p;
+            if (xprev) xprev[i] = pq;
             v[0] = (double)g.x * g.x + (double)g.y * g.y + (double)g.z * g.z;
             if (g2) g2[i] = v[0];
         }
@@ -2182,6 +2309,12 @@ struct DevLevel {
     int *prp = nullptr, *tvo = nullptr, *tvs = nullptr, *tlr = nullptr;
     int4* tetp = nullptr;  // tets with each corner's position in its tile's corner array (nullptr: not tiled)
     float3 *pcf = nullptr, *pcf2 = nullptr;
+    // fused sweeps (k_sweep_fused): per corner its tile-local vertex index and
+    // corner position; the tiles' vertex lists; a third position buffer
+    // (nullptr: the level does not run fused sweeps)
+    int4* tetq = nullptr;
+    int* tvid = nullptr;
+    float4* xz = nullptr;
     int4 *tri = nullptr, *tslot = nullptr;
     int *trp = nullptr, *trid = nullptr;
     float4 *tcf = nullptr, *tn = nullptr, *vn = nullptr;
@@ -2405,6 +2538,26 @@ int upload_level(mgsb_solver* s, DevLevel& d) {
             tp[t] = make_int4(c[0], c[1], c[2], c[3]);
         }
         if ((rc = up(s, d.owned, &d.tetp, tp.data(), tp.size()))) return rc;
+        // fused sweeps hold a tile's vertices in LDS and need every vertex in a tile
+        int maxnv = 0;
+        for (int k = 0; k < d.ntiles; k++) maxnv = std::max(maxnv, h.tvo[(size_t)k + 1] - h.tvo[(size_t)k]);
+        bool covered = true;
+        for (int v = 0; v < h.V && covered; v++) covered = h.prp[(size_t)v + 1] > h.prp[(size_t)v];
+        if (maxnv <= kFuseMaxNv && covered) {
+            std::vector<int4> tq(h.T);
+            for (int t = 0; t < h.T; t++) {
+                int c[4];
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t sh = 16 * (j & 1);
+                    const uint32_t pos = (h.tls[(size_t)t * 2 + (j >> 1)] >> sh) & 0xffffu;
+                    const uint32_t lv = (h.tlv[(size_t)t * 2 + (j >> 1)] >> sh) & 0xffffu;
+                    c[j] = (int)(lv | (pos << 16));
+                }
+                tq[t] = make_int4(c[0], c[1], c[2], c[3]);
+            }
+            if ((rc = up(s, d.owned, &d.tetq, tq.data(), tq.size()))) return rc;
+            if ((rc = up(s, d.owned, &d.tvid, h.tvid.data(), h.tvid.size()))) return rc;
+        }
     }
     std::vector<int4> tr4(std::max(1, h.S)), ts4(std::max(1, h.S));
     for (int t = 0; t < h.S; t++) {
@@ -2590,14 +2743,18 @@ double decimal_double(float f) {
     return (double)f;
 }
 
-int launch_vert(mgsb_solver* s, DevLevel& d, float omega, int flags, const int* skip = nullptr, int split = 0) {
-    const int bs = block_vert(s), lsh = vert_lsh(s);
+void relax_consts(mgsb_solver* s) {
     if (!(s->p.relax == s->relax_key)) {  // the relaxation constant as a double word
         s->relax_key = s->p.relax;
         const double r = decimal_double(s->p.relax);
         s->relax_hi = (float)r;
         s->relax_lo = (float)(r - (double)s->relax_hi);
     }
+}
+
+int launch_vert(mgsb_solver* s, DevLevel& d, float omega, int flags, const int* skip = nullptr, int split = 0) {
+    const int bs = block_vert(s), lsh = vert_lsh(s);
+    relax_consts(s);
     k_vert<<<nblk(d.V << lsh, bs), bs, 0, s->stream>>>(d.V, d.x, d.xprev, d.s, d.vc, gath_rp(s, d), gath_cf(s, d),
                                                         d.trp, d.tcf, s->dcol,
                                                         omega, s->relax_hi, s->relax_lo, flags, d.cflag, d.cdepth,
@@ -2752,20 +2909,83 @@ int vcycle_lin(mgsb_solver* s, int l, const int* skip) {
     return lin_smooth(s, l, s->p.coarse_sweeps, false, skip);
 }
 
+// Fused fine-level sweeps (k_sweep_fused) run on a tiled level set up for
+// them, with one lane per vertex.  Within a half-cycle the positions rotate
+// over x, x_prev and xz and the partials over pcf and pcf2; a half-cycle
+// starts with u in x (x_prev its history) and the partials at x in pcf, and
+// its residual kernel puts x and x_prev back.
+constexpr int kFuseAutoMaxV = 65536;
+bool fused(const mgsb_solver* s, const DevLevel& d) {
+    const bool want = s->p.fuse_sweeps == 2 || (s->p.fuse_sweeps == 1 && d.V <= kFuseAutoMaxV);
+    return want && tiled(s, d) && d.tetq && d.xz && vert_lsh(s) == 0;
+}
+struct FuseBufs {
+    float4* cur;   // positions (u) now
+    float4* prev;  // the smoother's previous iterate
+    int pin;       // the partials at cur: 0 pcf, 1 pcf2
+};
+float4* fuse_spare(DevLevel& d, const float4* a, const float4* b) {
+    for (float4* q : {d.x, d.xprev, d.xz})
+        if (q != a && q != b) return q;
+    return nullptr;
+}
+template <int MODE, int PHASE>
+void launch_fused_mode(mgsb_solver* s, DevLevel& d, const FuseBufs& fb, float4* out, float omega, int flags,
+                       const int* skip, const float4* ec, const float* amask) {
+    LevelMap& m = s->M[0];
+    k_sweep_fused<MODE, PHASE><<<d.ntiles, kTileTets, 0, s->stream>>>(
+        d.T, fb.cur, fb.prev, out, d.s, d.vc, d.prp, fb.pin ? d.pcf2 : d.pcf, d.tvo, d.tvs, d.tlr, d.tvid, d.tetq,
+        d.B, d.vol, s->p.stiffness, d.tk, fb.pin ? d.pcf : d.pcf2, s->dcol, flags, omega, s->relax_hi, s->relax_lo,
+        d.cflag, d.cdepth, d.cenergy, ec, m.iids, m.iw, amask, skip);
+}
+// One fused sweep: JACOBI (vertex update with omega, then the local step) or
+// PROLONG (x += mask P ec, then the local step; the smoother restarts).
+template <int PHASE>
+int launch_fused(mgsb_solver* s, DevLevel& d, FuseBufs& fb, float omega, int flags, const int* skip,
+                 const float4* ec = nullptr, const float* amask = nullptr) {
+    relax_consts(s);
+    float4* out = fuse_spare(d, fb.cur, fb.prev);
+    switch (tet_mode_of(s->p.if_mode)) {
+        case kTetModePolar: launch_fused_mode<kTetModePolar, PHASE>(s, d, fb, out, omega, flags, skip, ec, amask); break;
+        case kTetModeAc: launch_fused_mode<kTetModeAc, PHASE>(s, d, fb, out, omega, flags, skip, ec, amask); break;
+        default: launch_fused_mode<kTetModeStable, PHASE>(s, d, fb, out, omega, flags, skip, ec, amask);
+    }
+    HIPCHK(hipGetLastError());
+    fb.prev = PHASE == kFuseProlong ? out : fb.cur;
+    fb.cur = out;
+    fb.pin ^= 1;
+    return MGSB_OK;
+}
+
 // Fine-level residual on split positions (x holds u = x - s).  first: the
 // step's first evaluation, at s itself.  chk: the check applied to the norm
-// (kChkNone: no norm).
+// (kChkNone: no norm).  fb != nullptr: after fused sweeps, the residual at
+// fb->cur from the partials fb->pin; x and x_prev return to their buffers
+// (x_prev: x with restart, else the smoother's previous iterate).
 int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool restart, const int* skip,
-                  bool first = false) {
+                  bool first = false, const FuseBufs* fb = nullptr) {
     DevLevel& d = s->L[0];
     int rc;
     if (need_tet && (rc = launch_tet(s, d, tet_mode_of(s->p.if_mode), false, first ? d.s : d.x, skip, !first)))
         return rc;
     const int lsh = vert_lsh(s), nb = nblk(d.V << lsh, 256);
+    const float4* xin = fb ? fb->cur : d.x;
+    const float3* cfin = fb ? (fb->pin ? d.pcf2 : d.pcf) : gath_cf(s, d);
+    float4* xcanon = (fb && fb->cur != d.x) ? d.x : nullptr;
+    float4* xpout = restart ? d.xprev : nullptr;
+    const float4* xpsrc = nullptr;
+    if (fb) {
+        if (restart) {
+            xpout = fb->cur != d.xprev ? d.xprev : nullptr;
+        } else if (fb->prev != d.xprev) {
+            xpout = d.xprev;
+            xpsrc = fb->prev;
+        }
+    }
     if (s->body_checks && chk != kChkNone) {  // per-body norms, then the check
-        k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, gath_rp(s, d), gath_cf(s, d), s->dcol, kVertPoint,
-                                           write_r ? d.r : nullptr, nullptr, restart ? d.xprev : nullptr, skip, 1,
-                                           s->g2, d.cmask, lsh);
+        k_resid<<<nb, 256, 0, s->stream>>>(d.V, xin, d.s, d.vc, gath_rp(s, d), cfin, s->dcol, kVertPoint,
+                                           write_r ? d.r : nullptr, nullptr, xpout, skip, 1, s->g2, d.cmask, lsh,
+                                           xcanon, xpsrc);
         HIPCHK(hipGetLastError());
         k_body_check<<<s->nbodies, kBodyCheckThreads, 0, s->stream>>>(s->vb, s->g2, s->bodysq, s->res0b, s->bcount,
                                                                       s->vstate, chk, s->hmirror, skip, s->seqc);
@@ -2773,9 +2993,9 @@ int fine_residual(mgsb_solver* s, bool need_tet, bool write_r, int chk, bool res
         HIPCHK(hipGetLastError());
         return MGSB_OK;
     }
-    k_resid<<<nb, 256, 0, s->stream>>>(d.V, d.x, d.s, d.vc, gath_rp(s, d), gath_cf(s, d), s->dcol, kVertPoint, write_r ? d.r : nullptr,
-                                       chk != kChkNone ? s->vpart : nullptr, restart ? d.xprev : nullptr, skip, 1,
-                                       nullptr, d.cmask, lsh);
+    k_resid<<<nb, 256, 0, s->stream>>>(d.V, xin, d.s, d.vc, gath_rp(s, d), cfin, s->dcol, kVertPoint,
+                                       write_r ? d.r : nullptr, chk != kChkNone ? s->vpart : nullptr, xpout, skip, 1,
+                                       nullptr, d.cmask, lsh, xcanon, xpsrc);
     HIPCHK(hipGetLastError());
     if (chk != kChkNone) {
         k_check<<<1, 256, 0, s->stream>>>(nb, s->vpart, s->vstate, chk, s->hmirror, skip, s->seqc);
@@ -2881,6 +3101,14 @@ int enqueue_vc_pre(mgsb_solver* s) {
     const float rs = s->p.rho_smooth;
     int rc;
     float omega = 1.0f;
+    if (fused(s, f)) {  // each sweep's vertex update inside the next local step's launch
+        FuseBufs fb{f.x, f.xprev, 0};
+        for (int k = 0; k < s->p.pre_sweeps; k++) {
+            omega = cheb_next(omega, rs);
+            if ((rc = launch_fused<kFuseJacobi>(s, f, fb, omega, kVertPoint | kVertContactJacobi, skip))) return rc;
+        }
+        return fine_residual(s, false, true, s->p.check_every == 2 ? kChkMid : kChkNone, false, skip, false, &fb);
+    }
     for (int k = 0; k < s->p.pre_sweeps; k++) {
         omega = cheb_next(omega, rs);
         if (k > 0 && (rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, true))) return rc;
@@ -2901,10 +3129,21 @@ int enqueue_vc_post(mgsb_solver* s) {
     k_restrict<<<nblk(c1.V * 64, 256), 256, 0, s->stream>>>(c1.V, c1.rtp, c1.rti, c1.rtw, f.r, c1.vcl, c1.r, skip);
     HIPCHK(hipGetLastError());
     if ((rc = vcycle_lin(s, 1, skip))) return rc;
+    float omega = 1.0f;
+    if (fused(s, f)) {  // the prolongation, and each sweep's vertex update, inside a local step's launch
+        // the partials alternate so that the last sweep's land in pcf
+        FuseBufs fb{f.x, f.xprev, (1 + s->p.post_sweeps) & 1};
+        if ((rc = launch_fused<kFuseProlong>(s, f, fb, 1.0f, 0, skip, cur_e(c1), f.cmask))) return rc;
+        for (int k = 0; k < s->p.post_sweeps; k++) {
+            omega = cheb_next(omega, rs);
+            const int fl = kVertPoint | kVertContactJacobi | (k == s->p.post_sweeps - 1 ? kVertWriteCol : 0);
+            if ((rc = launch_fused<kFuseJacobi>(s, f, fb, omega, fl, skip))) return rc;
+        }
+        return fine_residual(s, false, false, kChkCycle, true, skip, false, &fb);
+    }
     k_prolong_add<<<nblk(f.V, bs), bs, 0, s->stream>>>(f.V, f.x, f.xprev, cur_e(c1), s->M[0].iids, s->M[0].iw, f.vc,
                                                        skip, f.cmask);
     HIPCHK(hipGetLastError());
-    float omega = 1.0f;
     for (int k = 0; k < s->p.post_sweeps; k++) {
         omega = cheb_next(omega, rs);
         if ((rc = launch_tet(s, f, tet_mode_of(s->p.if_mode), false, nullptr, skip, true))) return rc;
@@ -3359,6 +3598,7 @@ int alloc_vcycle(mgsb_solver* s) {
             if ((rc = up(s, d.owned, &d.cmask, one.data(), d.V))) return rc;
             // the residual at s of a warm-started start, for a coarse-first revert
             if ((rc = up<float4>(s, d.owned, &d.rr, nullptr, d.V))) return rc;
+            if (d.tetq && !d.xz && (rc = up<float4>(s, d.owned, &d.xz, nullptr, d.V))) return rc;
             if ((rc = up<float3>(s, d.owned, &d.cf2, nullptr, (size_t)d.T * 4))) return rc;
             if ((rc = up<float3>(s, d.owned, &d.pcf2, nullptr, (size_t)d.npart))) return rc;
             continue;
@@ -3500,8 +3740,8 @@ int validate_params(const mgsb_params* p, int nlevels) {
         set_error("check_every must be 0 (no checks), 1 (after each cycle) or 2 (also after pre-smoothing)");
         return MGSB_ERR_ARG;
     }
-    if (p->coarse_first < 0 || p->coarse_first > 1) {
-        set_error("coarse_first must be 0 or 1");
+    if (p->coarse_first < 0 || p->coarse_first > 1 || p->fuse_sweeps < 0 || p->fuse_sweeps > 2) {
+        set_error("coarse_first must be 0 or 1, fuse_sweeps 0, 1 or 2");
         return MGSB_ERR_ARG;
     }
     if (nlevels > 8) {
@@ -4632,9 +4872,15 @@ int mgsb_autotune(mgsb_solver* s, int32_t trial_steps, mgsb_tune_result* out) {
                 HIPCHK(hipMemsetAsync(f.xprev, 0, sizeof(float4) * f.V, s->stream));
             }
             HIPCHK(hipEventRecord(ea, s->stream));
-            for (int k = 0; k < (pass ? reps : 1) && !r; k++) {
-                r = launch_tet(s, f, tmode, false, nullptr, nullptr, split != 0);
-                if (!r) r = launch_vert(s, f, 1.0f, vflags, nullptr, split);
+            if (vcycle && s->L.size() >= 2 && fused(s, f)) {  // the vertex update inside the local step
+                FuseBufs fb{f.x, f.xprev, 0};
+                for (int k = 0; k < (pass ? reps : 1) && !r; k++)
+                    r = launch_fused<kFuseJacobi>(s, f, fb, 1.0f, vflags, nullptr);
+            } else {
+                for (int k = 0; k < (pass ? reps : 1) && !r; k++) {
+                    r = launch_tet(s, f, tmode, false, nullptr, nullptr, split != 0);
+                    if (!r) r = launch_vert(s, f, 1.0f, vflags, nullptr, split);
+                }
             }
             HIPCHK(hipEventRecord(eb, s->stream));
         }
@@ -4647,14 +4893,15 @@ int mgsb_autotune(mgsb_solver* s, int32_t trial_steps, mgsb_tune_result* out) {
         return MGSB_OK;
     };
     struct Shape {
-        int tg, bt, bv, vl;
+        int tg, bt, bv, vl, fz;
     };
-    std::vector<Shape> shapes{{p0.tile_gather ? 1 : 0, p0.block_tet, p0.block_vert, p0.vert_lanes}};
+    std::vector<Shape> shapes{{p0.tile_gather ? 1 : 0, p0.block_tet, p0.block_vert, p0.vert_lanes, p0.fuse_sweeps}};
     for (int bv : {64, 128, 256})
-        for (int vl : {1, 2}) shapes.push_back({1, p0.block_tet, bv, vl});
+        for (int vl : {1, 2}) shapes.push_back({1, p0.block_tet, bv, vl, 0});
+    if (vcycle) shapes.push_back({1, p0.block_tet, p0.block_vert, 1, 2});  // fused sweeps
     for (int bt : {64, 128, 256})
         for (int bv : {64, 128, 256})
-            for (int vl : {2, 4, 8}) shapes.push_back({0, bt, bv, vl});
+            for (int vl : {2, 4, 8}) shapes.push_back({0, bt, bv, vl, 0});
     double base_us = 0.0, best_us = 1e300;
     Shape best = shapes[0];
     for (size_t q = 0; q < shapes.size(); q++) {
@@ -4662,6 +4909,7 @@ int mgsb_autotune(mgsb_solver* s, int32_t trial_steps, mgsb_tune_result* out) {
         s->p.block_tet = shapes[q].bt;
         s->p.block_vert = shapes[q].bv;
         s->p.vert_lanes = shapes[q].vl;
+        s->p.fuse_sweeps = shapes[q].fz;
         double us = 0.0;
         if ((rc = sweep_us(us))) break;
         if (q == 0) base_us = us;
@@ -4685,6 +4933,7 @@ int mgsb_autotune(mgsb_solver* s, int32_t trial_steps, mgsb_tune_result* out) {
     s->p.block_tet = best.bt;
     s->p.block_vert = best.bv;
     s->p.vert_lanes = best.vl;
+    s->p.fuse_sweeps = best.fz;
     // 2. Steps per second over a window of trial_steps steps that starts
     // trial_steps steps after the saved state (past the first steps, whose
     // residual histories differ most from the run's), each candidate from
@@ -4757,6 +5006,7 @@ int mgsb_autotune(mgsb_solver* s, int32_t trial_steps, mgsb_tune_result* out) {
     out->tile_gather = s->p.tile_gather;
     out->sweep_us = best_us;
     out->sweep_us_default = base_us;
+    out->fuse_sweeps = fused(s, s->L[0]) ? 1 : 0;
     return MGSB_OK;
 }
 
@@ -4825,22 +5075,33 @@ int mgsb_body_residuals(mgsb_solver* s, double* res0, double* res, int32_t n) {
 }
 
 int mgsb_time_kernel(mgsb_solver* s, int32_t kernel, int32_t reps, double* avg_us) {
-    if (!s || kernel != 0 || reps < 1 || !avg_us) {
-        set_error("mgsb_time_kernel: kernel must be 0, reps >= 1, avg_us non-null");
+    if (!s || kernel < 0 || kernel > 1 || reps < 1 || !avg_us) {
+        set_error("mgsb_time_kernel: kernel must be 0 or 1, reps >= 1, avg_us non-null");
         return MGSB_ERR_ARG;
     }
     HIPCHK(hipSetDevice(s->device));
-    const bool was = s->timing;
-    s->timing = false;
     const bool split = s->p.schedule == MGSB_SCHED_VCYCLE || s->p.schedule == MGSB_SCHED_LEARNED;
     DevLevel& d = s->L[0];
-    int rc = launch_tet(s, d, tet_mode_of(s->p.if_mode), false, nullptr, nullptr, split);  // warm-up
+    if (kernel == 1 && !(split && s->L.size() >= 2 && fused(s, d))) {
+        set_error("mgsb_time_kernel: kernel 1 (fused sweep) needs a V-cycle schedule on a level that fuses its sweeps");
+        return MGSB_ERR_STATE;
+    }
+    const bool was = s->timing;
+    s->timing = false;
+    // kernel 1 reads x, x_prev and pcf and writes only xz and pcf2 (free between steps)
+    auto one = [&]() -> int {
+        if (kernel == 0) return launch_tet(s, d, tet_mode_of(s->p.if_mode), false, nullptr, nullptr, split);
+        FuseBufs fb{d.x, d.xprev, 0};
+        return launch_fused<kFuseJacobi>(s, d, fb, 1.0f, kVertPoint | kVertContactJacobi, nullptr);
+    };
+    HIPCHK(hipStreamSynchronize(s->stream));
+    int rc = one();  // warm-up
     if (rc) return rc;
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
     HIPCHK(hipEventRecord(a, s->stream));
-    for (int r = 0; r < reps && !rc; r++) rc = launch_tet(s, d, tet_mode_of(s->p.if_mode), false, nullptr, nullptr, split);
+    for (int r = 0; r < reps && !rc; r++) rc = one();
     hipError_t e = hipEventRecord(b, s->stream);
     if (e == hipSuccess) e = hipEventSynchronize(b);
     float ms = 0.f;

@@ -32,9 +32,10 @@ def test_autotune_restores_state_and_picks_valid_parameters():
     assert np.array_equal(a.positions(), x0) and np.array_equal(a.velocity(), v0)
     assert r["block_tet"] in (64, 128, 256) and r["block_vert"] in (64, 128, 256)
     assert r["pre_sweeps"] in (3, 4, 5, 6) and r["pre_sweeps"] == r["post_sweeps"]
-    assert r["vert_lanes"] in (1, 2, 4, 8) and r["tile_gather"] in (0, 1)
-    # 34 launch shapes, the window for the old and the new shape, three other sweep counts
-    assert r["trials"] == 39 and r["ms_per_step"] <= r["ms_per_step_default"]
+    assert r["vert_lanes"] in (1, 2, 4, 8) and r["tile_gather"] in (0, 1) and r["fuse_sweeps"] in (0, 1)
+    # 35 launch shapes (one of them the fused sweeps), the window for the old
+    # and the new shape, three other sweep counts
+    assert r["trials"] == 40 and r["ms_per_step"] <= r["ms_per_step_default"]
     assert 0 < r["sweep_us"] <= r["sweep_us_default"]
     assert a.params.pre_sweeps == r["pre_sweeps"] and a.params.tile_gather == r["tile_gather"]
     # with the original parameters restored the run continues as an untuned copy
@@ -42,7 +43,7 @@ def test_autotune_restores_state_and_picks_valid_parameters():
     c = make()
     c.step(30)
     c.autotune(trial_steps=4)
-    c.set_params(pre_sweeps=5, post_sweeps=5, block_tet=0, block_vert=0, vert_lanes=0, tile_gather=1)
+    c.set_params(pre_sweeps=5, post_sweeps=5, block_tet=0, block_vert=0, vert_lanes=0, tile_gather=1, fuse_sweeps=1)
     c.step(1)
     assert np.array_equal(c.positions(), b.positions())
     # tuned steps meet the target

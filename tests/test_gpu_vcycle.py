@@ -235,3 +235,63 @@ def test_large_box_contact_rim_converges():
           f"half-cycles/step {r['total_halves'] / r['steps']:.1f}")
     assert r["failed_steps"] == 0 and r["max_rel"] <= 1e-3
     assert r["total_halves"] / r["steps"] < 12
+
+
+def _fuse_pair_solver(case, fuse):
+    if case == "bodies":
+        b = W.batched(nbodies=5, seed=7, sizes=(2, 4))
+        lv = b["levels"]
+    else:
+        lv = W.beam_hierarchy((8, 4, 1))
+    grav = (0, 0, 0) if case == "collision" else (0, -9.8, 0)
+    phys = W.physics(gravity=grav)
+    kw = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, check_every=2, tol=1e-3, rho_smooth=0.9,
+              fuse_sweeps=fuse, use_graph=1)
+    if case in ("temporal_cf", "collision", "bodies"):
+        kw.update(warm_start=1, coarse_first=1)
+    if case == "plain":
+        kw.update(check_every=1)
+    if case == "collision":
+        kw.update(kcol=W.large_colliders()["k"])
+    s = M.Solver(lv, schedule=M.SCHED_VCYCLE, if_mode=M.IF_STABLE, **kw, **phys)
+    s.fix_plane(1, 10.0, 1e-5)
+    s.set_velocity(W.seeded_velocity(lv[0], W.top_fixflag(lv[0])))
+    if case == "bodies":
+        s.set_body_stiffness(b["E"])
+    if case == "collision":
+        c = W.large_colliders()
+        s.set_sphere(0, c["sphere"]["center"], c["sphere"]["radius"], c["sphere"]["vel"])
+        s.set_plane(c["plane"]["normal"], c["plane"]["d"], c["k"])
+    return s
+
+
+@pytest.mark.parametrize("case", ["plain", "temporal_cf", "collision", "bodies"])
+def test_fused_sweeps_bit_identical(case):
+    """fuse_sweeps = 1 runs each fine-level sweep's vertex update (and the
+    coarse correction's prolongation) inside the next local step's launch,
+    every tile updating its own vertices.  Every copy of a vertex's update
+    reads the same inputs in the same order, so the trajectory, the collision
+    output and the run statistics equal those of separate launches bit for
+    bit: the plain V-cycle, the temporal coarse-first start, a sphere and a
+    plane at penalty 1e6, and five bodies with per-body checks."""
+    a, b = _fuse_pair_solver(case, 0), _fuse_pair_solver(case, 1)
+    us = b.time_kernel(20, kernel=1)
+    halves = []
+    for k in range(40):
+        sa, sb = a.step(1), b.step(1)
+        assert (sa.halves, sa.cycles, sa.fallback) == (sb.halves, sb.cycles, sb.fallback), k
+        assert sa.rel == sb.rel, (k, sa.rel, sb.rel)
+        halves.append(sb.halves)
+    xa, xb = a.positions(), b.positions()
+    print(f"[fused {case}] 40 steps, half-cycles {sum(halves)}, max |dx| {np.abs(xa - xb).max():.3e}, "
+          f"fused sweep {us:.2f} us")
+    assert np.array_equal(xa, xb)
+    assert np.array_equal(a.velocity(), b.velocity())
+    assert a.run_stats() == b.run_stats()
+    if case == "collision":
+        ca, cb = a.collision(), b.collision()
+        for q, r in zip(ca, cb):
+            assert np.array_equal(q, r)
+    # the unfused solver has no fused sweep to time
+    with pytest.raises(M.MgsbError):
+        a.time_kernel(5, kernel=1)
