@@ -47,8 +47,11 @@ VCYCLE = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, tol=1
 TRAIN = dict(episodes=[(sd, 60) for sd in range(1, 33)] + [(40, 600)], epochs=30, batch=4096, lr=1e-3, seed=0)
 # HBM bytes per launch of the dominant kernel, (workload, kernel) ->
 # rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB
-# copy), from profiles/fused_sweep_and_local_step_counters.md
-TRAFFIC = {}
+# copy): profiles/fused_sweep_medium_box_driver_window.md and
+# profiles/k_tet_tiled_large_and_batched.md
+TRAFFIC = {("medium_box_12x120x12", "k_sweep_fused"): 9.69e6,
+           ("large_box_24x240x24_collision", "k_tet"): 58.96e6,
+           ("batched_64_boxes", "k_tet"): 344.9e6}
 
 
 def log(*a):
