@@ -249,7 +249,8 @@ struct DenseIn {
 struct DenseOut {
     float* Y;
     int n, out, ldy;
-    __device__ void store(int row, int col, float v, const float* /*prior*/) const {
+    __device__ float2 pre(int /*col*/) const { return make_float2(0.f, 1.f); }
+    __device__ void store(int row, int col, float v, const float* /*prior*/, float2 /*pre*/) const {
         if (row < n && col < out) Y[(long)row * ldy + col] = v;
     }
 };
@@ -285,6 +286,17 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int nout, In in, Out o
     if (wave == 0)
 #pragma unroll
         for (int kb = 0; kb < kKb; kb++) w4[kb] = frag_b(W4, kKb, 0, kb, lane);
+    // the biases and the output's constants in the same round trip
+    float bb[3][kTilesPerWave];
+#pragma unroll
+    for (int tt = 0; tt < kTilesPerWave; tt++) {
+        const int t = wave * kTilesPerWave + tt;
+        bb[0][tt] = b1[t * 16 + r];
+        bb[1][tt] = b2[t * 16 + r];
+        bb[2][tt] = b3[t * 16 + r];
+    }
+    const float bias4 = (wave == 0 && r < nout) ? b4[r] : 0.f;
+    const float2 opre = (wave == 0 && r < nout) ? out.pre(r) : make_float2(0.f, 1.f);
     // layer 1: K = 16 (inputs zero-padded)
     {
         bf16x4 a;
@@ -299,14 +311,13 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int nout, In in, Out o
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w1[tt], acc, 0, 0, 0);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const float v = acc[q] + b1[t * 16 + r];
+                const float v = acc[q] + bb[0][tt];
                 hs[0][kq + q][t * 16 + r] = f2bf(v > 0.f ? v : 0.f);
             }
         }
     }
     __syncthreads();
     // layers 2 and 3: K = H
-    const float* bs[2] = {b2, b3};
 #pragma unroll
     for (int L = 0; L < 2; L++) {
         const int src = L & 1, dst = src ^ 1;
@@ -321,7 +332,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int nout, In in, Out o
             }
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const float v = acc[q] + bs[L][t * 16 + r];
+                const float v = acc[q] + bb[1 + L][tt];
                 hs[dst][kq + q][t * 16 + r] = f2bf(v > 0.f ? v : 0.f);
             }
         }
@@ -337,7 +348,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, int nout, In in, Out o
     }
 #pragma unroll
     for (int q = 0; q < 4; q++)
-        if (r < nout) out.store(base + kq + q, r, acc[q] + b4[r], &pri[kq + q][0]);
+        if (r < nout) out.store(base + kq + q, r, acc[q] + bias4, &pri[kq + q][0], opre);
 }
 
 // ---------------------------------------------------------------------------
@@ -504,6 +515,7 @@ struct CoarseIn {
                 raw[1] = w.x * a.y + w.y * b.y + w.z * c.y + w.w * d.y;
                 raw[2] = w.x * a.z + w.y * b.z + w.z * c.z + w.w * d.z;
                 raw[3] = io.X1[row].x;
+                if (prior) prior[3] = io.vcl[row].z;  // the output's Dirichlet mask
             } else if (k0 == 4) {
                 const float4 X = io.X1[row];
                 const float3 e = inject4(io.D1, io.aids[row], io.aw[row]);
@@ -536,11 +548,14 @@ struct CoarseIn {
 struct CoarseOut {
     CoarseIO io;
     int n;
-    // prior: the row's injected last delta, as load4 computed it
-    __device__ void store(int row, int col, float v, const float* prior) const {
+    // the output column's (mean, sd), loaded with the weights
+    __device__ float2 pre(int col) const {
+        return make_float2(io.norm[2 * kCoarseFeat + col], io.norm[2 * kCoarseFeat + 3 + col]);
+    }
+    // prior: the row's injected last delta and its mask, as load4 read them
+    __device__ void store(int row, int col, float v, const float* prior, float2 ms) const {
         if (row >= n || col >= 3) return;
-        const float mu = io.norm[2 * kCoarseFeat + col], sd = io.norm[2 * kCoarseFeat + 3 + col];
-        reinterpret_cast<float*>(io.epred + row)[col] = io.vcl[row].z * (prior[col] + (v * sd + mu));
+        reinterpret_cast<float*>(io.epred + row)[col] = prior[3] * (prior[col] + (v * ms.y + ms.x));
     }
 };
 

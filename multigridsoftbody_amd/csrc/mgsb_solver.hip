@@ -806,15 +806,17 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
 // same order, so every copy equals the owner's.  Positions rotate over three
 // buffers (xcur and xprv read, xout written) and partials over two, so no
 // launch reads what it writes.  tetq: per corner, its tile-local vertex index
-// (low 16 bits) and its position in the tile's corner array (high 16).
+// (low 16 bits) and its position in the tile's corner array (high 16); tvq:
+// per (tile, vertex), the vertex, its row of partials and whether the tile
+// owns it.
 constexpr int kFuseJacobi = 0, kFuseProlong = 1;
 constexpr int kFuseMaxNv = 256;  // vertices of a tile the fused sweep holds in LDS
 template <int MODE, int PHASE>
 __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
     int T, const float4* __restrict__ xcur, const float4* __restrict__ xprv, float4* __restrict__ xout,
-    const float4* __restrict__ s, const float4* __restrict__ vc, const int* __restrict__ prp,
-    const float3* __restrict__ pin, const int* __restrict__ tvo, const int* __restrict__ tvs,
-    const int* __restrict__ tlr, const int* __restrict__ tvid, const int4* __restrict__ tetq,
+    const float4* __restrict__ s, const float4* __restrict__ vc, const float3* __restrict__ pin,
+    const int* __restrict__ tvo, const int* __restrict__ tvs, const int* __restrict__ tlr,
+    const int4* __restrict__ tvq, const int4* __restrict__ tetq,
     const float* __restrict__ B, const float* __restrict__ vol, float k, const float* __restrict__ tk,
     float3* __restrict__ pout, const DevColliders* __restrict__ col, int flags, float omega, float relax_hi,
     float relax_lo, int* __restrict__ cflag, float* __restrict__ cdepth, float* __restrict__ cenergy,
@@ -838,10 +840,12 @@ __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
         if (tk) kk = tk[g];
     }
     for (int l = threadIdx.x; l < nv; l += kTileTets) {
-        const int v = tvid[v0 + l];
+        // (vertex, its row of partials, owner flag): the rows and the vertex's
+        // streams load in one round trip
+        const int4 tq = tvq[v0 + l];
+        const int v = tq.x, r0 = tq.y, r1 = tq.z;
+        const bool owner = tq.w != 0;
         const float4 st = s[v];
-        const int r0 = prp[v];
-        const bool owner = tvs[v0 + l] == r0;
         float4 pn;
         if (PHASE == kFuseProlong) {
             const float3 e = prolong_at(v, ec, iids, iw);
@@ -853,7 +857,7 @@ __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
         } else {
             const float4 p = xcur[v], pp = xprv[v], c = vc[v];
             float fx = 0.f, fy = 0.f, fz = 0.f;
-            gather_sum(pin, r0, prp[v + 1], fx, fy, fz);
+            gather_sum(pin, r0, r1, fx, fy, fz);
             int flag;
             float depth, energy;
             pn = vertex_step(p, pp, st, c, make_float3(fx, fy, fz), v, nullptr, nullptr, col, omega, relax_hi,
@@ -2313,7 +2317,7 @@ struct DevLevel {
     // corner position; the tiles' vertex lists; a third position buffer
     // (nullptr: the level does not run fused sweeps)
     int4* tetq = nullptr;
-    int* tvid = nullptr;
+    int4* tvq = nullptr;  // per (tile, vertex): vertex id, its prp row [y, z), 1 if this tile owns it
     float4* xz = nullptr;
     int4 *tri = nullptr, *tslot = nullptr;
     int *trp = nullptr, *trid = nullptr;
@@ -2556,7 +2560,13 @@ int upload_level(mgsb_solver* s, DevLevel& d) {
                 tq[t] = make_int4(c[0], c[1], c[2], c[3]);
             }
             if ((rc = up(s, d.owned, &d.tetq, tq.data(), tq.size()))) return rc;
-            if ((rc = up(s, d.owned, &d.tvid, h.tvid.data(), h.tvid.size()))) return rc;
+            std::vector<int4> tv(h.tvid.size());
+            for (size_t q = 0; q < tv.size(); q++) {
+                const int v = h.tvid[q];
+                const int r0 = h.prp[(size_t)v], r1 = h.prp[(size_t)v + 1];
+                tv[q] = make_int4(v, r0, r1, h.tvs[q] == r0 ? 1 : 0);  // the first tile of a row owns the vertex
+            }
+            if ((rc = up(s, d.owned, &d.tvq, tv.data(), tv.size()))) return rc;
         }
     }
     std::vector<int4> tr4(std::max(1, h.S)), ts4(std::max(1, h.S));
@@ -2934,7 +2944,7 @@ void launch_fused_mode(mgsb_solver* s, DevLevel& d, const FuseBufs& fb, float4* 
                        const int* skip, const float4* ec, const float* amask) {
     LevelMap& m = s->M[0];
     k_sweep_fused<MODE, PHASE><<<d.ntiles, kTileTets, 0, s->stream>>>(
-        d.T, fb.cur, fb.prev, out, d.s, d.vc, d.prp, fb.pin ? d.pcf2 : d.pcf, d.tvo, d.tvs, d.tlr, d.tvid, d.tetq,
+        d.T, fb.cur, fb.prev, out, d.s, d.vc, fb.pin ? d.pcf2 : d.pcf, d.tvo, d.tvs, d.tlr, d.tvq, d.tetq,
         d.B, d.vol, s->p.stiffness, d.tk, fb.pin ? d.pcf : d.pcf2, s->dcol, flags, omega, s->relax_hi, s->relax_lo,
         d.cflag, d.cdepth, d.cenergy, ec, m.iids, m.iw, amask, skip);
 }
