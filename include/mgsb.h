@@ -146,6 +146,12 @@ typedef struct mgsb_params {
                            * of its ~2.5x shared vertices, which costs more than a launch
                            * on larger meshes); 2: fused at any size.  The results are
                            * identical either way. */
+    int32_t stall_guard;  /* V-cycle: when a check's residual exceeds 3 times the previous
+                           * check's (or the start's), the fine smoother drops its Chebyshev
+                           * extrapolation for the rest of the step (plain damped Jacobi;
+                           * 1, default) or keeps it (0).  Near inverted tets the
+                           * extrapolated iteration can jump instead of converging; the
+                           * residual, its target and the fixed point are unchanged. */
 } mgsb_params;
 
 typedef struct mgsb_step_stats {
@@ -336,6 +342,11 @@ int mgsb_residual(mgsb_solver* s, double* res);
 /* Run aggregates of the V-cycle schedules (no reference counterpart: the
  * reference has no residual); reset != 0 clears them after reading. */
 int mgsb_run_stats(mgsb_solver* s, mgsb_run_stats_t* out, int32_t reset);
+/* The relative residuals ||g|| / ||g(s)|| at the last step's checks, in order
+ * (the start, the warm-start judge, then one per checked half-cycle; at most
+ * 64): count gets their number, rel the first n.  A convergence history for
+ * the caller; no reference counterpart (the reference reports energies). */
+int mgsb_step_history(mgsb_solver* s, double* rel, int32_t n, int32_t* count);
 
 /* Learned coarse correction (MGSB_SCHED_LEARNED; no reference counterpart).
  * The step starts from the temporal extrapolation x_t = s + 2 u_{n-1} -

@@ -77,6 +77,7 @@ class mgsb_params(C.Structure):
         ("tile_gather", C.c_int32),
         ("coarse_first", C.c_int32),
         ("fuse_sweeps", C.c_int32),
+        ("stall_guard", C.c_int32),
     ]
 
 
@@ -164,6 +165,7 @@ SIGNATURES = {
     "mgsb_mlp_forward": (C.c_int, [_vp, _fp, C.c_int32, _fp, C.c_int32]),
     "mgsb_mlp_grad": (C.c_int, [_vp, _fp, _fp, C.c_int32, _fp, _dp]),
     "mgsb_run_stats": (C.c_int, [_vp, C.POINTER(mgsb_run_stats_t), C.c_int32]),
+    "mgsb_step_history": (C.c_int, [_vp, _dp, C.c_int32, _ip]),
     "mgsb_coarse_nfree": (C.c_int, [_vp, _ip]),
     "mgsb_set_timing": (C.c_int, [_vp, C.c_int32]),
     "mgsb_kernel_time": (C.c_int, [_vp, C.c_int32, _dp, C.POINTER(C.c_int64)]),
@@ -499,6 +501,14 @@ class Solver:
         r = C.c_double()
         _check(lib.mgsb_residual(self._h, C.byref(r)), "mgsb_residual")
         return r.value
+
+    def step_history(self):
+        """Relative residuals at the last step's checks (start, judge, then
+        each checked half-cycle)."""
+        out = np.zeros(64, np.float64)
+        n = C.c_int32()
+        _check(lib.mgsb_step_history(self._h, out.ctypes.data_as(_dp), 64, C.byref(n)), "mgsb_step_history")
+        return out[:min(64, n.value)]
 
     def run_stats(self, reset=False):
         st = mgsb_run_stats_t()

@@ -939,6 +939,7 @@ void mgsb_default_params(mgsb_params* p) {
     p->tile_gather = 1;
     p->coarse_first = 0;
     p->fuse_sweeps = 1;
+    p->stall_guard = 1;
 }
 
 const char* mgsb_version(void) { return "mgsb 0.1 (gfx950)"; }

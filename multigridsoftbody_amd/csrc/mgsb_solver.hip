@@ -768,8 +768,9 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
                                               const DevColliders* __restrict__ col, float omega, float relax_hi,
                                               float relax_lo, int flags, int* __restrict__ cflag, float* __restrict__ cdepth,
                                               float* __restrict__ cenergy, const int* __restrict__ skip, int split,
-                                              int lsh) {
+                                              int lsh, const int* __restrict__ calm) {
     if (skip && *skip) return;
+    if (calm && *calm) omega = 1.0f;  // stall guard: plain (damped) Jacobi for the rest of the step
     // 2^lsh lanes per vertex gather its row (gather_lanes); lane 0 finishes
     const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = g >> lsh, sub = g & ((1 << lsh) - 1);
@@ -821,8 +822,9 @@ __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
     float3* __restrict__ pout, const DevColliders* __restrict__ col, int flags, float omega, float relax_hi,
     float relax_lo, int* __restrict__ cflag, float* __restrict__ cdepth, float* __restrict__ cenergy,
     const float4* __restrict__ ec, const int4* __restrict__ iids, const float4* __restrict__ iw,
-    const float* __restrict__ amask, const int* __restrict__ skip) {
+    const float* __restrict__ amask, const int* __restrict__ skip, const int* __restrict__ calm) {
     if (skip && *skip) return;
+    if (calm && *calm) omega = 1.0f;  // stall guard (k_vert)
     __shared__ float4 lu[kFuseMaxNv], ls[kFuseMaxNv];  // the tile's new positions and s
     __shared__ TileLds L;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -1105,6 +1107,7 @@ constexpr size_t kScratchDoubles = 131072;  // 1 MiB
 constexpr size_t kScratchResults = 16;      // the last 16 doubles take the reduction results
 static __device__ __align__(256) double scratch[kScratchDoubles];
 
+constexpr int kHistMax = 64;  // checks of a step kept for mgsb_step_history
 struct VState {
     double res0;    // ||g(s)||
     double res;     // ||g(x)|| at the last check
@@ -1133,6 +1136,10 @@ struct VState {
                     // the finish piece, enqueued before the host reads the verdict, skips
     double slack;   // the learned stage is kept while its residual <= slack * res
     int cfirst;     // the step's first half-cycle is a coarse correction (coarse_first)
+    int guard;      // stall_guard: a check whose residual exceeds kStallRatio times the previous one's ...
+    int calm;       // ... turns the fine smoother's Chebyshev extrapolation off for the rest of the step
+    int nhist;      // checks of the current step so far, and their relative residuals
+    double hist[kHistMax];
 };
 
 // Check modes of the fused residual reduction.
@@ -1142,6 +1149,21 @@ constexpr int kChkCycle = 1;  // end of a cycle
 constexpr int kChkWarm = 2;   // after the learned warm start (judge, may reject it)
 constexpr int kChkMid = 3;    // after pre-smoothing
 constexpr int kChkTemporal = 4;  // after the temporal extrapolation
+
+// stall_guard: a check whose residual exceeds kStallRatio times the previous
+// check's turns the fine smoother's Chebyshev extrapolation off for the rest
+// of the step.  Steps that stall next to inverted tets (the blob) jump by 4-18x
+// between checks; contact steps (the large box) creep up by at most ~2.8x
+// while they converge, which the guard leaves alone.
+constexpr double kStallRatio = 3.0;
+
+// The warm-start judge: a stage is kept only if its residual r is at most the
+// best one so far (times slack for the learned stage); a non-finite residual
+// is never kept.
+__device__ __forceinline__ bool judge_keeps(double r, double best, int mode, double slack) {
+    const double bound = mode == kChkWarm ? slack * best : best;
+    return r <= bound;
+}
 
 // Convergence bookkeeping from the squared residual norm t.  hm (host-mapped,
 // optional) mirrors (converged, cycles, halves, fallback, accepted) for the
@@ -1166,6 +1188,7 @@ __device__ void vstate_apply(VState* st, double t, int mode, int* hm, double rel
         st->accepted = 0;
         st->revert = 0;
         st->cf_ok = 1;
+        st->calm = 0;
         st->converged = (st->check && r == 0.0) ? 1 : 0;
     } else if (mode == kChkTemporal || mode == kChkWarm) {
         // a warm-start stage is kept only if it lowers the best residual so
@@ -1174,8 +1197,7 @@ __device__ void vstate_apply(VState* st, double t, int mode, int* hm, double rel
             st->res_t = r;
         else
             st->res_ws = r;
-        const double bound = mode == kChkWarm ? st->slack * st->res : st->res;
-        if (!(r <= bound)) {
+        if (!judge_keeps(r, st->res, mode, st->slack)) {
             st->revert = st->accepted == 1 ? 2 : 1;
             st->fallback |= mode == kChkTemporal ? 1 : 2;
             st->cf_ok = 0;
@@ -1188,6 +1210,7 @@ __device__ void vstate_apply(VState* st, double t, int mode, int* hm, double rel
             if (st->check && ok(r)) st->converged = 1;
         }
     } else {
+        if (st->guard && !(r <= kStallRatio * st->res)) st->calm = 1;  // the iteration jumps: no extrapolation
         st->res = r;
         st->rel = rel_of(r);
         if (mode == kChkCycle) {
@@ -1200,6 +1223,8 @@ __device__ void vstate_apply(VState* st, double t, int mode, int* hm, double rel
         if (st->check && ok(r)) st->converged = 1;
     }
     st->notdone = (st->check == 0 || st->converged || (st->maxh > 0 && st->halves >= st->maxh)) ? 0 : 1;
+    if (mode == kChkStart) st->nhist = 0;
+    if (st->nhist < kHistMax) st->hist[st->nhist++] = st->res0 > 0.0 ? r / st->res0 : 0.0;
     (void)hm;  // the check kernel publishes the verdict with its count (bump_seq)
 }
 
@@ -1354,7 +1379,10 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* x, const flo
     double v[1] = {0.0};
     if (i < V) {
         const float4 p = x[i], st = s[i], c = vc[i];
-        const float4 pq = (xpsrc && sub == 0) ? xpsrc[i] : p;
+        // x_prev's new value: loaded from a valid buffer either way (a select
+        // between xpsrc[i] and the local p would put p on the stack)
+        const float4 pl = (xpsrc ? xpsrc : x)[i];
+        const float4 pq = xpsrc ? pl : p;
         const float3 f = gather_lanes(cf, rp[i], rp[i + 1], sub, lsh);
         if (sub == 0) {
             bool contact;
@@ -1555,8 +1583,9 @@ __global__ void __launch_bounds__(256) k_revert(int V, float4* __restrict__ x, f
 // already at s, x (which holds the warm start) returns to s without a
 // verdict.  Every workgroup sums the partials in the same fixed order
 // (lane-strided over blockDim 256, then wg_sum_fixed, as k_check) and takes
-// the same verdict on a fresh copy of the step's constants; workgroup 0
-// applies it to the step state and publishes it.  vstate_apply at the start check resets every field the
+// the same verdict (judge_keeps, as vstate_apply decides it: after the start
+// check the best residual is res0 and no stage is accepted, so a rejection
+// returns to s); workgroup 0 applies it to the step state and publishes it.  vstate_apply at the start check resets every field the
 // judge reads but check, tol and slack, so the copies decide alike.
 __global__ void __launch_bounds__(256) k_judge_revert(int nb, const double* __restrict__ part, VState* st, int* hm,
                                                       unsigned* seqc, int judge, int V, float4* __restrict__ x,
@@ -1572,22 +1601,19 @@ __global__ void __launch_bounds__(256) k_judge_revert(int nb, const double* __re
     wg_sum_fixed<2>(v);
     __shared__ int s_rv;
     if (threadIdx.x == 0) {
-        VState loc{};
-        VState* tgt = &loc;
         if (blockIdx.x == 0) {
-            tgt = st;
-        } else {
-            loc.check = st->check;
-            loc.tol = st->tol;
-            loc.slack = st->slack;
+            vstate_apply(st, v[0], kChkStart, hm);
+            if (!st->converged)
+                vstate_apply(st, v[1], judge, hm);
+            else
+                st->revert = 1;
+            s_rv = st->revert;
+            bump_seq(seqc, hm, st);
+        } else {  // the same decision from the same sums (vstate_apply at the start, then the judge)
+            const double r0 = sqrt(v[0]);
+            const bool conv = st->check && r0 == 0.0;
+            s_rv = (conv || !judge_keeps(sqrt(v[1]), r0, judge, st->slack)) ? 1 : 0;
         }
-        vstate_apply(tgt, v[0], kChkStart, hm);
-        if (!tgt->converged)
-            vstate_apply(tgt, v[1], judge, hm);
-        else
-            tgt->revert = 1;
-        s_rv = tgt->revert;
-        if (blockIdx.x == 0) bump_seq(seqc, hm, st);
     }
     __syncthreads();
     const int rv = s_rv;  // the dual start rejects to s only
@@ -2619,6 +2645,7 @@ int push_vstate(mgsb_solver* s) {
     v.tol = s->p.tol;
     v.maxh = max_halves(s);
     v.cfirst = coarse_first(s) ? 1 : 0;
+    v.guard = s->p.stall_guard ? 1 : 0;
     v.slack = s->p.learn_slack > 0.f ? (double)s->p.learn_slack : 1.0;
     HIPCHK(hipMemcpyAsync(s->vstate, &v, sizeof(VState), hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -2762,13 +2789,18 @@ void relax_consts(mgsb_solver* s) {
     }
 }
 
+// the stall guard's flag for the V-cycle's fine sweeps (split positions)
+const int* calm_flag(const mgsb_solver* s, int split) {
+    return (split && s->vstate && s->p.stall_guard) ? &s->vstate->calm : nullptr;
+}
+
 int launch_vert(mgsb_solver* s, DevLevel& d, float omega, int flags, const int* skip = nullptr, int split = 0) {
     const int bs = block_vert(s), lsh = vert_lsh(s);
     relax_consts(s);
     k_vert<<<nblk(d.V << lsh, bs), bs, 0, s->stream>>>(d.V, d.x, d.xprev, d.s, d.vc, gath_rp(s, d), gath_cf(s, d),
                                                         d.trp, d.tcf, s->dcol,
                                                         omega, s->relax_hi, s->relax_lo, flags, d.cflag, d.cdepth,
-                                                        d.cenergy, skip, split, lsh);
+                                                        d.cenergy, skip, split, lsh, calm_flag(s, split));
     HIPCHK(hipGetLastError());
     return MGSB_OK;
 }
@@ -2946,7 +2978,7 @@ void launch_fused_mode(mgsb_solver* s, DevLevel& d, const FuseBufs& fb, float4* 
     k_sweep_fused<MODE, PHASE><<<d.ntiles, kTileTets, 0, s->stream>>>(
         d.T, fb.cur, fb.prev, out, d.s, d.vc, fb.pin ? d.pcf2 : d.pcf, d.tvo, d.tvs, d.tlr, d.tvq, d.tetq,
         d.B, d.vol, s->p.stiffness, d.tk, fb.pin ? d.pcf : d.pcf2, s->dcol, flags, omega, s->relax_hi, s->relax_lo,
-        d.cflag, d.cdepth, d.cenergy, ec, m.iids, m.iw, amask, skip);
+        d.cflag, d.cdepth, d.cenergy, ec, m.iids, m.iw, amask, skip, calm_flag(s, 1));
 }
 // One fused sweep: JACOBI (vertex update with omega, then the local step) or
 // PROLONG (x += mask P ec, then the local step; the smoother restarts).
@@ -3750,8 +3782,9 @@ int validate_params(const mgsb_params* p, int nlevels) {
         set_error("check_every must be 0 (no checks), 1 (after each cycle) or 2 (also after pre-smoothing)");
         return MGSB_ERR_ARG;
     }
-    if (p->coarse_first < 0 || p->coarse_first > 1 || p->fuse_sweeps < 0 || p->fuse_sweeps > 2) {
-        set_error("coarse_first must be 0 or 1, fuse_sweeps 0, 1 or 2");
+    if (p->coarse_first < 0 || p->coarse_first > 1 || p->fuse_sweeps < 0 || p->fuse_sweeps > 2 ||
+        p->stall_guard < 0 || p->stall_guard > 1) {
+        set_error("coarse_first and stall_guard must be 0 or 1, fuse_sweeps 0, 1 or 2");
         return MGSB_ERR_ARG;
     }
     if (nlevels > 8) {
@@ -4490,6 +4523,22 @@ int mgsb_level_info(mgsb_solver* s, int32_t level, int32_t* nverts, int32_t* nte
     if (nverts) *nverts = s->L[level].V;
     if (ntets) *ntets = s->L[level].T;
     if (ntris) *ntris = s->L[level].S;
+    return MGSB_OK;
+}
+
+int mgsb_step_history(mgsb_solver* s, double* rel, int32_t n, int32_t* count) {
+    if (!s || !count || n < 0 || (n > 0 && !rel)) {
+        set_error("mgsb_step_history: bad arguments");
+        return MGSB_ERR_ARG;
+    }
+    *count = 0;
+    if (!s->vstate) return MGSB_OK;
+    HIPCHK(hipSetDevice(s->device));
+    VState v;
+    HIPCHK(hipMemcpyAsync(&v, s->vstate, sizeof(VState), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    *count = v.nhist;
+    for (int i = 0; i < std::min<int>(n, v.nhist); i++) rel[i] = v.hist[i];
     return MGSB_OK;
 }
 

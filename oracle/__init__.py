@@ -52,6 +52,7 @@ class orc_vparams(C.Structure):
         ("relax_coarse", C.c_float),
         ("rho_coarse", C.c_float),
         ("coarse_first", C.c_int),
+        ("stall_guard", C.c_int),
     ]
 
 
@@ -299,11 +300,12 @@ def barycentric4(P, tet4x3, fp64=False):
 
 
 def vparams(pre=2, post=2, coarse_sweeps=2, max_cycles=8, coarse_direct=1, check=1, tol=1e-3, rho_smooth=0.0,
-            relax_coarse=0.6, rho_coarse=0.0, coarse_first=0):
+            relax_coarse=0.6, rho_coarse=0.0, coarse_first=0, stall_guard=1):
     vp = orc_vparams()
     vp.pre, vp.post, vp.coarse_sweeps, vp.max_cycles = pre, post, coarse_sweeps, max_cycles
     vp.coarse_direct, vp.check, vp.tol, vp.rho_smooth = coarse_direct, check, tol, rho_smooth
     vp.relax_coarse, vp.rho_coarse, vp.coarse_first = relax_coarse, rho_coarse, coarse_first
+    vp.stall_guard = stall_guard
     return vp
 
 

@@ -1259,6 +1259,7 @@ typedef struct orc_vparams {
     int pre, post, coarse_sweeps, max_cycles, coarse_direct, check;
     float tol, rho_smooth, relax_coarse, rho_coarse;
     int coarse_first; /* with checks: the first half-cycle is the coarse correction */
+    int stall_guard;  /* a check above the previous one turns the fine Chebyshev off for the step */
 } orc_vparams;
 
 typedef struct orc_ml {
@@ -1639,8 +1640,9 @@ void PFX(ml_set_warm)(orc_ml* M, const float* e1, const float* u0) {
 
 /* Coarse-correction half of a cycle: the masked residual M->r[0] of the last
  * fine evaluation restricted, the linear V-cycle on the coarse levels,
- * prolongation, post-smoothing; returns the residual after it. */
-static double ml_coarse_half(orc_ml* M, orc_params* p, const orc_vparams* vp) {
+ * prolongation, post-smoothing; returns the residual after it.  calm: the
+ * stall guard has turned the Chebyshev extrapolation off (omega 1). */
+static double ml_coarse_half(orc_ml* M, orc_params* p, const orc_vparams* vp, int calm) {
     orc_level* L0 = M->L[0];
     real rs = vp->rho_smooth;
     ml_restrict(M, 0, M->r[0], M->r[1]);
@@ -1650,7 +1652,7 @@ static double ml_coarse_half(orc_ml* M, orc_params* p, const orc_vparams* vp) {
     real omega = 1;
     for (int k = 0; k < vp->post; k++) {
         omega = 4 / (4 - rs * rs * omega);
-        pd_iteration(L0, p, omega, 0, 1);
+        pd_iteration(L0, p, calm ? 1 : omega, 0, 1);
     }
     return ml_fine_residual(M, p);
 }
@@ -1661,7 +1663,10 @@ static double ml_coarse_half(orc_ml* M, orc_params* p, const orc_vparams* vp) {
  * completed cycles, halves the half-cycles run (pre-smoothing half, coarse
  * correction + post-smoothing half).  coarse_first (with checks): the first
  * half is a coarse-correction half on the start's residual (counted as one
- * half and one cycle), then max_cycles cycles at most. */
+ * half and one cycle), then max_cycles cycles at most.  stall_guard: once a
+ * check's residual exceeds 3 times the previous one (the start's, or the
+ * accepted warm start's), the fine sweeps drop the Chebyshev extrapolation
+ * (omega 1) for the rest of the step. */
 void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nsteps) {
     orc_level* L0 = M->L[0];
     real rs = vp->rho_smooth;
@@ -1715,8 +1720,15 @@ void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nstep
         }
         M->has_warm = 0;
         M->has_warm0 = 0;
+        int calm = 0;
+#define ORC_CHECK(r) \
+    do { \
+        if (vp->stall_guard && !((r) <= 3.0 * M->res)) calm = 1; \
+        M->res = (r); \
+    } while (0)
         if (cfirst && !conv) {
-            M->res = ml_coarse_half(M, p, vp);
+            double rc = ml_coarse_half(M, p, vp, calm);
+            ORC_CHECK(rc);
             M->cycles++;
             M->halves++;
             conv = M->res <= (double)vp->tol * M->res0;
@@ -1725,19 +1737,24 @@ void PFX(step_vcycle)(orc_ml* M, orc_params* p, const orc_vparams* vp, int nstep
             real omega = 1;
             for (int k = 0; k < vp->pre; k++) {
                 omega = 4 / (4 - rs * rs * omega);
-                pd_iteration(L0, p, omega, 0, 1);
+                pd_iteration(L0, p, calm ? 1 : omega, 0, 1);
             }
             double rm = ml_fine_residual(M, p);
             if (vp->check == 2) {
-                M->res = rm;
+                ORC_CHECK(rm);
                 M->halves++;
                 if (rm <= (double)vp->tol * M->res0) break;
             }
-            M->res = ml_coarse_half(M, p, vp);
+            double rc = ml_coarse_half(M, p, vp, calm);
+            if (vp->check)
+                ORC_CHECK(rc);
+            else
+                M->res = rc;
             M->cycles++;
             M->halves += vp->check == 2 ? 1 : 2;
             conv = vp->check && M->res <= (double)vp->tol * M->res0;
         }
+#undef ORC_CHECK
         calc_v(L0, p);
     }
 }

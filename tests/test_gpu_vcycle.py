@@ -295,3 +295,31 @@ def test_fused_sweeps_bit_identical(case):
     # the unfused solver has no fused sweep to time
     with pytest.raises(M.MgsbError):
         a.time_kernel(5, kernel=1)
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_stall_guard_matches_oracle_on_a_diverging_smoother(graph):
+    """rho_smooth 1.1 over-estimates the fine smoother's spectral radius: the
+    Chebyshev weights pass 2 and the first sweeps diverge, the first check's
+    residual jumps past 3 times the start's, and the guard runs the rest of
+    the step without extrapolation.  The device takes the same decisions as
+    the oracle (half-cycle counts), meets the target every step and stays on
+    its trajectory; the step's check history shows the jump."""
+    kw = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, check_every=2, tol=1e-3,
+              rho_smooth=1.1, stall_guard=1, use_graph=graph)
+    lv, s, ML, Ls, p = make_pair((4, 1), None, **kw)
+    vp = O.vparams(pre=5, post=5, coarse_sweeps=2, max_cycles=12, check=2, tol=1e-3, rho_smooth=1.1, stall_guard=1)
+    hg, ho = [], []
+    for k in range(20):
+        st = s.step(1)
+        ML.step(p, vp, 1)
+        hg.append(st.halves)
+        ho.append(ML.stats()["halves"])
+        assert st.rel <= 1e-3, (k, st.rel)
+        h = s.step_history()
+        assert h[0] == 1.0 and abs(h[-1] - st.rel) <= 1e-12 and len(h) == 1 + st.halves
+        assert np.any(h[1:] > 3 * h[:-1]), h
+    err = rel_err(s.positions(), Ls[0].positions())
+    print(f"[stall guard graph={graph}] halves gpu {hg} oracle {ho} rel(gpu,oracle)={err:.2e}")
+    assert sum(a == b for a, b in zip(hg, ho)) >= 18
+    assert err < REL_TOL

@@ -259,3 +259,44 @@ def test_coarse_first_half_is_the_coarse_correction_of_g_s():
     ML2.step(p, O.vparams(pre=3, post=0, coarse_sweeps=2, max_cycles=1, check=2, tol=0.0, rho_smooth=0.9), 1)
     u2 = Ls2[0].positions() - Ls2[0].get(O.S).reshape(-1, 3)
     assert np.abs(u2 - ref).max() > 100 * err * np.abs(ref).max()
+
+
+def test_stall_guard_inert_while_converging_and_rescues_a_diverging_smoother():
+    """stall_guard: a check whose residual exceeds 3 times the previous
+    check's turns the fine Chebyshev extrapolation off for the rest of the
+    step.  On steps whose checks decrease (rho 0.9, FP64, tolerance 1e-3) the
+    trajectory is the same with and without it, bit for bit.  With rho 1.1
+    (an over-estimate of the smoother's spectral radius) the Chebyshev
+    weights grow past 2 (1.43, 1.77, 2.15, 2.85, 7.3) and the sweeps
+    diverge: unguarded, no step reaches the target in 12 cycles and the
+    residual ends above the start's; guarded, every step meets it within
+    5 half-cycles, in FP32 as in FP64, on the same trajectory."""
+    lv = W.beam_hierarchy((4, 1))
+    phys = W.physics()
+    fix = W.top_fixflag(lv[0])
+    v0 = W.seeded_velocity(lv[0], fix)
+
+    def run(guard, fp64, rho, steps):
+        Ls = [O.Level(m.pos, m.tets, m.tris, stiffness=phys["stiffness"], density=phys["density"], fp64=fp64)
+              for m in lv]
+        Ls[0].set_fix(fix)
+        Ls[0].set_velocity(v0)
+        ML = O.MultiLevel(Ls, relax_coarse=0.6, dt=phys["dt"])
+        p = O.params(dt=phys["dt"], stiffness=phys["stiffness"], density=phys["density"], gravity=phys["gravity"],
+                     if_mode=3)
+        vp = O.vparams(pre=5, post=5, coarse_sweeps=2, max_cycles=12, check=2, tol=1e-3, rho_smooth=rho,
+                       stall_guard=guard)
+        st = []
+        for _ in range(steps):
+            ML.step(p, vp, 1)
+            st.append(ML.stats())
+        return Ls[0].positions(), st
+
+    assert np.array_equal(run(0, True, 0.9, 20)[0], run(1, True, 0.9, 20)[0])
+    _, wild = run(0, False, 1.1, 3)
+    assert all(s["halves"] == 24 and s["rel"] > 1.0 for s in wild)
+    x32, calm32 = run(1, False, 1.1, 20)
+    x64, calm64 = run(1, True, 1.1, 20)
+    for st in (calm32, calm64):
+        assert all(s["rel"] <= 1e-3 and s["halves"] <= 5 for s in st)
+    assert float(np.abs(x32 - x64).max() / np.abs(x64).max()) < 1e-5
