@@ -473,21 +473,41 @@ __device__ __forceinline__ void tile_put(TileLds& L, int4 q, const float (&fo)[1
     }
 }
 
-__device__ __forceinline__ void tile_sum(const TileLds& L, int tile, const int* __restrict__ tvo,
-                                         const int* __restrict__ tvs, const int* __restrict__ tlr,
-                                         float3* __restrict__ pcf) {
-    const int v0 = tvo[tile], nv = tvo[tile + 1] - v0;
-    const int* lr = tlr + v0 + tile;
-    for (int l = threadIdx.x; l < nv; l += kTileTets) {
-        const int r0 = lr[l], r1 = lr[l + 1];
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-        for (int e = r0; e < r1; e++) {
-            fx = fx + L.x[e];
-            fy = fy + L.y[e];
-            fz = fz + L.z[e];
-        }
-        pcf[tvs[v0 + l]] = make_float3(fx, fy, fz);
+// The tile's vertex range and, for lane threadIdx.x, its vertex's row of the
+// tile's corner array and its partial's slot: loaded before the local step,
+// so that tile_sum's first pass reads no index after the barrier.
+struct TileRow {
+    int v0, nv, r0, r1, slot;
+};
+__device__ __forceinline__ TileRow tile_row(int tile, const int* __restrict__ tvo, const int* __restrict__ tvs,
+                                            const int* __restrict__ tlr) {
+    TileRow t;
+    t.v0 = tvo[tile];
+    t.nv = tvo[tile + 1] - t.v0;
+    t.r0 = t.r1 = t.slot = 0;
+    const int l = threadIdx.x;
+    if (l < t.nv) {
+        t.r0 = tlr[t.v0 + tile + l];
+        t.r1 = tlr[t.v0 + tile + l + 1];
+        t.slot = tvs[t.v0 + l];
     }
+    return t;
+}
+__device__ __forceinline__ float3 tile_row_sum(const TileLds& L, int r0, int r1) {
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    for (int e = r0; e < r1; e++) {
+        fx = fx + L.x[e];
+        fy = fy + L.y[e];
+        fz = fz + L.z[e];
+    }
+    return make_float3(fx, fy, fz);
+}
+__device__ __forceinline__ void tile_sum(const TileLds& L, const TileRow& t, int tile, const int* __restrict__ tvs,
+                                         const int* __restrict__ tlr, float3* __restrict__ pcf) {
+    if ((int)threadIdx.x < t.nv) pcf[t.slot] = tile_row_sum(L, t.r0, t.r1);
+    const int* lr = tlr + t.v0 + tile;
+    for (int l = threadIdx.x + kTileTets; l < t.nv; l += kTileTets)
+        pcf[tvs[t.v0 + l]] = tile_row_sum(L, lr[l], lr[l + 1]);
 }
 
 // Triangle-sphere DCD of surface triangle t (PDSolver.cu:612-680): writes
@@ -558,6 +578,7 @@ __global__ void __launch_bounds__(kTileTets) k_tet_tiled(
     }
     __shared__ TileLds L;
     const int g = blk * kTileTets + threadIdx.x;
+    const TileRow tr = tile_row(blk, tvo, tvs, tlr);
     if (g < T) {
         const int4 q = tetp[g];
         float fo[12];
@@ -565,7 +586,7 @@ __global__ void __launch_bounds__(kTileTets) k_tet_tiled(
         tile_put(L, q, fo);
     }
     __syncthreads();
-    tile_sum(L, blk, tvo, tvs, tlr, pcf);
+    tile_sum(L, tr, blk, tvs, tlr, pcf);
 }
 
 // Start of a temporally warm-started V-cycle step: lanes 0..T-1 evaluate the
@@ -599,6 +620,7 @@ __global__ void __launch_bounds__(kTileTets) k_tet_dual_tiled(
     const int blk = at_s ? b0 - ntiles : b0;
     __shared__ TileLds L;
     const int g = blk * kTileTets + threadIdx.x;
+    const TileRow tr = tile_row(blk, tvo, tvs, tlr);
     if (g < T) {
         const int4 q = tetp[g];
         float fo[12];
@@ -606,7 +628,7 @@ __global__ void __launch_bounds__(kTileTets) k_tet_dual_tiled(
         tile_put(L, q, fo);
     }
     __syncthreads();
-    tile_sum(L, blk, tvo, tvs, tlr, at_s ? pcf2 : pcf);
+    tile_sum(L, tr, blk, tvs, tlr, at_s ? pcf2 : pcf);
 }
 
 // calculatePOS's under-relaxation (next - pos) * 0.6 + pos (PDSolver.cu:
@@ -812,6 +834,7 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
 // owns it.
 constexpr int kFuseJacobi = 0, kFuseProlong = 1;
 constexpr int kFuseMaxNv = 256;  // vertices of a tile the fused sweep holds in LDS
+static_assert(kFuseMaxNv <= kTileTets, "one lane per tile vertex: the sums' rows load with the vertex");
 template <int MODE, int PHASE>
 __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
     int T, const float4* __restrict__ xcur, const float4* __restrict__ xprv, float4* __restrict__ xout,
@@ -828,7 +851,10 @@ __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
     __shared__ float4 lu[kFuseMaxNv], ls[kFuseMaxNv];  // the tile's new positions and s
     __shared__ TileLds L;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int v0 = tvo[blk], nv = tvo[blk + 1] - v0;  // nv <= kFuseMaxNv (checked at set-up)
+    // lane l < nv takes tile vertex l in both phases (nv <= kFuseMaxNv,
+    // checked at set-up)
+    const TileRow tr = tile_row(blk, tvo, tvs, tlr);
+    const int v0 = tr.v0, nv = tr.nv;
     const int g = blk * kTileTets + threadIdx.x;
     // the tet's streams first: in flight during phase 1
     int4 q = make_int4(0, 0, 0, 0);
@@ -841,7 +867,8 @@ __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
         vt = vol[g];
         if (tk) kk = tk[g];
     }
-    for (int l = threadIdx.x; l < nv; l += kTileTets) {
+    const int l = threadIdx.x;
+    if (l < nv) {
         // (vertex, its row of partials, owner flag): the rows and the vertex's
         // streams load in one round trip
         const int4 tq = tvq[v0 + l];
@@ -890,7 +917,7 @@ __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
         }
     }
     __syncthreads();
-    tile_sum(L, blk, tvo, tvs, tlr, pout);
+    if (l < nv) pout[tr.slot] = tile_row_sum(L, tr.r0, tr.r1);
 }
 
 // updatePointInTet (PDSolver.cu:724-744): dst.x = dst.x_prev = sum w src.x.

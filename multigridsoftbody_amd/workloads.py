@@ -125,4 +125,6 @@ def blob_fixflag(mesh, xmax=-5.0):
     return (mesh.pos[:, 0] >= xmax).astype(np.float32)
 
 
-BLOB = dict(name="organ_blob_h0.225", steps=300, gravity=True)
+# max_cycles 30: a few steps hold inverted tets next to the pinned region and
+# plain MG from s then needs more than 12 cycles; the cap is a safety net
+BLOB = dict(name="organ_blob_h0.225", steps=300, gravity=True, max_cycles=30)

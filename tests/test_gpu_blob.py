@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 
 def blob_solver(lv, schedule, **kw):
-    base = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=12, check_every=2, tol=1e-3,
+    base = dict(pre_sweeps=5, post_sweeps=5, coarse_sweeps=2, max_cycles=W.BLOB["max_cycles"], check_every=2, tol=1e-3,
                 rho_smooth=0.9, use_graph=1)
     base.update(kw)
     s = M.Solver(lv, schedule=schedule, if_mode=M.IF_STABLE, **base, **W.physics())
@@ -28,10 +28,10 @@ def test_blob_vcycle_and_learned(h):
     (h = 0.225, 194,148 tets), 120 steps per schedule.  The blob swings about
     its pinned end by up to 15 m and a few steps hold inverted tets next to
     the pinned region, where the polar decomposition's force field is not
-    smooth; such a step can end at the cycle cap above the target.  Checked:
-    finite positions, pinned vertices at rest, at most 2 of the 120 steps over
-    the target and none above 2e-2, and the warm starts' gain in half-cycles
-    per step over plain MG."""
+    smooth and the Chebyshev extrapolation can make a check jump (the stall
+    guard then sweeps without it).  Checked: finite positions, pinned
+    vertices at rest, every step within the target, and the warm starts'
+    gain in half-cycles per step over plain MG."""
     lv = W.blob_hierarchy(h=h)
     assert lv[0].ntets > 20000
     if h == W.BLOB_H:
@@ -54,7 +54,7 @@ def test_blob_vcycle_and_learned(h):
         res[name] = run["total_halves"] / run["steps"]
         print(f"[blob h={h}] {name:9s} half-cycles/step {res[name]:.2f} fallbacks {run['fallbacks']} "
               f"steps over tol {run['failed_steps']} max rel {run['max_rel']:.2e}")
-        assert run["steps"] == 120 and run["failed_steps"] <= 2 and run["max_rel"] <= 2e-2
+        assert run["steps"] == 120 and run["failed_steps"] == 0 and run["max_rel"] <= 1e-3
         x = s.positions()
         assert np.all(np.isfinite(x))
         pinned = W.blob_fixflag(lv[0]) == 0
