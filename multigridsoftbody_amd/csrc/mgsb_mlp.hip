@@ -237,9 +237,11 @@ __device__ __forceinline__ bf16x4 frag_b(const short* __restrict__ Wb, int nkb, 
 // product.  Y = mlp(X).
 constexpr int kH = MLP_HIDDEN;
 // waves per inference workgroup: each computes kH / 16 / kMlpWaves of a
-// hidden layer's 16-column tiles (8 waves, one tile each, measured slower on
-// the learned stage: 10.6 against 9.0 us)
-constexpr int kMlpWaves = 4;
+// hidden layer's 16-column tiles.  Eight waves, one tile each: 7.0-7.3 us per
+// launch of the learned stage on the 1,025-vertex level against 7.4-7.5 with
+// four and 8.6 with two (with the activations' rows unpadded, whose reads
+// conflicted 16-way, eight had measured slower: 10.6 against 9.0)
+constexpr int kMlpWaves = 8;
 
 // Row-major FP32 inputs X (n x in) and outputs Y (row stride ldy).
 struct DenseIn {
@@ -266,7 +268,9 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp_infer(int n, int nout, I
                                                    const float* __restrict__ b2, const float* __restrict__ b3,
                                                    const float* __restrict__ b4, const int* __restrict__ skip) {
     if (skip && *skip) return;
-    __shared__ __align__(16) short hs[2][16][kH];
+    // rows padded by 8 bytes: the 16 rows a wave's A-fragment read spans
+    // (and the four rows of an activation store) fall in different banks
+    __shared__ __align__(16) short hs[2][16][kH + 4];
     // per-row values the input functor hands to the output functor (the
     // coarse correction's prior), written by wave 0's loads
     __shared__ float pri[16][4];
