@@ -297,7 +297,9 @@ def cpu_baseline(levels, cfg, schedule, budget_s=12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the workload's length: medium 500, large 300, batched 200, "
+                         "blob 300, small 200)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="medium", choices=["small", "medium", "large", "batched", "blob"])
     ap.add_argument("--schedule", default="ab",
@@ -316,8 +318,10 @@ def main():
               "(multi-GPU sharding of the bodies is not built).", file=sys.stderr)
         sys.exit(2)
 
-    log(f"[bench] workload={args.workload} schedule={args.schedule} steps={args.steps} warmup={args.warmup}")
     cfg, levels = build_workload(args.workload)
+    if args.steps is None:
+        args.steps = cfg["steps"]
+    log(f"[bench] workload={args.workload} schedule={args.schedule} steps={args.steps} warmup={args.warmup}")
     names = ["vcycle", "learned_s", "temporal", "learned"] if args.schedule == "ab" else [args.schedule]
     if args.schedule == "ab" and args.workload in ("large", "batched"):
         names = ["vcycle", "temporal"]  # the network is trained and measured on the medium box
