@@ -2291,7 +2291,25 @@ __global__ void __launch_bounds__(256) k_tail_gemv(int V, const float* __restric
     if (row >= V) return;
     const float* b = Bt + (size_t)row * V;
     float ax = 0.f, ay = 0.f, az = 0.f;
-    for (int j = lane; j < V; j += 64) {
+    // a lane's entries j = lane, lane + 64, ... in that order; eight rows'
+    // worth of loads in flight before their products
+    int j = lane;
+    for (; j + 7 * 64 < V; j += 8 * 64) {
+        float w[8];
+        float4 q[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            w[k] = b[j + k * 64];
+            q[k] = r[j + k * 64];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            ax = ax + w[k] * q[k].x;
+            ay = ay + w[k] * q[k].y;
+            az = az + w[k] * q[k].z;
+        }
+    }
+    for (; j < V; j += 64) {
         const float w = b[j];
         const float4 q = r[j];
         ax = ax + w * q.x;
