@@ -494,8 +494,25 @@ __device__ __forceinline__ TileRow tile_row(int tile, const int* __restrict__ tv
     return t;
 }
 __device__ __forceinline__ float3 tile_row_sum(const TileLds& L, int r0, int r1) {
+    // in row order; four entries' LDS reads in flight before their sums
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    for (int e = r0; e < r1; e++) {
+    int e = r0;
+    for (; e + 4 <= r1; e += 4) {
+        float x[4], y[4], z[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            x[k] = L.x[e + k];
+            y[k] = L.y[e + k];
+            z[k] = L.z[e + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            fx = fx + x[k];
+            fy = fy + y[k];
+            fz = fz + z[k];
+        }
+    }
+    for (; e < r1; e++) {
         fx = fx + L.x[e];
         fy = fy + L.y[e];
         fz = fz + L.z[e];
