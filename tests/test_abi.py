@@ -43,3 +43,18 @@ def test_errors_are_codes_not_exits(tmp_path):
     bad = tmp_path / "bad.msh"
     bad.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n3\n1 0 0 0\n")
     assert M.lib.mgsb_load_msh(str(bad).encode(), ctypes.byref(m)) == -3
+
+
+def test_diagnostics_reject_bad_arguments_without_a_device():
+    """The run aggregates and the per-step check history validate their
+    arguments before touching the device: a null handle or output, or a
+    negative length, is MGSB_ERR_ARG with a message."""
+    import multigridsoftbody_amd as M
+
+    n = ctypes.c_int32(-7)
+    buf = (ctypes.c_double * 4)()
+    assert M.lib.mgsb_step_history(None, buf, 4, ctypes.byref(n)) == -1
+    assert b"mgsb_step_history" in M.lib.mgsb_last_error()
+    assert M.lib.mgsb_step_history(None, None, 0, None) == -1
+    assert M.lib.mgsb_run_stats(None, None, 0) == -1
+    assert b"mgsb_run_stats" in M.lib.mgsb_last_error()
