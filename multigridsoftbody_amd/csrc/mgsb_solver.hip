@@ -1733,12 +1733,28 @@ __global__ void __launch_bounds__(256) k_restrict(int Vc, const int* __restrict_
     const int lane = threadIdx.x & 63;
     if (j >= Vc) return;  // whole wave
     float ax = 0.f, ay = 0.f, az = 0.f;
-    for (int e = rtp[j] + lane; e < rtp[j + 1]; e += 64) {
-        const float w = rtw[e];
-        const float4 q = rf[rti[e]];
-        ax = ax + w * q.x;
-        ay = ay + w * q.y;
-        az = az + w * q.z;
+    // a lane's entries e = start + lane, + 64, ... in that order, four of
+    // them (rows up to 256 entries) in one round trip of loads
+    const int e1 = rtp[j + 1];
+    for (int e = rtp[j] + lane; e < e1; e += 4 * 64) {
+        float w[4];
+        int id[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const bool on = e + k * 64 < e1;
+            w[k] = on ? rtw[e + k * 64] : 0.f;
+            id[k] = on ? rti[e + k * 64] : 0;
+        }
+        float4 q[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) q[k] = e + k * 64 < e1 ? rf[id[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (e + k * 64 < e1) {
+                ax = ax + w[k] * q[k].x;
+                ay = ay + w[k] * q[k].y;
+                az = az + w[k] * q[k].z;
+            }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
