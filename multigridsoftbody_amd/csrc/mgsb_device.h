@@ -30,6 +30,13 @@ __device__ __forceinline__ int xcd_block(int bid, int nb) {
     return x * per + (x < rem ? x : rem) + local;
 }
 
+// A kernel's device-side flags (skip, stall guard) are read through this: a
+// null flag reads the zero below, so the read is an unconditional load that
+// issues with the kernel's first loads.  A test of the pointer ahead of the
+// load would make every later load wait for that branch.
+__device__ const int kFlagOff = 0;
+__device__ __forceinline__ int load_flag(const int* f) { return *(f ? f : &kFlagOff); }
+
 __device__ __forceinline__ float3 f3(float4 a) { return make_float3(a.x, a.y, a.z); }
 __device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __device__ __forceinline__ float3 add3(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }

@@ -400,19 +400,24 @@ __device__ __forceinline__ void tet_forces_at(const float (&b)[9], float vt, flo
     }
 }
 
-template <int MODE>
-__device__ __forceinline__ void tet_local_forces(int g, int T, int4 id, const float4* __restrict__ x,
-                                                 const float* __restrict__ B, const float* __restrict__ vol,
-                                                 float k, const float4* __restrict__ sref,
-                                                 const float* __restrict__ tk, float (&fo)[12]) {
-    // every load is issued up front: the per-tet streams and the index (id,
-    // loaded by the caller) in the first round trip, the corner gathers of x
-    // (and s) in the second
+// The per-tet streams of tet g: Dm^-1 (B), rest volume, ARAP weight.
+struct TetIn {
     float b[9];
+    float vt, kk;
+};
+__device__ __forceinline__ TetIn tet_streams(int g, int T, const float* __restrict__ B, const float* __restrict__ vol,
+                                             float k, const float* __restrict__ tk) {
+    TetIn in;
 #pragma unroll
-    for (int q = 0; q < 9; q++) b[q] = B[(size_t)q * T + g];
-    const float vt = vol[g];
-    const float kk = tk ? tk[g] : k;  // per-body stiffness when several bodies have their own E
+    for (int q = 0; q < 9; q++) in.b[q] = B[(size_t)q * T + g];
+    in.vt = vol[g];
+    in.kk = tk ? tk[g] : k;  // per-body stiffness when several bodies have their own E
+    return in;
+}
+// The corner gathers of x (and s) and the local step's forces.
+template <int MODE>
+__device__ __forceinline__ void tet_forces_in(const TetIn& in, int4 id, const float4* __restrict__ x,
+                                              const float4* __restrict__ sref, float (&fo)[12]) {
     const float4 p0 = x[id.x], p1 = x[id.y], p2 = x[id.z], p3 = x[id.w];
     float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0, s2 = s0, s3 = s0;
     if (sref) {
@@ -421,11 +426,20 @@ __device__ __forceinline__ void tet_local_forces(int g, int T, int4 id, const fl
         s2 = sref[id.z];
         s3 = sref[id.w];
     }
-    tet_forces_at<MODE>(b, vt, kk, p0, p1, p2, p3, sref != nullptr, s0, s1, s2, s3, fo);
+    tet_forces_at<MODE>(in.b, in.vt, in.kk, p0, p1, p2, p3, sref != nullptr, s0, s1, s2, s3, fo);
+}
+template <int MODE>
+__device__ __forceinline__ void tet_local_forces(int g, int T, int4 id, const float4* __restrict__ x,
+                                                 const float* __restrict__ B, const float* __restrict__ vol,
+                                                 float k, const float4* __restrict__ sref,
+                                                 const float* __restrict__ tk, float (&fo)[12]) {
+    // every load is issued up front: the per-tet streams and the index (id,
+    // loaded by the caller) in the first round trip, the corner gathers of x
+    // (and s) in the second
+    const TetIn in = tet_streams(g, T, B, vol, k, tk);
+    tet_forces_in<MODE>(in, id, x, sref, fo);
 }
 
-// The local step of tet g with corner j's force stored to cf[slot_j] (the
-// per-corner CSR layout).
 template <int MODE>
 __device__ __forceinline__ void tet_local_step(int g, int T, const float4* __restrict__ x,
                                                const int4* __restrict__ tet, const float* __restrict__ B,
@@ -473,25 +487,31 @@ __device__ __forceinline__ void tile_put(TileLds& L, int4 q, const float (&fo)[1
     }
 }
 
-// The tile's vertex range and, for lane threadIdx.x, its vertex's row of the
-// tile's corner array and its partial's slot: loaded before the local step,
-// so that tile_sum's first pass reads no index after the barrier.
+// For lane threadIdx.x, its tile vertex's row of the tile's corner array,
+// its partial's slot and whether the lane holds a vertex, from the padded
+// per-tile rows (tpad[tile * stride + lane]: (row start, row end, slot, 1),
+// zeros past the tile's vertex count).  The address depends on the tile
+// alone, so the load goes out with the tet streams and the kernel's flags in
+// its first round trip, instead of after a load of the tile's vertex range.
+// Loaded before the local step, so tile_sum's first pass reads no index
+// after the barrier.
 struct TileRow {
-    int v0, nv, r0, r1, slot;
+    int r0, r1, slot, live;
 };
-__device__ __forceinline__ TileRow tile_row(int tile, const int* __restrict__ tvo, const int* __restrict__ tvs,
-                                            const int* __restrict__ tlr) {
-    TileRow t;
-    t.v0 = tvo[tile];
-    t.nv = tvo[tile + 1] - t.v0;
-    t.r0 = t.r1 = t.slot = 0;
+// A tiled level's per-tile metadata as the tiled kernels take it.
+struct TileMeta {
+    const int4* tpad;  // padded rows (tile_row), stride entries per tile
+    const int *tvo, *tvs, *tlr;
+    int stride;  // entries per tile in tpad: the largest tile's vertex count rounded up to 64, at most kTileTets
+    int over;    // 1 if a tile holds more than kTileTets vertices
+};
+__device__ __forceinline__ TileRow tile_row(int tile, int stride, const int4* __restrict__ tpad) {
+    // lanes past the stride read the tile's last entry and drop it: no branch
+    // around the load, which would wait for it at the branch's end
     const int l = threadIdx.x;
-    if (l < t.nv) {
-        t.r0 = tlr[t.v0 + tile + l];
-        t.r1 = tlr[t.v0 + tile + l + 1];
-        t.slot = tvs[t.v0 + l];
-    }
-    return t;
+    const int4 e = tpad[(size_t)tile * stride + min(l, stride - 1)];
+    const bool in = l < stride;
+    return TileRow{e.x, e.y, e.z, in ? e.w : 0};
 }
 __device__ __forceinline__ float3 tile_row_sum(const TileLds& L, int r0, int r1) {
     // in row order; four entries' LDS reads in flight before their sums
@@ -519,12 +539,18 @@ __device__ __forceinline__ float3 tile_row_sum(const TileLds& L, int r0, int r1)
     }
     return make_float3(fx, fy, fz);
 }
-__device__ __forceinline__ void tile_sum(const TileLds& L, const TileRow& t, int tile, const int* __restrict__ tvs,
+// over: the level has a tile of more than kTileTets vertices (none on the
+// generated meshes); its lanes then also take vertices kTileTets, ... of the
+// tile's list (tvo, tvs, tlr).
+__device__ __forceinline__ void tile_sum(const TileLds& L, const TileRow& t, int tile, int over,
+                                         const int* __restrict__ tvo, const int* __restrict__ tvs,
                                          const int* __restrict__ tlr, float3* __restrict__ pcf) {
-    if ((int)threadIdx.x < t.nv) pcf[t.slot] = tile_row_sum(L, t.r0, t.r1);
-    const int* lr = tlr + t.v0 + tile;
-    for (int l = threadIdx.x + kTileTets; l < t.nv; l += kTileTets)
-        pcf[tvs[t.v0 + l]] = tile_row_sum(L, lr[l], lr[l + 1]);
+    if (t.live) pcf[t.slot] = tile_row_sum(L, t.r0, t.r1);
+    if (!over) return;
+    const int v0 = tvo[tile], nv = tvo[tile + 1] - v0;
+    const int* lr = tlr + v0 + tile;
+    for (int l = threadIdx.x + kTileTets; l < nv; l += kTileTets)
+        pcf[tvs[v0 + l]] = tile_row_sum(L, lr[l], lr[l + 1]);
 }
 
 // Triangle-sphere DCD of surface triangle t (PDSolver.cu:612-680): writes
@@ -579,31 +605,37 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
 
 // The tiled form of k_tet (tile_put / tile_sum): workgroups 0..ntiles-1 run
 // one tile each (kTileTets lanes), the ones after them the triangle lanes.
+// The skip flag, the tile's rows and the tet's streams load in one round
+// trip; a skipped launch returns before the corner gathers.
 template <int MODE, bool TRI>
 __global__ void __launch_bounds__(kTileTets) k_tet_tiled(
     int T, int ntiles, const float4* __restrict__ x, const int4* __restrict__ tetp, const float* __restrict__ B,
-    const float* __restrict__ vol, const int* __restrict__ tvo, const int* __restrict__ tvs,
-    const int* __restrict__ tlr, float3* __restrict__ pcf, float k, int ntri,
+    const float* __restrict__ vol, TileMeta tm, float3* __restrict__ pcf, float k, int ntri,
     const int4* __restrict__ tri, const int4* __restrict__ tslot, float4* __restrict__ tcf,
     const DevColliders* __restrict__ col, const int* __restrict__ skip, const float4* __restrict__ sref,
     const float* __restrict__ tk) {
-    if (skip && *skip) return;
+    const int sk = load_flag(skip);
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     if (blk >= ntiles) {
+        if (sk) return;
         if (TRI) tri_lane((blk - ntiles) * kTileTets + threadIdx.x, ntri, x, tri, tslot, tcf, col);
         return;
     }
     __shared__ TileLds L;
     const int g = blk * kTileTets + threadIdx.x;
-    const TileRow tr = tile_row(blk, tvo, tvs, tlr);
+    const TileRow tr = tile_row(blk, tm.stride, tm.tpad);
+    // branch-free loads: the last tile's lanes past T read tet T - 1 (tile_row)
+    const int gc = min(g, T - 1);
+    const int4 q = tetp[gc];
+    const TetIn in = tet_streams(gc, T, B, vol, k, tk);
+    if (sk) return;  // the same flag for the whole workgroup
     if (g < T) {
-        const int4 q = tetp[g];
         float fo[12];
-        tet_local_forces<MODE>(g, T, tile_ids(q), x, B, vol, k, sref, tk, fo);
+        tet_forces_in<MODE>(in, tile_ids(q), x, sref, fo);
         tile_put(L, q, fo);
     }
     __syncthreads();
-    tile_sum(L, tr, blk, tvs, tlr, pcf);
+    tile_sum(L, tr, blk, tm.over, tm.tvo, tm.tvs, tm.tlr, pcf);
 }
 
 // Start of a temporally warm-started V-cycle step: lanes 0..T-1 evaluate the
@@ -629,15 +661,14 @@ __global__ void __launch_bounds__(256) k_tet_dual(int T, const float4* __restric
 template <int MODE>
 __global__ void __launch_bounds__(kTileTets) k_tet_dual_tiled(
     int T, int ntiles, const float4* __restrict__ x, const float4* __restrict__ s, const int4* __restrict__ tetp,
-    const float* __restrict__ B, const float* __restrict__ vol, const int* __restrict__ tvo,
-    const int* __restrict__ tvs, const int* __restrict__ tlr, float3* __restrict__ pcf,
+    const float* __restrict__ B, const float* __restrict__ vol, TileMeta tm, float3* __restrict__ pcf,
     float3* __restrict__ pcf2, float k, const float* __restrict__ tk) {
     const int b0 = xcd_block(blockIdx.x, gridDim.x);
     const bool at_s = b0 >= ntiles;
     const int blk = at_s ? b0 - ntiles : b0;
     __shared__ TileLds L;
     const int g = blk * kTileTets + threadIdx.x;
-    const TileRow tr = tile_row(blk, tvo, tvs, tlr);
+    const TileRow tr = tile_row(blk, tm.stride, tm.tpad);
     if (g < T) {
         const int4 q = tetp[g];
         float fo[12];
@@ -645,7 +676,7 @@ __global__ void __launch_bounds__(kTileTets) k_tet_dual_tiled(
         tile_put(L, q, fo);
     }
     __syncthreads();
-    tile_sum(L, tr, blk, tvs, tlr, at_s ? pcf2 : pcf);
+    tile_sum(L, tr, blk, tm.over, tm.tvo, tm.tvs, tm.tlr, at_s ? pcf2 : pcf);
 }
 
 // calculatePOS's under-relaxation (next - pos) * 0.6 + pos (PDSolver.cu:
@@ -846,49 +877,49 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
 // same order, so every copy equals the owner's.  Positions rotate over three
 // buffers (xcur and xprv read, xout written) and partials over two, so no
 // launch reads what it writes.  tetq: per corner, its tile-local vertex index
-// (low 16 bits) and its position in the tile's corner array (high 16); tvq:
-// per (tile, vertex), the vertex, its row of partials and whether the tile
-// owns it.
+// (low 16 bits) and its position in the tile's corner array (high 16); tvqp:
+// per (tile, vertex), padded as the tile rows (tile_row), the vertex, its row
+// of partials and whether the tile owns it.
 constexpr int kFuseJacobi = 0, kFuseProlong = 1;
 constexpr int kFuseMaxNv = 256;  // vertices of a tile the fused sweep holds in LDS
 static_assert(kFuseMaxNv <= kTileTets, "one lane per tile vertex: the sums' rows load with the vertex");
 template <int MODE, int PHASE>
 __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
     int T, const float4* __restrict__ xcur, const float4* __restrict__ xprv, float4* __restrict__ xout,
-    const float4* __restrict__ s, const float4* __restrict__ vc, const float3* __restrict__ pin,
-    const int* __restrict__ tvo, const int* __restrict__ tvs, const int* __restrict__ tlr,
-    const int4* __restrict__ tvq, const int4* __restrict__ tetq,
+    const float4* __restrict__ s, const float4* __restrict__ vc, const float3* __restrict__ pin, TileMeta tm,
+    const int4* __restrict__ tvqp, const int4* __restrict__ tetq,
     const float* __restrict__ B, const float* __restrict__ vol, float k, const float* __restrict__ tk,
     float3* __restrict__ pout, const DevColliders* __restrict__ col, int flags, float omega, float relax_hi,
     float relax_lo, int* __restrict__ cflag, float* __restrict__ cdepth, float* __restrict__ cenergy,
     const float4* __restrict__ ec, const int4* __restrict__ iids, const float4* __restrict__ iw,
     const float* __restrict__ amask, const int* __restrict__ skip, const int* __restrict__ calm) {
-    if (skip && *skip) return;
-    if (calm && *calm) omega = 1.0f;  // stall guard (k_vert)
+    // first round trip: the flags, the tile's rows (padded per tile, so their
+    // addresses need no earlier load) and the tet's streams
+    const int sk = load_flag(skip);
+    const int cm = load_flag(calm);
     __shared__ float4 lu[kFuseMaxNv], ls[kFuseMaxNv];  // the tile's new positions and s
     __shared__ TileLds L;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    // lane l < nv takes tile vertex l in both phases (nv <= kFuseMaxNv,
-    // checked at set-up)
-    const TileRow tr = tile_row(blk, tvo, tvs, tlr);
-    const int v0 = tr.v0, nv = tr.nv;
-    const int g = blk * kTileTets + threadIdx.x;
-    // the tet's streams first: in flight during phase 1
-    int4 q = make_int4(0, 0, 0, 0);
-    float b[9];
-    float vt = 0.f, kk = k;
-    if (g < T) {
-        q = tetq[g];
-#pragma unroll
-        for (int c = 0; c < 9; c++) b[c] = B[(size_t)c * T + g];
-        vt = vol[g];
-        if (tk) kk = tk[g];
-    }
+    // lane l takes tile vertex l in both phases (a tile has at most
+    // kFuseMaxNv vertices, checked at set-up)
     const int l = threadIdx.x;
-    if (l < nv) {
-        // (vertex, its row of partials, owner flag): the rows and the vertex's
-        // streams load in one round trip
-        const int4 tq = tvq[v0 + l];
+    const TileRow tr = tile_row(blk, tm.stride, tm.tpad);
+    // (vertex, its row of partials, owner flag), padded as tpad
+    const int4 tq = tvqp[(size_t)blk * tm.stride + min(l, tm.stride - 1)];
+    // the tet's streams; the last tile's lanes past T read tet T - 1 and
+    // drop its forces (branch-free loads, as tile_row)
+    const int g = blk * kTileTets + l;
+    const int gc = min(g, T - 1);
+    const int4 q = tetq[gc];
+    float b[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) b[c] = B[(size_t)c * T + gc];
+    const float vt = vol[gc];
+    const float kk = tk ? tk[gc] : k;
+    if (sk) return;  // the same flag for the whole workgroup
+    if (cm) omega = 1.0f;  // stall guard (k_vert)
+    if (tr.live) {
+        // second round trip: the vertex's streams and its row of partials
         const int v = tq.x, r0 = tq.y, r1 = tq.z;
         const bool owner = tq.w != 0;
         const float4 st = s[v];
@@ -934,7 +965,7 @@ __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
         }
     }
     __syncthreads();
-    if (l < nv) pout[tr.slot] = tile_row_sum(L, tr.r0, tr.r1);
+    if (tr.live) pout[tr.slot] = tile_row_sum(L, tr.r0, tr.r1);
 }
 
 // updatePointInTet (PDSolver.cu:724-744): dst.x = dst.x_prev = sum w src.x.
@@ -2416,12 +2447,14 @@ struct DevLevel {
     int ntiles = 0, npart = 0;
     int *prp = nullptr, *tvo = nullptr, *tvs = nullptr, *tlr = nullptr;
     int4* tetp = nullptr;  // tets with each corner's position in its tile's corner array (nullptr: not tiled)
+    int4* tpad = nullptr;  // padded tile rows (tile_row), tstride entries per tile
+    int tstride = 0, tover = 0;
     float3 *pcf = nullptr, *pcf2 = nullptr;
     // fused sweeps (k_sweep_fused): per corner its tile-local vertex index and
     // corner position; the tiles' vertex lists; a third position buffer
     // (nullptr: the level does not run fused sweeps)
     int4* tetq = nullptr;
-    int4* tvq = nullptr;  // per (tile, vertex): vertex id, its prp row [y, z), 1 if this tile owns it
+    int4* tvqp = nullptr;  // per (tile, vertex), padded as tpad: vertex id, its prp row [y, z), 1 if this tile owns it
     float4* xz = nullptr;
     int4 *tri = nullptr, *tslot = nullptr;
     int *trp = nullptr, *trid = nullptr;
@@ -2635,6 +2668,22 @@ int upload_level(mgsb_solver* s, DevLevel& d) {
     if ((rc = up(s, d.owned, &d.tvs, h.tvs.data(), h.tvs.size()))) return rc;
     if ((rc = up(s, d.owned, &d.tlr, h.tlr.data(), h.tlr.size()))) return rc;
     if ((rc = up<float3>(s, d.owned, &d.pcf, nullptr, (size_t)d.npart))) return rc;
+    {
+        // padded tile rows: tile k's lane l < its vertex count holds (row
+        // start, row end, slot, 1) of tile vertex l, the rest zeros
+        int maxnv = 0;
+        for (int k = 0; k < d.ntiles; k++) maxnv = std::max(maxnv, h.tvo[(size_t)k + 1] - h.tvo[(size_t)k]);
+        d.tstride = std::min(kTileTets, std::max(64, (maxnv + 63) / 64 * 64));
+        d.tover = maxnv > kTileTets ? 1 : 0;
+        std::vector<int4> tp((size_t)std::max(1, d.ntiles) * d.tstride, make_int4(0, 0, 0, 0));
+        for (int k = 0; k < d.ntiles; k++) {
+            const int v0 = h.tvo[(size_t)k], nv = std::min(kTileTets, h.tvo[(size_t)k + 1] - v0);
+            for (int l = 0; l < nv; l++)
+                tp[(size_t)k * d.tstride + l] =
+                    make_int4(h.tlr[(size_t)v0 + k + l], h.tlr[(size_t)v0 + k + l + 1], h.tvs[(size_t)v0 + l], 1);
+        }
+        if ((rc = up(s, d.owned, &d.tpad, tp.data(), tp.size()))) return rc;
+    }
     if (h.V <= (1 << 22)) {
         std::vector<int4> tp(h.T);
         for (int t = 0; t < h.T; t++) {
@@ -2664,13 +2713,18 @@ int upload_level(mgsb_solver* s, DevLevel& d) {
                 tq[t] = make_int4(c[0], c[1], c[2], c[3]);
             }
             if ((rc = up(s, d.owned, &d.tetq, tq.data(), tq.size()))) return rc;
-            std::vector<int4> tv(h.tvid.size());
-            for (size_t q = 0; q < tv.size(); q++) {
-                const int v = h.tvid[q];
-                const int r0 = h.prp[(size_t)v], r1 = h.prp[(size_t)v + 1];
-                tv[q] = make_int4(v, r0, r1, h.tvs[q] == r0 ? 1 : 0);  // the first tile of a row owns the vertex
+            std::vector<int4> tv((size_t)std::max(1, d.ntiles) * d.tstride, make_int4(0, 0, 0, 0));
+            for (int k = 0; k < d.ntiles; k++) {
+                const int v0 = h.tvo[(size_t)k], nv = h.tvo[(size_t)k + 1] - v0;
+                for (int l = 0; l < nv; l++) {
+                    const size_t q = (size_t)v0 + l;
+                    const int v = h.tvid[q];
+                    const int r0 = h.prp[(size_t)v], r1 = h.prp[(size_t)v + 1];
+                    // the first tile of a row owns the vertex
+                    tv[(size_t)k * d.tstride + l] = make_int4(v, r0, r1, h.tvs[q] == r0 ? 1 : 0);
+                }
             }
-            if ((rc = up(s, d.owned, &d.tvq, tv.data(), tv.size()))) return rc;
+            if ((rc = up(s, d.owned, &d.tvqp, tv.data(), tv.size()))) return rc;
         }
     }
     std::vector<int4> tr4(std::max(1, h.S)), ts4(std::max(1, h.S));
@@ -2761,6 +2815,8 @@ int timed_end(mgsb_solver* s, int kernel) {
     return MGSB_OK;
 }
 
+TileMeta tile_meta(const DevLevel& d) { return TileMeta{d.tpad, d.tvo, d.tvs, d.tlr, d.tstride, d.tover}; }
+
 // split: x holds the in-step correction u and the kernel adds F(s) (V-cycle steps).
 template <int MODE>
 void launch_tet_mode(mgsb_solver* s, DevLevel& d, int nb, int bs, const float4* xx, int ntri, const int* skip,
@@ -2768,7 +2824,7 @@ void launch_tet_mode(mgsb_solver* s, DevLevel& d, int nb, int bs, const float4* 
     if (tiled) {
         auto k = ntri ? k_tet_tiled<MODE, true> : k_tet_tiled<MODE, false>;
         k<<<d.ntiles + nblk(ntri, kTileTets), kTileTets, 0, s->stream>>>(
-            d.T, d.ntiles, xx, d.tetp, d.B, d.vol, d.tvo, d.tvs, d.tlr, d.pcf, s->p.stiffness, ntri, d.tri,
+            d.T, d.ntiles, xx, d.tetp, d.B, d.vol, tile_meta(d), d.pcf, s->p.stiffness, ntri, d.tri,
             d.tslot, d.tcf, s->dcol, skip, sref, d.tk);
         return;
     }
@@ -2821,8 +2877,8 @@ int launch_tet_dual(mgsb_solver* s, DevLevel& d) {
                      ? k_tet_dual_tiled<kTetModePolar>
                      : (tet_mode_of(s->p.if_mode) == kTetModeAc ? k_tet_dual_tiled<kTetModeAc>
                                                                   : k_tet_dual_tiled<kTetModeStable>);
-        k<<<nb, kTileTets, 0, s->stream>>>(d.T, d.ntiles, d.x, d.s, d.tetp, d.B, d.vol, d.tvo, d.tvs, d.tlr,
-                                           d.pcf, d.pcf2, s->p.stiffness, d.tk);
+        k<<<nb, kTileTets, 0, s->stream>>>(d.T, d.ntiles, d.x, d.s, d.tetp, d.B, d.vol, tile_meta(d), d.pcf,
+                                           d.pcf2, s->p.stiffness, d.tk);
         HIPCHK(hipGetLastError());
         return MGSB_OK;
     }
@@ -3054,7 +3110,7 @@ void launch_fused_mode(mgsb_solver* s, DevLevel& d, const FuseBufs& fb, float4* 
                        const int* skip, const float4* ec, const float* amask) {
     LevelMap& m = s->M[0];
     k_sweep_fused<MODE, PHASE><<<d.ntiles, kTileTets, 0, s->stream>>>(
-        d.T, fb.cur, fb.prev, out, d.s, d.vc, fb.pin ? d.pcf2 : d.pcf, d.tvo, d.tvs, d.tlr, d.tvq, d.tetq,
+        d.T, fb.cur, fb.prev, out, d.s, d.vc, fb.pin ? d.pcf2 : d.pcf, tile_meta(d), d.tvqp, d.tetq,
         d.B, d.vol, s->p.stiffness, d.tk, fb.pin ? d.pcf : d.pcf2, s->dcol, flags, omega, s->relax_hi, s->relax_lo,
         d.cflag, d.cdepth, d.cenergy, ec, m.iids, m.iw, amask, skip, calm_flag(s, 1));
 }
