@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "mgsb_device.h"
 #include "mgsb_internal.h"
 #include "mgsb_mlp.h"
 
@@ -267,7 +268,9 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp_infer(int n, int nout, I
                                                    const short* __restrict__ W4, const float* __restrict__ b1,
                                                    const float* __restrict__ b2, const float* __restrict__ b3,
                                                    const float* __restrict__ b4, const int* __restrict__ skip) {
-    if (skip && *skip) return;
+    // the flag loads with the weights and biases; a skipped launch returns
+    // before the inputs' loads
+    const int sk = load_flag(skip);
     // rows padded by 8 bytes: the 16 rows a wave's A-fragment read spans
     // (and the four rows of an activation store) fall in different banks
     __shared__ __align__(16) short hs[2][16][kH + 4];
@@ -305,6 +308,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves) k_mlp_infer(int n, int nout, I
     }
     const float bias4 = (wave == 0 && r < nout) ? b4[r] : 0.f;
     const float2 opre = (wave == 0 && r < nout) ? out.pre(r) : make_float2(0.f, 1.f);
+    if (sk) return;  // the same flag for the whole workgroup
     // layer 1: K = 16 (inputs zero-padded)
     {
         bf16x4 a;

@@ -222,25 +222,35 @@ __global__ void __launch_bounds__(256) k_finish_split(int V, float4* __restrict_
                                                       const float4* __restrict__ vc, float4* __restrict__ D1,
                                                       float4* __restrict__ D2, int temporal,
                                                       const int* __restrict__ skip) {
-    if (skip && *skip) return;
-    if (vs && blockIdx.x == 0 && threadIdx.x == 0) vstate_step_done(vs);
+    // the flag and the vertex's streams in one round trip (lanes past V read
+    // vertex V - 1 and return after the flag)
+    const int sk = load_flag(skip);
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int ic = min(i, V - 1);
+    const float4 u = x[ic], up = xprev[ic], st = s[ic], q = xlast[ic];
+    float4 u1 = make_float4(0.f, 0.f, 0.f, 0.f), u2 = u1, d1 = u1;
+    float f = 0.f;
+    if (U1) {
+        u1 = U1[ic];
+        u2 = U2[ic];
+        f = vc[ic].z;
+    }
+    if (D1) d1 = D1[ic];
+    if (sk) return;
+    if (vs && blockIdx.x == 0 && threadIdx.x == 0) vstate_step_done(vs);
     if (i >= V) return;
-    const float4 u = x[i], up = xprev[i], st = s[i], q = xlast[i];
     if (D1) {
         float4 a = u;
         if (temporal) {
-            const float4 u1 = U1[i], u2 = U2[i];
-            const float f = vc[i].z;
             a.x = a.x - f * (2.f * u1.x - u2.x);
             a.y = a.y - f * (2.f * u1.y - u2.y);
             a.z = a.z - f * (2.f * u1.z - u2.z);
         }
-        D2[i] = D1[i];
+        D2[i] = d1;
         D1[i] = make_float4(a.x, a.y, a.z, 0.f);
     }
     if (U1) {  // corrections of the last two steps (temporal warm start)
-        U2[i] = U1[i];
+        U2[i] = u1;
         U1[i] = make_float4(u.x, u.y, u.z, 0.f);
     }
     const float4 p = make_float4(st.x + u.x, st.y + u.y, st.z + u.z, 0.f);
@@ -839,18 +849,22 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
                                               float relax_lo, int flags, int* __restrict__ cflag, float* __restrict__ cdepth,
                                               float* __restrict__ cenergy, const int* __restrict__ skip, int split,
                                               int lsh, const int* __restrict__ calm) {
-    if (skip && *skip) return;
-    if (calm && *calm) omega = 1.0f;  // stall guard: plain (damped) Jacobi for the rest of the step
+    // the flags, the per-vertex streams and the row bounds in one round trip
+    // (lanes past V read vertex V - 1: no branch around the loads)
+    const int sk = load_flag(skip);
+    const int cm = load_flag(calm);
     // 2^lsh lanes per vertex gather its row (gather_lanes); lane 0 finishes
     const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int i = g >> lsh, sub = g & ((1 << lsh) - 1);
-    if (i >= V) return;
-    // per-vertex streams in the same round trip as the row bounds
-    const float4 p = x[i];  // x, or the correction u when split
-    const float4 st = s[i];
-    const float4 c = vc[i];  // (m/h^2, diag, fix, m)
-    const float4 pp = xprev[i];
-    const float3 fg = gather_lanes(cf, rp[i], rp[i + 1], sub, lsh);
+    const int ic = min(i, V - 1);
+    const float4 p = x[ic];  // x, or the correction u when split
+    const float4 st = s[ic];
+    const float4 c = vc[ic];  // (m/h^2, diag, fix, m)
+    const float4 pp = xprev[ic];
+    const int e0 = rp[ic], e1 = rp[ic + 1];
+    if (sk || i >= V) return;
+    if (cm) omega = 1.0f;  // stall guard: plain (damped) Jacobi for the rest of the step
+    const float3 fg = gather_lanes(cf, e0, e1, sub, lsh);
     if (sub) return;
     int flag;
     float depth, energy;
@@ -1361,14 +1375,26 @@ __device__ __forceinline__ void bump_seq(unsigned* seqc, int* hm, const VState* 
 // order on every run) and applies the check.  A separate launch: completing the norm inside the producing
 // kernel (the last block to arrive, found with a device-scope counter) costs
 // far more on this GPU, since the blocks' atomics on one counter serialise.
+constexpr int kCheckBatch = 4;
 __global__ void __launch_bounds__(256) k_check(int nb, const double* __restrict__ part, VState* st, int mode,
                                                int* hm, const int* __restrict__ skip, unsigned* seqc) {
-    if (skip && *skip) {
+    // the flag and up to kCheckBatch partials per lane in one round trip
+    // (addresses past nb read the last partial, which is dropped)
+    const int sk = load_flag(skip);
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < nb; b += kCheckBatch * blockDim.x) {
+        double q[kCheckBatch];
+#pragma unroll
+        for (int k = 0; k < kCheckBatch; k++) q[k] = part[min(b + k * (int)blockDim.x, nb - 1)];
+        if (sk) break;
+#pragma unroll
+        for (int k = 0; k < kCheckBatch; k++)
+            if (b + k * (int)blockDim.x < nb) v[0] += q[k];
+    }
+    if (sk) {
         if (threadIdx.x == 0) bump_seq(seqc, hm, st);
         return;
     }
-    double v[1] = {0.0};
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) v[0] += part[b];
     wg_sum_fixed<1>(v);
     if (threadIdx.x == 0) {
         vstate_apply(st, v[0], mode, hm);
@@ -1448,17 +1474,22 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* x, const flo
                                                float4* xprev, const int* __restrict__ skip, int split,
                                                double* __restrict__ g2, float* __restrict__ cmask, int lsh,
                                                float4* xcanon = nullptr, const float4* xpsrc = nullptr) {
-    if (skip && *skip) return;
+    // the flag, the vertex's streams and its row bounds in one round trip
+    // (lanes past V read vertex V - 1 and drop it: no branch around the loads)
+    const int sk = load_flag(skip);
     const int gi = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gi >> lsh, sub = gi & ((1 << lsh) - 1);  // 2^lsh lanes per vertex
+    const int ic = min(i, V - 1);
+    const float4 p = x[ic], st = s[ic], c = vc[ic];
+    // x_prev's new value: loaded from a valid buffer either way (a select
+    // between xpsrc[i] and the local p would put p on the stack)
+    const float4 pl = (xpsrc ? xpsrc : x)[ic];
+    const int e0 = rp[ic], e1 = rp[ic + 1];
+    if (sk) return;
     double v[1] = {0.0};
     if (i < V) {
-        const float4 p = x[i], st = s[i], c = vc[i];
-        // x_prev's new value: loaded from a valid buffer either way (a select
-        // between xpsrc[i] and the local p would put p on the stack)
-        const float4 pl = (xpsrc ? xpsrc : x)[i];
         const float4 pq = xpsrc ? pl : p;
-        const float3 f = gather_lanes(cf, rp[i], rp[i + 1], sub, lsh);
+        const float3 f = gather_lanes(cf, e0, e1, sub, lsh);
         if (sub == 0) {
             bool contact;
             const float3 g = vertex_residual(p, st, c, f.x, f.y, f.z, col, flags, split, contact);
@@ -1759,15 +1790,18 @@ __global__ void __launch_bounds__(256) k_restrict(int Vc, const int* __restrict_
                                                   const float* __restrict__ rtw, const float4* __restrict__ rf,
                                                   const float4* __restrict__ vcl, float4* __restrict__ rc,
                                                   const int* __restrict__ skip) {
-    if (skip && *skip) return;
     const int j = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (j >= Vc) return;  // whole wave
+    // the flag, the row bounds and the row's mask in one round trip
+    const int sk = load_flag(skip);
+    const int e0 = rtp[j], e1 = rtp[j + 1];
+    const float fx = vcl[j].z;
+    if (sk) return;
     float ax = 0.f, ay = 0.f, az = 0.f;
     // a lane's entries e = start + lane, + 64, ... in that order, four of
     // them (rows up to 256 entries) in one round trip of loads
-    const int e1 = rtp[j + 1];
-    for (int e = rtp[j] + lane; e < e1; e += 4 * 64) {
+    for (int e = e0 + lane; e < e1; e += 4 * 64) {
         float w[4];
         int id[4];
 #pragma unroll
@@ -1793,10 +1827,7 @@ __global__ void __launch_bounds__(256) k_restrict(int Vc, const int* __restrict_
         ay += __shfl_down(ay, off, 64);
         az += __shfl_down(az, off, 64);
     }
-    if (lane == 0) {
-        const float fx = vcl[j].z;
-        rc[j] = make_float4(ax * fx, ay * fx, az * fx, 0.f);
-    }
+    if (lane == 0) rc[j] = make_float4(ax * fx, ay * fx, az * fx, 0.f);
 }
 
 // dst[i] += mask_i * (P src)[i]; dst2 (optional) receives the new value.
@@ -1804,11 +1835,13 @@ __global__ void __launch_bounds__(256) k_prolong_add(int N, float4* __restrict__
                                                      const float4* __restrict__ src, const int4* __restrict__ ids,
                                                      const float4* __restrict__ w, const float4* __restrict__ maskv,
                                                      const int* __restrict__ skip, const float* __restrict__ amask) {
-    if (skip && *skip) return;
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= N) return;
+    // the flag and the vertex's binding in one round trip
+    const int sk = load_flag(skip);
     const int4 id = ids[i];
     const float4 ww = w[i];
+    if (sk) return;
     const float4 a = src[id.x], b = src[id.y], c = src[id.z], d = src[id.w];
     const float vx = ww.x * a.x + ww.y * b.x + ww.z * c.x + ww.w * d.x;
     const float vy = ww.x * a.y + ww.y * b.y + ww.z * c.y + ww.w * d.y;
@@ -2347,38 +2380,36 @@ __global__ void __launch_bounds__(256) k_store_cols(int V, const float4* __restr
     if (c0 + 2 < V) row[c0 + 2] = v.z;
 }
 
+constexpr int kTailBatch = 17;  // a 1,025-vertex first tail level in one batch
 __global__ void __launch_bounds__(256) k_tail_gemv(int V, const float* __restrict__ Bt, const float4* __restrict__ r,
                                                    float4* __restrict__ e, const int* __restrict__ skip) {
-    if (skip && *skip) return;
     const int row = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (row >= V) return;
     const float* b = Bt + (size_t)row * V;
     float ax = 0.f, ay = 0.f, az = 0.f;
-    // a lane's entries j = lane, lane + 64, ... in that order; eight rows'
-    // worth of loads in flight before their products
-    int j = lane;
-    for (; j + 7 * 64 < V; j += 8 * 64) {
-        float w[8];
-        float4 q[8];
+    // a lane's entries j = lane, lane + 64, ... in that order; kTailBatch of
+    // them, with the flag, in one round trip of loads before their products
+    // (rows up to kTailBatch * 64 entries; addresses past the row read its
+    // last entry, whose product is dropped)
+    const int sk = load_flag(skip);
+    for (int j = lane; j < V; j += kTailBatch * 64) {
+        float w[kTailBatch];
+        float4 q[kTailBatch];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            w[k] = b[j + k * 64];
-            q[k] = r[j + k * 64];
+        for (int k = 0; k < kTailBatch; k++) {
+            const int jc = min(j + k * 64, V - 1);
+            w[k] = b[jc];
+            q[k] = r[jc];
         }
+        if (sk) return;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            ax = ax + w[k] * q[k].x;
-            ay = ay + w[k] * q[k].y;
-            az = az + w[k] * q[k].z;
-        }
-    }
-    for (; j < V; j += 64) {
-        const float w = b[j];
-        const float4 q = r[j];
-        ax = ax + w * q.x;
-        ay = ay + w * q.y;
-        az = az + w * q.z;
+        for (int k = 0; k < kTailBatch; k++)
+            if (j + k * 64 < V) {
+                ax = ax + w[k] * q[k].x;
+                ay = ay + w[k] * q[k].y;
+                az = az + w[k] * q[k].z;
+            }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
