@@ -30,12 +30,14 @@ __device__ __forceinline__ int xcd_block(int bid, int nb) {
     return x * per + (x < rem ? x : rem) + local;
 }
 
-// A kernel's device-side flags (skip, stall guard) are read through this: a
-// null flag reads the zero below, so the read is an unconditional load that
-// issues with the kernel's first loads.  A test of the pointer ahead of the
-// load would make every later load wait for that branch.
-__device__ const int kFlagOff = 0;
-__device__ __forceinline__ int load_flag(const int* f) { return *(f ? f : &kFlagOff); }
+// A kernel's device-side flags (skip, stall guard): a null flag reads the
+// zero word below.  The kernels read their flags with their first loads and
+// test them after issuing those loads.  The address is a select of two
+// uniform global addresses, so the read is one scalar load in the kernel's
+// first batch, and its test waits for scalar loads only (a test of the
+// pointer ahead of the load costs two more scalar round trips).
+__device__ int kFlagZero = 0;  // never written
+__device__ __forceinline__ int load_flag(const int* __restrict__ f) { return *(f ? f : &kFlagZero); }
 
 __device__ __forceinline__ float3 f3(float4 a) { return make_float3(a.x, a.y, a.z); }
 __device__ __forceinline__ float3 sub3(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
@@ -206,8 +208,8 @@ __device__ __forceinline__ void polar_rf(const float F[9], float RF[9]) {
 // reference form stalls), so it is free of the reference's operation order:
 // its multiply-adds fuse, most divisions and square roots are the
 // hardware's reciprocal, reciprocal square root and square root (about 1
-// ulp each), and cos(acos(v) / 3) is a polynomial in sqrt((1 + v) / 2)
-// (below): the local step is bound by instruction issue on large meshes.
+// ulp each), and cos(phi / 3), phi / 3 in [0, pi / 3], is the hardware
+// cosine: the local step is bound by instruction issue on large meshes.
 // lambda and I_u stay IEEE (below).  C and C^2 are formed from their six
 // distinct entries, U^-1 = gu U + q C + gdiag I with U = fc C - r C^2 +
 // fdiag I is applied as one combination a C + b C^2 + g I, and R - F is
@@ -245,23 +247,8 @@ __device__ __forceinline__ void polar_rf_stable(const float F[9], float RF[9]) {
         float value = l * rcp_hw(k * k_root);
         value = value < -1.0f ? -1.0f : value;
         value = value > 1.0f ? 1.0f : value;
-        // cos(phi / 3), phi = acos(value), without acos and cos: with
-        // w = cos(phi / 2) = sqrt((1 + value) / 2) in [0, 1] it is
-        // c(w) = cos(2/3 acos w), analytic on [0, 1] (acos(w)^2 is analytic
-        // at w = 1).  Degree-8 polynomial from Chebyshev interpolation of c
-        // on [0, 1], Horner with fused multiply-adds: |error| < 6.7e-8 (one
-        // ulp of c in [0.5, 1]), c(0) = 0.5 and c(1) = 1 exactly.
-        const float w = sqrt_hw(0.5f + 0.5f * value);
-        float c = -6.3931436e-04f;
-        c = c * w + 3.7076864e-03f;
-        c = c * w + -1.0325493e-02f;
-        c = c * w + 1.9675719e-02f;
-        c = c * w + -3.1961296e-02f;
-        c = c * w + 5.3288657e-02f;
-        c = c * w + -1.1109568e-01f;
-        c = c * w + 5.773497e-01f;
-        c = c * w + 0.5f;
-        const float lambda2 = m + 2.0f / 3 * k_root * c;
+        const float phi = acosf(value);
+        const float lambda2 = m + 2.0f / 3 * k_root * __cosf(phi * (1.0f / 3));  // phi / 3 in [0, pi/3]
         // IEEE from here to I_u: on an inverted tet near a reflection the
         // difference under the root cancels and 1 / (I_u II_u - III_u) below
         // amplifies every error in lambda and in that root

@@ -222,9 +222,9 @@ __global__ void __launch_bounds__(256) k_finish_split(int V, float4* __restrict_
                                                       const float4* __restrict__ vc, float4* __restrict__ D1,
                                                       float4* __restrict__ D2, int temporal,
                                                       const int* __restrict__ skip) {
-    // the flag and the vertex's streams in one round trip (lanes past V read
-    // vertex V - 1 and return after the flag)
-    const int sk = load_flag(skip);
+    // the skip flag first (a skipped launch loads nothing), then the vertex's
+    // streams in one round trip (lanes past V read vertex V - 1)
+    if (load_flag(skip)) return;
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int ic = min(i, V - 1);
     const float4 u = x[ic], up = xprev[ic], st = s[ic], q = xlast[ic];
@@ -236,7 +236,6 @@ __global__ void __launch_bounds__(256) k_finish_split(int V, float4* __restrict_
         f = vc[ic].z;
     }
     if (D1) d1 = D1[ic];
-    if (sk) return;
     if (vs && blockIdx.x == 0 && threadIdx.x == 0) vstate_step_done(vs);
     if (i >= V) return;
     if (D1) {
@@ -499,8 +498,10 @@ __device__ __forceinline__ void tile_put(TileLds& L, int4 q, const float (&fo)[1
 
 // For lane threadIdx.x, its tile vertex's row of the tile's corner array,
 // its partial's slot and whether the lane holds a vertex, from the padded
-// per-tile rows (tpad[tile * stride + lane]: (row start, row end, slot, 1),
-// zeros past the tile's vertex count).  The address depends on the tile
+// per-tile rows (tpad[tile * stride + lane]: (slot, row start | row end << 16),
+// zeros past the tile's vertex count; a vertex of the tile has at least one
+// corner there, so row end > row start marks a live lane).  8 bytes a lane:
+// the padding lanes cost no more than the compact rows they replace.  The address depends on the tile
 // alone, so the load goes out with the tet streams and the kernel's flags in
 // its first round trip, instead of after a load of the tile's vertex range.
 // Loaded before the local step, so tile_sum's first pass reads no index
@@ -510,18 +511,18 @@ struct TileRow {
 };
 // A tiled level's per-tile metadata as the tiled kernels take it.
 struct TileMeta {
-    const int4* tpad;  // padded rows (tile_row), stride entries per tile
+    const int2* tpad;  // padded rows (tile_row), stride entries per tile
     const int *tvo, *tvs, *tlr;
-    int stride;  // entries per tile in tpad: the largest tile's vertex count rounded up to 64, at most kTileTets
+    int stride;  // entries per tile in tpad: the largest tile's vertex count rounded up to 16, at most kTileTets
     int over;    // 1 if a tile holds more than kTileTets vertices
 };
-__device__ __forceinline__ TileRow tile_row(int tile, int stride, const int4* __restrict__ tpad) {
+__device__ __forceinline__ TileRow tile_row(int tile, int stride, const int2* __restrict__ tpad) {
     // lanes past the stride read the tile's last entry and drop it: no branch
     // around the load, which would wait for it at the branch's end
     const int l = threadIdx.x;
-    const int4 e = tpad[(size_t)tile * stride + min(l, stride - 1)];
-    const bool in = l < stride;
-    return TileRow{e.x, e.y, e.z, in ? e.w : 0};
+    const int2 e = tpad[(size_t)tile * stride + min(l, stride - 1)];
+    const int r0 = e.y & 0xffff, r1 = (int)((unsigned)e.y >> 16);
+    return TileRow{r0, r1, e.x, (l < stride && r1 > r0) ? 1 : 0};
 }
 __device__ __forceinline__ float3 tile_row_sum(const TileLds& L, int r0, int r1) {
     // in row order; four entries' LDS reads in flight before their sums
@@ -615,8 +616,12 @@ __global__ void __launch_bounds__(256) k_tet(int T, const float4* __restrict__ x
 
 // The tiled form of k_tet (tile_put / tile_sum): workgroups 0..ntiles-1 run
 // one tile each (kTileTets lanes), the ones after them the triangle lanes.
-// The skip flag, the tile's rows and the tet's streams load in one round
-// trip; a skipped launch returns before the corner gathers.
+// The skip flag is tested before any other load: a skipped launch (the
+// recompute after a rejected warm start, skipped on most steps) would
+// otherwise stream 56 B per tet for nothing, and on the levels this kernel
+// serves (the fused sweeps take the small ones) that costs more than the
+// flag's round trip.  The tile's rows and the tet's streams then load in one
+// round trip.
 template <int MODE, bool TRI>
 __global__ void __launch_bounds__(kTileTets) k_tet_tiled(
     int T, int ntiles, const float4* __restrict__ x, const int4* __restrict__ tetp, const float* __restrict__ B,
@@ -624,10 +629,9 @@ __global__ void __launch_bounds__(kTileTets) k_tet_tiled(
     const int4* __restrict__ tri, const int4* __restrict__ tslot, float4* __restrict__ tcf,
     const DevColliders* __restrict__ col, const int* __restrict__ skip, const float4* __restrict__ sref,
     const float* __restrict__ tk) {
-    const int sk = load_flag(skip);
+    if (load_flag(skip)) return;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     if (blk >= ntiles) {
-        if (sk) return;
         if (TRI) tri_lane((blk - ntiles) * kTileTets + threadIdx.x, ntri, x, tri, tslot, tcf, col);
         return;
     }
@@ -638,7 +642,6 @@ __global__ void __launch_bounds__(kTileTets) k_tet_tiled(
     const int gc = min(g, T - 1);
     const int4 q = tetp[gc];
     const TetIn in = tet_streams(gc, T, B, vol, k, tk);
-    if (sk) return;  // the same flag for the whole workgroup
     if (g < T) {
         float fo[12];
         tet_forces_in<MODE>(in, tile_ids(q), x, sref, fo);
@@ -849,9 +852,10 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
                                               float relax_lo, int flags, int* __restrict__ cflag, float* __restrict__ cdepth,
                                               float* __restrict__ cenergy, const int* __restrict__ skip, int split,
                                               int lsh, const int* __restrict__ calm) {
-    // the flags, the per-vertex streams and the row bounds in one round trip
+    // the skip flag first (a skipped launch loads nothing), then the stall
+    // flag, the per-vertex streams and the row bounds in one round trip
     // (lanes past V read vertex V - 1: no branch around the loads)
-    const int sk = load_flag(skip);
+    if (load_flag(skip)) return;
     const int cm = load_flag(calm);
     // 2^lsh lanes per vertex gather its row (gather_lanes); lane 0 finishes
     const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -862,7 +866,7 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
     const float4 c = vc[ic];  // (m/h^2, diag, fix, m)
     const float4 pp = xprev[ic];
     const int e0 = rp[ic], e1 = rp[ic + 1];
-    if (sk || i >= V) return;
+    if (i >= V) return;
     if (cm) omega = 1.0f;  // stall guard: plain (damped) Jacobi for the rest of the step
     const float3 fg = gather_lanes(cf, e0, e1, sub, lsh);
     if (sub) return;
@@ -895,13 +899,17 @@ __global__ void __launch_bounds__(256) k_vert(int V, float4* __restrict__ x, flo
 // per (tile, vertex), padded as the tile rows (tile_row), the vertex, its row
 // of partials and whether the tile owns it.
 constexpr int kFuseJacobi = 0, kFuseProlong = 1;
+// tvqp's second word: the vertex's first partial (low kTvqRowBits) and its
+// count of partials (the rest)
+constexpr int kTvqRowBits = 26;
+constexpr int kTvqRowMask = (1 << kTvqRowBits) - 1;
 constexpr int kFuseMaxNv = 256;  // vertices of a tile the fused sweep holds in LDS
 static_assert(kFuseMaxNv <= kTileTets, "one lane per tile vertex: the sums' rows load with the vertex");
 template <int MODE, int PHASE>
 __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
     int T, const float4* __restrict__ xcur, const float4* __restrict__ xprv, float4* __restrict__ xout,
     const float4* __restrict__ s, const float4* __restrict__ vc, const float3* __restrict__ pin, TileMeta tm,
-    const int4* __restrict__ tvqp, const int4* __restrict__ tetq,
+    const int2* __restrict__ tvqp, const int4* __restrict__ tetq,
     const float* __restrict__ B, const float* __restrict__ vol, float k, const float* __restrict__ tk,
     float3* __restrict__ pout, const DevColliders* __restrict__ col, int flags, float omega, float relax_hi,
     float relax_lo, int* __restrict__ cflag, float* __restrict__ cdepth, float* __restrict__ cenergy,
@@ -918,8 +926,8 @@ __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
     // kFuseMaxNv vertices, checked at set-up)
     const int l = threadIdx.x;
     const TileRow tr = tile_row(blk, tm.stride, tm.tpad);
-    // (vertex, its row of partials, owner flag), padded as tpad
-    const int4 tq = tvqp[(size_t)blk * tm.stride + min(l, tm.stride - 1)];
+    // (vertex | owner << 31, row of partials start | count << 26), padded as tpad
+    const int2 tq = tvqp[(size_t)blk * tm.stride + min(l, tm.stride - 1)];
     // the tet's streams; the last tile's lanes past T read tet T - 1 and
     // drop its forces (branch-free loads, as tile_row)
     const int g = blk * kTileTets + l;
@@ -934,8 +942,9 @@ __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
     if (cm) omega = 1.0f;  // stall guard (k_vert)
     if (tr.live) {
         // second round trip: the vertex's streams and its row of partials
-        const int v = tq.x, r0 = tq.y, r1 = tq.z;
-        const bool owner = tq.w != 0;
+        const int v = tq.x & 0x7fffffff;
+        const int r0 = tq.y & kTvqRowMask, r1 = r0 + (int)((unsigned)tq.y >> kTvqRowBits);
+        const bool owner = tq.x < 0;
         const float4 st = s[v];
         float4 pn;
         if (PHASE == kFuseProlong) {
@@ -980,6 +989,113 @@ __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
     }
     __syncthreads();
     if (tr.live) pout[tr.slot] = tile_row_sum(L, tr.r0, tr.r1);
+}
+
+// The start of a warm-started V-cycle step in one launch (tiled layout, the
+// levels that run fused sweeps): k_st and k_tet_dual_tiled.  Workgroups
+// 0..ntiles-1 take the tiles at the warm start, ntiles..2 ntiles-1 the same
+// tiles at s.  Phase 1, over the tile's vertices: calculateST
+// (PDSolver.cu:72-116) in k_st's operation order, s and the warm start's
+// correction q (temporal extrapolation fix (2 U1 - U2), plus the prolongated
+// learned correction epred) into LDS; the vertex's owner tile of the first
+// half stores x_last = x, s and q (to xw: x is read by the other tiles of
+// this launch, so q goes to a spare buffer that the dual residual copies
+// into x and x_prev).  The damped velocity k_st stores is not stored: the
+// V-cycle's finish overwrites v from positions before anything reads it.
+// Phase 2: the local step at s + q (first half) or at s (second half), from
+// LDS, its partials as k_tet_dual_tiled writes them (pcf, pcf2).  Block 0
+// advances the sphere centres by vel dt, as k_st does.
+template <int MODE>
+__global__ void __launch_bounds__(kTileTets) k_start_fused(
+    int T, int ntiles, const float4* __restrict__ x, const float4* __restrict__ v, float4* __restrict__ xlast,
+    float4* __restrict__ s, float4* __restrict__ xw, const float4* __restrict__ vc, float dt, float damp, float gx,
+    float gy, float gz, DevColliders* col, const float4* __restrict__ U1, const float4* __restrict__ U2,
+    const float4* __restrict__ epred, const int4* __restrict__ iids, const float4* __restrict__ iw, TileMeta tm,
+    const int2* __restrict__ tvqp, const int4* __restrict__ tetq, const float* __restrict__ B,
+    const float* __restrict__ vol, float k, const float* __restrict__ tk, float3* __restrict__ pcf,
+    float3* __restrict__ pcf2) {
+    __shared__ float4 lu[kFuseMaxNv], ls[kFuseMaxNv];  // the tile's q and s
+    __shared__ TileLds L;
+    const int b0 = xcd_block(blockIdx.x, gridDim.x);
+    const bool at_s = b0 >= ntiles;
+    const int blk = at_s ? b0 - ntiles : b0;
+    const int l = threadIdx.x;
+    if (blockIdx.x == 0 && l == 0) {
+        for (int q = 0; q < col->nsph; q++) {
+            float4 c = col->sph[q];
+            const float4 w = col->vel[q];
+            c.x = c.x + w.x * dt;
+            c.y = c.y + w.y * dt;
+            c.z = c.z + w.z * dt;
+            col->sph[q] = c;
+        }
+    }
+    // first round trip: the tile's rows and the tet's streams (branch-free,
+    // as k_sweep_fused)
+    const TileRow tr = tile_row(blk, tm.stride, tm.tpad);
+    const int2 tq = tvqp[(size_t)blk * tm.stride + min(l, tm.stride - 1)];
+    const int g = blk * kTileTets + l;
+    const int gc = min(g, T - 1);
+    const int4 qt = tetq[gc];
+    float b[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) b[c] = B[(size_t)c * T + gc];
+    const float vt = vol[gc];
+    const float kk = tk ? tk[gc] : k;
+    if (tr.live) {
+        const int vv = tq.x & 0x7fffffff;
+        const float4 p0 = x[vv];
+        float4 p = p0, u = v[vv];
+        const float fix = vc[vv].z;
+        const float df = damp * fix;
+        u.x = u.x * df;
+        u.y = u.y * df;
+        u.z = u.z * df;
+        u.x = u.x + gx * dt * fix;
+        u.y = u.y + gy * dt * fix;
+        u.z = u.z + gz * dt * fix;
+        p.x = p.x + u.x * dt * fix;
+        p.y = p.y + u.y * dt * fix;
+        p.z = p.z + u.z * dt * fix;
+        float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (U1) {
+            const float4 a = U1[vv], bq = U2[vv];
+            q = make_float4(fix * (2.f * a.x - bq.x), fix * (2.f * a.y - bq.y), fix * (2.f * a.z - bq.z), 0.f);
+        }
+        if (epred) {
+            const float3 e = prolong_at(vv, epred, iids, iw);
+            q = make_float4(q.x + fix * e.x, q.y + fix * e.y, q.z + fix * e.z, 0.f);
+        }
+        lu[l] = q;
+        ls[l] = p;
+        if (tq.x < 0 && !at_s) {
+            xlast[vv] = p0;
+            s[vv] = p;
+            xw[vv] = q;
+        }
+    }
+    __syncthreads();
+    if (g < T) {
+        const int lv[4] = {qt.x & 0xffff, qt.y & 0xffff, qt.z & 0xffff, qt.w & 0xffff};
+        float fo[12];
+        if (at_s) {
+            const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+            tet_forces_at<MODE>(b, vt, kk, ls[lv[0]], ls[lv[1]], ls[lv[2]], ls[lv[3]], false, z, z, z, z, fo);
+        } else {
+            tet_forces_at<MODE>(b, vt, kk, lu[lv[0]], lu[lv[1]], lu[lv[2]], lu[lv[3]], true, ls[lv[0]], ls[lv[1]],
+                                ls[lv[2]], ls[lv[3]], fo);
+        }
+        const int pos[4] = {(int)((unsigned)qt.x >> 16), (int)((unsigned)qt.y >> 16), (int)((unsigned)qt.z >> 16),
+                            (int)((unsigned)qt.w >> 16)};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            L.x[pos[j]] = fo[j * 3 + 0];
+            L.y[pos[j]] = fo[j * 3 + 1];
+            L.z[pos[j]] = fo[j * 3 + 2];
+        }
+    }
+    __syncthreads();
+    if (tr.live) (at_s ? pcf2 : pcf)[tr.slot] = tile_row_sum(L, tr.r0, tr.r1);
 }
 
 // updatePointInTet (PDSolver.cu:724-744): dst.x = dst.x_prev = sum w src.x.
@@ -1474,9 +1590,10 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* x, const flo
                                                float4* xprev, const int* __restrict__ skip, int split,
                                                double* __restrict__ g2, float* __restrict__ cmask, int lsh,
                                                float4* xcanon = nullptr, const float4* xpsrc = nullptr) {
-    // the flag, the vertex's streams and its row bounds in one round trip
-    // (lanes past V read vertex V - 1 and drop it: no branch around the loads)
-    const int sk = load_flag(skip);
+    // the skip flag first (a skipped launch loads nothing), then the vertex's
+    // streams and its row bounds in one round trip (lanes past V read vertex
+    // V - 1 and drop it: no branch around the loads)
+    if (load_flag(skip)) return;
     const int gi = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gi >> lsh, sub = gi & ((1 << lsh) - 1);  // 2^lsh lanes per vertex
     const int ic = min(i, V - 1);
@@ -1485,7 +1602,6 @@ __global__ void __launch_bounds__(256) k_resid(int V, const float4* x, const flo
     // between xpsrc[i] and the local p would put p on the stack)
     const float4 pl = (xpsrc ? xpsrc : x)[ic];
     const int e0 = rp[ic], e1 = rp[ic + 1];
-    if (sk) return;
     double v[1] = {0.0};
     if (i < V) {
         const float4 pq = xpsrc ? pl : p;
@@ -1518,7 +1634,11 @@ __global__ void __launch_bounds__(256) k_resid_dual(int V, const float4* __restr
                                                     const float3* __restrict__ cf2,
                                                     const DevColliders* __restrict__ col, int flags,
                                                     float4* __restrict__ r, double* __restrict__ part,
-                                                    float* __restrict__ cmask, int lsh, float4* __restrict__ rs) {
+                                                    float* __restrict__ cmask, int lsh, float4* __restrict__ rs,
+                                                    float4* __restrict__ xo = nullptr,
+                                                    float4* __restrict__ xpo = nullptr) {
+    // xo, xpo (k_start_fused): x holds the warm start's correction in a spare
+    // buffer; it is copied into x and x_prev here
     const int gi = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gi >> lsh, sub = gi & ((1 << lsh) - 1);  // 2^lsh lanes per vertex
     double v[2] = {0.0, 0.0};
@@ -1541,6 +1661,10 @@ __global__ void __launch_bounds__(256) k_resid_dual(int V, const float4* __restr
             if (cmask) cmask[i] = am;
             if (r) r[i] = cmask ? make_float4(g.x * am, g.y * am, g.z * am, 0.f) : make_float4(g.x, g.y, g.z, 0.f);
             v[1] = (double)g.x * g.x + (double)g.y * g.y + (double)g.z * g.z;
+            if (xo) {
+                xo[i] = p;
+                xpo[i] = p;
+            }
         }
     }
     block_reduce_store<2>(v, part);
@@ -1699,10 +1823,28 @@ __global__ void __launch_bounds__(256) k_judge_revert(int nb, const double* __re
                                                       const float3* __restrict__ cf2, float3* __restrict__ cf,
                                                       float4* __restrict__ r, const float4* __restrict__ rs,
                                                       float* __restrict__ cmask) {
+    // the partials (kCheckBatch per lane), the check state the verdict reads
+    // and the vertex's row bounds in one round trip
+    const int chk = st->check;
+    const double slack = st->slack;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int ic = min(i, V - 1);
+    const int e0 = rp[ic], e1 = rp[ic + 1];
     double v[2] = {0.0, 0.0};
-    for (int q = threadIdx.x; q < nb; q += blockDim.x) {
-        v[0] += part[2 * q];
-        v[1] += part[2 * q + 1];
+    for (int q = threadIdx.x; q < nb; q += kCheckBatch * blockDim.x) {
+        double a[kCheckBatch], b[kCheckBatch];
+#pragma unroll
+        for (int k = 0; k < kCheckBatch; k++) {
+            const int qc = min(q + k * (int)blockDim.x, nb - 1);
+            a[k] = part[2 * qc];
+            b[k] = part[2 * qc + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < kCheckBatch; k++)
+            if (q + k * (int)blockDim.x < nb) {
+                v[0] += a[k];
+                v[1] += b[k];
+            }
     }
     wg_sum_fixed<2>(v);
     __shared__ int s_rv;
@@ -1717,16 +1859,15 @@ __global__ void __launch_bounds__(256) k_judge_revert(int nb, const double* __re
             bump_seq(seqc, hm, st);
         } else {  // the same decision from the same sums (vstate_apply at the start, then the judge)
             const double r0 = sqrt(v[0]);
-            const bool conv = st->check && r0 == 0.0;
-            s_rv = (conv || !judge_keeps(sqrt(v[1]), r0, judge, st->slack)) ? 1 : 0;
+            const bool conv = chk && r0 == 0.0;
+            s_rv = (conv || !judge_keeps(sqrt(v[1]), r0, judge, slack)) ? 1 : 0;
         }
     }
     __syncthreads();
     const int rv = s_rv;  // the dual start rejects to s only
     if (!rv) return;
-    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= V) return;
-    for (int e = rp[i]; e < rp[i + 1]; e++) cf[e] = cf2[e];
+    for (int e = e0; e < e1; e++) cf[e] = cf2[e];
     if (rs) {
         const float4 g = rs[i];
         r[i] = make_float4(g.x, g.y, g.z, 0.f);
@@ -1837,11 +1978,9 @@ __global__ void __launch_bounds__(256) k_prolong_add(int N, float4* __restrict__
                                                      const int* __restrict__ skip, const float* __restrict__ amask) {
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    // the flag and the vertex's binding in one round trip
-    const int sk = load_flag(skip);
+    if (load_flag(skip)) return;
     const int4 id = ids[i];
     const float4 ww = w[i];
-    if (sk) return;
     const float4 a = src[id.x], b = src[id.y], c = src[id.z], d = src[id.w];
     const float vx = ww.x * a.x + ww.y * b.x + ww.z * c.x + ww.w * d.x;
     const float vy = ww.x * a.y + ww.y * b.y + ww.z * c.y + ww.w * d.y;
@@ -2478,14 +2617,14 @@ struct DevLevel {
     int ntiles = 0, npart = 0;
     int *prp = nullptr, *tvo = nullptr, *tvs = nullptr, *tlr = nullptr;
     int4* tetp = nullptr;  // tets with each corner's position in its tile's corner array (nullptr: not tiled)
-    int4* tpad = nullptr;  // padded tile rows (tile_row), tstride entries per tile
+    int2* tpad = nullptr;  // padded tile rows (tile_row), tstride entries per tile
     int tstride = 0, tover = 0;
     float3 *pcf = nullptr, *pcf2 = nullptr;
     // fused sweeps (k_sweep_fused): per corner its tile-local vertex index and
     // corner position; the tiles' vertex lists; a third position buffer
     // (nullptr: the level does not run fused sweeps)
     int4* tetq = nullptr;
-    int4* tvqp = nullptr;  // per (tile, vertex), padded as tpad: vertex id, its prp row [y, z), 1 if this tile owns it
+    int2* tvqp = nullptr;  // per (tile, vertex), padded as tpad: vertex id | owner << 31, prp row start | count << 26
     float4* xz = nullptr;
     int4 *tri = nullptr, *tslot = nullptr;
     int *trp = nullptr, *trid = nullptr;
@@ -2704,14 +2843,16 @@ int upload_level(mgsb_solver* s, DevLevel& d) {
         // start, row end, slot, 1) of tile vertex l, the rest zeros
         int maxnv = 0;
         for (int k = 0; k < d.ntiles; k++) maxnv = std::max(maxnv, h.tvo[(size_t)k + 1] - h.tvo[(size_t)k]);
-        d.tstride = std::min(kTileTets, std::max(64, (maxnv + 63) / 64 * 64));
+        d.tstride = std::min(kTileTets, std::max(16, (maxnv + 15) / 16 * 16));
         d.tover = maxnv > kTileTets ? 1 : 0;
-        std::vector<int4> tp((size_t)std::max(1, d.ntiles) * d.tstride, make_int4(0, 0, 0, 0));
+        std::vector<int2> tp((size_t)std::max(1, d.ntiles) * d.tstride, make_int2(0, 0));
         for (int k = 0; k < d.ntiles; k++) {
             const int v0 = h.tvo[(size_t)k], nv = std::min(kTileTets, h.tvo[(size_t)k + 1] - v0);
-            for (int l = 0; l < nv; l++)
-                tp[(size_t)k * d.tstride + l] =
-                    make_int4(h.tlr[(size_t)v0 + k + l], h.tlr[(size_t)v0 + k + l + 1], h.tvs[(size_t)v0 + l], 1);
+            for (int l = 0; l < nv; l++) {
+                // rows of the tile's corner array: 4 kTileTets entries, so 16 bits each
+                const int r0 = h.tlr[(size_t)v0 + k + l], r1 = h.tlr[(size_t)v0 + k + l + 1];
+                tp[(size_t)k * d.tstride + l] = make_int2(h.tvs[(size_t)v0 + l], r0 | r1 << 16);
+            }
         }
         if ((rc = up(s, d.owned, &d.tpad, tp.data(), tp.size()))) return rc;
     }
@@ -2731,7 +2872,10 @@ int upload_level(mgsb_solver* s, DevLevel& d) {
         for (int k = 0; k < d.ntiles; k++) maxnv = std::max(maxnv, h.tvo[(size_t)k + 1] - h.tvo[(size_t)k]);
         bool covered = true;
         for (int v = 0; v < h.V && covered; v++) covered = h.prp[(size_t)v + 1] > h.prp[(size_t)v];
-        if (maxnv <= kFuseMaxNv && covered) {
+        // tvqp's packing: partial rows below 2^kTvqRowBits, at most 31 tiles a vertex
+        bool packs = (int64_t)h.prp.back() < ((int64_t)1 << kTvqRowBits);
+        for (int v = 0; v < h.V && packs; v++) packs = h.prp[(size_t)v + 1] - h.prp[(size_t)v] < 32;
+        if (maxnv <= kFuseMaxNv && covered && packs) {
             std::vector<int4> tq(h.T);
             for (int t = 0; t < h.T; t++) {
                 int c[4];
@@ -2744,7 +2888,7 @@ int upload_level(mgsb_solver* s, DevLevel& d) {
                 tq[t] = make_int4(c[0], c[1], c[2], c[3]);
             }
             if ((rc = up(s, d.owned, &d.tetq, tq.data(), tq.size()))) return rc;
-            std::vector<int4> tv((size_t)std::max(1, d.ntiles) * d.tstride, make_int4(0, 0, 0, 0));
+            std::vector<int2> tv((size_t)std::max(1, d.ntiles) * d.tstride, make_int2(0, 0));
             for (int k = 0; k < d.ntiles; k++) {
                 const int v0 = h.tvo[(size_t)k], nv = h.tvo[(size_t)k + 1] - v0;
                 for (int l = 0; l < nv; l++) {
@@ -2752,7 +2896,8 @@ int upload_level(mgsb_solver* s, DevLevel& d) {
                     const int v = h.tvid[q];
                     const int r0 = h.prp[(size_t)v], r1 = h.prp[(size_t)v + 1];
                     // the first tile of a row owns the vertex
-                    tv[(size_t)k * d.tstride + l] = make_int4(v, r0, r1, h.tvs[q] == r0 ? 1 : 0);
+                    const unsigned own = h.tvs[q] == r0 ? 0x80000000u : 0u;
+                    tv[(size_t)k * d.tstride + l] = make_int2((int)((unsigned)v | own), r0 | (r1 - r0) << kTvqRowBits);
                 }
             }
             if ((rc = up(s, d.owned, &d.tvqp, tv.data(), tv.size()))) return rc;
@@ -3231,6 +3376,22 @@ bool temporal_warm(const mgsb_solver* s) {
 // the stage costs the feature and inference launches, and the one judge of
 // the warm start (vstate_apply: kept if its residual is at most learn_slack
 // times g(s)'s, else undone by k_revert) covers both.
+// k_start_fused on level 0 (fused(s, d) holds): the inertial prediction with
+// sphere advance and the warm start (temporal extrapolation, plus epred when
+// the learned stage ran), then both local steps.
+int launch_start_fused(mgsb_solver* s, DevLevel& d, bool temporal, const float4* epred) {
+    auto k = tet_mode_of(s->p.if_mode) == kTetModePolar
+                 ? k_start_fused<kTetModePolar>
+                 : (tet_mode_of(s->p.if_mode) == kTetModeAc ? k_start_fused<kTetModeAc> : k_start_fused<kTetModeStable>);
+    k<<<2 * d.ntiles, kTileTets, 0, s->stream>>>(
+        d.T, d.ntiles, d.x, d.v, d.xlast, d.s, d.xz, d.vc, s->p.dt, s->p.damping, s->p.gravity[0], s->p.gravity[1],
+        s->p.gravity[2], s->dcol, temporal ? d.U1 : nullptr, temporal ? d.U2 : nullptr, epred,
+        epred ? s->M[0].iids : nullptr, epred ? s->M[0].iw : nullptr, tile_meta(d), d.tvqp, d.tetq, d.B, d.vol,
+        s->p.stiffness, d.tk, d.pcf, d.pcf2);
+    HIPCHK(hipGetLastError());
+    return MGSB_OK;
+}
+
 int enqueue_vc_start(mgsb_solver* s, bool learned) {
     DevLevel& f = s->L[0];
     DevLevel& c1 = s->L[1];
@@ -3260,12 +3421,25 @@ int enqueue_vc_start(mgsb_solver* s, bool learned) {
         if ((rc = mlp_infer_coarse(s->mlp, c1.V, io, s->stream))) return rc;
     }
     if (dual) {
-        if ((rc = launch_st(s, f, 1, 1, temporal, ep))) return rc;
-        if ((rc = launch_tet_dual(s, f))) return rc;
+        // on a level that runs fused sweeps, the inertial prediction and both
+        // local steps in one launch; the warm start's correction then sits in
+        // xz until the dual residual copies it into x and x_prev.  Not with
+        // the learned correction: its prolongation is a further dependent
+        // round trip in every tile's vertex phase (medium box, 20 steps: the
+        // temporal start 9105 -> 9170 steps/s with the fused start, the
+        // learned one 9107 -> 9040)
+        const bool fst = fused(s, f) && !ep;
+        if (fst) {
+            if ((rc = launch_start_fused(s, f, temporal, ep))) return rc;
+        } else {
+            if ((rc = launch_st(s, f, 1, 1, temporal, ep))) return rc;
+            if ((rc = launch_tet_dual(s, f))) return rc;
+        }
         const int lsh = vert_lsh(s), nbr = nblk(f.V << lsh, 256);
-        k_resid_dual<<<nbr, 256, 0, s->stream>>>(f.V, f.x, f.s, f.vc, gath_rp(s, f), gath_cf(s, f), gath_cf2(s, f),
-                                                 s->dcol, kVertPoint, cf ? f.r : nullptr, s->vpart, f.cmask, lsh,
-                                                 cf ? f.rr : nullptr);
+        k_resid_dual<<<nbr, 256, 0, s->stream>>>(f.V, fst ? f.xz : f.x, f.s, f.vc, gath_rp(s, f), gath_cf(s, f),
+                                                 gath_cf2(s, f), s->dcol, kVertPoint, cf ? f.r : nullptr, s->vpart,
+                                                 f.cmask, lsh, cf ? f.rr : nullptr, fst ? f.x : nullptr,
+                                                 fst ? f.xprev : nullptr);
         HIPCHK(hipGetLastError());
         // res0, the judge and its revert to s in one launch
         k_judge_revert<<<nbv, 256, 0, s->stream>>>(nbr, s->vpart, s->vstate, s->hmirror, s->seqc, judge, f.V, f.x,
