@@ -49,9 +49,9 @@ TRAIN = dict(episodes=[(sd, 60) for sd in range(1, 33)] + [(40, 600)], epochs=30
 # rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scales from a 256 MiB
 # copy): profiles/fused_sweep_medium_box_driver_window.md and
 # profiles/k_tet_tiled_large_and_batched.md
-TRAFFIC = {("medium_box_12x120x12", "k_sweep_fused"): 10.22e6,
-           ("large_box_24x240x24_collision", "k_tet"): 58.94e6,
-           ("batched_64_boxes", "k_tet"): 344.96e6}
+TRAFFIC = {("medium_box_12x120x12", "k_sweep_fused"): 10.02e6,
+           ("large_box_24x240x24_collision", "k_tet"): 60.27e6,
+           ("batched_64_boxes", "k_tet"): 349.26e6}
 
 
 def log(*a):
