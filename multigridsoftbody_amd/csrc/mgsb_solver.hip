@@ -148,6 +148,14 @@ __device__ __forceinline__ float3 prolong_at(int i, const float4* __restrict__ e
                        ww.x * a.z + ww.y * b.z + ww.z * c.z + ww.w * d.z);
 }
 
+// prolong_at with the binding (coarse tet corners, weights) already loaded:
+// the same sums in the same order.
+__device__ __forceinline__ float3 prolong_w(int4 id, float4 ww, const float4* __restrict__ e) {
+    const float4 a = e[id.x], b = e[id.y], c = e[id.z], d = e[id.w];
+    return make_float3(ww.x * a.x + ww.y * b.x + ww.z * c.x + ww.w * d.x, ww.x * a.y + ww.y * b.y + ww.z * c.y + ww.w * d.y,
+                       ww.x * a.z + ww.y * b.z + ww.z * c.z + ww.w * d.z);
+}
+
 // calculateST (PDSolver.cu:72-116).  Block 0 / lane 0 first advances sphere
 // centres by vel*dt (UpdateCollider before Step, Simulator.cpp:146-149).
 // split: x and x_prev take the in-step correction u (0, or with U1/U2 the
@@ -913,10 +921,12 @@ __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
     const float* __restrict__ B, const float* __restrict__ vol, float k, const float* __restrict__ tk,
     float3* __restrict__ pout, const DevColliders* __restrict__ col, int flags, float omega, float relax_hi,
     float relax_lo, int* __restrict__ cflag, float* __restrict__ cdepth, float* __restrict__ cenergy,
-    const float4* __restrict__ ec, const int4* __restrict__ iids, const float4* __restrict__ iw,
+    const float4* __restrict__ ec, const int4* __restrict__ tiid, const float4* __restrict__ tiw,
     const float* __restrict__ amask, const int* __restrict__ skip, const int* __restrict__ calm) {
     // first round trip: the flags, the tile's rows (padded per tile, so their
-    // addresses need no earlier load) and the tet's streams
+    // addresses need no earlier load), with PROLONG the vertices' bindings
+    // into the coarse level (tiid, tiw, padded the same way), and the tet's
+    // streams
     const int sk = load_flag(skip);
     const int cm = load_flag(calm);
     __shared__ float4 lu[kFuseMaxNv], ls[kFuseMaxNv];  // the tile's new positions and s
@@ -928,6 +938,12 @@ __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
     const TileRow tr = tile_row(blk, tm.stride, tm.tpad);
     // (vertex | owner << 31, row of partials start | count << 26), padded as tpad
     const int2 tq = tvqp[(size_t)blk * tm.stride + min(l, tm.stride - 1)];
+    int4 bid = make_int4(0, 0, 0, 0);
+    float4 bw = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (PHASE == kFuseProlong) {
+        bid = tiid[(size_t)blk * tm.stride + min(l, tm.stride - 1)];
+        bw = tiw[(size_t)blk * tm.stride + min(l, tm.stride - 1)];
+    }
     // the tet's streams; the last tile's lanes past T read tet T - 1 and
     // drop its forces (branch-free loads, as tile_row)
     const int g = blk * kTileTets + l;
@@ -948,7 +964,7 @@ __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
         const float4 st = s[v];
         float4 pn;
         if (PHASE == kFuseProlong) {
-            const float3 e = prolong_at(v, ec, iids, iw);
+            const float3 e = prolong_w(bid, bw, ec);
             const float m = amask ? vc[v].z * amask[v] : vc[v].z;
             pn = xcur[v];
             pn.x = pn.x + m * e.x;
@@ -1010,7 +1026,7 @@ __global__ void __launch_bounds__(kTileTets) k_start_fused(
     int T, int ntiles, const float4* __restrict__ x, const float4* __restrict__ v, float4* __restrict__ xlast,
     float4* __restrict__ s, float4* __restrict__ xw, const float4* __restrict__ vc, float dt, float damp, float gx,
     float gy, float gz, DevColliders* col, const float4* __restrict__ U1, const float4* __restrict__ U2,
-    const float4* __restrict__ epred, const int4* __restrict__ iids, const float4* __restrict__ iw, TileMeta tm,
+    const float4* __restrict__ epred, const int4* __restrict__ tiid, const float4* __restrict__ tiw, TileMeta tm,
     const int2* __restrict__ tvqp, const int4* __restrict__ tetq, const float* __restrict__ B,
     const float* __restrict__ vol, float k, const float* __restrict__ tk, float3* __restrict__ pcf,
     float3* __restrict__ pcf2) {
@@ -1034,6 +1050,14 @@ __global__ void __launch_bounds__(kTileTets) k_start_fused(
     // as k_sweep_fused)
     const TileRow tr = tile_row(blk, tm.stride, tm.tpad);
     const int2 tq = tvqp[(size_t)blk * tm.stride + min(l, tm.stride - 1)];
+    // the learned correction's binding, padded as the rows (first half only:
+    // the second needs s alone)
+    int4 bid = make_int4(0, 0, 0, 0);
+    float4 bw = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (epred && !at_s) {
+        bid = tiid[(size_t)blk * tm.stride + min(l, tm.stride - 1)];
+        bw = tiw[(size_t)blk * tm.stride + min(l, tm.stride - 1)];
+    }
     const int g = blk * kTileTets + l;
     const int gc = min(g, T - 1);
     const int4 qt = tetq[gc];
@@ -1058,12 +1082,12 @@ __global__ void __launch_bounds__(kTileTets) k_start_fused(
         p.y = p.y + u.y * dt * fix;
         p.z = p.z + u.z * dt * fix;
         float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (U1) {
+        if (U1 && !at_s) {
             const float4 a = U1[vv], bq = U2[vv];
             q = make_float4(fix * (2.f * a.x - bq.x), fix * (2.f * a.y - bq.y), fix * (2.f * a.z - bq.z), 0.f);
         }
-        if (epred) {
-            const float3 e = prolong_at(vv, epred, iids, iw);
+        if (epred && !at_s) {
+            const float3 e = prolong_w(bid, bw, epred);
             q = make_float4(q.x + fix * e.x, q.y + fix * e.y, q.z + fix * e.z, 0.f);
         }
         lu[l] = q;
@@ -2625,6 +2649,10 @@ struct DevLevel {
     // (nullptr: the level does not run fused sweeps)
     int4* tetq = nullptr;
     int2* tvqp = nullptr;  // per (tile, vertex), padded as tpad: vertex id | owner << 31, prp row start | count << 26
+    // level 0 of a V-cycle with fused sweeps: each tile vertex's binding into
+    // level 1 (coarse tet corners, weights), padded as tpad
+    int4* tiid = nullptr;
+    float4* tiw = nullptr;
     float4* xz = nullptr;
     int4 *tri = nullptr, *tslot = nullptr;
     int *trp = nullptr, *trid = nullptr;
@@ -3288,7 +3316,7 @@ void launch_fused_mode(mgsb_solver* s, DevLevel& d, const FuseBufs& fb, float4* 
     k_sweep_fused<MODE, PHASE><<<d.ntiles, kTileTets, 0, s->stream>>>(
         d.T, fb.cur, fb.prev, out, d.s, d.vc, fb.pin ? d.pcf2 : d.pcf, tile_meta(d), d.tvqp, d.tetq,
         d.B, d.vol, s->p.stiffness, d.tk, fb.pin ? d.pcf : d.pcf2, s->dcol, flags, omega, s->relax_hi, s->relax_lo,
-        d.cflag, d.cdepth, d.cenergy, ec, m.iids, m.iw, amask, skip, calm_flag(s, 1));
+        d.cflag, d.cdepth, d.cenergy, ec, d.tiid, d.tiw, amask, skip, calm_flag(s, 1));
 }
 // One fused sweep: JACOBI (vertex update with omega, then the local step) or
 // PROLONG (x += mask P ec, then the local step; the smoother restarts).
@@ -3386,7 +3414,7 @@ int launch_start_fused(mgsb_solver* s, DevLevel& d, bool temporal, const float4*
     k<<<2 * d.ntiles, kTileTets, 0, s->stream>>>(
         d.T, d.ntiles, d.x, d.v, d.xlast, d.s, d.xz, d.vc, s->p.dt, s->p.damping, s->p.gravity[0], s->p.gravity[1],
         s->p.gravity[2], s->dcol, temporal ? d.U1 : nullptr, temporal ? d.U2 : nullptr, epred,
-        epred ? s->M[0].iids : nullptr, epred ? s->M[0].iw : nullptr, tile_meta(d), d.tvqp, d.tetq, d.B, d.vol,
+        epred ? d.tiid : nullptr, epred ? d.tiw : nullptr, tile_meta(d), d.tvqp, d.tetq, d.B, d.vol,
         s->p.stiffness, d.tk, d.pcf, d.pcf2);
     HIPCHK(hipGetLastError());
     return MGSB_OK;
@@ -3423,12 +3451,8 @@ int enqueue_vc_start(mgsb_solver* s, bool learned) {
     if (dual) {
         // on a level that runs fused sweeps, the inertial prediction and both
         // local steps in one launch; the warm start's correction then sits in
-        // xz until the dual residual copies it into x and x_prev.  Not with
-        // the learned correction: its prolongation is a further dependent
-        // round trip in every tile's vertex phase (medium box, 20 steps: the
-        // temporal start 9105 -> 9170 steps/s with the fused start, the
-        // learned one 9107 -> 9040)
-        const bool fst = fused(s, f) && !ep;
+        // xz until the dual residual copies it into x and x_prev
+        const bool fst = fused(s, f);
         if (fst) {
             if ((rc = launch_start_fused(s, f, temporal, ep))) return rc;
         } else {
@@ -4278,6 +4302,26 @@ int mgsb_create(const mgsb_mesh* levels, int32_t nlevels, const mgsb_params* p, 
             if ((rc = up(s.get(), s->owned, &m.iw, w4.data(), w4.size()))) return rc;
             if ((rc = up(s.get(), s->owned, &m.aids, a4.data(), a4.size()))) return rc;
             if ((rc = up(s.get(), s->owned, &m.aw, b4.data(), b4.size()))) return rc;
+        }
+        if (nlevels >= 2 && s->L[0].tvqp) {
+            // level 0's binding into level 1 per (tile, vertex), padded as the
+            // tile rows: the fused prolongation and the fused start load it
+            // with the rows instead of after the vertex id
+            DevLevel& d = s->L[0];
+            const HostLevel& h = d.h;
+            const LevelMap& m = s->M[0];
+            std::vector<int4> ti((size_t)std::max(1, d.ntiles) * d.tstride, make_int4(0, 0, 0, 0));
+            std::vector<float4> tw(ti.size(), make_float4(0.f, 0.f, 0.f, 0.f));
+            for (int k = 0; k < d.ntiles; k++) {
+                const int v0 = h.tvo[(size_t)k], nv = h.tvo[(size_t)k + 1] - v0;
+                for (int l = 0; l < nv; l++) {
+                    const size_t v = (size_t)h.tvid[(size_t)v0 + l], q = (size_t)k * d.tstride + l;
+                    ti[q] = make_int4(m.h_iids[v * 4], m.h_iids[v * 4 + 1], m.h_iids[v * 4 + 2], m.h_iids[v * 4 + 3]);
+                    tw[q] = make_float4(m.h_iw[v * 4], m.h_iw[v * 4 + 1], m.h_iw[v * 4 + 2], m.h_iw[v * 4 + 3]);
+                }
+            }
+            if ((rc = up(s.get(), s->owned, &d.tiid, ti.data(), ti.size()))) return rc;
+            if ((rc = up(s.get(), s->owned, &d.tiw, tw.data(), tw.size()))) return rc;
         }
         if ((rc = dalloc(&s->dcol, 1))) return rc;
         s->owned.push_back(s->dcol);
