@@ -70,6 +70,38 @@ def test_single_tet_matches_oracle():
     assert np.abs(g[3] - pos[3]).max() > 1e-3 and err < 1e-4
 
 
+def test_tet_soup_tiles_over_256_vertices_match_oracle():
+    """600 disjoint tets (2,400 vertices): every tile of 256 consecutive tets
+    holds 1,024 vertices, more than a workgroup's lanes, so the tiled local
+    step's lanes also take the tile's vertices past the padded rows (the
+    overflow path).  One corner of each tet pinned, the others fall and
+    stretch under gravity through the reference's single-level schedule;
+    positions against the oracle."""
+    rng = np.random.default_rng(3)
+    base = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0.2, 0.3, 1.0]], np.float32)
+    n = 600
+    off = np.stack([np.arange(n) % 25 * 2.0, np.arange(n) // 25 * 2.0, np.zeros(n)], 1).astype(np.float32)
+    pos = (base[None] * rng.uniform(0.8, 1.2, (n, 1, 1)).astype(np.float32) + off[:, None]).reshape(-1, 3)
+    pos = pos.astype(np.float32)
+    tets = np.arange(4 * n, dtype=np.int32).reshape(n, 4)
+    mesh = M.Mesh(pos, tets)
+    fix = np.ones(4 * n, np.uint8)
+    fix[0::4] = 0
+    phys = W.physics()
+    s = M.Solver([mesh], schedule=M.SCHED_SINGLE, iters=10, **phys)
+    s.set_fixed(fix)
+    s.step(20)
+    L = O.Level(pos, tets, mesh.tris, stiffness=phys["stiffness"], density=phys["density"])
+    L.set_fix(fix.astype(np.float32))
+    L.step(O.params(dt=phys["dt"], stiffness=phys["stiffness"], density=phys["density"], iters=10), 20)
+    g, o = s.positions(), L.positions()
+    assert np.array_equal(g[0::4], pos[0::4])
+    moved = float(np.abs(g - pos).max())
+    err = float(np.abs(g - o).max() / np.abs(o).max())
+    print(f"[tet soup] moved {moved:.3e}, rel err vs oracle {err:.2e}")
+    assert moved > 1e-3 and err < 1e-4
+
+
 @pytest.mark.parametrize("bad", [dict(vert_lanes=3), dict(block_vert=100), dict(warm_start=5), dict(learn_gate=2)])
 def test_invalid_parameters_are_error_codes(bad):
     lv = W.beam_hierarchy((4, 1))
