@@ -922,13 +922,18 @@ __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
     float3* __restrict__ pout, const DevColliders* __restrict__ col, int flags, float omega, float relax_hi,
     float relax_lo, int* __restrict__ cflag, float* __restrict__ cdepth, float* __restrict__ cenergy,
     const float4* __restrict__ ec, const int4* __restrict__ tiid, const float4* __restrict__ tiw,
-    const float* __restrict__ amask, const int* __restrict__ skip, const int* __restrict__ calm) {
-    // first round trip: the flags, the tile's rows (padded per tile, so their
+    const float* __restrict__ amask, const int* __restrict__ skip, const int* __restrict__ calm,
+    const int* __restrict__ jrev, const float4* __restrict__ rs) {
+    // jrev, rs (PROLONG after k_restrict_judged): when the judge rejected the
+    // warm start, the prolongation starts from u = 0 with the active set at s
+    // (rs.w) in place of x and the mask at the warm start.
+    // First round trip: the flags, the tile's rows (padded per tile, so their
     // addresses need no earlier load), with PROLONG the vertices' bindings
     // into the coarse level (tiid, tiw, padded the same way), and the tet's
     // streams
     const int sk = load_flag(skip);
     const int cm = load_flag(calm);
+    const int rv = load_flag(jrev);
     __shared__ float4 lu[kFuseMaxNv], ls[kFuseMaxNv];  // the tile's new positions and s
     __shared__ TileLds L;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -965,8 +970,8 @@ __global__ void __launch_bounds__(kTileTets) k_sweep_fused(
         float4 pn;
         if (PHASE == kFuseProlong) {
             const float3 e = prolong_w(bid, bw, ec);
-            const float m = amask ? vc[v].z * amask[v] : vc[v].z;
-            pn = xcur[v];
+            const float m = amask ? vc[v].z * (rv ? rs[v].w : amask[v]) : vc[v].z;
+            pn = rv ? make_float4(0.f, 0.f, 0.f, 0.f) : xcur[v];
             pn.x = pn.x + m * e.x;
             pn.y = pn.y + m * e.y;
             pn.z = pn.z + m * e.z;
@@ -1908,6 +1913,106 @@ __global__ void __launch_bounds__(256) k_judge_revert(int nb, const double* __re
 // extrapolation (D1, D2, kept by k_finish_split) injected the same way;
 // normalised with (mu, sd).  Raw values optionally recorded.
 constexpr int kFeat = 12;
+// k_judge_revert folded into the coarse-first half that follows a fused
+// warm start (same captured sequence): every workgroup sums the dual
+// residual's partials in k_judge_revert's order (256 lanes, kCheckBatch
+// partials a lane, wg_sum_fixed) and takes the same verdict; workgroup 0
+// applies res0 and the judge to the check state and publishes the count and
+// verdict.  A converged start ends the launch (later kernels skip on the
+// converged flag); a rejected warm start restricts g(s) (rs) instead of g at
+// the warm start (rf), and the prolongation sweep after it starts from u = 0
+// with the active set at s (k_sweep_fused's jrev).  Those are the only
+// readers of what k_judge_revert's return to s writes (x, x_prev, the
+// partials, r, the active-set mask) before the half's own sweeps and
+// residual write them again.  Then the restriction as k_restrict.
+__global__ void __launch_bounds__(256) k_restrict_judged(int Vc, const int* __restrict__ rtp,
+                                                         const int* __restrict__ rti, const float* __restrict__ rtw,
+                                                         const float4* __restrict__ rf, const float4* __restrict__ rs,
+                                                         const float4* __restrict__ vcl, float4* __restrict__ rc, int nb,
+                                                         const double* __restrict__ part, VState* st, int* hm,
+                                                         unsigned* seqc, int judge) {
+    const int chk = st->check;
+    const double slack = st->slack;
+    const float tol = st->tol;
+    double v[2] = {0.0, 0.0};
+    for (int q = threadIdx.x; q < nb; q += kCheckBatch * blockDim.x) {
+        double a[kCheckBatch], b[kCheckBatch];
+#pragma unroll
+        for (int k = 0; k < kCheckBatch; k++) {
+            const int qc = min(q + k * (int)blockDim.x, nb - 1);
+            a[k] = part[2 * qc];
+            b[k] = part[2 * qc + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < kCheckBatch; k++)
+            if (q + k * (int)blockDim.x < nb) {
+                v[0] += a[k];
+                v[1] += b[k];
+            }
+    }
+    wg_sum_fixed<2>(v);
+    __shared__ int s_mode;  // 0 the warm start kept, 1 rejected (back to s), 2 the step has converged
+    if (threadIdx.x == 0) {
+        int mode;
+        if (blockIdx.x == 0) {
+            vstate_apply(st, v[0], kChkStart, hm);
+            if (!st->converged)
+                vstate_apply(st, v[1], judge, hm);
+            else
+                st->revert = 1;
+            mode = st->converged ? 2 : (st->revert ? 1 : 0);
+            bump_seq(seqc, hm, st);
+        } else {  // the same decision from the same sums (vstate_apply at the start, then the judge)
+            const double r0 = sqrt(v[0]), r1 = sqrt(v[1]);
+            const bool keep = judge_keeps(r1, r0, judge, slack);
+            if (chk && r0 == 0.0)
+                mode = 2;
+            else if (!keep)
+                mode = 1;
+            else
+                mode = (chk && r1 <= (double)tol * r0) ? 2 : 0;
+        }
+        s_mode = mode;
+    }
+    __syncthreads();
+    const int mode = s_mode;
+    if (mode == 2) return;
+    const float4* __restrict__ src = mode == 1 ? rs : rf;
+    const int j = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (j >= Vc) return;  // whole wave
+    const int e0 = rtp[j], e1 = rtp[j + 1];
+    const float fx = vcl[j].z;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    for (int e = e0 + lane; e < e1; e += 4 * 64) {
+        float w[4];
+        int id[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const bool on = e + k * 64 < e1;
+            w[k] = on ? rtw[e + k * 64] : 0.f;
+            id[k] = on ? rti[e + k * 64] : 0;
+        }
+        float4 q[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) q[k] = e + k * 64 < e1 ? src[id[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (e + k * 64 < e1) {
+                ax = ax + w[k] * q[k].x;
+                ay = ay + w[k] * q[k].y;
+                az = az + w[k] * q[k].z;
+            }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        ax += __shfl_down(ax, off, 64);
+        ay += __shfl_down(ay, off, 64);
+        az += __shfl_down(az, off, 64);
+    }
+    if (lane == 0) rc[j] = make_float4(ax * fx, ay * fx, az * fx, 0.f);
+}
+
 __global__ void __launch_bounds__(256) k_features(int Vc, const float4* __restrict__ vf, const int4* __restrict__ aids,
                                                   const float4* __restrict__ aw, const float4* __restrict__ X0,
                                                   const float4* __restrict__ D1, const float4* __restrict__ D2,
@@ -2747,6 +2852,12 @@ struct mgsb_solver {
     };
     std::map<std::string, PieceGraph> pseq;
     bool precaptured = false;  // the finite set of first rounds and follow-ups is in pseq
+    // the piece being enqueued is followed by a coarse-first half in the same
+    // round (enqueue_seq); a fused warm start then leaves its judge to that
+    // half's restriction (k_restrict_judged), whose arguments are kept here
+    bool next_is_post = false;
+    int judge_folded = 0;  // 0 none pending, else the judge's check mode
+    int judge_nb = 0;      // the dual residual's block count
     int halves_guess = 4;
     // learned schedule: after kGateAfter consecutive steps whose learned
     // warm start the judge undid, steps start without the MLP stage for
@@ -3311,24 +3422,24 @@ float4* fuse_spare(DevLevel& d, const float4* a, const float4* b) {
 }
 template <int MODE, int PHASE>
 void launch_fused_mode(mgsb_solver* s, DevLevel& d, const FuseBufs& fb, float4* out, float omega, int flags,
-                       const int* skip, const float4* ec, const float* amask) {
+                       const int* skip, const float4* ec, const float* amask, const int* jrev) {
     LevelMap& m = s->M[0];
     k_sweep_fused<MODE, PHASE><<<d.ntiles, kTileTets, 0, s->stream>>>(
         d.T, fb.cur, fb.prev, out, d.s, d.vc, fb.pin ? d.pcf2 : d.pcf, tile_meta(d), d.tvqp, d.tetq,
         d.B, d.vol, s->p.stiffness, d.tk, fb.pin ? d.pcf : d.pcf2, s->dcol, flags, omega, s->relax_hi, s->relax_lo,
-        d.cflag, d.cdepth, d.cenergy, ec, d.tiid, d.tiw, amask, skip, calm_flag(s, 1));
+        d.cflag, d.cdepth, d.cenergy, ec, d.tiid, d.tiw, amask, skip, calm_flag(s, 1), jrev, jrev ? d.rr : nullptr);
 }
 // One fused sweep: JACOBI (vertex update with omega, then the local step) or
 // PROLONG (x += mask P ec, then the local step; the smoother restarts).
 template <int PHASE>
 int launch_fused(mgsb_solver* s, DevLevel& d, FuseBufs& fb, float omega, int flags, const int* skip,
-                 const float4* ec = nullptr, const float* amask = nullptr) {
+                 const float4* ec = nullptr, const float* amask = nullptr, const int* jrev = nullptr) {
     relax_consts(s);
     float4* out = fuse_spare(d, fb.cur, fb.prev);
     switch (tet_mode_of(s->p.if_mode)) {
-        case kTetModePolar: launch_fused_mode<kTetModePolar, PHASE>(s, d, fb, out, omega, flags, skip, ec, amask); break;
-        case kTetModeAc: launch_fused_mode<kTetModeAc, PHASE>(s, d, fb, out, omega, flags, skip, ec, amask); break;
-        default: launch_fused_mode<kTetModeStable, PHASE>(s, d, fb, out, omega, flags, skip, ec, amask);
+        case kTetModePolar: launch_fused_mode<kTetModePolar, PHASE>(s, d, fb, out, omega, flags, skip, ec, amask, jrev); break;
+        case kTetModeAc: launch_fused_mode<kTetModeAc, PHASE>(s, d, fb, out, omega, flags, skip, ec, amask, jrev); break;
+        default: launch_fused_mode<kTetModeStable, PHASE>(s, d, fb, out, omega, flags, skip, ec, amask, jrev);
     }
     HIPCHK(hipGetLastError());
     fb.prev = PHASE == kFuseProlong ? out : fb.cur;
@@ -3465,6 +3576,11 @@ int enqueue_vc_start(mgsb_solver* s, bool learned) {
                                                  f.cmask, lsh, cf ? f.rr : nullptr, fst ? f.x : nullptr,
                                                  fst ? f.xprev : nullptr);
         HIPCHK(hipGetLastError());
+        if (fst && cf && s->next_is_post) {  // the judge runs in the next half's restriction
+            s->judge_folded = judge;
+            s->judge_nb = nbr;
+            return MGSB_OK;
+        }
         // res0, the judge and its revert to s in one launch
         k_judge_revert<<<nbv, 256, 0, s->stream>>>(nbr, s->vpart, s->vstate, s->hmirror, s->seqc, judge, f.V, f.x,
                                                    f.xprev, gath_rp(s, f), gath_cf2(s, f), gath_cf(s, f), f.r,
@@ -3531,14 +3647,25 @@ int enqueue_vc_post(mgsb_solver* s) {
     const int bs = block_vert(s);
     const float rs = s->p.rho_smooth;
     int rc;
-    k_restrict<<<nblk(c1.V * 64, 256), 256, 0, s->stream>>>(c1.V, c1.rtp, c1.rti, c1.rtw, f.r, c1.vcl, c1.r, skip);
+    const int judge = s->judge_folded;
+    s->judge_folded = 0;
+    if (judge) {  // the fused warm start's judge (k_restrict_judged)
+        k_restrict_judged<<<nblk(c1.V * 64, 256), 256, 0, s->stream>>>(
+            c1.V, c1.rtp, c1.rti, c1.rtw, f.r, f.rr, c1.vcl, c1.r, s->judge_nb, s->vpart, s->vstate, s->hmirror,
+            s->seqc, judge);
+        s->chk_launched++;
+    } else {
+        k_restrict<<<nblk(c1.V * 64, 256), 256, 0, s->stream>>>(c1.V, c1.rtp, c1.rti, c1.rtw, f.r, c1.vcl, c1.r, skip);
+    }
     HIPCHK(hipGetLastError());
     if ((rc = vcycle_lin(s, 1, skip))) return rc;
     float omega = 1.0f;
     if (fused(s, f)) {  // the prolongation, and each sweep's vertex update, inside a local step's launch
         // the partials alternate so that the last sweep's land in pcf
         FuseBufs fb{f.x, f.xprev, (1 + s->p.post_sweeps) & 1};
-        if ((rc = launch_fused<kFuseProlong>(s, f, fb, 1.0f, 0, skip, cur_e(c1), f.cmask))) return rc;
+        if ((rc = launch_fused<kFuseProlong>(s, f, fb, 1.0f, 0, skip, cur_e(c1), f.cmask,
+                                             judge ? &s->vstate->revert : nullptr)))
+            return rc;
         for (int k = 0; k < s->p.post_sweeps; k++) {
             omega = cheb_next(omega, rs);
             const int fl = kVertPoint | kVertContactJacobi | (k == s->p.post_sweeps - 1 ? kVertWriteCol : 0);
@@ -3593,6 +3720,19 @@ int enqueue_piece(mgsb_solver* s, int piece) {
     }
 }
 
+// Enqueues the pieces seq in order, telling each whether a coarse-first
+// half follows it in the same round.
+int enqueue_seq(mgsb_solver* s, const std::string& seq) {
+    int rc = MGSB_OK;
+    for (size_t i = 0; i < seq.size() && rc == MGSB_OK; i++) {
+        s->next_is_post = i + 1 < seq.size() && seq[i + 1] == (char)kPiecePost;
+        rc = enqueue_piece(s, seq[i]);
+    }
+    s->next_is_post = false;
+    s->judge_folded = 0;
+    return rc;
+}
+
 // Captures the pieces seq (kPiece* codes, one char each) as one graph and
 // keeps it (no launch).
 int capture_pieces(mgsb_solver* s, const std::string& seq, std::map<std::string, mgsb_solver::PieceGraph>::iterator& it) {
@@ -3600,9 +3740,7 @@ int capture_pieces(mgsb_solver* s, const std::string& seq, std::map<std::string,
     if (it == s->pseq.end()) {
         HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         const unsigned before = s->chk_launched;
-        int rc = MGSB_OK;
-        for (const char q : seq)
-            if ((rc = enqueue_piece(s, q))) break;
+        const int rc = enqueue_seq(s, seq);
         mgsb_solver::PieceGraph pg;
         pg.checks = s->chk_launched - before;  // captured, not run yet
         s->chk_launched = before;
@@ -3625,11 +3763,7 @@ int capture_pieces(mgsb_solver* s, const std::string& seq, std::map<std::string,
 // Enqueues the pieces seq in order: eagerly, or as one graph (captured at
 // the sequence's first use unless pre-captured).
 int run_pieces(mgsb_solver* s, const std::string& seq, bool graphable) {
-    if (!graphable) {
-        for (const char q : seq)
-            if (int rc = enqueue_piece(s, q)) return rc;
-        return MGSB_OK;
-    }
+    if (!graphable) return enqueue_seq(s, seq);
     std::map<std::string, mgsb_solver::PieceGraph>::iterator it;
     if (int rc = capture_pieces(s, seq, it)) return rc;
     HIPCHK(hipGraphLaunch(it->second.exec, s->stream));

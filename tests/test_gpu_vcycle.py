@@ -249,11 +249,17 @@ def _fuse_pair_solver(case, fuse):
               fuse_sweeps=fuse, use_graph=1)
     if case in ("temporal_cf", "collision", "bodies"):
         kw.update(warm_start=1, coarse_first=1)
+    if case == "learned_rejected":
+        kw.update(coarse_first=1, learn_gate=0)
     if case == "plain":
         kw.update(check_every=1)
     if case == "collision":
         kw.update(kcol=W.large_colliders()["k"])
-    s = M.Solver(lv, schedule=M.SCHED_VCYCLE, if_mode=M.IF_STABLE, **kw, **phys)
+    sched = M.SCHED_LEARNED if case == "learned_rejected" else M.SCHED_VCYCLE
+    s = M.Solver(lv, schedule=sched, if_mode=M.IF_STABLE, **kw, **phys)
+    if case == "learned_rejected":  # weights far too large: the judge rejects every warm start
+        n = s.learn_nparams()
+        s.learn_weights((50.0 * np.random.default_rng(5).normal(size=n)).astype(np.float32))
     s.fix_plane(1, 10.0, 1e-5)
     s.set_velocity(W.seeded_velocity(lv[0], W.top_fixflag(lv[0])))
     if case == "bodies":
@@ -265,7 +271,7 @@ def _fuse_pair_solver(case, fuse):
     return s
 
 
-@pytest.mark.parametrize("case", ["plain", "temporal_cf", "collision", "bodies"])
+@pytest.mark.parametrize("case", ["plain", "temporal_cf", "collision", "bodies", "learned_rejected"])
 def test_fused_sweeps_bit_identical(case):
     """fuse_sweeps = 1 runs each fine-level sweep's vertex update (and the
     coarse correction's prolongation) inside the next local step's launch,
@@ -273,7 +279,11 @@ def test_fused_sweeps_bit_identical(case):
     reads the same inputs in the same order, so the trajectory, the collision
     output and the run statistics equal those of separate launches bit for
     bit: the plain V-cycle, the temporal coarse-first start, a sphere and a
-    plane at penalty 1e6, and five bodies with per-body checks."""
+    plane at penalty 1e6, five bodies with per-body checks, and a learned
+    start the judge rejects on every step.  With fused sweeps the warm start
+    is one launch and its judge runs inside the first coarse half's
+    restriction (a rejection restricts g(s) and prolongs from u = 0); with
+    separate launches the judge is its own kernel that returns x to s."""
     a, b = _fuse_pair_solver(case, 0), _fuse_pair_solver(case, 1)
     us = b.time_kernel(20, kernel=1)
     halves = []
@@ -288,6 +298,8 @@ def test_fused_sweeps_bit_identical(case):
     assert np.array_equal(xa, xb)
     assert np.array_equal(a.velocity(), b.velocity())
     assert a.run_stats() == b.run_stats()
+    if case == "learned_rejected":
+        assert a.run_stats()["fallbacks"] >= 30
     if case == "collision":
         ca, cb = a.collision(), b.collision()
         for q, r in zip(ca, cb):
